@@ -1,0 +1,30 @@
+#!/usr/bin/env bash
+# Compiles DMD's own translation units (read in place under the reference
+# tree) plus oracle/ref_shim.cpp into oracle/_ref/libdmdref.so.
+# TEST INFRASTRUCTURE ONLY.  Needs only the C++ standard library:
+# src/force/coulomb_pme.cpp (fftw3.h) and src/sim/simulation_engine.cpp
+# (fftw3.h, hdf5.h) are not buildable in this image and are left out.
+set -euo pipefail
+REF=${DMD_REFERENCE:-/root/reference}
+HERE="$(cd "$(dirname "${BASH_SOURCE[0]}")" && pwd)"
+OUT="$HERE/_ref"
+if [ ! -d "$REF/src" ]; then
+    echo "build_ref: no reference tree at $REF; skipping" >&2
+    exit 0
+fi
+mkdir -p "$OUT"
+SRCS=(
+  core/cell.cpp core/system_data.cpp core/error.cpp
+  force/force_component.cpp force/force_engine.cpp force/lennard_jones.cpp
+  force/coulomb_direct.cpp force/coulomb_exclusion.cpp force/harmonic_bond.cpp
+  force/harmonic_angle.cpp force/periodic_dihedral.cpp force/harmonic_improper.cpp
+  integrate/integrator.cpp
+  thermostat/thermostat.cpp thermostat/berendsen.cpp thermostat/nose_hoover.cpp
+  barostat/barostat.cpp barostat/berendsen_barostat.cpp barostat/andersen_barostat.cpp
+)
+ARGS=()
+for s in "${SRCS[@]}"; do ARGS+=("$REF/src/$s"); done
+g++ -std=c++20 -O2 -fPIC -ffp-contract=off -fno-fast-math -shared \
+    -I"$REF/src" -o "$OUT/libdmdref.so.tmp" "${ARGS[@]}" "$HERE/ref_shim.cpp"
+mv "$OUT/libdmdref.so.tmp" "$OUT/libdmdref.so"
+echo "build_ref: $OUT/libdmdref.so"

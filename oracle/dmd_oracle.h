@@ -1,0 +1,132 @@
+/*
+ * dmd_oracle.h -- FP64 CPU restatement of DMD's step path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP
+ * library (dmd_amd/libdmdx.so) and the timed CPU baseline; it is never the
+ * thing measured on the GPU nor a fallback of the product path.
+ *
+ * Every function restates one reference function in the same operation
+ * order (see the file:line cited at each definition in dmd_oracle.c) and is
+ * compiled with -ffp-contract=off so that, in FAITHFUL mode, its results are
+ * bit-identical to the reference translation units compiled the same way.
+ * CORRECTED mode swaps in textbook formulas where DMD is physically wrong.
+ */
+#ifndef DMD_ORACLE_H
+#define DMD_ORACLE_H
+
+#include <stdint.h>
+#include "../include/dmdx.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_KE 138.935456          /* k_e, src/force/coulomb_direct.cpp:42  */
+#define ORC_KB 0.008314462618      /* src/core/constants.h:6                */
+#define ORC_PI 3.14159265358979323846
+
+/* ---- core */
+void orc_min_image(const double cell[9], double *dx, double *dy, double *dz);
+double orc_cell_volume(const double cell[9]);
+
+/* ---- pair lists: all i<j with minimum-image r^2 < rc^2, lexicographic.
+ * Return the number of pairs; only the first `cap` are written. */
+int64_t orc_pairs_n2(int64_t n, const double *x, const double *y, const double *z,
+                     const double cell[9], double rc, int32_t *pi, int32_t *pj, int64_t cap);
+int64_t orc_pairs_cells(int64_t n, const double *x, const double *y, const double *z,
+                        const double cell[9], double rc, int32_t *pi, int32_t *pj, int64_t cap);
+/* remove excluded pairs (excl given as any-order pairs); returns new count */
+int64_t orc_pairs_filter_excl(int64_t np, int32_t *pi, int32_t *pj,
+                              int64_t nex, const int32_t *ei, const int32_t *ej);
+
+/* ---- force components (accumulate into f and *energy) */
+void orc_lj_pairs(int64_t np, const int32_t *pi, const int32_t *pj,
+                  const double *x, const double *y, const double *z, const double cell[9],
+                  double rc, const int32_t *types, int32_t n_types,
+                  const double *sigma, const double *epsilon,
+                  double *fx, double *fy, double *fz, double *energy);
+void orc_coulomb_pairs(int64_t np, const int32_t *pi, const int32_t *pj,
+                       const double *x, const double *y, const double *z, const double cell[9],
+                       double rc, double alpha, const double *q,
+                       double *fx, double *fy, double *fz, double *energy);
+void orc_exclusion(int64_t ne, const int32_t *ei, const int32_t *ej,
+                   const double *x, const double *y, const double *z, const double cell[9],
+                   double alpha, const double *q, int mode,
+                   double *fx, double *fy, double *fz, double *energy);
+void orc_bonds(int64_t nb, const int32_t *bi, const int32_t *bj, const double *k, const double *r0,
+               const double *x, const double *y, const double *z, const double cell[9],
+               double *fx, double *fy, double *fz, double *energy);
+void orc_angles(int64_t na, const int32_t *ai, const int32_t *aj, const int32_t *ak,
+                const double *kt, const double *t0,
+                const double *x, const double *y, const double *z, const double cell[9], int mode,
+                double *fx, double *fy, double *fz, double *energy);
+void orc_dihedrals(int64_t nd, const int32_t *di, const int32_t *dj, const int32_t *dk,
+                   const int32_t *dl, const int32_t *per, const double *kp, const double *p0,
+                   const double *x, const double *y, const double *z, const double cell[9],
+                   int mode, double *fx, double *fy, double *fz, double *energy);
+void orc_impropers(int64_t nd, const int32_t *di, const int32_t *dj, const int32_t *dk,
+                   const int32_t *dl, const double *kp, const double *p0,
+                   const double *x, const double *y, const double *z, const double cell[9],
+                   int mode, double *fx, double *fy, double *fz, double *energy);
+
+/* ---- SPME */
+double orc_theta4(double u);
+double orc_dtheta4(double u);
+double orc_pme_self_energy(double alpha, int64_t n, const double *q, double k_e);
+/* in-place complex 3D DFT, x fastest; sign -1 forward, +1 backward, unnormalised */
+void orc_fft3d(int nx, int ny, int nz, double *re, double *im, int sign);
+void orc_pme(int64_t n, const double *x, const double *y, const double *z, const double *q,
+             const double cell[9], int nx, int ny, int nz, double alpha, int mode,
+             double *fx, double *fy, double *fz, double *energy);
+/* plain Ewald sum (real space over images within rc_real, reciprocal over
+ * |m_d| <= kmax), corrected physics; for pinning orc_pme */
+void orc_ewald(int64_t n, const double *x, const double *y, const double *z, const double *q,
+               const double cell[9], double alpha, double rc_real, int kmax,
+               double *fx, double *fy, double *fz, double *energy);
+
+/* ---- integration and coupling */
+void orc_half_kick(int64_t n, double *vx, double *vy, double *vz,
+                   const double *fx, const double *fy, const double *fz,
+                   const double *m, double dt);
+void orc_advance(int64_t n, double *x, double *y, double *z,
+                 const double *vx, const double *vy, const double *vz, double dt);
+double orc_kinetic_energy(int64_t n, const double *m, const double *vx, const double *vy,
+                          const double *vz);
+double orc_temperature(double ekin, int64_t n);
+void orc_berendsen(int64_t n, const double *m, double *vx, double *vy, double *vz,
+                   double dt, double T0, double tau);
+void orc_nose_hoover(int64_t n, const double *m, double *vx, double *vy, double *vz,
+                     double dt, double T0, double tau, double *zeta);
+double orc_virial(int64_t n, const double *x, const double *y, const double *z,
+                  const double *fx, const double *fy, const double *fz);
+double orc_pressure(double ekin, double w, double volume);
+void orc_berendsen_barostat(int64_t n, const double *m, const double *vx, const double *vy,
+                            const double *vz, double *x, double *y, double *z,
+                            const double *fx, const double *fy, const double *fz,
+                            double cell[9], double dt, double P0, double tau, double beta);
+void orc_andersen_barostat(int64_t n, const double *m, const double *vx, const double *vy,
+                           const double *vz, double *x, double *y, double *z,
+                           const double *fx, const double *fy, const double *fz,
+                           double cell[9], double dt, double P0, double tau, double *eps_dot);
+
+/* ---- simulation (SimulationEngine::step sequence) */
+typedef struct orc_sim orc_sim;
+orc_sim *orc_sim_create(const dmdx_system_desc *d);
+void orc_sim_destroy(orc_sim *s);
+void orc_sim_step(orc_sim *s, int64_t nsteps);
+/* one ForceEngine::compute (zeroes F and PE first, as step() does) */
+void orc_sim_compute(orc_sim *s);
+/* forces/energy at the current positions from a freshly built exact list;
+ * does not touch the list counter or any state */
+void orc_sim_forces_exact(orc_sim *s, double *fx, double *fy, double *fz, double *pe);
+void orc_sim_get(const orc_sim *s, double *x, double *y, double *z,
+                 double *vx, double *vy, double *vz, double *fx, double *fy, double *fz);
+void orc_sim_set_positions(orc_sim *s, const double *x, const double *y, const double *z);
+void orc_sim_set_velocities(orc_sim *s, const double *vx, const double *vy, const double *vz);
+/* scalars: [pe, ke, step, time, zeta, eps_dot, box_x, box_y, box_z, counter, n_pairs] */
+void orc_sim_scalars(const orc_sim *s, double out[11]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
