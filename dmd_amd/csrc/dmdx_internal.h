@@ -1,0 +1,232 @@
+// dmdx_internal.h -- shared definitions of the gfx950 step engine.
+//
+// Layout in HBM (one "batch" = one or more independent systems):
+//   * atoms of every system are packed in one SoA index space ("slots"),
+//     each system's range padded to a multiple of the wavefront (64), so a
+//     wavefront never straddles two systems and per-system reductions are
+//     per-wave partial sums summed in a fixed order (deterministic, and the
+//     same whether a system runs alone or inside a batch);
+//   * master state is FP64: x, y, z (unwrapped, as DMD never wraps),
+//     vx, vy, vz, fx, fy, fz, 1/m, m, q;
+//   * bonded terms of all systems are packed per kind, each system's range
+//     padded to 64; every term writes its per-atom contributions to a term
+//     force buffer and a per-atom gather adds them in DMD's component order
+//     (no force atomics; FP64 accumulation order equals the reference's);
+//   * neighbour lists are per system, column-major (neighbour k of all atoms
+//     of a system contiguous), so one-thread-per-atom loops read coalesced.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/dmdx.h"
+
+namespace dmdx {
+
+constexpr int WAVE = 64;
+constexpr double KB = 0.008314462618;  // src/core/constants.h:6
+constexpr double KE = 138.935456;      // src/force/coulomb_direct.cpp:42
+constexpr double PI = 3.14159265358979323846;
+
+// ------------------------------------------------------------- errors
+void set_error(int status, const std::string& msg);
+int fail(int status, const std::string& msg);  // set_error + return status
+
+#define DMDX_HIP(expr)                                                                 \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return ::dmdx::fail(DMDX_E_GPU, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// ---------------------------------------------------- per-system constants
+// Everything a kernel needs to know about one system.  Lives in device
+// memory; box fields are updated on the device by the barostat.
+struct SysDev {
+    int64_t atom_off;   // first slot
+    int32_t n;          // atoms
+    int32_t n_pad;      // slots (multiple of 64)
+    int32_t physics;    // DMDX_FAITHFUL / DMDX_CORRECTED
+    int32_t precision;  // DMDX_FP64 / DMDX_MIXED
+    double L[3];        // box
+    double invL[3];     // 1.0 / L (FP64 division, as the reference)
+    // pair terms
+    int32_t use_lj;
+    int32_t coul_pairs;  // real-space erfc pairs computed from the list
+    int32_t ntypes;
+    int32_t lj_off;      // offset of the n_types^2 sigma/epsilon tables
+    double rc_lj2, rc_c2, alpha;
+    // list
+    int32_t has_list;
+    int32_t interval;    // computes between rebuilds (<= 0: never auto)
+    double rlist, rlist2;
+    int64_t nbr_off;     // start of this system's M x n_pad list block
+    int32_t max_nbr;
+    int32_t cell_off;    // start of this system's cells
+    int32_t nc[3];       // cells per edge at allocation (upper bound)
+    int32_t excl_off;    // per-atom exclusion CSR base (slot-indexed)
+    // bonded terms: [off, off + n_pad) slots of the packed term arrays
+    int64_t bond_off, angle_off, dih_off, imp_off, ex_off;
+    int32_t n_bond, n_angle, n_dih, n_imp, n_ex;
+    int32_t ex_mode;     // exclusion term active (faithful / corrected form)
+    // PME
+    int32_t pme;
+    int32_t K[3];
+    int64_t grid_off;    // complex elements
+    int64_t tw_off[3];   // twiddle table offsets (complex elements)
+    int64_t bm_off[3];   // B-modulus table offsets
+    double self_energy;
+    double qsum;         // total charge (neutralising background, CORRECTED)
+    // integration
+    double dt;
+    int32_t thermostat, barostat;
+    double T0, ttau, Qnh;
+    double P0, ptau, beta, Wab;
+    // partial-sum ranges (in waves) for the deterministic reductions
+    int32_t aw_off, aw_n;      // atom waves
+    int32_t bw_off, bw_n;      // bond waves
+    int32_t gw_off, gw_n;      // angle waves
+    int32_t dw_off, dw_n;      // dihedral waves
+    int32_t iw_off, iw_n;      // improper waves
+    int32_t xw_off, xw_n;      // exclusion waves
+    int32_t pw_off, pw_n;      // PME workgroup partials
+};
+
+// Per-system mutable scalars.
+struct SysState {
+    double pe;           // potential energy of the last compute
+    double pe_terms[8];  // lj, coul, bond, angle, dih, imp, pme, excl
+    double ke;           // kinetic energy after the closing kick (thermostat input)
+    double ke_out;       // kinetic energy of the velocities at the end of the step
+    double lambda;       // pending velocity scale (thermostat), applied lazily
+    double zeta;         // Nose-Hoover friction
+    double eps_dot;      // Andersen piston velocity
+    double virial;
+    double pressure;
+    double xscale;       // barostat position scale of this step
+    int32_t counter;     // computes since the last list rebuild
+    int32_t rebuild;     // this compute rebuilds the list
+    int32_t reuse;       // this compute reuses the previous forces
+    int32_t valid;       // forces/energies correspond to the current positions
+    int32_t flags;       // bit0 neighbour overflow, bit1 non-finite energy
+    int32_t active;      // system participates in steps
+    int64_t steps_done;
+};
+
+enum { FLAG_NBR_OVERFLOW = 1, FLAG_NONFINITE = 2 };
+
+// term force buffer: one double3 per (term, role)
+struct TermF {
+    double x, y, z;
+};
+
+// one tile = up to B lines of one system's grid along one dimension
+struct FftTile {
+    int32_t sys;
+    int32_t line0;
+    int32_t nlines;
+    int32_t tile_local;  // index among this system's tiles of this pass
+};
+
+struct FftPlan {  // per system and dimension
+    int32_t K;
+    int32_t nrad;
+    int32_t radix[16];
+};
+
+// All device pointers of a batch (plain struct, passed to the launchers).
+struct Dev {
+    int nsys = 0;
+    int64_t nslots = 0;  // multiple of 64
+    int ncells = 0;
+    int precision = DMDX_FP64;
+    SysDev* sys = nullptr;
+    SysState* st = nullptr;
+    int* slot_sys = nullptr;
+    double *x = nullptr, *y = nullptr, *z = nullptr;
+    double *vx = nullptr, *vy = nullptr, *vz = nullptr;
+    double *fx = nullptr, *fy = nullptr, *fz = nullptr;
+    double *invm = nullptr, *m = nullptr, *q = nullptr;
+    int* type = nullptr;
+    // lists
+    int *acell = nullptr, *ccount = nullptr, *cstart = nullptr, *cfill = nullptr;
+    int *cell_atoms = nullptr, *cell_sys = nullptr, *order = nullptr, *nn = nullptr;
+    int* nbr = nullptr;
+    int *ex_start = nullptr, *ex_list = nullptr;
+    // LJ tables
+    double *lj_sig = nullptr, *lj_eps = nullptr;
+    float *lj_c12 = nullptr, *lj_c6 = nullptr;
+    // bonded (padded term counts)
+    int64_t nb = 0, na = 0, nd = 0, ni = 0, ne = 0;
+    int *b_sys = nullptr, *bi = nullptr, *bj = nullptr;
+    double *bk = nullptr, *br0 = nullptr;
+    int *a_sys = nullptr, *ai = nullptr, *aj = nullptr, *ak = nullptr;
+    double *akt = nullptr, *at0 = nullptr;
+    int *d_sys = nullptr, *di = nullptr, *dj = nullptr, *dk = nullptr, *dl = nullptr;
+    int* dper = nullptr;
+    double *dkp = nullptr, *dp0 = nullptr;
+    int *i_sys = nullptr, *ii = nullptr, *ij = nullptr, *ik = nullptr, *il = nullptr;
+    double *ikp = nullptr, *ip0 = nullptr;
+    int *e_sys = nullptr, *ei = nullptr, *ej = nullptr;
+    TermF* tf = nullptr;
+    int64_t tf_bond = 0, tf_angle = 0, tf_dih = 0, tf_imp = 0, tf_ex = 0;
+    int *c_start = nullptr, *c_mid = nullptr, *contrib = nullptr;
+    // PME
+    int64_t grid_total = 0;
+    void* grid = nullptr;  // complex, double or float per precision
+    int* grid_sys = nullptr;
+    void* tw = nullptr;  // twiddles, same real type
+    double* bmod = nullptr;
+    FftPlan* plans = nullptr;
+    FftTile* tiles[3] = {nullptr, nullptr, nullptr};
+    int ntiles[3] = {0, 0, 0};
+    size_t lds_bytes[3] = {0, 0, 0};
+    // partial sums (one per wave / PME tile)
+    double *p_lj = nullptr, *p_coul = nullptr, *p_bond = nullptr, *p_angle = nullptr;
+    double *p_dih = nullptr, *p_imp = nullptr, *p_pme = nullptr, *p_excl = nullptr;
+    double *p_ke = nullptr, *p_w = nullptr;
+    // what exists in the batch
+    bool any_list = false, any_pair = false, any_pme = false, any_excl = false;
+    bool any_gather = false, any_baro = false;
+};
+
+// host launchers (one per kernel group); return hipGetLastError()
+hipError_t launch_build_lists(const Dev& d, hipStream_t s);
+hipError_t launch_pair(const Dev& d, hipStream_t s, int accumulate);
+hipError_t launch_bonded(const Dev& d, hipStream_t s);
+hipError_t launch_pme(const Dev& d, hipStream_t s);
+hipError_t launch_pre_compute(const Dev& d, hipStream_t s, int phase, int force_rebuild,
+                              int reuse_enabled);
+hipError_t launch_pre_rebuild_only(const Dev& d, hipStream_t s);
+hipError_t launch_gather(const Dev& d, hipStream_t s);
+hipError_t launch_kick_drift(const Dev& d, hipStream_t s);
+hipError_t launch_kick_ke(const Dev& d, hipStream_t s);
+hipError_t launch_finalize(const Dev& d, hipStream_t s, int with_kinetic);
+hipError_t launch_barostat(const Dev& d, hipStream_t s);
+hipError_t launch_materialize(const Dev& d, hipStream_t s);
+
+inline unsigned blocks_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// ------------------------------------------------------- device helpers
+// Orthorhombic minimum image with DMD's expression order
+// (src/core/cell.cpp:26-38): d - round(d * (1/L)) * L, round half away.
+// `inv` must be 1.0 / L computed in FP64 (correctly rounded, so it equals
+// the value the reference recomputes per call).
+__device__ __forceinline__ double min_image_dmd(double d, double L, double inv)
+{
+    return d - round(d * inv) * L;
+}
+
+template <class T>
+__device__ __forceinline__ T wave_sum(T v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+
+}  // namespace dmdx

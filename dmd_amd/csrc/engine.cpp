@@ -1,0 +1,1067 @@
+// engine.cpp -- host side of the step engine: packing systems into device
+// buffers, the step sequence and the engine-level C-ABI.
+//
+// The step replaces SimulationEngine::step (src/sim/simulation_engine.cpp:
+// 53-86): compute, kick + drift, compute, kick, thermostat, barostat.  One
+// difference in what is launched, not in what is computed: the first compute
+// of a step sees the same positions, list and parameters as the second
+// compute of the previous step unless the list is rebuilt or a barostat moved
+// the atoms, and then yields the same forces and energy, so it is skipped
+// (the list counter still advances exactly as in the reference).
+#include "engine.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+
+namespace dmdx {
+
+static thread_local std::string g_last_error;
+
+void set_error(int status, const std::string& msg)
+{
+    (void)status;
+    g_last_error = msg;
+}
+
+int fail(int status, const std::string& msg)
+{
+    set_error(status, msg);
+    return status;
+}
+
+const char* last_error() { return g_last_error.c_str(); }
+
+static int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
+
+template <class T>
+static int dev_upload(T** dst, const std::vector<T>& src, std::vector<void*>& owned)
+{
+    size_t n = std::max<size_t>(src.size(), 1);
+    DMDX_HIP(hipMalloc((void**)dst, n * sizeof(T)));
+    owned.push_back(*dst);
+    if (!src.empty()) DMDX_HIP(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+    else DMDX_HIP(hipMemset(*dst, 0, sizeof(T)));
+    return DMDX_OK;
+}
+
+template <class T>
+static int dev_alloc_zero(T** dst, size_t n, std::vector<void*>& owned)
+{
+    n = std::max<size_t>(n, 1);
+    DMDX_HIP(hipMalloc((void**)dst, n * sizeof(T)));
+    owned.push_back(*dst);
+    DMDX_HIP(hipMemset(*dst, 0, n * sizeof(T)));
+    return DMDX_OK;
+}
+
+#define TRY(expr)                 \
+    do {                          \
+        int rc_ = (expr);         \
+        if (rc_ != DMDX_OK) return rc_; \
+    } while (0)
+
+// radices for a Stockham plan: 4s first, then 2, 3, 5, 7, then other primes
+static bool plan_radices(int K, FftPlan& P)
+{
+    P.K = K;
+    P.nrad = 0;
+    int n = K;
+    const int pref[] = {4, 2, 3, 5, 7, 11, 13};
+    for (int r : pref)
+        while (n % r == 0) {
+            if (P.nrad >= 16) return false;
+            P.radix[P.nrad++] = r;
+            n /= r;
+        }
+    return n == 1;
+}
+
+Batch::~Batch()
+{
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (void* p : owned) (void)hipFree(p);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+int Batch::create(const dmdx_system_desc* descs, int nsys, const BatchOptions& opt)
+{
+    if (nsys <= 0) return fail(DMDX_E_INVALID, "no systems");
+    options = opt;
+    DMDX_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    const int prec = descs[0].precision;
+    for (int s = 0; s < nsys; ++s)
+        if (descs[s].precision != prec)
+            return fail(DMDX_E_PARM, "all systems of a batch must share one precision");
+    d.precision = prec;
+    d.nsys = nsys;
+    hsys.resize(nsys);
+    hstate.assign(nsys, SysState{});
+    host.resize(nsys);
+
+    std::vector<int> slot_sys;
+    std::vector<double> x, y, z, vx, vy, vz, invm, m, q;
+    std::vector<int> type;
+    std::vector<double> lj_sig, lj_eps;
+    std::vector<float> c12, c6;
+    std::vector<int> cell_sys;
+    std::vector<int> b_sys, bi, bj, a_sys, ai, aj, ak, d_sys, di, dj, dk, dl, dper, i_sys, ii, ij,
+        ik, il, e_sys, ei, ej;
+    std::vector<double> bk, br0, akt, at0, dkp, dp0, ikp, ip0;
+    std::vector<std::vector<int>> ex_of;  // per slot, corrected-mode list exclusions
+    int64_t slot = 0, nbr_total = 0, grid_total = 0, tw_total = 0, bm_total = 0;
+    int ncells = 0, lj_total = 0, pw_total = 0;
+    std::vector<int> grid_sys;
+
+    // ---- pass 1: atoms, parameters, offsets
+    for (int s = 0; s < nsys; ++s) {
+        const dmdx_system_desc& D = descs[s];
+        SysDev& S = hsys[s];
+        std::memset(&S, 0, sizeof(S));
+        HostSys& H = host[s];
+        if (D.n_atoms <= 0) return fail(DMDX_E_INVALID, "system with no atoms");
+        if (!D.pos_x || !D.pos_y || !D.pos_z || !D.masses)
+            return fail(DMDX_E_INVALID, "positions and masses are required");
+        for (int k = 0; k < 3; ++k)
+            if (!(D.box[k] > 0.0)) return fail(DMDX_E_PARM, "box edges must be positive");
+        const int n = (int)D.n_atoms;
+        const int n_pad = (int)round_up(n, WAVE);
+        S.atom_off = slot;
+        S.n = n;
+        S.n_pad = n_pad;
+        S.physics = D.physics;
+        S.precision = D.precision;
+        for (int k = 0; k < 3; ++k) {
+            S.L[k] = D.box[k];
+            S.invL[k] = 1.0 / D.box[k];
+        }
+        const bool faithful = D.physics == DMDX_FAITHFUL;
+        // pair terms: LJ always from the list; real-space erfc only in
+        // CORRECTED (build_simulation never fills CoulombDirect's list)
+        S.use_lj = D.use_lj ? 1 : 0;
+        S.coul_pairs = (!faithful && D.coulomb_type != DMDX_COULOMB_NONE) ? 1 : 0;
+        if (opt.component == COMP_COULOMB_DIRECT) S.coul_pairs = 1;
+        S.ntypes = S.use_lj ? std::max(1, (int)D.lj_n_types) : 1;
+        S.lj_off = lj_total;
+        for (int t = 0; t < S.ntypes * S.ntypes; ++t) {
+            double sg = S.use_lj ? D.lj_sigma[t] : 1.0;
+            double ep = S.use_lj ? D.lj_epsilon[t] : 0.0;
+            lj_sig.push_back(sg);
+            lj_eps.push_back(ep);
+            double s6 = sg * sg * sg * sg * sg * sg;
+            c12.push_back((float)(4.0 * ep * s6 * s6));
+            c6.push_back((float)(4.0 * ep * s6));
+        }
+        lj_total += S.ntypes * S.ntypes;
+        S.rc_lj2 = D.lj_cutoff * D.lj_cutoff;
+        S.rc_c2 = D.coulomb_cutoff * D.coulomb_cutoff;
+        S.alpha = D.ewald_coeff;
+        double rl = 0.0;
+        if (S.use_lj) rl = D.lj_cutoff;
+        if (S.coul_pairs) rl = std::max(rl, D.coulomb_cutoff);
+        S.has_list = (S.use_lj || S.coul_pairs) ? 1 : 0;
+        S.rlist = rl + D.list_skin;
+        S.rlist2 = S.rlist * S.rlist;
+        S.interval = D.rebuild_interval > 0 ? D.rebuild_interval : 10;
+        if (opt.component == COMP_COULOMB_DIRECT) S.interval = 0;  // explicit rebuild only
+        if (S.has_list) {
+            double vol = D.box[0] * D.box[1] * D.box[2];
+            double dens = n / vol;
+            double expect = 4.0 / 3.0 * PI * S.rlist * S.rlist * S.rlist * dens;
+            int M = (int)std::ceil(1.6 * expect + 48.0);
+            M = std::min(M, n - 1);
+            M = std::max(M, 1);
+            if (opt.max_nbr > 0) M = std::min(std::max(opt.max_nbr, 1), std::max(n - 1, 1));
+            if (M > 2048)
+                return fail(DMDX_E_PARM, "neighbour list would exceed 2048 entries per atom; "
+                                         "reduce the cutoff");
+            S.max_nbr = M;
+            S.nbr_off = nbr_total;
+            nbr_total += (int64_t)M * n_pad;
+            for (int k = 0; k < 3; ++k) {
+                int c = (int)std::floor(D.box[k] / (S.rlist * (1.0 + 1e-9)));
+                // a barostat can grow the box: leave room for more cells
+                if (D.barostat != DMDX_BARO_NONE) c = (int)std::floor(1.2 * c) + 1;
+                S.nc[k] = std::max(1, c);
+            }
+            for (int k = 0; k < 3; ++k)
+                if (D.box[k] < 2.0 * (S.use_lj ? D.lj_cutoff : D.coulomb_cutoff))
+                    return fail(DMDX_E_PARM, "cutoff must be below half the box (minimum image)");
+        } else {
+            S.nc[0] = S.nc[1] = S.nc[2] = 1;
+        }
+        S.cell_off = ncells;
+        int nc3 = S.nc[0] * S.nc[1] * S.nc[2];
+        ncells += nc3;
+        for (int c = 0; c < nc3; ++c) cell_sys.push_back(s);
+
+        double qsum = 0.0, sum_q2 = 0.0;
+        for (int i = 0; i < n_pad; ++i) {
+            slot_sys.push_back(s);
+            bool real = i < n;
+            x.push_back(real ? D.pos_x[i] : 0.0);
+            y.push_back(real ? D.pos_y[i] : 0.0);
+            z.push_back(real ? D.pos_z[i] : 0.0);
+            vx.push_back(real && D.vel_x ? D.vel_x[i] : 0.0);
+            vy.push_back(real && D.vel_y ? D.vel_y[i] : 0.0);
+            vz.push_back(real && D.vel_z ? D.vel_z[i] : 0.0);
+            double mm = real ? D.masses[i] : 1.0;
+            m.push_back(mm);
+            invm.push_back(1.0 / mm);
+            double qq = (real && D.charges) ? D.charges[i] : 0.0;
+            q.push_back(qq);
+            if (real) {
+                qsum += qq;
+                sum_q2 += qq * qq;
+            }
+            int tt = (real && D.atom_types) ? D.atom_types[i] : 0;
+            if (S.use_lj && (tt < 0 || tt >= S.ntypes))
+                return fail(DMDX_E_PARM, "atom type out of range of the LJ tables");
+            type.push_back(tt);
+        }
+        S.qsum = qsum;
+        // PME (src/force/coulomb_pme.cpp:71-75 for the self energy)
+        S.pme = (D.coulomb_type == DMDX_COULOMB_PME || opt.component == COMP_PME) ? 1 : 0;
+        if (opt.component != COMP_NONE && opt.component != COMP_PME) S.pme = 0;
+        if (S.pme) {
+            if (D.pme_order != 4 && D.pme_order != 0)
+                return fail(DMDX_E_PARM, "only order-4 B-splines are implemented (as DMD)");
+            for (int k = 0; k < 3; ++k) {
+                S.K[k] = D.pme_grid[k];
+                if (S.K[k] < 4 || S.K[k] > 256) return fail(DMDX_E_PARM, "PME grid must be 4..256");
+                FftPlan P;
+                if (!plan_radices(S.K[k], P))
+                    return fail(DMDX_E_PARM, "PME grid size must factor into primes <= 13");
+            }
+            S.self_energy = -KE * D.ewald_coeff / std::sqrt(PI) * sum_q2;
+            S.grid_off = grid_total;
+            int64_t k3 = (int64_t)S.K[0] * S.K[1] * S.K[2];
+            grid_total += k3;
+            for (int64_t e = 0; e < k3; ++e) grid_sys.push_back(s);
+            for (int k = 0; k < 3; ++k) {
+                S.tw_off[k] = tw_total;
+                tw_total += S.K[k];
+                S.bm_off[k] = bm_total;
+                bm_total += S.K[k];
+            }
+        }
+        // integration (src/thermostat/nose_hoover.cpp:6-11,
+        // src/barostat/andersen_barostat.cpp:8-14)
+        S.dt = D.dt;
+        S.thermostat = D.thermostat;
+        S.T0 = D.thermo_temperature;
+        S.ttau = D.thermo_tau;
+        double dof = 3.0 * (double)n - 3.0;
+        if (dof < 1.0) dof = 1.0;
+        S.Qnh = dof * KB * S.T0 * S.ttau * S.ttau;
+        S.barostat = D.barostat;
+        S.P0 = D.baro_pressure;
+        S.ptau = D.baro_tau;
+        S.beta = D.baro_compressibility;
+        S.Wab = dof * KB * S.P0 * S.ptau * S.ptau;
+        S.aw_off = (int)(slot / WAVE);
+        S.aw_n = n_pad / WAVE;
+        slot += n_pad;
+
+        SysState& T = hstate[s];
+        std::memset(&T, 0, sizeof(T));
+        T.counter = D.step_since_rebuild;
+        T.lambda = 1.0;
+        T.active = 1;
+        T.valid = 0;
+
+        H.n = n;
+        H.has_barostat = D.barostat != DMDX_BARO_NONE;
+    }
+    d.nslots = slot;
+    d.ncells = ncells;
+
+    // ---- pass 2: bonded terms (global slot indices), exclusions, contributions
+    ex_of.assign(slot, {});
+    auto pad_terms = [](int64_t cnt) { return round_up(cnt, WAVE); };
+    auto check_atom = [](int a, int n) { return a >= 0 && a < n; };
+    std::vector<std::vector<int64_t>> pre(slot), post(slot);  // per slot, term-force indices
+    // term-force offsets need the padded totals first
+    int64_t NB = 0, NA = 0, ND = 0, NI = 0, NE = 0;
+    for (int s = 0; s < nsys; ++s) {
+        const dmdx_system_desc& D = descs[s];
+        bool excl_term = exclusion_term_active(D, opt);
+        NB += D.n_bonds > 0 ? pad_terms(D.n_bonds) : 0;
+        NA += D.n_angles > 0 ? pad_terms(D.n_angles) : 0;
+        ND += D.n_dihedrals > 0 ? pad_terms(D.n_dihedrals) : 0;
+        NI += D.n_impropers > 0 ? pad_terms(D.n_impropers) : 0;
+        NE += (excl_term && D.n_excl > 0) ? pad_terms(D.n_excl) : 0;
+    }
+    d.tf_bond = 0;
+    d.tf_angle = 2 * NB;
+    d.tf_dih = d.tf_angle + 3 * NA;
+    d.tf_imp = d.tf_dih + 4 * ND;
+    d.tf_ex = d.tf_imp + 4 * NI;
+    int64_t tf_total = d.tf_ex + 2 * NE;
+    int64_t ob = 0, oa = 0, od = 0, oi = 0, oe = 0;
+    for (int s = 0; s < nsys; ++s) {
+        const dmdx_system_desc& D = descs[s];
+        SysDev& S = hsys[s];
+        const int n = S.n;
+        const int64_t base = S.atom_off;
+        bool comp_ok_bond = opt.component == COMP_NONE || opt.component == COMP_BOND;
+        bool comp_ok_angle = opt.component == COMP_NONE || opt.component == COMP_ANGLE;
+        bool comp_ok_dih = opt.component == COMP_NONE || opt.component == COMP_DIHEDRAL;
+        bool comp_ok_imp = opt.component == COMP_NONE || opt.component == COMP_IMPROPER;
+        // bonds
+        S.bond_off = ob;
+        S.n_bond = comp_ok_bond ? (int)D.n_bonds : 0;
+        if (S.n_bond > 0) {
+            int64_t np = pad_terms(S.n_bond);
+            for (int64_t u = 0; u < np; ++u) {
+                bool real = u < S.n_bond;
+                int a = real ? D.bond_i[u] : 0, b = real ? D.bond_j[u] : 0;
+                if (real && (!check_atom(a, n) || !check_atom(b, n)))
+                    return fail(DMDX_E_PARM, "bond atom index out of range");
+                b_sys.push_back(s);
+                bi.push_back((int)(base + a));
+                bj.push_back((int)(base + b));
+                bk.push_back(real ? D.bond_k[u] : 0.0);
+                br0.push_back(real ? D.bond_r0[u] : 1.0);
+                if (real) {
+                    pre[base + a].push_back(d.tf_bond + 2 * (ob + u));
+                    pre[base + b].push_back(d.tf_bond + 2 * (ob + u) + 1);
+                }
+            }
+            ob += np;
+        }
+        // angles
+        S.angle_off = oa;
+        S.n_angle = comp_ok_angle ? (int)D.n_angles : 0;
+        if (S.n_angle > 0) {
+            int64_t np = pad_terms(S.n_angle);
+            for (int64_t u = 0; u < np; ++u) {
+                bool real = u < S.n_angle;
+                int a = real ? D.angle_i[u] : 0, b = real ? D.angle_j[u] : 1 % n,
+                    c = real ? D.angle_k[u] : 2 % n;
+                if (real && (!check_atom(a, n) || !check_atom(b, n) || !check_atom(c, n)))
+                    return fail(DMDX_E_PARM, "angle atom index out of range");
+                a_sys.push_back(s);
+                ai.push_back((int)(base + a));
+                aj.push_back((int)(base + b));
+                ak.push_back((int)(base + c));
+                akt.push_back(real ? D.angle_k_theta[u] : 0.0);
+                at0.push_back(real ? D.angle_theta0[u] : 0.0);
+                if (real) {
+                    pre[base + a].push_back(d.tf_angle + 3 * (oa + u));
+                    pre[base + b].push_back(d.tf_angle + 3 * (oa + u) + 1);
+                    pre[base + c].push_back(d.tf_angle + 3 * (oa + u) + 2);
+                }
+            }
+            oa += np;
+        }
+        // dihedrals
+        S.dih_off = od;
+        S.n_dih = comp_ok_dih ? (int)D.n_dihedrals : 0;
+        if (S.n_dih > 0) {
+            int64_t np = pad_terms(S.n_dih);
+            for (int64_t u = 0; u < np; ++u) {
+                bool real = u < S.n_dih;
+                int a = real ? D.dih_i[u] : 0, b = real ? D.dih_j[u] : 0, c = real ? D.dih_k[u] : 0,
+                    e = real ? D.dih_l[u] : 0;
+                if (real && (!check_atom(a, n) || !check_atom(b, n) || !check_atom(c, n) ||
+                             !check_atom(e, n)))
+                    return fail(DMDX_E_PARM, "dihedral atom index out of range");
+                d_sys.push_back(s);
+                di.push_back((int)(base + a));
+                dj.push_back((int)(base + b));
+                dk.push_back((int)(base + c));
+                dl.push_back((int)(base + e));
+                dper.push_back(real ? D.dih_periodicity[u] : 1);
+                dkp.push_back(real ? D.dih_k_phi[u] : 0.0);
+                dp0.push_back(real ? D.dih_phi0[u] : 0.0);
+                if (real)
+                    for (int r = 0; r < 4; ++r)
+                        pre[base + (r == 0 ? a : r == 1 ? b : r == 2 ? c : e)].push_back(
+                            d.tf_dih + 4 * (od + u) + r);
+            }
+            od += np;
+        }
+        // impropers
+        S.imp_off = oi;
+        S.n_imp = comp_ok_imp ? (int)D.n_impropers : 0;
+        if (S.n_imp > 0) {
+            int64_t np = pad_terms(S.n_imp);
+            for (int64_t u = 0; u < np; ++u) {
+                bool real = u < S.n_imp;
+                int a = real ? D.imp_i[u] : 0, b = real ? D.imp_j[u] : 0, c = real ? D.imp_k[u] : 0,
+                    e = real ? D.imp_l[u] : 0;
+                if (real && (!check_atom(a, n) || !check_atom(b, n) || !check_atom(c, n) ||
+                             !check_atom(e, n)))
+                    return fail(DMDX_E_PARM, "improper atom index out of range");
+                i_sys.push_back(s);
+                ii.push_back((int)(base + a));
+                ij.push_back((int)(base + b));
+                ik.push_back((int)(base + c));
+                il.push_back((int)(base + e));
+                ikp.push_back(real ? D.imp_k_phi[u] : 0.0);
+                ip0.push_back(real ? D.imp_phi0[u] : 0.0);
+                if (real)
+                    for (int r = 0; r < 4; ++r)
+                        pre[base + (r == 0 ? a : r == 1 ? b : r == 2 ? c : e)].push_back(
+                            d.tf_imp + 4 * (oi + u) + r);
+            }
+            oi += np;
+        }
+        // exclusions: list filter (CORRECTED) and the exclusion term
+        for (int64_t e = 0; e < D.n_excl; ++e) {
+            int a = D.excl_i[e], b = D.excl_j[e];
+            if (!check_atom(a, n) || !check_atom(b, n))
+                return fail(DMDX_E_PARM, "exclusion atom index out of range");
+        }
+        if (D.physics != DMDX_FAITHFUL && S.has_list && opt.component == COMP_NONE) {
+            for (int64_t e = 0; e < D.n_excl; ++e) {
+                int a = D.excl_i[e], b = D.excl_j[e];
+                ex_of[base + a].push_back((int)(base + b));
+                ex_of[base + b].push_back((int)(base + a));
+            }
+        }
+        S.ex_off = oe;
+        S.n_ex = exclusion_term_active(D, opt) ? (int)D.n_excl : 0;
+        S.ex_mode = S.n_ex > 0;
+        if (S.n_ex > 0) {
+            int64_t np = pad_terms(S.n_ex);
+            for (int64_t u = 0; u < np; ++u) {
+                bool real = u < S.n_ex;
+                int a = real ? D.excl_i[u] : 0, b = real ? D.excl_j[u] : 1 % n;
+                e_sys.push_back(s);
+                ei.push_back((int)(base + a));
+                ej.push_back((int)(base + b));
+                if (real) {
+                    post[base + a].push_back(d.tf_ex + 2 * (oe + u));
+                    post[base + b].push_back(d.tf_ex + 2 * (oe + u) + 1);
+                }
+            }
+            oe += np;
+        }
+        S.bw_off = (int)(S.bond_off / WAVE);
+        S.bw_n = (int)(S.n_bond > 0 ? pad_terms(S.n_bond) / WAVE : 0);
+        S.gw_off = (int)(S.angle_off / WAVE);
+        S.gw_n = (int)(S.n_angle > 0 ? pad_terms(S.n_angle) / WAVE : 0);
+        S.dw_off = (int)(S.dih_off / WAVE);
+        S.dw_n = (int)(S.n_dih > 0 ? pad_terms(S.n_dih) / WAVE : 0);
+        S.iw_off = (int)(S.imp_off / WAVE);
+        S.iw_n = (int)(S.n_imp > 0 ? pad_terms(S.n_imp) / WAVE : 0);
+        S.xw_off = (int)(S.ex_off / WAVE);
+        S.xw_n = (int)(S.n_ex > 0 ? pad_terms(S.n_ex) / WAVE : 0);
+    }
+    d.nb = ob;
+    d.na = oa;
+    d.nd = od;
+    d.ni = oi;
+    d.ne = oe;
+    std::vector<int> c_start(slot + 1, 0), c_mid(slot, 0), contrib;
+    std::vector<int> ex_start(slot + 1, 0), ex_list;
+    for (int64_t g = 0; g < slot; ++g) {
+        c_start[g] = (int)contrib.size();
+        for (int64_t v : pre[g]) contrib.push_back((int)v);
+        c_mid[g] = (int)contrib.size();
+        for (int64_t v : post[g]) contrib.push_back((int)v);
+        ex_start[g] = (int)ex_list.size();
+        for (int v : ex_of[g]) ex_list.push_back(v);
+    }
+    c_start[slot] = (int)contrib.size();
+    ex_start[slot] = (int)ex_list.size();
+
+    // ---- PME tables, plans and tiles
+    std::vector<double> twd, bmod;
+    std::vector<FftPlan> plans(3 * nsys);
+    std::vector<FftTile> tiles[3];
+    const size_t cx = prec == DMDX_FP64 ? 16 : 8;
+    for (int s = 0; s < nsys; ++s) {
+        SysDev& S = hsys[s];
+        if (!S.pme) continue;
+        for (int k = 0; k < 3; ++k) {
+            plan_radices(S.K[k], plans[3 * s + k]);
+            for (int j = 0; j < S.K[k]; ++j) {
+                double ang = -2.0 * PI * (double)j / (double)S.K[k];
+                twd.push_back(std::cos(ang));
+                twd.push_back(std::sin(ang));
+            }
+            for (int j = 0; j < S.K[k]; ++j) {
+                int mm = (j <= S.K[k] / 2) ? j : j - S.K[k];
+                double B;
+                if (S.physics == DMDX_FAITHFUL) {
+                    // src/force/coulomb_pme.cpp:159-163
+                    if (mm == 0) B = 1.0;
+                    else {
+                        double t = PI * std::abs(mm) / S.K[k];
+                        B = std::pow(std::sin(t) / t, 2 * 4);
+                    }
+                } else {
+                    double dd = 2.0 / 3.0 + std::cos(2.0 * PI * mm / S.K[k]) / 3.0;
+                    B = 1.0 / (dd * dd);
+                }
+                bmod.push_back(B);
+            }
+        }
+        // tiles: lines along dim k are indexed by the other two coordinates
+        S.pw_off = pw_total;
+        for (int k = 0; k < 3; ++k) {
+            int K = S.K[k];
+            int nlines = (k == 0) ? S.K[1] * S.K[2] : (k == 1 ? S.K[0] * S.K[2] : S.K[0] * S.K[1]);
+            int B = (int)std::max<size_t>(1, std::min<size_t>(32, (48 * 1024) / (2 * K * cx)));
+            int local = 0;
+            for (int l0 = 0; l0 < nlines; l0 += B) {
+                FftTile t;
+                t.sys = s;
+                t.line0 = l0;
+                t.nlines = std::min(B, nlines - l0);
+                t.tile_local = local++;
+                tiles[k].push_back(t);
+                d.lds_bytes[k] = std::max(d.lds_bytes[k], (size_t)(2 * t.nlines * K * cx));
+            }
+            if (k == 2) {
+                S.pw_n = local;
+                pw_total += local;
+            }
+        }
+    }
+    for (int k = 0; k < 3; ++k) d.ntiles[k] = (int)tiles[k].size();
+
+    // ---- what exists
+    for (int s = 0; s < nsys; ++s) {
+        const SysDev& S = hsys[s];
+        d.any_list |= S.has_list != 0;
+        d.any_pme |= S.pme != 0;
+        d.any_excl |= S.n_ex > 0;
+        d.any_baro |= S.barostat != DMDX_BARO_NONE;
+        host[s].has_list = S.has_list != 0;
+        host[s].interval = S.interval;
+        host[s].counter = hstate[s].counter;
+    }
+    d.any_gather = d.nb || d.na || d.nd || d.ni || d.ne || d.any_pme;
+
+    // ---- device allocations
+    TRY(dev_upload(&d.sys, hsys, owned));
+    TRY(dev_upload(&d.st, hstate, owned));
+    TRY(dev_upload(&d.slot_sys, slot_sys, owned));
+    TRY(dev_upload(&d.x, x, owned));
+    TRY(dev_upload(&d.y, y, owned));
+    TRY(dev_upload(&d.z, z, owned));
+    TRY(dev_upload(&d.vx, vx, owned));
+    TRY(dev_upload(&d.vy, vy, owned));
+    TRY(dev_upload(&d.vz, vz, owned));
+    TRY(dev_alloc_zero(&d.fx, slot, owned));
+    TRY(dev_alloc_zero(&d.fy, slot, owned));
+    TRY(dev_alloc_zero(&d.fz, slot, owned));
+    TRY(dev_upload(&d.invm, invm, owned));
+    TRY(dev_upload(&d.m, m, owned));
+    TRY(dev_upload(&d.q, q, owned));
+    TRY(dev_upload(&d.type, type, owned));
+    TRY(dev_alloc_zero(&d.acell, slot, owned));
+    TRY(dev_alloc_zero(&d.ccount, ncells, owned));
+    TRY(dev_alloc_zero(&d.cstart, ncells, owned));
+    TRY(dev_alloc_zero(&d.cfill, ncells, owned));
+    TRY(dev_alloc_zero(&d.cell_atoms, slot, owned));
+    TRY(dev_upload(&d.cell_sys, cell_sys, owned));
+    {
+        std::vector<int> ident(slot);
+        for (int64_t g = 0; g < slot; ++g) ident[g] = (int)g;
+        TRY(dev_upload(&d.order, ident, owned));
+    }
+    TRY(dev_alloc_zero(&d.nn, slot, owned));
+    TRY(dev_alloc_zero(&d.nbr, nbr_total, owned));
+    TRY(dev_upload(&d.ex_start, ex_start, owned));
+    TRY(dev_upload(&d.ex_list, ex_list, owned));
+    TRY(dev_upload(&d.lj_sig, lj_sig, owned));
+    TRY(dev_upload(&d.lj_eps, lj_eps, owned));
+    TRY(dev_upload(&d.lj_c12, c12, owned));
+    TRY(dev_upload(&d.lj_c6, c6, owned));
+    TRY(dev_upload(&d.b_sys, b_sys, owned));
+    TRY(dev_upload(&d.bi, bi, owned));
+    TRY(dev_upload(&d.bj, bj, owned));
+    TRY(dev_upload(&d.bk, bk, owned));
+    TRY(dev_upload(&d.br0, br0, owned));
+    TRY(dev_upload(&d.a_sys, a_sys, owned));
+    TRY(dev_upload(&d.ai, ai, owned));
+    TRY(dev_upload(&d.aj, aj, owned));
+    TRY(dev_upload(&d.ak, ak, owned));
+    TRY(dev_upload(&d.akt, akt, owned));
+    TRY(dev_upload(&d.at0, at0, owned));
+    TRY(dev_upload(&d.d_sys, d_sys, owned));
+    TRY(dev_upload(&d.di, di, owned));
+    TRY(dev_upload(&d.dj, dj, owned));
+    TRY(dev_upload(&d.dk, dk, owned));
+    TRY(dev_upload(&d.dl, dl, owned));
+    TRY(dev_upload(&d.dper, dper, owned));
+    TRY(dev_upload(&d.dkp, dkp, owned));
+    TRY(dev_upload(&d.dp0, dp0, owned));
+    TRY(dev_upload(&d.i_sys, i_sys, owned));
+    TRY(dev_upload(&d.ii, ii, owned));
+    TRY(dev_upload(&d.ij, ij, owned));
+    TRY(dev_upload(&d.ik, ik, owned));
+    TRY(dev_upload(&d.il, il, owned));
+    TRY(dev_upload(&d.ikp, ikp, owned));
+    TRY(dev_upload(&d.ip0, ip0, owned));
+    TRY(dev_upload(&d.e_sys, e_sys, owned));
+    TRY(dev_upload(&d.ei, ei, owned));
+    TRY(dev_upload(&d.ej, ej, owned));
+    TRY(dev_alloc_zero(&d.tf, tf_total, owned));
+    TRY(dev_upload(&d.c_start, c_start, owned));
+    TRY(dev_upload(&d.c_mid, c_mid, owned));
+    TRY(dev_upload(&d.contrib, contrib, owned));
+    d.grid_total = grid_total;
+    if (prec == DMDX_FP64) {
+        TRY(dev_alloc_zero((double**)&d.grid, 2 * grid_total, owned));
+        TRY(dev_upload((double**)&d.tw, twd, owned));
+    } else {
+        std::vector<float> twf(twd.begin(), twd.end());
+        TRY(dev_alloc_zero((float**)&d.grid, 2 * grid_total, owned));
+        TRY(dev_upload((float**)&d.tw, twf, owned));
+    }
+    TRY(dev_upload(&d.grid_sys, grid_sys, owned));
+    TRY(dev_upload(&d.bmod, bmod, owned));
+    TRY(dev_upload(&d.plans, plans, owned));
+    for (int k = 0; k < 3; ++k) TRY(dev_upload(&d.tiles[k], tiles[k], owned));
+    const int64_t nw = slot / WAVE;
+    TRY(dev_alloc_zero(&d.p_lj, nw, owned));
+    TRY(dev_alloc_zero(&d.p_coul, nw, owned));
+    TRY(dev_alloc_zero(&d.p_ke, nw, owned));
+    TRY(dev_alloc_zero(&d.p_w, nw, owned));
+    TRY(dev_alloc_zero(&d.p_bond, std::max<int64_t>(ob / WAVE, 1), owned));
+    TRY(dev_alloc_zero(&d.p_angle, std::max<int64_t>(oa / WAVE, 1), owned));
+    TRY(dev_alloc_zero(&d.p_dih, std::max<int64_t>(od / WAVE, 1), owned));
+    TRY(dev_alloc_zero(&d.p_imp, std::max<int64_t>(oi / WAVE, 1), owned));
+    TRY(dev_alloc_zero(&d.p_excl, std::max<int64_t>(oe / WAVE, 1), owned));
+    TRY(dev_alloc_zero(&d.p_pme, std::max(pw_total, 1), owned));
+    DMDX_HIP(hipDeviceSynchronize());
+    return DMDX_OK;
+}
+
+bool Batch::exclusion_term_active(const dmdx_system_desc& D, const BatchOptions& opt)
+{
+    if (opt.component == COMP_EXCLUSION) return true;
+    if (opt.component != COMP_NONE) return false;
+    // build_force_engine (src/sim/simulation_engine.cpp:168-200): PME always
+    // adds CoulombExclusion; "cutoff" adds it when exclusions exist.
+    // CORRECTED: only PME needs the reciprocal-space correction (excluded
+    // pairs are removed from the real-space list instead).
+    if (D.physics == DMDX_FAITHFUL)
+        return D.coulomb_type == DMDX_COULOMB_PME ||
+               (D.coulomb_type == DMDX_COULOMB_CUTOFF && D.n_excl > 0);
+    return D.coulomb_type == DMDX_COULOMB_PME;
+}
+
+#define LAUNCH(expr)                                                          \
+    do {                                                                      \
+        hipError_t e_ = (expr);                                               \
+        if (e_ != hipSuccess) return fail(DMDX_E_GPU, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// host mirror of the per-system list counters (the device keeps its own
+// authoritative copy); decides whether any list kernel has to be launched
+bool Batch::mirror_compute(int phase, int force_rebuild, bool* any_force)
+{
+    bool any_rebuild = false;
+    *any_force = false;
+    for (size_t s = 0; s < host.size(); ++s) {
+        HostSys& H = host[s];
+        if (!H.active) continue;
+        bool rebuild = H.has_list && (force_rebuild || (H.interval > 0 && H.counter >= H.interval));
+        bool reuse = phase == 1 && options.reuse_forces && H.valid && !rebuild;
+        if (H.has_list && H.interval > 0) H.counter = (rebuild ? 0 : H.counter) + 1;
+        H.valid = true;
+        any_rebuild |= rebuild;
+        *any_force |= !reuse;
+    }
+    return any_rebuild;
+}
+
+int Batch::enqueue_compute(int phase, int accumulate, int force_rebuild, bool uniform)
+{
+    bool any_force = true, any_rebuild = true;
+    if (!uniform) any_rebuild = mirror_compute(phase, force_rebuild, &any_force);
+    else {
+        bool dummy;
+        mirror_compute(phase, force_rebuild, &dummy);
+    }
+    LAUNCH(launch_pre_compute(d, stream, phase, force_rebuild, options.reuse_forces));
+    if (!any_force) return DMDX_OK;
+    if (d.any_list && any_rebuild) LAUNCH(launch_build_lists(d, stream));
+    LAUNCH(launch_pair(d, stream, accumulate));
+    LAUNCH(launch_bonded(d, stream));
+    LAUNCH(launch_pme(d, stream));
+    LAUNCH(launch_gather(d, stream));
+    return DMDX_OK;
+}
+
+int Batch::enqueue_step(bool uniform)
+{
+    TRY(enqueue_compute(1, 0, 0, uniform));
+    LAUNCH(launch_kick_drift(d, stream));
+    TRY(enqueue_compute(2, 0, 0, uniform));
+    LAUNCH(launch_kick_ke(d, stream));
+    LAUNCH(launch_finalize(d, stream, 1));
+    if (d.any_baro) {
+        LAUNCH(launch_barostat(d, stream));
+        for (auto& H : host)
+            if (H.has_barostat) H.valid = false;
+    }
+    return DMDX_OK;
+}
+
+int Batch::step(int64_t nsteps)
+{
+    if (nsteps < 0) return fail(DMDX_E_INVALID, "negative step count");
+    if (!options.use_graph) {
+        for (int64_t i = 0; i < nsteps; ++i) TRY(enqueue_step(false));
+        return DMDX_OK;
+    }
+    // graph mode: every compute launches every kernel (per-system decisions
+    // are taken on the device), so one captured block of graph_steps steps
+    // is valid for any counter state
+    const int G = std::max(1, options.graph_steps);
+    int64_t done = 0;
+    if (nsteps >= G && !graph_exec) {
+        DMDX_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        std::vector<HostSys> saved = host;
+        int rc = DMDX_OK;
+        for (int i = 0; i < G && rc == DMDX_OK; ++i) rc = enqueue_step(true);
+        hipGraph_t g = nullptr;
+        hipError_t ec = hipStreamEndCapture(stream, &g);
+        host = saved;  // the capture only recorded; nothing ran
+        if (rc != DMDX_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        DMDX_HIP(ec);
+        graph = g;
+        DMDX_HIP(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
+    }
+    while (nsteps - done >= G && graph_exec) {
+        DMDX_HIP(hipGraphLaunch(graph_exec, stream));
+        for (int i = 0; i < G; ++i) {
+            bool dummy;
+            mirror_compute(1, 0, &dummy);
+            mirror_compute(2, 0, &dummy);
+            for (auto& H : host)
+                if (H.has_barostat) H.valid = false;
+        }
+        done += G;
+    }
+    for (; done < nsteps; ++done) TRY(enqueue_step(false));
+    return DMDX_OK;
+}
+
+void Batch::invalidate_graph()
+{
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    graph_exec = nullptr;
+    graph = nullptr;
+}
+
+int Batch::compute(int accumulate)
+{
+    TRY(enqueue_compute(0, accumulate, 0, false));
+    LAUNCH(launch_finalize(d, stream, 0));
+    return DMDX_OK;
+}
+
+int Batch::rebuild_only()
+{
+    LAUNCH(launch_pre_rebuild_only(d, stream));
+    if (d.any_list) LAUNCH(launch_build_lists(d, stream));
+    for (auto& H : host)
+        if (H.has_list) H.counter = 0;
+    return DMDX_OK;
+}
+
+int Batch::copy(void* dst, const void* src, size_t bytes, hipMemcpyKind kind)
+{
+    DMDX_HIP(hipMemcpyAsync(dst, src, bytes, kind, stream));
+    DMDX_HIP(hipStreamSynchronize(stream));
+    return DMDX_OK;
+}
+
+int Batch::sync()
+{
+    DMDX_HIP(hipStreamSynchronize(stream));
+    return DMDX_OK;
+}
+
+int Batch::fetch_state()
+{
+    TRY(sync());
+    TRY(copy(hstate.data(), d.st, sizeof(SysState) * hstate.size(), hipMemcpyDeviceToHost));
+    return DMDX_OK;
+}
+
+int Batch::check_sys(int s) const
+{
+    if (s < 0 || s >= d.nsys) return fail(DMDX_E_INVALID, "system index out of range");
+    return DMDX_OK;
+}
+
+int Batch::get_arrays(int s, int what, double* a, double* b, double* c)
+{
+    TRY(check_sys(s));
+    if (what == 1) LAUNCH(launch_materialize(d, stream));
+    TRY(sync());
+    const SysDev& S = hsys[s];
+    double* src[3];
+    if (what == 0) { src[0] = d.x; src[1] = d.y; src[2] = d.z; }
+    else if (what == 1) { src[0] = d.vx; src[1] = d.vy; src[2] = d.vz; }
+    else if (what == 2) { src[0] = d.fx; src[1] = d.fy; src[2] = d.fz; }
+    else return fail(DMDX_E_INVALID, "what must be 0 (positions), 1 (velocities) or 2 (forces)");
+    double* dst[3] = {a, b, c};
+    for (int k = 0; k < 3; ++k)
+        if (dst[k]) TRY(copy(dst[k], src[k] + S.atom_off, sizeof(double) * S.n, hipMemcpyDeviceToHost));
+    return DMDX_OK;
+}
+
+int Batch::set_arrays(int s, int what, const double* a, const double* b, const double* c)
+{
+    TRY(check_sys(s));
+    TRY(sync());
+    const SysDev& S = hsys[s];
+    double* dst[3];
+    if (what == 0) { dst[0] = d.x; dst[1] = d.y; dst[2] = d.z; }
+    else if (what == 1) {
+        LAUNCH(launch_materialize(d, stream));
+        TRY(sync());
+        dst[0] = d.vx; dst[1] = d.vy; dst[2] = d.vz;
+    } else if (what == 2) { dst[0] = d.fx; dst[1] = d.fy; dst[2] = d.fz; }
+    else return fail(DMDX_E_INVALID, "what must be 0, 1 or 2");
+    const double* src[3] = {a, b, c};
+    for (int k = 0; k < 3; ++k)
+        if (src[k]) TRY(copy(dst[k] + S.atom_off, src[k], sizeof(double) * S.n, hipMemcpyHostToDevice));
+    if (what == 0) {
+        // positions changed: forces of the last compute no longer apply
+        TRY(fetch_state());
+        hstate[s].valid = 0;
+        TRY(copy(&d.st[s].valid, &hstate[s].valid, sizeof(int32_t), hipMemcpyHostToDevice));
+        host[s].valid = false;
+    }
+    return DMDX_OK;
+}
+
+int Batch::set_box(int s, const double L[3])
+{
+    TRY(check_sys(s));
+    SysDev& S = hsys[s];
+    bool same = S.L[0] == L[0] && S.L[1] == L[1] && S.L[2] == L[2];
+    if (same) return DMDX_OK;
+    TRY(sync());
+    // the device copy may have been changed by a barostat: read it back first
+    TRY(copy(&S, d.sys + s, sizeof(SysDev), hipMemcpyDeviceToHost));
+    for (int k = 0; k < 3; ++k) {
+        if (!(L[k] > 0.0)) return fail(DMDX_E_PARM, "box edges must be positive");
+        S.L[k] = L[k];
+        S.invL[k] = 1.0 / L[k];
+    }
+    TRY(copy(d.sys + s, &S, sizeof(SysDev), hipMemcpyHostToDevice));
+    host[s].valid = false;
+    TRY(fetch_state());
+    hstate[s].valid = 0;
+    TRY(copy(&d.st[s].valid, &hstate[s].valid, sizeof(int32_t), hipMemcpyHostToDevice));
+    return DMDX_OK;
+}
+
+int Batch::upload_sys(int s)
+{
+    TRY(copy(d.sys + s, &hsys[s], sizeof(SysDev), hipMemcpyHostToDevice));
+    return DMDX_OK;
+}
+
+int Batch::set_counter(int s, int counter)
+{
+    TRY(check_sys(s));
+    TRY(fetch_state());
+    hstate[s].counter = counter;
+    host[s].counter = counter;
+    TRY(copy(&d.st[s].counter, &counter, sizeof(int32_t), hipMemcpyHostToDevice));
+    return DMDX_OK;
+}
+
+int Batch::set_active(int s, int active)
+{
+    TRY(check_sys(s));
+    TRY(fetch_state());
+    hstate[s].active = active ? 1 : 0;
+    host[s].active = active != 0;
+    TRY(copy(&d.st[s].active, &hstate[s].active, sizeof(int32_t), hipMemcpyHostToDevice));
+    return DMDX_OK;
+}
+
+int Batch::get_pairs(int s, int32_t* pi, int32_t* pj, int64_t cap, int64_t* count)
+{
+    TRY(check_sys(s));
+    TRY(sync());
+    const SysDev& S = hsys[s];
+    std::vector<int> order(S.n_pad), nn(S.n_pad);
+    TRY(copy(order.data(), d.order + S.atom_off, sizeof(int) * S.n_pad, hipMemcpyDeviceToHost));
+    TRY(copy(nn.data(), d.nn + S.atom_off, sizeof(int) * S.n_pad, hipMemcpyDeviceToHost));
+    std::vector<int> blk;
+    if (S.has_list) {
+        blk.resize((size_t)S.max_nbr * S.n_pad);
+        TRY(copy(blk.data(), d.nbr + S.nbr_off, sizeof(int) * blk.size(), hipMemcpyDeviceToHost));
+    }
+    std::vector<int64_t> keys;
+    for (int t = 0; t < S.n; ++t) {
+        int i = order[t] - (int)S.atom_off;
+        for (int k = 0; k < nn[t]; ++k) {
+            int j = blk[(size_t)k * S.n_pad + t] - (int)S.atom_off;
+            if (j > i) keys.push_back(((int64_t)i << 32) | (int64_t)j);
+        }
+    }
+    std::sort(keys.begin(), keys.end());
+    *count = (int64_t)keys.size();
+    for (int64_t p = 0; p < (int64_t)keys.size() && p < cap; ++p) {
+        pi[p] = (int32_t)(keys[p] >> 32);
+        pj[p] = (int32_t)(keys[p] & 0xffffffff);
+    }
+    return DMDX_OK;
+}
+
+}  // namespace dmdx
+
+// ===================================================================== C-ABI
+using dmdx::Batch;
+
+struct dmdx_engine {
+    Batch b;
+};
+
+extern "C" {
+
+const char* dmdx_last_error(void) { return dmdx::last_error(); }
+
+int dmdx_device_info(int32_t* device_count, int64_t* free_bytes, int64_t* total_bytes)
+{
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) return dmdx::fail(DMDX_E_GPU, std::string("hipGetDeviceCount: ") + hipGetErrorString(e));
+    if (device_count) *device_count = n;
+    if (n > 0 && (free_bytes || total_bytes)) {
+        size_t f = 0, t = 0;
+        e = hipMemGetInfo(&f, &t);
+        if (e != hipSuccess) return dmdx::fail(DMDX_E_GPU, std::string("hipMemGetInfo: ") + hipGetErrorString(e));
+        if (free_bytes) *free_bytes = (int64_t)f;
+        if (total_bytes) *total_bytes = (int64_t)t;
+    }
+    return DMDX_OK;
+}
+
+int dmdx_engine_create(const dmdx_system_desc* descs, int32_t n_systems, dmdx_engine** out)
+{
+    if (!descs || !out) return dmdx::fail(DMDX_E_INVALID, "null argument");
+    auto* e = new dmdx_engine();
+    dmdx::BatchOptions opt;
+    int rc = e->b.create(descs, n_systems, opt);
+    if (rc != DMDX_OK) {
+        delete e;
+        return rc;
+    }
+    *out = e;
+    return DMDX_OK;
+}
+
+int dmdx_engine_destroy(dmdx_engine* e)
+{
+    delete e;
+    return DMDX_OK;
+}
+
+int dmdx_engine_set_options(dmdx_engine* e, int32_t reuse_forces, int32_t use_graph,
+                            int32_t graph_steps)
+{
+    if (!e) return dmdx::fail(DMDX_E_INVALID, "null engine");
+    e->b.invalidate_graph();
+    e->b.options.reuse_forces = reuse_forces != 0;
+    e->b.options.use_graph = use_graph != 0;
+    e->b.options.graph_steps = graph_steps > 0 ? graph_steps : 10;
+    return DMDX_OK;
+}
+
+int dmdx_engine_step(dmdx_engine* e, int64_t nsteps)
+{
+    if (!e) return dmdx::fail(DMDX_E_INVALID, "null engine");
+    return e->b.step(nsteps);
+}
+
+int dmdx_engine_compute(dmdx_engine* e)
+{
+    if (!e) return dmdx::fail(DMDX_E_INVALID, "null engine");
+    return e->b.compute(0);
+}
+
+int dmdx_engine_sync(dmdx_engine* e)
+{
+    if (!e) return dmdx::fail(DMDX_E_INVALID, "null engine");
+    return e->b.sync();
+}
+
+int dmdx_engine_get_arrays(dmdx_engine* e, int32_t sys, int32_t what, double* x, double* y, double* z)
+{
+    if (!e) return dmdx::fail(DMDX_E_INVALID, "null engine");
+    return e->b.get_arrays(sys, what, x, y, z);
+}
+
+int dmdx_engine_set_arrays(dmdx_engine* e, int32_t sys, int32_t what, const double* x,
+                           const double* y, const double* z)
+{
+    if (!e) return dmdx::fail(DMDX_E_INVALID, "null engine");
+    return e->b.set_arrays(sys, what, x, y, z);
+}
+
+int dmdx_engine_get_scalars(dmdx_engine* e, int32_t sys, dmdx_scalars* out)
+{
+    if (!e || !out) return dmdx::fail(DMDX_E_INVALID, "null argument");
+    int rc = e->b.check_sys(sys);
+    if (rc) return rc;
+    rc = e->b.fetch_state();
+    if (rc) return rc;
+    dmdx::SysDev S;
+    rc = e->b.copy(&S, e->b.d.sys + sys, sizeof(S), hipMemcpyDeviceToHost);
+    if (rc) return rc;
+    const dmdx::SysState& T = e->b.hstate[sys];
+    out->potential_energy = T.pe;
+    for (int k = 0; k < 8; ++k) out->energy_terms[k] = T.pe_terms[k];
+    out->kinetic_energy = T.ke_out;
+    out->zeta = T.zeta;
+    out->piston_velocity = T.eps_dot;
+    out->virial = T.virial;
+    out->pressure = T.pressure;
+    for (int k = 0; k < 3; ++k) out->box[k] = S.L[k];
+    out->step_since_rebuild = T.counter;
+    out->flags = T.flags;
+    out->steps_done = T.steps_done;
+    return DMDX_OK;
+}
+
+int dmdx_engine_set_step_since_rebuild(dmdx_engine* e, int32_t sys, int32_t counter)
+{
+    if (!e) return dmdx::fail(DMDX_E_INVALID, "null engine");
+    return e->b.set_counter(sys, counter);
+}
+
+int dmdx_engine_set_active(dmdx_engine* e, int32_t sys, int32_t active)
+{
+    if (!e) return dmdx::fail(DMDX_E_INVALID, "null engine");
+    return e->b.set_active(sys, active);
+}
+
+int dmdx_engine_get_pairs(dmdx_engine* e, int32_t sys, int32_t* pi, int32_t* pj, int64_t cap,
+                          int64_t* count)
+{
+    if (!e || !count) return dmdx::fail(DMDX_E_INVALID, "null argument");
+    return e->b.get_pairs(sys, pi, pj, cap, count);
+}
+
+int dmdx_engine_n_systems(dmdx_engine* e, int32_t* n)
+{
+    if (!e || !n) return dmdx::fail(DMDX_E_INVALID, "null argument");
+    *n = e->b.d.nsys;
+    return DMDX_OK;
+}
+
+}  // extern "C"

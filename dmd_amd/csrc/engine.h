@@ -1,0 +1,81 @@
+// engine.h -- host-side batch object of the step engine.
+#pragma once
+
+#include "dmdx_internal.h"
+
+namespace dmdx {
+
+const char* last_error();
+
+// Which single ForceComponent a batch stands for (ForceComponent-level API);
+// COMP_NONE = the whole ForceEngine of a SimulationEngine.
+enum Component {
+    COMP_NONE = 0,
+    COMP_LJ,
+    COMP_COULOMB_DIRECT,
+    COMP_EXCLUSION,
+    COMP_PME,
+    COMP_BOND,
+    COMP_ANGLE,
+    COMP_DIHEDRAL,
+    COMP_IMPROPER
+};
+
+struct BatchOptions {
+    int component = COMP_NONE;
+    bool reuse_forces = true;  // skip a first compute whose result is known
+    bool use_graph = false;    // replay captured blocks of steps
+    int graph_steps = 10;
+    int max_nbr = 0;           // 0: estimate from the density
+};
+
+// host mirror of what the host needs to decide launches
+struct HostSys {
+    int n = 0;
+    bool has_list = false;
+    int interval = 10;
+    int counter = 10;
+    bool valid = false;
+    bool active = true;
+    bool has_barostat = false;
+};
+
+class Batch {
+public:
+    ~Batch();
+    int create(const dmdx_system_desc* descs, int nsys, const BatchOptions& opt);
+    int step(int64_t nsteps);
+    int compute(int accumulate);  // one standalone ForceEngine::compute + energy
+    int rebuild_only();           // rebuild_pair_list
+    int sync();
+    int copy(void* dst, const void* src, size_t bytes, hipMemcpyKind kind);
+    int fetch_state();
+    int check_sys(int s) const;
+    int get_arrays(int s, int what, double* a, double* b, double* c);
+    int set_arrays(int s, int what, const double* a, const double* b, const double* c);
+    int set_box(int s, const double L[3]);
+    int set_counter(int s, int counter);
+    int set_active(int s, int active);
+    int get_pairs(int s, int32_t* pi, int32_t* pj, int64_t cap, int64_t* count);
+    int upload_sys(int s);
+    void invalidate_graph();
+
+    Dev d;
+    BatchOptions options;
+    std::vector<SysDev> hsys;
+    std::vector<SysState> hstate;
+    std::vector<HostSys> host;
+    hipStream_t stream = nullptr;
+
+private:
+    static bool exclusion_term_active(const dmdx_system_desc& D, const BatchOptions& opt);
+    bool mirror_compute(int phase, int force_rebuild, bool* any_force);
+    int enqueue_compute(int phase, int accumulate, int force_rebuild, bool uniform);
+    int enqueue_step(bool uniform);
+
+    std::vector<void*> owned;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+};
+
+}  // namespace dmdx

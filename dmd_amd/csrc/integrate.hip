@@ -1,0 +1,513 @@
+// integrate.hip -- per-atom force gather, velocity-Verlet kick/drift,
+// deterministic per-system reductions, thermostats and barostats (FP64).
+//
+//   k_pre_compute   per system: list-rebuild decision and counter
+//                   (src/force/lennard_jones.cpp:53-55, :91), force reuse
+//   k_gather        per atom: bonded contributions, PME interpolation
+//                   (src/force/coulomb_pme.cpp:205-283), exclusion
+//                   contributions -- in the reference's component order
+//   k_kick_drift    v = lambda v + F/m dt/2 ; x += v dt
+//                   (src/integrate/integrator.cpp:4-20; lambda is the
+//                   thermostat scale of the previous step, applied here)
+//   k_kick_ke       v += F/m dt/2 and per-wave kinetic-energy partials
+//                   (src/thermostat/thermostat.cpp:6-15)
+//   k_finalize      per system: energies, temperature, Berendsen
+//                   (src/thermostat/berendsen.cpp:13-28) or Nose-Hoover
+//                   (src/thermostat/nose_hoover.cpp:18-32) scale
+//   k_baro_*        virial, pressure and Berendsen / Andersen scaling
+//                   (src/barostat/*.cpp), only launched with a barostat
+#include "dmdx_internal.h"
+
+namespace dmdx {
+
+template <class R>
+struct Cx2 {
+    R re, im;
+};
+
+// phase: 0 standalone compute, 1 first compute of a step, 2 second compute;
+// force_rebuild: 1 = rebuild_pair_list requested by the host
+__global__ void k_pre_compute(const SysDev* __restrict__ sys, SysState* __restrict__ st, int nsys,
+                              int phase, int force_rebuild, int reuse_enabled)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsys) return;
+    SysState& T = st[s];
+    if (!T.active) {
+        T.rebuild = 0;
+        T.reuse = 1;
+        return;
+    }
+    const SysDev& S = sys[s];
+    int rebuild = 0;
+    if (S.has_list) rebuild = force_rebuild || (S.interval > 0 && T.counter >= S.interval);
+    T.rebuild = rebuild;
+    T.reuse = (phase == 1 && reuse_enabled && T.valid && !rebuild) ? 1 : 0;
+    if (S.has_list && S.interval > 0) T.counter = (rebuild ? 0 : T.counter) + 1;
+    T.valid = 1;
+}
+
+// explicit rebuild_pair_list (src/force/lennard_jones.cpp:18-41): builds the
+// list and leaves the counter at 0, without computing forces
+__global__ void k_pre_rebuild_only(const SysDev* __restrict__ sys, SysState* __restrict__ st,
+                                   int nsys)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsys) return;
+    st[s].rebuild = sys[s].has_list ? 1 : 0;
+    st[s].reuse = 1;  // no force kernels
+    if (sys[s].has_list) st[s].counter = 0;
+}
+
+// Per-atom gather.  cstart[g] .. cmid[g]: bonded contributions (bonds,
+// angles, dihedrals, impropers in term order); cmid[g] .. cstart[g+1]:
+// exclusion contributions.  PME sits between them, as in build_force_engine
+// (src/sim/simulation_engine.cpp:147-203).
+template <class R>
+__global__ void k_gather(const SysDev* __restrict__ sys, const SysState* __restrict__ st,
+                         const int* __restrict__ slot_sys, int64_t nslots,
+                         const int* __restrict__ cstart, const int* __restrict__ cmid,
+                         const int* __restrict__ contrib, const TermF* __restrict__ tf,
+                         const double* __restrict__ x, const double* __restrict__ y,
+                         const double* __restrict__ z, const double* __restrict__ q,
+                         const Cx2<R>* __restrict__ grid, double* __restrict__ fx,
+                         double* __restrict__ fy, double* __restrict__ fz)
+{
+    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nslots) return;
+    int s = slot_sys[g];
+    if (s < 0 || st[s].reuse || !st[s].active) return;
+    const SysDev& S = sys[s];
+    if (g - S.atom_off >= S.n) return;
+    int c0 = cstart[g], c1 = cmid[g], c2 = cstart[g + 1];
+    double qi = S.pme ? q[g] : 0.0;
+    if (c0 == c2 && qi == 0.0) return;
+    double Fx = fx[g], Fy = fy[g], Fz = fz[g];
+    for (int c = c0; c < c1; ++c) {
+        TermF v = tf[contrib[c]];
+        Fx += v.x;
+        Fy += v.y;
+        Fz += v.z;
+    }
+    if (qi != 0.0) {
+        const int nx = S.K[0], ny = S.K[1], nz = S.K[2];
+        const double Lx = S.L[0], Ly = S.L[1], Lz = S.L[2];
+        double frac_x = x[g] / Lx, frac_y = y[g] / Ly, frac_z = z[g] / Lz;
+        frac_x -= floor(frac_x);
+        frac_y -= floor(frac_y);
+        frac_z -= floor(frac_z);
+        double ux = frac_x * nx, uy = frac_y * ny, uz = frac_z * nz;
+        int m0x = (int)floor(ux) - 1, m0y = (int)floor(uy) - 1, m0z = (int)floor(uz) - 1;
+        R wx[4], wy[4], wz[4], dwx[4], dwy[4], dwz[4];
+        for (int d = 0; d < 4; ++d) {
+            R tx = (R)(ux - (m0x + d)), ty = (R)(uy - (m0y + d)), tz = (R)(uz - (m0z + d));
+            R a;
+            // theta4 / dtheta4, src/force/coulomb_pme.cpp:50-69
+            a = fabs(tx);
+            wx[d] = a >= R(2) ? R(0) : (a >= R(1) ? (R(2) - a) * (R(2) - a) * (R(2) - a) / R(6)
+                                                 : (R(4) - R(6) * a * a + R(3) * a * a * a) / R(6));
+            dwx[d] = a >= R(2) ? R(0)
+                               : (a >= R(1) ? -(tx >= R(0) ? R(1) : R(-1)) * (R(2) - a) * (R(2) - a) / R(2)
+                                            : (tx >= R(0) ? R(1) : R(-1)) * (R(9) * a * a - R(12) * a) / R(6));
+            a = fabs(ty);
+            wy[d] = a >= R(2) ? R(0) : (a >= R(1) ? (R(2) - a) * (R(2) - a) * (R(2) - a) / R(6)
+                                                 : (R(4) - R(6) * a * a + R(3) * a * a * a) / R(6));
+            dwy[d] = a >= R(2) ? R(0)
+                               : (a >= R(1) ? -(ty >= R(0) ? R(1) : R(-1)) * (R(2) - a) * (R(2) - a) / R(2)
+                                            : (ty >= R(0) ? R(1) : R(-1)) * (R(9) * a * a - R(12) * a) / R(6));
+            a = fabs(tz);
+            wz[d] = a >= R(2) ? R(0) : (a >= R(1) ? (R(2) - a) * (R(2) - a) * (R(2) - a) / R(6)
+                                                 : (R(4) - R(6) * a * a + R(3) * a * a * a) / R(6));
+            dwz[d] = a >= R(2) ? R(0)
+                               : (a >= R(1) ? -(tz >= R(0) ? R(1) : R(-1)) * (R(2) - a) * (R(2) - a) / R(2)
+                                            : (tz >= R(0) ? R(1) : R(-1)) * (R(9) * a * a - R(12) * a) / R(6));
+        }
+        // FAITHFUL keeps the 1/(nx ny nz) scaling of the potential grid
+        // (src/force/coulomb_pme.cpp:303-306)
+        const bool norm = S.physics == DMDX_FAITHFUL;
+        const R nrm = (R)(1.0 / ((double)nx * ny * nz));
+        const Cx2<R>* G = grid + S.grid_off;
+        R sx = R(0), sy = R(0), sz = R(0);
+        for (int iz = 0; iz < 4; ++iz) {
+            int gz = (m0z + iz) % nz;
+            if (gz < 0) gz += nz;
+            R w_z = wz[iz], dw_z = dwz[iz];
+            for (int iy = 0; iy < 4; ++iy) {
+                int gy = (m0y + iy) % ny;
+                if (gy < 0) gy += ny;
+                R w_y = wy[iy], dw_y = dwy[iy];
+                int64_t row = ((int64_t)gz * ny + gy) * nx;
+                for (int ix = 0; ix < 4; ++ix) {
+                    int gx = (m0x + ix) % nx;
+                    if (gx < 0) gx += nx;
+                    R phi = G[row + gx].re;
+                    if (norm) phi *= nrm;
+                    sx += phi * dwx[ix] * w_y * w_z;
+                    sy += phi * wx[ix] * dw_y * w_z;
+                    sz += phi * wx[ix] * w_y * dw_z;
+                }
+            }
+        }
+        double fxf = (double)nx / Lx, fyf = (double)ny / Ly, fzf = (double)nz / Lz;
+        Fx -= qi * fxf * (double)sx;
+        Fy -= qi * fyf * (double)sy;
+        Fz -= qi * fzf * (double)sz;
+    }
+    for (int c = c1; c < c2; ++c) {
+        TermF v = tf[contrib[c]];
+        Fx += v.x;
+        Fy += v.y;
+        Fz += v.z;
+    }
+    fx[g] = Fx;
+    fy[g] = Fy;
+    fz[g] = Fz;
+}
+
+template __global__ void k_gather<double>(const SysDev*, const SysState*, const int*, int64_t,
+                                          const int*, const int*, const int*, const TermF*,
+                                          const double*, const double*, const double*,
+                                          const double*, const Cx2<double>*, double*, double*,
+                                          double*);
+template __global__ void k_gather<float>(const SysDev*, const SysState*, const int*, int64_t,
+                                         const int*, const int*, const int*, const TermF*,
+                                         const double*, const double*, const double*,
+                                         const double*, const Cx2<float>*, double*, double*,
+                                         double*);
+
+__global__ void k_kick_drift(const SysDev* __restrict__ sys, const SysState* __restrict__ st,
+                             const int* __restrict__ slot_sys, int64_t nslots,
+                             const double* __restrict__ invm, const double* __restrict__ fx,
+                             const double* __restrict__ fy, const double* __restrict__ fz,
+                             double* __restrict__ vx, double* __restrict__ vy,
+                             double* __restrict__ vz, double* __restrict__ x,
+                             double* __restrict__ y, double* __restrict__ z)
+{
+    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nslots) return;
+    int s = slot_sys[g];
+    if (s < 0 || !st[s].active) return;
+    const SysDev& S = sys[s];
+    if (g - S.atom_off >= S.n) return;
+    const double lam = st[s].lambda;
+    const double dt = S.dt;
+    const double half_dt = 0.5 * dt;
+    const double im = invm[g];
+    double a = lam * vx[g], b = lam * vy[g], c = lam * vz[g];
+    a += fx[g] * im * half_dt;
+    b += fy[g] * im * half_dt;
+    c += fz[g] * im * half_dt;
+    vx[g] = a;
+    vy[g] = b;
+    vz[g] = c;
+    x[g] += a * dt;
+    y[g] += b * dt;
+    z[g] += c * dt;
+}
+
+__global__ void k_kick_ke(const SysDev* __restrict__ sys, const SysState* __restrict__ st,
+                          const int* __restrict__ slot_sys, int64_t nslots,
+                          const double* __restrict__ invm, const double* __restrict__ m,
+                          const double* __restrict__ fx, const double* __restrict__ fy,
+                          const double* __restrict__ fz, double* __restrict__ vx,
+                          double* __restrict__ vy, double* __restrict__ vz,
+                          double* __restrict__ ke_part)
+{
+    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nslots) return;
+    int s = slot_sys[g];
+    if (s < 0 || !st[s].active) return;
+    const SysDev& S = sys[s];
+    double ke = 0.0;
+    if (g - S.atom_off < S.n) {
+        const double half_dt = 0.5 * S.dt;
+        const double im = invm[g];
+        double a = vx[g] + fx[g] * im * half_dt;
+        double b = vy[g] + fy[g] * im * half_dt;
+        double c = vz[g] + fz[g] * im * half_dt;
+        vx[g] = a;
+        vy[g] = b;
+        vz[g] = c;
+        ke = 0.5 * m[g] * (a * a + b * b + c * c);
+    }
+    ke = wave_sum(ke);
+    if (lane_id() == 0) ke_part[g >> 6] = ke;
+}
+
+// sum of part[off .. off+n) by one wave: lane l sums l, l+64, ... in order,
+// then a fixed butterfly -- deterministic
+__device__ __forceinline__ double wave_range_sum(const double* part, int off, int n)
+{
+    double a = 0.0;
+    for (int k = lane_id(); k < n; k += 64) a += part[off + k];
+    return wave_sum(a);
+}
+
+struct Parts {
+    const double *lj, *coul, *bond, *angle, *dih, *imp, *pme, *excl, *ke;
+};
+
+__device__ __forceinline__ double temperature_of(double ekin, int n)
+{
+    if (n < 2) return 0.0;
+    double dof = 3.0 * (double)n - 3.0;
+    return 2.0 * ekin / (dof * KB);
+}
+
+// one 64-thread workgroup per system, after the closing kick
+__global__ void __launch_bounds__(64) k_finalize(const SysDev* __restrict__ sys,
+                                                 SysState* __restrict__ st, Parts P,
+                                                 int with_kinetic)
+{
+    int s = blockIdx.x;
+    SysState& T = st[s];
+    if (!T.active) return;
+    const SysDev& S = sys[s];
+    double e[8];
+    e[0] = wave_range_sum(P.lj, S.aw_off, S.aw_n);
+    e[1] = wave_range_sum(P.coul, S.aw_off, S.aw_n);
+    e[2] = wave_range_sum(P.bond, S.bw_off, S.bw_n);
+    e[3] = wave_range_sum(P.angle, S.gw_off, S.gw_n);
+    e[4] = wave_range_sum(P.dih, S.dw_off, S.dw_n);
+    e[5] = wave_range_sum(P.imp, S.iw_off, S.iw_n);
+    e[6] = wave_range_sum(P.pme, S.pw_off, S.pw_n);
+    e[7] = wave_range_sum(P.excl, S.xw_off, S.xw_n);
+    double ke = with_kinetic ? wave_range_sum(P.ke, S.aw_off, S.aw_n) : 0.0;
+    if (lane_id() != 0) return;
+    if (S.pme) {
+        e[6] += S.self_energy;
+        if (S.physics != DMDX_FAITHFUL && S.qsum != 0.0) {
+            double V = S.L[0] * S.L[1] * S.L[2];
+            e[6] += -PI * KE * S.qsum * S.qsum / (2.0 * V * S.alpha * S.alpha);
+        }
+    }
+    // the pair terms, bonded terms and PME add, the exclusion term subtracts
+    double pe = 0.0;
+    for (int k = 0; k < 7; ++k) pe += e[k];
+    pe -= e[7];
+    for (int k = 0; k < 8; ++k) T.pe_terms[k] = e[k];
+    T.pe = pe;
+    if (!with_kinetic) {
+        if (!isfinite(pe)) T.flags |= FLAG_NONFINITE;
+        return;
+    }
+    T.ke = ke;
+    double lam = 1.0;
+    const int n = S.n;
+    if (S.thermostat == DMDX_THERMO_BERENDSEN) {
+        if (S.ttau > 0.0 && ke > 0.0) {
+            double T_inst = temperature_of(ke, n);
+            double l = sqrt(1.0 + S.dt / S.ttau * (S.T0 / T_inst - 1.0));
+            if (!(l <= 0.0)) lam = l;
+        }
+    } else if (S.thermostat == DMDX_THERMO_NOSE_HOOVER) {
+        double T_inst = temperature_of(ke, n);
+        T.zeta += 0.5 * S.dt * KB * (T_inst - S.T0) / S.Qnh;
+        lam = exp(-T.zeta * 0.5 * S.dt);
+    }
+    T.lambda = lam;
+    T.ke_out = ke * lam * lam;
+    T.steps_done += 1;
+    if (!isfinite(pe) || !isfinite(ke) || !isfinite(lam)) T.flags |= FLAG_NONFINITE;
+}
+
+// ---- barostat path: materialise lambda, kinetic energy and virial
+__global__ void k_baro_ke_virial(const SysDev* __restrict__ sys, const SysState* __restrict__ st,
+                                 const int* __restrict__ slot_sys, int64_t nslots,
+                                 const double* __restrict__ m, double* __restrict__ vx,
+                                 double* __restrict__ vy, double* __restrict__ vz,
+                                 const double* __restrict__ x, const double* __restrict__ y,
+                                 const double* __restrict__ z, const double* __restrict__ fx,
+                                 const double* __restrict__ fy, const double* __restrict__ fz,
+                                 double* __restrict__ ke_part, double* __restrict__ w_part)
+{
+    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nslots) return;
+    int s = slot_sys[g];
+    if (s < 0 || !st[s].active) return;
+    const SysDev& S = sys[s];
+    if (S.barostat == DMDX_BARO_NONE) return;
+    double ke = 0.0, w = 0.0;
+    if (g - S.atom_off < S.n) {
+        const double lam = st[s].lambda;
+        double a = vx[g] * lam, b = vy[g] * lam, c = vz[g] * lam;
+        vx[g] = a;
+        vy[g] = b;
+        vz[g] = c;
+        ke = 0.5 * m[g] * (a * a + b * b + c * c);
+        w = x[g] * fx[g] + y[g] * fy[g] + z[g] * fz[g];
+    }
+    ke = wave_sum(ke);
+    w = wave_sum(w);
+    if (lane_id() == 0) {
+        ke_part[g >> 6] = ke;
+        w_part[g >> 6] = w;
+    }
+}
+
+__global__ void __launch_bounds__(64) k_baro_finalize(SysDev* __restrict__ sys,
+                                                      SysState* __restrict__ st,
+                                                      const double* __restrict__ ke_part,
+                                                      const double* __restrict__ w_part)
+{
+    int s = blockIdx.x;
+    SysState& T = st[s];
+    SysDev& S = sys[s];
+    if (!T.active || S.barostat == DMDX_BARO_NONE) return;
+    double ke = wave_range_sum(ke_part, S.aw_off, S.aw_n);
+    double w = wave_range_sum(w_part, S.aw_off, S.aw_n);
+    if (lane_id() != 0) return;
+    T.lambda = 1.0;
+    T.ke_out = ke;
+    double volume = S.L[0] * S.L[1] * S.L[2];
+    double scale = 1.0;
+    T.xscale = 1.0;
+    if (volume <= 0.0) return;
+    double P_inst = (2.0 * ke + w) / (3.0 * volume);
+    T.virial = w;
+    T.pressure = P_inst;
+    if (S.barostat == DMDX_BARO_BERENDSEN) {
+        // src/barostat/berendsen_barostat.cpp:14-66
+        if (S.ptau <= 0.0) return;
+        double beta_internal = S.beta / 0.0602214086;
+        double mu = 1.0 - (S.dt / S.ptau) * beta_internal * (S.P0 - P_inst);
+        if (mu <= 0.0) mu = 0.0;
+        scale = cbrt(mu);
+    } else {
+        // src/barostat/andersen_barostat.cpp:20-45
+        double forcing = 3.0 * volume * (P_inst - S.P0) / S.Wab;
+        T.eps_dot = T.eps_dot + 0.5 * S.dt * forcing;
+        scale = exp(T.eps_dot * S.dt);
+        T.eps_dot = T.eps_dot + 0.5 * S.dt * forcing;
+    }
+    T.xscale = scale;
+    for (int d = 0; d < 3; ++d) {
+        S.L[d] *= scale;
+        S.invL[d] = 1.0 / S.L[d];
+    }
+    T.valid = 0;
+}
+
+__global__ void k_scale_positions(const SysDev* __restrict__ sys, const SysState* __restrict__ st,
+                                  const int* __restrict__ slot_sys, int64_t nslots,
+                                  double* __restrict__ x, double* __restrict__ y,
+                                  double* __restrict__ z)
+{
+    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nslots) return;
+    int s = slot_sys[g];
+    if (s < 0 || !st[s].active) return;
+    const SysDev& S = sys[s];
+    if (S.barostat == DMDX_BARO_NONE || g - S.atom_off >= S.n) return;
+    double sc = st[s].xscale;
+    x[g] *= sc;
+    y[g] *= sc;
+    z[g] *= sc;
+}
+
+// apply the pending thermostat scale to the stored velocities (state reads)
+__global__ void k_materialize(const SysDev* __restrict__ sys, const SysState* __restrict__ st,
+                              const int* __restrict__ slot_sys, int64_t nslots,
+                              double* __restrict__ vx, double* __restrict__ vy,
+                              double* __restrict__ vz)
+{
+    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nslots) return;
+    int s = slot_sys[g];
+    if (s < 0) return;
+    const SysDev& S = sys[s];
+    if (g - S.atom_off >= S.n) return;
+    double lam = st[s].lambda;
+    vx[g] = vx[g] * lam;
+    vy[g] = vy[g] * lam;
+    vz[g] = vz[g] * lam;
+}
+
+__global__ void k_lambda_reset(SysState* __restrict__ st, int nsys)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < nsys) st[s].lambda = 1.0;
+}
+
+}  // namespace dmdx
+
+namespace dmdx {
+
+hipError_t launch_pre_compute(const Dev& d, hipStream_t s, int phase, int force_rebuild,
+                              int reuse_enabled)
+{
+    k_pre_compute<<<blocks_for(d.nsys, 64), 64, 0, s>>>(d.sys, d.st, d.nsys, phase,
+                                                        force_rebuild, reuse_enabled);
+    return hipGetLastError();
+}
+
+hipError_t launch_pre_rebuild_only(const Dev& d, hipStream_t s)
+{
+    k_pre_rebuild_only<<<blocks_for(d.nsys, 64), 64, 0, s>>>(d.sys, d.st, d.nsys);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather(const Dev& d, hipStream_t s)
+{
+    if (!d.any_gather) return hipSuccess;
+    const int BS = 256;
+    if (d.precision == DMDX_FP64)
+        k_gather<double><<<blocks_for(d.nslots, BS), BS, 0, s>>>(
+            d.sys, d.st, d.slot_sys, d.nslots, d.c_start, d.c_mid, d.contrib, d.tf, d.x, d.y, d.z,
+            d.q, reinterpret_cast<const Cx2<double>*>(d.grid), d.fx, d.fy, d.fz);
+    else
+        k_gather<float><<<blocks_for(d.nslots, BS), BS, 0, s>>>(
+            d.sys, d.st, d.slot_sys, d.nslots, d.c_start, d.c_mid, d.contrib, d.tf, d.x, d.y, d.z,
+            d.q, reinterpret_cast<const Cx2<float>*>(d.grid), d.fx, d.fy, d.fz);
+    return hipGetLastError();
+}
+
+hipError_t launch_kick_drift(const Dev& d, hipStream_t s)
+{
+    const int BS = 256;
+    k_kick_drift<<<blocks_for(d.nslots, BS), BS, 0, s>>>(d.sys, d.st, d.slot_sys, d.nslots,
+                                                         d.invm, d.fx, d.fy, d.fz, d.vx, d.vy,
+                                                         d.vz, d.x, d.y, d.z);
+    return hipGetLastError();
+}
+
+hipError_t launch_kick_ke(const Dev& d, hipStream_t s)
+{
+    const int BS = 256;
+    k_kick_ke<<<blocks_for(d.nslots, BS), BS, 0, s>>>(d.sys, d.st, d.slot_sys, d.nslots, d.invm,
+                                                      d.m, d.fx, d.fy, d.fz, d.vx, d.vy, d.vz,
+                                                      d.p_ke);
+    return hipGetLastError();
+}
+
+hipError_t launch_finalize(const Dev& d, hipStream_t s, int with_kinetic)
+{
+    Parts P{d.p_lj, d.p_coul, d.p_bond, d.p_angle, d.p_dih, d.p_imp, d.p_pme, d.p_excl, d.p_ke};
+    k_finalize<<<d.nsys, 64, 0, s>>>(d.sys, d.st, P, with_kinetic);
+    return hipGetLastError();
+}
+
+hipError_t launch_barostat(const Dev& d, hipStream_t s)
+{
+    if (!d.any_baro) return hipSuccess;
+    const int BS = 256;
+    k_baro_ke_virial<<<blocks_for(d.nslots, BS), BS, 0, s>>>(d.sys, d.st, d.slot_sys, d.nslots,
+                                                             d.m, d.vx, d.vy, d.vz, d.x, d.y,
+                                                             d.z, d.fx, d.fy, d.fz, d.p_ke,
+                                                             d.p_w);
+    k_baro_finalize<<<d.nsys, 64, 0, s>>>(d.sys, d.st, d.p_ke, d.p_w);
+    k_scale_positions<<<blocks_for(d.nslots, BS), BS, 0, s>>>(d.sys, d.st, d.slot_sys, d.nslots,
+                                                              d.x, d.y, d.z);
+    return hipGetLastError();
+}
+
+hipError_t launch_materialize(const Dev& d, hipStream_t s)
+{
+    const int BS = 256;
+    k_materialize<<<blocks_for(d.nslots, BS), BS, 0, s>>>(d.sys, d.st, d.slot_sys, d.nslots,
+                                                          d.vx, d.vy, d.vz);
+    k_lambda_reset<<<blocks_for(d.nsys, 64), 64, 0, s>>>(d.st, d.nsys);
+    return hipGetLastError();
+}
+
+}  // namespace dmdx

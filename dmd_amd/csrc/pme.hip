@@ -1,0 +1,350 @@
+// pme.hip -- smooth particle-mesh Ewald, reciprocal part, on gfx950.
+//
+// Replaces CoulombPME::compute (src/force/coulomb_pme.cpp:287-318) and its
+// FFTW calls (:154, :198).  Per compute:
+//   k_pme_zero      clear the complex grid
+//   k_pme_spread    order-4 B-spline charge spreading (:79-136), 64 atomic
+//                   adds per charged atom
+//   k_fft_pass x,y  forward 1D FFT passes along x then y
+//   k_fft_pass z+   forward z pass, influence function G(m) with the energy
+//                   0.5 G |Q(m)|^2 (:143-201), inverse z pass -- the whole
+//                   convolution of each z line happens in LDS
+//   k_fft_pass y,x  inverse passes
+//   (gather)        in k_gather (integrate.hip), per atom
+// The grid is complex (real input, imaginary part 0) with x fastest.  The FFT
+// is a mixed-radix Stockham autosort transform (Govindaraju et al. 2008,
+// "High performance discrete Fourier transforms on graphics processors"):
+// each workgroup loads a tile of B lines into LDS, runs all radix stages
+// there (ping-pong buffers) and writes back; y and z tiles are B adjacent x
+// columns so global loads/stores are coalesced.  Templated on the real type:
+// double for the FP64 path, float for the mixed path.
+#include "dmdx_internal.h"
+
+namespace dmdx {
+
+template <class R>
+struct Cx {
+    R re, im;
+};
+
+template <class R>
+__device__ __forceinline__ Cx<R> cmul(Cx<R> a, Cx<R> b)
+{
+    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+
+// centred cubic B-spline and derivative, src/force/coulomb_pme.cpp:50-69
+template <class R>
+__device__ __forceinline__ R theta4(R u)
+{
+    R a = fabs(u);
+    if (a >= R(2)) return R(0);
+    if (a >= R(1)) {
+        R t = R(2) - a;
+        return t * t * t / R(6);
+    }
+    return (R(4) - R(6) * a * a + R(3) * a * a * a) / R(6);
+}
+
+template <class R>
+__device__ __forceinline__ R dtheta4(R u)
+{
+    R a = fabs(u);
+    if (a >= R(2)) return R(0);
+    R s = (u >= R(0)) ? R(1) : R(-1);
+    if (a >= R(1)) {
+        R t = R(2) - a;
+        return -s * t * t / R(2);
+    }
+    return s * (R(9) * a * a - R(12) * a) / R(6);
+}
+
+template <class R>
+__global__ void k_pme_zero(const SysDev* __restrict__ sys, const SysState* __restrict__ st,
+                           const int* __restrict__ gsys, int64_t ntot, Cx<R>* __restrict__ grid)
+{
+    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ntot) return;
+    int s = gsys[e];
+    if (s < 0 || st[s].reuse || !st[s].active) return;
+    grid[e] = {R(0), R(0)};
+}
+
+__device__ __forceinline__ void atomic_add_real(double* p, double v) { unsafeAtomicAdd(p, v); }
+__device__ __forceinline__ void atomic_add_real(float* p, float v) { unsafeAtomicAdd(p, v); }
+
+// per atom: fractional coordinate, first grid index and offset, in FP64 (the
+// reference's expressions), then the weights in R
+template <class R>
+__global__ void k_pme_spread(const SysDev* __restrict__ sys, const SysState* __restrict__ st,
+                             const int* __restrict__ slot_sys, int64_t nslots,
+                             const double* __restrict__ x, const double* __restrict__ y,
+                             const double* __restrict__ z, const double* __restrict__ q,
+                             Cx<R>* __restrict__ grid)
+{
+    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nslots) return;
+    int s = slot_sys[g];
+    if (s < 0 || st[s].reuse || !st[s].active) return;
+    const SysDev& S = sys[s];
+    if (!S.pme || g - S.atom_off >= S.n) return;
+    double qi = q[g];
+    if (qi == 0.0) return;
+    const int nx = S.K[0], ny = S.K[1], nz = S.K[2];
+    double frac_x = x[g] / S.L[0], frac_y = y[g] / S.L[1], frac_z = z[g] / S.L[2];
+    frac_x -= floor(frac_x);
+    frac_y -= floor(frac_y);
+    frac_z -= floor(frac_z);
+    double ux = frac_x * nx, uy = frac_y * ny, uz = frac_z * nz;
+    int m0x = (int)floor(ux) - 1, m0y = (int)floor(uy) - 1, m0z = (int)floor(uz) - 1;
+    R wx[4], wy[4], wz[4];
+    for (int d = 0; d < 4; ++d) {
+        wx[d] = theta4<R>((R)(ux - (m0x + d)));
+        wy[d] = theta4<R>((R)(uy - (m0y + d)));
+        wz[d] = theta4<R>((R)(uz - (m0z + d)));
+    }
+    Cx<R>* G = grid + S.grid_off;
+    const R qr = (R)qi;
+    for (int iz = 0; iz < 4; ++iz) {
+        int gz = (m0z + iz) % nz;
+        if (gz < 0) gz += nz;
+        R w_z = wz[iz];
+        for (int iy = 0; iy < 4; ++iy) {
+            int gy = (m0y + iy) % ny;
+            if (gy < 0) gy += ny;
+            R w_yz = wy[iy] * w_z;
+            int64_t row = ((int64_t)gz * ny + gy) * nx;
+            for (int ix = 0; ix < 4; ++ix) {
+                int gx = (m0x + ix) % nx;
+                if (gx < 0) gx += nx;
+                atomic_add_real(&G[row + gx].re, qr * wx[ix] * w_yz);
+            }
+        }
+    }
+}
+
+// R-point DFT of v[0..R) in place, with w^m = tw[(m * (K / R)) % K]
+template <class R>
+__device__ __forceinline__ void small_dft(Cx<R>* v, int rdx, const Cx<R>* tw, int K, int sign)
+{
+    if (rdx == 2) {
+        Cx<R> a = v[0], b = v[1];
+        v[0] = {a.re + b.re, a.im + b.im};
+        v[1] = {a.re - b.re, a.im - b.im};
+        return;
+    }
+    if (rdx == 4) {
+        Cx<R> a0 = v[0], a1 = v[1], a2 = v[2], a3 = v[3];
+        Cx<R> s02 = {a0.re + a2.re, a0.im + a2.im}, d02 = {a0.re - a2.re, a0.im - a2.im};
+        Cx<R> s13 = {a1.re + a3.re, a1.im + a3.im}, d13 = {a1.re - a3.re, a1.im - a3.im};
+        // multiply d13 by w4 = sign * i
+        Cx<R> jd13 = sign < 0 ? Cx<R>{d13.im, -d13.re} : Cx<R>{-d13.im, d13.re};
+        v[0] = {s02.re + s13.re, s02.im + s13.im};
+        v[1] = {d02.re + jd13.re, d02.im + jd13.im};
+        v[2] = {s02.re - s13.re, s02.im - s13.im};
+        v[3] = {d02.re - jd13.re, d02.im - jd13.im};
+        return;
+    }
+    Cx<R> out[16];
+    int step = K / rdx;
+    for (int k = 0; k < rdx; ++k) {
+        Cx<R> acc = v[0];
+        for (int r = 1; r < rdx; ++r) {
+            Cx<R> w = tw[((r * k) % rdx) * step];
+            if (sign > 0) w.im = -w.im;
+            Cx<R> p = cmul(v[r], w);
+            acc.re += p.re;
+            acc.im += p.im;
+        }
+        out[k] = acc;
+    }
+    for (int k = 0; k < rdx; ++k) v[k] = out[k];
+}
+
+// all Stockham stages of nl lines of length K held in LDS (a -> b ping-pong);
+// returns the buffer that holds the result
+template <class R>
+__device__ Cx<R>* stockham_lds(Cx<R>* a, Cx<R>* b, int nl, const FftPlan& P,
+                               const Cx<R>* __restrict__ tw, int sign)
+{
+    const int K = P.K;
+    int Ns = 1;
+    for (int st = 0; st < P.nrad; ++st) {
+        const int Rr = P.radix[st];
+        const int nb = K / Rr;
+        const int sub = K / (Ns * Rr);
+        for (int w = threadIdx.x; w < nl * nb; w += blockDim.x) {
+            int line = w / nb, j = w - line * nb;
+            const Cx<R>* src = a + line * K;
+            Cx<R>* dst = b + line * K;
+            int k = j % Ns;
+            Cx<R> v[16];
+            for (int r = 0; r < Rr; ++r) {
+                Cx<R> tv = tw[(k * r * sub) % K];
+                if (sign > 0) tv.im = -tv.im;
+                v[r] = r ? cmul(src[j + r * nb], tv) : src[j];
+            }
+            small_dft<R>(v, Rr, tw, K, sign);
+            int idxD = (j / Ns) * Ns * Rr + k;
+            for (int r = 0; r < Rr; ++r) dst[idxD + r * Ns] = v[r];
+        }
+        __syncthreads();
+        Cx<R>* t = a;
+        a = b;
+        b = t;
+        Ns *= Rr;
+    }
+    return a;
+}
+
+// dim: 0 = x, 1 = y, 2 = z.  conv = 1 on the forward z pass: after the
+// forward transform multiply by G(m), accumulate the energy and run the
+// inverse z transform in place.
+template <class R>
+__global__ void k_fft_pass(const FftTile* __restrict__ tiles, const SysDev* __restrict__ sys,
+                           const SysState* __restrict__ st, const FftPlan* __restrict__ plans,
+                           Cx<R>* __restrict__ grid, const Cx<R>* __restrict__ twiddles,
+                           const double* __restrict__ bmod, int dim, int sign, int conv,
+                           double* __restrict__ epart)
+{
+    extern __shared__ unsigned char smem_raw[];
+    const FftTile T = tiles[blockIdx.x];
+    const int s = T.sys;
+    if (st[s].reuse || !st[s].active) return;
+    const SysDev& S = sys[s];
+    const FftPlan& P = plans[3 * s + dim];
+    const int K = P.K;
+    const int nl = T.nlines;
+    const int Kx = S.K[0], Ky = S.K[1];
+    Cx<R>* bufA = reinterpret_cast<Cx<R>*>(smem_raw);
+    Cx<R>* bufB = bufA + nl * K;
+    Cx<R>* G = grid + S.grid_off;
+    const Cx<R>* tw = twiddles + S.tw_off[dim];
+    // element (line b, index k) -> grid offset
+    auto addr = [&](int b, int k) -> int64_t {
+        int l = T.line0 + b;
+        if (dim == 0) return (int64_t)l * Kx + k;
+        if (dim == 1) {
+            int xx = l % Kx, zz = l / Kx;
+            return ((int64_t)zz * Ky + k) * Kx + xx;
+        }
+        int xx = l % Kx, yy = l / Kx;
+        return ((int64_t)k * Ky + yy) * Kx + xx;
+    };
+    const int tot = nl * K;
+    if (dim == 0) {
+        for (int e = threadIdx.x; e < tot; e += blockDim.x) bufA[e] = G[(int64_t)T.line0 * Kx + e];
+    } else {
+        for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+            int b = e % nl, k = e / nl;
+            bufA[b * K + k] = G[addr(b, k)];
+        }
+    }
+    __syncthreads();
+    Cx<R>* res = stockham_lds<R>(bufA, bufB, nl, P, tw, sign);
+    if (conv) {
+        // influence function on the full spectrum: lines are z columns at
+        // fixed (x, y); m_d = k <= K_d/2 ? k : k - K_d.  Reference
+        // expressions of src/force/coulomb_pme.cpp:165-196.
+        __shared__ double red[256];
+        const double Lx = S.L[0], Ly = S.L[1], Lz = S.L[2];
+        const double V = Lx * Ly * Lz;
+        const double alpha = S.alpha;
+        const double* bmx = bmod + S.bm_off[0];
+        const double* bmy = bmod + S.bm_off[1];
+        const double* bmz = bmod + S.bm_off[2];
+        const int Kz = S.K[2];
+        double esum = 0.0;
+        for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+            int b = e / K, kz = e - b * K;
+            int l = T.line0 + b;
+            int kx = l % Kx, ky = l / Kx;
+            int mx = (kx <= Kx / 2) ? kx : kx - Kx;
+            int my = (ky <= Ky / 2) ? ky : ky - Ky;
+            int mz = (kz <= Kz / 2) ? kz : kz - Kz;
+            Cx<R> v = res[e];
+            if (mx == 0 && my == 0 && mz == 0) {
+                res[e] = {R(0), R(0)};
+                continue;
+            }
+            double msq = ((double)mx * mx) / (Lx * Lx) + ((double)my * my) / (Ly * Ly) +
+                         ((double)mz * mz) / (Lz * Lz);
+            double k2 = 4.0 * PI * PI * msq;
+            double B = bmx[kx] * bmy[ky] * bmz[kz];
+            double green = 4.0 * PI * KE / V * exp(-k2 / (4.0 * alpha * alpha)) * B / k2;
+            double re = (double)v.re, im = (double)v.im;
+            esum += 0.5 * green * (re * re + im * im);
+            R gr = (R)green;
+            res[e] = {v.re * gr, v.im * gr};
+        }
+        red[threadIdx.x] = esum;
+        __syncthreads();
+        for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+            if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) epart[S.pw_off + T.tile_local] = red[0];
+        Cx<R>* other = (res == bufA) ? bufB : bufA;
+        res = stockham_lds<R>(res, other, nl, P, tw, -sign);
+    }
+    if (dim == 0) {
+        for (int e = threadIdx.x; e < tot; e += blockDim.x) G[(int64_t)T.line0 * Kx + e] = res[e];
+    } else {
+        for (int e = threadIdx.x; e < tot; e += blockDim.x) {
+            int b = e % nl, k = e / nl;
+            G[addr(b, k)] = res[b * K + k];
+        }
+    }
+}
+
+template __global__ void k_pme_zero<double>(const SysDev*, const SysState*, const int*, int64_t,
+                                            Cx<double>*);
+template __global__ void k_pme_zero<float>(const SysDev*, const SysState*, const int*, int64_t,
+                                           Cx<float>*);
+template __global__ void k_pme_spread<double>(const SysDev*, const SysState*, const int*, int64_t,
+                                              const double*, const double*, const double*,
+                                              const double*, Cx<double>*);
+template __global__ void k_pme_spread<float>(const SysDev*, const SysState*, const int*, int64_t,
+                                             const double*, const double*, const double*,
+                                             const double*, Cx<float>*);
+template __global__ void k_fft_pass<double>(const FftTile*, const SysDev*, const SysState*,
+                                            const FftPlan*, Cx<double>*, const Cx<double>*,
+                                            const double*, int, int, int, double*);
+template __global__ void k_fft_pass<float>(const FftTile*, const SysDev*, const SysState*,
+                                           const FftPlan*, Cx<float>*, const Cx<float>*,
+                                           const double*, int, int, int, double*);
+
+}  // namespace dmdx
+
+namespace dmdx {
+
+template <class R>
+static hipError_t launch_pme_t(const Dev& d, hipStream_t s)
+{
+    const int BS = 256;
+    Cx<R>* grid = reinterpret_cast<Cx<R>*>(d.grid);
+    const Cx<R>* tw = reinterpret_cast<const Cx<R>*>(d.tw);
+    k_pme_zero<R><<<blocks_for(d.grid_total, BS), BS, 0, s>>>(d.sys, d.st, d.grid_sys,
+                                                              d.grid_total, grid);
+    k_pme_spread<R><<<blocks_for(d.nslots, BS), BS, 0, s>>>(d.sys, d.st, d.slot_sys, d.nslots,
+                                                            d.x, d.y, d.z, d.q, grid);
+    // forward x, forward y, forward z + convolution + inverse z, inverse y, inverse x
+    const int dims[5] = {0, 1, 2, 1, 0};
+    const int signs[5] = {-1, -1, -1, +1, +1};
+    for (int p = 0; p < 5; ++p) {
+        int dim = dims[p];
+        if (!d.ntiles[dim]) continue;
+        k_fft_pass<R><<<d.ntiles[dim], 256, d.lds_bytes[dim], s>>>(
+            d.tiles[dim], d.sys, d.st, d.plans, grid, tw, d.bmod, dim, signs[p], p == 2 ? 1 : 0,
+            d.p_pme);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_pme(const Dev& d, hipStream_t s)
+{
+    if (!d.any_pme) return hipSuccess;
+    return d.precision == DMDX_FP64 ? launch_pme_t<double>(d, s) : launch_pme_t<float>(d, s);
+}
+
+}  // namespace dmdx

@@ -1,0 +1,146 @@
+"""SimulationEngine mirror over the C-ABI (include/dmdx.h, dmdx_engine_*).
+
+``Engine`` replaces DMD's SimulationEngine (src/sim/simulation_engine.h:16-76)
+for one system or for a batch of independent systems that step together in
+the same device launches.  State lives in HBM; the getters copy to host.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _abi
+from .system import SystemSpec
+
+
+class Scalars(C.Structure):
+    _fields_ = [
+        ("potential_energy", C.c_double), ("energy_terms", C.c_double * 8),
+        ("kinetic_energy", C.c_double), ("zeta", C.c_double), ("piston_velocity", C.c_double),
+        ("virial", C.c_double), ("pressure", C.c_double), ("box", C.c_double * 3),
+        ("step_since_rebuild", C.c_int32), ("flags", C.c_int32), ("steps_done", C.c_int64),
+    ]
+
+
+TERM_NAMES = ("lj", "coulomb", "bond", "angle", "dihedral", "improper", "pme", "exclusion")
+
+_dp = C.POINTER(C.c_double)
+
+
+def _setup(lib):
+    if getattr(lib, "_engine_ready", False):
+        return
+    lib.dmdx_engine_create.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_void_p)]
+    for name in ("dmdx_engine_destroy", "dmdx_engine_compute", "dmdx_engine_sync"):
+        getattr(lib, name).argtypes = [C.c_void_p]
+    lib.dmdx_engine_step.argtypes = [C.c_void_p, C.c_int64]
+    lib.dmdx_engine_set_options.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]
+    lib.dmdx_engine_get_arrays.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _dp, _dp, _dp]
+    lib.dmdx_engine_set_arrays.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _dp, _dp, _dp]
+    lib.dmdx_engine_get_scalars.argtypes = [C.c_void_p, C.c_int32, C.POINTER(Scalars)]
+    lib.dmdx_engine_set_step_since_rebuild.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
+    lib.dmdx_engine_set_active.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
+    lib.dmdx_engine_get_pairs.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_int32),
+                                          C.POINTER(C.c_int32), C.c_int64,
+                                          C.POINTER(C.c_int64)]
+    lib.dmdx_engine_n_systems.argtypes = [C.c_void_p, C.POINTER(C.c_int32)]
+    lib._engine_ready = True
+
+
+class Engine:
+    """Device-resident SimulationEngine for one or more systems."""
+
+    def __init__(self, specs, reuse_forces: bool = True, use_graph: bool = False,
+                 graph_steps: int = 10):
+        if isinstance(specs, SystemSpec):
+            specs = [specs]
+        self.specs = list(specs)
+        self.lib = _abi.load()
+        _setup(self.lib)
+        descs = (_abi.SystemDesc * len(self.specs))()
+        for k, s in enumerate(self.specs):
+            descs[k] = s.to_desc()
+        h = C.c_void_p()
+        _abi.check(self.lib.dmdx_engine_create(C.cast(descs, C.c_void_p), len(self.specs),
+                                               C.byref(h)))
+        self.h = h
+        self.set_options(reuse_forces, use_graph, graph_steps)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.dmdx_engine_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_options(self, reuse_forces=True, use_graph=False, graph_steps=10):
+        _abi.check(self.lib.dmdx_engine_set_options(self.h, int(reuse_forces), int(use_graph),
+                                                    int(graph_steps)))
+
+    def step(self, nsteps: int = 1):
+        _abi.check(self.lib.dmdx_engine_step(self.h, int(nsteps)))
+
+    def compute(self):
+        _abi.check(self.lib.dmdx_engine_compute(self.h))
+
+    def sync(self):
+        _abi.check(self.lib.dmdx_engine_sync(self.h))
+
+    def _get(self, sys, what):
+        n = self.specs[sys].n
+        a = [np.empty(n) for _ in range(3)]
+        _abi.check(self.lib.dmdx_engine_get_arrays(self.h, sys, what,
+                                                   *[v.ctypes.data_as(_dp) for v in a]))
+        return np.stack(a, 1)
+
+    def _set(self, sys, what, arr):
+        arr = np.asarray(arr, dtype=np.float64)
+        a = [np.ascontiguousarray(arr[:, k]) for k in range(3)]
+        _abi.check(self.lib.dmdx_engine_set_arrays(self.h, sys, what,
+                                                   *[v.ctypes.data_as(_dp) for v in a]))
+
+    def positions(self, sys: int = 0):
+        return self._get(sys, 0)
+
+    def velocities(self, sys: int = 0):
+        return self._get(sys, 1)
+
+    def forces(self, sys: int = 0):
+        return self._get(sys, 2)
+
+    def set_positions(self, pos, sys: int = 0):
+        self._set(sys, 0, pos)
+
+    def set_velocities(self, vel, sys: int = 0):
+        self._set(sys, 1, vel)
+
+    def scalars(self, sys: int = 0) -> dict:
+        s = Scalars()
+        _abi.check(self.lib.dmdx_engine_get_scalars(self.h, sys, C.byref(s)))
+        return {
+            "pe": s.potential_energy, "terms": dict(zip(TERM_NAMES, list(s.energy_terms))),
+            "ke": s.kinetic_energy, "zeta": s.zeta, "eps_dot": s.piston_velocity,
+            "virial": s.virial, "pressure": s.pressure, "box": tuple(s.box),
+            "counter": s.step_since_rebuild, "flags": s.flags, "steps_done": s.steps_done,
+        }
+
+    def set_step_since_rebuild(self, counter: int, sys: int = 0):
+        _abi.check(self.lib.dmdx_engine_set_step_since_rebuild(self.h, sys, int(counter)))
+
+    def set_active(self, sys: int, active: bool):
+        _abi.check(self.lib.dmdx_engine_set_active(self.h, sys, int(active)))
+
+    def pairs(self, sys: int = 0):
+        cnt = C.c_int64(0)
+        _abi.check(self.lib.dmdx_engine_get_pairs(self.h, sys, None, None, 0, C.byref(cnt)))
+        pi = np.empty(cnt.value, np.int32)
+        pj = np.empty(cnt.value, np.int32)
+        _abi.check(self.lib.dmdx_engine_get_pairs(
+            self.h, sys, pi.ctypes.data_as(C.POINTER(C.c_int32)),
+            pj.ctypes.data_as(C.POINTER(C.c_int32)), cnt.value, C.byref(cnt)))
+        return pi, pj

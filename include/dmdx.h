@@ -127,6 +127,82 @@ typedef struct dmdx_system_desc {
     int32_t precision;             /* DMDX_FP64 | DMDX_MIXED                  */
 } dmdx_system_desc;
 
+/* Scalars of one system after the last step / compute. */
+typedef struct dmdx_scalars {
+    double potential_energy;       /* SystemData::potential_energy            */
+    double energy_terms[8];        /* lj, coulomb real space, bond, angle,
+                                      dihedral, improper, pme (incl. self),
+                                      exclusion (subtracted)                  */
+    double kinetic_energy;         /* of the velocities at the end of step()  */
+    double zeta;                   /* NoseHooverThermostat::zeta()            */
+    double piston_velocity;        /* AndersenBarostat::piston_velocity()     */
+    double virial;                 /* virial() of the last barostat call      */
+    double pressure;               /* pressure() of the last barostat call    */
+    double box[3];                 /* Cell::matrix[0], [4], [8]               */
+    int32_t step_since_rebuild;    /* LennardJones::step_since_rebuild()      */
+    int32_t flags;                 /* 1: neighbour-list overflow, 2: non-finite */
+    int64_t steps_done;
+} dmdx_scalars;
+
+/* ---- errors and device ------------------------------------------------ */
+
+/* Message of the last error raised on the calling thread ("" if none). */
+const char *dmdx_last_error(void);
+
+/* Number of visible GPUs and free/total HBM of the current one (any pointer
+ * may be NULL). */
+int dmdx_device_info(int32_t *device_count, int64_t *free_bytes, int64_t *total_bytes);
+
+/* ---- engine: SimulationEngine (src/sim/simulation_engine.h:16-76) ------ */
+
+typedef struct dmdx_engine dmdx_engine;
+
+/* build_simulation (src/sim/simulation_engine.cpp:248-386) for n_systems
+ * independent systems that step together in the same launches.  All systems
+ * must share one precision.  State is copied to HBM and stays there. */
+int dmdx_engine_create(const dmdx_system_desc *descs, int32_t n_systems, dmdx_engine **out);
+int dmdx_engine_destroy(dmdx_engine *e);
+
+/* reuse_forces: skip a first compute whose result equals the previous
+ * compute's (default 1); use_graph: replay hipGraph-captured blocks of
+ * graph_steps steps (default 0). */
+int dmdx_engine_set_options(dmdx_engine *e, int32_t reuse_forces, int32_t use_graph,
+                            int32_t graph_steps);
+
+/* SimulationEngine::step() repeated nsteps times
+ * (src/sim/simulation_engine.cpp:53-86).  Asynchronous: returns once the
+ * work is queued; dmdx_engine_sync or any get call waits for it. */
+int dmdx_engine_step(dmdx_engine *e, int64_t nsteps);
+
+/* One ForceEngine::compute on the current positions with forces and energy
+ * zeroed first (src/force/force_engine.cpp:10-25 as called by step()). */
+int dmdx_engine_compute(dmdx_engine *e);
+
+int dmdx_engine_sync(dmdx_engine *e);
+int dmdx_engine_n_systems(dmdx_engine *e, int32_t *n);
+
+/* what: 0 positions, 1 velocities, 2 forces (SystemData::pos_* / vel_* /
+ * forces_*).  Host arrays of n_atoms doubles; NULL skips a component. */
+int dmdx_engine_get_arrays(dmdx_engine *e, int32_t sys, int32_t what, double *x, double *y,
+                           double *z);
+int dmdx_engine_set_arrays(dmdx_engine *e, int32_t sys, int32_t what, const double *x,
+                           const double *y, const double *z);
+int dmdx_engine_get_scalars(dmdx_engine *e, int32_t sys, dmdx_scalars *out);
+
+/* LennardJones::set_step_since_rebuild (src/force/lennard_jones.h:37), as
+ * restored by SimulationEngine::load_checkpoint (simulation_engine.cpp:127-137). */
+int dmdx_engine_set_step_since_rebuild(dmdx_engine *e, int32_t sys, int32_t counter);
+
+/* Exclude (0) or include (1) a system in subsequent steps. */
+int dmdx_engine_set_active(dmdx_engine *e, int32_t sys, int32_t active);
+
+/* Current pair list of a system as the sorted set {(i, j) : i < j} (local
+ * atom indices), i.e. the pairs_i_/pairs_j_ of LennardJones
+ * (src/force/lennard_jones.h:43).  *count receives the number of pairs; at
+ * most cap are written. */
+int dmdx_engine_get_pairs(dmdx_engine *e, int32_t sys, int32_t *pi, int32_t *pj, int64_t cap,
+                          int64_t *count);
+
 #ifdef __cplusplus
 }
 #endif

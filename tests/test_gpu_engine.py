@@ -1,0 +1,211 @@
+"""HIP step engine (dmd_amd/libdmdx.so) against the FP64 oracle.
+
+Bars: the FP64 plain path reproduces the oracle trajectory within 1e-10
+relative forces over 100 steps (an LJ-only system is bit-identical); the
+mixed path is within 1e-5 force RMS relative error of the CORRECTED FP64
+oracle on identical positions."""
+import numpy as np
+import pytest
+
+import oracle as O
+from dmd_amd import _abi
+from dmd_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+FA, CO = _abi.FAITHFUL, _abi.CORRECTED
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def engine(spec, **kw):
+    from dmd_amd.engine import Engine
+    return Engine(spec, **kw)
+
+
+def rel_max(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def rms_rel(a, b):
+    return float(np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum()))
+
+
+def jittered_argon(n_cells=10, skin=0.0, seed=42, amp=0.01):
+    s = W.argon_fcc(n_cells=n_cells, skin=skin, seed=seed)
+    rng = np.random.default_rng(seed)
+    return s.with_(pos=s.pos + rng.uniform(-amp, amp, s.pos.shape))
+
+
+def test_c0_pair_set_and_forces_bitexact():
+    say("case: argon 4000 FP64 faithful, pair set + forces")
+    spec = jittered_argon()
+    e = engine(spec)
+    e.compute()
+    gi, gj = e.pairs()
+    oi, oj = O.pairs(spec.pos, spec.box, spec.lj_cutoff)
+    assert np.array_equal(gi, oi) and np.array_equal(gj, oj)
+    sim = O.Sim(spec)
+    sim.compute()
+    st = sim.state()
+    F = e.forces()
+    assert np.array_equal(F, st["force"]), rel_max(F, st["force"])
+    assert abs(e.scalars()["pe"] - st["pe"]) <= 1e-12 * abs(st["pe"])
+
+
+@pytest.mark.parametrize("skin", [0.0, 0.1])
+def test_c0_trajectory_100_steps(skin):
+    say(f"case: argon 4000 FP64 faithful, 100 steps, skin {skin}")
+    spec = jittered_argon(skin=skin)
+    e = engine(spec)
+    sim = O.Sim(spec)
+    worst = 0.0
+    for block in range(4):
+        e.step(25)
+        sim.step(25)
+        st = sim.state()
+        worst = max(worst, rel_max(e.forces(), st["force"]))
+        assert rel_max(e.positions(), st["pos"]) < 1e-12
+    assert worst <= 1e-10, worst
+    # LJ-only NVE: identical additions in identical order -> identical bits
+    assert np.array_equal(e.positions(), st["pos"])
+    assert np.array_equal(e.velocities(), st["vel"])
+    sc = e.scalars()
+    assert sc["counter"] == st["counter"]
+    assert abs(sc["pe"] - st["pe"]) <= 1e-12 * abs(st["pe"])
+
+
+def test_reuse_of_forces_changes_nothing():
+    say("case: argon 864 FP64, force reuse on/off")
+    spec = jittered_argon(n_cells=6, skin=0.05)
+    a = engine(spec, reuse_forces=True)
+    b = engine(spec, reuse_forces=False)
+    a.step(37)
+    b.step(37)
+    assert np.array_equal(a.positions(), b.positions())
+    assert np.array_equal(a.velocities(), b.velocities())
+    assert a.scalars()["pe"] == b.scalars()["pe"]
+
+
+@pytest.mark.parametrize("thermo", [_abi.THERMO_BERENDSEN, _abi.THERMO_NOSE_HOOVER])
+def test_thermostats(thermo):
+    say(f"case: argon 864 FP64 thermostat {thermo}")
+    spec = jittered_argon(n_cells=6).with_(thermostat=thermo, thermo_temperature=120.0,
+                                           thermo_tau=0.05)
+    e = engine(spec)
+    sim = O.Sim(spec)
+    e.step(50)
+    sim.step(50)
+    st = sim.state()
+    assert rel_max(e.forces(), st["force"]) <= 1e-10
+    assert rel_max(e.velocities(), st["vel"]) <= 1e-10
+    sc = e.scalars()
+    assert abs(sc["zeta"] - st["zeta"]) <= 1e-10 * max(abs(st["zeta"]), 1e-30)
+    assert abs(sc["ke"] - st["ke"]) <= 1e-10 * st["ke"]
+
+
+@pytest.mark.parametrize("baro", [_abi.BARO_BERENDSEN, _abi.BARO_ANDERSEN])
+def test_barostats(baro):
+    say(f"case: argon 864 FP64 barostat {baro}")
+    spec = jittered_argon(n_cells=6, skin=0.1).with_(
+        barostat=baro, baro_pressure=100.0, baro_tau=0.5, lj_cutoff=0.8)
+    e = engine(spec)
+    sim = O.Sim(spec)
+    e.step(30)
+    sim.step(30)
+    st = sim.state()
+    assert rel_max(e.positions(), st["pos"]) <= 1e-10
+    assert abs(e.scalars()["box"][0] - st["box_x"]) <= 1e-12 * st["box_x"]
+
+
+@pytest.mark.parametrize("mode", [FA, CO])
+def test_bonded_chain(mode):
+    say(f"case: chain melt 600 FP64 physics {mode}")
+    spec = W.chain_melt(600, seed=5).with_(physics=mode, list_skin=0.05 if mode else 0.0)
+    e = engine(spec)
+    sim = O.Sim(spec)
+    e.compute()
+    sim.compute()
+    st = sim.state()
+    assert rel_max(e.forces(), st["force"]) <= 1e-12
+    steps = 20 if mode == FA else 100
+    e.step(steps)
+    sim.step(steps)
+    st = sim.state()
+    assert rel_max(e.forces(), st["force"]) <= 1e-10
+    assert rel_max(e.positions(), st["pos"]) <= 1e-10
+
+
+def test_water_pme_fp64_corrected():
+    say("case: water 648 FP64 corrected PME")
+    spec = W.water_box(n_mol=216, seed=3)
+    e = engine(spec)
+    sim = O.Sim(spec)
+    e.compute()
+    F_ref, pe_ref = sim.forces_exact()
+    assert rel_max(e.forces(), F_ref) <= 1e-10
+    assert abs(e.scalars()["pe"] - pe_ref) <= 1e-10 * abs(pe_ref)
+    e.step(20)
+    sim.step(20)
+    st = sim.state()
+    assert rel_max(e.forces(), st["force"]) <= 1e-9
+
+
+def test_water_pme_fp64_faithful_components():
+    say("case: water 648 FP64 faithful (PME layout-consistent)")
+    spec = W.water_box(n_mol=216, seed=4).with_(physics=FA, list_skin=0.0)
+    e = engine(spec)
+    sim = O.Sim(spec)
+    e.compute()
+    sim.compute()
+    st = sim.state()
+    assert rel_max(e.forces(), st["force"]) <= 1e-10
+    assert abs(e.scalars()["pe"] - st["pe"]) <= 1e-10 * abs(st["pe"])
+
+
+@pytest.mark.parametrize("kind", ["argon", "water", "chain"])
+def test_mixed_force_rms(kind):
+    say(f"case: mixed precision forces, {kind}")
+    if kind == "argon":
+        spec = jittered_argon(skin=0.1).with_(physics=CO)
+    elif kind == "water":
+        spec = W.water_box(n_side=10, seed=2)
+    else:
+        spec = W.chain_melt(2000, seed=9)
+    e = engine(spec.with_(precision=_abi.MIXED))
+    e.compute()
+    F_ref, pe_ref = O.Sim(spec).forces_exact()
+    err = rms_rel(e.forces(), F_ref)
+    say(f"  force rms rel err {err:.3e}")
+    assert err <= 1e-5
+    assert abs(e.scalars()["pe"] - pe_ref) <= 1e-5 * abs(pe_ref)
+
+
+def test_batch_alone_equals_in_mix():
+    say("case: batch of 3 vs alone, FP64")
+    specs = [jittered_argon(n_cells=5, seed=1, skin=0.1).with_(physics=CO),
+             W.chain_melt(700, seed=2),
+             jittered_argon(n_cells=6, seed=3, skin=0.1).with_(physics=CO)]
+    mix = engine(specs)
+    mix.step(40)
+    for k, s in enumerate(specs):
+        alone = engine(s)
+        alone.step(40)
+        assert np.array_equal(alone.positions(), mix.positions(k)), k
+        assert alone.scalars()["pe"] == mix.scalars(k)["pe"]
+
+
+def test_graph_replay_matches_eager():
+    say("case: graph replay vs eager, argon 864 FP64")
+    spec = jittered_argon(n_cells=6, skin=0.1).with_(thermostat=_abi.THERMO_BERENDSEN,
+                                                      thermo_temperature=100.0)
+    a = engine(spec, use_graph=True, graph_steps=5)
+    b = engine(spec)
+    for _ in range(4):
+        a.step(10)
+        b.step(10)
+        a.sync()
+        assert np.array_equal(a.positions(), b.positions())
+        assert np.array_equal(a.velocities(), b.velocities())
