@@ -80,6 +80,11 @@ static bool plan_radices(int K, FftPlan& P)
 
 Batch::~Batch()
 {
+    for (auto& u : ev_used) {
+        (void)hipEventDestroy(u.second.first);
+        (void)hipEventDestroy(u.second.second);
+    }
+    for (auto ev : ev_pool) (void)hipEventDestroy(ev);
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (graph) (void)hipGraphDestroy(graph);
     for (void* p : owned) (void)hipFree(p);
@@ -675,6 +680,51 @@ bool Batch::mirror_compute(int phase, int force_rebuild, bool* any_force)
     return any_rebuild;
 }
 
+static int g_acc = 0;
+static hipError_t launch_pair_acc(const Dev& d, hipStream_t s) { return launch_pair(d, s, g_acc); }
+
+// launch fn, bracketed by a pair of HIP events on the batch stream when
+// profiling is on (events are read back by kernel_time)
+hipError_t Batch::timed(int group, hipError_t (*fn)(const Dev&, hipStream_t))
+{
+    if (!options.profile) return fn(d, stream);
+    hipEvent_t a, b;
+    for (hipEvent_t* ev : {&a, &b}) {
+        if (!ev_pool.empty()) {
+            *ev = ev_pool.back();
+            ev_pool.pop_back();
+        } else {
+            hipError_t e = hipEventCreate(ev);
+            if (e != hipSuccess) return e;
+        }
+    }
+    hipError_t e = hipEventRecord(a, stream);
+    if (e != hipSuccess) return e;
+    e = fn(d, stream);
+    if (e != hipSuccess) return e;
+    e = hipEventRecord(b, stream);
+    ev_used.push_back({group, {a, b}});
+    return e;
+}
+
+int Batch::kernel_time(int group, double* total_ms, int64_t* launches)
+{
+    if (group < 0 || group >= KG_COUNT) return fail(DMDX_E_INVALID, "unknown kernel group");
+    TRY(sync());
+    for (auto& u : ev_used) {
+        float ms = 0.0f;
+        DMDX_HIP(hipEventElapsedTime(&ms, u.second.first, u.second.second));
+        kg_ms[u.first] += ms;
+        kg_n[u.first] += 1;
+        ev_pool.push_back(u.second.first);
+        ev_pool.push_back(u.second.second);
+    }
+    ev_used.clear();
+    *total_ms = kg_ms[group];
+    *launches = kg_n[group];
+    return DMDX_OK;
+}
+
 int Batch::enqueue_compute(int phase, int accumulate, int force_rebuild, bool uniform)
 {
     bool any_force = true, any_rebuild = true;
@@ -685,20 +735,21 @@ int Batch::enqueue_compute(int phase, int accumulate, int force_rebuild, bool un
     }
     LAUNCH(launch_pre_compute(d, stream, phase, force_rebuild, options.reuse_forces));
     if (!any_force) return DMDX_OK;
-    if (d.any_list && any_rebuild) LAUNCH(launch_build_lists(d, stream));
-    LAUNCH(launch_pair(d, stream, accumulate));
-    LAUNCH(launch_bonded(d, stream));
-    LAUNCH(launch_pme(d, stream));
-    LAUNCH(launch_gather(d, stream));
+    if (d.any_list && any_rebuild) LAUNCH(timed(KG_LIST, launch_build_lists));
+    g_acc = accumulate;
+    LAUNCH(timed(KG_PAIR, launch_pair_acc));
+    if (d.nb || d.na || d.nd || d.ni || d.ne) LAUNCH(timed(KG_BONDED, launch_bonded));
+    if (d.any_pme) LAUNCH(timed(KG_PME, launch_pme));
+    if (d.any_gather) LAUNCH(timed(KG_GATHER, launch_gather));
     return DMDX_OK;
 }
 
 int Batch::enqueue_step(bool uniform)
 {
     TRY(enqueue_compute(1, 0, 0, uniform));
-    LAUNCH(launch_kick_drift(d, stream));
+    LAUNCH(timed(KG_INTEGRATE, launch_kick_drift));
     TRY(enqueue_compute(2, 0, 0, uniform));
-    LAUNCH(launch_kick_ke(d, stream));
+    LAUNCH(timed(KG_INTEGRATE, launch_kick_ke));
     LAUNCH(launch_finalize(d, stream, 1));
     if (d.any_baro) {
         LAUNCH(launch_barostat(d, stream));
@@ -1055,6 +1106,21 @@ int dmdx_engine_get_pairs(dmdx_engine* e, int32_t sys, int32_t* pi, int32_t* pj,
 {
     if (!e || !count) return dmdx::fail(DMDX_E_INVALID, "null argument");
     return e->b.get_pairs(sys, pi, pj, cap, count);
+}
+
+int dmdx_engine_set_profiling(dmdx_engine* e, int32_t on)
+{
+    if (!e) return dmdx::fail(DMDX_E_INVALID, "null engine");
+    if (on && e->b.options.use_graph)
+        return dmdx::fail(DMDX_E_INVALID, "kernel timing needs eager launches (use_graph = 0)");
+    e->b.options.profile = on != 0;
+    return DMDX_OK;
+}
+
+int dmdx_engine_kernel_time(dmdx_engine* e, int32_t group, double* total_ms, int64_t* launches)
+{
+    if (!e || !total_ms || !launches) return dmdx::fail(DMDX_E_INVALID, "null argument");
+    return e->b.kernel_time(group, total_ms, launches);
 }
 
 int dmdx_engine_n_systems(dmdx_engine* e, int32_t* n)

@@ -27,7 +27,11 @@ struct BatchOptions {
     bool use_graph = false;    // replay captured blocks of steps
     int graph_steps = 10;
     int max_nbr = 0;           // 0: estimate from the density
+    bool profile = false;      // HIP events around each kernel group
 };
+
+// kernel groups timed when profiling is on
+enum KernelGroup { KG_PAIR = 0, KG_LIST, KG_BONDED, KG_PME, KG_GATHER, KG_INTEGRATE, KG_COUNT };
 
 // host mirror of what the host needs to decide launches
 struct HostSys {
@@ -73,7 +77,17 @@ private:
     int enqueue_compute(int phase, int accumulate, int force_rebuild, bool uniform);
     int enqueue_step(bool uniform);
 
+    hipError_t timed(int group, hipError_t (*fn)(const Dev&, hipStream_t));
     std::vector<void*> owned;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_used;
+
+public:
+    int kernel_time(int group, double* total_ms, int64_t* launches);
+    double kg_ms[KG_COUNT] = {0};
+    int64_t kg_n[KG_COUNT] = {0};
+
+private:
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
 };

@@ -45,7 +45,23 @@ def _setup(lib):
                                           C.POINTER(C.c_int32), C.c_int64,
                                           C.POINTER(C.c_int64)]
     lib.dmdx_engine_n_systems.argtypes = [C.c_void_p, C.POINTER(C.c_int32)]
+    lib.dmdx_engine_set_profiling.argtypes = [C.c_void_p, C.c_int32]
+    lib.dmdx_engine_kernel_time.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_double),
+                                            C.POINTER(C.c_int64)]
+    lib.dmdx_copy_bandwidth.argtypes = [C.c_int64, C.c_int32, C.POINTER(C.c_double)]
     lib._engine_ready = True
+
+
+KERNEL_GROUPS = ("pair", "list", "bonded", "pme", "gather", "integrate")
+
+
+def copy_bandwidth(nbytes: int = 1 << 28, repeats: int = 20) -> float:
+    """GB/s of a plain device-to-device copy kernel (read + write bytes)."""
+    lib = _abi.load()
+    _setup(lib)
+    g = C.c_double()
+    _abi.check(lib.dmdx_copy_bandwidth(int(nbytes), int(repeats), C.byref(g)))
+    return g.value
 
 
 class Engine:
@@ -134,6 +150,18 @@ class Engine:
 
     def set_active(self, sys: int, active: bool):
         _abi.check(self.lib.dmdx_engine_set_active(self.h, sys, int(active)))
+
+    def set_profiling(self, on: bool):
+        _abi.check(self.lib.dmdx_engine_set_profiling(self.h, int(on)))
+
+    def kernel_times(self) -> dict:
+        """{group: (total_ms, launches)} of the HIP-event-timed kernel groups."""
+        out = {}
+        for k, name in enumerate(KERNEL_GROUPS):
+            ms, n = C.c_double(), C.c_int64()
+            _abi.check(self.lib.dmdx_engine_kernel_time(self.h, k, C.byref(ms), C.byref(n)))
+            out[name] = (ms.value, n.value)
+        return out
 
     def pairs(self, sys: int = 0):
         cnt = C.c_int64(0)

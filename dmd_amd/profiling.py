@@ -1,0 +1,84 @@
+"""Summaries of rocprofv3 result databases (rocpd SQLite).
+
+    python -m dmd_amd.profiling stats DB [--top N]
+        kernel table (calls, total, average duration) from a
+        `rocprofv3 --kernel-trace --stats` run, as markdown
+
+    python -m dmd_amd.profiling traffic FETCH_DB WRITE_DB --copy-bytes B
+        HBM bytes per dispatch per kernel from two `rocprofv3 --pmc` runs
+        (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch), each scaled by what
+        the same counter reads for k_copy_calibration, a plain copy kernel
+        that reads and writes exactly B bytes per launch; JSON
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sqlite3
+import sys
+
+
+def short(name: str) -> str:
+    n = name.split("(")[0]
+    return n.replace("dmdx::", "")
+
+
+def kernel_stats(db: str):
+    con = sqlite3.connect(db)
+    rows = con.execute(
+        "select name, total_calls, total_duration, average, percentage from top_kernels "
+        "order by total_duration desc").fetchall()
+    return [(short(r[0]), int(r[1]), float(r[2]), float(r[3]), float(r[4])) for r in rows]
+
+
+def counter_means(db: str, counter: str):
+    con = sqlite3.connect(db)
+    rows = con.execute(
+        "select kernel_name, avg(value), count(*) from counters_collection "
+        "where counter_name = ? group by kernel_name", (counter,)).fetchall()
+    return {short(r[0]): (float(r[1]), int(r[2])) for r in rows}
+
+
+def traffic(fetch_db: str, write_db: str, copy_bytes: int):
+    f = counter_means(fetch_db, "FETCH_SIZE")
+    w = counter_means(write_db, "WRITE_SIZE")
+    cf = f["k_copy_calibration"][0]
+    cw = w["k_copy_calibration"][0]
+    sf = copy_bytes / 1024.0 / cf
+    sw = copy_bytes / 1024.0 / cw
+    out = {"calibration": {"kernel": "k_copy_calibration", "bytes_read": copy_bytes,
+                           "bytes_written": copy_bytes, "fetch_kib": cf, "write_kib": cw,
+                           "fetch_scale": sf, "write_scale": sw}, "kernels": {}}
+    for k in sorted(set(f) | set(w)):
+        fb = f.get(k, (0.0, 0))[0] * 1024.0 * sf
+        wb = w.get(k, (0.0, 0))[0] * 1024.0 * sw
+        out["kernels"][k] = {"fetch_bytes": fb, "write_bytes": wb,
+                             "hbm_bytes_per_launch": fb + wb,
+                             "dispatches": max(f.get(k, (0, 0))[1], w.get(k, (0, 0))[1])}
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    a = sub.add_parser("stats")
+    a.add_argument("db")
+    a.add_argument("--top", type=int, default=25)
+    b = sub.add_parser("traffic")
+    b.add_argument("fetch_db")
+    b.add_argument("write_db")
+    b.add_argument("--copy-bytes", type=int, default=1 << 28)
+    args = ap.parse_args(argv)
+    if args.cmd == "stats":
+        rows = kernel_stats(args.db)
+        # the top_kernels view reports durations in microseconds
+        print("| kernel | calls | total ms | avg us | % |")
+        print("|---|---:|---:|---:|---:|")
+        for name, calls, tot, avg, pct in rows[: args.top]:
+            print(f"| {name} | {calls} | {tot / 1e3:.3f} | {avg:.1f} | {pct:.2f} |")
+    else:
+        print(json.dumps(traffic(args.fetch_db, args.write_db, args.copy_bytes), indent=1))
+
+
+if __name__ == "__main__":
+    sys.exit(main())

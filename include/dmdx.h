@@ -203,6 +203,19 @@ int dmdx_engine_set_active(dmdx_engine *e, int32_t sys, int32_t active);
 int dmdx_engine_get_pairs(dmdx_engine *e, int32_t sys, int32_t *pi, int32_t *pj, int64_t cap,
                           int64_t *count);
 
+/* ---- measurement (no counterpart in the reference) -------------------- */
+
+/* Bracket every kernel group with HIP events on the engine's stream (eager
+ * launches only).  Groups: 0 pair forces, 1 list build, 2 bonded terms,
+ * 3 PME, 4 force gather, 5 kick/drift.  dmdx_engine_kernel_time waits for
+ * the stream and returns the accumulated time and launch count of a group. */
+int dmdx_engine_set_profiling(dmdx_engine *e, int32_t on);
+int dmdx_engine_kernel_time(dmdx_engine *e, int32_t group, double *total_ms, int64_t *launches);
+
+/* Device-to-device copy of nbytes, repeated; reports the achieved GB/s
+ * (read + write).  Calibrates hardware byte counters. */
+int dmdx_copy_bandwidth(int64_t nbytes, int32_t repeats, double *gbps);
+
 #ifdef __cplusplus
 }
 #endif
