@@ -49,12 +49,12 @@ __device__ __forceinline__ void finish_energy(double e, double* part, int64_t u)
 }
 
 // ---------------------------------------------------------------- bonds
-__global__ void k_bonds(BondedArgs a, const int* __restrict__ term_sys, int64_t nterm,
+__device__ __forceinline__ void k_bonds_unit(int64_t unit_, BondedArgs a, const int* __restrict__ term_sys, int64_t nterm,
                         const int* __restrict__ bi, const int* __restrict__ bj,
                         const double* __restrict__ bk, const double* __restrict__ br0,
                         int64_t tf_off, double* __restrict__ part)
 {
-    int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t u = unit_ * blockDim.x + threadIdx.x;
     if (u >= nterm) return;
     int s = term_sys[u];
     if (s < 0) return;
@@ -79,14 +79,22 @@ __global__ void k_bonds(BondedArgs a, const int* __restrict__ term_sys, int64_t 
     finish_energy(energy, part, u);
 }
 
+__global__ void k_bonds(BondedArgs a, const int* __restrict__ term_sys, int64_t nterm,
+                        const int* __restrict__ bi, const int* __restrict__ bj,
+                        const double* __restrict__ bk, const double* __restrict__ br0,
+                        int64_t tf_off, double* __restrict__ part, int64_t nunits_)
+{
+    DMDX_FOR_UNITS(u_, nunits_) k_bonds_unit(u_, a, term_sys, nterm, bi, bj, bk, br0, tf_off, part);
+}
+
 // --------------------------------------------------------------- angles
-__global__ void k_angles(BondedArgs a, const int* __restrict__ term_sys, int64_t nterm,
+__device__ __forceinline__ void k_angles_unit(int64_t unit_, BondedArgs a, const int* __restrict__ term_sys, int64_t nterm,
                          const int* __restrict__ ai_, const int* __restrict__ aj_,
                          const int* __restrict__ ak_, const double* __restrict__ kth,
                          const double* __restrict__ th0, int64_t tf_off,
                          double* __restrict__ part)
 {
-    int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t u = unit_ * blockDim.x + threadIdx.x;
     if (u >= nterm) return;
     int s = term_sys[u];
     if (s < 0) return;
@@ -128,6 +136,15 @@ __global__ void k_angles(BondedArgs a, const int* __restrict__ term_sys, int64_t
         energy = 0.5 * kt * dtheta * dtheta;
     }
     finish_energy(energy, part, u);
+}
+
+__global__ void k_angles(BondedArgs a, const int* __restrict__ term_sys, int64_t nterm,
+                         const int* __restrict__ ai_, const int* __restrict__ aj_,
+                         const int* __restrict__ ak_, const double* __restrict__ kth,
+                         const double* __restrict__ th0, int64_t tf_off,
+                         double* __restrict__ part, int64_t nunits_)
+{
+    DMDX_FOR_UNITS(u_, nunits_) k_angles_unit(u_, a, term_sys, nterm, ai_, aj_, ak_, kth, th0, tf_off, part);
 }
 
 // ------------------------------------------------------------- torsions
@@ -240,14 +257,14 @@ __device__ __forceinline__ void torsion_zero(TermF* tf, int64_t base)
     for (int r = 0; r < 4; ++r) put(tf, base + r, 0.0, 0.0, 0.0);
 }
 
-__global__ void k_dihedrals(BondedArgs a, const int* __restrict__ term_sys, int64_t nterm,
+__device__ __forceinline__ void k_dihedrals_unit(int64_t unit_, BondedArgs a, const int* __restrict__ term_sys, int64_t nterm,
                             const int* __restrict__ di, const int* __restrict__ dj,
                             const int* __restrict__ dk, const int* __restrict__ dl,
                             const int* __restrict__ per, const double* __restrict__ kp,
                             const double* __restrict__ p0, int64_t tf_off,
                             double* __restrict__ part)
 {
-    int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t u = unit_ * blockDim.x + threadIdx.x;
     if (u >= nterm) return;
     int s = term_sys[u];
     if (s < 0) return;
@@ -277,13 +294,23 @@ __global__ void k_dihedrals(BondedArgs a, const int* __restrict__ term_sys, int6
     finish_energy(energy, part, u);
 }
 
-__global__ void k_impropers(BondedArgs a, const int* __restrict__ term_sys, int64_t nterm,
+__global__ void k_dihedrals(BondedArgs a, const int* __restrict__ term_sys, int64_t nterm,
+                            const int* __restrict__ di, const int* __restrict__ dj,
+                            const int* __restrict__ dk, const int* __restrict__ dl,
+                            const int* __restrict__ per, const double* __restrict__ kp,
+                            const double* __restrict__ p0, int64_t tf_off,
+                            double* __restrict__ part, int64_t nunits_)
+{
+    DMDX_FOR_UNITS(u_, nunits_) k_dihedrals_unit(u_, a, term_sys, nterm, di, dj, dk, dl, per, kp, p0, tf_off, part);
+}
+
+__device__ __forceinline__ void k_impropers_unit(int64_t unit_, BondedArgs a, const int* __restrict__ term_sys, int64_t nterm,
                             const int* __restrict__ di, const int* __restrict__ dj,
                             const int* __restrict__ dk, const int* __restrict__ dl,
                             const double* __restrict__ kp, const double* __restrict__ p0,
                             int64_t tf_off, double* __restrict__ part)
 {
-    int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t u = unit_ * blockDim.x + threadIdx.x;
     if (u >= nterm) return;
     int s = term_sys[u];
     if (s < 0) return;
@@ -311,15 +338,24 @@ __global__ void k_impropers(BondedArgs a, const int* __restrict__ term_sys, int6
     finish_energy(energy, part, u);
 }
 
+__global__ void k_impropers(BondedArgs a, const int* __restrict__ term_sys, int64_t nterm,
+                            const int* __restrict__ di, const int* __restrict__ dj,
+                            const int* __restrict__ dk, const int* __restrict__ dl,
+                            const double* __restrict__ kp, const double* __restrict__ p0,
+                            int64_t tf_off, double* __restrict__ part, int64_t nunits_)
+{
+    DMDX_FOR_UNITS(u_, nunits_) k_impropers_unit(u_, a, term_sys, nterm, di, dj, dk, dl, kp, p0, tf_off, part);
+}
+
 // ------------------------------------------------------------ exclusions
 // The reference subtracts the whole term sum from the energy
 // (energy_out -= energy, src/force/coulomb_exclusion.cpp:55): the partials
 // hold +sum and the finalize subtracts.
-__global__ void k_exclusions(BondedArgs a, const int* __restrict__ term_sys, int64_t nterm,
+__device__ __forceinline__ void k_exclusions_unit(int64_t unit_, BondedArgs a, const int* __restrict__ term_sys, int64_t nterm,
                              const int* __restrict__ ei, const int* __restrict__ ej,
                              int64_t tf_off, double* __restrict__ part)
 {
-    int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t u = unit_ * blockDim.x + threadIdx.x;
     if (u >= nterm) return;
     int s = term_sys[u];
     if (s < 0) return;
@@ -353,6 +389,13 @@ __global__ void k_exclusions(BondedArgs a, const int* __restrict__ term_sys, int
     finish_energy(energy, part, u);
 }
 
+__global__ void k_exclusions(BondedArgs a, const int* __restrict__ term_sys, int64_t nterm,
+                             const int* __restrict__ ei, const int* __restrict__ ej,
+                             int64_t tf_off, double* __restrict__ part, int64_t nunits_)
+{
+    DMDX_FOR_UNITS(u_, nunits_) k_exclusions_unit(u_, a, term_sys, nterm, ei, ej, tf_off, part);
+}
+
 }  // namespace dmdx
 
 namespace dmdx {
@@ -362,20 +405,20 @@ hipError_t launch_bonded(const Dev& d, hipStream_t s)
     const int BS = 256;
     BondedArgs a{d.sys, d.st, d.x, d.y, d.z, d.q, d.tf};
     if (d.nb)
-        k_bonds<<<blocks_for(d.nb, BS), BS, 0, s>>>(a, d.b_sys, d.nb, d.bi, d.bj, d.bk, d.br0,
-                                                    d.tf_bond, d.p_bond);
+        k_bonds<<<grid_for(blocks_for(d.nb, BS)), BS, 0, s>>>(a, d.b_sys, d.nb, d.bi, d.bj, d.bk, d.br0,
+                                                    d.tf_bond, d.p_bond, (int64_t)(blocks_for(d.nb, BS)));
     if (d.na)
-        k_angles<<<blocks_for(d.na, BS), BS, 0, s>>>(a, d.a_sys, d.na, d.ai, d.aj, d.ak, d.akt,
-                                                     d.at0, d.tf_angle, d.p_angle);
+        k_angles<<<grid_for(blocks_for(d.na, BS)), BS, 0, s>>>(a, d.a_sys, d.na, d.ai, d.aj, d.ak, d.akt,
+                                                     d.at0, d.tf_angle, d.p_angle, (int64_t)(blocks_for(d.na, BS)));
     if (d.nd)
-        k_dihedrals<<<blocks_for(d.nd, BS), BS, 0, s>>>(a, d.d_sys, d.nd, d.di, d.dj, d.dk, d.dl,
-                                                        d.dper, d.dkp, d.dp0, d.tf_dih, d.p_dih);
+        k_dihedrals<<<grid_for(blocks_for(d.nd, BS)), BS, 0, s>>>(a, d.d_sys, d.nd, d.di, d.dj, d.dk, d.dl,
+                                                        d.dper, d.dkp, d.dp0, d.tf_dih, d.p_dih, (int64_t)(blocks_for(d.nd, BS)));
     if (d.ni)
-        k_impropers<<<blocks_for(d.ni, BS), BS, 0, s>>>(a, d.i_sys, d.ni, d.ii, d.ij, d.ik, d.il,
-                                                        d.ikp, d.ip0, d.tf_imp, d.p_imp);
+        k_impropers<<<grid_for(blocks_for(d.ni, BS)), BS, 0, s>>>(a, d.i_sys, d.ni, d.ii, d.ij, d.ik, d.il,
+                                                        d.ikp, d.ip0, d.tf_imp, d.p_imp, (int64_t)(blocks_for(d.ni, BS)));
     if (d.ne)
-        k_exclusions<<<blocks_for(d.ne, BS), BS, 0, s>>>(a, d.e_sys, d.ne, d.ei, d.ej, d.tf_ex,
-                                                         d.p_excl);
+        k_exclusions<<<grid_for(blocks_for(d.ne, BS)), BS, 0, s>>>(a, d.e_sys, d.ne, d.ei, d.ej, d.tf_ex,
+                                                         d.p_excl, (int64_t)(blocks_for(d.ne, BS)));
     return hipGetLastError();
 }
 

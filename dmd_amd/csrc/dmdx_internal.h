@@ -61,7 +61,10 @@ struct SysDev {
     double rc_lj2, rc_c2, alpha;
     // list
     int32_t has_list;
-    int32_t interval;    // computes between rebuilds (<= 0: never auto)
+    int32_t interval;    // > 0: computes between rebuilds (DMD cadence);
+                         //   0: rebuild when an atom moved > skin/2 (Verlet
+                         //      criterion); < 0: explicit rebuilds only
+    double half_skin2;   // (skin/2)^2 for the displacement criterion
     double rlist, rlist2;
     int64_t nbr_off;     // start of this system's M x n_pad list block
     int32_t max_nbr;
@@ -113,6 +116,8 @@ struct SysState {
     int32_t valid;       // forces/energies correspond to the current positions
     int32_t flags;       // bit0 neighbour overflow, bit1 non-finite energy
     int32_t active;      // system participates in steps
+    int32_t built;       // a list has been built since creation
+    int64_t rebuilds;    // list builds since creation
     int64_t steps_done;
 };
 
@@ -154,8 +159,16 @@ struct Dev {
     // lists
     int *acell = nullptr, *ccount = nullptr, *cstart = nullptr, *cfill = nullptr;
     int *cell_atoms = nullptr, *cell_sys = nullptr, *order = nullptr, *nn = nullptr;
-    int* nbr = nullptr;
+    int* nbr = nullptr;  // row-major: row of atom slot t at nbr_off + (t - atom_off) * M
     int *ex_start = nullptr, *ex_list = nullptr;
+    double4* pos4s = nullptr;  // (x, y, z, q) in cell order (list build; mixed pair kernel)
+    double4* xref = nullptr;   // positions at the last list build (displacement criterion)
+    float4* pos4f = nullptr;   // mixed list build: wrapped FP32 positions in cell order
+    double* p_disp = nullptr;  // per-wave max squared displacement
+    bool any_auto = false;     // some system uses the displacement criterion
+    int* typ_s = nullptr;      // mixed path: LJ type in cell order
+    int pair_tpa = 16;         // mixed pair kernel: lanes per atom (8, 16, 32)
+    int build_tpa = 16;        // mixed list build: lanes per atom (8, 16, 32)
     // LJ tables
     double *lj_sig = nullptr, *lj_eps = nullptr;
     float *lj_c12 = nullptr, *lj_c6 = nullptr;
@@ -196,10 +209,12 @@ struct Dev {
 // host launchers (one per kernel group); return hipGetLastError()
 hipError_t launch_build_lists(const Dev& d, hipStream_t s);
 hipError_t launch_pair(const Dev& d, hipStream_t s, int accumulate);
+hipError_t launch_pack(const Dev& d, hipStream_t s);
 hipError_t launch_bonded(const Dev& d, hipStream_t s);
 hipError_t launch_pme(const Dev& d, hipStream_t s);
 hipError_t launch_pre_compute(const Dev& d, hipStream_t s, int phase, int force_rebuild,
                               int reuse_enabled);
+hipError_t launch_displacement(const Dev& d, hipStream_t s);
 hipError_t launch_pre_rebuild_only(const Dev& d, hipStream_t s);
 hipError_t launch_gather(const Dev& d, hipStream_t s);
 hipError_t launch_kick_drift(const Dev& d, hipStream_t s);
@@ -209,6 +224,20 @@ hipError_t launch_barostat(const Dev& d, hipStream_t s);
 hipError_t launch_materialize(const Dev& d, hipStream_t s);
 
 inline unsigned blocks_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// Fixed-size grids: a launch covers `units` work units with at most
+// MAX_GRID workgroups (a multiple of 8) that loop over them, so launches whose
+// units mostly have nothing to do (e.g. systems that are not rebuilding their
+// lists) cost a few microseconds instead of the dispatch of one workgroup per
+// unit.  See DMDX_FOR_UNITS.
+constexpr int64_t MAX_GRID = 2048;
+inline unsigned grid_for(int64_t units)
+{
+    int64_t g = (units + 7) / 8 * 8;
+    if (g > MAX_GRID) g = MAX_GRID;
+    if (g < 8) g = 8;
+    return (unsigned)g;
+}
 
 // ------------------------------------------------------- device helpers
 // Orthorhombic minimum image with DMD's expression order
@@ -220,6 +249,33 @@ __device__ __forceinline__ double min_image_dmd(double d, double L, double inv)
     return d - round(d * inv) * L;
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup p runs on
+// XCD p % 8).  Give each XCD a contiguous range of logical blocks, so the
+// atoms (and systems) one XCD touches stay in its own L2.
+__device__ __forceinline__ unsigned xcd_block()
+{
+    const unsigned p = blockIdx.x, G = gridDim.x;
+    const unsigned x = p & 7u, q = p >> 3;
+    const unsigned n = G >> 3, r = G & 7u;
+    return (x < r) ? x * (n + 1) + q : r * (n + 1) + (x - r) * n + q;
+}
+
+// Plain grid-stride loop over work units [0, U): consecutive units go to
+// different workgroups (and XCDs) -- for launches whose work is concentrated
+// in a few systems, where spreading beats L2 locality.
+#define DMDX_FOR_UNITS_SPREAD(u, U) \
+    for (int64_t u = blockIdx.x; u < (int64_t)(U); u += gridDim.x)
+
+// Loop over work units [0, U): each XCD (workgroups p with p % 8 == x) owns
+// the contiguous share [U*x/8, U*(x+1)/8) and its workgroups stride through
+// it.  gridDim.x must be a multiple of 8 (grid_for).  All threads of a
+// workgroup see the same unit sequence.
+#define DMDX_FOR_UNITS(u, U)                                                          \
+    for (int64_t u##_x = blockIdx.x & 7u, u##_n = gridDim.x >> 3,                    \
+                 u##_end = ((int64_t)(U) * (u##_x + 1)) >> 3,                          \
+                 u = (((int64_t)(U) * u##_x) >> 3) + (blockIdx.x >> 3);                \
+         u < u##_end; u += u##_n)
+
 template <class T>
 __device__ __forceinline__ T wave_sum(T v)
 {
@@ -228,5 +284,13 @@ __device__ __forceinline__ T wave_sum(T v)
 }
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+
+// sum over the TPA lanes of a lane group (TPA a power of two <= 64)
+template <int TPA, class T>
+__device__ __forceinline__ T group_sum(T v)
+{
+    for (int o = TPA / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
 
 }  // namespace dmdx

@@ -169,8 +169,14 @@ int Batch::create(const dmdx_system_desc* descs, int nsys, const BatchOptions& o
         S.has_list = (S.use_lj || S.coul_pairs) ? 1 : 0;
         S.rlist = rl + D.list_skin;
         S.rlist2 = S.rlist * S.rlist;
-        S.interval = D.rebuild_interval > 0 ? D.rebuild_interval : 10;
-        if (opt.component == COMP_COULOMB_DIRECT) S.interval = 0;  // explicit rebuild only
+        // rebuild_interval: > 0 cadence, 0 DMD's 10, -1 Verlet criterion,
+        // <= -2 explicit rebuilds only
+        if (D.rebuild_interval > 0) S.interval = D.rebuild_interval;
+        else if (D.rebuild_interval == 0) S.interval = 10;
+        else if (D.rebuild_interval == -1) S.interval = 0;
+        else S.interval = -1;
+        if (opt.component == COMP_COULOMB_DIRECT) S.interval = -1;
+        S.half_skin2 = 0.25 * D.list_skin * D.list_skin;
         if (S.has_list) {
             double vol = D.box[0] * D.box[1] * D.box[2];
             double dens = n / vol;
@@ -179,6 +185,7 @@ int Batch::create(const dmdx_system_desc* descs, int nsys, const BatchOptions& o
             M = std::min(M, n - 1);
             M = std::max(M, 1);
             if (opt.max_nbr > 0) M = std::min(std::max(opt.max_nbr, 1), std::max(n - 1, 1));
+            M = (M + 3) / 4 * 4;
             if (M > 2048)
                 return fail(DMDX_E_PARM, "neighbour list would exceed 2048 entries per atom; "
                                          "reduce the cutoff");
@@ -541,6 +548,7 @@ int Batch::create(const dmdx_system_desc* descs, int nsys, const BatchOptions& o
         host[s].has_list = S.has_list != 0;
         host[s].interval = S.interval;
         host[s].counter = hstate[s].counter;
+        d.any_auto |= S.has_list && S.interval == 0;
     }
     d.any_gather = d.nb || d.na || d.nd || d.ni || d.ne || d.any_pme;
 
@@ -627,8 +635,15 @@ int Batch::create(const dmdx_system_desc* descs, int nsys, const BatchOptions& o
     TRY(dev_upload(&d.plans, plans, owned));
     for (int k = 0; k < 3; ++k) TRY(dev_upload(&d.tiles[k], tiles[k], owned));
     const int64_t nw = slot / WAVE;
-    TRY(dev_alloc_zero(&d.p_lj, nw, owned));
-    TRY(dev_alloc_zero(&d.p_coul, nw, owned));
+    TRY(dev_alloc_zero(&d.p_lj, slot, owned));
+    TRY(dev_alloc_zero(&d.p_coul, slot, owned));
+    TRY(dev_alloc_zero(&d.pos4s, slot, owned));
+    TRY(dev_alloc_zero(&d.xref, slot, owned));
+    TRY(dev_alloc_zero(&d.p_disp, nw, owned));
+    if (prec != DMDX_FP64) {
+        TRY(dev_alloc_zero(&d.typ_s, slot, owned));
+        TRY(dev_alloc_zero(&d.pos4f, slot, owned));
+    }
     TRY(dev_alloc_zero(&d.p_ke, nw, owned));
     TRY(dev_alloc_zero(&d.p_w, nw, owned));
     TRY(dev_alloc_zero(&d.p_bond, std::max<int64_t>(ob / WAVE, 1), owned));
@@ -670,11 +685,18 @@ bool Batch::mirror_compute(int phase, int force_rebuild, bool* any_force)
     for (size_t s = 0; s < host.size(); ++s) {
         HostSys& H = host[s];
         if (!H.active) continue;
-        bool rebuild = H.has_list && (force_rebuild || (H.interval > 0 && H.counter >= H.interval));
-        bool reuse = phase == 1 && options.reuse_forces && H.valid && !rebuild;
+        bool rebuild = false, maybe = false;
+        if (H.has_list) {
+            if (force_rebuild) rebuild = true;
+            else if (H.interval > 0) rebuild = H.counter >= H.interval;
+            else if (H.interval == 0) maybe = !H.built || !(phase == 1 && H.valid);
+        }
+        bool reuse = phase == 1 && options.reuse_forces && H.valid && !rebuild && !maybe;
         if (H.has_list && H.interval > 0) H.counter = (rebuild ? 0 : H.counter) + 1;
+        // the device builds a list at the first compute of every system
+        if (H.has_list) H.built = true;
         H.valid = true;
-        any_rebuild |= rebuild;
+        any_rebuild |= rebuild || maybe;
         *any_force |= !reuse;
     }
     return any_rebuild;
@@ -733,10 +755,12 @@ int Batch::enqueue_compute(int phase, int accumulate, int force_rebuild, bool un
         bool dummy;
         mirror_compute(phase, force_rebuild, &dummy);
     }
+    if (d.any_auto && any_rebuild) LAUNCH(launch_displacement(d, stream));
     LAUNCH(launch_pre_compute(d, stream, phase, force_rebuild, options.reuse_forces));
     if (!any_force) return DMDX_OK;
     if (d.any_list && any_rebuild) LAUNCH(timed(KG_LIST, launch_build_lists));
     g_acc = accumulate;
+    LAUNCH(launch_pack(d, stream));
     LAUNCH(timed(KG_PAIR, launch_pair_acc));
     if (d.nb || d.na || d.nd || d.ni || d.ne) LAUNCH(timed(KG_BONDED, launch_bonded));
     if (d.any_pme) LAUNCH(timed(KG_PME, launch_pme));
@@ -822,7 +846,10 @@ int Batch::rebuild_only()
     LAUNCH(launch_pre_rebuild_only(d, stream));
     if (d.any_list) LAUNCH(launch_build_lists(d, stream));
     for (auto& H : host)
-        if (H.has_list) H.counter = 0;
+        if (H.has_list) {
+            H.counter = 0;
+            H.built = true;
+        }
     return DMDX_OK;
 }
 
@@ -956,11 +983,14 @@ int Batch::get_pairs(int s, int32_t* pi, int32_t* pj, int64_t cap, int64_t* coun
         blk.resize((size_t)S.max_nbr * S.n_pad);
         TRY(copy(blk.data(), d.nbr + S.nbr_off, sizeof(int) * blk.size(), hipMemcpyDeviceToHost));
     }
+    // FP64 rows hold atom slots, mixed rows hold cell-order slots
+    const bool sorted_ids = S.precision != DMDX_FP64;
     std::vector<int64_t> keys;
     for (int t = 0; t < S.n; ++t) {
         int i = order[t] - (int)S.atom_off;
         for (int k = 0; k < nn[t]; ++k) {
-            int j = blk[(size_t)k * S.n_pad + t] - (int)S.atom_off;
+            int e = blk[(size_t)t * S.max_nbr + k];
+            int j = (sorted_ids ? order[e - (int)S.atom_off] : e) - (int)S.atom_off;
             if (j > i) keys.push_back(((int64_t)i << 32) | (int64_t)j);
         }
     }
@@ -1086,6 +1116,7 @@ int dmdx_engine_get_scalars(dmdx_engine* e, int32_t sys, dmdx_scalars* out)
     out->step_since_rebuild = T.counter;
     out->flags = T.flags;
     out->steps_done = T.steps_done;
+    out->list_builds = T.rebuilds;
     return DMDX_OK;
 }
 
@@ -1121,6 +1152,18 @@ int dmdx_engine_kernel_time(dmdx_engine* e, int32_t group, double* total_ms, int
 {
     if (!e || !total_ms || !launches) return dmdx::fail(DMDX_E_INVALID, "null argument");
     return e->b.kernel_time(group, total_ms, launches);
+}
+
+int dmdx_engine_set_tuning(dmdx_engine* e, int32_t key, int32_t value)
+{
+    if (!e) return dmdx::fail(DMDX_E_INVALID, "null engine");
+    if (value != 8 && value != 16 && value != 32)
+        return dmdx::fail(DMDX_E_PARM, "lanes per atom must be 8, 16 or 32");
+    if (key == 0) e->b.d.pair_tpa = value;
+    else if (key == 1) e->b.d.build_tpa = value;
+    else return dmdx::fail(DMDX_E_INVALID, "unknown tuning key");
+    e->b.invalidate_graph();
+    return DMDX_OK;
 }
 
 int dmdx_engine_n_systems(dmdx_engine* e, int32_t* n)

@@ -40,6 +40,7 @@ struct HostSys {
     int interval = 10;
     int counter = 10;
     bool valid = false;
+    bool built = false;
     bool active = true;
     bool has_barostat = false;
 };

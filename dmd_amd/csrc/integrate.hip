@@ -26,13 +26,27 @@ struct Cx2 {
 };
 
 // phase: 0 standalone compute, 1 first compute of a step, 2 second compute;
-// force_rebuild: 1 = rebuild_pair_list requested by the host
-__global__ void k_pre_compute(const SysDev* __restrict__ sys, SysState* __restrict__ st, int nsys,
-                              int phase, int force_rebuild, int reuse_enabled)
+// force_rebuild: 1 = rebuild_pair_list requested by the host.
+// Rebuild rule (SysDev::interval): > 0 DMD's cadence (counter reaches the
+// interval, src/force/lennard_jones.cpp:53-55); 0 the Verlet criterion: some
+// atom moved more than skin/2 since the last build (checked whenever the
+// positions may have changed since the last compute).
+__global__ void __launch_bounds__(64) k_pre_compute(const SysDev* __restrict__ sys,
+                                                    SysState* __restrict__ st, int nsys,
+                                                    int phase, int force_rebuild,
+                                                    int reuse_enabled,
+                                                    const double* __restrict__ p_disp)
 {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    // one wave per system; lane 0 decides
+    int s = blockIdx.x;
     if (s >= nsys) return;
     SysState& T = st[s];
+    const SysDev& S0 = sys[s];
+    double dmax = 0.0;
+    if (p_disp && S0.has_list && S0.interval == 0 && T.active && T.built && !(phase == 1 && T.valid))
+        for (int w = lane_id(); w < S0.aw_n; w += 64) dmax = fmax(dmax, p_disp[S0.aw_off + w]);
+    for (int o = 32; o > 0; o >>= 1) dmax = fmax(dmax, __shfl_xor(dmax, o, 64));
+    if (lane_id() != 0) return;
     if (!T.active) {
         T.rebuild = 0;
         T.reuse = 1;
@@ -40,10 +54,26 @@ __global__ void k_pre_compute(const SysDev* __restrict__ sys, SysState* __restri
     }
     const SysDev& S = sys[s];
     int rebuild = 0;
-    if (S.has_list) rebuild = force_rebuild || (S.interval > 0 && T.counter >= S.interval);
+    if (S.has_list) {
+        if (force_rebuild) {
+            rebuild = 1;
+        } else if (S.interval > 0) {
+            rebuild = T.counter >= S.interval;
+        } else if (S.interval == 0) {
+            if (!T.built) {
+                rebuild = 1;
+            } else if (!(phase == 1 && T.valid)) {
+                rebuild = dmax > S.half_skin2;
+            }
+        }
+    }
     T.rebuild = rebuild;
+    if (rebuild) {
+        T.built = 1;
+        T.rebuilds += 1;
+    }
     T.reuse = (phase == 1 && reuse_enabled && T.valid && !rebuild) ? 1 : 0;
-    if (S.has_list && S.interval > 0) T.counter = (rebuild ? 0 : T.counter) + 1;
+    if (S.has_list && S.interval >= 0) T.counter = (rebuild ? 0 : T.counter) + 1;
     T.valid = 1;
 }
 
@@ -56,7 +86,10 @@ __global__ void k_pre_rebuild_only(const SysDev* __restrict__ sys, SysState* __r
     if (s >= nsys) return;
     st[s].rebuild = sys[s].has_list ? 1 : 0;
     st[s].reuse = 1;  // no force kernels
-    if (sys[s].has_list) st[s].counter = 0;
+    if (sys[s].has_list) {
+        st[s].counter = 0;
+        st[s].built = 1;
+    }
 }
 
 // Per-atom gather.  cstart[g] .. cmid[g]: bonded contributions (bonds,
@@ -64,7 +97,7 @@ __global__ void k_pre_rebuild_only(const SysDev* __restrict__ sys, SysState* __r
 // exclusion contributions.  PME sits between them, as in build_force_engine
 // (src/sim/simulation_engine.cpp:147-203).
 template <class R>
-__global__ void k_gather(const SysDev* __restrict__ sys, const SysState* __restrict__ st,
+__device__ __forceinline__ void k_gather_unit(int64_t unit_, const SysDev* __restrict__ sys, const SysState* __restrict__ st,
                          const int* __restrict__ slot_sys, int64_t nslots,
                          const int* __restrict__ cstart, const int* __restrict__ cmid,
                          const int* __restrict__ contrib, const TermF* __restrict__ tf,
@@ -73,7 +106,7 @@ __global__ void k_gather(const SysDev* __restrict__ sys, const SysState* __restr
                          const Cx2<R>* __restrict__ grid, double* __restrict__ fx,
                          double* __restrict__ fy, double* __restrict__ fz)
 {
-    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t g = unit_ * blockDim.x + threadIdx.x;
     if (g >= nslots) return;
     int s = slot_sys[g];
     if (s < 0 || st[s].reuse || !st[s].active) return;
@@ -164,18 +197,21 @@ __global__ void k_gather(const SysDev* __restrict__ sys, const SysState* __restr
     fz[g] = Fz;
 }
 
-template __global__ void k_gather<double>(const SysDev*, const SysState*, const int*, int64_t,
-                                          const int*, const int*, const int*, const TermF*,
-                                          const double*, const double*, const double*,
-                                          const double*, const Cx2<double>*, double*, double*,
-                                          double*);
-template __global__ void k_gather<float>(const SysDev*, const SysState*, const int*, int64_t,
-                                         const int*, const int*, const int*, const TermF*,
-                                         const double*, const double*, const double*,
-                                         const double*, const Cx2<float>*, double*, double*,
-                                         double*);
+template <class R>
+__global__ void k_gather(const SysDev* __restrict__ sys, const SysState* __restrict__ st,
+                         const int* __restrict__ slot_sys, int64_t nslots,
+                         const int* __restrict__ cstart, const int* __restrict__ cmid,
+                         const int* __restrict__ contrib, const TermF* __restrict__ tf,
+                         const double* __restrict__ x, const double* __restrict__ y,
+                         const double* __restrict__ z, const double* __restrict__ q,
+                         const Cx2<R>* __restrict__ grid, double* __restrict__ fx,
+                         double* __restrict__ fy, double* __restrict__ fz, int64_t nunits_)
+{
+    DMDX_FOR_UNITS(u_, nunits_) k_gather_unit<R>(u_, sys, st, slot_sys, nslots, cstart, cmid, contrib, tf, x, y, z, q, grid, fx, fy, fz);
+}
 
-__global__ void k_kick_drift(const SysDev* __restrict__ sys, const SysState* __restrict__ st,
+
+__device__ __forceinline__ void k_kick_drift_unit(int64_t unit_, const SysDev* __restrict__ sys, const SysState* __restrict__ st,
                              const int* __restrict__ slot_sys, int64_t nslots,
                              const double* __restrict__ invm, const double* __restrict__ fx,
                              const double* __restrict__ fy, const double* __restrict__ fz,
@@ -183,7 +219,7 @@ __global__ void k_kick_drift(const SysDev* __restrict__ sys, const SysState* __r
                              double* __restrict__ vz, double* __restrict__ x,
                              double* __restrict__ y, double* __restrict__ z)
 {
-    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t g = unit_ * blockDim.x + threadIdx.x;
     if (g >= nslots) return;
     int s = slot_sys[g];
     if (s < 0 || !st[s].active) return;
@@ -205,7 +241,18 @@ __global__ void k_kick_drift(const SysDev* __restrict__ sys, const SysState* __r
     z[g] += c * dt;
 }
 
-__global__ void k_kick_ke(const SysDev* __restrict__ sys, const SysState* __restrict__ st,
+__global__ void k_kick_drift(const SysDev* __restrict__ sys, const SysState* __restrict__ st,
+                             const int* __restrict__ slot_sys, int64_t nslots,
+                             const double* __restrict__ invm, const double* __restrict__ fx,
+                             const double* __restrict__ fy, const double* __restrict__ fz,
+                             double* __restrict__ vx, double* __restrict__ vy,
+                             double* __restrict__ vz, double* __restrict__ x,
+                             double* __restrict__ y, double* __restrict__ z, int64_t nunits_)
+{
+    DMDX_FOR_UNITS(u_, nunits_) k_kick_drift_unit(u_, sys, st, slot_sys, nslots, invm, fx, fy, fz, vx, vy, vz, x, y, z);
+}
+
+__device__ __forceinline__ void k_kick_ke_unit(int64_t unit_, const SysDev* __restrict__ sys, const SysState* __restrict__ st,
                           const int* __restrict__ slot_sys, int64_t nslots,
                           const double* __restrict__ invm, const double* __restrict__ m,
                           const double* __restrict__ fx, const double* __restrict__ fy,
@@ -213,7 +260,7 @@ __global__ void k_kick_ke(const SysDev* __restrict__ sys, const SysState* __rest
                           double* __restrict__ vy, double* __restrict__ vz,
                           double* __restrict__ ke_part)
 {
-    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t g = unit_ * blockDim.x + threadIdx.x;
     if (g >= nslots) return;
     int s = slot_sys[g];
     if (s < 0 || !st[s].active) return;
@@ -234,13 +281,31 @@ __global__ void k_kick_ke(const SysDev* __restrict__ sys, const SysState* __rest
     if (lane_id() == 0) ke_part[g >> 6] = ke;
 }
 
-// sum of part[off .. off+n) by one wave: lane l sums l, l+64, ... in order,
-// then a fixed butterfly -- deterministic
-__device__ __forceinline__ double wave_range_sum(const double* part, int off, int n)
+__global__ void k_kick_ke(const SysDev* __restrict__ sys, const SysState* __restrict__ st,
+                          const int* __restrict__ slot_sys, int64_t nslots,
+                          const double* __restrict__ invm, const double* __restrict__ m,
+                          const double* __restrict__ fx, const double* __restrict__ fy,
+                          const double* __restrict__ fz, double* __restrict__ vx,
+                          double* __restrict__ vy, double* __restrict__ vz,
+                          double* __restrict__ ke_part, int64_t nunits_)
+{
+    DMDX_FOR_UNITS(u_, nunits_) k_kick_ke_unit(u_, sys, st, slot_sys, nslots, invm, m, fx, fy, fz, vx, vy, vz, ke_part);
+}
+
+// sum of part[off .. off+n) by a 256-thread workgroup: thread t sums t,
+// t+256, ... in order, then a fixed tree over the workgroup -- deterministic;
+// every thread gets the result
+__device__ double block_range_sum(const double* part, int64_t off, int64_t n, double* red)
 {
     double a = 0.0;
-    for (int k = lane_id(); k < n; k += 64) a += part[off + k];
-    return wave_sum(a);
+    for (int64_t k = threadIdx.x; k < n; k += 256) a += part[off + k];
+    a = wave_sum(a);
+    __syncthreads();
+    if (lane_id() == 0) red[threadIdx.x >> 6] = a;
+    __syncthreads();
+    double r = (red[0] + red[1]) + (red[2] + red[3]);
+    __syncthreads();
+    return r;
 }
 
 struct Parts {
@@ -254,26 +319,28 @@ __device__ __forceinline__ double temperature_of(double ekin, int n)
     return 2.0 * ekin / (dof * KB);
 }
 
-// one 64-thread workgroup per system, after the closing kick
-__global__ void __launch_bounds__(64) k_finalize(const SysDev* __restrict__ sys,
-                                                 SysState* __restrict__ st, Parts P,
-                                                 int with_kinetic)
+// one 256-thread workgroup per system, after the closing kick
+__global__ void __launch_bounds__(256) k_finalize(const SysDev* __restrict__ sys,
+                                                  SysState* __restrict__ st, Parts P,
+                                                  int with_kinetic)
 {
+    __shared__ double red[4];
     int s = blockIdx.x;
     SysState& T = st[s];
     if (!T.active) return;
     const SysDev& S = sys[s];
     double e[8];
-    e[0] = wave_range_sum(P.lj, S.aw_off, S.aw_n);
-    e[1] = wave_range_sum(P.coul, S.aw_off, S.aw_n);
-    e[2] = wave_range_sum(P.bond, S.bw_off, S.bw_n);
-    e[3] = wave_range_sum(P.angle, S.gw_off, S.gw_n);
-    e[4] = wave_range_sum(P.dih, S.dw_off, S.dw_n);
-    e[5] = wave_range_sum(P.imp, S.iw_off, S.iw_n);
-    e[6] = wave_range_sum(P.pme, S.pw_off, S.pw_n);
-    e[7] = wave_range_sum(P.excl, S.xw_off, S.xw_n);
-    double ke = with_kinetic ? wave_range_sum(P.ke, S.aw_off, S.aw_n) : 0.0;
-    if (lane_id() != 0) return;
+    // pair energies are per atom slot
+    e[0] = block_range_sum(P.lj, S.atom_off, S.n_pad, red);
+    e[1] = block_range_sum(P.coul, S.atom_off, S.n_pad, red);
+    e[2] = block_range_sum(P.bond, S.bw_off, S.bw_n, red);
+    e[3] = block_range_sum(P.angle, S.gw_off, S.gw_n, red);
+    e[4] = block_range_sum(P.dih, S.dw_off, S.dw_n, red);
+    e[5] = block_range_sum(P.imp, S.iw_off, S.iw_n, red);
+    e[6] = block_range_sum(P.pme, S.pw_off, S.pw_n, red);
+    e[7] = block_range_sum(P.excl, S.xw_off, S.xw_n, red);
+    double ke = with_kinetic ? block_range_sum(P.ke, S.aw_off, S.aw_n, red) : 0.0;
+    if (threadIdx.x != 0) return;
     if (S.pme) {
         e[6] += S.self_energy;
         if (S.physics != DMDX_FAITHFUL && S.qsum != 0.0) {
@@ -312,7 +379,7 @@ __global__ void __launch_bounds__(64) k_finalize(const SysDev* __restrict__ sys,
 }
 
 // ---- barostat path: materialise lambda, kinetic energy and virial
-__global__ void k_baro_ke_virial(const SysDev* __restrict__ sys, const SysState* __restrict__ st,
+__device__ __forceinline__ void k_baro_ke_virial_unit(int64_t unit_, const SysDev* __restrict__ sys, const SysState* __restrict__ st,
                                  const int* __restrict__ slot_sys, int64_t nslots,
                                  const double* __restrict__ m, double* __restrict__ vx,
                                  double* __restrict__ vy, double* __restrict__ vz,
@@ -321,7 +388,7 @@ __global__ void k_baro_ke_virial(const SysDev* __restrict__ sys, const SysState*
                                  const double* __restrict__ fy, const double* __restrict__ fz,
                                  double* __restrict__ ke_part, double* __restrict__ w_part)
 {
-    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t g = unit_ * blockDim.x + threadIdx.x;
     if (g >= nslots) return;
     int s = slot_sys[g];
     if (s < 0 || !st[s].active) return;
@@ -345,18 +412,31 @@ __global__ void k_baro_ke_virial(const SysDev* __restrict__ sys, const SysState*
     }
 }
 
-__global__ void __launch_bounds__(64) k_baro_finalize(SysDev* __restrict__ sys,
-                                                      SysState* __restrict__ st,
-                                                      const double* __restrict__ ke_part,
-                                                      const double* __restrict__ w_part)
+__global__ void k_baro_ke_virial(const SysDev* __restrict__ sys, const SysState* __restrict__ st,
+                                 const int* __restrict__ slot_sys, int64_t nslots,
+                                 const double* __restrict__ m, double* __restrict__ vx,
+                                 double* __restrict__ vy, double* __restrict__ vz,
+                                 const double* __restrict__ x, const double* __restrict__ y,
+                                 const double* __restrict__ z, const double* __restrict__ fx,
+                                 const double* __restrict__ fy, const double* __restrict__ fz,
+                                 double* __restrict__ ke_part, double* __restrict__ w_part, int64_t nunits_)
 {
+    DMDX_FOR_UNITS(u_, nunits_) k_baro_ke_virial_unit(u_, sys, st, slot_sys, nslots, m, vx, vy, vz, x, y, z, fx, fy, fz, ke_part, w_part);
+}
+
+__global__ void __launch_bounds__(256) k_baro_finalize(SysDev* __restrict__ sys,
+                                                       SysState* __restrict__ st,
+                                                       const double* __restrict__ ke_part,
+                                                       const double* __restrict__ w_part)
+{
+    __shared__ double red[4];
     int s = blockIdx.x;
     SysState& T = st[s];
     SysDev& S = sys[s];
     if (!T.active || S.barostat == DMDX_BARO_NONE) return;
-    double ke = wave_range_sum(ke_part, S.aw_off, S.aw_n);
-    double w = wave_range_sum(w_part, S.aw_off, S.aw_n);
-    if (lane_id() != 0) return;
+    double ke = block_range_sum(ke_part, S.aw_off, S.aw_n, red);
+    double w = block_range_sum(w_part, S.aw_off, S.aw_n, red);
+    if (threadIdx.x != 0) return;
     T.lambda = 1.0;
     T.ke_out = ke;
     double volume = S.L[0] * S.L[1] * S.L[2];
@@ -388,12 +468,12 @@ __global__ void __launch_bounds__(64) k_baro_finalize(SysDev* __restrict__ sys,
     T.valid = 0;
 }
 
-__global__ void k_scale_positions(const SysDev* __restrict__ sys, const SysState* __restrict__ st,
+__device__ __forceinline__ void k_scale_positions_unit(int64_t unit_, const SysDev* __restrict__ sys, const SysState* __restrict__ st,
                                   const int* __restrict__ slot_sys, int64_t nslots,
                                   double* __restrict__ x, double* __restrict__ y,
                                   double* __restrict__ z)
 {
-    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t g = unit_ * blockDim.x + threadIdx.x;
     if (g >= nslots) return;
     int s = slot_sys[g];
     if (s < 0 || !st[s].active) return;
@@ -405,13 +485,21 @@ __global__ void k_scale_positions(const SysDev* __restrict__ sys, const SysState
     z[g] *= sc;
 }
 
+__global__ void k_scale_positions(const SysDev* __restrict__ sys, const SysState* __restrict__ st,
+                                  const int* __restrict__ slot_sys, int64_t nslots,
+                                  double* __restrict__ x, double* __restrict__ y,
+                                  double* __restrict__ z, int64_t nunits_)
+{
+    DMDX_FOR_UNITS(u_, nunits_) k_scale_positions_unit(u_, sys, st, slot_sys, nslots, x, y, z);
+}
+
 // apply the pending thermostat scale to the stored velocities (state reads)
-__global__ void k_materialize(const SysDev* __restrict__ sys, const SysState* __restrict__ st,
+__device__ __forceinline__ void k_materialize_unit(int64_t unit_, const SysDev* __restrict__ sys, const SysState* __restrict__ st,
                               const int* __restrict__ slot_sys, int64_t nslots,
                               double* __restrict__ vx, double* __restrict__ vy,
                               double* __restrict__ vz)
 {
-    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t g = unit_ * blockDim.x + threadIdx.x;
     if (g >= nslots) return;
     int s = slot_sys[g];
     if (s < 0) return;
@@ -421,6 +509,14 @@ __global__ void k_materialize(const SysDev* __restrict__ sys, const SysState* __
     vx[g] = vx[g] * lam;
     vy[g] = vy[g] * lam;
     vz[g] = vz[g] * lam;
+}
+
+__global__ void k_materialize(const SysDev* __restrict__ sys, const SysState* __restrict__ st,
+                              const int* __restrict__ slot_sys, int64_t nslots,
+                              double* __restrict__ vx, double* __restrict__ vy,
+                              double* __restrict__ vz, int64_t nunits_)
+{
+    DMDX_FOR_UNITS(u_, nunits_) k_materialize_unit(u_, sys, st, slot_sys, nslots, vx, vy, vz);
 }
 
 __global__ void k_lambda_reset(SysState* __restrict__ st, int nsys)
@@ -436,8 +532,8 @@ namespace dmdx {
 hipError_t launch_pre_compute(const Dev& d, hipStream_t s, int phase, int force_rebuild,
                               int reuse_enabled)
 {
-    k_pre_compute<<<blocks_for(d.nsys, 64), 64, 0, s>>>(d.sys, d.st, d.nsys, phase,
-                                                        force_rebuild, reuse_enabled);
+    k_pre_compute<<<d.nsys, 64, 0, s>>>(d.sys, d.st, d.nsys, phase, force_rebuild,
+                                        reuse_enabled, d.any_auto ? d.p_disp : nullptr);
     return hipGetLastError();
 }
 
@@ -452,38 +548,38 @@ hipError_t launch_gather(const Dev& d, hipStream_t s)
     if (!d.any_gather) return hipSuccess;
     const int BS = 256;
     if (d.precision == DMDX_FP64)
-        k_gather<double><<<blocks_for(d.nslots, BS), BS, 0, s>>>(
+        k_gather<double><<<grid_for(blocks_for(d.nslots, BS)), BS, 0, s>>>(
             d.sys, d.st, d.slot_sys, d.nslots, d.c_start, d.c_mid, d.contrib, d.tf, d.x, d.y, d.z,
-            d.q, reinterpret_cast<const Cx2<double>*>(d.grid), d.fx, d.fy, d.fz);
+            d.q, reinterpret_cast<const Cx2<double>*>(d.grid), d.fx, d.fy, d.fz, (int64_t)(blocks_for(d.nslots, BS)));
     else
-        k_gather<float><<<blocks_for(d.nslots, BS), BS, 0, s>>>(
+        k_gather<float><<<grid_for(blocks_for(d.nslots, BS)), BS, 0, s>>>(
             d.sys, d.st, d.slot_sys, d.nslots, d.c_start, d.c_mid, d.contrib, d.tf, d.x, d.y, d.z,
-            d.q, reinterpret_cast<const Cx2<float>*>(d.grid), d.fx, d.fy, d.fz);
+            d.q, reinterpret_cast<const Cx2<float>*>(d.grid), d.fx, d.fy, d.fz, (int64_t)(blocks_for(d.nslots, BS)));
     return hipGetLastError();
 }
 
 hipError_t launch_kick_drift(const Dev& d, hipStream_t s)
 {
     const int BS = 256;
-    k_kick_drift<<<blocks_for(d.nslots, BS), BS, 0, s>>>(d.sys, d.st, d.slot_sys, d.nslots,
+    k_kick_drift<<<grid_for(blocks_for(d.nslots, BS)), BS, 0, s>>>(d.sys, d.st, d.slot_sys, d.nslots,
                                                          d.invm, d.fx, d.fy, d.fz, d.vx, d.vy,
-                                                         d.vz, d.x, d.y, d.z);
+                                                         d.vz, d.x, d.y, d.z, (int64_t)(blocks_for(d.nslots, BS)));
     return hipGetLastError();
 }
 
 hipError_t launch_kick_ke(const Dev& d, hipStream_t s)
 {
     const int BS = 256;
-    k_kick_ke<<<blocks_for(d.nslots, BS), BS, 0, s>>>(d.sys, d.st, d.slot_sys, d.nslots, d.invm,
+    k_kick_ke<<<grid_for(blocks_for(d.nslots, BS)), BS, 0, s>>>(d.sys, d.st, d.slot_sys, d.nslots, d.invm,
                                                       d.m, d.fx, d.fy, d.fz, d.vx, d.vy, d.vz,
-                                                      d.p_ke);
+                                                      d.p_ke, (int64_t)(blocks_for(d.nslots, BS)));
     return hipGetLastError();
 }
 
 hipError_t launch_finalize(const Dev& d, hipStream_t s, int with_kinetic)
 {
     Parts P{d.p_lj, d.p_coul, d.p_bond, d.p_angle, d.p_dih, d.p_imp, d.p_pme, d.p_excl, d.p_ke};
-    k_finalize<<<d.nsys, 64, 0, s>>>(d.sys, d.st, P, with_kinetic);
+    k_finalize<<<d.nsys, 256, 0, s>>>(d.sys, d.st, P, with_kinetic);
     return hipGetLastError();
 }
 
@@ -491,21 +587,21 @@ hipError_t launch_barostat(const Dev& d, hipStream_t s)
 {
     if (!d.any_baro) return hipSuccess;
     const int BS = 256;
-    k_baro_ke_virial<<<blocks_for(d.nslots, BS), BS, 0, s>>>(d.sys, d.st, d.slot_sys, d.nslots,
+    k_baro_ke_virial<<<grid_for(blocks_for(d.nslots, BS)), BS, 0, s>>>(d.sys, d.st, d.slot_sys, d.nslots,
                                                              d.m, d.vx, d.vy, d.vz, d.x, d.y,
                                                              d.z, d.fx, d.fy, d.fz, d.p_ke,
-                                                             d.p_w);
-    k_baro_finalize<<<d.nsys, 64, 0, s>>>(d.sys, d.st, d.p_ke, d.p_w);
-    k_scale_positions<<<blocks_for(d.nslots, BS), BS, 0, s>>>(d.sys, d.st, d.slot_sys, d.nslots,
-                                                              d.x, d.y, d.z);
+                                                             d.p_w, (int64_t)(blocks_for(d.nslots, BS)));
+    k_baro_finalize<<<d.nsys, 256, 0, s>>>(d.sys, d.st, d.p_ke, d.p_w);
+    k_scale_positions<<<grid_for(blocks_for(d.nslots, BS)), BS, 0, s>>>(d.sys, d.st, d.slot_sys, d.nslots,
+                                                              d.x, d.y, d.z, (int64_t)(blocks_for(d.nslots, BS)));
     return hipGetLastError();
 }
 
 hipError_t launch_materialize(const Dev& d, hipStream_t s)
 {
     const int BS = 256;
-    k_materialize<<<blocks_for(d.nslots, BS), BS, 0, s>>>(d.sys, d.st, d.slot_sys, d.nslots,
-                                                          d.vx, d.vy, d.vz);
+    k_materialize<<<grid_for(blocks_for(d.nslots, BS)), BS, 0, s>>>(d.sys, d.st, d.slot_sys, d.nslots,
+                                                          d.vx, d.vy, d.vz, (int64_t)(blocks_for(d.nslots, BS)));
     k_lambda_reset<<<blocks_for(d.nsys, 64), 64, 0, s>>>(d.st, d.nsys);
     return hipGetLastError();
 }

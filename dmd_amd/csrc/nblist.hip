@@ -40,11 +40,11 @@ __device__ __forceinline__ int cell_coord(double x, double L, int nu)
     return c < 0 ? 0 : (c >= nu ? nu - 1 : c);
 }
 
-__global__ void k_cells_reset(const SysDev* __restrict__ sys, const SysState* __restrict__ st,
+__device__ __forceinline__ void k_cells_reset_unit(int64_t unit_, const SysDev* __restrict__ sys, const SysState* __restrict__ st,
                               const int* __restrict__ cell_sys, int ncells_total,
                               int* __restrict__ ccount, int* __restrict__ cfill)
 {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    int c = (int)(unit_ * blockDim.x + threadIdx.x);
     if (c >= ncells_total) return;
     int s = cell_sys[c];
     if (!st[s].rebuild) return;
@@ -52,13 +52,20 @@ __global__ void k_cells_reset(const SysDev* __restrict__ sys, const SysState* __
     cfill[c] = 0;
 }
 
-__global__ void k_cell_count(const SysDev* __restrict__ sys, const SysState* __restrict__ st,
+__global__ void k_cells_reset(const SysDev* __restrict__ sys, const SysState* __restrict__ st,
+                              const int* __restrict__ cell_sys, int ncells_total,
+                              int* __restrict__ ccount, int* __restrict__ cfill, int64_t nunits_)
+{
+    DMDX_FOR_UNITS(u_, nunits_) k_cells_reset_unit(u_, sys, st, cell_sys, ncells_total, ccount, cfill);
+}
+
+__device__ __forceinline__ void k_cell_count_unit(int64_t unit_, const SysDev* __restrict__ sys, const SysState* __restrict__ st,
                              const int* __restrict__ slot_sys, int64_t nslots,
                              const double* __restrict__ x, const double* __restrict__ y,
                              const double* __restrict__ z, int* __restrict__ acell,
                              int* __restrict__ ccount)
 {
-    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t g = unit_ * blockDim.x + threadIdx.x;
     if (g >= nslots) return;
     int s = slot_sys[g];
     if (s < 0 || !st[s].rebuild) return;
@@ -71,6 +78,15 @@ __global__ void k_cell_count(const SysDev* __restrict__ sys, const SysState* __r
     int c = (cz * ny + cy) * nx + cx;
     acell[g] = c;
     atomicAdd(&ccount[S.cell_off + c], 1);
+}
+
+__global__ void k_cell_count(const SysDev* __restrict__ sys, const SysState* __restrict__ st,
+                             const int* __restrict__ slot_sys, int64_t nslots,
+                             const double* __restrict__ x, const double* __restrict__ y,
+                             const double* __restrict__ z, int* __restrict__ acell,
+                             int* __restrict__ ccount, int64_t nunits_)
+{
+    DMDX_FOR_UNITS(u_, nunits_) k_cell_count_unit(u_, sys, st, slot_sys, nslots, x, y, z, acell, ccount);
 }
 
 // one workgroup (256 threads) per system
@@ -102,12 +118,12 @@ __global__ void k_cell_scan(const SysDev* __restrict__ sys, const SysState* __re
     }
 }
 
-__global__ void k_cell_fill(const SysDev* __restrict__ sys, const SysState* __restrict__ st,
+__device__ __forceinline__ void k_cell_fill_unit(int64_t unit_, const SysDev* __restrict__ sys, const SysState* __restrict__ st,
                             const int* __restrict__ slot_sys, int64_t nslots,
                             const int* __restrict__ acell, const int* __restrict__ cstart,
                             int* __restrict__ cfill, int* __restrict__ cell_atoms)
 {
-    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t g = unit_ * blockDim.x + threadIdx.x;
     if (g >= nslots) return;
     int s = slot_sys[g];
     if (s < 0 || !st[s].rebuild) return;
@@ -118,30 +134,157 @@ __global__ void k_cell_fill(const SysDev* __restrict__ sys, const SysState* __re
     cell_atoms[S.atom_off + cstart[S.cell_off + c] + pos] = (int)g;
 }
 
-__global__ void k_cell_sort(const SysDev* __restrict__ sys, const SysState* __restrict__ st,
-                            const int* __restrict__ cell_sys, int ncells_total,
-                            const int* __restrict__ ccount, const int* __restrict__ cstart,
-                            int* __restrict__ cell_atoms)
+__global__ void k_cell_fill(const SysDev* __restrict__ sys, const SysState* __restrict__ st,
+                            const int* __restrict__ slot_sys, int64_t nslots,
+                            const int* __restrict__ acell, const int* __restrict__ cstart,
+                            int* __restrict__ cfill, int* __restrict__ cell_atoms, int64_t nunits_)
 {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    DMDX_FOR_UNITS(u_, nunits_) k_cell_fill_unit(u_, sys, st, slot_sys, nslots, acell, cstart, cfill, cell_atoms);
+}
+
+// one 64-thread workgroup per cell: bitonic sort of the cell's slots in LDS
+// (deterministic order within a cell; cells above the LDS capacity fall back
+// to an insertion sort by one thread)
+constexpr int CELL_SORT_CAP = 1024;
+
+__device__ __forceinline__ void k_cell_sort_unit(int64_t unit_, const SysDev* __restrict__ sys,
+                                                  const SysState* __restrict__ st,
+                                                  const int* __restrict__ cell_sys,
+                                                  int ncells_total,
+                                                  const int* __restrict__ ccount,
+                                                  const int* __restrict__ cstart,
+                                                  int* __restrict__ cell_atoms)
+{
+    const int c = (int)unit_;
     if (c >= ncells_total) return;
-    int s = cell_sys[c];
+    const int s = cell_sys[c];
     if (!st[s].rebuild) return;
     const SysDev& S = sys[s];
-    int lc = c - S.cell_off;
-    int ncell = cells_in_use(S, 0) * cells_in_use(S, 1) * cells_in_use(S, 2);
+    const int lc = c - S.cell_off;
+    const int ncell = cells_in_use(S, 0) * cells_in_use(S, 1) * cells_in_use(S, 2);
     if (lc >= ncell) return;
     int* a = cell_atoms + S.atom_off + cstart[c];
-    int n = ccount[c];
-    for (int i = 1; i < n; ++i) {
-        int v = a[i];
-        int j = i - 1;
-        while (j >= 0 && a[j] > v) {
-            a[j + 1] = a[j];
-            --j;
+    const int n = ccount[c];
+    if (n <= 1) return;
+    if (n > CELL_SORT_CAP) {
+        if (threadIdx.x == 0) {
+            for (int i = 1; i < n; ++i) {
+                int v = a[i];
+                int j = i - 1;
+                while (j >= 0 && a[j] > v) {
+                    a[j + 1] = a[j];
+                    --j;
+                }
+                a[j + 1] = v;
+            }
         }
-        a[j + 1] = v;
+        return;
     }
+    __shared__ int buf[CELL_SORT_CAP];
+    int p = 1;
+    while (p < n) p <<= 1;
+    for (int k = threadIdx.x; k < p; k += 64) buf[k] = (k < n) ? a[k] : 0x7fffffff;
+    __syncthreads();
+    for (int size = 2; size <= p; size <<= 1) {
+        for (int half = size >> 1; half > 0; half >>= 1) {
+            for (int k = threadIdx.x; k < p; k += 64) {
+                int l = k ^ half;
+                if (l > k) {
+                    bool up = (k & size) == 0;
+                    int u = buf[k], v = buf[l];
+                    if ((u > v) == up) {
+                        buf[k] = v;
+                        buf[l] = u;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int k = threadIdx.x; k < n; k += 64) a[k] = buf[k];
+}
+
+__global__ void __launch_bounds__(64) k_cell_sort(const SysDev* __restrict__ sys,
+                                                  const SysState* __restrict__ st,
+                                                  const int* __restrict__ cell_sys,
+                                                  int ncells_total,
+                                                  const int* __restrict__ ccount,
+                                                  const int* __restrict__ cstart,
+                                                  int* __restrict__ cell_atoms, int64_t nunits_)
+{
+    DMDX_FOR_UNITS(u_, nunits_)
+    {
+        k_cell_sort_unit(u_, sys, st, cell_sys, ncells_total, ccount, cstart, cell_atoms);
+        __syncthreads();  // LDS is reused by the next unit
+    }
+}
+
+// after the cell sort: positions in cell order (contiguous per cell, read by
+// the scan below) and the reference positions of the displacement criterion
+__device__ __forceinline__ void k_pack_build_unit(int64_t unit_, const SysDev* __restrict__ sys, const SysState* __restrict__ st,
+                             const int* __restrict__ slot_sys, int64_t nslots,
+                             const int* __restrict__ cell_atoms, const double* __restrict__ x,
+                             const double* __restrict__ y, const double* __restrict__ z,
+                             const double* __restrict__ q, double4* __restrict__ pos4s,
+                             double4* __restrict__ xref, float4* __restrict__ pos4f)
+{
+    int64_t t = unit_ * blockDim.x + threadIdx.x;
+    if (t >= nslots) return;
+    int s = slot_sys[t];
+    if (s < 0 || !st[s].rebuild) return;
+    const SysDev& S = sys[s];
+    if (t - S.atom_off >= S.n) return;
+    int i = cell_atoms[t];
+    pos4s[t] = make_double4(x[i], y[i], z[i], q[i]);
+    xref[i] = make_double4(x[i], y[i], z[i], 0.0);
+    if (pos4f) {
+        double wx = x[i] - floor(x[i] * S.invL[0]) * S.L[0];
+        double wy = y[i] - floor(y[i] * S.invL[1]) * S.L[1];
+        double wz = z[i] - floor(z[i] * S.invL[2]) * S.L[2];
+        pos4f[t] = make_float4((float)wx, (float)wy, (float)wz, 0.0f);
+    }
+}
+
+__global__ void k_pack_build(const SysDev* __restrict__ sys, const SysState* __restrict__ st,
+                             const int* __restrict__ slot_sys, int64_t nslots,
+                             const int* __restrict__ cell_atoms, const double* __restrict__ x,
+                             const double* __restrict__ y, const double* __restrict__ z,
+                             const double* __restrict__ q, double4* __restrict__ pos4s,
+                             double4* __restrict__ xref, float4* __restrict__ pos4f,
+                             int64_t nunits_)
+{
+    DMDX_FOR_UNITS(u_, nunits_) k_pack_build_unit(u_, sys, st, slot_sys, nslots, cell_atoms, x, y, z, q, pos4s, xref, pos4f);
+}
+
+// per atom: squared displacement since the last build, per-wave maximum
+__device__ __forceinline__ void k_displacement_unit(int64_t unit_, const SysDev* __restrict__ sys, const SysState* __restrict__ st,
+                               const int* __restrict__ slot_sys, int64_t nslots,
+                               const double* __restrict__ x, const double* __restrict__ y,
+                               const double* __restrict__ z, const double4* __restrict__ xref,
+                               double* __restrict__ p_disp)
+{
+    int64_t g = unit_ * blockDim.x + threadIdx.x;
+    if (g >= nslots) return;
+    int s = slot_sys[g];
+    if (s < 0) return;
+    const SysDev& S = sys[s];
+    double d2 = 0.0;
+    if (S.interval == 0 && st[s].active && st[s].built && g - S.atom_off < S.n) {
+        double4 r = xref[g];
+        double dx = x[g] - r.x, dy = y[g] - r.y, dz = z[g] - r.z;
+        d2 = dx * dx + dy * dy + dz * dz;
+    }
+    for (int o = 32; o > 0; o >>= 1) d2 = fmax(d2, __shfl_xor(d2, o, 64));
+    if ((threadIdx.x & 63) == 0) p_disp[g >> 6] = d2;
+}
+
+__global__ void k_displacement(const SysDev* __restrict__ sys, const SysState* __restrict__ st,
+                               const int* __restrict__ slot_sys, int64_t nslots,
+                               const double* __restrict__ x, const double* __restrict__ y,
+                               const double* __restrict__ z, const double4* __restrict__ xref,
+                               double* __restrict__ p_disp, int64_t nunits_)
+{
+    DMDX_FOR_UNITS(u_, nunits_) k_displacement_unit(u_, sys, st, slot_sys, nslots, x, y, z, xref, p_disp);
 }
 
 // neighbour-cell offsets along one edge: {-1,0,1} with >= 3 cells, {0,1}
@@ -163,44 +306,69 @@ __device__ __forceinline__ bool is_excluded(const int* __restrict__ ex_start,
     return false;
 }
 
-// thread t (slot order) handles the atom cell_atoms[t]; its list lives in
-// column (t - atom_off) of the system's M x n_pad block.
-__global__ void k_nbr_build(const SysDev* __restrict__ sys, SysState* __restrict__ st,
-                            const int* __restrict__ slot_sys, int64_t nslots,
-                            const double* __restrict__ x, const double* __restrict__ y,
-                            const double* __restrict__ z, const int* __restrict__ acell,
-                            const int* __restrict__ ccount, const int* __restrict__ cstart,
-                            const int* __restrict__ cell_atoms, const int* __restrict__ ex_start,
-                            const int* __restrict__ ex_list, int* __restrict__ order,
-                            int* __restrict__ nn, int* __restrict__ nbr)
+// A group of TPA lanes handles the atom of slot t (cell order): the lanes
+// test TPA candidates of a neighbour cell at a time, compact the hits with a
+// wavefront ballot and append them to the atom's row of the list, so a row
+// is written with contiguous stores.  The list is row-major: the neighbours of
+// slot t occupy nbr[nbr_off + (t - atom_off) * M + k].
+//   FP64 path:  entries are atom slots (summation order is restored by
+//               k_nbr_sort);
+//   mixed path: entries are cell-order slots, i.e. indices into the packed
+//               cell-ordered positions (k_pack), which keeps the gathers of
+//               the pair kernel spatially local.
+template <int TPA>
+__device__ __forceinline__ void k_nbr_build_unit(int64_t unit_, 
+    const SysDev* __restrict__ sys, SysState* __restrict__ st, const int* __restrict__ slot_sys,
+    int64_t nslots, const double* __restrict__ x, const double* __restrict__ y,
+    const double* __restrict__ z, const int* __restrict__ acell, const int* __restrict__ ccount,
+    const int* __restrict__ cstart, const int* __restrict__ cell_atoms,
+    const int* __restrict__ ex_start, const int* __restrict__ ex_list, int* __restrict__ order,
+    int* __restrict__ nn, int* __restrict__ nbr, const double4* __restrict__ pos4s,
+    const float4* __restrict__ pos4f)
 {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    constexpr int GPB = 256 / TPA;
+    const int lane = threadIdx.x % TPA;
+    const int64_t t = unit_ * GPB + threadIdx.x / TPA;
     if (t >= nslots) return;
-    int s = slot_sys[t];
+    const int s = slot_sys[t];
     if (s < 0 || !st[s].rebuild) return;
     const SysDev& S = sys[s];
-    int lt = (int)(t - S.atom_off);
+    const int lt = (int)(t - S.atom_off);
     if (lt >= S.n) {
-        nn[t] = 0;
-        order[t] = -1;
+        if (lane == 0) {
+            nn[t] = 0;
+            order[t] = -1;
+        }
         return;
     }
-    int i = cell_atoms[t];
-    order[t] = i;
-    int nx = cells_in_use(S, 0), ny = cells_in_use(S, 1), nz = cells_in_use(S, 2);
-    int ci = acell[i];
-    int cx = ci % nx, cy = (ci / nx) % ny, cz = ci / (nx * ny);
+    const int i = cell_atoms[t];
+    if (lane == 0) order[t] = i;
+    const int nx = cells_in_use(S, 0), ny = cells_in_use(S, 1), nz = cells_in_use(S, 2);
+    const int ci = acell[i];
+    const int cx = ci % nx, cy = (ci / nx) % ny, cz = ci / (nx * ny);
     int ox[3], oy[3], oz[3];
-    int nox = edge_offsets(nx, ox), noy = edge_offsets(ny, oy), noz = edge_offsets(nz, oz);
-    const double xi = x[i], yi = y[i], zi = z[i];
+    const int nox = edge_offsets(nx, ox), noy = edge_offsets(ny, oy), noz = edge_offsets(nz, oz);
+    const double4 pi4 = pos4s[t];
+    const double xi = pi4.x, yi = pi4.y, zi = pi4.z;
+    // mixed path: FP32 test on wrapped positions (error < 2e-6 nm here) and
+    // the exact FP64 decision only within 1e-5 nm of the list radius, so the
+    // pair set is exactly the FP64 one
+    const bool fast = pos4f != nullptr;
+    const float4 pif = fast ? pos4f[t] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float Lxf = (float)S.L[0], Lyf = (float)S.L[1], Lzf = (float)S.L[2];
+    const float ixf = (float)S.invL[0], iyf = (float)S.invL[1], izf = (float)S.invL[2];
+    const float lo2 = (float)((S.rlist - 1e-5) * (S.rlist - 1e-5));
+    const float hi2 = (float)((S.rlist + 1e-5) * (S.rlist + 1e-5));
     const double Lx = S.L[0], Ly = S.L[1], Lz = S.L[2];
     const double ix = S.invL[0], iy = S.invL[1], iz = S.invL[2];
     const double rl2 = S.rlist2;
     const bool check_ex = ex_start[i + 1] > ex_start[i];
-    int* col = nbr + S.nbr_off + lt;
-    const int64_t stride = S.n_pad;
-    int k = 0;
+    const bool sorted_ids = S.precision != DMDX_FP64;
+    int* row = nbr + S.nbr_off + (int64_t)lt * S.max_nbr;
     const int M = S.max_nbr;
+    const int gshift = (threadIdx.x & 63) / TPA * TPA;  // first lane of my group in the wave
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    int k = 0;
     for (int a = 0; a < noz; ++a) {
         int zz = cz + oz[a];
         zz = zz < 0 ? zz + nz : (zz >= nz ? zz - nz : zz);
@@ -210,44 +378,89 @@ __global__ void k_nbr_build(const SysDev* __restrict__ sys, SysState* __restrict
             for (int c = 0; c < nox; ++c) {
                 int xx = cx + ox[c];
                 xx = xx < 0 ? xx + nx : (xx >= nx ? xx - nx : xx);
-                int cell = S.cell_off + (zz * ny + yy) * nx + xx;
-                const int* ca = cell_atoms + S.atom_off + cstart[cell];
-                int cnt = ccount[cell];
-                for (int m = 0; m < cnt; ++m) {
-                    int j = ca[m];
-                    if (j == i) continue;
-                    double dx = x[j] - xi;
-                    double dy = y[j] - yi;
-                    double dz = z[j] - zi;
-                    dx = min_image_dmd(dx, Lx, ix);
-                    dy = min_image_dmd(dy, Ly, iy);
-                    dz = min_image_dmd(dz, Lz, iz);
-                    double r2 = dx * dx + dy * dy + dz * dz;
-                    if (!(r2 < rl2)) continue;
-                    if (check_ex && is_excluded(ex_start, ex_list, i, j)) continue;
-                    if (k < M) col[(int64_t)k * stride] = j;
-                    ++k;
+                const int cell = S.cell_off + (zz * ny + yy) * nx + xx;
+                const int64_t base = S.atom_off + cstart[cell];
+                const int cnt = ccount[cell];
+                for (int m0 = 0; m0 < cnt; m0 += TPA) {
+                    const int m = m0 + lane;
+                    bool hit = false;
+                    int j = -1;
+                    if (m < cnt && base + m != t) {
+                        bool exact = true;
+                        if (fast) {
+                            const float4 pf = pos4f[base + m];
+                            float fx = pf.x - pif.x, fy = pf.y - pif.y, fz = pf.z - pif.z;
+                            fx -= rintf(fx * ixf) * Lxf;
+                            fy -= rintf(fy * iyf) * Lyf;
+                            fz -= rintf(fz * izf) * Lzf;
+                            const float r2f = fx * fx + fy * fy + fz * fz;
+                            if (r2f < lo2) {
+                                hit = true;
+                                exact = false;
+                            } else if (r2f >= hi2) {
+                                exact = false;
+                            }
+                        }
+                        if (exact) {
+                            const double4 pj = pos4s[base + m];
+                            double dx = pj.x - xi;
+                            double dy = pj.y - yi;
+                            double dz = pj.z - zi;
+                            dx = min_image_dmd(dx, Lx, ix);
+                            dy = min_image_dmd(dy, Ly, iy);
+                            dz = min_image_dmd(dz, Lz, iz);
+                            double r2 = dx * dx + dy * dy + dz * dz;
+                            hit = r2 < rl2;
+                        }
+                        if (hit && (check_ex || !sorted_ids)) j = cell_atoms[base + m];
+                        if (hit && check_ex) hit = !is_excluded(ex_start, ex_list, i, j);
+                    }
+                    if (TPA == 1) {
+                        if (hit) {
+                            if (k < M) row[k] = sorted_ids ? (int)(base + m) : j;
+                            ++k;
+                        }
+                    } else {
+                        const uint64_t bal = __ballot(hit);
+                        const uint64_t mine = (bal >> gshift) & (TPA == 64 ? ~0ull : ((1ull << TPA) - 1));
+                        const int pos = k + __popcll(mine & below);
+                        if (hit && pos < M) row[pos] = sorted_ids ? (int)(base + m) : j;
+                        k += __popcll(mine);
+                    }
                 }
             }
         }
     }
     if (k > M) {
-        atomicOr(&st[s].flags, FLAG_NBR_OVERFLOW);
+        if (lane == 0) atomicOr(&st[s].flags, FLAG_NBR_OVERFLOW);
         k = M;
     }
-    nn[t] = k;
+    if (lane == 0) nn[t] = k;
 }
 
-// one 64-thread workgroup per atom slot: bitonic sort of its column in LDS
+template <int TPA>
+__global__ void __launch_bounds__(256) k_nbr_build(
+    const SysDev* __restrict__ sys, SysState* __restrict__ st, const int* __restrict__ slot_sys,
+    int64_t nslots, const double* __restrict__ x, const double* __restrict__ y,
+    const double* __restrict__ z, const int* __restrict__ acell, const int* __restrict__ ccount,
+    const int* __restrict__ cstart, const int* __restrict__ cell_atoms,
+    const int* __restrict__ ex_start, const int* __restrict__ ex_list, int* __restrict__ order,
+    int* __restrict__ nn, int* __restrict__ nbr, const double4* __restrict__ pos4s,
+    const float4* __restrict__ pos4f, int64_t nunits_)
+{
+    DMDX_FOR_UNITS_SPREAD(u_, nunits_) k_nbr_build_unit<TPA>(u_, sys, st, slot_sys, nslots, x, y, z, acell, ccount, cstart, cell_atoms, ex_start, ex_list, order, nn, nbr, pos4s, pos4f);
+}
+
+// one 64-thread workgroup per atom slot: bitonic sort of its row in LDS
 constexpr int SORT_CAP = 2048;
 
-__global__ void __launch_bounds__(64) k_nbr_sort(const SysDev* __restrict__ sys,
+__device__ __forceinline__ void k_nbr_sort_unit(int64_t unit_, const SysDev* __restrict__ sys,
                                                  const SysState* __restrict__ st,
                                                  const int* __restrict__ slot_sys,
                                                  const int* __restrict__ nn,
                                                  int* __restrict__ nbr)
 {
-    int64_t t = blockIdx.x;
+    int64_t t = unit_;
     int s = slot_sys[t];
     if (s < 0 || !st[s].rebuild) return;
     const SysDev& S = sys[s];
@@ -257,12 +470,11 @@ __global__ void __launch_bounds__(64) k_nbr_sort(const SysDev* __restrict__ sys,
     int n = nn[t];
     if (n <= 1) return;
     __shared__ int buf[SORT_CAP];
-    int* col = nbr + S.nbr_off + lt;
-    const int64_t stride = S.n_pad;
+    int* row = nbr + S.nbr_off + (int64_t)lt * S.max_nbr;
     int p = 1;
     while (p < n) p <<= 1;
     if (p > SORT_CAP) return;  // capacity is checked on the host (max_nbr <= SORT_CAP)
-    for (int k = threadIdx.x; k < p; k += 64) buf[k] = (k < n) ? col[(int64_t)k * stride] : 0x7fffffff;
+    for (int k = threadIdx.x; k < p; k += 64) buf[k] = (k < n) ? row[k] : 0x7fffffff;
     __syncthreads();
     for (int size = 2; size <= p; size <<= 1) {
         for (int stride2 = size >> 1; stride2 > 0; stride2 >>= 1) {
@@ -280,7 +492,49 @@ __global__ void __launch_bounds__(64) k_nbr_sort(const SysDev* __restrict__ sys,
             __syncthreads();
         }
     }
-    for (int k = threadIdx.x; k < n; k += 64) col[(int64_t)k * stride] = buf[k];
+    for (int k = threadIdx.x; k < n; k += 64) row[k] = buf[k];
+}
+
+__global__ void __launch_bounds__(64) k_nbr_sort(const SysDev* __restrict__ sys,
+                                                 const SysState* __restrict__ st,
+                                                 const int* __restrict__ slot_sys,
+                                                 const int* __restrict__ nn,
+                                                 int* __restrict__ nbr, int64_t nunits_)
+{
+    DMDX_FOR_UNITS(u_, nunits_)
+    {
+        k_nbr_sort_unit(u_, sys, st, slot_sys, nn, nbr);
+        __syncthreads();  // LDS is reused by the next unit
+    }
+}
+
+// mixed path, every compute: positions and charges in cell order
+__device__ __forceinline__ void k_pack_unit(int64_t unit_, const SysDev* __restrict__ sys, const SysState* __restrict__ st,
+                       const int* __restrict__ slot_sys, int64_t nslots,
+                       const int* __restrict__ order, const double* __restrict__ x,
+                       const double* __restrict__ y, const double* __restrict__ z,
+                       const double* __restrict__ q, const int* __restrict__ type,
+                       double4* __restrict__ pos4s, int* __restrict__ typ_s)
+{
+    int64_t t = unit_ * blockDim.x + threadIdx.x;
+    if (t >= nslots) return;
+    int s = slot_sys[t];
+    if (s < 0 || st[s].reuse || !st[s].active) return;
+    const SysDev& S = sys[s];
+    if (t - S.atom_off >= S.n) return;
+    int i = order[t];
+    pos4s[t] = make_double4(x[i], y[i], z[i], q[i]);
+    typ_s[t] = type[i];
+}
+
+__global__ void k_pack(const SysDev* __restrict__ sys, const SysState* __restrict__ st,
+                       const int* __restrict__ slot_sys, int64_t nslots,
+                       const int* __restrict__ order, const double* __restrict__ x,
+                       const double* __restrict__ y, const double* __restrict__ z,
+                       const double* __restrict__ q, const int* __restrict__ type,
+                       double4* __restrict__ pos4s, int* __restrict__ typ_s, int64_t nunits_)
+{
+    DMDX_FOR_UNITS(u_, nunits_) k_pack_unit(u_, sys, st, slot_sys, nslots, order, x, y, z, q, type, pos4s, typ_s);
 }
 
 }  // namespace dmdx
@@ -291,22 +545,55 @@ hipError_t launch_build_lists(const Dev& d, hipStream_t s)
 {
     const int BS = 256;
     if (d.ncells > 0)
-        k_cells_reset<<<blocks_for(d.ncells, BS), BS, 0, s>>>(d.sys, d.st, d.cell_sys, d.ncells,
-                                                              d.ccount, d.cfill);
-    k_cell_count<<<blocks_for(d.nslots, BS), BS, 0, s>>>(d.sys, d.st, d.slot_sys, d.nslots, d.x,
-                                                         d.y, d.z, d.acell, d.ccount);
+        k_cells_reset<<<grid_for(blocks_for(d.ncells, BS)), BS, 0, s>>>(d.sys, d.st, d.cell_sys, d.ncells,
+                                                              d.ccount, d.cfill, (int64_t)(blocks_for(d.ncells, BS)));
+    k_cell_count<<<grid_for(blocks_for(d.nslots, BS)), BS, 0, s>>>(d.sys, d.st, d.slot_sys, d.nslots, d.x,
+                                                         d.y, d.z, d.acell, d.ccount, (int64_t)(blocks_for(d.nslots, BS)));
     k_cell_scan<<<d.nsys, 256, 0, s>>>(d.sys, d.st, d.ccount, d.cstart);
-    k_cell_fill<<<blocks_for(d.nslots, BS), BS, 0, s>>>(d.sys, d.st, d.slot_sys, d.nslots,
+    k_cell_fill<<<grid_for(blocks_for(d.nslots, BS)), BS, 0, s>>>(d.sys, d.st, d.slot_sys, d.nslots,
                                                         d.acell, d.cstart, d.cfill,
-                                                        d.cell_atoms);
+                                                        d.cell_atoms, (int64_t)(blocks_for(d.nslots, BS)));
     if (d.ncells > 0)
-        k_cell_sort<<<blocks_for(d.ncells, BS), BS, 0, s>>>(d.sys, d.st, d.cell_sys, d.ncells,
-                                                            d.ccount, d.cstart, d.cell_atoms);
-    k_nbr_build<<<blocks_for(d.nslots, BS), BS, 0, s>>>(
-        d.sys, d.st, d.slot_sys, d.nslots, d.x, d.y, d.z, d.acell, d.ccount, d.cstart,
-        d.cell_atoms, d.ex_start, d.ex_list, d.order, d.nn, d.nbr);
+        k_cell_sort<<<grid_for((unsigned)d.ncells), 64, 0, s>>>(d.sys, d.st, d.cell_sys, d.ncells, d.ccount,
+                                                      d.cstart, d.cell_atoms, (int64_t)((unsigned)d.ncells));
+    k_pack_build<<<grid_for(blocks_for(d.nslots, BS)), BS, 0, s>>>(d.sys, d.st, d.slot_sys, d.nslots,
+                                                         d.cell_atoms, d.x, d.y, d.z, d.q,
+                                                         d.pos4s, d.xref, d.pos4f, (int64_t)(blocks_for(d.nslots, BS)));
     if (d.precision == DMDX_FP64)
-        k_nbr_sort<<<(unsigned)d.nslots, 64, 0, s>>>(d.sys, d.st, d.slot_sys, d.nn, d.nbr);
+        k_nbr_build<1><<<grid_for(blocks_for(d.nslots, BS)), BS, 0, s>>>(
+            d.sys, d.st, d.slot_sys, d.nslots, d.x, d.y, d.z, d.acell, d.ccount, d.cstart,
+            d.cell_atoms, d.ex_start, d.ex_list, d.order, d.nn, d.nbr, d.pos4s, nullptr, (int64_t)(blocks_for(d.nslots, BS)));
+    else if (d.build_tpa == 8)
+        k_nbr_build<8><<<grid_for(blocks_for(d.nslots * 8, BS)), BS, 0, s>>>(
+            d.sys, d.st, d.slot_sys, d.nslots, d.x, d.y, d.z, d.acell, d.ccount, d.cstart,
+            d.cell_atoms, d.ex_start, d.ex_list, d.order, d.nn, d.nbr, d.pos4s, d.pos4f, (int64_t)(blocks_for(d.nslots * 8, BS)));
+    else if (d.build_tpa == 32)
+        k_nbr_build<32><<<grid_for(blocks_for(d.nslots * 32, BS)), BS, 0, s>>>(
+            d.sys, d.st, d.slot_sys, d.nslots, d.x, d.y, d.z, d.acell, d.ccount, d.cstart,
+            d.cell_atoms, d.ex_start, d.ex_list, d.order, d.nn, d.nbr, d.pos4s, d.pos4f, (int64_t)(blocks_for(d.nslots * 32, BS)));
+    else
+        k_nbr_build<16><<<grid_for(blocks_for(d.nslots * 16, BS)), BS, 0, s>>>(
+            d.sys, d.st, d.slot_sys, d.nslots, d.x, d.y, d.z, d.acell, d.ccount, d.cstart,
+            d.cell_atoms, d.ex_start, d.ex_list, d.order, d.nn, d.nbr, d.pos4s, d.pos4f, (int64_t)(blocks_for(d.nslots * 16, BS)));
+    if (d.precision == DMDX_FP64)
+        k_nbr_sort<<<grid_for((unsigned)d.nslots), 64, 0, s>>>(d.sys, d.st, d.slot_sys, d.nn, d.nbr, (int64_t)((unsigned)d.nslots));
+    return hipGetLastError();
+}
+
+hipError_t launch_displacement(const Dev& d, hipStream_t s)
+{
+    if (!d.any_auto) return hipSuccess;
+    k_displacement<<<grid_for(blocks_for(d.nslots, 256)), 256, 0, s>>>(d.sys, d.st, d.slot_sys, d.nslots,
+                                                             d.x, d.y, d.z, d.xref, d.p_disp, (int64_t)(blocks_for(d.nslots, 256)));
+    return hipGetLastError();
+}
+
+hipError_t launch_pack(const Dev& d, hipStream_t s)
+{
+    if (d.precision == DMDX_FP64) return hipSuccess;
+    k_pack<<<grid_for(blocks_for(d.nslots, 256)), 256, 0, s>>>(d.sys, d.st, d.slot_sys, d.nslots, d.order,
+                                                     d.x, d.y, d.z, d.q, d.type, d.pos4s,
+                                                     d.typ_s, (int64_t)(blocks_for(d.nslots, 256)));
     return hipGetLastError();
 }
 

@@ -60,14 +60,21 @@ __device__ __forceinline__ R dtheta4(R u)
 }
 
 template <class R>
-__global__ void k_pme_zero(const SysDev* __restrict__ sys, const SysState* __restrict__ st,
+__device__ __forceinline__ void k_pme_zero_unit(int64_t unit_, const SysDev* __restrict__ sys, const SysState* __restrict__ st,
                            const int* __restrict__ gsys, int64_t ntot, Cx<R>* __restrict__ grid)
 {
-    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t e = unit_ * blockDim.x + threadIdx.x;
     if (e >= ntot) return;
     int s = gsys[e];
     if (s < 0 || st[s].reuse || !st[s].active) return;
     grid[e] = {R(0), R(0)};
+}
+
+template <class R>
+__global__ void k_pme_zero(const SysDev* __restrict__ sys, const SysState* __restrict__ st,
+                           const int* __restrict__ gsys, int64_t ntot, Cx<R>* __restrict__ grid, int64_t nunits_)
+{
+    DMDX_FOR_UNITS(u_, nunits_) k_pme_zero_unit<R>(u_, sys, st, gsys, ntot, grid);
 }
 
 __device__ __forceinline__ void atomic_add_real(double* p, double v) { unsafeAtomicAdd(p, v); }
@@ -76,13 +83,13 @@ __device__ __forceinline__ void atomic_add_real(float* p, float v) { unsafeAtomi
 // per atom: fractional coordinate, first grid index and offset, in FP64 (the
 // reference's expressions), then the weights in R
 template <class R>
-__global__ void k_pme_spread(const SysDev* __restrict__ sys, const SysState* __restrict__ st,
+__device__ __forceinline__ void k_pme_spread_unit(int64_t unit_, const SysDev* __restrict__ sys, const SysState* __restrict__ st,
                              const int* __restrict__ slot_sys, int64_t nslots,
                              const double* __restrict__ x, const double* __restrict__ y,
                              const double* __restrict__ z, const double* __restrict__ q,
                              Cx<R>* __restrict__ grid)
 {
-    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t g = unit_ * blockDim.x + threadIdx.x;
     if (g >= nslots) return;
     int s = slot_sys[g];
     if (s < 0 || st[s].reuse || !st[s].active) return;
@@ -121,6 +128,16 @@ __global__ void k_pme_spread(const SysDev* __restrict__ sys, const SysState* __r
             }
         }
     }
+}
+
+template <class R>
+__global__ void k_pme_spread(const SysDev* __restrict__ sys, const SysState* __restrict__ st,
+                             const int* __restrict__ slot_sys, int64_t nslots,
+                             const double* __restrict__ x, const double* __restrict__ y,
+                             const double* __restrict__ z, const double* __restrict__ q,
+                             Cx<R>* __restrict__ grid, int64_t nunits_)
+{
+    DMDX_FOR_UNITS(u_, nunits_) k_pme_spread_unit<R>(u_, sys, st, slot_sys, nslots, x, y, z, q, grid);
 }
 
 // R-point DFT of v[0..R) in place, with w^m = tw[(m * (K / R)) % K]
@@ -201,14 +218,14 @@ __device__ Cx<R>* stockham_lds(Cx<R>* a, Cx<R>* b, int nl, const FftPlan& P,
 // forward transform multiply by G(m), accumulate the energy and run the
 // inverse z transform in place.
 template <class R>
-__global__ void k_fft_pass(const FftTile* __restrict__ tiles, const SysDev* __restrict__ sys,
+__device__ __forceinline__ void k_fft_pass_unit(int64_t unit_, const FftTile* __restrict__ tiles, const SysDev* __restrict__ sys,
                            const SysState* __restrict__ st, const FftPlan* __restrict__ plans,
                            Cx<R>* __restrict__ grid, const Cx<R>* __restrict__ twiddles,
                            const double* __restrict__ bmod, int dim, int sign, int conv,
                            double* __restrict__ epart)
 {
     extern __shared__ unsigned char smem_raw[];
-    const FftTile T = tiles[blockIdx.x];
+    const FftTile T = tiles[unit_];
     const int s = T.sys;
     if (st[s].reuse || !st[s].active) return;
     const SysDev& S = sys[s];
@@ -297,22 +314,20 @@ __global__ void k_fft_pass(const FftTile* __restrict__ tiles, const SysDev* __re
     }
 }
 
-template __global__ void k_pme_zero<double>(const SysDev*, const SysState*, const int*, int64_t,
-                                            Cx<double>*);
-template __global__ void k_pme_zero<float>(const SysDev*, const SysState*, const int*, int64_t,
-                                           Cx<float>*);
-template __global__ void k_pme_spread<double>(const SysDev*, const SysState*, const int*, int64_t,
-                                              const double*, const double*, const double*,
-                                              const double*, Cx<double>*);
-template __global__ void k_pme_spread<float>(const SysDev*, const SysState*, const int*, int64_t,
-                                             const double*, const double*, const double*,
-                                             const double*, Cx<float>*);
-template __global__ void k_fft_pass<double>(const FftTile*, const SysDev*, const SysState*,
-                                            const FftPlan*, Cx<double>*, const Cx<double>*,
-                                            const double*, int, int, int, double*);
-template __global__ void k_fft_pass<float>(const FftTile*, const SysDev*, const SysState*,
-                                           const FftPlan*, Cx<float>*, const Cx<float>*,
-                                           const double*, int, int, int, double*);
+template <class R>
+__global__ void k_fft_pass(const FftTile* __restrict__ tiles, const SysDev* __restrict__ sys,
+                           const SysState* __restrict__ st, const FftPlan* __restrict__ plans,
+                           Cx<R>* __restrict__ grid, const Cx<R>* __restrict__ twiddles,
+                           const double* __restrict__ bmod, int dim, int sign, int conv,
+                           double* __restrict__ epart, int64_t nunits_)
+{
+    DMDX_FOR_UNITS(u_, nunits_)
+    {
+        k_fft_pass_unit<R>(u_, tiles, sys, st, plans, grid, twiddles, bmod, dim, sign, conv, epart);
+        __syncthreads();  // LDS is reused by the next unit
+    }
+}
+
 
 }  // namespace dmdx
 
@@ -324,19 +339,19 @@ static hipError_t launch_pme_t(const Dev& d, hipStream_t s)
     const int BS = 256;
     Cx<R>* grid = reinterpret_cast<Cx<R>*>(d.grid);
     const Cx<R>* tw = reinterpret_cast<const Cx<R>*>(d.tw);
-    k_pme_zero<R><<<blocks_for(d.grid_total, BS), BS, 0, s>>>(d.sys, d.st, d.grid_sys,
-                                                              d.grid_total, grid);
-    k_pme_spread<R><<<blocks_for(d.nslots, BS), BS, 0, s>>>(d.sys, d.st, d.slot_sys, d.nslots,
-                                                            d.x, d.y, d.z, d.q, grid);
+    k_pme_zero<R><<<grid_for(blocks_for(d.grid_total, BS)), BS, 0, s>>>(d.sys, d.st, d.grid_sys,
+                                                              d.grid_total, grid, (int64_t)(blocks_for(d.grid_total, BS)));
+    k_pme_spread<R><<<grid_for(blocks_for(d.nslots, BS)), BS, 0, s>>>(d.sys, d.st, d.slot_sys, d.nslots,
+                                                            d.x, d.y, d.z, d.q, grid, (int64_t)(blocks_for(d.nslots, BS)));
     // forward x, forward y, forward z + convolution + inverse z, inverse y, inverse x
     const int dims[5] = {0, 1, 2, 1, 0};
     const int signs[5] = {-1, -1, -1, +1, +1};
     for (int p = 0; p < 5; ++p) {
         int dim = dims[p];
         if (!d.ntiles[dim]) continue;
-        k_fft_pass<R><<<d.ntiles[dim], 256, d.lds_bytes[dim], s>>>(
+        k_fft_pass<R><<<grid_for(d.ntiles[dim]), 256, d.lds_bytes[dim], s>>>(
             d.tiles[dim], d.sys, d.st, d.plans, grid, tw, d.bmod, dim, signs[p], p == 2 ? 1 : 0,
-            d.p_pme);
+            d.p_pme, (int64_t)(d.ntiles[dim]));
     }
     return hipGetLastError();
 }

@@ -20,6 +20,7 @@ class Scalars(C.Structure):
         ("kinetic_energy", C.c_double), ("zeta", C.c_double), ("piston_velocity", C.c_double),
         ("virial", C.c_double), ("pressure", C.c_double), ("box", C.c_double * 3),
         ("step_since_rebuild", C.c_int32), ("flags", C.c_int32), ("steps_done", C.c_int64),
+        ("list_builds", C.c_int64),
     ]
 
 
@@ -49,6 +50,7 @@ def _setup(lib):
     lib.dmdx_engine_kernel_time.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_double),
                                             C.POINTER(C.c_int64)]
     lib.dmdx_copy_bandwidth.argtypes = [C.c_int64, C.c_int32, C.POINTER(C.c_double)]
+    lib.dmdx_engine_set_tuning.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
     lib._engine_ready = True
 
 
@@ -143,6 +145,7 @@ class Engine:
             "ke": s.kinetic_energy, "zeta": s.zeta, "eps_dot": s.piston_velocity,
             "virial": s.virial, "pressure": s.pressure, "box": tuple(s.box),
             "counter": s.step_since_rebuild, "flags": s.flags, "steps_done": s.steps_done,
+            "list_builds": s.list_builds,
         }
 
     def set_step_since_rebuild(self, counter: int, sys: int = 0):
@@ -150,6 +153,12 @@ class Engine:
 
     def set_active(self, sys: int, active: bool):
         _abi.check(self.lib.dmdx_engine_set_active(self.h, sys, int(active)))
+
+    def set_tuning(self, pair_lanes: int | None = None, build_lanes: int | None = None):
+        if pair_lanes is not None:
+            _abi.check(self.lib.dmdx_engine_set_tuning(self.h, 0, int(pair_lanes)))
+        if build_lanes is not None:
+            _abi.check(self.lib.dmdx_engine_set_tuning(self.h, 1, int(build_lanes)))
 
     def set_profiling(self, on: bool):
         _abi.check(self.lib.dmdx_engine_set_profiling(self.h, int(on)))

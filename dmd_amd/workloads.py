@@ -8,6 +8,10 @@ centre-of-mass momentum removed and a rescale to the target temperature with
 3N-3 degrees of freedom, the steps of build_simulation's gen_vel
 (src/sim/simulation_engine.cpp:339-383).
 
+CORRECTED-physics workloads rebuild their lists by the Verlet criterion
+(rebuild_interval -1: an atom moved more than half the skin since the last
+build); argon_fcc keeps DMD's fixed cadence (every 10 force computes).
+
 Workloads (sizes as named):
   argon_fcc      4000-atom LJ argon on an FCC lattice, rho* 0.8442, NVE.
   water_box      flexible 3-site water (O -0.82, H +0.41), SPME, Nose-Hoover.
@@ -140,7 +144,7 @@ def lj_fluid(n: int, rho_star: float, t_star: float, seed: int, cutoff: float | 
         lj_sigma=np.array([[AR_SIGMA]]), lj_epsilon=np.array([[AR_EPS]]),
         lj_cutoff=cutoff if cutoff is not None else 2.5 * AR_SIGMA, list_skin=skin, dt=dt,
         thermostat=thermostat, thermo_temperature=temp, thermo_tau=0.1,
-        physics=_abi.CORRECTED, name=f"lj_fluid_{n}")
+        physics=_abi.CORRECTED, rebuild_interval=-1, name=f"lj_fluid_{n}")
     return spec.with_(**kw) if kw else spec
 
 
@@ -209,7 +213,7 @@ def water_box(n_side: int = 16, cutoff: float = 0.9, rtol: float = 1e-5, grid: i
         coulomb_type=_abi.COULOMB_PME, coulomb_cutoff=cutoff,
         ewald_coeff=ewald_alpha(cutoff, rtol), pme_grid=(g, g, g), dt=dt,
         thermostat=thermostat, thermo_temperature=temperature, thermo_tau=0.1,
-        physics=_abi.CORRECTED, name=f"water_{n}")
+        physics=_abi.CORRECTED, rebuild_interval=-1, name=f"water_{n}")
     return spec.with_(**kw) if kw else spec
 
 
@@ -284,7 +288,7 @@ def chain_melt(n: int, seed: int, rho_star: float = 0.85, t_star: float = 1.0,
         bonds=bonds, angles=angles, dihedrals=dihedrals, exclusions=excl,
         coulomb_type=_abi.COULOMB_NONE, dt=dt, thermostat=thermostat,
         thermo_temperature=temp, thermo_tau=0.1, physics=_abi.CORRECTED,
-        name=f"chain_melt_{n}")
+        rebuild_interval=-1, name=f"chain_melt_{n}")
     return spec.with_(**kw) if kw else spec
 
 

@@ -89,7 +89,10 @@ typedef struct dmdx_system_desc {
     const double *lj_epsilon;      /* n_types*n_types, row-major              */
     double lj_cutoff;
     double list_skin;              /* list built at cutoff+skin (0 = DMD)     */
-    int32_t rebuild_interval;      /* computes between rebuilds (DMD: 10)     */
+    int32_t rebuild_interval;      /* > 0: computes between rebuilds; 0: DMD's
+                                      10; -1: rebuild when an atom has moved
+                                      more than list_skin/2 since the last
+                                      build (Verlet criterion)               */
     int32_t step_since_rebuild;    /* initial counter (DMD: 10)               */
 
     /* bonded terms (HarmonicBondParams etc., src/force/harmonic_*.h) */
@@ -142,6 +145,7 @@ typedef struct dmdx_scalars {
     int32_t step_since_rebuild;    /* LennardJones::step_since_rebuild()      */
     int32_t flags;                 /* 1: neighbour-list overflow, 2: non-finite */
     int64_t steps_done;
+    int64_t list_builds;           /* neighbour-list builds since creation    */
 } dmdx_scalars;
 
 /* ---- errors and device ------------------------------------------------ */
@@ -211,6 +215,12 @@ int dmdx_engine_get_pairs(dmdx_engine *e, int32_t sys, int32_t *pi, int32_t *pj,
  * the stream and returns the accumulated time and launch count of a group. */
 int dmdx_engine_set_profiling(dmdx_engine *e, int32_t on);
 int dmdx_engine_kernel_time(dmdx_engine *e, int32_t group, double *total_ms, int64_t *launches);
+
+/* Launch shape of the mixed-precision kernels (no counterpart in the
+ * reference): key 0 = lanes per atom of the pair kernel, key 1 = lanes per
+ * atom of the list build; value 8, 16 or 32.  Results do not depend on it
+ * beyond FP32 summation order. */
+int dmdx_engine_set_tuning(dmdx_engine *e, int32_t key, int32_t value);
 
 /* Device-to-device copy of nbytes, repeated; reports the achieved GB/s
  * (read + write).  Calibrates hardware byte counters. */

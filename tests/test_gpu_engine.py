@@ -123,7 +123,9 @@ def test_barostats(baro):
 @pytest.mark.parametrize("mode", [FA, CO])
 def test_bonded_chain(mode):
     say(f"case: chain melt 600 FP64 physics {mode}")
-    spec = W.chain_melt(600, seed=5).with_(physics=mode, list_skin=0.05 if mode else 0.0)
+    # FAITHFUL: DMD's list semantics (no skin, rebuild every 10 computes)
+    spec = W.chain_melt(600, seed=5).with_(physics=mode, list_skin=0.05 if mode else 0.0,
+                                           rebuild_interval=-1 if mode else 10)
     e = engine(spec)
     sim = O.Sim(spec)
     e.compute()
@@ -155,7 +157,7 @@ def test_water_pme_fp64_corrected():
 
 def test_water_pme_fp64_faithful_components():
     say("case: water 648 FP64 faithful (PME layout-consistent)")
-    spec = W.water_box(n_mol=216, seed=4).with_(physics=FA, list_skin=0.0)
+    spec = W.water_box(n_mol=216, seed=4).with_(physics=FA, list_skin=0.0, rebuild_interval=10)
     e = engine(spec)
     sim = O.Sim(spec)
     e.compute()
@@ -176,6 +178,13 @@ def test_mixed_force_rms(kind):
         spec = W.chain_melt(2000, seed=9)
     e = engine(spec.with_(precision=_abi.MIXED))
     e.compute()
+    # the mixed path takes its pair decisions in FP64: same set as the oracle
+    gi, gj = e.pairs()
+    rl = max(spec.lj_cutoff, spec.coulomb_cutoff if spec.coulomb_type else 0.0) + spec.list_skin
+    oi, oj = O.pairs(spec.pos, spec.box, rl, "cells")
+    if spec.exclusions is not None:
+        oi, oj = O.filter_excl(oi, oj, *spec.exclusions)
+    assert np.array_equal(gi, oi) and np.array_equal(gj, oj)
     F_ref, pe_ref = O.Sim(spec).forces_exact()
     err = rms_rel(e.forces(), F_ref)
     say(f"  force rms rel err {err:.3e}")
