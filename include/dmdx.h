@@ -207,6 +207,77 @@ int dmdx_engine_set_active(dmdx_engine *e, int32_t sys, int32_t active);
 int dmdx_engine_get_pairs(dmdx_engine *e, int32_t sys, int32_t *pi, int32_t *pj, int64_t cap,
                           int64_t *count);
 
+/* ---- components: ForceComponent (src/force/force_component.h:8-22) ----- */
+
+typedef struct dmdx_component dmdx_component;
+
+/* LennardJones(LennardJonesParams, cutoff) (src/force/lennard_jones.h:15);
+ * sigma/epsilon are n_types*n_types row-major (lennard_jones.h:7-11).  The
+ * list has no skin and is rebuilt every 10 computes, the counter starting
+ * at 10 (lennard_jones.cpp:9-10, :53-55, :91). */
+int dmdx_lj_create(int32_t n_types, const double *sigma, const double *epsilon, double cutoff,
+                   dmdx_component **out);
+/* LennardJones::set_atom_types (lennard_jones.h:28) */
+int dmdx_lj_set_atom_types(dmdx_component *c, const int32_t *types, int64_t n);
+/* LennardJones::step_since_rebuild / set_step_since_rebuild (lennard_jones.h:36-37) */
+int dmdx_lj_get_step_since_rebuild(dmdx_component *c, int32_t *out);
+int dmdx_lj_set_step_since_rebuild(dmdx_component *c, int32_t s);
+
+/* CoulombDirect(CoulombDirectParams{cutoff, ewald_coefficient})
+ * (src/force/coulomb_direct.h:6-19): erfc real-space pairs over a list that
+ * only dmdx_component_rebuild_pair_list fills (as in the reference). */
+int dmdx_coulomb_direct_create(double cutoff, double ewald_coefficient, dmdx_component **out);
+
+/* CoulombExclusion (src/force/coulomb_exclusion.h:6-25) */
+int dmdx_coulomb_exclusion_create(dmdx_component **out);
+/* CoulombExclusion::set_exclusions (coulomb_exclusion.h:21) */
+int dmdx_coulomb_exclusion_set_exclusions(dmdx_component *c, const int32_t *excl_i,
+                                          const int32_t *excl_j, int64_t n);
+
+/* CoulombPME(PMEParams{cutoff, ewald_coefficient, nx, ny, nz, spline_order})
+ * (src/force/coulomb_pme.h:7-16); spline_order 4 only, as the reference. */
+int dmdx_pme_create(double cutoff, double ewald_coefficient, int32_t nx, int32_t ny, int32_t nz,
+                    int32_t spline_order, dmdx_component **out);
+/* CoulombPME::theta4 / dtheta4 / calc_self_energy (coulomb_pme.h:39-41) */
+int dmdx_pme_theta4(double u, double *out);
+int dmdx_pme_dtheta4(double u, double *out);
+int dmdx_pme_self_energy(double alpha, const double *charges, int64_t n, double k_e, double *out);
+
+/* HarmonicBond, HarmonicAngle, PeriodicDihedral, HarmonicImproper from their
+ * parameter structs (src/force/harmonic_bond.h:7-11, harmonic_angle.h:7-11,
+ * periodic_dihedral.h:7-12, harmonic_improper.h:7-11). */
+int dmdx_harmonic_bond_create(int64_t n, const int32_t *i, const int32_t *j, const double *k,
+                              const double *r0, dmdx_component **out);
+int dmdx_harmonic_angle_create(int64_t n, const int32_t *i, const int32_t *j, const int32_t *k,
+                               const double *k_theta, const double *theta0, dmdx_component **out);
+int dmdx_periodic_dihedral_create(int64_t n, const int32_t *i, const int32_t *j, const int32_t *k,
+                                  const int32_t *l, const int32_t *periodicity,
+                                  const double *k_phi, const double *phi0, dmdx_component **out);
+int dmdx_harmonic_improper_create(int64_t n, const int32_t *i, const int32_t *j, const int32_t *k,
+                                  const int32_t *l, const double *k_phi, const double *phi0,
+                                  dmdx_component **out);
+
+/* set_charges of CoulombDirect / CoulombPME / CoulombExclusion
+ * (coulomb_direct.h:26, coulomb_pme.h:26, coulomb_exclusion.h:20) */
+int dmdx_component_set_charges(dmdx_component *c, const double *charges, int64_t n);
+
+/* rebuild_pair_list of LennardJones / CoulombDirect (lennard_jones.h:29-34,
+ * coulomb_direct.h:27-32); cell is Cell::matrix (row-major 3x3, orthorhombic). */
+int dmdx_component_rebuild_pair_list(dmdx_component *c, const double *x, const double *y,
+                                     const double *z, int64_t n, const double cell[9]);
+
+/* ForceComponent::compute: forces and *energy_out are ADDED to. */
+int dmdx_component_compute(dmdx_component *c, const double *x, const double *y, const double *z,
+                           int64_t n, const double cell[9], double *fx, double *fy, double *fz,
+                           double *energy_out);
+
+/* ForceComponent::name(): "LennardJones", "CoulombDirect", ... */
+int dmdx_component_name(dmdx_component *c, char *buf, int64_t cap);
+
+/* DMDX_FAITHFUL (default) | DMDX_CORRECTED, DMDX_FP64 (default) | DMDX_MIXED */
+int dmdx_component_set_mode(dmdx_component *c, int32_t physics, int32_t precision);
+int dmdx_component_destroy(dmdx_component *c);
+
 /* ---- measurement (no counterpart in the reference) -------------------- */
 
 /* Bracket every kernel group with HIP events on the engine's stream (eager
