@@ -28,3 +28,23 @@ g++ -std=c++20 -O2 -fPIC -ffp-contract=off -fno-fast-math -shared \
     -I"$REF/src" -o "$OUT/libdmdref.so.tmp" "${ARGS[@]}" "$HERE/ref_shim.cpp"
 mv "$OUT/libdmdref.so.tmp" "$OUT/libdmdref.so"
 echo "build_ref: $OUT/libdmdref.so"
+
+# tests/adapter_check.cpp: include/dmdx_dmd.hpp inside DMD's own classes,
+# linked against DMD's translation units and the HIP library
+LIB="$HERE/../dmd_amd/libdmdx.so"
+if [ -f "$LIB" ]; then
+  ADP=(
+    core/cell.cpp core/system_data.cpp core/error.cpp
+    force/force_component.cpp force/force_engine.cpp force/lennard_jones.cpp
+    force/coulomb_direct.cpp force/coulomb_exclusion.cpp force/harmonic_bond.cpp
+    force/harmonic_angle.cpp force/periodic_dihedral.cpp force/harmonic_improper.cpp
+    integrate/integrator.cpp sim/checkpoint.cpp
+  )
+  AARGS=()
+  for s in "${ADP[@]}"; do AARGS+=("$REF/src/$s"); done
+  g++ -std=c++20 -O2 -ffp-contract=off -fno-fast-math -I"$REF/src" -I"$HERE/../include" \
+      -o "$OUT/adapter_check.tmp" "$HERE/../tests/adapter_check.cpp" "${AARGS[@]}" \
+      -L"$HERE/../dmd_amd" -ldmdx -Wl,-rpath,'$ORIGIN/../../dmd_amd'
+  mv "$OUT/adapter_check.tmp" "$OUT/adapter_check"
+  echo "build_ref: $OUT/adapter_check"
+fi
