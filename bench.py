@@ -11,7 +11,15 @@ LJ on non-excluded pairs), Berendsen NVT, dt 1 fs -- all replicas advanced
 together in the same launches, pair arithmetic in FP32, integration and
 reductions in FP64.  One "step" is one velocity-Verlet step of every
 replica; ``value`` = sum(atoms) * K / wall seconds of the K timed steps
-(state resident in HBM before the timed region starts).
+(state resident in HBM before the timed region starts).  The default K =
+2000 is the whole run of the workload from its generated start (the chain
+melts relax from a hot initial packing during it).  Per-step latency
+percentiles come from separately synchronised steps after the timed region.
+
+    python bench.py --workload serving [--requests N --rate R ...]
+
+Serving: Poisson arrivals of independent LJ / water requests on
+dmdx_server (see run_serving).
 
 Prints ONE JSON line.
 """
@@ -86,7 +94,11 @@ def percentile(v, p):
     return float(v[min(len(v) - 1, int(math.ceil(p / 100.0 * len(v))) - 1)])
 
 
-def run_batched(specs, steps, warmup):
+def run_batched(specs, steps, warmup, latency_steps):
+    """Timed region: `steps` steps of every replica enqueued back to back,
+    bracketed by device synchronisations (kernel-group HIP events on the
+    engine's stream).  Then, outside the timed region, `latency_steps`
+    single steps, each synchronised, for the per-step latency percentiles."""
     from dmd_amd import _abi
     from dmd_amd.engine import Engine
     mixed = [s.with_(precision=_abi.MIXED) for s in specs]
@@ -94,19 +106,20 @@ def run_batched(specs, steps, warmup):
     eng.step(warmup)
     eng.sync()
     eng.set_profiling(True)
-    eng.kernel_times()  # reset accumulation point
     base = eng.kernel_times()
-    lat = []
     t0 = time.perf_counter()
-    for _ in range(steps):
-        a = time.perf_counter()
-        eng.step(1)
-        eng.sync()
-        lat.append(time.perf_counter() - a)
+    eng.step(steps)
+    eng.sync()
     wall = time.perf_counter() - t0
     kt = eng.kernel_times()
     kt = {k: (kt[k][0] - base[k][0], kt[k][1] - base[k][1]) for k in kt}
     eng.set_profiling(False)
+    lat = []
+    for _ in range(latency_steps):
+        a = time.perf_counter()
+        eng.step(1)
+        eng.sync()
+        lat.append(time.perf_counter() - a)
     flags = [eng.scalars(k)["flags"] for k in range(len(specs))]
     return eng, wall, lat, kt, flags
 
@@ -167,22 +180,56 @@ def accuracy(specs, n_check=4):
 
 
 def load_traffic(kernel, workload):
-    p = os.path.join(ROOT, "profiles", "traffic.json")
+    """HBM bytes per launch of `kernel` from profiles/traffic_<workload>.json
+    (written by `python -m dmd_amd.profiling traffic` from two rocprofv3
+    --pmc runs, FETCH_SIZE and WRITE_SIZE, calibrated on a copy kernel)."""
+    p = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
     if not os.path.exists(p):
         return None
     try:
-        data = json.load(open(p))
-        rec = data.get(workload, {}).get(kernel)
+        rec = json.load(open(p)).get("kernels", {}).get(kernel)
         return rec.get("hbm_bytes_per_launch") if rec else None
     except (OSError, ValueError):
         return None
 
 
+def run_serving(args):
+    """Poisson arrivals of independent requests (LJ argon 1k-32k atoms or
+    216-8192 waters with SPME; 1000-20000 steps log-uniform) served by
+    dmdx_server: admission by device memory, batched cohorts, retirement at
+    each request's step target."""
+    from dmd_amd import workloads as W
+    from dmd_amd.server import Server, run_poisson
+    reqs = W.serving_requests(args.requests, seed=args.seed, steps_scale=args.steps_scale)
+    log(f"[bench] serving: {len(reqs)} requests at {args.rate}/s, "
+        f"{sum(s.n * n for s, n in reqs):.4g} atom-steps")
+    srv = Server(chunk_steps=args.chunk, max_cohorts=args.cohorts, graph_steps=args.graph_steps)
+    rep = run_poisson(srv, reqs, rate_per_s=args.rate, seed=args.seed, log=log)
+    srv.close()
+    out = {
+        "metric": "serving: aggregate atom-steps/s under Poisson arrivals; p50/p99 per-step "
+                  "and completion latency",
+        "value": rep["atom_steps_per_s"], "unit": "atom-steps/s", "n_gpus": 1,
+        "steps": None, "warmup": 0, "ms_per_step": rep["step_latency_ms"]["p50"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32 pair/PME arithmetic, fp64 accumulation and integration",
+        "data": "synthetic (seeded generators; no datasets)",
+        "config": {"workload": f"serving: {len(reqs)} requests, 50% LJ argon N~U[1k,32k], "
+                               "50% water U[216,8192] molecules (SPME), steps log-uniform "
+                               f"[1000,20000] x {args.steps_scale}, Poisson {args.rate}/s",
+                   "chunk_steps": args.chunk, "max_cohorts": args.cohorts,
+                   "graph_steps": args.graph_steps},
+        "serving": rep,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--latency-steps", type=int, default=200)
     ap.add_argument("--workload", default="replica_mix")
     ap.add_argument("--replicas", type=int, default=256)
     ap.add_argument("--seed", type=int, default=42)
@@ -192,6 +239,12 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="batched leg only (profiling runs)")
+    ap.add_argument("--requests", type=int, default=120, help="serving: number of requests")
+    ap.add_argument("--rate", type=float, default=8.0, help="serving: arrivals per second")
+    ap.add_argument("--steps-scale", type=float, default=1.0, help="serving: step-count factor")
+    ap.add_argument("--chunk", type=int, default=50, help="serving: steps per round")
+    ap.add_argument("--cohorts", type=int, default=8, help="serving: concurrent cohorts")
+    ap.add_argument("--graph-steps", type=int, default=0, help="serving: captured block size")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -200,13 +253,16 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "serving":
+        run_serving(args)
+        return
     from dmd_amd import engine as E
     log(f"[bench] building workload {args.workload}")
     specs, desc = make_workload(args.workload, args.replicas, args.seed)
     n_atoms = sum(s.n for s in specs)
     log(f"[bench] {len(specs)} systems, {n_atoms} atoms; batched mixed: warmup {args.warmup}, "
         f"timed {args.steps}")
-    eng, wall, lat, kt, flags = run_batched(specs, args.steps, args.warmup)
+    eng, wall, lat, kt, flags = run_batched(specs, args.steps, args.warmup, args.latency_steps)
     value = n_atoms * args.steps / wall
     ms_per_step = wall / args.steps * 1e3
     log(f"[bench] batched mixed: {value:.4g} atom-steps/s, {ms_per_step:.3f} ms/step")

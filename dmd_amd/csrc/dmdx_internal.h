@@ -39,7 +39,8 @@ int fail(int status, const std::string& msg);  // set_error + return status
     do {                                                                               \
         hipError_t e_ = (expr);                                                        \
         if (e_ != hipSuccess)                                                          \
-            return ::dmdx::fail(DMDX_E_GPU, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+            return ::dmdx::fail(e_ == hipErrorOutOfMemory ? DMDX_E_OOM : DMDX_E_GPU,     \
+                                std::string(#expr) + ": " + hipGetErrorString(e_));    \
     } while (0)
 
 // ---------------------------------------------------- per-system constants
@@ -119,9 +120,17 @@ struct SysState {
     int32_t built;       // a list has been built since creation
     int64_t rebuilds;    // list builds since creation
     int64_t steps_done;
+    int64_t stop_at;     // > 0: the system leaves the batch once steps_done reaches it
 };
 
 enum { FLAG_NBR_OVERFLOW = 1, FLAG_NONFINITE = 2 };
+
+// Charges are spread onto the PME grid as 64-bit fixed-point integers with
+// 2^44 units per e: integer atomic adds are associative, so the grid (and
+// everything after it) does not depend on the order in which atoms land,
+// and a system gives the same result alone or in any batch.  Resolution
+// 5.7e-14 e per contribution; |grid value| must stay below 2^19 e.
+constexpr double QGRID_SCALE = 17592186044416.0;  // 2^44
 
 // term force buffer: one double3 per (term, role)
 struct TermF {
@@ -190,6 +199,7 @@ struct Dev {
     // PME
     int64_t grid_total = 0;
     void* grid = nullptr;  // complex, double or float per precision
+    unsigned long long* qgrid = nullptr;  // charge spreading, fixed point (QGRID_SCALE)
     int* grid_sys = nullptr;
     void* tw = nullptr;  // twiddles, same real type
     double* bmod = nullptr;

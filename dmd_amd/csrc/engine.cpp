@@ -36,22 +36,22 @@ const char* last_error() { return g_last_error.c_str(); }
 static int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 
 template <class T>
-static int dev_upload(T** dst, const std::vector<T>& src, std::vector<void*>& owned)
+static int dev_upload(T** dst, const std::vector<T>& src, DevMem& owned)
 {
     size_t n = std::max<size_t>(src.size(), 1);
     DMDX_HIP(hipMalloc((void**)dst, n * sizeof(T)));
-    owned.push_back(*dst);
+    owned.add(*dst, n * sizeof(T));
     if (!src.empty()) DMDX_HIP(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
     else DMDX_HIP(hipMemset(*dst, 0, sizeof(T)));
     return DMDX_OK;
 }
 
 template <class T>
-static int dev_alloc_zero(T** dst, size_t n, std::vector<void*>& owned)
+static int dev_alloc_zero(T** dst, size_t n, DevMem& owned)
 {
     n = std::max<size_t>(n, 1);
     DMDX_HIP(hipMalloc((void**)dst, n * sizeof(T)));
-    owned.push_back(*dst);
+    owned.add(*dst, n * sizeof(T));
     DMDX_HIP(hipMemset(*dst, 0, n * sizeof(T)));
     return DMDX_OK;
 }
@@ -87,7 +87,7 @@ Batch::~Batch()
     for (auto ev : ev_pool) (void)hipEventDestroy(ev);
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (graph) (void)hipGraphDestroy(graph);
-    for (void* p : owned) (void)hipFree(p);
+    for (void* p : owned.ptrs) (void)hipFree(p);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -622,6 +622,7 @@ int Batch::create(const dmdx_system_desc* descs, int nsys, const BatchOptions& o
     TRY(dev_upload(&d.c_mid, c_mid, owned));
     TRY(dev_upload(&d.contrib, contrib, owned));
     d.grid_total = grid_total;
+    TRY(dev_alloc_zero(&d.qgrid, grid_total, owned));
     if (prec == DMDX_FP64) {
         TRY(dev_alloc_zero((double**)&d.grid, 2 * grid_total, owned));
         TRY(dev_upload((double**)&d.tw, twd, owned));
@@ -966,7 +967,19 @@ int Batch::set_active(int s, int active)
     TRY(fetch_state());
     hstate[s].active = active ? 1 : 0;
     host[s].active = active != 0;
-    TRY(copy(&d.st[s].active, &hstate[s].active, sizeof(int32_t), hipMemcpyHostToDevice));
+    // re-admitting a system clears its failure flags (otherwise it would
+    // leave again before its next step)
+    if (active) hstate[s].flags = 0;
+    TRY(copy(&d.st[s], &hstate[s], sizeof(SysState), hipMemcpyHostToDevice));
+    return DMDX_OK;
+}
+
+int Batch::set_stop_at(const std::vector<int64_t>& stop)
+{
+    if ((int)stop.size() != d.nsys) return fail(DMDX_E_MISMATCH, "one step target per system");
+    TRY(fetch_state());
+    for (int s = 0; s < d.nsys; ++s) hstate[s].stop_at = stop[s];
+    TRY(copy(d.st, hstate.data(), sizeof(SysState) * hstate.size(), hipMemcpyHostToDevice));
     return DMDX_OK;
 }
 

@@ -45,6 +45,17 @@ struct HostSys {
     bool has_barostat = false;
 };
 
+// device allocations of a batch, with their total size
+struct DevMem {
+    std::vector<void*> ptrs;
+    size_t bytes = 0;
+    void add(void* p, size_t b)
+    {
+        ptrs.push_back(p);
+        bytes += b;
+    }
+};
+
 class Batch {
 public:
     ~Batch();
@@ -61,6 +72,9 @@ public:
     int set_box(int s, const double L[3]);
     int set_counter(int s, int counter);
     int set_active(int s, int active);
+    // step targets: system s leaves the batch (becomes inactive) at the start
+    // of the first step after steps_done reaches stop[s] (0 = no target)
+    int set_stop_at(const std::vector<int64_t>& stop);
     int get_pairs(int s, int32_t* pi, int32_t* pj, int64_t cap, int64_t* count);
     int upload_sys(int s);
     void invalidate_graph();
@@ -71,6 +85,7 @@ public:
     std::vector<SysState> hstate;
     std::vector<HostSys> host;
     hipStream_t stream = nullptr;
+    size_t device_bytes() const { return owned.bytes; }
 
 private:
     static bool exclusion_term_active(const dmdx_system_desc& D, const BatchOptions& opt);
@@ -79,7 +94,7 @@ private:
     int enqueue_step(bool uniform);
 
     hipError_t timed(int group, hipError_t (*fn)(const Dev&, hipStream_t));
-    std::vector<void*> owned;
+    DevMem owned;
     std::vector<hipEvent_t> ev_pool;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_used;
 

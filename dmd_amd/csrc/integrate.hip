@@ -47,6 +47,12 @@ __global__ void __launch_bounds__(64) k_pre_compute(const SysDev* __restrict__ s
         for (int w = lane_id(); w < S0.aw_n; w += 64) dmax = fmax(dmax, p_disp[S0.aw_off + w]);
     for (int o = 32; o > 0; o >>= 1) dmax = fmax(dmax, __shfl_xor(dmax, o, 64));
     if (lane_id() != 0) return;
+    // a system leaves the batch before the first compute of a step once it
+    // has reached its step target or has failed (neighbour overflow,
+    // non-finite energy): its state stays that of the end of the last step,
+    // and the other systems of the batch are unaffected
+    if (phase == 1 && T.active && (T.flags != 0 || (T.stop_at > 0 && T.steps_done >= T.stop_at)))
+        T.active = 0;
     if (!T.active) {
         T.rebuild = 0;
         T.reuse = 1;
@@ -129,6 +135,10 @@ __device__ __forceinline__ void k_gather_unit(int64_t unit_, const SysDev* __res
         frac_x -= floor(frac_x);
         frac_y -= floor(frac_y);
         frac_z -= floor(frac_z);
+        // non-finite coordinates (a failed system) must still index the grid
+        frac_x = (frac_x >= 0.0 && frac_x <= 1.0) ? frac_x : 0.0;
+        frac_y = (frac_y >= 0.0 && frac_y <= 1.0) ? frac_y : 0.0;
+        frac_z = (frac_z >= 0.0 && frac_z <= 1.0) ? frac_z : 0.0;
         double ux = frac_x * nx, uy = frac_y * ny, uz = frac_z * nz;
         int m0x = (int)floor(ux) - 1, m0y = (int)floor(uy) - 1, m0z = (int)floor(uz) - 1;
         R wx[4], wy[4], wz[4], dwx[4], dwy[4], dwz[4];

@@ -36,6 +36,7 @@ __device__ __forceinline__ int cell_coord(double x, double L, int nu)
 {
     double f = x / L;
     f -= floor(f);
+    if (!(f >= 0.0 && f <= 1.0)) f = 0.0;  // non-finite coordinate (a failed system)
     int c = (int)(f * nu);
     return c < 0 ? 0 : (c >= nu ? nu - 1 : c);
 }

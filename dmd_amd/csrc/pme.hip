@@ -2,9 +2,10 @@
 //
 // Replaces CoulombPME::compute (src/force/coulomb_pme.cpp:287-318) and its
 // FFTW calls (:154, :198).  Per compute:
-//   k_pme_zero      clear the complex grid
-//   k_pme_spread    order-4 B-spline charge spreading (:79-136), 64 atomic
-//                   adds per charged atom
+//   k_pme_zero      clear the fixed-point charge grid
+//   k_pme_spread    order-4 B-spline charge spreading (:79-136), 64 integer
+//                   atomic adds per charged atom (order-independent sums)
+//   k_pme_convert   fixed point -> complex grid
 //   k_fft_pass x,y  forward 1D FFT passes along x then y
 //   k_fft_pass z+   forward z pass, influence function G(m) with the energy
 //                   0.5 G |Q(m)|^2 (:143-201), inverse z pass -- the whole
@@ -59,26 +60,51 @@ __device__ __forceinline__ R dtheta4(R u)
     return s * (R(9) * a * a - R(12) * a) / R(6);
 }
 
-template <class R>
-__device__ __forceinline__ void k_pme_zero_unit(int64_t unit_, const SysDev* __restrict__ sys, const SysState* __restrict__ st,
-                           const int* __restrict__ gsys, int64_t ntot, Cx<R>* __restrict__ grid)
+__device__ __forceinline__ void k_pme_zero_unit(int64_t unit_, const SysState* __restrict__ st,
+                                                const int* __restrict__ gsys, int64_t ntot,
+                                                unsigned long long* __restrict__ qgrid)
 {
     int64_t e = unit_ * blockDim.x + threadIdx.x;
     if (e >= ntot) return;
     int s = gsys[e];
     if (s < 0 || st[s].reuse || !st[s].active) return;
-    grid[e] = {R(0), R(0)};
+    qgrid[e] = 0ull;
+}
+
+__global__ void k_pme_zero(const SysState* __restrict__ st, const int* __restrict__ gsys,
+                           int64_t ntot, unsigned long long* __restrict__ qgrid, int64_t nunits_)
+{
+    DMDX_FOR_UNITS(u_, nunits_) k_pme_zero_unit(u_, st, gsys, ntot, qgrid);
+}
+
+// fixed-point grid -> complex grid of the FFT (imaginary part 0)
+template <class R>
+__device__ __forceinline__ void k_pme_convert_unit(int64_t unit_, const SysState* __restrict__ st,
+                                                   const int* __restrict__ gsys, int64_t ntot,
+                                                   const unsigned long long* __restrict__ qgrid,
+                                                   Cx<R>* __restrict__ grid)
+{
+    int64_t e = unit_ * blockDim.x + threadIdx.x;
+    if (e >= ntot) return;
+    int s = gsys[e];
+    if (s < 0 || st[s].reuse || !st[s].active) return;
+    const double v = (double)(long long)qgrid[e] * (1.0 / QGRID_SCALE);
+    grid[e] = {R(v), R(0)};
 }
 
 template <class R>
-__global__ void k_pme_zero(const SysDev* __restrict__ sys, const SysState* __restrict__ st,
-                           const int* __restrict__ gsys, int64_t ntot, Cx<R>* __restrict__ grid, int64_t nunits_)
+__global__ void k_pme_convert(const SysState* __restrict__ st, const int* __restrict__ gsys,
+                              int64_t ntot, const unsigned long long* __restrict__ qgrid,
+                              Cx<R>* __restrict__ grid, int64_t nunits_)
 {
-    DMDX_FOR_UNITS(u_, nunits_) k_pme_zero_unit<R>(u_, sys, st, gsys, ntot, grid);
+    DMDX_FOR_UNITS(u_, nunits_) k_pme_convert_unit<R>(u_, st, gsys, ntot, qgrid, grid);
 }
 
-__device__ __forceinline__ void atomic_add_real(double* p, double v) { unsafeAtomicAdd(p, v); }
-__device__ __forceinline__ void atomic_add_real(float* p, float v) { unsafeAtomicAdd(p, v); }
+// two's-complement fixed point: unsigned wrap-around addition is signed addition
+__device__ __forceinline__ void atomic_add_fixed(unsigned long long* p, double v)
+{
+    atomicAdd(p, (unsigned long long)__double2ll_rn(v * QGRID_SCALE));
+}
 
 // per atom: fractional coordinate, first grid index and offset, in FP64 (the
 // reference's expressions), then the weights in R
@@ -87,7 +113,7 @@ __device__ __forceinline__ void k_pme_spread_unit(int64_t unit_, const SysDev* _
                              const int* __restrict__ slot_sys, int64_t nslots,
                              const double* __restrict__ x, const double* __restrict__ y,
                              const double* __restrict__ z, const double* __restrict__ q,
-                             Cx<R>* __restrict__ grid)
+                             unsigned long long* __restrict__ qgrid)
 {
     int64_t g = unit_ * blockDim.x + threadIdx.x;
     if (g >= nslots) return;
@@ -102,6 +128,10 @@ __device__ __forceinline__ void k_pme_spread_unit(int64_t unit_, const SysDev* _
     frac_x -= floor(frac_x);
     frac_y -= floor(frac_y);
     frac_z -= floor(frac_z);
+    // non-finite coordinates (a failed system) must still index the grid
+    frac_x = (frac_x >= 0.0 && frac_x <= 1.0) ? frac_x : 0.0;
+    frac_y = (frac_y >= 0.0 && frac_y <= 1.0) ? frac_y : 0.0;
+    frac_z = (frac_z >= 0.0 && frac_z <= 1.0) ? frac_z : 0.0;
     double ux = frac_x * nx, uy = frac_y * ny, uz = frac_z * nz;
     int m0x = (int)floor(ux) - 1, m0y = (int)floor(uy) - 1, m0z = (int)floor(uz) - 1;
     R wx[4], wy[4], wz[4];
@@ -110,7 +140,7 @@ __device__ __forceinline__ void k_pme_spread_unit(int64_t unit_, const SysDev* _
         wy[d] = theta4<R>((R)(uy - (m0y + d)));
         wz[d] = theta4<R>((R)(uz - (m0z + d)));
     }
-    Cx<R>* G = grid + S.grid_off;
+    unsigned long long* G = qgrid + S.grid_off;
     const R qr = (R)qi;
     for (int iz = 0; iz < 4; ++iz) {
         int gz = (m0z + iz) % nz;
@@ -124,7 +154,7 @@ __device__ __forceinline__ void k_pme_spread_unit(int64_t unit_, const SysDev* _
             for (int ix = 0; ix < 4; ++ix) {
                 int gx = (m0x + ix) % nx;
                 if (gx < 0) gx += nx;
-                atomic_add_real(&G[row + gx].re, qr * wx[ix] * w_yz);
+                atomic_add_fixed(&G[row + gx], (double)(qr * wx[ix] * w_yz));
             }
         }
     }
@@ -135,9 +165,9 @@ __global__ void k_pme_spread(const SysDev* __restrict__ sys, const SysState* __r
                              const int* __restrict__ slot_sys, int64_t nslots,
                              const double* __restrict__ x, const double* __restrict__ y,
                              const double* __restrict__ z, const double* __restrict__ q,
-                             Cx<R>* __restrict__ grid, int64_t nunits_)
+                             unsigned long long* __restrict__ qgrid, int64_t nunits_)
 {
-    DMDX_FOR_UNITS(u_, nunits_) k_pme_spread_unit<R>(u_, sys, st, slot_sys, nslots, x, y, z, q, grid);
+    DMDX_FOR_UNITS(u_, nunits_) k_pme_spread_unit<R>(u_, sys, st, slot_sys, nslots, x, y, z, q, qgrid);
 }
 
 // R-point DFT of v[0..R) in place, with w^m = tw[(m * (K / R)) % K]
@@ -339,10 +369,11 @@ static hipError_t launch_pme_t(const Dev& d, hipStream_t s)
     const int BS = 256;
     Cx<R>* grid = reinterpret_cast<Cx<R>*>(d.grid);
     const Cx<R>* tw = reinterpret_cast<const Cx<R>*>(d.tw);
-    k_pme_zero<R><<<grid_for(blocks_for(d.grid_total, BS)), BS, 0, s>>>(d.sys, d.st, d.grid_sys,
-                                                              d.grid_total, grid, (int64_t)(blocks_for(d.grid_total, BS)));
+    const int64_t gu = blocks_for(d.grid_total, BS);
+    k_pme_zero<<<grid_for(gu), BS, 0, s>>>(d.st, d.grid_sys, d.grid_total, d.qgrid, gu);
     k_pme_spread<R><<<grid_for(blocks_for(d.nslots, BS)), BS, 0, s>>>(d.sys, d.st, d.slot_sys, d.nslots,
-                                                            d.x, d.y, d.z, d.q, grid, (int64_t)(blocks_for(d.nslots, BS)));
+                                                            d.x, d.y, d.z, d.q, d.qgrid, (int64_t)(blocks_for(d.nslots, BS)));
+    k_pme_convert<R><<<grid_for(gu), BS, 0, s>>>(d.st, d.grid_sys, d.grid_total, d.qgrid, grid, gu);
     // forward x, forward y, forward z + convolution + inverse z, inverse y, inverse x
     const int dims[5] = {0, 1, 2, 1, 0};
     const int signs[5] = {-1, -1, -1, +1, +1};

@@ -19,8 +19,12 @@ import sys
 
 
 def short(name: str) -> str:
-    n = name.split("(")[0]
-    return n.replace("dmdx::", "")
+    """Kernel symbol without return type, namespace and template arguments."""
+    n = name.split("(")[0].strip()
+    if n.startswith("void "):
+        n = n[5:]
+    n = n.replace("dmdx::", "")
+    return n.split("<")[0] if not n.startswith("__") else n
 
 
 def kernel_stats(db: str):

@@ -307,3 +307,27 @@ def replica_mix(n_replicas: int = 256, n_min: int = 500, n_max: int = 8000, seed
         else:
             out.append(chain_melt(n, s, **kw))
     return out
+
+
+def serving_requests(n_requests: int, seed: int = 42, steps_scale: float = 1.0,
+                     n_lj=(1000, 32000), n_water=(216, 8192), steps=(1000, 20000),
+                     precision=_abi.MIXED) -> list:
+    """Request stream for the serving loop: with equal probability an LJ argon
+    fluid of N ~ U[1k, 32k] atoms (rho* 0.8442, T* 0.72, rc 0.85 nm, skin
+    0.1 nm, NVE) or a box of U[216, 8192] flexible waters (SPME, grid
+    round(L/0.1) made 7-smooth, Nose-Hoover 300 K); step count log-uniform
+    in [1000, 20000], times `steps_scale`.  Returns [(spec, n_steps), ...]."""
+    rng = rng_for(derive_seed(seed, 4))
+    out = []
+    for r in range(n_requests):
+        s = derive_seed(seed, 4, r)
+        if rng.random() < 0.5:
+            n = int(rng.integers(n_lj[0], n_lj[1] + 1))
+            spec = lj_fluid(n, 0.8442, 0.72, s, cutoff=0.85, skin=0.1,
+                            thermostat=_abi.THERMO_NONE, precision=precision)
+        else:
+            m = int(rng.integers(n_water[0], n_water[1] + 1))
+            spec = water_box(n_mol=m, seed=s, precision=precision)
+        k = math.exp(rng.uniform(math.log(steps[0]), math.log(steps[1])))
+        out.append((spec, max(1, int(round(k * steps_scale)))))
+    return out

@@ -297,6 +297,72 @@ int dmdx_engine_set_tuning(dmdx_engine *e, int32_t key, int32_t value);
  * (read + write).  Calibrates hardware byte counters. */
 int dmdx_copy_bandwidth(int64_t nbytes, int32_t repeats, double *gbps);
 
+/* ---- serving many simulations on one device ----------------------------
+ * Replaces running many SimulationEngine::run loops side by side
+ * (src/sim/simulation_engine.cpp:89-106, driven per process by
+ * python/dmd/runner.py:44-58).  A request is a system plus a step count.
+ * Waiting requests are admitted when their estimated device memory fits in
+ * the budget and grouped into cohorts: one batch per cohort, each on its own
+ * stream, several cohorts stepped concurrently.  A request leaves its
+ * cohort exactly at its step target, or when it fails (neighbour-list
+ * overflow, non-finite energy) -- the other systems are not disturbed, and a
+ * request's result is the same alone or in any mix.  One host thread drives
+ * a server: submit at any time, then call dmdx_server_pump repeatedly. */
+typedef struct dmdx_server dmdx_server;
+
+typedef struct dmdx_server_options {
+    int64_t hbm_budget_bytes;      /* device memory the server may hold;
+                                      0 = 90% of the free memory at creation */
+    int32_t max_cohorts;           /* cohorts stepped concurrently (0: 8)     */
+    int32_t chunk_steps;           /* steps per cohort per pump (0: 50)       */
+    int32_t graph_steps;           /* > 0: replay captured blocks of steps    */
+    int32_t min_admit;             /* form a cohort once this many requests
+                                      wait (0: 1) ...                          */
+    double max_wait_ms;            /* ... or once the oldest waited this long */
+    int32_t max_cohort_systems;    /* systems per cohort (0: no limit)        */
+} dmdx_server_options;
+
+enum { DMDX_REQ_QUEUED = 0, DMDX_REQ_RUNNING = 1, DMDX_REQ_DONE = 2, DMDX_REQ_FAILED = 3 };
+
+typedef struct dmdx_request_info {
+    int32_t state;                 /* DMDX_REQ_*                              */
+    int32_t flags;                 /* as dmdx_scalars::flags                  */
+    int64_t n_atoms;
+    int64_t steps_target, steps_done;
+    double t_submit, t_admit, t_done;  /* seconds on the server clock (-1: not yet) */
+    int64_t est_bytes;             /* admission estimate of device memory     */
+    char error[128];               /* why a FAILED request failed             */
+} dmdx_request_info;
+
+typedef struct dmdx_server_stats {
+    int64_t queued, running, retired, failed;
+    int64_t cohorts, cohorts_made, rounds;
+    int64_t bytes_in_use, bytes_budget;
+    double estimate_scale;         /* learned actual/estimate memory ratio    */
+    double clock_s;                /* server clock now                        */
+} dmdx_server_stats;
+
+int dmdx_server_create(const dmdx_server_options *opt, dmdx_server **out);
+int dmdx_server_destroy(dmdx_server *srv);
+/* copies desc; returns the request id */
+int dmdx_server_submit(dmdx_server *srv, const dmdx_system_desc *desc, int64_t n_steps,
+                       int64_t *id);
+/* one round: admit, advance every cohort by up to chunk_steps, wait, retire */
+int dmdx_server_pump(dmdx_server *srv, int32_t *n_retired);
+int dmdx_server_request_info(dmdx_server *srv, int64_t id, dmdx_request_info *out);
+/* final state of a finished request: what 0 positions, 1 velocities, 2 forces */
+int dmdx_server_result(dmdx_server *srv, int64_t id, int32_t what, double *x, double *y,
+                       double *z);
+int dmdx_server_result_scalars(dmdx_server *srv, int64_t id, dmdx_scalars *out);
+/* forget a finished request and its results */
+int dmdx_server_release(dmdx_server *srv, int64_t id);
+int dmdx_server_get_stats(dmdx_server *srv, dmdx_server_stats *out);
+/* per-step latency samples: each pump contributes, per cohort, the round's
+ * wall time divided by the steps it advanced (ms), weighted by the steps the
+ * requests advanced; count = number of samples (copied up to cap) */
+int dmdx_server_step_latency(dmdx_server *srv, double *ms, int64_t *weight, int64_t cap,
+                             int64_t *count);
+
 #ifdef __cplusplus
 }
 #endif
