@@ -90,6 +90,8 @@ def count_pairs(eng, specs):
 
 
 def percentile(v, p):
+    if len(v) == 0:
+        return None
     v = np.sort(np.asarray(v))
     return float(v[min(len(v) - 1, int(math.ceil(p / 100.0 * len(v))) - 1)])
 
@@ -103,6 +105,8 @@ def run_batched(specs, steps, warmup, latency_steps):
     from dmd_amd.engine import Engine
     mixed = [s.with_(precision=_abi.MIXED) for s in specs]
     eng = Engine(mixed, reuse_forces=True, use_graph=False)
+    tune = eng.autotune(steps=5, cache=os.environ.get("DMDX_TUNE_CACHE"))
+    log(f"[bench] autotune: {tune}")
     eng.step(warmup)
     eng.sync()
     eng.set_profiling(True)
@@ -121,7 +125,7 @@ def run_batched(specs, steps, warmup, latency_steps):
         eng.sync()
         lat.append(time.perf_counter() - a)
     flags = [eng.scalars(k)["flags"] for k in range(len(specs))]
-    return eng, wall, lat, kt, flags
+    return eng, wall, lat, kt, flags, tune
 
 
 def run_plain(specs, steps, budget_s):
@@ -262,7 +266,8 @@ def main():
     n_atoms = sum(s.n for s in specs)
     log(f"[bench] {len(specs)} systems, {n_atoms} atoms; batched mixed: warmup {args.warmup}, "
         f"timed {args.steps}")
-    eng, wall, lat, kt, flags = run_batched(specs, args.steps, args.warmup, args.latency_steps)
+    eng, wall, lat, kt, flags, tune = run_batched(specs, args.steps, args.warmup,
+                                                  args.latency_steps)
     value = n_atoms * args.steps / wall
     ms_per_step = wall / args.steps * 1e3
     log(f"[bench] batched mixed: {value:.4g} atom-steps/s, {ms_per_step:.3f} ms/step")
@@ -294,9 +299,11 @@ def main():
         "data": "synthetic (seeded generators; no datasets)",
         "config": {"workload": desc, "systems": len(specs), "atoms": n_atoms,
                    "precision": "mixed", "parallelism": "batched systems, one GPU"},
-        "latency_ms": {"p50": percentile(lat, 50) * 1e3, "p99": percentile(lat, 99) * 1e3},
+        "latency_ms": {"p50": percentile(lat, 50) * 1e3 if lat else None,
+                       "p99": percentile(lat, 99) * 1e3 if lat else None},
         "roofline": roofline,
         "failed_systems": int(sum(1 for f in flags if f)),
+        "autotune": tune,
     }
     if not args.quick:
         out["force_rms_rel_err_mixed_vs_fp64"] = accuracy(specs)

@@ -1,0 +1,177 @@
+// autotune.cpp -- launch-shape autotuner for the mixed-precision kernels.
+//
+// The pair kernel and the list build of the mixed path give each atom a
+// group of 8, 16 or 32 lanes, and the pair kernel takes 2 or 4 list entries
+// per lane per pass (4 hides more gather latency per wave, 2 needs fewer
+// registers so more waves fit).  Which is fastest depends on the systems (how
+// many neighbours an atom has, how many atoms a batch holds), so the choice
+// is measured: a scratch copy of the engine's batch, set to rebuild its lists
+// at every compute, is stepped with each candidate and the per-launch times
+// of the two kernels are read from HIP events.  The engine's own state is
+// not touched.  Results are cached in a text file keyed by a signature of the
+// systems, so a workload is timed once.  There is no counterpart in the
+// reference (its loops are serial).
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
+
+#include "engine.h"
+
+namespace {
+
+uint64_t fnv(uint64_t h, const void* p, size_t n)
+{
+    const unsigned char* c = static_cast<const unsigned char*>(p);
+    for (size_t k = 0; k < n; ++k) {
+        h ^= c[k];
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+template <class T>
+uint64_t mix(uint64_t h, T v)
+{
+    return fnv(h, &v, sizeof(v));
+}
+
+// what the launch shapes depend on: sizes, densities, cutoffs and terms
+uint64_t signature(const dmdx_engine* e)
+{
+    uint64_t h = 1469598103934665603ull;
+    h = mix(h, (int64_t)e->descs.size());
+    for (const auto& dc : e->descs) {
+        const dmdx_system_desc& D = dc->d;
+        h = mix(h, D.n_atoms);
+        for (int k = 0; k < 3; ++k) h = mix(h, (int64_t)std::llround(D.box[k] * 1e3));
+        h = mix(h, D.use_lj);
+        h = mix(h, (int64_t)std::llround(D.lj_cutoff * 1e4));
+        h = mix(h, (int64_t)std::llround(D.list_skin * 1e4));
+        h = mix(h, D.coulomb_type);
+        h = mix(h, (int64_t)std::llround(D.coulomb_cutoff * 1e4));
+        h = mix(h, D.n_excl);
+        h = mix(h, D.precision);
+        h = mix(h, D.physics);
+    }
+    return h;
+}
+
+bool cache_lookup(const char* path, uint64_t key, int* pair, int* width, int* build)
+{
+    if (!path || !*path) return false;
+    std::ifstream f(path);
+    std::string line;
+    while (std::getline(f, line)) {
+        std::istringstream ss(line);
+        std::string k;
+        int p = 0, w = 0, b = 0;
+        if (!(ss >> k >> p >> w >> b)) continue;
+        if (std::strtoull(k.c_str(), nullptr, 16) == key) {
+            *pair = p;
+            *width = w;
+            *build = b;
+            return true;
+        }
+    }
+    return false;
+}
+
+void cache_store(const char* path, uint64_t key, int pair, int width, int build, const double* t)
+{
+    if (!path || !*path) return;
+    std::ofstream f(path, std::ios::app);
+    char buf[320];
+    std::snprintf(buf, sizeof(buf),
+                  "%016llx %d %d %d  # pair ms 8x2 %.4f 8x4 %.4f 16x2 %.4f 16x4 %.4f 32x2 %.4f "
+                  "32x4 %.4f build ms 8 %.4f 16 %.4f 32 %.4f\n",
+                  (unsigned long long)key, pair, width, build, t[0], t[1], t[2], t[3], t[4], t[5],
+                  t[6], t[7], t[8]);
+    f << buf;
+}
+
+}  // namespace
+
+extern "C" int dmdx_engine_autotune(dmdx_engine* e, int32_t steps, const char* cache_path,
+                                    int32_t* pair_lanes, int32_t* pair_width,
+                                    int32_t* build_lanes, double* times_ms)
+{
+    if (!e) return dmdx::fail(DMDX_E_INVALID, "null engine");
+    // pair kernel: (lanes per atom, entries per lane per pass); list build:
+    // lanes per atom.  Pass c times pair shape c and build shape c % 3.
+    const int pl[6] = {8, 8, 16, 16, 32, 32};
+    const int pw[6] = {2, 4, 2, 4, 2, 4};
+    const int bl[3] = {8, 16, 32};
+    double t[9];
+    for (double& v : t) v = -1.0;
+    auto report = [&]() {
+        if (pair_lanes) *pair_lanes = e->b.d.pair_tpa;
+        if (pair_width) *pair_width = e->b.d.pair_width;
+        if (build_lanes) *build_lanes = e->b.d.build_tpa;
+        if (times_ms)
+            for (int k = 0; k < 9; ++k) times_ms[k] = t[k];
+    };
+    if (e->b.d.precision == DMDX_FP64) {  // one lane per atom, nothing to tune
+        report();
+        return DMDX_OK;
+    }
+    const uint64_t key = signature(e);
+    int cp = 0, cw = 0, cb = 0;
+    if (cache_lookup(cache_path, key, &cp, &cw, &cb) && (cp == 8 || cp == 16 || cp == 32) &&
+        (cw == 2 || cw == 4) && (cb == 8 || cb == 16 || cb == 32)) {
+        e->b.d.pair_tpa = cp;
+        e->b.d.pair_width = cw;
+        e->b.d.build_tpa = cb;
+        e->b.invalidate_graph();
+        report();
+        return DMDX_OK;
+    }
+    // scratch batch: the same systems, lists rebuilt at every compute
+    std::vector<dmdx_system_desc> ds;
+    for (const auto& dc : e->descs) {
+        dmdx_system_desc D = dc->d;
+        D.rebuild_interval = 1;
+        ds.push_back(D);
+    }
+    dmdx::BatchOptions bo;
+    bo.profile = true;
+    bo.reuse_forces = false;
+    dmdx::Batch scratch;
+    int rc = scratch.create(ds.data(), (int)ds.size(), bo);
+    if (rc) return rc;
+    const int n = steps > 0 ? steps : 5;
+    for (int c = 0; c < 6; ++c) {
+        scratch.d.pair_tpa = pl[c];
+        scratch.d.pair_width = pw[c];
+        scratch.d.build_tpa = bl[c % 3];
+        rc = scratch.step(1);  // first launches of this shape
+        if (rc) return rc;
+        double p0, l0;
+        int64_t np0, nl0;
+        rc = scratch.kernel_time(dmdx::KG_PAIR, &p0, &np0);
+        if (!rc) rc = scratch.kernel_time(dmdx::KG_LIST, &l0, &nl0);
+        if (!rc) rc = scratch.step(n);
+        double p1, l1;
+        int64_t np1, nl1;
+        if (!rc) rc = scratch.kernel_time(dmdx::KG_PAIR, &p1, &np1);
+        if (!rc) rc = scratch.kernel_time(dmdx::KG_LIST, &l1, &nl1);
+        if (rc) return rc;
+        t[c] = (p1 - p0) / (double)std::max<int64_t>(1, np1 - np0);
+        const double tb = (l1 - l0) / (double)std::max<int64_t>(1, nl1 - nl0);
+        t[6 + c % 3] = t[6 + c % 3] < 0.0 ? tb : std::min(t[6 + c % 3], tb);
+    }
+    int bp = 0, bb = 0;
+    for (int c = 1; c < 6; ++c)
+        if (t[c] < t[bp]) bp = c;
+    for (int c = 1; c < 3; ++c)
+        if (t[6 + c] < t[6 + bb]) bb = c;
+    e->b.d.pair_tpa = pl[bp];
+    e->b.d.pair_width = pw[bp];
+    e->b.d.build_tpa = bl[bb];
+    e->b.invalidate_graph();
+    cache_store(cache_path, key, pl[bp], pw[bp], bl[bb], t);
+    report();
+    return DMDX_OK;
+}

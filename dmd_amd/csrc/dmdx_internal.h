@@ -121,9 +121,19 @@ struct SysState {
     int64_t rebuilds;    // list builds since creation
     int64_t steps_done;
     int64_t stop_at;     // > 0: the system leaves the batch once steps_done reaches it
+    int32_t nub[3];      // cells per edge of the last list build
+    int32_t pad_;
 };
 
 enum { FLAG_NBR_OVERFLOW = 1, FLAG_NONFINITE = 2 };
+
+// Mixed-path list entries: bits 0-25 the neighbour's cell-order slot, bits
+// 26-31 the neighbour cell's offset from the atom's cell along x, y, z
+// (each stored +1, in {0, 1, 2}).  With cell-local coordinates l and cell
+// width w the displacement is d = l_j - l_i + offset * w (minimum image
+// when the system has >= 3 cells per edge).
+constexpr int NBR_SLOT_BITS = 26;
+constexpr uint32_t NBR_SLOT_MASK = (1u << NBR_SLOT_BITS) - 1u;
 
 // Charges are spread onto the PME grid as 64-bit fixed-point integers with
 // 2^44 units per e: integer atomic adds are associative, so the grid (and
@@ -173,11 +183,14 @@ struct Dev {
     double4* pos4s = nullptr;  // (x, y, z, q) in cell order (list build; mixed pair kernel)
     double4* xref = nullptr;   // positions at the last list build (displacement criterion)
     float4* pos4f = nullptr;   // mixed list build: wrapped FP32 positions in cell order
+    double4* org = nullptr;    // mixed path: origin of each slot's cell-local frame (FP64)
+    float4* lpos = nullptr;    // mixed pair kernel: cell-local FP32 positions and charge
     double* p_disp = nullptr;  // per-wave max squared displacement
     bool any_auto = false;     // some system uses the displacement criterion
     int* typ_s = nullptr;      // mixed path: LJ type in cell order
     int pair_tpa = 16;         // mixed pair kernel: lanes per atom (8, 16, 32)
     int build_tpa = 16;        // mixed list build: lanes per atom (8, 16, 32)
+    int pair_width = 4;        // mixed pair kernel: entries per lane per pass (2, 4)
     // LJ tables
     double *lj_sig = nullptr, *lj_eps = nullptr;
     float *lj_c12 = nullptr, *lj_c6 = nullptr;
@@ -213,6 +226,7 @@ struct Dev {
     double *p_ke = nullptr, *p_w = nullptr;
     // what exists in the batch
     bool any_list = false, any_pair = false, any_pme = false, any_excl = false;
+    bool any_coul_pairs = false;  // some system computes real-space erfc pairs
     bool any_gather = false, any_baro = false;
 };
 

@@ -543,6 +543,7 @@ int Batch::create(const dmdx_system_desc* descs, int nsys, const BatchOptions& o
         const SysDev& S = hsys[s];
         d.any_list |= S.has_list != 0;
         d.any_pme |= S.pme != 0;
+        d.any_coul_pairs |= S.coul_pairs != 0;
         d.any_excl |= S.n_ex > 0;
         d.any_baro |= S.barostat != DMDX_BARO_NONE;
         host[s].has_list = S.has_list != 0;
@@ -642,8 +643,12 @@ int Batch::create(const dmdx_system_desc* descs, int nsys, const BatchOptions& o
     TRY(dev_alloc_zero(&d.xref, slot, owned));
     TRY(dev_alloc_zero(&d.p_disp, nw, owned));
     if (prec != DMDX_FP64) {
+        if (slot > (int64_t)NBR_SLOT_MASK)
+            return fail(DMDX_E_PARM, "a mixed-precision batch holds at most 2^26 atom slots");
         TRY(dev_alloc_zero(&d.typ_s, slot, owned));
         TRY(dev_alloc_zero(&d.pos4f, slot, owned));
+        TRY(dev_alloc_zero(&d.org, slot, owned));
+        TRY(dev_alloc_zero(&d.lpos, slot, owned));
     }
     TRY(dev_alloc_zero(&d.p_ke, nw, owned));
     TRY(dev_alloc_zero(&d.p_w, nw, owned));
@@ -1003,6 +1008,7 @@ int Batch::get_pairs(int s, int32_t* pi, int32_t* pj, int64_t cap, int64_t* coun
         int i = order[t] - (int)S.atom_off;
         for (int k = 0; k < nn[t]; ++k) {
             int e = blk[(size_t)t * S.max_nbr + k];
+            if (sorted_ids) e = (int)((uint32_t)e & NBR_SLOT_MASK);
             int j = (sorted_ids ? order[e - (int)S.atom_off] : e) - (int)S.atom_off;
             if (j > i) keys.push_back(((int64_t)i << 32) | (int64_t)j);
         }
@@ -1021,9 +1027,6 @@ int Batch::get_pairs(int s, int32_t* pi, int32_t* pj, int64_t cap, int64_t* coun
 // ===================================================================== C-ABI
 using dmdx::Batch;
 
-struct dmdx_engine {
-    Batch b;
-};
 
 extern "C" {
 
@@ -1055,6 +1058,7 @@ int dmdx_engine_create(const dmdx_system_desc* descs, int32_t n_systems, dmdx_en
         delete e;
         return rc;
     }
+    for (int s = 0; s < n_systems; ++s) e->descs.push_back(std::make_unique<dmdx::DescCopy>(descs[s]));
     *out = e;
     return DMDX_OK;
 }
@@ -1170,11 +1174,17 @@ int dmdx_engine_kernel_time(dmdx_engine* e, int32_t group, double* total_ms, int
 int dmdx_engine_set_tuning(dmdx_engine* e, int32_t key, int32_t value)
 {
     if (!e) return dmdx::fail(DMDX_E_INVALID, "null engine");
-    if (value != 8 && value != 16 && value != 32)
-        return dmdx::fail(DMDX_E_PARM, "lanes per atom must be 8, 16 or 32");
-    if (key == 0) e->b.d.pair_tpa = value;
-    else if (key == 1) e->b.d.build_tpa = value;
-    else return dmdx::fail(DMDX_E_INVALID, "unknown tuning key");
+    if (key == 2) {
+        if (value != 2 && value != 4)
+            return dmdx::fail(DMDX_E_PARM, "entries per lane must be 2 or 4");
+        e->b.d.pair_width = value;
+    } else {
+        if (value != 8 && value != 16 && value != 32)
+            return dmdx::fail(DMDX_E_PARM, "lanes per atom must be 8, 16 or 32");
+        if (key == 0) e->b.d.pair_tpa = value;
+        else if (key == 1) e->b.d.build_tpa = value;
+        else return dmdx::fail(DMDX_E_INVALID, "unknown tuning key");
+    }
     e->b.invalidate_graph();
     return DMDX_OK;
 }

@@ -1,6 +1,9 @@
 // engine.h -- host-side batch object of the step engine.
 #pragma once
 
+#include <memory>
+#include <vector>
+
 #include "dmdx_internal.h"
 
 namespace dmdx {
@@ -43,6 +46,66 @@ struct HostSys {
     bool built = false;
     bool active = true;
     bool has_barostat = false;
+};
+
+// deep copy of a system description (the library copies its inputs)
+struct DescCopy {
+    dmdx_system_desc d;
+    std::vector<std::vector<double>> dv;
+    std::vector<std::vector<int32_t>> iv;
+
+    template <class T>
+    const T* keep(std::vector<std::vector<T>>& store, const T* p, int64_t n)
+    {
+        if (!p || n <= 0) return nullptr;
+        store.emplace_back(p, p + n);
+        return store.back().data();
+    }
+
+    explicit DescCopy(const dmdx_system_desc& s) : d(s)
+    {
+        const int64_t n = s.n_atoms;
+        dv.reserve(32);
+        iv.reserve(32);
+        d.pos_x = keep(dv, s.pos_x, n);
+        d.pos_y = keep(dv, s.pos_y, n);
+        d.pos_z = keep(dv, s.pos_z, n);
+        d.vel_x = keep(dv, s.vel_x, n);
+        d.vel_y = keep(dv, s.vel_y, n);
+        d.vel_z = keep(dv, s.vel_z, n);
+        d.masses = keep(dv, s.masses, n);
+        d.charges = keep(dv, s.charges, n);
+        d.atom_types = keep(iv, s.atom_types, n);
+        const int64_t t2 = (int64_t)s.lj_n_types * s.lj_n_types;
+        d.lj_sigma = keep(dv, s.lj_sigma, t2);
+        d.lj_epsilon = keep(dv, s.lj_epsilon, t2);
+        d.bond_i = keep(iv, s.bond_i, s.n_bonds);
+        d.bond_j = keep(iv, s.bond_j, s.n_bonds);
+        d.bond_k = keep(dv, s.bond_k, s.n_bonds);
+        d.bond_r0 = keep(dv, s.bond_r0, s.n_bonds);
+        d.angle_i = keep(iv, s.angle_i, s.n_angles);
+        d.angle_j = keep(iv, s.angle_j, s.n_angles);
+        d.angle_k = keep(iv, s.angle_k, s.n_angles);
+        d.angle_k_theta = keep(dv, s.angle_k_theta, s.n_angles);
+        d.angle_theta0 = keep(dv, s.angle_theta0, s.n_angles);
+        d.dih_i = keep(iv, s.dih_i, s.n_dihedrals);
+        d.dih_j = keep(iv, s.dih_j, s.n_dihedrals);
+        d.dih_k = keep(iv, s.dih_k, s.n_dihedrals);
+        d.dih_l = keep(iv, s.dih_l, s.n_dihedrals);
+        d.dih_periodicity = keep(iv, s.dih_periodicity, s.n_dihedrals);
+        d.dih_k_phi = keep(dv, s.dih_k_phi, s.n_dihedrals);
+        d.dih_phi0 = keep(dv, s.dih_phi0, s.n_dihedrals);
+        d.imp_i = keep(iv, s.imp_i, s.n_impropers);
+        d.imp_j = keep(iv, s.imp_j, s.n_impropers);
+        d.imp_k = keep(iv, s.imp_k, s.n_impropers);
+        d.imp_l = keep(iv, s.imp_l, s.n_impropers);
+        d.imp_k_phi = keep(dv, s.imp_k_phi, s.n_impropers);
+        d.imp_phi0 = keep(dv, s.imp_phi0, s.n_impropers);
+        d.excl_i = keep(iv, s.excl_i, s.n_excl);
+        d.excl_j = keep(iv, s.excl_j, s.n_excl);
+    }
+    DescCopy(const DescCopy&) = delete;
+    DescCopy& operator=(const DescCopy&) = delete;
 };
 
 // device allocations of a batch, with their total size
@@ -109,3 +172,10 @@ private:
 };
 
 }  // namespace dmdx
+
+// handle behind dmdx_engine_* (include/dmdx.h)
+struct dmdx_engine {
+    dmdx::Batch b;
+    // host copies of the descriptions (the autotuner builds scratch batches)
+    std::vector<std::unique_ptr<dmdx::DescCopy>> descs;
+};

@@ -481,7 +481,7 @@ __global__ void __launch_bounds__(256) k_baro_finalize(SysDev* __restrict__ sys,
 __device__ __forceinline__ void k_scale_positions_unit(int64_t unit_, const SysDev* __restrict__ sys, const SysState* __restrict__ st,
                                   const int* __restrict__ slot_sys, int64_t nslots,
                                   double* __restrict__ x, double* __restrict__ y,
-                                  double* __restrict__ z)
+                                  double* __restrict__ z, double4* __restrict__ org)
 {
     int64_t g = unit_ * blockDim.x + threadIdx.x;
     if (g >= nslots) return;
@@ -493,14 +493,18 @@ __device__ __forceinline__ void k_scale_positions_unit(int64_t unit_, const SysD
     x[g] *= sc;
     y[g] *= sc;
     z[g] *= sc;
+    if (org) {  // mixed path: cell-local frames scale with the box
+        double4 o = org[g];
+        org[g] = make_double4(o.x * sc, o.y * sc, o.z * sc, o.w);
+    }
 }
 
 __global__ void k_scale_positions(const SysDev* __restrict__ sys, const SysState* __restrict__ st,
                                   const int* __restrict__ slot_sys, int64_t nslots,
                                   double* __restrict__ x, double* __restrict__ y,
-                                  double* __restrict__ z, int64_t nunits_)
+                                  double* __restrict__ z, double4* __restrict__ org, int64_t nunits_)
 {
-    DMDX_FOR_UNITS(u_, nunits_) k_scale_positions_unit(u_, sys, st, slot_sys, nslots, x, y, z);
+    DMDX_FOR_UNITS(u_, nunits_) k_scale_positions_unit(u_, sys, st, slot_sys, nslots, x, y, z, org);
 }
 
 // apply the pending thermostat scale to the stored velocities (state reads)
@@ -603,7 +607,7 @@ hipError_t launch_barostat(const Dev& d, hipStream_t s)
                                                              d.p_w, (int64_t)(blocks_for(d.nslots, BS)));
     k_baro_finalize<<<d.nsys, 256, 0, s>>>(d.sys, d.st, d.p_ke, d.p_w);
     k_scale_positions<<<grid_for(blocks_for(d.nslots, BS)), BS, 0, s>>>(d.sys, d.st, d.slot_sys, d.nslots,
-                                                              d.x, d.y, d.z, (int64_t)(blocks_for(d.nslots, BS)));
+                                                              d.x, d.y, d.z, d.org, (int64_t)(blocks_for(d.nslots, BS)));
     return hipGetLastError();
 }
 

@@ -18,6 +18,7 @@
 //   k_nbr_build    per atom (cell order): scan the neighbour cells
 //   k_nbr_sort     per atom: ascending neighbour order (FP64 path: makes the
 //                  per-atom force sum follow the reference's summation order)
+//   k_pack         mixed path, every compute: cell-local FP32 positions
 #include "dmdx_internal.h"
 
 namespace dmdx {
@@ -91,13 +92,14 @@ __global__ void k_cell_count(const SysDev* __restrict__ sys, const SysState* __r
 }
 
 // one workgroup (256 threads) per system
-__global__ void k_cell_scan(const SysDev* __restrict__ sys, const SysState* __restrict__ st,
+__global__ void k_cell_scan(const SysDev* __restrict__ sys, SysState* __restrict__ st,
                             const int* __restrict__ ccount, int* __restrict__ cstart)
 {
     int s = blockIdx.x;
     if (!st[s].rebuild) return;
     const SysDev& S = sys[s];
     int ncell = cells_in_use(S, 0) * cells_in_use(S, 1) * cells_in_use(S, 2);
+    if (threadIdx.x < 3) st[s].nub[threadIdx.x] = cells_in_use(S, threadIdx.x);
     __shared__ int part[256];
     int t = threadIdx.x;
     int chunk = (ncell + 255) / 256;
@@ -224,10 +226,12 @@ __global__ void __launch_bounds__(64) k_cell_sort(const SysDev* __restrict__ sys
 // the scan below) and the reference positions of the displacement criterion
 __device__ __forceinline__ void k_pack_build_unit(int64_t unit_, const SysDev* __restrict__ sys, const SysState* __restrict__ st,
                              const int* __restrict__ slot_sys, int64_t nslots,
-                             const int* __restrict__ cell_atoms, const double* __restrict__ x,
+                             const int* __restrict__ cell_atoms, const int* __restrict__ acell,
+                             const double* __restrict__ x,
                              const double* __restrict__ y, const double* __restrict__ z,
                              const double* __restrict__ q, double4* __restrict__ pos4s,
-                             double4* __restrict__ xref, float4* __restrict__ pos4f)
+                             double4* __restrict__ xref, float4* __restrict__ pos4f,
+                             double4* __restrict__ org)
 {
     int64_t t = unit_ * blockDim.x + threadIdx.x;
     if (t >= nslots) return;
@@ -239,22 +243,33 @@ __device__ __forceinline__ void k_pack_build_unit(int64_t unit_, const SysDev* _
     pos4s[t] = make_double4(x[i], y[i], z[i], q[i]);
     xref[i] = make_double4(x[i], y[i], z[i], 0.0);
     if (pos4f) {
+        // mixed path: cell-local frame centred on the atom's cell,
+        // l = wrapped position - cell centre (|l| <= w/2 at the build, so FP32
+        // keeps ~3e-8 nm), origin = x - l; pos4f holds l for the list build
         double wx = x[i] - floor(x[i] * S.invL[0]) * S.L[0];
         double wy = y[i] - floor(y[i] * S.invL[1]) * S.L[1];
         double wz = z[i] - floor(z[i] * S.invL[2]) * S.L[2];
-        pos4f[t] = make_float4((float)wx, (float)wy, (float)wz, 0.0f);
+        const int nx = cells_in_use(S, 0), ny = cells_in_use(S, 1), nz = cells_in_use(S, 2);
+        const int c = acell[i];
+        const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
+        const double lx = wx - (cx + 0.5) * (S.L[0] / nx);
+        const double ly = wy - (cy + 0.5) * (S.L[1] / ny);
+        const double lz = wz - (cz + 0.5) * (S.L[2] / nz);
+        pos4f[t] = make_float4((float)lx, (float)ly, (float)lz, 0.0f);
+        org[t] = make_double4(x[i] - lx, y[i] - ly, z[i] - lz, q[i]);
     }
 }
 
 __global__ void k_pack_build(const SysDev* __restrict__ sys, const SysState* __restrict__ st,
                              const int* __restrict__ slot_sys, int64_t nslots,
-                             const int* __restrict__ cell_atoms, const double* __restrict__ x,
+                             const int* __restrict__ cell_atoms, const int* __restrict__ acell,
+                             const double* __restrict__ x,
                              const double* __restrict__ y, const double* __restrict__ z,
                              const double* __restrict__ q, double4* __restrict__ pos4s,
                              double4* __restrict__ xref, float4* __restrict__ pos4f,
-                             int64_t nunits_)
+                             double4* __restrict__ org, int64_t nunits_)
 {
-    DMDX_FOR_UNITS(u_, nunits_) k_pack_build_unit(u_, sys, st, slot_sys, nslots, cell_atoms, x, y, z, q, pos4s, xref, pos4f);
+    DMDX_FOR_UNITS(u_, nunits_) k_pack_build_unit(u_, sys, st, slot_sys, nslots, cell_atoms, acell, x, y, z, q, pos4s, xref, pos4f, org);
 }
 
 // per atom: squared displacement since the last build, per-wave maximum
@@ -312,11 +327,14 @@ __device__ __forceinline__ bool is_excluded(const int* __restrict__ ex_start,
 // wavefront ballot and append them to the atom's row of the list, so a row
 // is written with contiguous stores.  The list is row-major: the neighbours of
 // slot t occupy nbr[nbr_off + (t - atom_off) * M + k].
-//   FP64 path:  entries are atom slots (summation order is restored by
-//               k_nbr_sort);
-//   mixed path: entries are cell-order slots, i.e. indices into the packed
-//               cell-ordered positions (k_pack), which keeps the gathers of
-//               the pair kernel spatially local.
+//   FP64 path (this kernel, one lane per atom): entries are atom slots
+//               (summation order is restored by k_nbr_sort);
+//   mixed path (k_nbr_build_mixed): entries are cell-order slots, i.e.
+//               indices into the packed cell-ordered positions (k_pack),
+//               which keeps the gathers of the pair kernel spatially local,
+//               with the neighbour cell's offset in the top bits
+//               (NBR_SLOT_BITS): the pair kernel rebuilds the displacement
+//               from FP32 cell-local coordinates.
 template <int TPA>
 __device__ __forceinline__ void k_nbr_build_unit(int64_t unit_, 
     const SysDev* __restrict__ sys, SysState* __restrict__ st, const int* __restrict__ slot_sys,
@@ -351,15 +369,6 @@ __device__ __forceinline__ void k_nbr_build_unit(int64_t unit_,
     const int nox = edge_offsets(nx, ox), noy = edge_offsets(ny, oy), noz = edge_offsets(nz, oz);
     const double4 pi4 = pos4s[t];
     const double xi = pi4.x, yi = pi4.y, zi = pi4.z;
-    // mixed path: FP32 test on wrapped positions (error < 2e-6 nm here) and
-    // the exact FP64 decision only within 1e-5 nm of the list radius, so the
-    // pair set is exactly the FP64 one
-    const bool fast = pos4f != nullptr;
-    const float4 pif = fast ? pos4f[t] : make_float4(0.f, 0.f, 0.f, 0.f);
-    const float Lxf = (float)S.L[0], Lyf = (float)S.L[1], Lzf = (float)S.L[2];
-    const float ixf = (float)S.invL[0], iyf = (float)S.invL[1], izf = (float)S.invL[2];
-    const float lo2 = (float)((S.rlist - 1e-5) * (S.rlist - 1e-5));
-    const float hi2 = (float)((S.rlist + 1e-5) * (S.rlist + 1e-5));
     const double Lx = S.L[0], Ly = S.L[1], Lz = S.L[2];
     const double ix = S.invL[0], iy = S.invL[1], iz = S.invL[2];
     const double rl2 = S.rlist2;
@@ -380,6 +389,9 @@ __device__ __forceinline__ void k_nbr_build_unit(int64_t unit_,
                 int xx = cx + ox[c];
                 xx = xx < 0 ? xx + nx : (xx >= nx ? xx - nx : xx);
                 const int cell = S.cell_off + (zz * ny + yy) * nx + xx;
+                const uint32_t shift = ((uint32_t)(ox[c] + 1) << NBR_SLOT_BITS) |
+                                       ((uint32_t)(oy[b] + 1) << (NBR_SLOT_BITS + 2)) |
+                                       ((uint32_t)(oz[a] + 1) << (NBR_SLOT_BITS + 4));
                 const int64_t base = S.atom_off + cstart[cell];
                 const int cnt = ccount[cell];
                 for (int m0 = 0; m0 < cnt; m0 += TPA) {
@@ -387,22 +399,7 @@ __device__ __forceinline__ void k_nbr_build_unit(int64_t unit_,
                     bool hit = false;
                     int j = -1;
                     if (m < cnt && base + m != t) {
-                        bool exact = true;
-                        if (fast) {
-                            const float4 pf = pos4f[base + m];
-                            float fx = pf.x - pif.x, fy = pf.y - pif.y, fz = pf.z - pif.z;
-                            fx -= rintf(fx * ixf) * Lxf;
-                            fy -= rintf(fy * iyf) * Lyf;
-                            fz -= rintf(fz * izf) * Lzf;
-                            const float r2f = fx * fx + fy * fy + fz * fz;
-                            if (r2f < lo2) {
-                                hit = true;
-                                exact = false;
-                            } else if (r2f >= hi2) {
-                                exact = false;
-                            }
-                        }
-                        if (exact) {
+                        {
                             const double4 pj = pos4s[base + m];
                             double dx = pj.x - xi;
                             double dy = pj.y - yi;
@@ -418,14 +415,14 @@ __device__ __forceinline__ void k_nbr_build_unit(int64_t unit_,
                     }
                     if (TPA == 1) {
                         if (hit) {
-                            if (k < M) row[k] = sorted_ids ? (int)(base + m) : j;
+                            if (k < M) row[k] = sorted_ids ? (int)((uint32_t)(base + m) | shift) : j;
                             ++k;
                         }
                     } else {
                         const uint64_t bal = __ballot(hit);
                         const uint64_t mine = (bal >> gshift) & (TPA == 64 ? ~0ull : ((1ull << TPA) - 1));
                         const int pos = k + __popcll(mine & below);
-                        if (hit && pos < M) row[pos] = sorted_ids ? (int)(base + m) : j;
+                        if (hit && pos < M) row[pos] = sorted_ids ? (int)((uint32_t)(base + m) | shift) : j;
                         k += __popcll(mine);
                     }
                 }
@@ -450,6 +447,127 @@ __global__ void __launch_bounds__(256) k_nbr_build(
     const float4* __restrict__ pos4f, int64_t nunits_)
 {
     DMDX_FOR_UNITS_SPREAD(u_, nunits_) k_nbr_build_unit<TPA>(u_, sys, st, slot_sys, nslots, x, y, z, acell, ccount, cstart, cell_atoms, ex_start, ex_list, order, nn, nbr, pos4s, pos4f);
+}
+
+// Mixed path.  Same scan as k_nbr_build (neighbour cells in the same order,
+// atoms of a cell in slot order, ballot compaction into contiguous rows) on
+// the cell-local FP32 frames: a candidate's displacement is
+// l_j - l_i + offset * w, with the FP32 minimum image only when a system has
+// fewer than 3 cells along an edge.  Candidates within 1e-5 nm of the list
+// radius are decided by the reference's own FP64 expression on pos4s, so
+// the pair set is exactly the FP64 one.  Entries carry the cell offset
+// (NBR_SLOT_BITS) for the pair kernel.
+template <int TPA>
+__device__ __forceinline__ void k_nbr_build_mixed_unit(int64_t unit_,
+    const SysDev* __restrict__ sys, SysState* __restrict__ st, const int* __restrict__ slot_sys,
+    int64_t nslots, const int* __restrict__ acell, const int* __restrict__ ccount,
+    const int* __restrict__ cstart, const int* __restrict__ cell_atoms,
+    const int* __restrict__ ex_start, const int* __restrict__ ex_list, int* __restrict__ order,
+    int* __restrict__ nn, int* __restrict__ nbr, const double4* __restrict__ pos4s,
+    const float4* __restrict__ pos4f)
+{
+    constexpr int GPB = 256 / TPA;
+    const int lane = threadIdx.x % TPA;
+    const int64_t t = unit_ * GPB + threadIdx.x / TPA;
+    if (t >= nslots) return;
+    const int s = slot_sys[t];
+    if (s < 0 || !st[s].rebuild) return;
+    const SysDev& S = sys[s];
+    const int lt = (int)(t - S.atom_off);
+    if (lt >= S.n) {
+        if (lane == 0) {
+            nn[t] = 0;
+            order[t] = -1;
+        }
+        return;
+    }
+    const int i = cell_atoms[t];
+    if (lane == 0) order[t] = i;
+    const int nx = cells_in_use(S, 0), ny = cells_in_use(S, 1), nz = cells_in_use(S, 2);
+    const int ci = acell[i];
+    const int cx = ci % nx, cy = (ci / nx) % ny, cz = ci / (nx * ny);
+    const float4 li = pos4f[t];
+    const float wx = (float)(S.L[0] / nx), wy = (float)(S.L[1] / ny), wz = (float)(S.L[2] / nz);
+    const bool wrap = nx < 3 || ny < 3 || nz < 3;
+    const float lo2 = (float)((S.rlist - 1e-5) * (S.rlist - 1e-5));
+    const float hi2 = (float)((S.rlist + 1e-5) * (S.rlist + 1e-5));
+    const bool check_ex = ex_start[i + 1] > ex_start[i];
+    int* row = nbr + S.nbr_off + (int64_t)lt * S.max_nbr;
+    const int M = S.max_nbr;
+    const int gshift = (threadIdx.x & 63) / TPA * TPA;  // first lane of my group in the wave
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const int lox = nx >= 3 ? -1 : 0, hix = nx >= 2 ? 1 : 0;
+    const int loy = ny >= 3 ? -1 : 0, hiy = ny >= 2 ? 1 : 0;
+    const int loz = nz >= 3 ? -1 : 0, hiz = nz >= 2 ? 1 : 0;
+    int k = 0;
+    for (int oz = loz; oz <= hiz; ++oz) {
+        int zz = cz + oz;
+        zz = zz < 0 ? zz + nz : (zz >= nz ? zz - nz : zz);
+        for (int oy = loy; oy <= hiy; ++oy) {
+            int yy = cy + oy;
+            yy = yy < 0 ? yy + ny : (yy >= ny ? yy - ny : yy);
+            for (int ox = lox; ox <= hix; ++ox) {
+                int xx = cx + ox;
+                xx = xx < 0 ? xx + nx : (xx >= nx ? xx - nx : xx);
+                const int cell = S.cell_off + (zz * ny + yy) * nx + xx;
+                const int64_t base = S.atom_off + cstart[cell];
+                const int cnt = ccount[cell];
+                const uint32_t shift = ((uint32_t)(ox + 1) << NBR_SLOT_BITS) |
+                                       ((uint32_t)(oy + 1) << (NBR_SLOT_BITS + 2)) |
+                                       ((uint32_t)(oz + 1) << (NBR_SLOT_BITS + 4));
+                const float sx = ox * wx - li.x, sy = oy * wy - li.y, sz = oz * wz - li.z;
+                for (int m0 = 0; m0 < cnt; m0 += TPA) {
+                    const int m = m0 + lane;
+                    bool hit = false;
+                    if (m < cnt && base + m != t) {
+                        const float4 lj = pos4f[base + m];
+                        float dx = lj.x + sx, dy = lj.y + sy, dz = lj.z + sz;
+                        if (wrap) {
+                            dx -= rintf(dx * (float)S.invL[0]) * (float)S.L[0];
+                            dy -= rintf(dy * (float)S.invL[1]) * (float)S.L[1];
+                            dz -= rintf(dz * (float)S.invL[2]) * (float)S.L[2];
+                        }
+                        const float r2f = dx * dx + dy * dy + dz * dz;
+                        hit = r2f < lo2;
+                        if (!hit && r2f < hi2) {
+                            // the reference's expression (src/core/cell.cpp:26-38,
+                            // src/force/lennard_jones.cpp:29-36)
+                            const double4 pi4 = pos4s[t];
+                            const double4 pj = pos4s[base + m];
+                            double ddx = min_image_dmd(pj.x - pi4.x, S.L[0], S.invL[0]);
+                            double ddy = min_image_dmd(pj.y - pi4.y, S.L[1], S.invL[1]);
+                            double ddz = min_image_dmd(pj.z - pi4.z, S.L[2], S.invL[2]);
+                            hit = ddx * ddx + ddy * ddy + ddz * ddz < S.rlist2;
+                        }
+                        if (hit && check_ex)
+                            hit = !is_excluded(ex_start, ex_list, i, cell_atoms[base + m]);
+                    }
+                    const uint64_t bal = __ballot(hit);
+                    const uint64_t mine = (bal >> gshift) & (TPA == 64 ? ~0ull : ((1ull << TPA) - 1));
+                    const int pos = k + __popcll(mine & below);
+                    if (hit && pos < M) row[pos] = (int)((uint32_t)(base + m) | shift);
+                    k += __popcll(mine);
+                }
+            }
+        }
+    }
+    if (k > M) {
+        if (lane == 0) atomicOr(&st[s].flags, FLAG_NBR_OVERFLOW);
+        k = M;
+    }
+    if (lane == 0) nn[t] = k;
+}
+
+template <int TPA>
+__global__ void __launch_bounds__(256) k_nbr_build_mixed(
+    const SysDev* __restrict__ sys, SysState* __restrict__ st, const int* __restrict__ slot_sys,
+    int64_t nslots, const int* __restrict__ acell, const int* __restrict__ ccount,
+    const int* __restrict__ cstart, const int* __restrict__ cell_atoms,
+    const int* __restrict__ ex_start, const int* __restrict__ ex_list, int* __restrict__ order,
+    int* __restrict__ nn, int* __restrict__ nbr, const double4* __restrict__ pos4s,
+    const float4* __restrict__ pos4f, int64_t nunits_)
+{
+    DMDX_FOR_UNITS_SPREAD(u_, nunits_) k_nbr_build_mixed_unit<TPA>(u_, sys, st, slot_sys, nslots, acell, ccount, cstart, cell_atoms, ex_start, ex_list, order, nn, nbr, pos4s, pos4f);
 }
 
 // one 64-thread workgroup per atom slot: bitonic sort of its row in LDS
@@ -509,13 +627,14 @@ __global__ void __launch_bounds__(64) k_nbr_sort(const SysDev* __restrict__ sys,
     }
 }
 
-// mixed path, every compute: positions and charges in cell order
+// mixed path, every compute: cell-local FP32 positions and charges in cell
+// order (x - origin of the slot's frame, fixed at the last list build)
 __device__ __forceinline__ void k_pack_unit(int64_t unit_, const SysDev* __restrict__ sys, const SysState* __restrict__ st,
                        const int* __restrict__ slot_sys, int64_t nslots,
                        const int* __restrict__ order, const double* __restrict__ x,
                        const double* __restrict__ y, const double* __restrict__ z,
-                       const double* __restrict__ q, const int* __restrict__ type,
-                       double4* __restrict__ pos4s, int* __restrict__ typ_s)
+                       const int* __restrict__ type, const double4* __restrict__ org,
+                       float4* __restrict__ lpos, int* __restrict__ typ_s)
 {
     int64_t t = unit_ * blockDim.x + threadIdx.x;
     if (t >= nslots) return;
@@ -524,7 +643,9 @@ __device__ __forceinline__ void k_pack_unit(int64_t unit_, const SysDev* __restr
     const SysDev& S = sys[s];
     if (t - S.atom_off >= S.n) return;
     int i = order[t];
-    pos4s[t] = make_double4(x[i], y[i], z[i], q[i]);
+    const double4 o = org[t];
+    lpos[t] = make_float4((float)(x[i] - o.x), (float)(y[i] - o.y), (float)(z[i] - o.z),
+                          (float)o.w);
     typ_s[t] = type[i];
 }
 
@@ -532,10 +653,10 @@ __global__ void k_pack(const SysDev* __restrict__ sys, const SysState* __restric
                        const int* __restrict__ slot_sys, int64_t nslots,
                        const int* __restrict__ order, const double* __restrict__ x,
                        const double* __restrict__ y, const double* __restrict__ z,
-                       const double* __restrict__ q, const int* __restrict__ type,
-                       double4* __restrict__ pos4s, int* __restrict__ typ_s, int64_t nunits_)
+                       const int* __restrict__ type, const double4* __restrict__ org,
+                       float4* __restrict__ lpos, int* __restrict__ typ_s, int64_t nunits_)
 {
-    DMDX_FOR_UNITS(u_, nunits_) k_pack_unit(u_, sys, st, slot_sys, nslots, order, x, y, z, q, type, pos4s, typ_s);
+    DMDX_FOR_UNITS(u_, nunits_) k_pack_unit(u_, sys, st, slot_sys, nslots, order, x, y, z, type, org, lpos, typ_s);
 }
 
 }  // namespace dmdx
@@ -558,24 +679,24 @@ hipError_t launch_build_lists(const Dev& d, hipStream_t s)
         k_cell_sort<<<grid_for((unsigned)d.ncells), 64, 0, s>>>(d.sys, d.st, d.cell_sys, d.ncells, d.ccount,
                                                       d.cstart, d.cell_atoms, (int64_t)((unsigned)d.ncells));
     k_pack_build<<<grid_for(blocks_for(d.nslots, BS)), BS, 0, s>>>(d.sys, d.st, d.slot_sys, d.nslots,
-                                                         d.cell_atoms, d.x, d.y, d.z, d.q,
-                                                         d.pos4s, d.xref, d.pos4f, (int64_t)(blocks_for(d.nslots, BS)));
+                                                         d.cell_atoms, d.acell, d.x, d.y, d.z, d.q,
+                                                         d.pos4s, d.xref, d.pos4f, d.org,
+                                                         (int64_t)(blocks_for(d.nslots, BS)));
     if (d.precision == DMDX_FP64)
         k_nbr_build<1><<<grid_for(blocks_for(d.nslots, BS)), BS, 0, s>>>(
             d.sys, d.st, d.slot_sys, d.nslots, d.x, d.y, d.z, d.acell, d.ccount, d.cstart,
             d.cell_atoms, d.ex_start, d.ex_list, d.order, d.nn, d.nbr, d.pos4s, nullptr, (int64_t)(blocks_for(d.nslots, BS)));
-    else if (d.build_tpa == 8)
-        k_nbr_build<8><<<grid_for(blocks_for(d.nslots * 8, BS)), BS, 0, s>>>(
-            d.sys, d.st, d.slot_sys, d.nslots, d.x, d.y, d.z, d.acell, d.ccount, d.cstart,
-            d.cell_atoms, d.ex_start, d.ex_list, d.order, d.nn, d.nbr, d.pos4s, d.pos4f, (int64_t)(blocks_for(d.nslots * 8, BS)));
-    else if (d.build_tpa == 32)
-        k_nbr_build<32><<<grid_for(blocks_for(d.nslots * 32, BS)), BS, 0, s>>>(
-            d.sys, d.st, d.slot_sys, d.nslots, d.x, d.y, d.z, d.acell, d.ccount, d.cstart,
-            d.cell_atoms, d.ex_start, d.ex_list, d.order, d.nn, d.nbr, d.pos4s, d.pos4f, (int64_t)(blocks_for(d.nslots * 32, BS)));
-    else
-        k_nbr_build<16><<<grid_for(blocks_for(d.nslots * 16, BS)), BS, 0, s>>>(
-            d.sys, d.st, d.slot_sys, d.nslots, d.x, d.y, d.z, d.acell, d.ccount, d.cstart,
-            d.cell_atoms, d.ex_start, d.ex_list, d.order, d.nn, d.nbr, d.pos4s, d.pos4f, (int64_t)(blocks_for(d.nslots * 16, BS)));
+    else {
+        const int tpa = d.build_tpa;
+        const int64_t units = blocks_for(d.nslots * tpa, BS);
+#define DMDX_BUILD_ARGS                                                                       \
+    d.sys, d.st, d.slot_sys, d.nslots, d.acell, d.ccount, d.cstart, d.cell_atoms, d.ex_start, \
+        d.ex_list, d.order, d.nn, d.nbr, d.pos4s, d.pos4f, units
+        if (tpa == 8) k_nbr_build_mixed<8><<<grid_for(units), BS, 0, s>>>(DMDX_BUILD_ARGS);
+        else if (tpa == 32) k_nbr_build_mixed<32><<<grid_for(units), BS, 0, s>>>(DMDX_BUILD_ARGS);
+        else k_nbr_build_mixed<16><<<grid_for(units), BS, 0, s>>>(DMDX_BUILD_ARGS);
+#undef DMDX_BUILD_ARGS
+    }
     if (d.precision == DMDX_FP64)
         k_nbr_sort<<<grid_for((unsigned)d.nslots), 64, 0, s>>>(d.sys, d.st, d.slot_sys, d.nn, d.nbr, (int64_t)((unsigned)d.nslots));
     return hipGetLastError();
@@ -593,7 +714,7 @@ hipError_t launch_pack(const Dev& d, hipStream_t s)
 {
     if (d.precision == DMDX_FP64) return hipSuccess;
     k_pack<<<grid_for(blocks_for(d.nslots, 256)), 256, 0, s>>>(d.sys, d.st, d.slot_sys, d.nslots, d.order,
-                                                     d.x, d.y, d.z, d.q, d.type, d.pos4s,
+                                                     d.x, d.y, d.z, d.type, d.org, d.lpos,
                                                      d.typ_s, (int64_t)(blocks_for(d.nslots, 256)));
     return hipGetLastError();
 }

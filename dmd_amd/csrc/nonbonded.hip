@@ -122,15 +122,19 @@ __global__ void k_pair_fp64(const SysDev* __restrict__ sys, const SysState* __re
 }
 
 // Mixed precision.  A group of TPA lanes handles one atom (cell-order slot
-// t): lane l takes neighbours l, l+TPA, ... of the atom's row, reads them from
-// the cell-ordered packed positions (spatially local), and the group sums its
-// partial forces and energies with shuffles.  lj_c12/lj_c6 hold 4 eps sig^12
-// and 4 eps sig^6 in FP32.
-template <int TPA>
-__device__ __forceinline__ void k_pair_mixed_unit(int64_t unit_, 
+// t): lane l takes neighbours Wl..Wl+W-1, Wl+W TPA.., of the atom's row, reads
+// their cell-local FP32 positions (spatially local gathers of 16 bytes) and
+// rebuilds each displacement as l_j - l_i + offset * w from the offset
+// stored in the list entry -- no FP64 and no minimum-image rounding per pair
+// when the system has at least 3 cells per edge.  The W entries of a pass
+// are summed in FP32 and the pass sums accumulate in FP64; the group sums
+// its partial forces and energies with shuffles.
+// lj_c12/lj_c6 hold 4 eps sig^12 and 4 eps sig^6 in FP32.
+template <int TPA, int W, bool COUL>
+__device__ __forceinline__ void k_pair_mixed_unit(int64_t unit_,
     const SysDev* __restrict__ sys, const SysState* __restrict__ st,
     const int* __restrict__ slot_sys, int64_t nslots, const int* __restrict__ order,
-    const int* __restrict__ nn, const int* __restrict__ nbr, const double4* __restrict__ pos4s,
+    const int* __restrict__ nn, const int* __restrict__ nbr, const float4* __restrict__ lpos,
     const int* __restrict__ typ_s, const float* __restrict__ lj_c12,
     const float* __restrict__ lj_c6, double* __restrict__ fx, double* __restrict__ fy,
     double* __restrict__ fz, double* __restrict__ e_lj_part, double* __restrict__ e_c_part,
@@ -145,7 +149,8 @@ __device__ __forceinline__ void k_pair_mixed_unit(int64_t unit_,
     if (t >= nslots) return;
     const int s = slot_sys[t];
     if (s < 0) return;
-    if (st[s].reuse || !st[s].active) return;
+    const SysState& T = st[s];
+    if (T.reuse || !T.active) return;
     const SysDev& S = sys[s];
     const int lt = (int)(t - S.atom_off);
     if (lt >= S.n) {
@@ -155,15 +160,23 @@ __device__ __forceinline__ void k_pair_mixed_unit(int64_t unit_,
         }
         return;
     }
-    const double4 pi = pos4s[t];
-    const double Lx = S.L[0], Ly = S.L[1], Lz = S.L[2];
-    const double ix = S.invL[0], iy = S.invL[1], iz = S.invL[2];
+    const float4 pi = lpos[t];
+    // cell widths of the frames (cells of the last build; a barostat scales
+    // the box, the positions and the frame origins together)
+    const float wx = (float)(S.L[0] / T.nub[0]);
+    const float wy = (float)(S.L[1] / T.nub[1]);
+    const float wz = (float)(S.L[2] / T.nub[2]);
+    const bool wrap = T.nub[0] < 3 || T.nub[1] < 3 || T.nub[2] < 3;
+    const float Lx = (float)S.L[0], Ly = (float)S.L[1], Lz = (float)S.L[2];
+    const float ix = (float)S.invL[0], iy = (float)S.invL[1], iz = (float)S.invL[2];
     const int nt = S.ntypes;
+    const bool lj = S.use_lj, cp = COUL && S.coul_pairs;
+    const bool typed = lj && nt > 1;
     const float* c12t = lj_c12 + S.lj_off;
     const float* c6t = lj_c6 + S.lj_off;
-    const bool lj = S.use_lj, cp = S.coul_pairs;
-    const int ti = lj ? typ_s[t] : 0;
-    const float qi = cp ? (float)(KE * pi.w) : 0.0f;
+    const int ti = typed ? typ_s[t] : 0;
+    const float c12_0 = c12t[0], c6_0 = c6t[0];
+    const float qi = cp ? (float)(KE * (double)pi.w) : 0.0f;
     const float alpha = (float)S.alpha;
     const float two_a_rpi = (float)(2.0 * S.alpha / sqrt(PI));
     const float a2 = (float)(S.alpha * S.alpha);
@@ -171,56 +184,81 @@ __device__ __forceinline__ void k_pair_mixed_unit(int64_t unit_,
     const int* row = nbr + S.nbr_off + (int64_t)lt * S.max_nbr;
     const int cnt = nn[t];
     double fxi = 0.0, fyi = 0.0, fzi = 0.0, ej = 0.0, ec = 0.0;
-    // rows are 16-byte aligned with M a multiple of 4: each lane takes 4
-    // consecutive entries per pass and issues their 4 gathers before any
-    // arithmetic (memory-level parallelism for the dependent loads)
-    for (int k0 = lane * 4; k0 < cnt; k0 += TPA * 4) {
-        const int4 jj = *reinterpret_cast<const int4*>(row + k0);
-        const int jv[4] = {jj.x, jj.y, jj.z, jj.w};
-        double4 pj[4];
-        int tj[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int j = (k0 + q < cnt) ? jv[q] : (int)t;
-            pj[q] = pos4s[j];
-            tj[q] = lj ? typ_s[j] : 0;
+    // rows are 16-byte aligned with M a multiple of 4: each lane takes W (2
+    // or 4) consecutive entries per pass and issues their W gathers before
+    // any arithmetic (memory-level parallelism for the dependent loads; W=2
+    // needs fewer registers, so more waves fit)
+    for (int k0 = lane * W; k0 < cnt; k0 += TPA * W) {
+        uint32_t ev[W];
+        if constexpr (W == 4) {
+            const int4 jj = *reinterpret_cast<const int4*>(row + k0);
+            ev[0] = (uint32_t)jj.x;
+            ev[1] = (uint32_t)jj.y;
+            ev[2] = (uint32_t)jj.z;
+            ev[3] = (uint32_t)jj.w;
+        } else {
+            const int2 jj = *reinterpret_cast<const int2*>(row + k0);
+            ev[0] = (uint32_t)jj.x;
+            ev[1] = (uint32_t)jj.y;
         }
+        float4 pj[W];
+        int tj[W];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < W; ++q) {
+            const int j = (k0 + q < cnt) ? (int)(ev[q] & NBR_SLOT_MASK) : (int)t;
+            pj[q] = lpos[j];
+            tj[q] = typed ? typ_s[j] : 0;
+        }
+        // the W entries of a pass are summed in FP32, then added in FP64
+        float px = 0.0f, py = 0.0f, pz = 0.0f, pel = 0.0f, pec = 0.0f;
+#pragma unroll
+        for (int q = 0; q < W; ++q) {
             if (k0 + q >= cnt) continue;
-            double ddx = pj[q].x - pi.x, ddy = pj[q].y - pi.y, ddz = pj[q].z - pi.z;
-            ddx -= rint(ddx * ix) * Lx;
-            ddy -= rint(ddy * iy) * Ly;
-            ddz -= rint(ddz * iz) * Lz;
-            const float dx = (float)ddx, dy = (float)ddy, dz = (float)ddz;
+            const uint32_t e = ev[q];
+            const float ox = (float)((int)((e >> NBR_SLOT_BITS) & 3u) - 1);
+            const float oy = (float)((int)((e >> (NBR_SLOT_BITS + 2)) & 3u) - 1);
+            const float oz = (float)((int)((e >> (NBR_SLOT_BITS + 4)) & 3u) - 1);
+            float dx = (pj[q].x - pi.x) + ox * wx;
+            float dy = (pj[q].y - pi.y) + oy * wy;
+            float dz = (pj[q].z - pi.z) + oz * wz;
+            if (wrap) {
+                dx -= rintf(dx * ix) * Lx;
+                dy -= rintf(dy * iy) * Ly;
+                dz -= rintf(dz * iz) * Lz;
+            }
             const float r2 = dx * dx + dy * dy + dz * dz;
             const float rinv2 = __builtin_amdgcn_rcpf(r2);  // 1 ulp hardware reciprocal
             float fs = 0.0f, el = 0.0f, ecl = 0.0f;
             if (lj && r2 < cut2) {
-                const float c12 = c12t[ti * nt + tj[q]], c6 = c6t[ti * nt + tj[q]];
+                const float c12 = typed ? c12t[ti * nt + tj[q]] : c12_0;
+                const float c6 = typed ? c6t[ti * nt + tj[q]] : c6_0;
                 const float r6i = rinv2 * rinv2 * rinv2;
                 const float a12 = c12 * r6i * r6i, a6 = c6 * r6i;
                 fs += (12.0f * a12 - 6.0f * a6) * rinv2;
                 el = a12 - a6;
             }
-            if (cp && r2 < ccut2) {
+            if (COUL && cp && r2 < ccut2) {
                 const float rinv = rsqrtf(r2);
                 const float r = r2 * rinv;
-                const float qq = qi * (float)pj[q].w;
+                const float qq = qi * pj[q].w;
                 const float erfc_ar = erfcf(alpha * r);
                 const float ex = __expf(-a2 * r2);
                 fs += qq * (erfc_ar * rinv + two_a_rpi * ex) * rinv2;
                 ecl = qq * erfc_ar * rinv;
             }
-            const double fsd = (double)fs;
-            fxi -= fsd * ddx;
-            fyi -= fsd * ddy;
-            fzi -= fsd * ddz;
-            if (jv[q] > t) {  // each unordered pair once
-                ej += (double)el;
-                ec += (double)ecl;
+            px += fs * dx;
+            py += fs * dy;
+            pz += fs * dz;
+            if ((int64_t)(e & NBR_SLOT_MASK) > t) {  // each unordered pair once
+                pel += el;
+                pec += ecl;
             }
         }
+        fxi -= (double)px;
+        fyi -= (double)py;
+        fzi -= (double)pz;
+        ej += (double)pel;
+        ec += (double)pec;
     }
     fxi = group_sum<TPA>(fxi);
     fyi = group_sum<TPA>(fyi);
@@ -242,22 +280,45 @@ __device__ __forceinline__ void k_pair_mixed_unit(int64_t unit_,
     }
 }
 
-template <int TPA>
+template <int TPA, int W, bool COUL>
 __global__ void __launch_bounds__(256) k_pair_mixed(
     const SysDev* __restrict__ sys, const SysState* __restrict__ st,
     const int* __restrict__ slot_sys, int64_t nslots, const int* __restrict__ order,
-    const int* __restrict__ nn, const int* __restrict__ nbr, const double4* __restrict__ pos4s,
+    const int* __restrict__ nn, const int* __restrict__ nbr, const float4* __restrict__ lpos,
     const int* __restrict__ typ_s, const float* __restrict__ lj_c12,
     const float* __restrict__ lj_c6, double* __restrict__ fx, double* __restrict__ fy,
     double* __restrict__ fz, double* __restrict__ e_lj_part, double* __restrict__ e_c_part,
     int accumulate, int64_t nunits_)
 {
-    DMDX_FOR_UNITS(u_, nunits_) k_pair_mixed_unit<TPA>(u_, sys, st, slot_sys, nslots, order, nn, nbr, pos4s, typ_s, lj_c12, lj_c6, fx, fy, fz, e_lj_part, e_c_part, accumulate);
+    DMDX_FOR_UNITS(u_, nunits_) k_pair_mixed_unit<TPA, W, COUL>(u_, sys, st, slot_sys, nslots, order, nn, nbr, lpos, typ_s, lj_c12, lj_c6, fx, fy, fz, e_lj_part, e_c_part, accumulate);
 }
 
 }  // namespace dmdx
 
 namespace dmdx {
+
+template <bool COUL>
+static hipError_t launch_pair_mixed(const Dev& d, hipStream_t s, int accumulate)
+{
+    const int BS = 256;
+    const int tpa = d.pair_tpa;
+    const int64_t units = blocks_for(d.nslots * tpa, BS);
+#define DMDX_PAIR_ARGS                                                                       \
+    d.sys, d.st, d.slot_sys, d.nslots, d.order, d.nn, d.nbr, d.lpos, d.typ_s, d.lj_c12, d.lj_c6, \
+        d.fx, d.fy, d.fz, d.p_lj, d.p_coul, accumulate, units
+    const unsigned g = grid_for(units);
+    if (d.pair_width == 2) {
+        if (tpa == 8) k_pair_mixed<8, 2, COUL><<<g, BS, 0, s>>>(DMDX_PAIR_ARGS);
+        else if (tpa == 32) k_pair_mixed<32, 2, COUL><<<g, BS, 0, s>>>(DMDX_PAIR_ARGS);
+        else k_pair_mixed<16, 2, COUL><<<g, BS, 0, s>>>(DMDX_PAIR_ARGS);
+    } else {
+        if (tpa == 8) k_pair_mixed<8, 4, COUL><<<g, BS, 0, s>>>(DMDX_PAIR_ARGS);
+        else if (tpa == 32) k_pair_mixed<32, 4, COUL><<<g, BS, 0, s>>>(DMDX_PAIR_ARGS);
+        else k_pair_mixed<16, 4, COUL><<<g, BS, 0, s>>>(DMDX_PAIR_ARGS);
+    }
+#undef DMDX_PAIR_ARGS
+    return hipGetLastError();
+}
 
 hipError_t launch_pair(const Dev& d, hipStream_t s, int accumulate)
 {
@@ -266,18 +327,10 @@ hipError_t launch_pair(const Dev& d, hipStream_t s, int accumulate)
         k_pair_fp64<<<grid_for(blocks_for(d.nslots, BS)), BS, 0, s>>>(
             d.sys, d.st, d.slot_sys, d.nslots, d.order, d.nn, d.nbr, d.x, d.y, d.z, d.q, d.type,
             d.lj_sig, d.lj_eps, d.fx, d.fy, d.fz, d.p_lj, d.p_coul, accumulate, (int64_t)(blocks_for(d.nslots, BS)));
-    else if (d.pair_tpa == 8)
-        k_pair_mixed<8><<<grid_for(blocks_for(d.nslots * 8, BS)), BS, 0, s>>>(
-            d.sys, d.st, d.slot_sys, d.nslots, d.order, d.nn, d.nbr, d.pos4s, d.typ_s, d.lj_c12,
-            d.lj_c6, d.fx, d.fy, d.fz, d.p_lj, d.p_coul, accumulate, (int64_t)(blocks_for(d.nslots * 8, BS)));
-    else if (d.pair_tpa == 32)
-        k_pair_mixed<32><<<grid_for(blocks_for(d.nslots * 32, BS)), BS, 0, s>>>(
-            d.sys, d.st, d.slot_sys, d.nslots, d.order, d.nn, d.nbr, d.pos4s, d.typ_s, d.lj_c12,
-            d.lj_c6, d.fx, d.fy, d.fz, d.p_lj, d.p_coul, accumulate, (int64_t)(blocks_for(d.nslots * 32, BS)));
+    else if (d.any_coul_pairs)
+        return launch_pair_mixed<true>(d, s, accumulate);
     else
-        k_pair_mixed<16><<<grid_for(blocks_for(d.nslots * 16, BS)), BS, 0, s>>>(
-            d.sys, d.st, d.slot_sys, d.nslots, d.order, d.nn, d.nbr, d.pos4s, d.typ_s, d.lj_c12,
-            d.lj_c6, d.fx, d.fy, d.fz, d.p_lj, d.p_coul, accumulate, (int64_t)(blocks_for(d.nslots * 16, BS)));
+        return launch_pair_mixed<false>(d, s, accumulate);
     return hipGetLastError();
 }
 

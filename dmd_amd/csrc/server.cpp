@@ -35,66 +35,6 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
-// deep copy of a system description (the library copies its inputs)
-struct DescCopy {
-    dmdx_system_desc d;
-    std::vector<std::vector<double>> dv;
-    std::vector<std::vector<int32_t>> iv;
-
-    template <class T>
-    const T* keep(std::vector<std::vector<T>>& store, const T* p, int64_t n)
-    {
-        if (!p || n <= 0) return nullptr;
-        store.emplace_back(p, p + n);
-        return store.back().data();
-    }
-
-    explicit DescCopy(const dmdx_system_desc& s) : d(s)
-    {
-        const int64_t n = s.n_atoms;
-        dv.reserve(32);
-        iv.reserve(32);
-        d.pos_x = keep(dv, s.pos_x, n);
-        d.pos_y = keep(dv, s.pos_y, n);
-        d.pos_z = keep(dv, s.pos_z, n);
-        d.vel_x = keep(dv, s.vel_x, n);
-        d.vel_y = keep(dv, s.vel_y, n);
-        d.vel_z = keep(dv, s.vel_z, n);
-        d.masses = keep(dv, s.masses, n);
-        d.charges = keep(dv, s.charges, n);
-        d.atom_types = keep(iv, s.atom_types, n);
-        const int64_t t2 = (int64_t)s.lj_n_types * s.lj_n_types;
-        d.lj_sigma = keep(dv, s.lj_sigma, t2);
-        d.lj_epsilon = keep(dv, s.lj_epsilon, t2);
-        d.bond_i = keep(iv, s.bond_i, s.n_bonds);
-        d.bond_j = keep(iv, s.bond_j, s.n_bonds);
-        d.bond_k = keep(dv, s.bond_k, s.n_bonds);
-        d.bond_r0 = keep(dv, s.bond_r0, s.n_bonds);
-        d.angle_i = keep(iv, s.angle_i, s.n_angles);
-        d.angle_j = keep(iv, s.angle_j, s.n_angles);
-        d.angle_k = keep(iv, s.angle_k, s.n_angles);
-        d.angle_k_theta = keep(dv, s.angle_k_theta, s.n_angles);
-        d.angle_theta0 = keep(dv, s.angle_theta0, s.n_angles);
-        d.dih_i = keep(iv, s.dih_i, s.n_dihedrals);
-        d.dih_j = keep(iv, s.dih_j, s.n_dihedrals);
-        d.dih_k = keep(iv, s.dih_k, s.n_dihedrals);
-        d.dih_l = keep(iv, s.dih_l, s.n_dihedrals);
-        d.dih_periodicity = keep(iv, s.dih_periodicity, s.n_dihedrals);
-        d.dih_k_phi = keep(dv, s.dih_k_phi, s.n_dihedrals);
-        d.dih_phi0 = keep(dv, s.dih_phi0, s.n_dihedrals);
-        d.imp_i = keep(iv, s.imp_i, s.n_impropers);
-        d.imp_j = keep(iv, s.imp_j, s.n_impropers);
-        d.imp_k = keep(iv, s.imp_k, s.n_impropers);
-        d.imp_l = keep(iv, s.imp_l, s.n_impropers);
-        d.imp_k_phi = keep(dv, s.imp_k_phi, s.n_impropers);
-        d.imp_phi0 = keep(dv, s.imp_phi0, s.n_impropers);
-        d.excl_i = keep(iv, s.excl_i, s.n_excl);
-        d.excl_j = keep(iv, s.excl_j, s.n_excl);
-    }
-    DescCopy(const DescCopy&) = delete;
-    DescCopy& operator=(const DescCopy&) = delete;
-};
-
 // Device bytes a system will hold in a batch: per-slot state and scratch,
 // the neighbour list at the width the batch will choose, PME grids and
 // bonded term buffers.  An estimate; the server scales it by the largest
@@ -133,7 +73,7 @@ int64_t estimate_bytes(const dmdx_system_desc& D)
 
 struct Request {
     int64_t id = 0;
-    std::unique_ptr<DescCopy> desc;
+    std::unique_ptr<dmdx::DescCopy> desc;
     int64_t n_steps = 0;
     int32_t state = DMDX_REQ_QUEUED;
     int32_t flags = 0;
@@ -395,7 +335,7 @@ int dmdx_server_submit(dmdx_server* srv, const dmdx_system_desc* desc, int64_t n
         return dmdx::fail(DMDX_E_INVALID, "unknown precision");
     Request r;
     r.id = srv->next_id++;
-    r.desc = std::make_unique<DescCopy>(*desc);
+    r.desc = std::make_unique<dmdx::DescCopy>(*desc);
     r.n_steps = n_steps;
     r.t_submit = srv->now();
     r.est_bytes = estimate_bytes(*desc);

@@ -1,0 +1,230 @@
+"""DMD's Python core API (module ``_dmd_core``) over the device engine.
+
+DMD's runner does ``from _dmd_core import SimulationConfig, Engine``
+(python/dmd/runner.py:3) and drives ``Engine.run_n`` (runner.py:46-58).
+This module offers the same names with the same attributes -- the
+``SimulationConfig`` fields bound in python/dmd/core.cpp:106-410 and the
+``Engine`` surface of core.cpp:415-427 -- so the runner, the system builder
+and the validation scripts can drive the MI355X path by importing from here
+instead.  Differences, all explicit:
+
+* ``gen_vel``: DMD draws with std::mt19937 + std::normal_distribution, whose
+  output is implementation-defined (simulation_engine.cpp:339-383); here the
+  velocities are drawn with a seeded Philox generator, COM removed and
+  rescaled to ``init_temperature`` with 3N-3 degrees of freedom, as DMD does.
+* ``constraint_type = "shake"`` and the Andersen thermostat are not on the
+  device and raise ``NotImplementedError``.
+* trajectory and energy files (``nstxout``, H5MD) are not written; the
+  state is read through the properties.
+* ``precision`` (``"fp64"`` or ``"mixed"``) and ``physics`` (``"faithful"``
+  reproduces DMD's arithmetic including its quirks, ``"corrected"`` the
+  textbook forms) are extra attributes; the defaults reproduce DMD.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from . import _abi
+from .system import KB, SystemSpec
+
+
+class ThermostatConfig:
+    """src/sim/simulation_config.h:13-18"""
+
+    def __init__(self):
+        self.type = "none"
+        self.temperature = 300.0
+        self.tau = 0.1
+        self.frequency = 10.0
+
+
+class BarostatConfig:
+    """src/sim/simulation_config.h:20-25"""
+
+    def __init__(self):
+        self.type = "none"
+        self.pressure = 1.0
+        self.tau = 1.0
+        self.compressibility = 4.5e-5
+
+
+_ARRAYS_F = ("mass", "charges", "pos_x", "pos_y", "pos_z", "vel_x", "vel_y", "vel_z",
+             "lj_sigma", "lj_epsilon", "bond_k", "bond_r0", "angle_k_theta", "angle_theta0",
+             "dihedral_k_phi", "dihedral_phi0", "improper_k_phi", "improper_phi0",
+             "constraint_dist")
+_ARRAYS_I = ("atom_types", "bond_i", "bond_j", "angle_i", "angle_j", "angle_k",
+             "dihedral_i", "dihedral_j", "dihedral_k", "dihedral_l", "dihedral_periodicity",
+             "improper_i", "improper_j", "improper_k", "improper_l", "excl_i", "excl_j",
+             "constraint_i", "constraint_j")
+
+
+class SimulationConfig:
+    """Field names and defaults of SimulationConfig (simulation_config.h:27-84)
+    as bound by python/dmd/core.cpp; array fields are numpy arrays."""
+
+    def __init__(self):
+        self.dt = 0.002
+        self.n_steps = 1000
+        self.init_temperature = 300.0
+        self.gen_vel = False
+        self.seed = 42
+        self.box_size = 3.0
+        self.use_lj = False
+        self.lj_cutoff = 1.2
+        self.coulomb_type = "cutoff"
+        self.coulomb_cutoff = 1.2
+        self.pme_order = 4
+        self.pme_grid_spacing = 0.12
+        self.ewald_coeff = 0.34
+        self.trajectory_path = "trajectory.h5"
+        self.nstxout = 0
+        self.nstvout = 0
+        self.nstenergy = 10
+        self.energy_path = "energy.h5"
+        self.checkpoint_interval = 500
+        self.checkpoint_path = "checkpoint.bin"
+        self.thermostat = ThermostatConfig()
+        self.barostat = BarostatConfig()
+        self.constraint_type = "none"
+        self.constraint_tolerance = 1e-6
+        for a in _ARRAYS_F:
+            setattr(self, a, np.zeros(0))
+        for a in _ARRAYS_I:
+            setattr(self, a, np.zeros(0, dtype=np.int32))
+        # device-path options (not in DMD)
+        self.precision = "fp64"
+        self.physics = "faithful"
+
+    def __setattr__(self, k, v):
+        if k in _ARRAYS_F:
+            v = np.asarray(v, dtype=np.float64).ravel()
+        elif k in _ARRAYS_I:
+            v = np.asarray(v, dtype=np.int32).ravel()
+        object.__setattr__(self, k, v)
+
+
+def _velocities(cfg, masses):
+    """gen_vel: Maxwell-Boltzmann at init_temperature, COM removed, rescaled
+    with 3N-3 degrees of freedom (as simulation_engine.cpp:339-383)."""
+    n = len(masses)
+    rng = np.random.Generator(np.random.Philox(key=int(cfg.seed)))
+    sig = np.sqrt(KB * cfg.init_temperature / masses)
+    v = rng.standard_normal((n, 3)) * sig[:, None]
+    v -= (masses[:, None] * v).sum(0) / masses.sum()
+    ke = 0.5 * (masses[:, None] * v * v).sum()
+    if ke > 0 and n > 1:
+        v *= math.sqrt(cfg.init_temperature / (2.0 * ke / ((3 * n - 3) * KB)))
+    return v
+
+
+def spec_from_config(cfg: SimulationConfig) -> SystemSpec:
+    """build_simulation (simulation_engine.cpp:248-386) as a SystemSpec."""
+    if cfg.constraint_type == "shake" and len(cfg.constraint_i):
+        raise NotImplementedError("SHAKE/RATTLE constraints are not on the device path")
+    thermo = {"none": _abi.THERMO_NONE, "berendsen": _abi.THERMO_BERENDSEN,
+              "nose-hoover": _abi.THERMO_NOSE_HOOVER}
+    baro = {"none": _abi.BARO_NONE, "berendsen": _abi.BARO_BERENDSEN,
+            "andersen": _abi.BARO_ANDERSEN}
+    if cfg.thermostat.type not in thermo:
+        raise NotImplementedError(f"thermostat {cfg.thermostat.type!r} is not on the device path")
+    if cfg.barostat.type not in baro:
+        raise ValueError(f"Unknown barostat type: {cfg.barostat.type}")
+    n = len(cfg.pos_x)
+    pos = np.stack([cfg.pos_x, cfg.pos_y, cfg.pos_z], 1)
+    masses = np.asarray(cfg.mass, dtype=np.float64)
+    if cfg.gen_vel and n > 0:
+        vel = _velocities(cfg, masses)
+    elif len(cfg.vel_x) == n:
+        vel = np.stack([cfg.vel_x, cfg.vel_y, cfg.vel_z], 1)
+    else:
+        vel = np.zeros((n, 3))
+    nt = int(round(math.sqrt(len(cfg.lj_sigma)))) if len(cfg.lj_sigma) else 1
+    types = cfg.atom_types if len(cfg.atom_types) else np.zeros(n, np.int32)
+    charges = cfg.charges if len(cfg.charges) else np.zeros(n)
+    L = float(cfg.box_size)
+    kw = {}
+    if len(cfg.bond_i):
+        kw["bonds"] = (cfg.bond_i, cfg.bond_j, cfg.bond_k, cfg.bond_r0)
+    if len(cfg.angle_i):
+        kw["angles"] = (cfg.angle_i, cfg.angle_j, cfg.angle_k, cfg.angle_k_theta,
+                        cfg.angle_theta0)
+    if len(cfg.dihedral_i):
+        kw["dihedrals"] = (cfg.dihedral_i, cfg.dihedral_j, cfg.dihedral_k, cfg.dihedral_l,
+                           cfg.dihedral_periodicity, cfg.dihedral_k_phi, cfg.dihedral_phi0)
+    if len(cfg.improper_i):
+        kw["impropers"] = (cfg.improper_i, cfg.improper_j, cfg.improper_k, cfg.improper_l,
+                           cfg.improper_k_phi, cfg.improper_phi0)
+    if len(cfg.excl_i):
+        kw["exclusions"] = (cfg.excl_i, cfg.excl_j)
+    ct = cfg.coulomb_type
+    coul = (_abi.COULOMB_PME if ct == "pme" else
+            _abi.COULOMB_CUTOFF if ct in ("direct", "cutoff") else _abi.COULOMB_NONE)
+    g = max(8, int(L / cfg.pme_grid_spacing + 0.5))  # simulation_engine.cpp:183-185
+    return SystemSpec(
+        pos=pos, vel=vel, masses=masses, box=(L, L, L), charges=charges, atom_types=types,
+        use_lj=bool(cfg.use_lj),
+        lj_sigma=np.asarray(cfg.lj_sigma).reshape(nt, nt) if len(cfg.lj_sigma) else None,
+        lj_epsilon=np.asarray(cfg.lj_epsilon).reshape(nt, nt) if len(cfg.lj_epsilon) else None,
+        lj_cutoff=float(cfg.lj_cutoff), list_skin=0.0, rebuild_interval=10,
+        step_since_rebuild=10, coulomb_type=coul, coulomb_cutoff=float(cfg.coulomb_cutoff),
+        ewald_coeff=float(cfg.ewald_coeff), pme_grid=(g, g, g), pme_order=int(cfg.pme_order),
+        dt=float(cfg.dt), thermostat=thermo[cfg.thermostat.type],
+        thermo_temperature=float(cfg.thermostat.temperature),
+        thermo_tau=float(cfg.thermostat.tau), barostat=baro[cfg.barostat.type],
+        baro_pressure=float(cfg.barostat.pressure), baro_tau=float(cfg.barostat.tau),
+        baro_compressibility=float(cfg.barostat.compressibility),
+        physics=_abi.CORRECTED if cfg.physics == "corrected" else _abi.FAITHFUL,
+        precision=_abi.MIXED if cfg.precision == "mixed" else _abi.FP64, **kw)
+
+
+class Engine:
+    """EngineWrapper of python/dmd/core.cpp:15-93 on the device."""
+
+    def __init__(self, cfg: SimulationConfig):
+        from .engine import Engine as _Engine
+        self._spec = spec_from_config(cfg)
+        self._e = _Engine(self._spec)
+        self._cfg = cfg
+        self._step = 0
+
+    def run(self):
+        self.run_n(int(self._cfg.n_steps))
+
+    def run_n(self, steps: int):
+        self._e.step(int(steps))
+        self._step += int(steps)
+
+    @property
+    def current_step(self) -> int:
+        return self._step
+
+    @property
+    def current_time(self) -> float:
+        return self._step * float(self._cfg.dt)
+
+    @property
+    def positions(self):
+        return self._e.positions()
+
+    @property
+    def velocities(self):
+        return self._e.velocities()
+
+    @property
+    def forces(self):
+        return self._e.forces()
+
+    @property
+    def box_size(self) -> float:
+        # the configured edge, as EngineWrapper::box_size (core.cpp:22-24,46)
+        return float(self._cfg.box_size)
+
+    @property
+    def n_atoms(self) -> int:
+        return self._spec.n
+
+    @property
+    def potential_energy(self) -> float:
+        return self._e.scalars()["pe"]

@@ -51,6 +51,9 @@ def _setup(lib):
                                             C.POINTER(C.c_int64)]
     lib.dmdx_copy_bandwidth.argtypes = [C.c_int64, C.c_int32, C.POINTER(C.c_double)]
     lib.dmdx_engine_set_tuning.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
+    lib.dmdx_engine_autotune.argtypes = [C.c_void_p, C.c_int32, C.c_char_p,
+                                         C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                         C.POINTER(C.c_int32), C.POINTER(C.c_double)]
     lib._engine_ready = True
 
 
@@ -154,11 +157,29 @@ class Engine:
     def set_active(self, sys: int, active: bool):
         _abi.check(self.lib.dmdx_engine_set_active(self.h, sys, int(active)))
 
-    def set_tuning(self, pair_lanes: int | None = None, build_lanes: int | None = None):
+    def set_tuning(self, pair_lanes: int | None = None, build_lanes: int | None = None,
+                   pair_width: int | None = None):
         if pair_lanes is not None:
             _abi.check(self.lib.dmdx_engine_set_tuning(self.h, 0, int(pair_lanes)))
         if build_lanes is not None:
             _abi.check(self.lib.dmdx_engine_set_tuning(self.h, 1, int(build_lanes)))
+        if pair_width is not None:
+            _abi.check(self.lib.dmdx_engine_set_tuning(self.h, 2, int(pair_width)))
+
+    def autotune(self, steps: int = 5, cache: str | None = None) -> dict:
+        """Time the launch shapes of the mixed pair kernel (8/16/32 lanes x
+        2/4 entries per pass) and list build (8/16/32 lanes) on a scratch
+        copy of the systems and apply the fastest (dmdx_engine_autotune).
+        `cache`: text file of earlier results."""
+        p, w, b = C.c_int32(), C.c_int32(), C.c_int32()
+        t = (C.c_double * 9)()
+        _abi.check(self.lib.dmdx_engine_autotune(
+            self.h, int(steps), cache.encode() if cache else None, C.byref(p), C.byref(w),
+            C.byref(b), t))
+        shapes = ("8x2", "8x4", "16x2", "16x4", "32x2", "32x4")
+        return {"pair_lanes": p.value, "pair_width": w.value, "build_lanes": b.value,
+                "pair_ms": {k: t[i] for i, k in enumerate(shapes)},
+                "build_ms": {8: t[6], 16: t[7], 32: t[8]}}
 
     def set_profiling(self, on: bool):
         _abi.check(self.lib.dmdx_engine_set_profiling(self.h, int(on)))

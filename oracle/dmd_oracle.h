@@ -10,6 +10,34 @@
  * compiled with -ffp-contract=off so that, in FAITHFUL mode, its results are
  * bit-identical to the reference translation units compiled the same way.
  * CORRECTED mode swaps in textbook formulas where DMD is physically wrong.
+ *
+ * How each function is pinned (tests/test_oracle_*.py):
+ *   min_image, cell_volume      DMD's test_cell.cpp values; half-away rounding
+ *   pairs_n2 / pairs_cells      O(N^2) brute force = DMD's algorithm; the two
+ *                               agree as sorted sets; DMD's own build (_ref)
+ *   lj, coulomb pairs, exclusion,
+ *   bonds, angles, dihedrals,
+ *   impropers                   bit-exact against DMD's translation units
+ *                               (oracle/_ref) in FAITHFUL mode; DMD's unit
+ *                               test values; finite differences of the
+ *                               energy (CORRECTED) and the documented
+ *                               anti-gradient quirks (FAITHFUL)
+ *   theta4, dtheta4, self energy
+ *                               DMD's test_coulomb_pme.cpp values; closed form
+ *   fft3d                       numpy.fft on random data
+ *   pme (CORRECTED)             plain Ewald sum (order-4 convergence), NaCl
+ *                               Madelung constant, finite differences
+ *   pme (FAITHFUL)              parity partially pinned: forces are exactly
+ *                               -grad(E)/(nx ny nz) (finite differences); the
+ *                               multiplied B-spline modulus is pinned only as
+ *                               "farther from Ewald than CORRECTED"; DMD's
+ *                               padded-grid indexing cannot be reproduced and
+ *                               is replaced by a consistent layout
+ *   ewald                       Madelung constant, finite differences
+ *   integrator, KE, T, thermostats, virial, pressure, barostats,
+ *   sim_* (step sequence)       bit-exact against DMD's translation units
+ *                               (_ref); closed forms; DMD's thermostat and
+ *                               barostat unit-test values
  */
 #ifndef DMD_ORACLE_H
 #define DMD_ORACLE_H

@@ -1,0 +1,48 @@
+"""Launch-shape autotuner (dmdx_engine_autotune): picks one of the measured
+candidates for each kernel, leaves the engine's state alone, caches its
+choice, and the tuned engine still meets the mixed-precision force bar."""
+import numpy as np
+import pytest
+
+from dmd_amd import _abi
+from dmd_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def test_autotune_choice_cache_and_state(tmp_path):
+    from dmd_amd.engine import Engine
+    specs = W.replica_mix(n_replicas=12, n_min=500, n_max=3000, seed=7, precision=_abi.MIXED)
+    e = Engine(specs)
+    e.step(3)
+    before = [e.positions(k) for k in range(len(specs))]
+    cache = str(tmp_path / "tune.txt")
+    r = e.autotune(steps=3, cache=cache)
+    print(r, flush=True)
+    assert r["pair_lanes"] in (8, 16, 32) and r["build_lanes"] in (8, 16, 32)
+    assert r["pair_width"] in (2, 4)
+    pm = r["pair_ms"]
+    assert all(v > 0 for v in pm.values())
+    assert pm[f"{r['pair_lanes']}x{r['pair_width']}"] == min(pm.values())
+    bm = r["build_ms"]
+    assert bm[r["build_lanes"]] == min(bm.values())
+    # the engine's own systems did not move
+    for k in range(len(specs)):
+        assert np.array_equal(e.positions(k), before[k])
+    # a second engine on the same systems hits the cache
+    e2 = Engine(specs)
+    r2 = e2.autotune(steps=3, cache=cache)
+    assert (r2["pair_lanes"], r2["pair_width"], r2["build_lanes"]) == \
+        (r["pair_lanes"], r["pair_width"], r["build_lanes"])
+    assert all(v == -1.0 for v in r2["pair_ms"].values())
+    # forces of the tuned engine against the FP64 path
+    e.compute()
+    f64 = Engine([s.with_(precision=_abi.FP64) for s in specs])
+    e3 = Engine(specs)
+    e3.autotune(steps=1, cache=cache)
+    e3.compute()
+    f64.compute()
+    for k in range(len(specs)):
+        a, b = f64.forces(k), e3.forces(k)
+        err = float(np.sqrt(((b - a) ** 2).sum() / (a ** 2).sum()))
+        assert err <= 1e-5, (k, err)
