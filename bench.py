@@ -105,7 +105,7 @@ def run_batched(specs, steps, warmup, latency_steps):
     from dmd_amd.engine import Engine
     mixed = [s.with_(precision=_abi.MIXED) for s in specs]
     eng = Engine(mixed, reuse_forces=True, use_graph=False)
-    tune = eng.autotune(steps=5, cache=os.environ.get("DMDX_TUNE_CACHE"))
+    tune = eng.autotune(steps=10, cache=os.environ.get("DMDX_TUNE_CACHE"))
     log(f"[bench] autotune: {tune}")
     eng.step(warmup)
     eng.sync()

@@ -4,11 +4,13 @@
 // group of 8, 16 or 32 lanes, and the pair kernel takes 2 or 4 list entries
 // per lane per pass (4 hides more gather latency per wave, 2 needs fewer
 // registers so more waves fit).  Which is fastest depends on the systems (how
-// many neighbours an atom has, how many atoms a batch holds), so the choice
-// is measured: a scratch copy of the engine's batch, set to rebuild its lists
-// at every compute, is stepped with each candidate and the per-launch times
-// of the two kernels are read from HIP events.  The engine's own state is
-// not touched.  Results are cached in a text file keyed by a signature of the
+// many neighbours an atom has, how many atoms a batch holds, how many rebuild
+// at once), so the choice is measured: a scratch copy of the engine's batch
+// is stepped with each candidate and the times of the two kernel groups are
+// read from HIP events -- per launch for the pair kernel, per step for the
+// list build, whose candidates take turns step by step so that each sees the
+// same mix of full and partial rebuilds.  The engine's own state is not
+// touched.  Results are cached in a text file keyed by a signature of the
 // systems, so a workload is timed once.  There is no counterpart in the
 // reference (its loops are serial).
 #include <cmath>
@@ -128,40 +130,46 @@ extern "C" int dmdx_engine_autotune(dmdx_engine* e, int32_t steps, const char* c
         report();
         return DMDX_OK;
     }
-    // scratch batch: the same systems, lists rebuilt at every compute
+    // scratch batch: the same systems from their initial state, stepped as
+    // the engine would step them (lists rebuilt when the systems' own rule
+    // says so, i.e. mostly partial rebuilds)
     std::vector<dmdx_system_desc> ds;
-    for (const auto& dc : e->descs) {
-        dmdx_system_desc D = dc->d;
-        D.rebuild_interval = 1;
-        ds.push_back(D);
-    }
+    for (const auto& dc : e->descs) ds.push_back(dc->d);
     dmdx::BatchOptions bo;
     bo.profile = true;
-    bo.reuse_forces = false;
+    bo.reuse_forces = e->b.options.reuse_forces;
     dmdx::Batch scratch;
     int rc = scratch.create(ds.data(), (int)ds.size(), bo);
     if (rc) return rc;
+    scratch.d.pair_width = e->b.d.pair_width;
     const int n = steps > 0 ? steps : 5;
+    double ms0, ms1;
+    int64_t n0, n1;
+    // pair kernel: per-launch time of each shape over n steps
     for (int c = 0; c < 6; ++c) {
         scratch.d.pair_tpa = pl[c];
         scratch.d.pair_width = pw[c];
-        scratch.d.build_tpa = bl[c % 3];
         rc = scratch.step(1);  // first launches of this shape
-        if (rc) return rc;
-        double p0, l0;
-        int64_t np0, nl0;
-        rc = scratch.kernel_time(dmdx::KG_PAIR, &p0, &np0);
-        if (!rc) rc = scratch.kernel_time(dmdx::KG_LIST, &l0, &nl0);
+        if (!rc) rc = scratch.kernel_time(dmdx::KG_PAIR, &ms0, &n0);
         if (!rc) rc = scratch.step(n);
-        double p1, l1;
-        int64_t np1, nl1;
-        if (!rc) rc = scratch.kernel_time(dmdx::KG_PAIR, &p1, &np1);
-        if (!rc) rc = scratch.kernel_time(dmdx::KG_LIST, &l1, &nl1);
+        if (!rc) rc = scratch.kernel_time(dmdx::KG_PAIR, &ms1, &n1);
         if (rc) return rc;
-        t[c] = (p1 - p0) / (double)std::max<int64_t>(1, np1 - np0);
-        const double tb = (l1 - l0) / (double)std::max<int64_t>(1, nl1 - nl0);
-        t[6 + c % 3] = t[6 + c % 3] < 0.0 ? tb : std::min(t[6 + c % 3], tb);
+        t[c] = (ms1 - ms0) / (double)std::max<int64_t>(1, n1 - n0);
     }
+    // list build: the candidates take turns step by step, so each sees the
+    // same mix of full and partial rebuilds; time per step of the list group
+    double acc[3] = {0.0, 0.0, 0.0};
+    rc = scratch.kernel_time(dmdx::KG_LIST, &ms0, &n0);
+    if (rc) return rc;
+    for (int k = 0; k < 3 * n; ++k) {
+        scratch.d.build_tpa = bl[k % 3];
+        rc = scratch.step(1);
+        if (!rc) rc = scratch.kernel_time(dmdx::KG_LIST, &ms1, &n1);
+        if (rc) return rc;
+        acc[k % 3] += ms1 - ms0;
+        ms0 = ms1;
+    }
+    for (int c = 0; c < 3; ++c) t[6 + c] = acc[c] / n;
     int bp = 0, bb = 0;
     for (int c = 1; c < 6; ++c)
         if (t[c] < t[bp]) bp = c;

@@ -503,25 +503,37 @@ __device__ __forceinline__ void k_nbr_build_mixed_unit(int64_t unit_,
     for (int oz = loz; oz <= hiz; ++oz) {
         int zz = cz + oz;
         zz = zz < 0 ? zz + nz : (zz >= nz ? zz - nz : zz);
+        const float sz = oz * wz - li.z;
         for (int oy = loy; oy <= hiy; ++oy) {
             int yy = cy + oy;
             yy = yy < 0 ? yy + ny : (yy >= ny ? yy - ny : yy);
-            for (int ox = lox; ox <= hix; ++ox) {
+            const float sy = oy * wy - li.y;
+            const int rowc = S.cell_off + (zz * ny + yy) * nx;
+            // the x neighbours of this (y, z) row of cells, in the order
+            // x-1, x, x+1; runs of consecutive cell indices hold their atoms
+            // contiguously in cell order, so each run is scanned as one range
+            // (same candidates in the same order as cell by cell)
+            int ox = lox;
+            while (ox <= hix) {
                 int xx = cx + ox;
                 xx = xx < 0 ? xx + nx : (xx >= nx ? xx - nx : xx);
-                const int cell = S.cell_off + (zz * ny + yy) * nx + xx;
-                const int64_t base = S.atom_off + cstart[cell];
-                const int cnt = ccount[cell];
-                const uint32_t shift = ((uint32_t)(ox + 1) << NBR_SLOT_BITS) |
-                                       ((uint32_t)(oy + 1) << (NBR_SLOT_BITS + 2)) |
-                                       ((uint32_t)(oz + 1) << (NBR_SLOT_BITS + 4));
-                const float sx = ox * wx - li.x, sy = oy * wy - li.y, sz = oz * wz - li.z;
+                int len = 1;
+                while (ox + len <= hix && xx + len < nx) ++len;
+                const int c0 = rowc + xx;
+                const int64_t base = S.atom_off + cstart[c0];
+                const int e1 = ccount[c0];
+                const int e2 = len > 1 ? e1 + ccount[c0 + 1] : 0x7fffffff;
+                const int cnt = len > 2 ? e2 + ccount[c0 + 2] : (len > 1 ? e2 : e1);
+                const int ox0 = ox;
+                const uint32_t syz = ((uint32_t)(oy + 1) << (NBR_SLOT_BITS + 2)) |
+                                     ((uint32_t)(oz + 1) << (NBR_SLOT_BITS + 4));
                 for (int m0 = 0; m0 < cnt; m0 += TPA) {
                     const int m = m0 + lane;
+                    const int oxm = ox0 + (m >= e1 ? 1 : 0) + (m >= e2 ? 1 : 0);
                     bool hit = false;
                     if (m < cnt && base + m != t) {
                         const float4 lj = pos4f[base + m];
-                        float dx = lj.x + sx, dy = lj.y + sy, dz = lj.z + sz;
+                        float dx = lj.x + (oxm * wx - li.x), dy = lj.y + sy, dz = lj.z + sz;
                         if (wrap) {
                             dx -= rintf(dx * (float)S.invL[0]) * (float)S.L[0];
                             dy -= rintf(dy * (float)S.invL[1]) * (float)S.L[1];
@@ -545,9 +557,12 @@ __device__ __forceinline__ void k_nbr_build_mixed_unit(int64_t unit_,
                     const uint64_t bal = __ballot(hit);
                     const uint64_t mine = (bal >> gshift) & (TPA == 64 ? ~0ull : ((1ull << TPA) - 1));
                     const int pos = k + __popcll(mine & below);
-                    if (hit && pos < M) row[pos] = (int)((uint32_t)(base + m) | shift);
+                    if (hit && pos < M)
+                        row[pos] = (int)((uint32_t)(base + m) | syz |
+                                         ((uint32_t)(oxm + 1) << NBR_SLOT_BITS));
                     k += __popcll(mine);
                 }
+                ox += len;
             }
         }
     }

@@ -297,13 +297,14 @@ int dmdx_engine_set_tuning(dmdx_engine *e, int32_t key, int32_t value);
 /* Autotuner (no counterpart in the reference): times the launch shapes of
  * the mixed pair kernel (8/16/32 lanes per atom x 2/4 list entries per lane
  * per pass) and of the list build (8/16/32 lanes per atom) on a scratch copy
- * of the engine's systems (lists rebuilt at every compute; the engine's own
- * state is untouched), `steps` steps per candidate, and applies the fastest
- * of each.  cache_path (may be NULL): text file of earlier results keyed by
- * a signature of the systems; a hit applies the cached choice without
- * timing.  times_ms (9 doubles, may be NULL): per-launch ms of the pair
- * kernel at 8x2, 8x4, 16x2, 16x4, 32x2, 32x4, then of the list build at 8,
- * 16, 32 lanes (-1 when not timed).  FP64 engines have nothing to tune.
+ * of the engine's systems stepped from their initial state (the engine's
+ * own state is untouched), `steps` steps per candidate, and applies the
+ * fastest of each.  cache_path (may be NULL): text file of earlier results
+ * keyed by a signature of the systems; a hit applies the cached choice
+ * without timing.  times_ms (9 doubles, may be NULL): per-launch ms of the
+ * pair kernel at 8x2, 8x4, 16x2, 16x4, 32x2, 32x4, then per-step ms of the
+ * list build at 8, 16, 32 lanes (-1 when not timed).  FP64 engines have
+ * nothing to tune.
  * Mixed-precision results depend on the shape only through FP32 summation
  * order (well inside the mixed-precision tolerance). */
 int dmdx_engine_autotune(dmdx_engine *e, int32_t steps, const char *cache_path,
