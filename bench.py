@@ -202,6 +202,7 @@ def run_serving(args):
     216-8192 waters with SPME; 1000-20000 steps log-uniform) served by
     dmdx_server: admission by device memory, batched cohorts, retirement at
     each request's step target."""
+    from dmd_amd import _abi
     from dmd_amd import workloads as W
     from dmd_amd.server import Server, run_poisson
     reqs = W.serving_requests(args.requests, seed=args.seed, steps_scale=args.steps_scale)
@@ -210,6 +211,14 @@ def run_serving(args):
     srv = Server(chunk_steps=args.chunk, max_cohorts=args.cohorts, graph_steps=args.graph_steps)
     rep = run_poisson(srv, reqs, rate_per_s=args.rate, seed=args.seed, log=log)
     srv.close()
+    # the plain path on the same request mix: each request alone, FP64,
+    # eager, a bounded number of steps (throughput per request is flat in
+    # the step count)
+    plain_value, used = None, 0
+    if not args.no_plain:
+        log("[bench] serving: plain one-system-at-a-time FP64 GPU path on the same requests")
+        plain_value, used = run_plain([s.with_(precision=_abi.FP64) for s, _ in reqs],
+                                      args.plain_steps, args.plain_budget)
     out = {
         "metric": "serving: aggregate atom-steps/s under Poisson arrivals; p50/p99 per-step "
                   "and completion latency",
@@ -225,6 +234,11 @@ def run_serving(args):
                    "graph_steps": args.graph_steps},
         "serving": rep,
     }
+    if plain_value:
+        out["plain_gpu"] = {"value": plain_value, "unit": "atom-steps/s",
+                            "sample": f"first {used} requests, each alone, {args.plain_steps} "
+                                      "steps, FP64, eager"}
+        out["speedup_vs_plain"] = rep["atom_steps_per_s"] / plain_value
     print(json.dumps(out))
 
 

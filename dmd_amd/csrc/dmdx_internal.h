@@ -78,6 +78,7 @@ struct SysDev {
     int32_t ex_mode;     // exclusion term active (faithful / corrected form)
     // PME
     int32_t pme;
+    int32_t ginf_static;  // mixed path: influence function tabulated (no barostat)
     int32_t K[3];
     int64_t grid_off;    // complex elements
     int64_t tw_off[3];   // twiddle table offsets (complex elements)
@@ -215,6 +216,7 @@ struct Dev {
     unsigned long long* qgrid = nullptr;  // charge spreading, fixed point (QGRID_SCALE)
     int* grid_sys = nullptr;
     void* tw = nullptr;  // twiddles, same real type
+    float* ginf = nullptr;  // mixed path: tabulated influence function (fixed boxes)
     double* bmod = nullptr;
     FftPlan* plans = nullptr;
     FftTile* tiles[3] = {nullptr, nullptr, nullptr};
@@ -246,6 +248,8 @@ hipError_t launch_kick_ke(const Dev& d, hipStream_t s);
 hipError_t launch_finalize(const Dev& d, hipStream_t s, int with_kinetic);
 hipError_t launch_barostat(const Dev& d, hipStream_t s);
 hipError_t launch_materialize(const Dev& d, hipStream_t s);
+hipError_t launch_remap_rows(hipStream_t s, int* nbr, int64_t count, int* order, int64_t n_order,
+                             int64_t delta, int mixed);
 
 inline unsigned blocks_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 

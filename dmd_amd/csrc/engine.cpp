@@ -519,7 +519,12 @@ int Batch::create(const dmdx_system_desc* descs, int nsys, const BatchOptions& o
         for (int k = 0; k < 3; ++k) {
             int K = S.K[k];
             int nlines = (k == 0) ? S.K[1] * S.K[2] : (k == 1 ? S.K[0] * S.K[2] : S.K[0] * S.K[1]);
-            int B = (int)std::max<size_t>(1, std::min<size_t>(32, (48 * 1024) / (2 * K * cx)));
+            // lines per workgroup: enough workgroups to spread a small grid
+            // over the chip (about nlines/256, a power of two in 4..32),
+            // within 48 KiB of LDS for the two ping-pong buffers
+            int B = 4;
+            while (B < 32 && 2 * B * 256 <= nlines) B *= 2;
+            B = (int)std::max<size_t>(1, std::min<size_t>((size_t)B, (48 * 1024) / (2 * K * cx)));
             int local = 0;
             for (int l0 = 0; l0 < nlines; l0 += B) {
                 FftTile t;
@@ -537,6 +542,41 @@ int Batch::create(const dmdx_system_desc* descs, int nsys, const BatchOptions& o
         }
     }
     for (int k = 0; k < 3; ++k) d.ntiles[k] = (int)tiles[k].size();
+    // mixed path: the influence function of a fixed box, tabulated once
+    // (same expression as the on-the-fly FP64 form in k_fft_pass)
+    std::vector<float> ginf;
+    if (prec != DMDX_FP64) {
+        ginf.assign((size_t)std::max<int64_t>(grid_total, 1), 0.0f);
+        for (int s = 0; s < nsys; ++s) {
+            SysDev& S = hsys[s];
+            if (!S.pme || S.barostat != DMDX_BARO_NONE) continue;
+            S.ginf_static = 1;
+            const int Kx = S.K[0], Ky = S.K[1], Kz = S.K[2];
+            const double Lx = S.L[0], Ly = S.L[1], Lz = S.L[2];
+            const double V = Lx * Ly * Lz;
+            const double* bmx = bmod.data() + S.bm_off[0];
+            const double* bmy = bmod.data() + S.bm_off[1];
+            const double* bmz = bmod.data() + S.bm_off[2];
+            for (int kz = 0; kz < Kz; ++kz)
+                for (int ky = 0; ky < Ky; ++ky)
+                    for (int kx = 0; kx < Kx; ++kx) {
+                        int mx = (kx <= Kx / 2) ? kx : kx - Kx;
+                        int my = (ky <= Ky / 2) ? ky : ky - Ky;
+                        int mz = (kz <= Kz / 2) ? kz : kz - Kz;
+                        double green = 0.0;
+                        if (mx != 0 || my != 0 || mz != 0) {
+                            double msq = ((double)mx * mx) / (Lx * Lx) +
+                                         ((double)my * my) / (Ly * Ly) +
+                                         ((double)mz * mz) / (Lz * Lz);
+                            double k2 = 4.0 * PI * PI * msq;
+                            double B = bmx[kx] * bmy[ky] * bmz[kz];
+                            green = 4.0 * PI * KE / V * std::exp(-k2 / (4.0 * S.alpha * S.alpha)) *
+                                    B / k2;
+                        }
+                        ginf[S.grid_off + ((int64_t)kz * Ky + ky) * Kx + kx] = (float)green;
+                    }
+        }
+    }
 
     // ---- what exists
     for (int s = 0; s < nsys; ++s) {
@@ -631,6 +671,7 @@ int Batch::create(const dmdx_system_desc* descs, int nsys, const BatchOptions& o
         std::vector<float> twf(twd.begin(), twd.end());
         TRY(dev_alloc_zero((float**)&d.grid, 2 * grid_total, owned));
         TRY(dev_upload((float**)&d.tw, twf, owned));
+        TRY(dev_upload(&d.ginf, ginf, owned));
     }
     TRY(dev_upload(&d.grid_sys, grid_sys, owned));
     TRY(dev_upload(&d.bmod, bmod, owned));
@@ -976,6 +1017,54 @@ int Batch::set_active(int s, int active)
     // leave again before its next step)
     if (active) hstate[s].flags = 0;
     TRY(copy(&d.st[s], &hstate[s], sizeof(SysState), hipMemcpyHostToDevice));
+    return DMDX_OK;
+}
+
+int Batch::adopt(Batch& src, int ss, int ds)
+{
+    TRY(check_sys(ds));
+    TRY(src.check_sys(ss));
+    if (src.d.precision != d.precision) return fail(DMDX_E_MISMATCH, "precisions differ");
+    TRY(src.fetch_state());
+    TRY(sync());
+    SysDev A, B;
+    TRY(src.copy(&A, src.d.sys + ss, sizeof(SysDev), hipMemcpyDeviceToHost));
+    TRY(copy(&B, d.sys + ds, sizeof(SysDev), hipMemcpyDeviceToHost));
+    if (A.n != B.n || A.n_pad != B.n_pad || A.max_nbr != B.max_nbr || A.has_list != B.has_list)
+        return fail(DMDX_E_MISMATCH, "systems were not created from the same description");
+    for (int k = 0; k < 3; ++k) {  // a barostat may have changed the box
+        B.L[k] = A.L[k];
+        B.invL[k] = A.invL[k];
+    }
+    hsys[ds].L[0] = B.L[0], hsys[ds].L[1] = B.L[1], hsys[ds].L[2] = B.L[2];
+    hsys[ds].invL[0] = B.invL[0], hsys[ds].invL[1] = B.invL[1], hsys[ds].invL[2] = B.invL[2];
+    TRY(copy(d.sys + ds, &B, sizeof(SysDev), hipMemcpyHostToDevice));
+    hstate[ds] = src.hstate[ss];
+    TRY(copy(d.st + ds, &hstate[ds], sizeof(SysState), hipMemcpyHostToDevice));
+    host[ds].counter = src.host[ss].counter;
+    host[ds].valid = src.host[ss].valid;
+    host[ds].built = src.host[ss].built;
+    host[ds].active = src.host[ss].active;
+    const int64_t n = A.n, np = A.n_pad, ga = A.atom_off, gb = B.atom_off;
+    auto d2d = [&](void* dst, const void* from, size_t bytes) -> int {
+        DMDX_HIP(hipMemcpyAsync(dst, from, bytes, hipMemcpyDeviceToDevice, stream));
+        return DMDX_OK;
+    };
+    double* const sa[9] = {src.d.x, src.d.y, src.d.z, src.d.vx, src.d.vy, src.d.vz,
+                           src.d.fx, src.d.fy, src.d.fz};
+    double* const sb[9] = {d.x, d.y, d.z, d.vx, d.vy, d.vz, d.fx, d.fy, d.fz};
+    for (int k = 0; k < 9; ++k) TRY(d2d(sb[k] + gb, sa[k] + ga, sizeof(double) * n));
+    TRY(d2d(d.xref + gb, src.d.xref + ga, sizeof(double4) * np));
+    if (A.has_list) {
+        TRY(d2d(d.order + gb, src.d.order + ga, sizeof(int) * np));
+        TRY(d2d(d.nn + gb, src.d.nn + ga, sizeof(int) * np));
+        const int64_t cnt = (int64_t)A.max_nbr * np;
+        TRY(d2d(d.nbr + B.nbr_off, src.d.nbr + A.nbr_off, sizeof(int) * cnt));
+        if (d.org) TRY(d2d(d.org + gb, src.d.org + ga, sizeof(double4) * np));
+        LAUNCH(launch_remap_rows(stream, d.nbr + B.nbr_off, cnt, d.order + gb, np, gb - ga,
+                                 d.precision != DMDX_FP64 ? 1 : 0));
+    }
+    TRY(sync());
     return DMDX_OK;
 }
 

@@ -138,6 +138,12 @@ public:
     // step targets: system s leaves the batch (becomes inactive) at the start
     // of the first step after steps_done reaches stop[s] (0 = no target)
     int set_stop_at(const std::vector<int64_t>& stop);
+    // continue system `src_sys` of `src` as system `dst` of this batch: both
+    // were created from the same description; copies the dynamic state
+    // (positions, velocities, forces, box, scalars, thermostat/barostat
+    // state, list and list counters, cell-local frames), so the system's
+    // trajectory is the same as if it had stayed in `src`
+    int adopt(Batch& src, int src_sys, int dst);
     int get_pairs(int s, int32_t* pi, int32_t* pj, int64_t cap, int64_t* count);
     int upload_sys(int s);
     void invalidate_graph();

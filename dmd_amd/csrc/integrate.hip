@@ -302,14 +302,22 @@ __global__ void k_kick_ke(const SysDev* __restrict__ sys, const SysState* __rest
     DMDX_FOR_UNITS(u_, nunits_) k_kick_ke_unit(u_, sys, st, slot_sys, nslots, invm, m, fx, fy, fz, vx, vy, vz, ke_part);
 }
 
-// sum of part[off .. off+n) by a 256-thread workgroup: thread t sums t,
-// t+256, ... in order, then a fixed tree over the workgroup -- deterministic;
-// every thread gets the result
-__device__ double block_range_sum(const double* part, int64_t off, int64_t n, double* red)
+// strided partial sum of part[off .. off+n) for a 256-thread workgroup:
+// thread t sums t, t+256, ... in order
+__device__ __forceinline__ double strided_sum(const double* __restrict__ part, int64_t off,
+                                              int64_t n)
 {
     double a = 0.0;
     for (int64_t k = threadIdx.x; k < n; k += 256) a += part[off + k];
-    a = wave_sum(a);
+    return a;
+}
+
+// sum of part[off .. off+n) by a 256-thread workgroup: the strided sums,
+// then a fixed tree over the workgroup -- deterministic; every thread gets
+// the result
+__device__ double block_range_sum(const double* part, int64_t off, int64_t n, double* red)
+{
+    double a = wave_sum(strided_sum(part, off, n));
     __syncthreads();
     if (lane_id() == 0) red[threadIdx.x >> 6] = a;
     __syncthreads();
@@ -334,22 +342,34 @@ __global__ void __launch_bounds__(256) k_finalize(const SysDev* __restrict__ sys
                                                   SysState* __restrict__ st, Parts P,
                                                   int with_kinetic)
 {
-    __shared__ double red[4];
+    __shared__ double red[9][4];
     int s = blockIdx.x;
     SysState& T = st[s];
     if (!T.active) return;
     const SysDev& S = sys[s];
+    // the nine sums (8 energy terms and the kinetic energy) in one pass: each
+    // thread's strided sums, then a fixed tree over waves and over the 4
+    // waves -- deterministic, one barrier
+    double a[9];
+    a[0] = strided_sum(P.lj, S.atom_off, S.n_pad);  // pair energies are per atom slot
+    a[1] = strided_sum(P.coul, S.atom_off, S.n_pad);
+    a[2] = strided_sum(P.bond, S.bw_off, S.bw_n);
+    a[3] = strided_sum(P.angle, S.gw_off, S.gw_n);
+    a[4] = strided_sum(P.dih, S.dw_off, S.dw_n);
+    a[5] = strided_sum(P.imp, S.iw_off, S.iw_n);
+    a[6] = strided_sum(P.pme, S.pw_off, S.pw_n);
+    a[7] = strided_sum(P.excl, S.xw_off, S.xw_n);
+    a[8] = with_kinetic ? strided_sum(P.ke, S.aw_off, S.aw_n) : 0.0;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) a[k] = wave_sum(a[k]);
+    if (lane_id() == 0)
+#pragma unroll
+        for (int k = 0; k < 9; ++k) red[k][threadIdx.x >> 6] = a[k];
+    __syncthreads();
     double e[8];
-    // pair energies are per atom slot
-    e[0] = block_range_sum(P.lj, S.atom_off, S.n_pad, red);
-    e[1] = block_range_sum(P.coul, S.atom_off, S.n_pad, red);
-    e[2] = block_range_sum(P.bond, S.bw_off, S.bw_n, red);
-    e[3] = block_range_sum(P.angle, S.gw_off, S.gw_n, red);
-    e[4] = block_range_sum(P.dih, S.dw_off, S.dw_n, red);
-    e[5] = block_range_sum(P.imp, S.iw_off, S.iw_n, red);
-    e[6] = block_range_sum(P.pme, S.pw_off, S.pw_n, red);
-    e[7] = block_range_sum(P.excl, S.xw_off, S.xw_n, red);
-    double ke = with_kinetic ? block_range_sum(P.ke, S.aw_off, S.aw_n, red) : 0.0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) e[k] = (red[k][0] + red[k][1]) + (red[k][2] + red[k][3]);
+    double ke = (red[8][0] + red[8][1]) + (red[8][2] + red[8][3]);
     if (threadIdx.x != 0) return;
     if (S.pme) {
         e[6] += S.self_energy;

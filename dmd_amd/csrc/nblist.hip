@@ -678,6 +678,35 @@ __global__ void k_pack(const SysDev* __restrict__ sys, const SysState* __restric
 
 namespace dmdx {
 
+// moving a system between batches: its list entries and cell-order map
+// refer to slots, which shift by `delta` (mixed entries keep their offset bits)
+__global__ void k_remap_rows(int* __restrict__ nbr, int64_t count, int* __restrict__ order,
+                             int64_t n_order, int64_t delta, int mixed)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < count; k += stride) {
+        const uint32_t u = (uint32_t)nbr[k];
+        nbr[k] = mixed ? (int)((((u & NBR_SLOT_MASK) + (uint32_t)delta) & NBR_SLOT_MASK) |
+                               (u & ~NBR_SLOT_MASK))
+                       : (int)(u + (uint32_t)delta);
+    }
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n_order; k += stride)
+        if (order[k] >= 0) order[k] = (int)(order[k] + delta);
+}
+
+}  // namespace dmdx
+
+namespace dmdx {
+
+hipError_t launch_remap_rows(hipStream_t s, int* nbr, int64_t count, int* order,
+                             int64_t n_order, int64_t delta, int mixed)
+{
+    const int64_t n = std::max(count, n_order);
+    const unsigned g = (unsigned)std::min<int64_t>(1024, std::max<int64_t>(1, (n + 255) / 256));
+    k_remap_rows<<<g, 256, 0, s>>>(nbr, count, order, n_order, delta, mixed);
+    return hipGetLastError();
+}
+
 hipError_t launch_build_lists(const Dev& d, hipStream_t s)
 {
     const int BS = 256;

@@ -251,8 +251,8 @@ template <class R>
 __device__ __forceinline__ void k_fft_pass_unit(int64_t unit_, const FftTile* __restrict__ tiles, const SysDev* __restrict__ sys,
                            const SysState* __restrict__ st, const FftPlan* __restrict__ plans,
                            Cx<R>* __restrict__ grid, const Cx<R>* __restrict__ twiddles,
-                           const double* __restrict__ bmod, int dim, int sign, int conv,
-                           double* __restrict__ epart)
+                           const double* __restrict__ bmod, const float* __restrict__ ginf,
+                           int dim, int sign, int conv, double* __restrict__ epart)
 {
     extern __shared__ unsigned char smem_raw[];
     const FftTile T = tiles[unit_];
@@ -314,11 +314,16 @@ __device__ __forceinline__ void k_fft_pass_unit(int64_t unit_, const FftTile* __
                 res[e] = {R(0), R(0)};
                 continue;
             }
-            double msq = ((double)mx * mx) / (Lx * Lx) + ((double)my * my) / (Ly * Ly) +
-                         ((double)mz * mz) / (Lz * Lz);
-            double k2 = 4.0 * PI * PI * msq;
-            double B = bmx[kx] * bmy[ky] * bmz[kz];
-            double green = 4.0 * PI * KE / V * exp(-k2 / (4.0 * alpha * alpha)) * B / k2;
+            double green;
+            if (sizeof(R) == sizeof(float) && S.ginf_static) {
+                green = (double)ginf[S.grid_off + ((int64_t)kz * Ky + ky) * Kx + kx];
+            } else {
+                double msq = ((double)mx * mx) / (Lx * Lx) + ((double)my * my) / (Ly * Ly) +
+                             ((double)mz * mz) / (Lz * Lz);
+                double k2 = 4.0 * PI * PI * msq;
+                double B = bmx[kx] * bmy[ky] * bmz[kz];
+                green = 4.0 * PI * KE / V * exp(-k2 / (4.0 * alpha * alpha)) * B / k2;
+            }
             double re = (double)v.re, im = (double)v.im;
             esum += 0.5 * green * (re * re + im * im);
             R gr = (R)green;
@@ -348,12 +353,14 @@ template <class R>
 __global__ void k_fft_pass(const FftTile* __restrict__ tiles, const SysDev* __restrict__ sys,
                            const SysState* __restrict__ st, const FftPlan* __restrict__ plans,
                            Cx<R>* __restrict__ grid, const Cx<R>* __restrict__ twiddles,
-                           const double* __restrict__ bmod, int dim, int sign, int conv,
-                           double* __restrict__ epart, int64_t nunits_)
+                           const double* __restrict__ bmod, const float* __restrict__ ginf,
+                           int dim, int sign, int conv, double* __restrict__ epart,
+                           int64_t nunits_)
 {
     DMDX_FOR_UNITS(u_, nunits_)
     {
-        k_fft_pass_unit<R>(u_, tiles, sys, st, plans, grid, twiddles, bmod, dim, sign, conv, epart);
+        k_fft_pass_unit<R>(u_, tiles, sys, st, plans, grid, twiddles, bmod, ginf, dim, sign, conv,
+                           epart);
         __syncthreads();  // LDS is reused by the next unit
     }
 }
@@ -381,7 +388,7 @@ static hipError_t launch_pme_t(const Dev& d, hipStream_t s)
         int dim = dims[p];
         if (!d.ntiles[dim]) continue;
         k_fft_pass<R><<<grid_for(d.ntiles[dim]), 256, d.lds_bytes[dim], s>>>(
-            d.tiles[dim], d.sys, d.st, d.plans, grid, tw, d.bmod, dim, signs[p], p == 2 ? 1 : 0,
+            d.tiles[dim], d.sys, d.st, d.plans, grid, tw, d.bmod, d.ginf, dim, signs[p], p == 2 ? 1 : 0,
             d.p_pme, (int64_t)(d.ntiles[dim]));
     }
     return hipGetLastError();

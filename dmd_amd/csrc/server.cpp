@@ -3,7 +3,10 @@
 // Requests (a system plus a step count) are submitted at any time.  The
 // server groups waiting requests into cohorts: each cohort is one Batch (all
 // its systems advanced by the same launches, on the cohort's own stream), and
-// several cohorts run concurrently.  A request is admitted when its estimated
+// several cohorts run concurrently.  After each round the cohorts of one
+// precision are merged into one batch, each system moving with its whole
+// state (Batch::adopt), so new arrivals join the big batch after one round
+// and finished requests stop occupying it.  A request is admitted when its estimated
 // device memory fits in what the budget has left; it leaves its cohort
 // exactly at its step target (the device freezes it at the start of the
 // first step after steps_done reaches the target) or when it fails
@@ -107,7 +110,7 @@ struct dmdx_server {
     int64_t budget = 0, in_use = 0;
     double est_scale = 1.25;
     Clock::time_point t0;
-    int64_t rounds = 0, retired = 0, failed = 0, cohorts_made = 0;
+    int64_t rounds = 0, retired = 0, failed = 0, cohorts_made = 0, merges = 0;
     std::vector<double> lat_ms;
     std::vector<int64_t> lat_w;
 
@@ -220,6 +223,76 @@ int make_cohort(dmdx_server* srv, const std::vector<int64_t>& ids, std::vector<i
     srv->in_use += c->bytes;
     srv->cohorts.push_back(std::move(c));
     srv->cohorts_made += 1;
+    return DMDX_OK;
+}
+
+// Merge every cohort of each precision into one batch: a new batch is built
+// from the live requests' descriptions and each system continues there with
+// its whole state (Batch::adopt).  Skipped when the new batch would not fit
+// next to the old ones.
+int merge(dmdx_server* srv)
+{
+    for (int prec : {DMDX_FP64, DMDX_MIXED}) {
+        std::vector<int> cs;
+        for (size_t ci = 0; ci < srv->cohorts.size(); ++ci)
+            if (srv->cohorts[ci]->b->d.precision == prec) cs.push_back((int)ci);
+        if (cs.size() < 2) continue;
+        std::vector<int64_t> ids;
+        int64_t need = 0;
+        for (int ci : cs)
+            for (int64_t id : srv->cohorts[ci]->ids) {
+                const Request& r = srv->reqs[id];
+                if (r.state != DMDX_REQ_RUNNING) continue;
+                ids.push_back(id);
+                need += (int64_t)(r.est_bytes * srv->est_scale);
+            }
+        if (ids.empty() || srv->in_use + need > srv->budget) continue;
+        std::vector<dmdx_system_desc> descs;
+        for (int64_t id : ids) descs.push_back(srv->reqs[id].desc->d);
+        BatchOptions bo;
+        bo.reuse_forces = true;
+        bo.use_graph = srv->opt.graph_steps > 0;
+        bo.graph_steps = std::max(1, (int)srv->opt.graph_steps);
+        auto b = std::make_unique<Batch>();
+        int rc = b->create(descs.data(), (int)descs.size(), bo);
+        if (rc == DMDX_E_OOM) continue;
+        if (rc) return rc;
+        const int64_t actual = (int64_t)b->device_bytes();
+        if (srv->in_use + actual > srv->budget) continue;
+        // launch shapes of the first cohort (all cohorts use the defaults)
+        b->d.pair_tpa = srv->cohorts[cs[0]]->b->d.pair_tpa;
+        b->d.pair_width = srv->cohorts[cs[0]]->b->d.pair_width;
+        b->d.build_tpa = srv->cohorts[cs[0]]->b->d.build_tpa;
+        for (size_t k = 0; k < ids.size(); ++k) {
+            Request& r = srv->reqs[ids[k]];
+            rc = b->adopt(*srv->cohorts[r.cohort]->b, r.sys, (int)k);
+            if (rc) return rc;
+        }
+        auto c = std::make_unique<Cohort>();
+        c->bytes = actual;
+        c->b = std::move(b);
+        c->ids = ids;
+        c->live = (int)ids.size();
+        // retire the old cohorts (their live systems now run in c)
+        std::vector<std::unique_ptr<Cohort>> keep;
+        for (size_t ci = 0; ci < srv->cohorts.size(); ++ci) {
+            if (std::find(cs.begin(), cs.end(), (int)ci) != cs.end()) srv->in_use -= srv->cohorts[ci]->bytes;
+            else keep.push_back(std::move(srv->cohorts[ci]));
+        }
+        keep.push_back(std::move(c));
+        srv->in_use += actual;
+        srv->cohorts = std::move(keep);
+        for (size_t ci = 0; ci < srv->cohorts.size(); ++ci) {
+            const auto& ids2 = srv->cohorts[ci]->ids;
+            for (size_t k = 0; k < ids2.size(); ++k) {
+                Request& r = srv->reqs[ids2[k]];
+                if (r.state != DMDX_REQ_RUNNING) continue;
+                r.cohort = (int)ci;
+                r.sys = (int)k;
+            }
+        }
+        srv->merges += 1;
+    }
     return DMDX_OK;
 }
 
@@ -411,6 +484,10 @@ int dmdx_server_pump(dmdx_server* srv, int32_t* n_retired)
             Request& r = srv->reqs[id];
             if (r.state == DMDX_REQ_RUNNING) r.cohort = (int)ci;
         }
+    if (srv->opt.keep_cohorts == 0) {
+        rc = merge(srv);
+        if (rc) return rc;
+    }
     if (n_retired) *n_retired = (int32_t)(srv->retired - before);
     return DMDX_OK;
 }
@@ -485,6 +562,7 @@ int dmdx_server_get_stats(dmdx_server* srv, dmdx_server_stats* out)
     out->failed = srv->failed;
     out->cohorts = (int64_t)srv->cohorts.size();
     out->cohorts_made = srv->cohorts_made;
+    out->merges = srv->merges;
     out->rounds = srv->rounds;
     out->bytes_in_use = srv->in_use;
     out->bytes_budget = srv->budget;

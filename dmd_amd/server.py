@@ -29,6 +29,7 @@ class ServerOptions(C.Structure):
         ("hbm_budget_bytes", C.c_int64), ("max_cohorts", C.c_int32),
         ("chunk_steps", C.c_int32), ("graph_steps", C.c_int32), ("min_admit", C.c_int32),
         ("max_wait_ms", C.c_double), ("max_cohort_systems", C.c_int32),
+        ("keep_cohorts", C.c_int32),
     ]
 
 
@@ -45,7 +46,8 @@ class ServerStats(C.Structure):
     _fields_ = [
         ("queued", C.c_int64), ("running", C.c_int64), ("retired", C.c_int64),
         ("failed", C.c_int64), ("cohorts", C.c_int64), ("cohorts_made", C.c_int64),
-        ("rounds", C.c_int64), ("bytes_in_use", C.c_int64), ("bytes_budget", C.c_int64),
+        ("rounds", C.c_int64), ("merges", C.c_int64), ("bytes_in_use", C.c_int64),
+        ("bytes_budget", C.c_int64),
         ("estimate_scale", C.c_double), ("clock_s", C.c_double),
     ]
 
@@ -73,12 +75,12 @@ class Server:
 
     def __init__(self, hbm_budget_bytes: int = 0, max_cohorts: int = 8, chunk_steps: int = 50,
                  graph_steps: int = 0, min_admit: int = 1, max_wait_ms: float = 0.0,
-                 max_cohort_systems: int = 0):
+                 max_cohort_systems: int = 0, merge: bool = True):
         self.lib = _abi.load()
         _setup(self.lib)
         o = ServerOptions(int(hbm_budget_bytes), int(max_cohorts), int(chunk_steps),
                           int(graph_steps), int(min_admit), float(max_wait_ms),
-                          int(max_cohort_systems))
+                          int(max_cohort_systems), 0 if merge else 1)
         h = C.c_void_p()
         _abi.check(self.lib.dmdx_server_create(C.byref(o), C.byref(h)))
         self.h = h
@@ -214,7 +216,7 @@ def run_poisson(server: Server, requests, rate_per_s: float, seed: int = 42,
         "queue_wait_s": {"p50": float(np.percentile(wait, 50)) if len(wait) else math.nan,
                          "p99": float(np.percentile(wait, 99)) if len(wait) else math.nan},
         "failed": sum(1 for i in infos if i["state"] == "failed"),
-        "cohorts_made": st["cohorts_made"], "rounds": st["rounds"],
+        "cohorts_made": st["cohorts_made"], "rounds": st["rounds"], "merges": st["merges"],
         "bytes_budget": st["bytes_budget"], "estimate_scale": st["estimate_scale"],
     }
     if keep_results:

@@ -321,10 +321,13 @@ int dmdx_copy_bandwidth(int64_t nbytes, int32_t repeats, double *gbps);
  * python/dmd/runner.py:44-58).  A request is a system plus a step count.
  * Waiting requests are admitted when their estimated device memory fits in
  * the budget and grouped into cohorts: one batch per cohort, each on its own
- * stream, several cohorts stepped concurrently.  A request leaves its
- * cohort exactly at its step target, or when it fails (neighbour-list
- * overflow, non-finite energy) -- the other systems are not disturbed, and a
- * request's result is the same alone or in any mix.  One host thread drives
+ * stream, several cohorts stepped concurrently; after each pump the
+ * cohorts are merged into one batch (each system moves with its whole
+ * state, so its trajectory does not change), which keeps the batches large
+ * as requests come and go.  A request leaves its batch exactly at its step
+ * target, or when it fails (neighbour-list overflow, non-finite energy) --
+ * the other systems are not disturbed, and a request's result is the same
+ * alone or in any mix.  One host thread drives
  * a server: submit at any time, then call dmdx_server_pump repeatedly. */
 typedef struct dmdx_server dmdx_server;
 
@@ -338,6 +341,10 @@ typedef struct dmdx_server_options {
                                       wait (0: 1) ...                          */
     double max_wait_ms;            /* ... or once the oldest waited this long */
     int32_t max_cohort_systems;    /* systems per cohort (0: no limit)        */
+    int32_t keep_cohorts;          /* 0: after each pump, merge the cohorts of
+                                      one precision into one batch (systems
+                                      move with their whole state, results
+                                      unchanged); 1: never merge             */
 } dmdx_server_options;
 
 enum { DMDX_REQ_QUEUED = 0, DMDX_REQ_RUNNING = 1, DMDX_REQ_DONE = 2, DMDX_REQ_FAILED = 3 };
@@ -354,7 +361,7 @@ typedef struct dmdx_request_info {
 
 typedef struct dmdx_server_stats {
     int64_t queued, running, retired, failed;
-    int64_t cohorts, cohorts_made, rounds;
+    int64_t cohorts, cohorts_made, rounds, merges;
     int64_t bytes_in_use, bytes_budget;
     double estimate_scale;         /* learned actual/estimate memory ratio    */
     double clock_s;                /* server clock now                        */

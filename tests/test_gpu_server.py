@@ -45,10 +45,11 @@ def alone(spec, n):
 def drain(srv, n_total, max_pumps=2000):
     done = 0
     for _ in range(max_pumps):
-        done += srv.pump()
         if done >= n_total:
             return
-    raise AssertionError("server did not retire every request")
+        done += srv.pump()
+    if done < n_total:
+        raise AssertionError("server did not retire every request")
 
 
 def same(a, b):
@@ -141,3 +142,26 @@ def test_run_poisson_small():
     assert rep["atom_steps"] == sum(s.n * n for s, n in reqs)
     for k in ("step_latency_ms", "completion_s"):
         assert np.isfinite(rep[k]["p50"]) and rep[k]["p99"] >= rep[k]["p50"]
+
+
+@pytest.mark.parametrize("precision", [M, _abi.FP64])
+def test_server_merges_keep_results(precision):
+    """Requests arriving one per round: every round merges the newcomer's
+    cohort into the running batch (systems move with their whole state, lists
+    included); each result still equals the system run alone."""
+    from dmd_amd.server import Server
+    reqs = [(s.with_(precision=precision), n) for s, n in mix()]
+    say(f"case: merging server, precision {precision}")
+    ref = [alone(s, n) for s, n in reqs]
+    srv = Server(chunk_steps=3)
+    ids = []
+    for s, n in reqs:
+        ids.append(srv.submit(s, n))
+        srv.pump()
+    drain(srv, len(reqs) - srv.stats()["retired"])
+    st = srv.stats()
+    say(f"  merges {st['merges']}, cohorts made {st['cohorts_made']}")
+    assert st["merges"] >= 3 and st["retired"] == len(reqs)
+    for rid, r in zip(ids, ref):
+        assert srv.info(rid)["state"] == "done"
+        same(srv.result(rid), r)
