@@ -79,6 +79,7 @@ struct SysDev {
     // PME
     int32_t pme;
     int32_t ginf_static;  // mixed path: influence function tabulated (no barostat)
+    int32_t cdiv;         // cells per list radius along an edge (FP64 1; mixed 1 or 2)
     int32_t K[3];
     int64_t grid_off;    // complex elements
     int64_t tw_off[3];   // twiddle table offsets (complex elements)
@@ -128,12 +129,14 @@ struct SysState {
 
 enum { FLAG_NBR_OVERFLOW = 1, FLAG_NONFINITE = 2 };
 
-// Mixed-path list entries: bits 0-25 the neighbour's cell-order slot, bits
-// 26-31 the neighbour cell's offset from the atom's cell along x, y, z
-// (each stored +1, in {0, 1, 2}).  With cell-local coordinates l and cell
-// width w the displacement is d = l_j - l_i + offset * w (minimum image
-// when the system has >= 3 cells per edge).
-constexpr int NBR_SLOT_BITS = 26;
+// Mixed-path list entries: bits 0-22 the neighbour's cell-order slot
+// relative to its system's first slot, bits 23-31 the neighbour cell's
+// offset from the atom's cell along x, y, z (3 bits each, stored +2, in
+// 0..4; the mixed path's cells are at least rlist / cdiv wide with cdiv 1 or
+// 2, so neighbours lie within +-cdiv cells).  With cell-local coordinates l
+// and cell width w the displacement is d = l_j - l_i + offset * w (the
+// minimum image when the system has >= 2 cdiv + 1 cells per edge).
+constexpr int NBR_SLOT_BITS = 23;
 constexpr uint32_t NBR_SLOT_MASK = (1u << NBR_SLOT_BITS) - 1u;
 
 // Charges are spread onto the PME grid as 64-bit fixed-point integers with
@@ -154,6 +157,13 @@ struct FftTile {
     int32_t line0;
     int32_t nlines;
     int32_t tile_local;  // index among this system's tiles of this pass
+};
+
+// one tile = up to 256 consecutive cell-order slots of one PME system
+struct SpreadTile {
+    int32_t sys;
+    int32_t cnt;
+    int64_t t0;
 };
 
 struct FftPlan {  // per system and dimension
@@ -217,6 +227,8 @@ struct Dev {
     int* grid_sys = nullptr;
     void* tw = nullptr;  // twiddles, same real type
     float* ginf = nullptr;  // mixed path: tabulated influence function (fixed boxes)
+    SpreadTile* stiles = nullptr;  // charge-spreading tiles
+    int nstiles = 0;
     double* bmod = nullptr;
     FftPlan* plans = nullptr;
     FftTile* tiles[3] = {nullptr, nullptr, nullptr};
@@ -249,7 +261,7 @@ hipError_t launch_finalize(const Dev& d, hipStream_t s, int with_kinetic);
 hipError_t launch_barostat(const Dev& d, hipStream_t s);
 hipError_t launch_materialize(const Dev& d, hipStream_t s);
 hipError_t launch_remap_rows(hipStream_t s, int* nbr, int64_t count, int* order, int64_t n_order,
-                             int64_t delta, int mixed);
+                             int64_t delta);
 
 inline unsigned blocks_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 

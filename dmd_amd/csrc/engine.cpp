@@ -148,6 +148,7 @@ int Batch::create(const dmdx_system_desc* descs, int nsys, const BatchOptions& o
         S.use_lj = D.use_lj ? 1 : 0;
         S.coul_pairs = (!faithful && D.coulomb_type != DMDX_COULOMB_NONE) ? 1 : 0;
         if (opt.component == COMP_COULOMB_DIRECT) S.coul_pairs = 1;
+        else if (opt.component != COMP_NONE) S.coul_pairs = 0;  // e.g. PME: reciprocal part only
         S.ntypes = S.use_lj ? std::max(1, (int)D.lj_n_types) : 1;
         S.lj_off = lj_total;
         for (int t = 0; t < S.ntypes * S.ntypes; ++t) {
@@ -192,8 +193,13 @@ int Batch::create(const dmdx_system_desc* descs, int nsys, const BatchOptions& o
             S.max_nbr = M;
             S.nbr_off = nbr_total;
             nbr_total += (int64_t)M * n_pad;
+            // mixed path, dense systems (many atoms per list-radius cell, e.g.
+            // water): cells of half the list radius, searched +-2, cut the
+            // candidates by ~40%; sparse systems keep one cell per radius
+            const double per_cell = n / (D.box[0] * D.box[1] * D.box[2]) * S.rlist * S.rlist * S.rlist;
+            S.cdiv = (D.precision != DMDX_FP64 && per_cell >= 40.0) ? 2 : 1;
             for (int k = 0; k < 3; ++k) {
-                int c = (int)std::floor(D.box[k] / (S.rlist * (1.0 + 1e-9)));
+                int c = (int)std::floor(D.box[k] * S.cdiv / (S.rlist * (1.0 + 1e-9)));
                 // a barostat can grow the box: leave room for more cells
                 if (D.barostat != DMDX_BARO_NONE) c = (int)std::floor(1.2 * c) + 1;
                 S.nc[k] = std::max(1, c);
@@ -202,8 +208,11 @@ int Batch::create(const dmdx_system_desc* descs, int nsys, const BatchOptions& o
                 if (D.box[k] < 2.0 * (S.use_lj ? D.lj_cutoff : D.coulomb_cutoff))
                     return fail(DMDX_E_PARM, "cutoff must be below half the box (minimum image)");
         } else {
+            S.cdiv = 1;
             S.nc[0] = S.nc[1] = S.nc[2] = 1;
         }
+        if (D.precision != DMDX_FP64 && (int64_t)n_pad > (int64_t)NBR_SLOT_MASK)
+            return fail(DMDX_E_PARM, "a mixed-precision system holds at most 2^23 atoms");
         S.cell_off = ncells;
         int nc3 = S.nc[0] * S.nc[1] * S.nc[2];
         ncells += nc3;
@@ -674,6 +683,22 @@ int Batch::create(const dmdx_system_desc* descs, int nsys, const BatchOptions& o
         TRY(dev_upload(&d.ginf, ginf, owned));
     }
     TRY(dev_upload(&d.grid_sys, grid_sys, owned));
+    {
+        std::vector<SpreadTile> st_tiles;
+        for (int s = 0; s < nsys; ++s) {
+            const SysDev& S = hsys[s];
+            if (!S.pme) continue;
+            for (int64_t t0 = 0; t0 < S.n; t0 += 256) {
+                SpreadTile T;
+                T.sys = s;
+                T.t0 = S.atom_off + t0;
+                T.cnt = (int32_t)std::min<int64_t>(256, S.n - t0);
+                st_tiles.push_back(T);
+            }
+        }
+        d.nstiles = (int)st_tiles.size();
+        TRY(dev_upload(&d.stiles, st_tiles, owned));
+    }
     TRY(dev_upload(&d.bmod, bmod, owned));
     TRY(dev_upload(&d.plans, plans, owned));
     for (int k = 0; k < 3; ++k) TRY(dev_upload(&d.tiles[k], tiles[k], owned));
@@ -684,8 +709,6 @@ int Batch::create(const dmdx_system_desc* descs, int nsys, const BatchOptions& o
     TRY(dev_alloc_zero(&d.xref, slot, owned));
     TRY(dev_alloc_zero(&d.p_disp, nw, owned));
     if (prec != DMDX_FP64) {
-        if (slot > (int64_t)NBR_SLOT_MASK)
-            return fail(DMDX_E_PARM, "a mixed-precision batch holds at most 2^26 atom slots");
         TRY(dev_alloc_zero(&d.typ_s, slot, owned));
         TRY(dev_alloc_zero(&d.pos4f, slot, owned));
         TRY(dev_alloc_zero(&d.org, slot, owned));
@@ -1061,8 +1084,9 @@ int Batch::adopt(Batch& src, int ss, int ds)
         const int64_t cnt = (int64_t)A.max_nbr * np;
         TRY(d2d(d.nbr + B.nbr_off, src.d.nbr + A.nbr_off, sizeof(int) * cnt));
         if (d.org) TRY(d2d(d.org + gb, src.d.org + ga, sizeof(double4) * np));
-        LAUNCH(launch_remap_rows(stream, d.nbr + B.nbr_off, cnt, d.order + gb, np, gb - ga,
-                                 d.precision != DMDX_FP64 ? 1 : 0));
+        // FP64 entries are atom slots; mixed entries are local to the system
+        LAUNCH(launch_remap_rows(stream, d.nbr + B.nbr_off, d.precision == DMDX_FP64 ? cnt : 0,
+                                 d.order + gb, np, gb - ga));
     }
     TRY(sync());
     return DMDX_OK;
@@ -1097,8 +1121,8 @@ int Batch::get_pairs(int s, int32_t* pi, int32_t* pj, int64_t cap, int64_t* coun
         int i = order[t] - (int)S.atom_off;
         for (int k = 0; k < nn[t]; ++k) {
             int e = blk[(size_t)t * S.max_nbr + k];
-            if (sorted_ids) e = (int)((uint32_t)e & NBR_SLOT_MASK);
-            int j = (sorted_ids ? order[e - (int)S.atom_off] : e) - (int)S.atom_off;
+            if (sorted_ids) e = (int)((uint32_t)e & NBR_SLOT_MASK);  // local cell-order slot
+            int j = (sorted_ids ? order[e] : e) - (int)S.atom_off;
             if (j > i) keys.push_back(((int64_t)i << 32) | (int64_t)j);
         }
     }

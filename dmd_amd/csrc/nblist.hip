@@ -23,11 +23,12 @@
 
 namespace dmdx {
 
-// cells in use along one edge: as many as fit with edge >= rlist, at most the
-// allocated count.  Fewer than 3 is handled by the offset sets below.
+// cells in use along one edge: as many as fit with edge >= rlist / cdiv, at
+// most the allocated count.  Small counts are handled by the offset sets of
+// the scans.
 __device__ __forceinline__ int cells_in_use(const SysDev& S, int d)
 {
-    int nu = (int)floor(S.L[d] / (S.rlist * (1.0 + 1e-9)));
+    int nu = (int)floor(S.L[d] * S.cdiv / (S.rlist * (1.0 + 1e-9)));
     if (nu > S.nc[d]) nu = S.nc[d];
     if (nu < 1) nu = 1;
     return nu;
@@ -449,11 +450,12 @@ __global__ void __launch_bounds__(256) k_nbr_build(
     DMDX_FOR_UNITS_SPREAD(u_, nunits_) k_nbr_build_unit<TPA>(u_, sys, st, slot_sys, nslots, x, y, z, acell, ccount, cstart, cell_atoms, ex_start, ex_list, order, nn, nbr, pos4s, pos4f);
 }
 
-// Mixed path.  Same scan as k_nbr_build (neighbour cells in the same order,
-// atoms of a cell in slot order, ballot compaction into contiguous rows) on
-// the cell-local FP32 frames: a candidate's displacement is
-// l_j - l_i + offset * w, with the FP32 minimum image only when a system has
-// fewer than 3 cells along an edge.  Candidates within 1e-5 nm of the list
+// Mixed path.  Cells of at least rlist / cdiv (SysDev::cdiv: 2 for dense
+// systems, else 1), neighbour cells within +-cdiv, atoms of a cell in slot
+// order, ballot compaction into contiguous rows, on the cell-local FP32
+// frames: a candidate's displacement is l_j - l_i + offset * w, with the FP32
+// minimum image only when a system has fewer than 2 cdiv + 1 cells along an
+// edge.  Candidates within 1e-5 nm of the list
 // radius are decided by the reference's own FP64 expression on pos4s, so
 // the pair set is exactly the FP64 one.  Entries carry the cell offset
 // (NBR_SLOT_BITS) for the pair kernel.
@@ -488,7 +490,11 @@ __device__ __forceinline__ void k_nbr_build_mixed_unit(int64_t unit_,
     const int cx = ci % nx, cy = (ci / nx) % ny, cz = ci / (nx * ny);
     const float4 li = pos4f[t];
     const float wx = (float)(S.L[0] / nx), wy = (float)(S.L[1] / ny), wz = (float)(S.L[2] / nz);
-    const bool wrap = nx < 3 || ny < 3 || nz < 3;
+    // neighbour cells: +-cdiv with >= 2 cdiv + 1 cells (cells are at least
+    // rlist / cdiv wide); with fewer cells, every cell once, and an FP32
+    // minimum image per candidate
+    const int reach = S.cdiv, full = 2 * reach + 1;
+    const bool wrap = nx < full || ny < full || nz < full;
     const float lo2 = (float)((S.rlist - 1e-5) * (S.rlist - 1e-5));
     const float hi2 = (float)((S.rlist + 1e-5) * (S.rlist + 1e-5));
     const bool check_ex = ex_start[i + 1] > ex_start[i];
@@ -496,9 +502,11 @@ __device__ __forceinline__ void k_nbr_build_mixed_unit(int64_t unit_,
     const int M = S.max_nbr;
     const int gshift = (threadIdx.x & 63) / TPA * TPA;  // first lane of my group in the wave
     const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    const int lox = nx >= 3 ? -1 : 0, hix = nx >= 2 ? 1 : 0;
-    const int loy = ny >= 3 ? -1 : 0, hiy = ny >= 2 ? 1 : 0;
-    const int loz = nz >= 3 ? -1 : 0, hiz = nz >= 2 ? 1 : 0;
+    auto lo_of = [&](int nu) { return nu >= full ? -reach : -((nu - 1) / 2); };
+    auto hi_of = [&](int nu) { return nu >= full ? reach : nu - 1 - (nu - 1) / 2; };
+    const int lox = lo_of(nx), hix = hi_of(nx);
+    const int loy = lo_of(ny), hiy = hi_of(ny);
+    const int loz = lo_of(nz), hiz = hi_of(nz);
     int k = 0;
     for (int oz = loz; oz <= hiz; ++oz) {
         int zz = cz + oz;
@@ -509,8 +517,8 @@ __device__ __forceinline__ void k_nbr_build_mixed_unit(int64_t unit_,
             yy = yy < 0 ? yy + ny : (yy >= ny ? yy - ny : yy);
             const float sy = oy * wy - li.y;
             const int rowc = S.cell_off + (zz * ny + yy) * nx;
-            // the x neighbours of this (y, z) row of cells, in the order
-            // x-1, x, x+1; runs of consecutive cell indices hold their atoms
+            // the x neighbours of this (y, z) row of cells, in increasing
+            // offset; runs of consecutive cell indices hold their atoms
             // contiguously in cell order, so each run is scanned as one range
             // (same candidates in the same order as cell by cell)
             int ox = lox;
@@ -521,15 +529,22 @@ __device__ __forceinline__ void k_nbr_build_mixed_unit(int64_t unit_,
                 while (ox + len <= hix && xx + len < nx) ++len;
                 const int c0 = rowc + xx;
                 const int64_t base = S.atom_off + cstart[c0];
-                const int e1 = ccount[c0];
-                const int e2 = len > 1 ? e1 + ccount[c0 + 1] : 0x7fffffff;
-                const int cnt = len > 2 ? e2 + ccount[c0 + 2] : (len > 1 ? e2 : e1);
+                // ends of the run's cells, relative to the run start
+                int e[5];
+                int acc = 0;
+#pragma unroll
+                for (int q = 0; q < 5; ++q) {
+                    acc += q < len ? ccount[c0 + (q < len ? q : 0)] : 0;
+                    e[q] = q < len ? acc : 0x7fffffff;
+                }
+                const int cnt = acc;
                 const int ox0 = ox;
-                const uint32_t syz = ((uint32_t)(oy + 1) << (NBR_SLOT_BITS + 2)) |
-                                     ((uint32_t)(oz + 1) << (NBR_SLOT_BITS + 4));
+                const uint32_t syz = ((uint32_t)(oy + 2) << (NBR_SLOT_BITS + 3)) |
+                                     ((uint32_t)(oz + 2) << (NBR_SLOT_BITS + 6));
                 for (int m0 = 0; m0 < cnt; m0 += TPA) {
                     const int m = m0 + lane;
-                    const int oxm = ox0 + (m >= e1 ? 1 : 0) + (m >= e2 ? 1 : 0);
+                    const int oxm = ox0 + (m >= e[0] ? 1 : 0) + (m >= e[1] ? 1 : 0) +
+                                    (m >= e[2] ? 1 : 0) + (m >= e[3] ? 1 : 0);
                     bool hit = false;
                     if (m < cnt && base + m != t) {
                         const float4 lj = pos4f[base + m];
@@ -558,8 +573,8 @@ __device__ __forceinline__ void k_nbr_build_mixed_unit(int64_t unit_,
                     const uint64_t mine = (bal >> gshift) & (TPA == 64 ? ~0ull : ((1ull << TPA) - 1));
                     const int pos = k + __popcll(mine & below);
                     if (hit && pos < M)
-                        row[pos] = (int)((uint32_t)(base + m) | syz |
-                                         ((uint32_t)(oxm + 1) << NBR_SLOT_BITS));
+                        row[pos] = (int)((uint32_t)(base + m - S.atom_off) | syz |
+                                         ((uint32_t)(oxm + 2) << NBR_SLOT_BITS));
                     k += __popcll(mine);
                 }
                 ox += len;
@@ -678,18 +693,14 @@ __global__ void k_pack(const SysDev* __restrict__ sys, const SysState* __restric
 
 namespace dmdx {
 
-// moving a system between batches: its list entries and cell-order map
-// refer to slots, which shift by `delta` (mixed entries keep their offset bits)
+// moving a system between batches: FP64 list entries and the cell-order map
+// refer to atom slots, which shift by `delta`
 __global__ void k_remap_rows(int* __restrict__ nbr, int64_t count, int* __restrict__ order,
-                             int64_t n_order, int64_t delta, int mixed)
+                             int64_t n_order, int64_t delta)
 {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < count; k += stride) {
-        const uint32_t u = (uint32_t)nbr[k];
-        nbr[k] = mixed ? (int)((((u & NBR_SLOT_MASK) + (uint32_t)delta) & NBR_SLOT_MASK) |
-                               (u & ~NBR_SLOT_MASK))
-                       : (int)(u + (uint32_t)delta);
-    }
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < count; k += stride)
+        nbr[k] = (int)((uint32_t)nbr[k] + (uint32_t)delta);
     for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n_order; k += stride)
         if (order[k] >= 0) order[k] = (int)(order[k] + delta);
 }
@@ -699,11 +710,11 @@ __global__ void k_remap_rows(int* __restrict__ nbr, int64_t count, int* __restri
 namespace dmdx {
 
 hipError_t launch_remap_rows(hipStream_t s, int* nbr, int64_t count, int* order,
-                             int64_t n_order, int64_t delta, int mixed)
+                             int64_t n_order, int64_t delta)
 {
     const int64_t n = std::max(count, n_order);
     const unsigned g = (unsigned)std::min<int64_t>(1024, std::max<int64_t>(1, (n + 255) / 256));
-    k_remap_rows<<<g, 256, 0, s>>>(nbr, count, order, n_order, delta, mixed);
+    k_remap_rows<<<g, 256, 0, s>>>(nbr, count, order, n_order, delta);
     return hipGetLastError();
 }
 

@@ -126,7 +126,7 @@ __global__ void k_pair_fp64(const SysDev* __restrict__ sys, const SysState* __re
 // their cell-local FP32 positions (spatially local gathers of 16 bytes) and
 // rebuilds each displacement as l_j - l_i + offset * w from the offset
 // stored in the list entry -- no FP64 and no minimum-image rounding per pair
-// when the system has at least 3 cells per edge.  The W entries of a pass
+// when the system has at least 2 cdiv + 1 cells per edge.  The W entries of a pass
 // are summed in FP32 and the pass sums accumulate in FP64; the group sums
 // its partial forces and energies with shuffles.
 // lj_c12/lj_c6 hold 4 eps sig^12 and 4 eps sig^6 in FP32.
@@ -166,7 +166,9 @@ __device__ __forceinline__ void k_pair_mixed_unit(int64_t unit_,
     const float wx = (float)(S.L[0] / T.nub[0]);
     const float wy = (float)(S.L[1] / T.nub[1]);
     const float wz = (float)(S.L[2] / T.nub[2]);
-    const bool wrap = T.nub[0] < 3 || T.nub[1] < 3 || T.nub[2] < 3;
+    const int full = 2 * S.cdiv + 1;  // cells per edge for offsets to be minimum images
+    const bool wrap = T.nub[0] < full || T.nub[1] < full || T.nub[2] < full;
+    const int64_t a0 = S.atom_off;
     const float Lx = (float)S.L[0], Ly = (float)S.L[1], Lz = (float)S.L[2];
     const float ix = (float)S.invL[0], iy = (float)S.invL[1], iz = (float)S.invL[2];
     const int nt = S.ntypes;
@@ -205,7 +207,7 @@ __device__ __forceinline__ void k_pair_mixed_unit(int64_t unit_,
         int tj[W];
 #pragma unroll
         for (int q = 0; q < W; ++q) {
-            const int j = (k0 + q < cnt) ? (int)(ev[q] & NBR_SLOT_MASK) : (int)t;
+            const int64_t j = (k0 + q < cnt) ? a0 + (int64_t)(ev[q] & NBR_SLOT_MASK) : t;
             pj[q] = lpos[j];
             tj[q] = typed ? typ_s[j] : 0;
         }
@@ -215,9 +217,9 @@ __device__ __forceinline__ void k_pair_mixed_unit(int64_t unit_,
         for (int q = 0; q < W; ++q) {
             if (k0 + q >= cnt) continue;
             const uint32_t e = ev[q];
-            const float ox = (float)((int)((e >> NBR_SLOT_BITS) & 3u) - 1);
-            const float oy = (float)((int)((e >> (NBR_SLOT_BITS + 2)) & 3u) - 1);
-            const float oz = (float)((int)((e >> (NBR_SLOT_BITS + 4)) & 3u) - 1);
+            const float ox = (float)((int)((e >> NBR_SLOT_BITS) & 7u) - 2);
+            const float oy = (float)((int)((e >> (NBR_SLOT_BITS + 3)) & 7u) - 2);
+            const float oz = (float)((int)((e >> (NBR_SLOT_BITS + 6)) & 7u) - 2);
             float dx = (pj[q].x - pi.x) + ox * wx;
             float dy = (pj[q].y - pi.y) + oy * wy;
             float dz = (pj[q].z - pi.z) + oz * wz;
@@ -249,7 +251,7 @@ __device__ __forceinline__ void k_pair_mixed_unit(int64_t unit_,
             px += fs * dx;
             py += fs * dy;
             pz += fs * dz;
-            if ((int64_t)(e & NBR_SLOT_MASK) > t) {  // each unordered pair once
+            if (a0 + (int64_t)(e & NBR_SLOT_MASK) > t) {  // each unordered pair once
                 pel += el;
                 pec += ecl;
             }

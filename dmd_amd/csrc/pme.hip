@@ -3,8 +3,9 @@
 // Replaces CoulombPME::compute (src/force/coulomb_pme.cpp:287-318) and its
 // FFTW calls (:154, :198).  Per compute:
 //   k_pme_zero      clear the fixed-point charge grid
-//   k_pme_spread    order-4 B-spline charge spreading (:79-136), 64 integer
-//                   atomic adds per charged atom (order-independent sums)
+//   k_pme_spread_tiles  order-4 B-spline charge spreading (:79-136), 64
+//                   integer atomic adds per charged atom (order-independent
+//                   sums), staged in LDS per tile of cell-ordered atoms
 //   k_pme_convert   fixed point -> complex grid
 //   k_fft_pass x,y  forward 1D FFT passes along x then y
 //   k_fft_pass z+   forward z pass, influence function G(m) with the energy
@@ -170,17 +171,151 @@ __global__ void k_pme_spread(const SysDev* __restrict__ sys, const SysState* __r
     DMDX_FOR_UNITS(u_, nunits_) k_pme_spread_unit<R>(u_, sys, st, slot_sys, nslots, x, y, z, q, qgrid);
 }
 
-// R-point DFT of v[0..R) in place, with w^m = tw[(m * (K / R)) % K]
+// LDS-staged spreading.  A tile is up to 256 consecutive cell-order slots of
+// one system (spatially compact: atoms sorted by cell).  The workgroup finds
+// the bounding box of its atoms' 4x4x4 stencils; when the box fits in LDS
+// (SPREAD_LDS_POINTS int64 values) the 64 contributions per atom are added
+// there with LDS atomics and the box is flushed to the grid with one global
+// atomic per touched point, else every contribution goes to the grid
+// directly.  The sums are integers, so both routes give the same grid as
+// k_pme_spread.
+constexpr int SPREAD_LDS_POINTS = 4096;
+
 template <class R>
-__device__ __forceinline__ void small_dft(Cx<R>* v, int rdx, const Cx<R>* tw, int K, int sign)
+__device__ __forceinline__ void spread_tile_unit(int64_t unit_, const SpreadTile* __restrict__ tiles,
+                                                 const SysDev* __restrict__ sys,
+                                                 const SysState* __restrict__ st,
+                                                 const int* __restrict__ order,
+                                                 const double* __restrict__ x,
+                                                 const double* __restrict__ y,
+                                                 const double* __restrict__ z,
+                                                 const double* __restrict__ q,
+                                                 unsigned long long* __restrict__ qgrid)
 {
-    if (rdx == 2) {
+    __shared__ unsigned long long acc[SPREAD_LDS_POINTS];
+    __shared__ int box[6];
+    const SpreadTile T = tiles[unit_];
+    const int s = T.sys;
+    if (st[s].reuse || !st[s].active) return;  // uniform per workgroup
+    const SysDev& S = sys[s];
+    const int nx = S.K[0], ny = S.K[1], nz = S.K[2];
+    const int64_t t = T.t0 + threadIdx.x;
+    bool mine = threadIdx.x < T.cnt;
+    int64_t g = 0;
+    double qi = 0.0;
+    if (mine) {
+        g = S.has_list ? (int64_t)order[t] : t;  // cell order when a list exists
+        qi = q[g];
+        mine = qi != 0.0;
+    }
+    int m0x = 0, m0y = 0, m0z = 0;
+    R wx[4], wy[4], wz[4];
+    if (mine) {
+        double frac_x = x[g] / S.L[0], frac_y = y[g] / S.L[1], frac_z = z[g] / S.L[2];
+        frac_x -= floor(frac_x);
+        frac_y -= floor(frac_y);
+        frac_z -= floor(frac_z);
+        frac_x = (frac_x >= 0.0 && frac_x <= 1.0) ? frac_x : 0.0;
+        frac_y = (frac_y >= 0.0 && frac_y <= 1.0) ? frac_y : 0.0;
+        frac_z = (frac_z >= 0.0 && frac_z <= 1.0) ? frac_z : 0.0;
+        const double ux = frac_x * nx, uy = frac_y * ny, uz = frac_z * nz;
+        m0x = (int)floor(ux) - 1;
+        m0y = (int)floor(uy) - 1;
+        m0z = (int)floor(uz) - 1;
+        for (int d = 0; d < 4; ++d) {
+            wx[d] = theta4<R>((R)(ux - (m0x + d)));
+            wy[d] = theta4<R>((R)(uy - (m0y + d)));
+            wz[d] = theta4<R>((R)(uz - (m0z + d)));
+        }
+    }
+    if (threadIdx.x < 6) box[threadIdx.x] = (threadIdx.x & 1) ? -0x40000000 : 0x40000000;
+    __syncthreads();
+    if (mine) {
+        atomicMin(&box[0], m0x);
+        atomicMax(&box[1], m0x);
+        atomicMin(&box[2], m0y);
+        atomicMax(&box[3], m0y);
+        atomicMin(&box[4], m0z);
+        atomicMax(&box[5], m0z);
+    }
+    __syncthreads();
+    const int bx0 = box[0], by0 = box[2], bz0 = box[4];
+    const int bnx = box[1] - bx0 + 4, bny = box[3] - by0 + 4, bnz = box[5] - bz0 + 4;
+    const bool any = box[1] >= box[0];
+    const bool staged = any && (int64_t)bnx * bny * bnz <= SPREAD_LDS_POINTS;
+    unsigned long long* G = qgrid + S.grid_off;
+    const R qr = (R)qi;
+    if (staged) {
+        const int npts = bnx * bny * bnz;
+        for (int k = threadIdx.x; k < npts; k += blockDim.x) acc[k] = 0ull;
+        __syncthreads();
+        if (mine) {
+            for (int iz = 0; iz < 4; ++iz) {
+                const R w_z = wz[iz];
+                for (int iy = 0; iy < 4; ++iy) {
+                    const R w_yz = wy[iy] * w_z;
+                    const int rowl = ((m0z + iz - bz0) * bny + (m0y + iy - by0)) * bnx + (m0x - bx0);
+                    for (int ix = 0; ix < 4; ++ix)
+                        atomicAdd(&acc[rowl + ix], (unsigned long long)__double2ll_rn(
+                                                       (double)(qr * wx[ix] * w_yz) * QGRID_SCALE));
+                }
+            }
+        }
+        __syncthreads();
+        for (int k = threadIdx.x; k < npts; k += blockDim.x) {
+            const unsigned long long v = acc[k];
+            if (v == 0ull) continue;
+            const int lx = k % bnx, ly = (k / bnx) % bny, lz = k / (bnx * bny);
+            int gx = (bx0 + lx) % nx, gy = (by0 + ly) % ny, gz = (bz0 + lz) % nz;
+            gx += gx < 0 ? nx : 0;
+            gy += gy < 0 ? ny : 0;
+            gz += gz < 0 ? nz : 0;
+            atomicAdd(&G[((int64_t)gz * ny + gy) * nx + gx], v);
+        }
+    } else if (mine) {
+        for (int iz = 0; iz < 4; ++iz) {
+            int gz = (m0z + iz) % nz;
+            if (gz < 0) gz += nz;
+            const R w_z = wz[iz];
+            for (int iy = 0; iy < 4; ++iy) {
+                int gy = (m0y + iy) % ny;
+                if (gy < 0) gy += ny;
+                const R w_yz = wy[iy] * w_z;
+                const int64_t row = ((int64_t)gz * ny + gy) * nx;
+                for (int ix = 0; ix < 4; ++ix) {
+                    int gx = (m0x + ix) % nx;
+                    if (gx < 0) gx += nx;
+                    atomic_add_fixed(&G[row + gx], (double)(qr * wx[ix] * w_yz));
+                }
+            }
+        }
+    }
+}
+
+template <class R>
+__global__ void __launch_bounds__(256) k_pme_spread_tiles(
+    const SpreadTile* __restrict__ tiles, const SysDev* __restrict__ sys,
+    const SysState* __restrict__ st, const int* __restrict__ order,
+    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
+    const double* __restrict__ q, unsigned long long* __restrict__ qgrid, int64_t nunits_)
+{
+    DMDX_FOR_UNITS(u_, nunits_)
+    {
+        spread_tile_unit<R>(u_, tiles, sys, st, order, x, y, z, q, qgrid);
+        __syncthreads();  // LDS is reused by the next unit
+    }
+}
+
+// RAD-point DFT of v[0..RAD) in place; w^m = tw[m * (K / RAD)] (m < RAD).
+// RAD is a compile-time constant, so v stays in registers.
+template <class R, int RAD>
+__device__ __forceinline__ void small_dft(Cx<R>* v, const Cx<R>* __restrict__ tw, int K, int sign)
+{
+    if constexpr (RAD == 2) {
         Cx<R> a = v[0], b = v[1];
         v[0] = {a.re + b.re, a.im + b.im};
         v[1] = {a.re - b.re, a.im - b.im};
-        return;
-    }
-    if (rdx == 4) {
+    } else if constexpr (RAD == 4) {
         Cx<R> a0 = v[0], a1 = v[1], a2 = v[2], a3 = v[3];
         Cx<R> s02 = {a0.re + a2.re, a0.im + a2.im}, d02 = {a0.re - a2.re, a0.im - a2.im};
         Cx<R> s13 = {a1.re + a3.re, a1.im + a3.im}, d13 = {a1.re - a3.re, a1.im - a3.im};
@@ -190,50 +325,76 @@ __device__ __forceinline__ void small_dft(Cx<R>* v, int rdx, const Cx<R>* tw, in
         v[1] = {d02.re + jd13.re, d02.im + jd13.im};
         v[2] = {s02.re - s13.re, s02.im - s13.im};
         v[3] = {d02.re - jd13.re, d02.im - jd13.im};
-        return;
-    }
-    Cx<R> out[16];
-    int step = K / rdx;
-    for (int k = 0; k < rdx; ++k) {
-        Cx<R> acc = v[0];
-        for (int r = 1; r < rdx; ++r) {
-            Cx<R> w = tw[((r * k) % rdx) * step];
-            if (sign > 0) w.im = -w.im;
-            Cx<R> p = cmul(v[r], w);
-            acc.re += p.re;
-            acc.im += p.im;
+    } else {
+        const int step = K / RAD;
+        Cx<R> w[RAD];
+#pragma unroll
+        for (int m = 0; m < RAD; ++m) {
+            w[m] = tw[m * step];
+            if (sign > 0) w[m].im = -w[m].im;
         }
-        out[k] = acc;
+        Cx<R> out[RAD];
+#pragma unroll
+        for (int k = 0; k < RAD; ++k) {
+            Cx<R> acc = v[0];
+#pragma unroll
+            for (int r = 1; r < RAD; ++r) {
+                Cx<R> p = cmul(v[r], w[(r * k) % RAD]);
+                acc.re += p.re;
+                acc.im += p.im;
+            }
+            out[k] = acc;
+        }
+#pragma unroll
+        for (int k = 0; k < RAD; ++k) v[k] = out[k];
     }
-    for (int k = 0; k < rdx; ++k) v[k] = out[k];
+}
+
+// one radix-RAD Stockham stage over nl lines of length K in LDS (a -> b)
+template <class R, int RAD>
+__device__ __forceinline__ void stockham_stage(const Cx<R>* a, Cx<R>* b, int nl, int K, int Ns,
+                                               const Cx<R>* __restrict__ tw, int sign)
+{
+    const int nb = K / RAD;
+    const int sub = K / (Ns * RAD);
+    for (int w = threadIdx.x; w < nl * nb; w += blockDim.x) {
+        const int line = w / nb, j = w - line * nb;
+        const Cx<R>* src = a + line * K;
+        Cx<R>* dst = b + line * K;
+        const int k = j % Ns;
+        Cx<R> v[RAD];
+        v[0] = src[j];
+#pragma unroll
+        for (int r = 1; r < RAD; ++r) {
+            Cx<R> tv = tw[k * r * sub];  // k r sub < K
+            if (sign > 0) tv.im = -tv.im;
+            v[r] = cmul(src[j + r * nb], tv);
+        }
+        small_dft<R, RAD>(v, tw, K, sign);
+        const int idxD = (j - k) * RAD + k;  // (j / Ns) * Ns * RAD + k
+#pragma unroll
+        for (int r = 0; r < RAD; ++r) dst[idxD + r * Ns] = v[r];
+    }
 }
 
 // all Stockham stages of nl lines of length K held in LDS (a -> b ping-pong);
 // returns the buffer that holds the result
 template <class R>
-__device__ Cx<R>* stockham_lds(Cx<R>* a, Cx<R>* b, int nl, const FftPlan& P,
+__device__ __forceinline__ Cx<R>* stockham_lds(Cx<R>* a, Cx<R>* b, int nl, const FftPlan& P,
                                const Cx<R>* __restrict__ tw, int sign)
 {
     const int K = P.K;
     int Ns = 1;
     for (int st = 0; st < P.nrad; ++st) {
         const int Rr = P.radix[st];
-        const int nb = K / Rr;
-        const int sub = K / (Ns * Rr);
-        for (int w = threadIdx.x; w < nl * nb; w += blockDim.x) {
-            int line = w / nb, j = w - line * nb;
-            const Cx<R>* src = a + line * K;
-            Cx<R>* dst = b + line * K;
-            int k = j % Ns;
-            Cx<R> v[16];
-            for (int r = 0; r < Rr; ++r) {
-                Cx<R> tv = tw[(k * r * sub) % K];
-                if (sign > 0) tv.im = -tv.im;
-                v[r] = r ? cmul(src[j + r * nb], tv) : src[j];
-            }
-            small_dft<R>(v, Rr, tw, K, sign);
-            int idxD = (j / Ns) * Ns * Rr + k;
-            for (int r = 0; r < Rr; ++r) dst[idxD + r * Ns] = v[r];
+        switch (Rr) {
+        case 2: stockham_stage<R, 2>(a, b, nl, K, Ns, tw, sign); break;
+        case 3: stockham_stage<R, 3>(a, b, nl, K, Ns, tw, sign); break;
+        case 4: stockham_stage<R, 4>(a, b, nl, K, Ns, tw, sign); break;
+        case 5: stockham_stage<R, 5>(a, b, nl, K, Ns, tw, sign); break;
+        case 7: stockham_stage<R, 7>(a, b, nl, K, Ns, tw, sign); break;
+        case 11: stockham_stage<R, 11>(a, b, nl, K, Ns, tw, sign); break;
+        default: stockham_stage<R, 13>(a, b, nl, K, Ns, tw, sign); break;
         }
         __syncthreads();
         Cx<R>* t = a;
@@ -350,7 +511,7 @@ __device__ __forceinline__ void k_fft_pass_unit(int64_t unit_, const FftTile* __
 }
 
 template <class R>
-__global__ void k_fft_pass(const FftTile* __restrict__ tiles, const SysDev* __restrict__ sys,
+__global__ void __launch_bounds__(256) k_fft_pass(const FftTile* __restrict__ tiles, const SysDev* __restrict__ sys,
                            const SysState* __restrict__ st, const FftPlan* __restrict__ plans,
                            Cx<R>* __restrict__ grid, const Cx<R>* __restrict__ twiddles,
                            const double* __restrict__ bmod, const float* __restrict__ ginf,
@@ -378,8 +539,9 @@ static hipError_t launch_pme_t(const Dev& d, hipStream_t s)
     const Cx<R>* tw = reinterpret_cast<const Cx<R>*>(d.tw);
     const int64_t gu = blocks_for(d.grid_total, BS);
     k_pme_zero<<<grid_for(gu), BS, 0, s>>>(d.st, d.grid_sys, d.grid_total, d.qgrid, gu);
-    k_pme_spread<R><<<grid_for(blocks_for(d.nslots, BS)), BS, 0, s>>>(d.sys, d.st, d.slot_sys, d.nslots,
-                                                            d.x, d.y, d.z, d.q, d.qgrid, (int64_t)(blocks_for(d.nslots, BS)));
+    k_pme_spread_tiles<R><<<grid_for(d.nstiles), BS, 0, s>>>(d.stiles, d.sys, d.st, d.order, d.x,
+                                                               d.y, d.z, d.q, d.qgrid,
+                                                               (int64_t)d.nstiles);
     k_pme_convert<R><<<grid_for(gu), BS, 0, s>>>(d.st, d.grid_sys, d.grid_total, d.qgrid, grid, gu);
     // forward x, forward y, forward z + convolution + inverse z, inverse y, inverse x
     const int dims[5] = {0, 1, 2, 1, 0};

@@ -111,6 +111,7 @@ struct dmdx_server {
     double est_scale = 1.25;
     Clock::time_point t0;
     int64_t rounds = 0, retired = 0, failed = 0, cohorts_made = 0, merges = 0;
+    int64_t atom_steps = 0;  // atoms x steps advanced over all requests
     std::vector<double> lat_ms;
     std::vector<int64_t> lat_w;
 
@@ -464,6 +465,7 @@ int dmdx_server_pump(dmdx_server* srv, int32_t* n_retired)
             if (adv > 0 && c.chunk > 0) {
                 srv->lat_ms.push_back(round_ms / (double)c.chunk);
                 srv->lat_w.push_back(adv);
+                srv->atom_steps += adv * r.desc->d.n_atoms;
             }
             r.steps_done = T.steps_done;
             r.flags = T.flags;
@@ -568,6 +570,7 @@ int dmdx_server_get_stats(dmdx_server* srv, dmdx_server_stats* out)
     out->bytes_budget = srv->budget;
     out->estimate_scale = srv->est_scale;
     out->clock_s = srv->now();
+    out->atom_steps = srv->atom_steps;
     return DMDX_OK;
 }
 

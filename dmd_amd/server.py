@@ -48,7 +48,7 @@ class ServerStats(C.Structure):
         ("failed", C.c_int64), ("cohorts", C.c_int64), ("cohorts_made", C.c_int64),
         ("rounds", C.c_int64), ("merges", C.c_int64), ("bytes_in_use", C.c_int64),
         ("bytes_budget", C.c_int64),
-        ("estimate_scale", C.c_double), ("clock_s", C.c_double),
+        ("estimate_scale", C.c_double), ("clock_s", C.c_double), ("atom_steps", C.c_int64),
     ]
 
 
@@ -172,9 +172,11 @@ def run_poisson(server: Server, requests, rate_per_s: float, seed: int = 42,
                 keep_results: bool = False, log=None) -> dict:
     """Submit `requests` ([(spec, n_steps), ...]) with exponential
     inter-arrival times of mean 1/rate_per_s (seeded), pump until every
-    request has retired, and report aggregate atom-steps/s, per-step latency
-    p50/p99 (weighted by steps), completion latency p50/p99 (submit ->
-    retire) and queueing delay p50/p99 (submit -> admit)."""
+    request has retired, and report aggregate atom-steps/s over the whole run
+    (arrivals and drain), the sustained atom-steps/s while requests keep
+    arriving (from a quarter of the arrival span to the last arrival),
+    per-step latency p50/p99 (weighted by steps), completion latency p50/p99
+    (submit -> retire) and queueing delay p50/p99 (submit -> admit)."""
     rng = np.random.Generator(np.random.Philox(key=seed))
     gaps = rng.exponential(1.0 / rate_per_s, len(requests))
     arrive = np.cumsum(gaps) - gaps[0]
@@ -183,6 +185,7 @@ def run_poisson(server: Server, requests, rate_per_s: float, seed: int = 42,
     nxt = 0
     retired = 0
     last_log = t0
+    trace = []  # (seconds since start, atom-steps advanced so far) after each pump
     while retired < len(requests):
         now = time.perf_counter() - t0
         while nxt < len(requests) and arrive[nxt] <= now:
@@ -195,6 +198,7 @@ def run_poisson(server: Server, requests, rate_per_s: float, seed: int = 42,
                 time.sleep(max(0.0, min(0.01, arrive[nxt] - now)))
             continue
         retired += server.pump()
+        trace.append((time.perf_counter() - t0, server.stats()["atom_steps"]))
         if log and time.perf_counter() - last_log > 5.0:
             last_log = time.perf_counter()
             log(f"[serve] t={last_log - t0:.1f}s submitted {nxt}/{len(requests)} "
@@ -206,9 +210,18 @@ def run_poisson(server: Server, requests, rate_per_s: float, seed: int = 42,
     atom_steps = sum(i["n_atoms"] * i["steps_done"] for i in infos)
     ms, w = server.step_latency()
     st = server.stats()
+    # sustained rate while requests keep arriving: from a quarter of the
+    # arrival span (load established) to the last arrival
+    span = float(arrive[-1]) if len(arrive) else 0.0
+    ta, tb = 0.25 * span, span
+    pts = [(t, a) for t, a in trace if ta <= t <= tb]
+    sustained = ((pts[-1][1] - pts[0][1]) / (pts[-1][0] - pts[0][0])
+                 if len(pts) >= 2 and pts[-1][0] > pts[0][0] else None)
     out = {
         "requests": len(requests), "rate_per_s": rate_per_s, "wall_s": wall,
         "atom_steps": int(atom_steps), "atom_steps_per_s": atom_steps / wall,
+        "sustained_atom_steps_per_s": sustained,
+        "sustained_window_s": [ta, tb],
         "step_latency_ms": {"p50": weighted_percentile(ms, w, 50),
                             "p99": weighted_percentile(ms, w, 99)},
         "completion_s": {"p50": float(np.percentile(comp, 50)),

@@ -365,6 +365,7 @@ typedef struct dmdx_server_stats {
     int64_t bytes_in_use, bytes_budget;
     double estimate_scale;         /* learned actual/estimate memory ratio    */
     double clock_s;                /* server clock now                        */
+    int64_t atom_steps;            /* atoms x steps advanced so far           */
 } dmdx_server_stats;
 
 int dmdx_server_create(const dmdx_server_options *opt, dmdx_server **out);
