@@ -166,6 +166,12 @@ struct SpreadTile {
     int64_t t0;
 };
 
+// one owner-computes spreading block of a system's grid
+struct GridTile {
+    int32_t sys;
+    int32_t g0[3];
+};
+
 struct FftPlan {  // per system and dimension
     int32_t K;
     int32_t nrad;
@@ -227,8 +233,10 @@ struct Dev {
     int* grid_sys = nullptr;
     void* tw = nullptr;  // twiddles, same real type
     float* ginf = nullptr;  // mixed path: tabulated influence function (fixed boxes)
-    SpreadTile* stiles = nullptr;  // charge-spreading tiles
+    SpreadTile* stiles = nullptr;  // charge-spreading tiles (atomic route)
     int nstiles = 0;
+    GridTile* gtiles = nullptr;    // owner-computes spreading blocks
+    int ngtiles = 0;
     double* bmod = nullptr;
     FftPlan* plans = nullptr;
     FftTile* tiles[3] = {nullptr, nullptr, nullptr};

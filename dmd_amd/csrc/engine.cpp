@@ -684,10 +684,39 @@ int Batch::create(const dmdx_system_desc* descs, int nsys, const BatchOptions& o
     }
     TRY(dev_upload(&d.grid_sys, grid_sys, owned));
     {
+        // owner-computes spreading where atoms stay within skin/2 of their
+        // build cells (mixed path, Verlet criterion, fixed box); 8^3 blocks
         std::vector<SpreadTile> st_tiles;
+        std::vector<GridTile> g_tiles;
         for (int s = 0; s < nsys; ++s) {
             const SysDev& S = hsys[s];
             if (!S.pme) continue;
+            bool gather = prec != DMDX_FP64 && S.has_list && S.interval == 0 &&
+                          S.barostat == DMDX_BARO_NONE && opt.component == COMP_NONE;
+            if (gather) {  // a block scans at most 255 cells (k_pme_spread_gather)
+                int64_t cells = 1;
+                for (int k = 0; k < 3; ++k) {
+                    int nu = (int)std::floor(S.L[k] * S.cdiv / (S.rlist * (1.0 + 1e-9)));
+                    nu = std::max(1, std::min(nu, S.nc[k]));
+                    const double w = S.L[k] / nu;
+                    const double span = 11.0 * S.L[k] / S.K[k] + 2.0 * std::sqrt(S.half_skin2);
+                    cells *= std::min<int64_t>(nu, (int64_t)std::floor(span / w) + 2);
+                }
+                gather = cells <= 255;
+            }
+            if (gather) {
+                for (int gz = 0; gz < S.K[2]; gz += 8)
+                    for (int gy = 0; gy < S.K[1]; gy += 8)
+                        for (int gx = 0; gx < S.K[0]; gx += 8) {
+                            GridTile G;
+                            G.sys = s;
+                            G.g0[0] = gx;
+                            G.g0[1] = gy;
+                            G.g0[2] = gz;
+                            g_tiles.push_back(G);
+                        }
+                continue;
+            }
             for (int64_t t0 = 0; t0 < S.n; t0 += 256) {
                 SpreadTile T;
                 T.sys = s;
@@ -698,6 +727,8 @@ int Batch::create(const dmdx_system_desc* descs, int nsys, const BatchOptions& o
         }
         d.nstiles = (int)st_tiles.size();
         TRY(dev_upload(&d.stiles, st_tiles, owned));
+        d.ngtiles = (int)g_tiles.size();
+        TRY(dev_upload(&d.gtiles, g_tiles, owned));
     }
     TRY(dev_upload(&d.bmod, bmod, owned));
     TRY(dev_upload(&d.plans, plans, owned));
@@ -1084,6 +1115,11 @@ int Batch::adopt(Batch& src, int ss, int ds)
         const int64_t cnt = (int64_t)A.max_nbr * np;
         TRY(d2d(d.nbr + B.nbr_off, src.d.nbr + A.nbr_off, sizeof(int) * cnt));
         if (d.org) TRY(d2d(d.org + gb, src.d.org + ga, sizeof(double4) * np));
+        // cells of the last build (read between builds by the owner-computes
+        // charge spreading); starts are relative to the system's first slot
+        const int64_t nc3 = (int64_t)A.nc[0] * A.nc[1] * A.nc[2];
+        TRY(d2d(d.ccount + B.cell_off, src.d.ccount + A.cell_off, sizeof(int) * nc3));
+        TRY(d2d(d.cstart + B.cell_off, src.d.cstart + A.cell_off, sizeof(int) * nc3));
         // FP64 entries are atom slots; mixed entries are local to the system
         LAUNCH(launch_remap_rows(stream, d.nbr + B.nbr_off, d.precision == DMDX_FP64 ? cnt : 0,
                                  d.order + gb, np, gb - ga));

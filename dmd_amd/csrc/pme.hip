@@ -3,9 +3,12 @@
 // Replaces CoulombPME::compute (src/force/coulomb_pme.cpp:287-318) and its
 // FFTW calls (:154, :198).  Per compute:
 //   k_pme_zero      clear the fixed-point charge grid
-//   k_pme_spread_tiles  order-4 B-spline charge spreading (:79-136), 64
-//                   integer atomic adds per charged atom (order-independent
-//                   sums), staged in LDS per tile of cell-ordered atoms
+//   k_pme_spread_tiles / k_pme_spread_gather  order-4 B-spline charge
+//                   spreading (:79-136) in 64-bit fixed point: integer
+//                   atomics staged in LDS per tile of cell-ordered atoms, or
+//                   owner-computes grid blocks that gather the atoms of the
+//                   nearby list cells (no global atomics); both give the
+//                   same grid
 //   k_pme_convert   fixed point -> complex grid
 //   k_fft_pass x,y  forward 1D FFT passes along x then y
 //   k_fft_pass z+   forward z pass, influence function G(m) with the energy
@@ -292,6 +295,186 @@ __device__ __forceinline__ void spread_tile_unit(int64_t unit_, const SpreadTile
     }
 }
 
+// Owner-computes spreading (no global atomics).  A workgroup owns one
+// GT x GT x GT block of a system's grid: it scans the atoms of the list cells
+// that can reach the block (cells of the last list build, widened by the
+// displacement the Verlet criterion allows, skin/2), adds each atom's
+// contributions that fall inside the block into LDS (the same fixed-point
+// values as k_pme_spread_tiles, so the grid is identical) and stores the
+// block.  Used for systems whose atoms stay within skin/2 of their build
+// cells: mixed path, Verlet criterion, fixed box.
+constexpr int GT = 8;
+constexpr int GCELLS = 255;  // cells a block may scan (host checks)
+
+template <class R>
+__device__ __forceinline__ void spread_gather_unit(int64_t unit_, const GridTile* __restrict__ tiles,
+                                                   const SysDev* __restrict__ sys,
+                                                   const SysState* __restrict__ st,
+                                                   const int* __restrict__ ccount,
+                                                   const int* __restrict__ cstart,
+                                                   const int* __restrict__ order,
+                                                   const double* __restrict__ x,
+                                                   const double* __restrict__ y,
+                                                   const double* __restrict__ z,
+                                                   const double* __restrict__ q,
+                                                   unsigned long long* __restrict__ qgrid)
+{
+    __shared__ unsigned long long acc[GT * GT * GT];
+    const GridTile Tl = tiles[unit_];
+    const int s = Tl.sys;
+    const SysState& T = st[s];
+    if (T.reuse || !T.active) return;  // uniform per workgroup
+    const SysDev& S = sys[s];
+    const int K0 = S.K[0], K1 = S.K[1], K2 = S.K[2];
+    const int g0[3] = {Tl.g0[0], Tl.g0[1], Tl.g0[2]};
+    for (int k = threadIdx.x; k < GT * GT * GT; k += blockDim.x) acc[k] = 0ull;
+    // cell range per axis: atoms whose stencil reaches [g0, g0 + GT) have
+    // u in [g0 - 2, g0 + GT + 1), i.e. x in that range times L/K, and are at
+    // most skin/2 from where the cells were built
+    const double drift = sqrt(S.half_skin2);
+    int clo[3], ncl[3], nu[3];
+    for (int d = 0; d < 3; ++d) {
+        const int Kd = S.K[d];
+        nu[d] = T.nub[d];
+        const double h = S.L[d] / Kd, w = S.L[d] / nu[d];
+        const double xa = (g0[d] - 2) * h - drift, xb = (g0[d] + GT + 1) * h + drift;
+        int ca = (int)floor(xa / w), cb = (int)floor(xb / w);
+        if (cb - ca + 1 >= nu[d]) {
+            ca = 0;
+            cb = nu[d] - 1;
+        }
+        clo[d] = ca;
+        ncl[d] = cb - ca + 1;
+    }
+    // the cells' atom ranges, then all candidates spread over the workgroup
+    // (cells hold ~10 atoms: a wave per cell would leave most lanes idle)
+    __shared__ int cbase[GCELLS], cpre[GCELLS + 1];
+    const int ncells = ncl[0] * ncl[1] * ncl[2];  // <= GCELLS (checked on the host)
+    int mycnt = 0;
+    if ((int)threadIdx.x < ncells) {
+        const int c = threadIdx.x;
+        int cx = clo[0] + c % ncl[0], cy = clo[1] + (c / ncl[0]) % ncl[1],
+            cz = clo[2] + c / (ncl[0] * ncl[1]);
+        cx += cx < 0 ? nu[0] : (cx >= nu[0] ? -nu[0] : 0);
+        cy += cy < 0 ? nu[1] : (cy >= nu[1] ? -nu[1] : 0);
+        cz += cz < 0 ? nu[2] : (cz >= nu[2] ? -nu[2] : 0);
+        const int cell = S.cell_off + (cz * nu[1] + cy) * nu[0] + cx;
+        cbase[c] = cstart[cell];
+        mycnt = ccount[cell];
+    }
+    // exclusive scan of the counts over the workgroup (fixed tree)
+    cpre[threadIdx.x < GCELLS ? threadIdx.x : 0] = 0;
+    __syncthreads();
+    if ((int)threadIdx.x < ncells) cpre[threadIdx.x + 1] = mycnt;
+    __syncthreads();
+    for (int o = 1; o < GCELLS; o <<= 1) {
+        int v = 0;
+        if ((int)threadIdx.x <= ncells && (int)threadIdx.x >= o) v = cpre[threadIdx.x - o];
+        __syncthreads();
+        if ((int)threadIdx.x <= ncells) cpre[threadIdx.x] += v;
+        __syncthreads();
+    }
+    const int total = cpre[ncells];
+    for (int k = threadIdx.x; k < total; k += blockDim.x) {
+        // cell of candidate k: last c with cpre[c] <= k
+        int lo = 0, hi = ncells - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (cpre[mid] <= k) lo = mid;
+            else hi = mid - 1;
+        }
+        const int64_t g = order[S.atom_off + cbase[lo] + (k - cpre[lo])];
+        const double qi = q[g];
+        if (qi == 0.0) continue;
+        // x first: most candidates miss the block along some axis
+        double frac_x = x[g] / S.L[0];
+        frac_x -= floor(frac_x);
+        frac_x = (frac_x >= 0.0 && frac_x <= 1.0) ? frac_x : 0.0;
+        const double ux = frac_x * K0;
+        const int m0x = (int)floor(ux) - 1;
+        int lx[4];
+        bool any = false;
+        for (int d = 0; d < 4; ++d) {
+            int p = m0x + d;
+            p += p < 0 ? K0 : (p >= K0 ? -K0 : 0);
+            lx[d] = (p >= g0[0] && p < g0[0] + GT) ? p - g0[0] : -1;
+            any |= lx[d] >= 0;
+        }
+        if (!any) continue;
+        double frac_y = y[g] / S.L[1];
+        frac_y -= floor(frac_y);
+        frac_y = (frac_y >= 0.0 && frac_y <= 1.0) ? frac_y : 0.0;
+        const double uy = frac_y * K1;
+        const int m0y = (int)floor(uy) - 1;
+        int ly[4];
+        any = false;
+        for (int d = 0; d < 4; ++d) {
+            int p = m0y + d;
+            p += p < 0 ? K1 : (p >= K1 ? -K1 : 0);
+            ly[d] = (p >= g0[1] && p < g0[1] + GT) ? p - g0[1] : -1;
+            any |= ly[d] >= 0;
+        }
+        if (!any) continue;
+        double frac_z = z[g] / S.L[2];
+        frac_z -= floor(frac_z);
+        frac_z = (frac_z >= 0.0 && frac_z <= 1.0) ? frac_z : 0.0;
+        const double uz = frac_z * K2;
+        const int m0z = (int)floor(uz) - 1;
+        int lz[4];
+        any = false;
+        for (int d = 0; d < 4; ++d) {
+            int p = m0z + d;
+            p += p < 0 ? K2 : (p >= K2 ? -K2 : 0);
+            lz[d] = (p >= g0[2] && p < g0[2] + GT) ? p - g0[2] : -1;
+            any |= lz[d] >= 0;
+        }
+        if (!any) continue;
+        R wx[4], wy[4], wz[4];
+        for (int d = 0; d < 4; ++d) {
+            wx[d] = theta4<R>((R)(ux - (m0x + d)));
+            wy[d] = theta4<R>((R)(uy - (m0y + d)));
+            wz[d] = theta4<R>((R)(uz - (m0z + d)));
+        }
+        const R qr = (R)qi;
+        for (int iz = 0; iz < 4; ++iz) {
+            if (lz[iz] < 0) continue;
+            const R w_z = wz[iz];
+            for (int iy = 0; iy < 4; ++iy) {
+                if (ly[iy] < 0) continue;
+                const R w_yz = wy[iy] * w_z;
+                for (int ix = 0; ix < 4; ++ix) {
+                    if (lx[ix] < 0) continue;
+                    atomicAdd(&acc[(lz[iz] * GT + ly[iy]) * GT + lx[ix]],
+                              (unsigned long long)__double2ll_rn(
+                                  (double)(qr * wx[ix] * w_yz) * QGRID_SCALE));
+                }
+            }
+        }
+    }
+    __syncthreads();
+    unsigned long long* G = qgrid + S.grid_off;
+    for (int k = threadIdx.x; k < GT * GT * GT; k += blockDim.x) {
+        const int lx = k % GT, ly = (k / GT) % GT, lz = k / (GT * GT);
+        const int px = g0[0] + lx, py = g0[1] + ly, pz = g0[2] + lz;
+        if (px < K0 && py < K1 && pz < K2) G[((int64_t)pz * K1 + py) * K0 + px] = acc[k];
+    }
+}
+
+template <class R>
+__global__ void __launch_bounds__(256) k_pme_spread_gather(
+    const GridTile* __restrict__ tiles, const SysDev* __restrict__ sys,
+    const SysState* __restrict__ st, const int* __restrict__ ccount,
+    const int* __restrict__ cstart, const int* __restrict__ order,
+    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
+    const double* __restrict__ q, unsigned long long* __restrict__ qgrid, int64_t nunits_)
+{
+    DMDX_FOR_UNITS(u_, nunits_)
+    {
+        spread_gather_unit<R>(u_, tiles, sys, st, ccount, cstart, order, x, y, z, q, qgrid);
+        __syncthreads();  // LDS is reused by the next unit
+    }
+}
+
 template <class R>
 __global__ void __launch_bounds__(256) k_pme_spread_tiles(
     const SpreadTile* __restrict__ tiles, const SysDev* __restrict__ sys,
@@ -539,9 +722,14 @@ static hipError_t launch_pme_t(const Dev& d, hipStream_t s)
     const Cx<R>* tw = reinterpret_cast<const Cx<R>*>(d.tw);
     const int64_t gu = blocks_for(d.grid_total, BS);
     k_pme_zero<<<grid_for(gu), BS, 0, s>>>(d.st, d.grid_sys, d.grid_total, d.qgrid, gu);
-    k_pme_spread_tiles<R><<<grid_for(d.nstiles), BS, 0, s>>>(d.stiles, d.sys, d.st, d.order, d.x,
-                                                               d.y, d.z, d.q, d.qgrid,
-                                                               (int64_t)d.nstiles);
+    if (d.nstiles)
+        k_pme_spread_tiles<R><<<grid_for(d.nstiles), BS, 0, s>>>(d.stiles, d.sys, d.st, d.order,
+                                                                   d.x, d.y, d.z, d.q, d.qgrid,
+                                                                   (int64_t)d.nstiles);
+    if (d.ngtiles)
+        k_pme_spread_gather<R><<<grid_for(d.ngtiles), BS, 0, s>>>(
+            d.gtiles, d.sys, d.st, d.ccount, d.cstart, d.order, d.x, d.y, d.z, d.q, d.qgrid,
+            (int64_t)d.ngtiles);
     k_pme_convert<R><<<grid_for(gu), BS, 0, s>>>(d.st, d.grid_sys, d.grid_total, d.qgrid, grid, gu);
     // forward x, forward y, forward z + convolution + inverse z, inverse y, inverse x
     const int dims[5] = {0, 1, 2, 1, 0};
