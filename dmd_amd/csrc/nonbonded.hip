@@ -243,8 +243,13 @@ __device__ __forceinline__ void k_pair_mixed_unit(int64_t unit_,
                 const float rinv = rsqrtf(r2);
                 const float r = r2 * rinv;
                 const float qq = qi * pj[q].w;
-                const float erfc_ar = erfcf(alpha * r);
                 const float ex = __expf(-a2 * r2);
+                // erfc(a r) by Abramowitz & Stegun 7.1.26 (|error| <= 1.5e-7),
+                // sharing exp(-(a r)^2) with the force term
+                const float tt = __builtin_amdgcn_rcpf(1.0f + 0.3275911f * (alpha * r));
+                const float erfc_ar =
+                    tt * (0.254829592f + tt * (-0.284496736f + tt * (1.421413741f +
+                          tt * (-1.453152027f + tt * 1.061405429f)))) * ex;
                 fs += qq * (erfc_ar * rinv + two_a_rpi * ex) * rinv2;
                 ecl = qq * erfc_ar * rinv;
             }

@@ -173,8 +173,8 @@ def run_poisson(server: Server, requests, rate_per_s: float, seed: int = 42,
     """Submit `requests` ([(spec, n_steps), ...]) with exponential
     inter-arrival times of mean 1/rate_per_s (seeded), pump until every
     request has retired, and report aggregate atom-steps/s over the whole run
-    (arrivals and drain), the sustained atom-steps/s while requests keep
-    arriving (from a quarter of the arrival span to the last arrival),
+    (arrivals and drain), the loaded atom-steps/s (from the last arrival
+    until the running requests fall below half their peak),
     per-step latency p50/p99 (weighted by steps), completion latency p50/p99
     (submit -> retire) and queueing delay p50/p99 (submit -> admit)."""
     rng = np.random.Generator(np.random.Philox(key=seed))
@@ -185,7 +185,7 @@ def run_poisson(server: Server, requests, rate_per_s: float, seed: int = 42,
     nxt = 0
     retired = 0
     last_log = t0
-    trace = []  # (seconds since start, atom-steps advanced so far) after each pump
+    trace = []  # (seconds since start, atom-steps so far, requests running) after each pump
     while retired < len(requests):
         now = time.perf_counter() - t0
         while nxt < len(requests) and arrive[nxt] <= now:
@@ -198,7 +198,8 @@ def run_poisson(server: Server, requests, rate_per_s: float, seed: int = 42,
                 time.sleep(max(0.0, min(0.01, arrive[nxt] - now)))
             continue
         retired += server.pump()
-        trace.append((time.perf_counter() - t0, server.stats()["atom_steps"]))
+        st2 = server.stats()
+        trace.append((time.perf_counter() - t0, st2["atom_steps"], st2["running"]))
         if log and time.perf_counter() - last_log > 5.0:
             last_log = time.perf_counter()
             log(f"[serve] t={last_log - t0:.1f}s submitted {nxt}/{len(requests)} "
@@ -210,18 +211,21 @@ def run_poisson(server: Server, requests, rate_per_s: float, seed: int = 42,
     atom_steps = sum(i["n_atoms"] * i["steps_done"] for i in infos)
     ms, w = server.step_latency()
     st = server.stats()
-    # sustained rate while requests keep arriving: from a quarter of the
-    # arrival span (load established) to the last arrival
+    # loaded rate: from the last arrival until the running requests fall
+    # below half their peak -- the server's throughput while it is full
     span = float(arrive[-1]) if len(arrive) else 0.0
-    ta, tb = 0.25 * span, span
-    pts = [(t, a) for t, a in trace if ta <= t <= tb]
+    peak = max((r for _, _, r in trace), default=0)
+    ta = span
+    tb = next((t for t, _, r in trace if t > ta and r < 0.5 * peak), None)
+    pts = [(t, a) for t, a, _ in trace if ta <= t <= (tb if tb is not None else t)]
     sustained = ((pts[-1][1] - pts[0][1]) / (pts[-1][0] - pts[0][0])
                  if len(pts) >= 2 and pts[-1][0] > pts[0][0] else None)
+    tb = pts[-1][0] if pts else ta
     out = {
         "requests": len(requests), "rate_per_s": rate_per_s, "wall_s": wall,
         "atom_steps": int(atom_steps), "atom_steps_per_s": atom_steps / wall,
-        "sustained_atom_steps_per_s": sustained,
-        "sustained_window_s": [ta, tb],
+        "loaded_atom_steps_per_s": sustained,
+        "loaded_window_s": [ta, tb], "peak_running": peak,
         "step_latency_ms": {"p50": weighted_percentile(ms, w, 50),
                             "p99": weighted_percentile(ms, w, 99)},
         "completion_s": {"p50": float(np.percentile(comp, 50)),
