@@ -497,7 +497,14 @@ __device__ __forceinline__ void k_nbr_build_mixed_unit(int64_t unit_,
     const bool wrap = nx < full || ny < full || nz < full;
     const float lo2 = (float)((S.rlist - 1e-5) * (S.rlist - 1e-5));
     const float hi2 = (float)((S.rlist + 1e-5) * (S.rlist + 1e-5));
-    const bool check_ex = ex_start[i + 1] > ex_start[i];
+    // the atom's excluded partners, held in registers when there are few
+    // (chains: 1-2 and 1-3 partners; water: the other two atoms)
+    const int exb = ex_start[i], exn = ex_start[i + 1] - exb;
+    const bool check_ex = exn > 0;
+    constexpr int EXR = 6;
+    int exr[EXR];
+#pragma unroll
+    for (int q = 0; q < EXR; ++q) exr[q] = q < exn ? ex_list[exb + q] : -1;
     int* row = nbr + S.nbr_off + (int64_t)lt * S.max_nbr;
     const int M = S.max_nbr;
     const int gshift = (threadIdx.x & 63) / TPA * TPA;  // first lane of my group in the wave
@@ -541,13 +548,17 @@ __device__ __forceinline__ void k_nbr_build_mixed_unit(int64_t unit_,
                 const int ox0 = ox;
                 const uint32_t syz = ((uint32_t)(oy + 2) << (NBR_SLOT_BITS + 3)) |
                                      ((uint32_t)(oz + 2) << (NBR_SLOT_BITS + 6));
+                // the next pass's position is loaded before this pass is
+                // processed, so the gather latency overlaps the arithmetic
+                float4 nxt = lane < cnt ? pos4f[base + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
                 for (int m0 = 0; m0 < cnt; m0 += TPA) {
                     const int m = m0 + lane;
+                    const float4 lj = nxt;
+                    if (m + TPA < cnt) nxt = pos4f[base + m + TPA];
                     const int oxm = ox0 + (m >= e[0] ? 1 : 0) + (m >= e[1] ? 1 : 0) +
                                     (m >= e[2] ? 1 : 0) + (m >= e[3] ? 1 : 0);
                     bool hit = false;
                     if (m < cnt && base + m != t) {
-                        const float4 lj = pos4f[base + m];
                         float dx = lj.x + (oxm * wx - li.x), dy = lj.y + sy, dz = lj.z + sz;
                         if (wrap) {
                             dx -= rintf(dx * (float)S.invL[0]) * (float)S.L[0];
@@ -566,8 +577,17 @@ __device__ __forceinline__ void k_nbr_build_mixed_unit(int64_t unit_,
                             double ddz = min_image_dmd(pj.z - pi4.z, S.L[2], S.invL[2]);
                             hit = ddx * ddx + ddy * ddy + ddz * ddz < S.rlist2;
                         }
-                        if (hit && check_ex)
-                            hit = !is_excluded(ex_start, ex_list, i, cell_atoms[base + m]);
+                        if (hit && check_ex) {
+                            const int j = cell_atoms[base + m];
+                            if (exn <= EXR) {
+                                bool ex = false;
+#pragma unroll
+                                for (int q = 0; q < EXR; ++q) ex |= exr[q] == j;
+                                hit = !ex;
+                            } else {
+                                hit = !is_excluded(ex_start, ex_list, i, j);
+                            }
+                        }
                     }
                     const uint64_t bal = __ballot(hit);
                     const uint64_t mine = (bal >> gshift) & (TPA == 64 ? ~0ull : ((1ull << TPA) - 1));
