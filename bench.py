@@ -70,10 +70,13 @@ def make_workload(name, n_replicas, seed):
 
 def count_pairs(eng, specs):
     """In-cutoff unique pairs of every system (list pairs filtered by the
-    cutoffs at the current positions) -> algorithmic FLOP per pair launch."""
+    cutoffs at the current positions) -> algorithmic FLOP per pair launch;
+    also the unique pairs of each system's list (within the list radius)."""
     n_lj = n_c = n_both = 0
+    list_pairs = []
     for k, s in enumerate(specs):
         pi, pj = eng.pairs(k)
+        list_pairs.append(len(pi))
         if len(pi) == 0:
             continue
         pos = eng.positions(k)
@@ -86,7 +89,8 @@ def count_pairs(eng, specs):
         n_both += int((lj & c).sum())
         n_lj += int((lj & ~c).sum())
         n_c += int((c & ~lj).sum())
-    return n_lj * FLOP_LJ + n_c * FLOP_ERFC + n_both * FLOP_FUSED, n_lj + n_c + n_both
+    return (n_lj * FLOP_LJ + n_c * FLOP_ERFC + n_both * FLOP_FUSED, n_lj + n_c + n_both,
+            list_pairs)
 
 
 def percentile(v, p):
@@ -110,12 +114,14 @@ def run_batched(specs, steps, warmup, latency_steps):
     eng.step(warmup)
     eng.sync()
     eng.set_profiling(True)
+    builds0 = [eng.scalars(k)["list_builds"] for k in range(len(specs))]
     base = eng.kernel_times()
     t0 = time.perf_counter()
     eng.step(steps)
     eng.sync()
     wall = time.perf_counter() - t0
     kt = eng.kernel_times()
+    builds = [eng.scalars(k)["list_builds"] - builds0[k] for k in range(len(specs))]
     kt = {k: (kt[k][0] - base[k][0], kt[k][1] - base[k][1]) for k in kt}
     eng.set_profiling(False)
     lat = []
@@ -125,7 +131,7 @@ def run_batched(specs, steps, warmup, latency_steps):
         eng.sync()
         lat.append(time.perf_counter() - a)
     flags = [eng.scalars(k)["flags"] for k in range(len(specs))]
-    return eng, wall, lat, kt, flags, tune
+    return eng, wall, lat, kt, flags, tune, builds
 
 
 def run_plain(specs, steps, budget_s):
@@ -282,19 +288,20 @@ def main():
     n_atoms = sum(s.n for s in specs)
     log(f"[bench] {len(specs)} systems, {n_atoms} atoms; batched mixed: warmup {args.warmup}, "
         f"timed {args.steps}")
-    eng, wall, lat, kt, flags, tune = run_batched(specs, args.steps, args.warmup,
-                                                  args.latency_steps)
+    eng, wall, lat, kt, flags, tune, builds = run_batched(specs, args.steps, args.warmup,
+                                                          args.latency_steps)
     value = n_atoms * args.steps / wall
     ms_per_step = wall / args.steps * 1e3
     log(f"[bench] batched mixed: {value:.4g} atom-steps/s, {ms_per_step:.3f} ms/step")
 
-    # dominant kernel group and its roofline
+    # rooflines: the pair kernel (FP32 VALU) and the list build (HBM bytes
+    # of the lists it writes); "roofline" is the one whose group dominates
     dom = max(kt, key=lambda k: kt[k][0])
-    flop_per_launch, n_pairs = count_pairs(eng, specs)
+    flop_per_launch, n_pairs, list_pairs = count_pairs(eng, specs)
     pair_ms, pair_n = kt["pair"]
     pair_avg_s = pair_ms / max(pair_n, 1) * 1e-3
     achieved = flop_per_launch / pair_avg_s / 1e12 if pair_n else 0.0
-    roofline = {
+    roof_pair = {
         "kernel": "k_pair_mixed (LJ + real-space erfc pair forces)",
         "bound": "valu",
         "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
@@ -302,9 +309,26 @@ def main():
         "traffic": load_traffic("k_pair_mixed", args.workload),
         "algorithmic_flop_per_launch": flop_per_launch, "pairs_in_cutoff": n_pairs,
         "avg_launch_ms": round(pair_avg_s * 1e3, 4), "launches": pair_n,
-        "dominant_group": dom,
-        "group_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in kt.items()},
     }
+    # list build: per rebuilt system, 24 B read per atom and 4 B per row
+    # entry (2 per unique pair within the list radius), over the timed steps
+    list_ms, list_n = kt["list"]
+    list_bytes = sum(b * (24 * s.n + 8 * p) for b, s, p in zip(builds, specs, list_pairs))
+    list_avg_s = list_ms / max(list_n, 1) * 1e-3
+    gbs = list_bytes / max(list_n, 1) / list_avg_s / 1e9 if list_n else 0.0
+    roof_list = {
+        "kernel": "k_nbr_build_mixed + cell binning (list rebuilds of the systems that need one)",
+        "bound": "hbm", "achieved": round(gbs, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+        "frac": round(gbs / PEAK_HBM_GBS, 5),
+        "traffic": load_traffic("k_nbr_build_mixed", args.workload),
+        "algorithmic_bytes_per_launch": list_bytes / max(list_n, 1),
+        "system_rebuilds": int(sum(builds)), "avg_launch_ms": round(list_avg_s * 1e3, 4),
+        "launches": list_n,
+    }
+    roofline = dict(roof_list if dom == "list" else roof_pair)
+    roofline["dominant_group"] = dom
+    roofline["group_ms_per_step"] = {k: round(v[0] / args.steps, 4) for k, v in kt.items()}
+    other = roof_pair if dom == "list" else roof_list
     eng.close()
 
     out = {
@@ -318,6 +342,7 @@ def main():
         "latency_ms": {"p50": percentile(lat, 50) * 1e3 if lat else None,
                        "p99": percentile(lat, 99) * 1e3 if lat else None},
         "roofline": roofline,
+        ("roofline_pair" if dom == "list" else "roofline_list"): other,
         "failed_systems": int(sum(1 for f in flags if f)),
         "autotune": tune,
     }

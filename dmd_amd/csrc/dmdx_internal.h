@@ -233,6 +233,11 @@ struct Dev {
     int* grid_sys = nullptr;
     void* tw = nullptr;  // twiddles, same real type
     float* ginf = nullptr;  // mixed path: tabulated influence function (fixed boxes)
+    // rebuild worklist (k_rb_list): systems rebuilding their lists this step
+    int* rb_sys = nullptr;       // nsys
+    int64_t* rb_pre = nullptr;   // nsys + 1: running slot totals
+    int64_t* rb_cpre = nullptr;  // nsys + 1: running cell totals
+    int* rb_n = nullptr;         // 1
     SpreadTile* stiles = nullptr;  // charge-spreading tiles (atomic route)
     int nstiles = 0;
     GridTile* gtiles = nullptr;    // owner-computes spreading blocks

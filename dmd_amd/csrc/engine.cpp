@@ -625,6 +625,10 @@ int Batch::create(const dmdx_system_desc* descs, int nsys, const BatchOptions& o
     TRY(dev_alloc_zero(&d.cfill, ncells, owned));
     TRY(dev_alloc_zero(&d.cell_atoms, slot, owned));
     TRY(dev_upload(&d.cell_sys, cell_sys, owned));
+    TRY(dev_alloc_zero(&d.rb_sys, (size_t)nsys, owned));
+    TRY(dev_alloc_zero(&d.rb_pre, (size_t)nsys + 1, owned));
+    TRY(dev_alloc_zero(&d.rb_cpre, (size_t)nsys + 1, owned));
+    TRY(dev_alloc_zero(&d.rb_n, 1, owned));
     {
         std::vector<int> ident(slot);
         for (int64_t g = 0; g < slot; ++g) ident[g] = (int)g;

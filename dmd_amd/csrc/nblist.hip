@@ -43,12 +43,116 @@ __device__ __forceinline__ int cell_coord(double x, double L, int nu)
     return c < 0 ? 0 : (c >= nu ? nu - 1 : c);
 }
 
-__device__ __forceinline__ void k_cells_reset_unit(int64_t unit_, const SysDev* __restrict__ sys, const SysState* __restrict__ st,
+// ---- rebuild worklist: the systems whose lists are rebuilt in this launch,
+// with running totals of their slots and cells, so the list kernels visit
+// only those systems' slots and cells (a partial rebuild touches a few
+// percent of a batch)
+struct RbList {
+    const int* sys;        // rebuilding systems, in system order
+    const int64_t* pre;    // pre[k]: slots (n_pad) of rebuilding systems before k
+    const int64_t* cpre;   // cpre[k]: allocated cells of rebuilding systems before k
+    const int* n;          // number of rebuilding systems
+};
+
+__device__ __forceinline__ int rb_find(const int64_t* __restrict__ pre, int n, int64_t v)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pre[mid] <= v) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// one 1024-thread workgroup: block-wide scans over the systems
+__global__ void __launch_bounds__(1024) k_rb_list(const SysDev* __restrict__ sys,
+                                                  const SysState* __restrict__ st, int nsys,
+                                                  int* __restrict__ rb_sys,
+                                                  int64_t* __restrict__ rb_pre,
+                                                  int64_t* __restrict__ rb_cpre,
+                                                  int* __restrict__ rb_n)
+{
+    __shared__ int sf[1024];
+    __shared__ int64_t sp[1024], sc[1024];
+    __shared__ int base_n;
+    __shared__ int64_t base_p, base_c;
+    if (threadIdx.x == 0) {
+        base_n = 0;
+        base_p = 0;
+        base_c = 0;
+    }
+    __syncthreads();
+    for (int s0 = 0; s0 < nsys; s0 += 1024) {
+        const int s = s0 + threadIdx.x;
+        const int f = (s < nsys && st[s].rebuild) ? 1 : 0;
+        const int64_t np = f ? sys[s].n_pad : 0;
+        const int64_t nc = f ? (int64_t)sys[s].nc[0] * sys[s].nc[1] * sys[s].nc[2] : 0;
+        sf[threadIdx.x] = f;
+        sp[threadIdx.x] = np;
+        sc[threadIdx.x] = nc;
+        __syncthreads();
+        for (int o = 1; o < 1024; o <<= 1) {  // inclusive scans
+            int a = 0;
+            int64_t b = 0, c = 0;
+            if ((int)threadIdx.x >= o) {
+                a = sf[threadIdx.x - o];
+                b = sp[threadIdx.x - o];
+                c = sc[threadIdx.x - o];
+            }
+            __syncthreads();
+            sf[threadIdx.x] += a;
+            sp[threadIdx.x] += b;
+            sc[threadIdx.x] += c;
+            __syncthreads();
+        }
+        if (f) {
+            const int k = base_n + sf[threadIdx.x] - 1;
+            rb_sys[k] = s;
+            rb_pre[k] = base_p + sp[threadIdx.x] - np;
+            rb_cpre[k] = base_c + sc[threadIdx.x] - nc;
+        }
+        __syncthreads();
+        if (threadIdx.x == 1023) {
+            base_n += sf[1023];
+            base_p += sp[1023];
+            base_c += sc[1023];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        rb_n[0] = base_n;
+        rb_pre[base_n] = base_p;
+        rb_cpre[base_n] = base_c;
+    }
+}
+
+// global slot of compact slot v, or -1
+__device__ __forceinline__ int64_t rb_slot(const RbList& R, const SysDev* __restrict__ sys, int64_t v)
+{
+    const int n = *R.n;
+    if (v >= R.pre[n]) return -1;
+    const int k = rb_find(R.pre, n, v);
+    return sys[R.sys[k]].atom_off + (v - R.pre[k]);
+}
+
+// global cell of compact cell v, or -1
+__device__ __forceinline__ int rb_cell(const RbList& R, const SysDev* __restrict__ sys, int64_t v)
+{
+    const int n = *R.n;
+    if (v >= R.cpre[n]) return -1;
+    const int k = rb_find(R.cpre, n, v);
+    return sys[R.sys[k]].cell_off + (int)(v - R.cpre[k]);
+}
+
+// loop of a fixed grid over the compact units [0, U), U read on the device
+#define DMDX_FOR_RB_UNITS(u, U) for (int64_t u = blockIdx.x; u < (int64_t)(U); u += gridDim.x)
+
+__device__ __forceinline__ void k_cells_reset_unit(int c, const SysDev* __restrict__ sys, const SysState* __restrict__ st,
                               const int* __restrict__ cell_sys, int ncells_total,
                               int* __restrict__ ccount, int* __restrict__ cfill)
 {
-    int c = (int)(unit_ * blockDim.x + threadIdx.x);
-    if (c >= ncells_total) return;
+    if (c < 0 || c >= ncells_total) return;
     int s = cell_sys[c];
     if (!st[s].rebuild) return;
     ccount[c] = 0;
@@ -57,19 +161,20 @@ __device__ __forceinline__ void k_cells_reset_unit(int64_t unit_, const SysDev* 
 
 __global__ void k_cells_reset(const SysDev* __restrict__ sys, const SysState* __restrict__ st,
                               const int* __restrict__ cell_sys, int ncells_total,
-                              int* __restrict__ ccount, int* __restrict__ cfill, int64_t nunits_)
+                              int* __restrict__ ccount, int* __restrict__ cfill, RbList R)
 {
-    DMDX_FOR_UNITS(u_, nunits_) k_cells_reset_unit(u_, sys, st, cell_sys, ncells_total, ccount, cfill);
+    DMDX_FOR_RB_UNITS(u_, (R.cpre[*R.n] + 255) / 256)
+        k_cells_reset_unit(rb_cell(R, sys, u_ * 256 + threadIdx.x), sys, st, cell_sys, ncells_total,
+                           ccount, cfill);
 }
 
-__device__ __forceinline__ void k_cell_count_unit(int64_t unit_, const SysDev* __restrict__ sys, const SysState* __restrict__ st,
+__device__ __forceinline__ void k_cell_count_unit(int64_t g, const SysDev* __restrict__ sys, const SysState* __restrict__ st,
                              const int* __restrict__ slot_sys, int64_t nslots,
                              const double* __restrict__ x, const double* __restrict__ y,
                              const double* __restrict__ z, int* __restrict__ acell,
                              int* __restrict__ ccount)
 {
-    int64_t g = unit_ * blockDim.x + threadIdx.x;
-    if (g >= nslots) return;
+    if (g < 0 || g >= nslots) return;
     int s = slot_sys[g];
     if (s < 0 || !st[s].rebuild) return;
     const SysDev& S = sys[s];
@@ -87,9 +192,11 @@ __global__ void k_cell_count(const SysDev* __restrict__ sys, const SysState* __r
                              const int* __restrict__ slot_sys, int64_t nslots,
                              const double* __restrict__ x, const double* __restrict__ y,
                              const double* __restrict__ z, int* __restrict__ acell,
-                             int* __restrict__ ccount, int64_t nunits_)
+                             int* __restrict__ ccount, RbList R)
 {
-    DMDX_FOR_UNITS(u_, nunits_) k_cell_count_unit(u_, sys, st, slot_sys, nslots, x, y, z, acell, ccount);
+    DMDX_FOR_RB_UNITS(u_, (R.pre[*R.n] + 255) / 256)
+        k_cell_count_unit(rb_slot(R, sys, u_ * 256 + threadIdx.x), sys, st, slot_sys, nslots, x, y,
+                          z, acell, ccount);
 }
 
 // one workgroup (256 threads) per system
@@ -122,13 +229,12 @@ __global__ void k_cell_scan(const SysDev* __restrict__ sys, SysState* __restrict
     }
 }
 
-__device__ __forceinline__ void k_cell_fill_unit(int64_t unit_, const SysDev* __restrict__ sys, const SysState* __restrict__ st,
+__device__ __forceinline__ void k_cell_fill_unit(int64_t g, const SysDev* __restrict__ sys, const SysState* __restrict__ st,
                             const int* __restrict__ slot_sys, int64_t nslots,
                             const int* __restrict__ acell, const int* __restrict__ cstart,
                             int* __restrict__ cfill, int* __restrict__ cell_atoms)
 {
-    int64_t g = unit_ * blockDim.x + threadIdx.x;
-    if (g >= nslots) return;
+    if (g < 0 || g >= nslots) return;
     int s = slot_sys[g];
     if (s < 0 || !st[s].rebuild) return;
     const SysDev& S = sys[s];
@@ -141,9 +247,11 @@ __device__ __forceinline__ void k_cell_fill_unit(int64_t unit_, const SysDev* __
 __global__ void k_cell_fill(const SysDev* __restrict__ sys, const SysState* __restrict__ st,
                             const int* __restrict__ slot_sys, int64_t nslots,
                             const int* __restrict__ acell, const int* __restrict__ cstart,
-                            int* __restrict__ cfill, int* __restrict__ cell_atoms, int64_t nunits_)
+                            int* __restrict__ cfill, int* __restrict__ cell_atoms, RbList R)
 {
-    DMDX_FOR_UNITS(u_, nunits_) k_cell_fill_unit(u_, sys, st, slot_sys, nslots, acell, cstart, cfill, cell_atoms);
+    DMDX_FOR_RB_UNITS(u_, (R.pre[*R.n] + 255) / 256)
+        k_cell_fill_unit(rb_slot(R, sys, u_ * 256 + threadIdx.x), sys, st, slot_sys, nslots, acell,
+                         cstart, cfill, cell_atoms);
 }
 
 // one 64-thread workgroup per cell: bitonic sort of the cell's slots in LDS
@@ -151,7 +259,7 @@ __global__ void k_cell_fill(const SysDev* __restrict__ sys, const SysState* __re
 // to an insertion sort by one thread)
 constexpr int CELL_SORT_CAP = 1024;
 
-__device__ __forceinline__ void k_cell_sort_unit(int64_t unit_, const SysDev* __restrict__ sys,
+__device__ __forceinline__ void k_cell_sort_unit(int cell, const SysDev* __restrict__ sys,
                                                   const SysState* __restrict__ st,
                                                   const int* __restrict__ cell_sys,
                                                   int ncells_total,
@@ -159,8 +267,8 @@ __device__ __forceinline__ void k_cell_sort_unit(int64_t unit_, const SysDev* __
                                                   const int* __restrict__ cstart,
                                                   int* __restrict__ cell_atoms)
 {
-    const int c = (int)unit_;
-    if (c >= ncells_total) return;
+    const int c = cell;
+    if (c < 0 || c >= ncells_total) return;
     const int s = cell_sys[c];
     if (!st[s].rebuild) return;
     const SysDev& S = sys[s];
@@ -214,18 +322,19 @@ __global__ void __launch_bounds__(64) k_cell_sort(const SysDev* __restrict__ sys
                                                   int ncells_total,
                                                   const int* __restrict__ ccount,
                                                   const int* __restrict__ cstart,
-                                                  int* __restrict__ cell_atoms, int64_t nunits_)
+                                                  int* __restrict__ cell_atoms, RbList R)
 {
-    DMDX_FOR_UNITS(u_, nunits_)
+    DMDX_FOR_RB_UNITS(u_, R.cpre[*R.n])
     {
-        k_cell_sort_unit(u_, sys, st, cell_sys, ncells_total, ccount, cstart, cell_atoms);
+        k_cell_sort_unit(rb_cell(R, sys, u_), sys, st, cell_sys, ncells_total, ccount, cstart,
+                         cell_atoms);
         __syncthreads();  // LDS is reused by the next unit
     }
 }
 
 // after the cell sort: positions in cell order (contiguous per cell, read by
 // the scan below) and the reference positions of the displacement criterion
-__device__ __forceinline__ void k_pack_build_unit(int64_t unit_, const SysDev* __restrict__ sys, const SysState* __restrict__ st,
+__device__ __forceinline__ void k_pack_build_unit(int64_t t, const SysDev* __restrict__ sys, const SysState* __restrict__ st,
                              const int* __restrict__ slot_sys, int64_t nslots,
                              const int* __restrict__ cell_atoms, const int* __restrict__ acell,
                              const double* __restrict__ x,
@@ -234,8 +343,7 @@ __device__ __forceinline__ void k_pack_build_unit(int64_t unit_, const SysDev* _
                              double4* __restrict__ xref, float4* __restrict__ pos4f,
                              double4* __restrict__ org)
 {
-    int64_t t = unit_ * blockDim.x + threadIdx.x;
-    if (t >= nslots) return;
+    if (t < 0 || t >= nslots) return;
     int s = slot_sys[t];
     if (s < 0 || !st[s].rebuild) return;
     const SysDev& S = sys[s];
@@ -268,9 +376,11 @@ __global__ void k_pack_build(const SysDev* __restrict__ sys, const SysState* __r
                              const double* __restrict__ y, const double* __restrict__ z,
                              const double* __restrict__ q, double4* __restrict__ pos4s,
                              double4* __restrict__ xref, float4* __restrict__ pos4f,
-                             double4* __restrict__ org, int64_t nunits_)
+                             double4* __restrict__ org, RbList R)
 {
-    DMDX_FOR_UNITS(u_, nunits_) k_pack_build_unit(u_, sys, st, slot_sys, nslots, cell_atoms, acell, x, y, z, q, pos4s, xref, pos4f, org);
+    DMDX_FOR_RB_UNITS(u_, (R.pre[*R.n] + 255) / 256)
+        k_pack_build_unit(rb_slot(R, sys, u_ * 256 + threadIdx.x), sys, st, slot_sys, nslots,
+                          cell_atoms, acell, x, y, z, q, pos4s, xref, pos4f, org);
 }
 
 // per atom: squared displacement since the last build, per-wave maximum
@@ -460,7 +570,7 @@ __global__ void __launch_bounds__(256) k_nbr_build(
 // the pair set is exactly the FP64 one.  Entries carry the cell offset
 // (NBR_SLOT_BITS) for the pair kernel.
 template <int TPA>
-__device__ __forceinline__ void k_nbr_build_mixed_unit(int64_t unit_,
+__device__ __forceinline__ void k_nbr_build_mixed_unit(int64_t t,
     const SysDev* __restrict__ sys, SysState* __restrict__ st, const int* __restrict__ slot_sys,
     int64_t nslots, const int* __restrict__ acell, const int* __restrict__ ccount,
     const int* __restrict__ cstart, const int* __restrict__ cell_atoms,
@@ -468,10 +578,8 @@ __device__ __forceinline__ void k_nbr_build_mixed_unit(int64_t unit_,
     int* __restrict__ nn, int* __restrict__ nbr, const double4* __restrict__ pos4s,
     const float4* __restrict__ pos4f)
 {
-    constexpr int GPB = 256 / TPA;
     const int lane = threadIdx.x % TPA;
-    const int64_t t = unit_ * GPB + threadIdx.x / TPA;
-    if (t >= nslots) return;
+    if (t < 0 || t >= nslots) return;
     const int s = slot_sys[t];
     if (s < 0 || !st[s].rebuild) return;
     const SysDev& S = sys[s];
@@ -615,9 +723,13 @@ __global__ void __launch_bounds__(256) k_nbr_build_mixed(
     const int* __restrict__ cstart, const int* __restrict__ cell_atoms,
     const int* __restrict__ ex_start, const int* __restrict__ ex_list, int* __restrict__ order,
     int* __restrict__ nn, int* __restrict__ nbr, const double4* __restrict__ pos4s,
-    const float4* __restrict__ pos4f, int64_t nunits_)
+    const float4* __restrict__ pos4f, RbList R)
 {
-    DMDX_FOR_UNITS_SPREAD(u_, nunits_) k_nbr_build_mixed_unit<TPA>(u_, sys, st, slot_sys, nslots, acell, ccount, cstart, cell_atoms, ex_start, ex_list, order, nn, nbr, pos4s, pos4f);
+    constexpr int GPB = 256 / TPA;  // atoms per workgroup pass
+    DMDX_FOR_RB_UNITS(u_, (R.pre[*R.n] + GPB - 1) / GPB)
+        k_nbr_build_mixed_unit<TPA>(rb_slot(R, sys, u_ * GPB + threadIdx.x / TPA), sys, st,
+                                    slot_sys, nslots, acell, ccount, cstart, cell_atoms,
+                                    ex_start, ex_list, order, nn, nbr, pos4s, pos4f);
 }
 
 // one 64-thread workgroup per atom slot: bitonic sort of its row in LDS
@@ -741,22 +853,26 @@ hipError_t launch_remap_rows(hipStream_t s, int* nbr, int64_t count, int* order,
 hipError_t launch_build_lists(const Dev& d, hipStream_t s)
 {
     const int BS = 256;
+    // Worklist of rebuilding systems first; the cell and list kernels then
+    // visit only their cells and slots.  Grids are sized for a full rebuild
+    // and stride over the compact ranges, whose lengths they read on the
+    // device.
+    k_rb_list<<<1, 1024, 0, s>>>(d.sys, d.st, d.nsys, d.rb_sys, d.rb_pre, d.rb_cpre, d.rb_n);
+    const RbList R{d.rb_sys, d.rb_pre, d.rb_cpre, d.rb_n};
     if (d.ncells > 0)
-        k_cells_reset<<<grid_for(blocks_for(d.ncells, BS)), BS, 0, s>>>(d.sys, d.st, d.cell_sys, d.ncells,
-                                                              d.ccount, d.cfill, (int64_t)(blocks_for(d.ncells, BS)));
-    k_cell_count<<<grid_for(blocks_for(d.nslots, BS)), BS, 0, s>>>(d.sys, d.st, d.slot_sys, d.nslots, d.x,
-                                                         d.y, d.z, d.acell, d.ccount, (int64_t)(blocks_for(d.nslots, BS)));
+        k_cells_reset<<<grid_for(blocks_for(d.ncells, BS)), BS, 0, s>>>(d.sys, d.st, d.cell_sys,
+                                                                        d.ncells, d.ccount, d.cfill, R);
+    k_cell_count<<<grid_for(blocks_for(d.nslots, BS)), BS, 0, s>>>(
+        d.sys, d.st, d.slot_sys, d.nslots, d.x, d.y, d.z, d.acell, d.ccount, R);
     k_cell_scan<<<d.nsys, 256, 0, s>>>(d.sys, d.st, d.ccount, d.cstart);
-    k_cell_fill<<<grid_for(blocks_for(d.nslots, BS)), BS, 0, s>>>(d.sys, d.st, d.slot_sys, d.nslots,
-                                                        d.acell, d.cstart, d.cfill,
-                                                        d.cell_atoms, (int64_t)(blocks_for(d.nslots, BS)));
+    k_cell_fill<<<grid_for(blocks_for(d.nslots, BS)), BS, 0, s>>>(
+        d.sys, d.st, d.slot_sys, d.nslots, d.acell, d.cstart, d.cfill, d.cell_atoms, R);
     if (d.ncells > 0)
-        k_cell_sort<<<grid_for((unsigned)d.ncells), 64, 0, s>>>(d.sys, d.st, d.cell_sys, d.ncells, d.ccount,
-                                                      d.cstart, d.cell_atoms, (int64_t)((unsigned)d.ncells));
-    k_pack_build<<<grid_for(blocks_for(d.nslots, BS)), BS, 0, s>>>(d.sys, d.st, d.slot_sys, d.nslots,
-                                                         d.cell_atoms, d.acell, d.x, d.y, d.z, d.q,
-                                                         d.pos4s, d.xref, d.pos4f, d.org,
-                                                         (int64_t)(blocks_for(d.nslots, BS)));
+        k_cell_sort<<<grid_for((unsigned)d.ncells), 64, 0, s>>>(d.sys, d.st, d.cell_sys, d.ncells,
+                                                               d.ccount, d.cstart, d.cell_atoms, R);
+    k_pack_build<<<grid_for(blocks_for(d.nslots, BS)), BS, 0, s>>>(
+        d.sys, d.st, d.slot_sys, d.nslots, d.cell_atoms, d.acell, d.x, d.y, d.z, d.q, d.pos4s,
+        d.xref, d.pos4f, d.org, R);
     if (d.precision == DMDX_FP64)
         k_nbr_build<1><<<grid_for(blocks_for(d.nslots, BS)), BS, 0, s>>>(
             d.sys, d.st, d.slot_sys, d.nslots, d.x, d.y, d.z, d.acell, d.ccount, d.cstart,
@@ -766,7 +882,7 @@ hipError_t launch_build_lists(const Dev& d, hipStream_t s)
         const int64_t units = blocks_for(d.nslots * tpa, BS);
 #define DMDX_BUILD_ARGS                                                                       \
     d.sys, d.st, d.slot_sys, d.nslots, d.acell, d.ccount, d.cstart, d.cell_atoms, d.ex_start, \
-        d.ex_list, d.order, d.nn, d.nbr, d.pos4s, d.pos4f, units
+        d.ex_list, d.order, d.nn, d.nbr, d.pos4s, d.pos4f, R
         if (tpa == 8) k_nbr_build_mixed<8><<<grid_for(units), BS, 0, s>>>(DMDX_BUILD_ARGS);
         else if (tpa == 32) k_nbr_build_mixed<32><<<grid_for(units), BS, 0, s>>>(DMDX_BUILD_ARGS);
         else k_nbr_build_mixed<16><<<grid_for(units), BS, 0, s>>>(DMDX_BUILD_ARGS);
