@@ -22,7 +22,7 @@ STATUS_NAMES = {
 FAITHFUL, CORRECTED = 0, 1
 FP64, MIXED = 0, 1
 COULOMB_NONE, COULOMB_CUTOFF, COULOMB_PME = 0, 1, 2
-THERMO_NONE, THERMO_BERENDSEN, THERMO_NOSE_HOOVER = 0, 1, 2
+THERMO_NONE, THERMO_BERENDSEN, THERMO_NOSE_HOOVER, THERMO_ANDERSEN = 0, 1, 2, 3
 BARO_NONE, BARO_BERENDSEN, BARO_ANDERSEN = 0, 1, 2
 
 _dp = C.POINTER(C.c_double)
@@ -59,6 +59,9 @@ class SystemDesc(C.Structure):
         ("barostat", C.c_int32), ("baro_pressure", C.c_double), ("baro_tau", C.c_double),
         ("baro_compressibility", C.c_double),
         ("physics", C.c_int32), ("precision", C.c_int32),
+        ("n_constraints", C.c_int64), ("con_i", _ip), ("con_j", _ip),
+        ("con_distance", _dp), ("con_tolerance", C.c_double),
+        ("andersen_frequency", C.c_double), ("andersen_calls", C.c_int64),
     ]
 
 

@@ -91,6 +91,11 @@ struct SysDev {
     int32_t thermostat, barostat;
     double T0, ttau, Qnh;
     double P0, ptau, beta, Wab;
+    double and_prob;     // Andersen collision probability per call (frequency * dt)
+    // SHAKE / RATTLE
+    int32_t n_con;       // constraints of this system
+    int32_t pad_con_;
+    double con_tol;      // ConstraintsParams::tolerance
     // partial-sum ranges (in waves) for the deterministic reductions
     int32_t aw_off, aw_n;      // atom waves
     int32_t bw_off, bw_n;      // bond waves
@@ -125,6 +130,7 @@ struct SysState {
     int64_t stop_at;     // > 0: the system leaves the batch once steps_done reaches it
     int32_t nub[3];      // cells per edge of the last list build
     int32_t pad_;
+    int64_t and_calls;   // Andersen apply() calls made (generator seed 42 + calls)
 };
 
 enum { FLAG_NBR_OVERFLOW = 1, FLAG_NONFINITE = 2 };
@@ -233,6 +239,15 @@ struct Dev {
     int* grid_sys = nullptr;
     void* tw = nullptr;  // twiddles, same real type
     float* ginf = nullptr;  // mixed path: tabulated influence function (fixed boxes)
+    // SHAKE / RATTLE: constraints grouped by connected component (atoms
+    // linked by constraints), each component's constraints in the caller's
+    // order; one thread iterates a component
+    int ncomp = 0;
+    int* k_sys = nullptr;      // ncomp: system of the component
+    int* k_start = nullptr;    // ncomp + 1: first constraint of the component
+    int *k_i = nullptr, *k_j = nullptr;  // global slots
+    double* k_d = nullptr;     // target distances
+    bool any_andersen = false;
     // rebuild worklist (k_rb_list): systems rebuilding their lists this step
     int* rb_sys = nullptr;       // nsys
     int64_t* rb_pre = nullptr;   // nsys + 1: running slot totals
@@ -273,6 +288,10 @@ hipError_t launch_kick_ke(const Dev& d, hipStream_t s);
 hipError_t launch_finalize(const Dev& d, hipStream_t s, int with_kinetic);
 hipError_t launch_barostat(const Dev& d, hipStream_t s);
 hipError_t launch_materialize(const Dev& d, hipStream_t s);
+hipError_t launch_shake(const Dev& d, hipStream_t s);
+hipError_t launch_rattle(const Dev& d, hipStream_t s);
+hipError_t launch_andersen(const Dev& d, hipStream_t s);
+hipError_t launch_ke_refresh(const Dev& d, hipStream_t s);
 hipError_t launch_remap_rows(hipStream_t s, int* nbr, int64_t count, int* order, int64_t n_order,
                              int64_t delta);
 

@@ -271,7 +271,12 @@ int Batch::create(const dmdx_system_desc* descs, int nsys, const BatchOptions& o
         // integration (src/thermostat/nose_hoover.cpp:6-11,
         // src/barostat/andersen_barostat.cpp:8-14)
         S.dt = D.dt;
+        if (D.thermostat < DMDX_THERMO_NONE || D.thermostat > DMDX_THERMO_ANDERSEN)
+            return fail(DMDX_E_PARM, "unknown thermostat");
+        if (D.barostat < DMDX_BARO_NONE || D.barostat > DMDX_BARO_ANDERSEN)
+            return fail(DMDX_E_PARM, "unknown barostat");
         S.thermostat = D.thermostat;
+        S.and_prob = D.andersen_frequency * D.dt;  // src/thermostat/andersen.cpp:15
         S.T0 = D.thermo_temperature;
         S.ttau = D.thermo_tau;
         double dof = 3.0 * (double)n - 3.0;
@@ -289,6 +294,7 @@ int Batch::create(const dmdx_system_desc* descs, int nsys, const BatchOptions& o
         SysState& T = hstate[s];
         std::memset(&T, 0, sizeof(T));
         T.counter = D.step_since_rebuild;
+        T.and_calls = D.andersen_calls;
         T.lambda = 1.0;
         T.active = 1;
         T.valid = 0;
@@ -587,6 +593,70 @@ int Batch::create(const dmdx_system_desc* descs, int nsys, const BatchOptions& o
         }
     }
 
+    // ---- SHAKE / RATTLE constraints (src/integrate/constraints.cpp): group
+    // each system's constraints by connected component of the atoms they
+    // link, keeping the caller's order inside a component; components are
+    // ordered by their first constraint.  The reference's sweeps touch
+    // different components' atoms independently, so one thread per component
+    // reproduces them exactly.
+    std::vector<int> k_sys, k_start, k_i, k_j;
+    std::vector<double> k_d;
+    for (int s = 0; s < nsys; ++s) {
+        const dmdx_system_desc& D = descs[s];
+        SysDev& S = hsys[s];
+        if (D.n_constraints <= 0 || opt.component != COMP_NONE) continue;
+        if (!D.con_i || !D.con_j || !D.con_distance)
+            return fail(DMDX_E_INVALID, "constraint arrays are required");
+        if (D.n_constraints > (int64_t)1 << 30) return fail(DMDX_E_PARM, "too many constraints");
+        const int n = S.n;
+        const int64_t nc = D.n_constraints;
+        std::vector<int> parent(n);
+        for (int a = 0; a < n; ++a) parent[a] = a;
+        auto root = [&](int a) {
+            while (parent[a] != a) a = parent[a] = parent[parent[a]];
+            return a;
+        };
+        for (int64_t c = 0; c < nc; ++c) {
+            const int a = D.con_i[c], b = D.con_j[c];
+            if (a < 0 || a >= n || b < 0 || b >= n || a == b)
+                return fail(DMDX_E_PARM, "constraint atom index out of range or repeated");
+            if (!(D.con_distance[c] > 0.0))
+                return fail(DMDX_E_PARM, "constraint distances must be positive");
+            const int ra = root(a), rb = root(b);
+            if (ra != rb) parent[std::max(ra, rb)] = std::min(ra, rb);
+        }
+        std::vector<int> comp_of(n, -1);
+        std::vector<std::vector<int64_t>> members;
+        for (int64_t c = 0; c < nc; ++c) {
+            const int r = root(D.con_i[c]);
+            if (comp_of[r] < 0) {
+                comp_of[r] = (int)members.size();
+                members.emplace_back();
+            }
+            members[comp_of[r]].push_back(c);
+        }
+        for (const auto& mem : members) {
+            k_sys.push_back(s);
+            k_start.push_back((int)k_i.size());
+            for (int64_t c : mem) {
+                k_i.push_back((int)(S.atom_off + D.con_i[c]));
+                k_j.push_back((int)(S.atom_off + D.con_j[c]));
+                k_d.push_back(D.con_distance[c]);
+            }
+        }
+        S.n_con = (int32_t)nc;
+        S.con_tol = D.con_tolerance;
+    }
+    d.ncomp = (int)k_sys.size();
+    k_start.push_back((int)k_i.size());
+    if (d.ncomp > 0) {
+        TRY(dev_upload(&d.k_sys, k_sys, owned));
+        TRY(dev_upload(&d.k_start, k_start, owned));
+        TRY(dev_upload(&d.k_i, k_i, owned));
+        TRY(dev_upload(&d.k_j, k_j, owned));
+        TRY(dev_upload(&d.k_d, k_d, owned));
+    }
+
     // ---- what exists
     for (int s = 0; s < nsys; ++s) {
         const SysDev& S = hsys[s];
@@ -595,6 +665,7 @@ int Batch::create(const dmdx_system_desc* descs, int nsys, const BatchOptions& o
         d.any_coul_pairs |= S.coul_pairs != 0;
         d.any_excl |= S.n_ex > 0;
         d.any_baro |= S.barostat != DMDX_BARO_NONE;
+        d.any_andersen |= S.thermostat == DMDX_THERMO_ANDERSEN;
         host[s].has_list = S.has_list != 0;
         host[s].interval = S.interval;
         host[s].counter = hstate[s].counter;
@@ -877,8 +948,12 @@ int Batch::enqueue_step(bool uniform)
 {
     TRY(enqueue_compute(1, 0, 0, uniform));
     LAUNCH(timed(KG_INTEGRATE, launch_kick_drift));
+    if (d.ncomp) LAUNCH(timed(KG_INTEGRATE, launch_shake));
     TRY(enqueue_compute(2, 0, 0, uniform));
     LAUNCH(timed(KG_INTEGRATE, launch_kick_ke));
+    if (d.ncomp) LAUNCH(timed(KG_INTEGRATE, launch_rattle));
+    if (d.any_andersen) LAUNCH(timed(KG_INTEGRATE, launch_andersen));
+    if (d.ncomp || d.any_andersen) LAUNCH(launch_ke_refresh(d, stream));
     LAUNCH(launch_finalize(d, stream, 1));
     if (d.any_baro) {
         LAUNCH(launch_barostat(d, stream));

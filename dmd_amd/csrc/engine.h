@@ -103,6 +103,9 @@ struct DescCopy {
         d.imp_phi0 = keep(dv, s.imp_phi0, s.n_impropers);
         d.excl_i = keep(iv, s.excl_i, s.n_excl);
         d.excl_j = keep(iv, s.excl_j, s.n_excl);
+        d.con_i = keep(iv, s.con_i, s.n_constraints);
+        d.con_j = keep(iv, s.con_j, s.n_constraints);
+        d.con_distance = keep(dv, s.con_distance, s.n_constraints);
     }
     DescCopy(const DescCopy&) = delete;
     DescCopy& operator=(const DescCopy&) = delete;

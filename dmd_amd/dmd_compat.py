@@ -12,8 +12,9 @@ instead.  Differences, all explicit:
   output is implementation-defined (simulation_engine.cpp:339-383); here the
   velocities are drawn with a seeded Philox generator, COM removed and
   rescaled to ``init_temperature`` with 3N-3 degrees of freedom, as DMD does.
-* ``constraint_type = "shake"`` and the Andersen thermostat are not on the
-  device and raise ``NotImplementedError``.
+* ``constraint_type = "shake"`` (SHAKE after the drift, RATTLE after the
+  closing kick) and the Andersen thermostat (DMD's own mt19937_64 stream)
+  run on the device like the other couplings.
 * trajectory and energy files (``nstxout``, H5MD) are not written; the
   state is read through the properties.
 * ``precision`` (``"fp64"`` or ``"mixed"``) and ``physics`` (``"faithful"``
@@ -121,14 +122,12 @@ def _velocities(cfg, masses):
 
 def spec_from_config(cfg: SimulationConfig) -> SystemSpec:
     """build_simulation (simulation_engine.cpp:248-386) as a SystemSpec."""
-    if cfg.constraint_type == "shake" and len(cfg.constraint_i):
-        raise NotImplementedError("SHAKE/RATTLE constraints are not on the device path")
     thermo = {"none": _abi.THERMO_NONE, "berendsen": _abi.THERMO_BERENDSEN,
-              "nose-hoover": _abi.THERMO_NOSE_HOOVER}
+              "nose-hoover": _abi.THERMO_NOSE_HOOVER, "andersen": _abi.THERMO_ANDERSEN}
     baro = {"none": _abi.BARO_NONE, "berendsen": _abi.BARO_BERENDSEN,
             "andersen": _abi.BARO_ANDERSEN}
     if cfg.thermostat.type not in thermo:
-        raise NotImplementedError(f"thermostat {cfg.thermostat.type!r} is not on the device path")
+        raise ValueError(f"Unknown thermostat type: {cfg.thermostat.type}")
     if cfg.barostat.type not in baro:
         raise ValueError(f"Unknown barostat type: {cfg.barostat.type}")
     n = len(cfg.pos_x)
@@ -158,6 +157,9 @@ def spec_from_config(cfg: SimulationConfig) -> SystemSpec:
                            cfg.improper_k_phi, cfg.improper_phi0)
     if len(cfg.excl_i):
         kw["exclusions"] = (cfg.excl_i, cfg.excl_j)
+    if cfg.constraint_type == "shake" and len(cfg.constraint_i):  # simulation_engine.cpp:293-301
+        kw["constraints"] = (cfg.constraint_i, cfg.constraint_j, cfg.constraint_dist)
+        kw["constraint_tolerance"] = float(cfg.constraint_tolerance)
     ct = cfg.coulomb_type
     coul = (_abi.COULOMB_PME if ct == "pme" else
             _abi.COULOMB_CUTOFF if ct in ("direct", "cutoff") else _abi.COULOMB_NONE)
@@ -172,7 +174,8 @@ def spec_from_config(cfg: SimulationConfig) -> SystemSpec:
         ewald_coeff=float(cfg.ewald_coeff), pme_grid=(g, g, g), pme_order=int(cfg.pme_order),
         dt=float(cfg.dt), thermostat=thermo[cfg.thermostat.type],
         thermo_temperature=float(cfg.thermostat.temperature),
-        thermo_tau=float(cfg.thermostat.tau), barostat=baro[cfg.barostat.type],
+        thermo_tau=float(cfg.thermostat.tau),
+        andersen_frequency=float(cfg.thermostat.frequency), barostat=baro[cfg.barostat.type],
         baro_pressure=float(cfg.barostat.pressure), baro_tau=float(cfg.barostat.tau),
         baro_compressibility=float(cfg.barostat.compressibility),
         physics=_abi.CORRECTED if cfg.physics == "corrected" else _abi.FAITHFUL,

@@ -66,6 +66,10 @@ class SystemSpec:
     baro_compressibility: float = 4.5e-5
     physics: int = _abi.FAITHFUL
     precision: int = _abi.FP64
+    constraints: tuple | None = None     # (i, j, distance) SHAKE/RATTLE pairs
+    constraint_tolerance: float = 1e-6   # SimulationConfig::constraint_tolerance
+    andersen_frequency: float = 10.0     # AndersenParams::frequency (1/ps)
+    andersen_calls: int = 0              # apply() calls already made
     name: str = ""
     _keep: list = field(default_factory=list, repr=False)
 
@@ -154,5 +158,12 @@ class SystemSpec:
         d.baro_compressibility = float(self.baro_compressibility)
         d.physics = int(self.physics)
         d.precision = int(self.precision)
+        if self.constraints is not None and len(self.constraints[0]):
+            ci, cj, cd = self.constraints
+            d.n_constraints = len(ci)
+            d.con_i, d.con_j, d.con_distance = ip(ci), ip(cj), dp(cd)
+        d.con_tolerance = float(self.constraint_tolerance)
+        d.andersen_frequency = float(self.andersen_frequency)
+        d.andersen_calls = int(self.andersen_calls)
         self._keep = keep
         return d

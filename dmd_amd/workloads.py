@@ -172,9 +172,12 @@ def _random_rotations(rng, n):
 def water_box(n_side: int = 16, cutoff: float = 0.9, rtol: float = 1e-5, grid: int | None = None,
               spacing: float = 0.1, seed: int = 42, dt: float = 0.0005, temperature: float = 300.0,
               thermostat=_abi.THERMO_NOSE_HOOVER, skin: float = 0.1, n_mol: int | None = None,
-              **kw) -> SystemSpec:
+              rigid: bool = False, **kw) -> SystemSpec:
     """Flexible 3-site water at 1.0 g/cm^3: O on a jittered cubic lattice
-    (+-0.02 nm), H at r0 with the HOH angle theta0 in a random orientation."""
+    (+-0.02 nm), H at r0 with the HOH angle theta0 in a random orientation.
+    rigid: no bond or angle terms; instead three SHAKE/RATTLE constraints per
+    molecule (O-H1, O-H2, H1-H2) at the same geometry, as
+    docs/adr/0004-use-shake-for-water.md lays them out."""
     nmol = n_mol if n_mol is not None else n_side ** 3
     side = int(math.ceil(nmol ** (1.0 / 3.0) - 1e-9))
     vol = nmol * (WATER_M_O + 2 * WATER_M_H) / (NA * 1.0) * 1e21  # nm^3
@@ -202,6 +205,13 @@ def water_box(n_side: int = 16, cutoff: float = 0.9, rtol: float = 1e-5, grid: i
              np.full(2 * nmol, WATER_KB), np.full(2 * nmol, WATER_R0))
     angles = (base + 1, base, base + 2, np.full(nmol, WATER_KA), np.full(nmol, WATER_THETA0))
     excl = (np.concatenate([base, base, base + 1]), np.concatenate([base + 1, base + 2, base + 2]))
+    cons = None
+    if rigid:
+        d_hh = 2.0 * WATER_R0 * math.sin(half)
+        cons = (np.stack([base, base, base + 1], 1).ravel(),
+                np.stack([base + 1, base + 2, base + 2], 1).ravel(),
+                np.tile([WATER_R0, WATER_R0, d_hh], nmol))
+        bonds = angles = None
     sig = np.array([[WATER_SIG_O, 1.0], [1.0, 1.0]])
     eps = np.array([[WATER_EPS_O, 0.0], [0.0, 0.0]])
     vel = maxwell_boltzmann(rng, masses, temperature)
@@ -213,7 +223,8 @@ def water_box(n_side: int = 16, cutoff: float = 0.9, rtol: float = 1e-5, grid: i
         coulomb_type=_abi.COULOMB_PME, coulomb_cutoff=cutoff,
         ewald_coeff=ewald_alpha(cutoff, rtol), pme_grid=(g, g, g), dt=dt,
         thermostat=thermostat, thermo_temperature=temperature, thermo_tau=0.1,
-        physics=_abi.CORRECTED, rebuild_interval=-1, name=f"water_{n}")
+        physics=_abi.CORRECTED, rebuild_interval=-1, constraints=cons,
+        name=f"{'rigid_' if rigid else ''}water_{n}")
     return spec.with_(**kw) if kw else spec
 
 

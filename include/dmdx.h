@@ -62,7 +62,12 @@ enum { DMDX_COULOMB_NONE = 0, DMDX_COULOMB_CUTOFF = 1, DMDX_COULOMB_PME = 2 };
 
 /* ---- coupling (ThermostatConfig / BarostatConfig,
  *      src/sim/simulation_config.h:13-25) */
-enum { DMDX_THERMO_NONE = 0, DMDX_THERMO_BERENDSEN = 1, DMDX_THERMO_NOSE_HOOVER = 2 };
+enum {
+    DMDX_THERMO_NONE = 0,
+    DMDX_THERMO_BERENDSEN = 1,
+    DMDX_THERMO_NOSE_HOOVER = 2,
+    DMDX_THERMO_ANDERSEN = 3
+};
 enum { DMDX_BARO_NONE = 0, DMDX_BARO_BERENDSEN = 1, DMDX_BARO_ANDERSEN = 2 };
 
 /*
@@ -128,6 +133,24 @@ typedef struct dmdx_system_desc {
 
     int32_t physics;               /* DMDX_FAITHFUL | DMDX_CORRECTED          */
     int32_t precision;             /* DMDX_FP64 | DMDX_MIXED                  */
+
+    /* SHAKE / RATTLE distance constraints (ConstraintsParams,
+     * src/integrate/constraints.h:8-13): positions are corrected after the
+     * drift and velocities after the closing kick
+     * (src/sim/simulation_engine.cpp:67,78).  build_simulation creates them
+     * for constraint_type "shake" (:293-301).  0 constraints: none. */
+    int64_t n_constraints;
+    const int32_t *con_i, *con_j;
+    const double *con_distance;    /* target distances (nm)                   */
+    double con_tolerance;          /* SimulationConfig::constraint_tolerance  */
+
+    /* Andersen thermostat (DMDX_THERMO_ANDERSEN; AndersenParams,
+     * src/thermostat/andersen.h:5-8): thermo_temperature is its T0. */
+    double andersen_frequency;     /* collisions per ps                       */
+    int64_t andersen_calls;        /* apply() calls already made: the call
+                                      k draws from mt19937_64 seeded with
+                                      42 + k (src/thermostat/andersen.h:18,
+                                      andersen.cpp:19,31)                     */
 } dmdx_system_desc;
 
 /* Scalars of one system after the last step / compute. */

@@ -352,14 +352,25 @@ public:
         d.excl_j = ej_.data();
         d.dt = cfg.dt;
         const auto& t = cfg.thermostat;
-        d.thermostat = t.type == "berendsen" ? DMDX_THERMO_BERENDSEN
+        d.thermostat = t.type == "berendsen"     ? DMDX_THERMO_BERENDSEN
                        : t.type == "nose-hoover" ? DMDX_THERMO_NOSE_HOOVER
+                       : t.type == "andersen"    ? DMDX_THERMO_ANDERSEN
                                                  : DMDX_THERMO_NONE;
         if (t.type != "none" && d.thermostat == DMDX_THERMO_NONE)
-            throw std::runtime_error("thermostat not available on the device (the stochastic "
-                                     "Andersen collisions are not ported): " + t.type);
+            throw std::runtime_error("Unknown thermostat type: " + t.type);
         d.thermo_temperature = t.temperature;
         d.thermo_tau = t.tau;
+        d.andersen_frequency = t.frequency;
+        // constraint_type "shake" with pairs (src/sim/simulation_engine.cpp:293-301)
+        if (cfg.constraint_type == "shake" && !cfg.constraint_i.empty()) {
+            ci_.assign(cfg.constraint_i.begin(), cfg.constraint_i.end());
+            cj_.assign(cfg.constraint_j.begin(), cfg.constraint_j.end());
+            d.n_constraints = (int64_t)ci_.size();
+            d.con_i = ci_.data();
+            d.con_j = cj_.data();
+            d.con_distance = cfg.constraint_dist.data();
+            d.con_tolerance = cfg.constraint_tolerance;
+        }
         const auto& b = cfg.barostat;
         d.barostat = b.type == "berendsen" ? DMDX_BARO_BERENDSEN
                      : b.type == "andersen" ? DMDX_BARO_ANDERSEN
@@ -455,7 +466,7 @@ private:
     int step_ = 0;
     double time_ = 0.0;
     std::vector<int32_t> types_, bi_, bj_, ai_, aj_, ak_, di_, dj_, dk_, dl_, dp_, ii_, ij_, ik_,
-        il_, ei_, ej_;
+        il_, ei_, ej_, ci_, cj_;
     std::vector<double> bk_, br_, akt_, at0_, dkp_, dp0_, ikp_, ip0_;
 };
 

@@ -343,6 +343,71 @@ def integrate_op(kind, m, pos, vel, force, box, dt, target=0.0, tau=0.0, beta=0.
     return np.stack([x, y, z], 1), np.stack([vx, vy, vz], 1), b, st.value
 
 
+def shake(pos, box, m, ci, cj, dist, tol, use_ref=False):
+    """SHAKE positions (src/integrate/constraints.cpp:9-49); returns pos."""
+    m = f64(m)
+    n = len(m)
+    x, y, z = [a.copy() for a in xyz(pos)]
+    ci, cj, dist = i32(ci), i32(cj), f64(dist)
+    if use_ref:
+        v = [np.zeros(n) for _ in range(3)]
+        ref().ref_constraints(C.c_int(0), C.c_int64(len(ci)), P(ci), P(cj), P(dist),
+                              C.c_double(tol), C.c_int64(n), P(m), P(x), P(y), P(z),
+                              *[P(a) for a in v], P(f64(box)))
+    else:
+        lib().orc_shake(C.c_int64(len(ci)), P(ci), P(cj), P(dist), C.c_double(tol),
+                        P(cell9(box)), P(m), P(x), P(y), P(z))
+    return np.stack([x, y, z], 1)
+
+
+def rattle(pos, vel, box, m, ci, cj, tol, use_ref=False):
+    """RATTLE velocities (src/integrate/constraints.cpp:51-91); returns vel."""
+    m = f64(m)
+    n = len(m)
+    x, y, z = xyz(pos)
+    vx, vy, vz = [a.copy() for a in xyz(vel)]
+    ci, cj = i32(ci), i32(cj)
+    if use_ref:
+        xs = [a.copy() for a in (x, y, z)]
+        d = np.ones(len(ci))
+        ref().ref_constraints(C.c_int(1), C.c_int64(len(ci)), P(ci), P(cj), P(d),
+                              C.c_double(tol), C.c_int64(n), P(m), *[P(a) for a in xs],
+                              P(vx), P(vy), P(vz), P(f64(box)))
+    else:
+        lib().orc_rattle(C.c_int64(len(ci)), P(ci), P(cj), C.c_double(tol), P(cell9(box)),
+                         P(m), P(x), P(y), P(z), P(vx), P(vy), P(vz))
+    return np.stack([vx, vy, vz], 1)
+
+
+class _Mt64(C.Structure):
+    _fields_ = [("mt", C.c_uint64 * 312), ("idx", C.c_int)]
+
+
+def mt64(seed, count):
+    """The first `count` outputs of mt19937_64 seeded with `seed`."""
+    g = _Mt64()
+    o = lib()
+    o.orc_mt64_next.restype = C.c_uint64
+    o.orc_mt64_seed(C.byref(g), C.c_uint64(seed))
+    return np.array([o.orc_mt64_next(C.byref(g)) for _ in range(count)], dtype=np.uint64)
+
+
+def andersen(m, vel, dt, T0, frequency, calls=0, use_ref=False):
+    """One Andersen apply() (src/thermostat/andersen.cpp:14-32) after `calls`
+    earlier calls; returns vel."""
+    m = f64(m)
+    n = len(m)
+    vx, vy, vz = [a.copy() for a in xyz(vel)]
+    if use_ref:
+        ref().ref_andersen(C.c_int64(n), P(m), P(vx), P(vy), P(vz), C.c_double(dt),
+                           C.c_double(T0), C.c_double(frequency), C.c_int64(calls))
+    else:
+        lib().orc_andersen(C.c_int64(n), P(m), P(vx), P(vy), P(vz), C.c_double(dt),
+                           C.c_double(T0), C.c_double(frequency),
+                           C.c_uint32((42 + calls) & 0xFFFFFFFF))
+    return np.stack([vx, vy, vz], 1)
+
+
 # ------------------------------------------------------------- simulation
 
 class Sim:
@@ -439,6 +504,15 @@ class RefSim:
             C.c_double(spec.thermo_temperature), C.c_double(spec.thermo_tau),
             C.c_int(spec.barostat), C.c_double(spec.baro_pressure), C.c_double(spec.baro_tau),
             C.c_double(spec.baro_compressibility))
+        if spec.constraints is not None and len(spec.constraints[0]):
+            ci, cj, cd = i32(spec.constraints[0]), i32(spec.constraints[1]), f64(spec.constraints[2])
+            r.ref_sim_set_constraints(C.c_void_p(self.h), C.c_int64(len(ci)), P(ci), P(cj), P(cd),
+                                      C.c_double(spec.constraint_tolerance))
+        if spec.thermostat == 3:
+            if spec.andersen_calls:
+                raise ValueError("the reference thermostat starts at seed 42")
+            r.ref_sim_set_andersen(C.c_void_p(self.h), C.c_double(spec.thermo_temperature),
+                                   C.c_double(spec.andersen_frequency))
 
     def __del__(self):
         if getattr(self, "h", None):

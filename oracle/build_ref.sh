@@ -18,8 +18,9 @@ SRCS=(
   force/force_component.cpp force/force_engine.cpp force/lennard_jones.cpp
   force/coulomb_direct.cpp force/coulomb_exclusion.cpp force/harmonic_bond.cpp
   force/harmonic_angle.cpp force/periodic_dihedral.cpp force/harmonic_improper.cpp
-  integrate/integrator.cpp
+  integrate/integrator.cpp integrate/constraints.cpp
   thermostat/thermostat.cpp thermostat/berendsen.cpp thermostat/nose_hoover.cpp
+  thermostat/andersen.cpp
   barostat/barostat.cpp barostat/berendsen_barostat.cpp barostat/andersen_barostat.cpp
 )
 ARGS=()

@@ -996,6 +996,172 @@ void orc_andersen_barostat(int64_t n, const double *m, const double *vx, const d
 
 /* ------------------------------------------------------------ simulation */
 
+/* ------------------------------------------------------------ constraints */
+
+/* src/integrate/constraints.cpp:9-49: Gauss-Seidel sweeps over all
+ * constraints in order, at most 100, until a sweep's largest |r^2 - d^2| is
+ * below the tolerance; positions only. */
+void orc_shake(int64_t nc, const int32_t *ci, const int32_t *cj, const double *dist, double tol,
+               const double cell[9], const double *m, double *x, double *y, double *z)
+{
+    for (int iter = 0; iter < 100; ++iter) {
+        double max_error = 0.0;
+        for (int64_t c = 0; c < nc; ++c) {
+            int ai = ci[c], aj = cj[c];
+            double dx = x[aj] - x[ai];
+            double dy = y[aj] - y[ai];
+            double dz = z[aj] - z[ai];
+            orc_min_image(cell, &dx, &dy, &dz);
+            double r2 = dx * dx + dy * dy + dz * dz;
+            double d2 = dist[c] * dist[c];
+            double error = r2 - d2;
+            if (fabs(error) > max_error) max_error = fabs(error);
+            if (fabs(error) > tol) {
+                double imi = 1.0 / m[ai];
+                double imj = 1.0 / m[aj];
+                double ims = imi + imj;
+                double lambda = error / (2.0 * d2 * ims);
+                x[ai] += lambda * dx * imi;
+                y[ai] += lambda * dy * imi;
+                z[ai] += lambda * dz * imi;
+                x[aj] -= lambda * dx * imj;
+                y[aj] -= lambda * dy * imj;
+                z[aj] -= lambda * dz * imj;
+            }
+        }
+        if (max_error < tol) break;
+    }
+}
+
+/* src/integrate/constraints.cpp:51-91: the same sweeps on |r . v_rel|,
+ * velocities only. */
+void orc_rattle(int64_t nc, const int32_t *ci, const int32_t *cj, double tol,
+                const double cell[9], const double *m, const double *x, const double *y,
+                const double *z, double *vx, double *vy, double *vz)
+{
+    for (int iter = 0; iter < 100; ++iter) {
+        double max_error = 0.0;
+        for (int64_t c = 0; c < nc; ++c) {
+            int ai = ci[c], aj = cj[c];
+            double dx = x[aj] - x[ai];
+            double dy = y[aj] - y[ai];
+            double dz = z[aj] - z[ai];
+            orc_min_image(cell, &dx, &dy, &dz);
+            double dvx = vx[aj] - vx[ai];
+            double dvy = vy[aj] - vy[ai];
+            double dvz = vz[aj] - vz[ai];
+            double r_dot_v = dx * dvx + dy * dvy + dz * dvz;
+            double r2 = dx * dx + dy * dy + dz * dz;
+            double err = fabs(r_dot_v);
+            if (err > max_error) max_error = err;
+            if (err > tol) {
+                double imi = 1.0 / m[ai];
+                double imj = 1.0 / m[aj];
+                double ims = imi + imj;
+                double lambda = r_dot_v / (r2 * ims);
+                vx[ai] += lambda * dx * imi;
+                vy[ai] += lambda * dy * imi;
+                vz[ai] += lambda * dz * imi;
+                vx[aj] -= lambda * dx * imj;
+                vy[aj] -= lambda * dy * imj;
+                vz[aj] -= lambda * dz * imj;
+            }
+        }
+        if (max_error < tol) break;
+    }
+}
+
+/* ------------------------------------------------------------ Andersen */
+
+/* mt19937_64 as the C++ standard defines it ([rand.eng.mers], parameters
+ * from [rand.predef]), written from that definition: seeding
+ * x_i = f (x_{i-1} ^ (x_{i-1} >> 62)) + i, the twist over the 312-word
+ * state, then tempering. */
+void orc_mt64_seed(orc_mt64 *g, uint64_t seed)
+{
+    g->mt[0] = seed;
+    for (int i = 1; i < 312; ++i)
+        g->mt[i] = 6364136223846793005ull * (g->mt[i - 1] ^ (g->mt[i - 1] >> 62)) + (uint64_t)i;
+    g->idx = 312;
+}
+
+uint64_t orc_mt64_next(orc_mt64 *g)
+{
+    if (g->idx >= 312) {
+        for (int i = 0; i < 312; ++i) {
+            uint64_t x = (g->mt[i] & 0xFFFFFFFF80000000ull) |
+                         (g->mt[(i + 1) % 312] & 0x7FFFFFFFull);
+            uint64_t xa = x >> 1;
+            if (x & 1ull) xa ^= 0xB5026F5AA96619E9ull;
+            g->mt[i] = g->mt[(i + 156) % 312] ^ xa;
+        }
+        g->idx = 0;
+    }
+    uint64_t y = g->mt[g->idx++];
+    y ^= (y >> 29) & 0x5555555555555555ull;
+    y ^= (y << 17) & 0x71D67FFFEDA60000ull;
+    y ^= (y << 37) & 0xFFF7EEE000000000ull;
+    y ^= y >> 43;
+    return y;
+}
+
+/* std::generate_canonical<double, 53> over a 64-bit engine (libstdc++,
+ * GCC 11): one output, converted and scaled by 2^-64, kept below 1. */
+static double mt64_canonical(orc_mt64 *g)
+{
+    double r = (double)orc_mt64_next(g) / 18446744073709551616.0;
+    if (r >= 1.0) r = nextafter(1.0, 0.0);
+    return r;
+}
+
+/* std::normal_distribution<double>(0, 1) of libstdc++: Marsaglia's polar
+ * method, returning y * mult and keeping x * mult for the next draw. */
+typedef struct {
+    double saved;
+    int available;
+} orc_normal;
+
+static double orc_normal_draw(orc_normal *nd, orc_mt64 *g)
+{
+    double ret;
+    if (nd->available) {
+        nd->available = 0;
+        ret = nd->saved;
+    } else {
+        double x, y, r2;
+        do {
+            x = 2.0 * mt64_canonical(g) - 1.0;
+            y = 2.0 * mt64_canonical(g) - 1.0;
+            r2 = x * x + y * y;
+        } while (r2 > 1.0 || r2 == 0.0);
+        double mult = sqrt(-2.0 * log(r2) / r2);
+        nd->saved = x * mult;
+        nd->available = 1;
+        ret = y * mult;
+    }
+    return ret * 1.0 + 0.0;
+}
+
+/* src/thermostat/andersen.cpp:14-32, one apply() call with the generator
+ * seeded by `seed` (42 + calls made before). */
+void orc_andersen(int64_t n, const double *m, double *vx, double *vy, double *vz, double dt,
+                  double T0, double frequency, uint32_t seed)
+{
+    double prob = frequency * dt;
+    if (prob <= 0.0) return;
+    orc_mt64 g;
+    orc_mt64_seed(&g, (uint64_t)seed);
+    orc_normal nd = {0.0, 0};
+    for (int64_t i = 0; i < n; ++i) {
+        if (mt64_canonical(&g) < prob) {
+            double sigma = sqrt(ORC_KB * T0 / m[i]);
+            vx[i] = orc_normal_draw(&nd, &g) * sigma;
+            vy[i] = orc_normal_draw(&nd, &g) * sigma;
+            vz[i] = orc_normal_draw(&nd, &g) * sigma;
+        }
+    }
+}
+
 struct orc_sim {
     int64_t n;
     double cell[9];
@@ -1017,6 +1183,11 @@ struct orc_sim {
     int baro;
     double P0, ptau, beta;
     int physics;
+    int64_t nc;
+    int32_t *ci, *cj;
+    double *cd, ctol;
+    double and_freq;
+    int64_t and_calls;
     /* state */
     int32_t *pi, *pj;
     int64_t np, pcap;
@@ -1077,6 +1248,11 @@ orc_sim *orc_sim_create(const dmdx_system_desc *d)
     DUPD(ikp, imp_k_phi, s->ni); DUPD(ip0, imp_phi0, s->ni);
     s->ne = d->n_excl;
     DUPI(ei, excl_i, s->ne); DUPI(ej, excl_j, s->ne);
+    s->nc = d->n_constraints > 0 ? d->n_constraints : 0;
+    DUPI(ci, con_i, s->nc); DUPI(cj, con_j, s->nc); DUPD(cd, con_distance, s->nc);
+    s->ctol = d->con_tolerance;
+    s->and_freq = d->andersen_frequency;
+    s->and_calls = d->andersen_calls;
 #undef DUPI
 #undef DUPD
     s->coulomb = d->coulomb_type;
@@ -1104,7 +1280,8 @@ void orc_sim_destroy(orc_sim *s)
     void *ptrs[] = {s->x, s->y, s->z, s->vx, s->vy, s->vz, s->fx, s->fy, s->fz, s->m, s->q,
                     s->types, s->sig, s->eps, s->bi, s->bj, s->bk, s->br0, s->ai, s->aj, s->ak,
                     s->akt, s->at0, s->di, s->dj, s->dk, s->dl, s->dper, s->dkp, s->dp0, s->ii,
-                    s->ij, s->ik, s->il, s->ikp, s->ip0, s->ei, s->ej, s->pi, s->pj};
+                    s->ij, s->ik, s->il, s->ikp, s->ip0, s->ei, s->ej, s->pi, s->pj,
+                    s->ci, s->cj, s->cd};
     for (size_t k = 0; k < sizeof(ptrs) / sizeof(ptrs[0]); ++k) free(ptrs[k]);
     free(s);
 }
@@ -1221,9 +1398,20 @@ void orc_sim_step(orc_sim *s, int64_t nsteps)
         orc_sim_compute(s);
         orc_half_kick(s->n, s->vx, s->vy, s->vz, s->fx, s->fy, s->fz, s->m, s->dt);
         orc_advance(s->n, s->x, s->y, s->z, s->vx, s->vy, s->vz, s->dt);
+        if (s->nc > 0)
+            orc_shake(s->nc, s->ci, s->cj, s->cd, s->ctol, s->cell, s->m, s->x, s->y, s->z);
         orc_sim_compute(s);
         orc_half_kick(s->n, s->vx, s->vy, s->vz, s->fx, s->fy, s->fz, s->m, s->dt);
-        if (s->thermo == DMDX_THERMO_BERENDSEN)
+        if (s->nc > 0)
+            orc_rattle(s->nc, s->ci, s->cj, s->ctol, s->cell, s->m, s->x, s->y, s->z, s->vx,
+                       s->vy, s->vz);
+        if (s->thermo == DMDX_THERMO_ANDERSEN) {
+            if (!(s->and_freq * s->dt <= 0.0)) {  /* andersen.cpp:16 */
+                orc_andersen(s->n, s->m, s->vx, s->vy, s->vz, s->dt, s->T0, s->and_freq,
+                             (uint32_t)(42u + (uint32_t)s->and_calls));
+                ++s->and_calls;
+            }
+        } else if (s->thermo == DMDX_THERMO_BERENDSEN)
             orc_berendsen(s->n, s->m, s->vx, s->vy, s->vz, s->dt, s->T0, s->ttau);
         else if (s->thermo == DMDX_THERMO_NOSE_HOOVER)
             orc_nose_hoover(s->n, s->m, s->vx, s->vy, s->vz, s->dt, s->T0, s->ttau, &s->zeta);

@@ -38,6 +38,14 @@
  *   sim_* (step sequence)       bit-exact against DMD's translation units
  *                               (_ref); closed forms; DMD's thermostat and
  *                               barostat unit-test values
+ *   shake, rattle               DMD's test_constraints.cpp cases; bit-exact
+ *                               against DMD's Constraints (_ref); converged
+ *                               bond lengths and r.v
+ *   mt64                        the C++ standard's known answer (10000th
+ *                               output of the default-seeded mt19937_64)
+ *   andersen                    bit-exact against DMD's AndersenThermostat
+ *                               (_ref, GCC 11 libstdc++ distributions);
+ *                               collision fraction and Maxwell variance
  */
 #ifndef DMD_ORACLE_H
 #define DMD_ORACLE_H
@@ -118,6 +126,23 @@ void orc_half_kick(int64_t n, double *vx, double *vy, double *vz,
                    const double *m, double dt);
 void orc_advance(int64_t n, double *x, double *y, double *z,
                  const double *vx, const double *vy, const double *vz, double dt);
+/* ---- constraints (src/integrate/constraints.cpp) */
+void orc_shake(int64_t nc, const int32_t *ci, const int32_t *cj, const double *dist, double tol,
+               const double cell[9], const double *m, double *x, double *y, double *z);
+void orc_rattle(int64_t nc, const int32_t *ci, const int32_t *cj, double tol,
+                const double cell[9], const double *m, const double *x, const double *y,
+                const double *z, double *vx, double *vy, double *vz);
+
+/* ---- Andersen thermostat (src/thermostat/andersen.cpp) and its stream */
+typedef struct orc_mt64 {
+    uint64_t mt[312];
+    int idx;
+} orc_mt64;
+void orc_mt64_seed(orc_mt64 *g, uint64_t seed);
+uint64_t orc_mt64_next(orc_mt64 *g);
+void orc_andersen(int64_t n, const double *m, double *vx, double *vy, double *vz, double dt,
+                  double T0, double frequency, uint32_t seed);
+
 double orc_kinetic_energy(int64_t n, const double *m, const double *vx, const double *vy,
                           const double *vz);
 double orc_temperature(double ekin, int64_t n);

@@ -21,7 +21,9 @@
 #include "force/periodic_dihedral.h"
 #include "force/harmonic_improper.h"
 #include "integrate/integrator.h"
+#include "integrate/constraints.h"
 #include "thermostat/thermostat.h"
+#include "thermostat/andersen.h"
 #include "thermostat/berendsen.h"
 #include "thermostat/nose_hoover.h"
 #include "barostat/barostat.h"
@@ -318,6 +320,51 @@ void ref_apply(int kind, int64_t n, const double* m, double* x, double* y, doubl
     box[2] = cell.matrix[8];
 }
 
+// SHAKE (kind 0, positions) or RATTLE (kind 1, velocities) of DMD's
+// Constraints on one state; box is the orthorhombic cell.
+void ref_constraints(int kind, int64_t nc, const int32_t* ci, const int32_t* cj,
+                     const double* dist, double tol, int64_t n, const double* m, double* x,
+                     double* y, double* z, double* vx, double* vy, double* vz,
+                     const double box[3])
+{
+    std::vector<double> zero(size_t(n), 0.0);
+    SystemData s = fill_state(n, m, x, y, z, vx, vy, vz, zero.data(), zero.data(), zero.data());
+    Cell cell(box[0], box[1], box[2]);
+    ConstraintsParams p;
+    p.i.assign(ci, ci + nc);
+    p.j.assign(cj, cj + nc);
+    p.distance_target.assign(dist, dist + nc);
+    p.tolerance = tol;
+    Constraints c(std::move(p));
+    if (kind == 0) c.apply(s, cell);
+    else c.apply_rattle(s, cell);
+    back(s, x, y, z, vx, vy, vz);
+}
+
+// DMD's AndersenThermostat: `calls` apply() calls on a scratch copy (each
+// advances the member seed), then one on the state.
+void ref_andersen(int64_t n, const double* m, double* vx, double* vy, double* vz, double dt,
+                  double T0, double frequency, int64_t calls)
+{
+    std::vector<double> zero(size_t(n), 0.0);
+    SystemData s = fill_state(n, m, zero.data(), zero.data(), zero.data(), vx, vy, vz,
+                              zero.data(), zero.data(), zero.data());
+    AndersenParams p;
+    p.temperature = T0;
+    p.frequency = frequency;
+    AndersenThermostat t(p);
+    for (int64_t k = 0; k < calls; ++k) {
+        SystemData scratch = s;
+        t.apply(scratch, dt);
+    }
+    t.apply(s, dt);
+    for (int64_t i = 0; i < n; ++i) {
+        vx[i] = s.vel_x[i];
+        vy[i] = s.vel_y[i];
+        vz[i] = s.vel_z[i];
+    }
+}
+
 // ---- step-level driver: ForceEngine + Integrator + thermostat/barostat in
 // the order of SimulationEngine::step.  Components are added in
 // build_force_engine order (LJ, bonds, angles, dihedrals, impropers,
@@ -331,6 +378,7 @@ struct ref_sim {
     SystemData sys;
     double dt = 0.0;
     LennardJones* lj = nullptr;
+    std::unique_ptr<Constraints> con;
 };
 
 ref_sim* ref_sim_create(int64_t n, const double* x, const double* y, const double* z,
@@ -444,6 +492,27 @@ ref_sim* ref_sim_create(int64_t n, const double* x, const double* y, const doubl
     return r;
 }
 
+// constraint_type "shake" (src/sim/simulation_engine.cpp:293-301)
+void ref_sim_set_constraints(ref_sim* r, int64_t nc, const int32_t* ci, const int32_t* cj,
+                             const double* dist, double tol)
+{
+    ConstraintsParams p;
+    p.i.assign(ci, ci + nc);
+    p.j.assign(cj, cj + nc);
+    p.distance_target.assign(dist, dist + nc);
+    p.tolerance = tol;
+    r->con = std::make_unique<Constraints>(std::move(p));
+}
+
+// thermostat type "andersen" (src/sim/simulation_engine.cpp:220-225)
+void ref_sim_set_andersen(ref_sim* r, double T0, double frequency)
+{
+    AndersenParams p;
+    p.temperature = T0;
+    p.frequency = frequency;
+    r->thermo = std::make_unique<AndersenThermostat>(p);
+}
+
 void ref_sim_step(ref_sim* r, int64_t nsteps)
 {
     auto& s = r->sys;
@@ -455,12 +524,14 @@ void ref_sim_step(ref_sim* r, int64_t nsteps)
         r->fe.compute(s, r->cell);
         r->integ.half_kick(s, r->dt);
         r->integ.advance(s, r->dt);
+        if (r->con) r->con->apply(s, r->cell);
         s.forces_x.assign(s.n_atoms, 0.0);
         s.forces_y.assign(s.n_atoms, 0.0);
         s.forces_z.assign(s.n_atoms, 0.0);
         s.potential_energy = 0.0;
         r->fe.compute(s, r->cell);
         r->integ.half_kick(s, r->dt);
+        if (r->con) r->con->apply_rattle(s, r->cell);
         if (r->thermo) r->thermo->apply(s, r->dt);
         if (r->baro) r->baro->apply(s, r->cell, r->dt);
     }

@@ -39,14 +39,22 @@ def test_config_translation():
     c.coulomb_type = "none"
     assert spec_from_config(c).coulomb_type == _abi.COULOMB_NONE
     c.thermostat.type = "andersen"
-    with pytest.raises(NotImplementedError):
+    c.thermostat.frequency = 5.0
+    sp = spec_from_config(c)
+    assert sp.thermostat == _abi.THERMO_ANDERSEN and sp.andersen_frequency == 5.0
+    c.thermostat.type = "bogus"
+    with pytest.raises(ValueError):
         spec_from_config(c)
     c.thermostat.type = "nose-hoover"
     assert spec_from_config(c).thermostat == _abi.THERMO_NOSE_HOOVER
     c.constraint_type = "shake"
     c.constraint_i, c.constraint_j, c.constraint_dist = [0], [1], [0.1]
-    with pytest.raises(NotImplementedError):
-        spec_from_config(c)
+    c.constraint_tolerance = 1e-7
+    sp = spec_from_config(c)
+    assert [list(a) for a in sp.constraints] == [[0], [1], [0.1]]
+    assert sp.constraint_tolerance == 1e-7
+    c.constraint_type = "none"
+    assert spec_from_config(c).constraints is None
 
 
 def test_gen_vel():
