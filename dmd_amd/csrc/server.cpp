@@ -71,6 +71,7 @@ int64_t estimate_bytes(const dmdx_system_desc& D)
     const int64_t roles = 2 * D.n_bonds + 3 * D.n_angles + 4 * D.n_dihedrals + 4 * D.n_impropers +
                           2 * D.n_excl;
     b += roles * 40 + (D.n_bonds + D.n_angles + D.n_dihedrals + D.n_impropers + D.n_excl) * 48;
+    if (D.n_constraints > 0) b += D.n_constraints * 24;  // slots, distance, component index
     return b + 4096;
 }
 

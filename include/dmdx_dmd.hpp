@@ -57,6 +57,16 @@ struct ComponentHandle {
     ~ComponentHandle() { dmdx_component_destroy(h); }
 };
 
+// Cell::minimum_image is the identity for the non-orthorhombic types
+// (src/core/cell.cpp:35-37): no periodic images at all.  The device path
+// implements the orthorhombic image only and refuses the others.
+inline void require_orthorhombic(const Cell& cell, const std::string& name)
+{
+    if (cell.type != CellType::orthorhombic)
+        throw ForceException(ForceError::ParmError, name.c_str(),
+                             "only orthorhombic cells are supported on the device");
+}
+
 inline void compute_on(dmdx_component* h, const std::string& name, std::span<const double> x,
                        std::span<const double> y, std::span<const double> z, const Cell& cell,
                        std::span<double> fx, std::span<double> fy, std::span<double> fz,
@@ -65,6 +75,7 @@ inline void compute_on(dmdx_component* h, const std::string& name, std::span<con
     if (y.size() != x.size() || z.size() != x.size() || fx.size() != x.size() ||
         fy.size() != x.size() || fz.size() != x.size())
         throw ForceException(ForceError::Mismatch, name.c_str(), "span sizes differ");
+    require_orthorhombic(cell, name);
     check(dmdx_component_compute(h, x.data(), y.data(), z.data(), (int64_t)x.size(), cell.matrix,
                                  fx.data(), fy.data(), fz.data(), &energy_out),
           name);
@@ -107,6 +118,7 @@ public:
     void rebuild_pair_list_gpu(std::span<const double> x, std::span<const double> y,
                                std::span<const double> z, const Cell& cell)
     {
+        require_orthorhombic(cell, "LennardJones");
         check(dmdx_component_rebuild_pair_list(h_.h, x.data(), y.data(), z.data(),
                                                (int64_t)x.size(), cell.matrix),
               "LennardJones");
@@ -146,6 +158,7 @@ public:
     void rebuild_pair_list(std::span<const double> x, std::span<const double> y,
                            std::span<const double> z, const Cell& cell)
     {
+        require_orthorhombic(cell, name_);
         check(dmdx_component_rebuild_pair_list(h_.h, x.data(), y.data(), z.data(),
                                                (int64_t)x.size(), cell.matrix),
               name_);

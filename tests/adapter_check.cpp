@@ -232,6 +232,15 @@ void interface_checks(bool have_gpu)
         mismatch = ex.err == ForceError::Mismatch && ex.component == "LennardJones";
     }
     expect(mismatch, "span size mismatch raises ForceException(Mismatch)");
+    bool refused = false;
+    try {
+        const double tri[9] = {3.0, 0.0, 0.0, 0.5, 3.0, 0.0, 0.0, 0.0, 3.0};
+        std::vector<double> fx(4, 0.0), fy(4, 0.0), fz(4, 0.0);
+        c->compute(x, y, z, Cell(tri, CellType::triclinic), fx, fy, fz, e);
+    } catch (const ForceException& ex) {
+        refused = ex.err == ForceError::ParmError;
+    }
+    expect(refused, "a triclinic Cell raises ForceException(ParmError)");
     if (!have_gpu) {
         bool gpu_err = false;
         std::vector<double> fx(4, 0.0), fy(4, 0.0), fz(4, 0.0);
