@@ -137,9 +137,10 @@ enum { FLAG_NBR_OVERFLOW = 1, FLAG_NONFINITE = 2 };
 
 // Mixed-path list entries: bits 0-22 the neighbour's cell-order slot
 // relative to its system's first slot, bits 23-31 the neighbour cell's
-// offset from the atom's cell along x, y, z (3 bits each, stored +2, in
-// 0..4; the mixed path's cells are at least rlist / cdiv wide with cdiv 1 or
-// 2, so neighbours lie within +-cdiv cells).  With cell-local coordinates l
+// offset from the atom's cell along x, y, z (3-bit two's complement fields,
+// -2..2, read back with one signed bit-field extract; the mixed path's cells
+// are at least rlist / cdiv wide with cdiv 1 or 2, so neighbours lie within
+// +-cdiv cells).  With cell-local coordinates l
 // and cell width w the displacement is d = l_j - l_i + offset * w (the
 // minimum image when the system has >= 2 cdiv + 1 cells per edge).
 constexpr int NBR_SLOT_BITS = 23;
@@ -269,6 +270,7 @@ struct Dev {
     // what exists in the batch
     bool any_list = false, any_pair = false, any_pme = false, any_excl = false;
     bool any_coul_pairs = false;  // some system computes real-space erfc pairs
+    bool any_plain_pairs = false;  // some system does not (LJ-only pair variant)
     bool any_gather = false, any_baro = false;
 };
 

@@ -663,6 +663,7 @@ int Batch::create(const dmdx_system_desc* descs, int nsys, const BatchOptions& o
         d.any_list |= S.has_list != 0;
         d.any_pme |= S.pme != 0;
         d.any_coul_pairs |= S.coul_pairs != 0;
+        d.any_plain_pairs |= S.coul_pairs == 0;
         d.any_excl |= S.n_ex > 0;
         d.any_baro |= S.barostat != DMDX_BARO_NONE;
         d.any_andersen |= S.thermostat == DMDX_THERMO_ANDERSEN;

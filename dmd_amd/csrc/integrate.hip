@@ -302,13 +302,31 @@ __global__ void k_kick_ke(const SysDev* __restrict__ sys, const SysState* __rest
     DMDX_FOR_UNITS(u_, nunits_) k_kick_ke_unit(u_, sys, st, slot_sys, nslots, invm, m, fx, fy, fz, vx, vy, vz, ke_part);
 }
 
+__device__ __forceinline__ double strided_sum_from(const double* __restrict__ part, int64_t off,
+                                                   int64_t n, double a, int64_t k);
+
 // strided partial sum of part[off .. off+n) for a 256-thread workgroup:
 // thread t sums t, t+256, ... in order
 __device__ __forceinline__ double strided_sum(const double* __restrict__ part, int64_t off,
                                               int64_t n)
 {
-    double a = 0.0;
-    for (int64_t k = threadIdx.x; k < n; k += 256) a += part[off + k];
+    // eight loads in flight before the in-order adds (the same sum, bit for
+    // bit, as the plain loop)
+    return strided_sum_from(part, off, n, 0.0, threadIdx.x);
+}
+
+// the same from element k on, starting from a
+__device__ __forceinline__ double strided_sum_from(const double* __restrict__ part, int64_t off,
+                                                   int64_t n, double a, int64_t k)
+{
+    for (; k + 7 * 256 < n; k += 8 * 256) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = part[off + k + u * 256];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) a += v[u];
+    }
+    for (; k < n; k += 256) a += part[off + k];
     return a;
 }
 
@@ -350,16 +368,20 @@ __global__ void __launch_bounds__(256) k_finalize(const SysDev* __restrict__ sys
     // the nine sums (8 energy terms and the kinetic energy) in one pass: each
     // thread's strided sums, then a fixed tree over waves and over the 4
     // waves -- deterministic, one barrier
+    // pair energies are per atom slot, the others per wave or workgroup
+    const double* arr[9] = {P.lj, P.coul, P.bond, P.angle, P.dih, P.imp, P.pme, P.excl, P.ke};
+    const int64_t off[9] = {S.atom_off, S.atom_off, S.bw_off, S.gw_off, S.dw_off,
+                            S.iw_off,   S.pw_off,   S.xw_off, S.aw_off};
+    const int64_t cnt[9] = {S.n_pad, S.n_pad, S.bw_n, S.gw_n, S.dw_n,
+                            S.iw_n,  S.pw_n,  S.xw_n, with_kinetic ? S.aw_n : 0};
+    // the first element of each of the nine sums is loaded before any is
+    // added (nine loads in flight), then the strided remainders
     double a[9];
-    a[0] = strided_sum(P.lj, S.atom_off, S.n_pad);  // pair energies are per atom slot
-    a[1] = strided_sum(P.coul, S.atom_off, S.n_pad);
-    a[2] = strided_sum(P.bond, S.bw_off, S.bw_n);
-    a[3] = strided_sum(P.angle, S.gw_off, S.gw_n);
-    a[4] = strided_sum(P.dih, S.dw_off, S.dw_n);
-    a[5] = strided_sum(P.imp, S.iw_off, S.iw_n);
-    a[6] = strided_sum(P.pme, S.pw_off, S.pw_n);
-    a[7] = strided_sum(P.excl, S.xw_off, S.xw_n);
-    a[8] = with_kinetic ? strided_sum(P.ke, S.aw_off, S.aw_n) : 0.0;
+    const int64_t t0 = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) a[k] = (t0 < cnt[k]) ? arr[k][off[k] + t0] : 0.0;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) a[k] = strided_sum_from(arr[k], off[k], cnt[k], 0.0 + a[k], t0 + 256);
 #pragma unroll
     for (int k = 0; k < 9; ++k) a[k] = wave_sum(a[k]);
     if (lane_id() == 0)

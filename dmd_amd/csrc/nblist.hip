@@ -21,6 +21,8 @@
 //   k_pack         mixed path, every compute: cell-local FP32 positions
 #include "dmdx_internal.h"
 
+#include <type_traits>
+
 namespace dmdx {
 
 // cells in use along one edge: as many as fit with edge >= rlist / cdiv, at
@@ -579,8 +581,10 @@ __device__ __forceinline__ void k_nbr_build_mixed_unit(int64_t t,
     const float4* __restrict__ pos4f)
 {
     const int lane = threadIdx.x % TPA;
-    if (t < 0 || t >= nslots) return;
-    const int s = slot_sys[t];
+    if (t < 0 || t >= nslots) return;  // whole waves (compact ranges are multiples of 64)
+    // the wave's atoms are consecutive slots of one system (systems own
+    // whole 64-slot blocks): system constants are scalar
+    const int s = __builtin_amdgcn_readfirstlane(slot_sys[t]);
     if (s < 0 || !st[s].rebuild) return;
     const SysDev& S = sys[s];
     const int lt = (int)(t - S.atom_off);
@@ -622,6 +626,10 @@ __device__ __forceinline__ void k_nbr_build_mixed_unit(int64_t t,
     const int lox = lo_of(nx), hix = hi_of(nx);
     const int loy = lo_of(ny), hiy = hi_of(ny);
     const int loz = lo_of(nz), hiz = hi_of(nz);
+    // the wave-uniform minimum-image switch selects a loop variant outside
+    // the candidate loop
+    auto scan = [&](auto wrap_c) -> int {
+    constexpr bool WRAP = decltype(wrap_c)::value;
     int k = 0;
     for (int oz = loz; oz <= hiz; ++oz) {
         int zz = cz + oz;
@@ -654,8 +662,8 @@ __device__ __forceinline__ void k_nbr_build_mixed_unit(int64_t t,
                 }
                 const int cnt = acc;
                 const int ox0 = ox;
-                const uint32_t syz = ((uint32_t)(oy + 2) << (NBR_SLOT_BITS + 3)) |
-                                     ((uint32_t)(oz + 2) << (NBR_SLOT_BITS + 6));
+                const uint32_t syz = (((uint32_t)oy & 7u) << (NBR_SLOT_BITS + 3)) |
+                                     (((uint32_t)oz & 7u) << (NBR_SLOT_BITS + 6));
                 // the next pass's position is loaded before this pass is
                 // processed, so the gather latency overlaps the arithmetic
                 float4 nxt = lane < cnt ? pos4f[base + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -668,7 +676,7 @@ __device__ __forceinline__ void k_nbr_build_mixed_unit(int64_t t,
                     bool hit = false;
                     if (m < cnt && base + m != t) {
                         float dx = lj.x + (oxm * wx - li.x), dy = lj.y + sy, dz = lj.z + sz;
-                        if (wrap) {
+                        if constexpr (WRAP) {
                             dx -= rintf(dx * (float)S.invL[0]) * (float)S.L[0];
                             dy -= rintf(dy * (float)S.invL[1]) * (float)S.L[1];
                             dz -= rintf(dz * (float)S.invL[2]) * (float)S.L[2];
@@ -702,13 +710,16 @@ __device__ __forceinline__ void k_nbr_build_mixed_unit(int64_t t,
                     const int pos = k + __popcll(mine & below);
                     if (hit && pos < M)
                         row[pos] = (int)((uint32_t)(base + m - S.atom_off) | syz |
-                                         ((uint32_t)(oxm + 2) << NBR_SLOT_BITS));
+                                         (((uint32_t)oxm & 7u) << NBR_SLOT_BITS));
                     k += __popcll(mine);
                 }
                 ox += len;
             }
         }
     }
+    return k;
+    };
+    int k = wrap ? scan(std::true_type{}) : scan(std::false_type{});
     if (k > M) {
         if (lane == 0) atomicOr(&st[s].flags, FLAG_NBR_OVERFLOW);
         k = M;

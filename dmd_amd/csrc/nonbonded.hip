@@ -21,6 +21,8 @@
 //   exp -- runs in FP32 and each pair force is added to an FP64 accumulator.
 #include "dmdx_internal.h"
 
+#include <type_traits>
+
 namespace dmdx {
 
 // accumulate: 1 = start from the forces already in fx/fy/fz (ForceComponent
@@ -122,7 +124,7 @@ __global__ void k_pair_fp64(const SysDev* __restrict__ sys, const SysState* __re
 }
 
 // Mixed precision.  A group of TPA lanes handles one atom (cell-order slot
-// t): lane l takes neighbours Wl..Wl+W-1, Wl+W TPA.., of the atom's row, reads
+// t): lane l takes neighbours l, l + TPA, l + 2 TPA, ... of the atom's row, reads
 // their cell-local FP32 positions (spatially local gathers of 16 bytes) and
 // rebuilds each displacement as l_j - l_i + offset * w from the offset
 // stored in the list entry -- no FP64 and no minimum-image rounding per pair
@@ -146,12 +148,20 @@ __device__ __forceinline__ void k_pair_mixed_unit(int64_t unit_,
     constexpr int GPB = 256 / TPA;
     const int lane = threadIdx.x % TPA;
     const int64_t t = unit_ * GPB + threadIdx.x / TPA;
-    if (t >= nslots) return;
-    const int s = slot_sys[t];
+    if (t >= nslots) return;  // whole waves: nslots is a multiple of 64
+    // A wave's atoms are 64 / TPA consecutive slots inside one 64-slot block,
+    // and a system owns whole blocks: the system is wave-uniform, so its
+    // constants live in scalar registers and the branches on them are
+    // scalar branches.
+    const int s = __builtin_amdgcn_readfirstlane(slot_sys[t]);
     if (s < 0) return;
     const SysState& T = st[s];
     if (T.reuse || !T.active) return;
     const SysDev& S = sys[s];
+    // Each system is computed by the one variant its own terms select
+    // (COUL: real-space Coulomb pairs), whatever else shares the batch, so
+    // its forces are bit-identical alone and in any mix.
+    if ((S.coul_pairs != 0) != COUL) return;
     const int lt = (int)(t - S.atom_off);
     if (lt >= S.n) {
         if (lane == 0) {
@@ -172,7 +182,7 @@ __device__ __forceinline__ void k_pair_mixed_unit(int64_t unit_,
     const float Lx = (float)S.L[0], Ly = (float)S.L[1], Lz = (float)S.L[2];
     const float ix = (float)S.invL[0], iy = (float)S.invL[1], iz = (float)S.invL[2];
     const int nt = S.ntypes;
-    const bool lj = S.use_lj, cp = COUL && S.coul_pairs;
+    const bool lj = S.use_lj, cp = COUL;
     const bool typed = lj && nt > 1;
     const float* c12t = lj_c12 + S.lj_off;
     const float* c6t = lj_c6 + S.lj_off;
@@ -184,88 +194,139 @@ __device__ __forceinline__ void k_pair_mixed_unit(int64_t unit_,
     const float a2 = (float)(S.alpha * S.alpha);
     const float cut2 = (float)S.rc_lj2, ccut2 = (float)S.rc_c2;
     const int* row = nbr + S.nbr_off + (int64_t)lt * S.max_nbr;
+    const float4* __restrict__ lps = lpos + a0;  // this system's slots
+    const int* __restrict__ tys = typ_s + a0;
     const int cnt = nn[t];
     double fxi = 0.0, fyi = 0.0, fzi = 0.0, ej = 0.0, ec = 0.0;
-    // rows are 16-byte aligned with M a multiple of 4: each lane takes W (2
-    // or 4) consecutive entries per pass and issues their W gathers before
-    // any arithmetic (memory-level parallelism for the dependent loads; W=2
-    // needs fewer registers, so more waves fit)
-    for (int k0 = lane * W; k0 < cnt; k0 += TPA * W) {
+    // Each lane takes W (2 or 4) entries per pass and issues their W gathers
+    // before any arithmetic (memory-level parallelism for the dependent
+    // loads; W=2 needs fewer registers, so more waves fit).  The wave-uniform
+    // switches (minimum-image wrap, typed LJ tables) select a loop variant
+    // outside the loop, so its body carries no branch on them.
+    auto passes = [&](auto wrap_c, auto typed_c) {
+        constexpr bool WRAP = decltype(wrap_c)::value;
+        constexpr bool TYPED = decltype(typed_c)::value;
+        constexpr int STRIDE = TPA * W;
+        // Pass p covers entries [p STRIDE, (p + 1) STRIDE) of the row.
+        // Coulomb variant: lane l takes entries l, l + TPA, ..., so each
+        // gather instruction reads TPA consecutive entries of a row --
+        // consecutive cell-order slots, a cache line or two -- and the
+        // entries of pass p + 1 are loaded while pass p is gathered and
+        // computed; entry reads are clamped into the row (index < M) and
+        // entries at or past cnt never address memory.  LJ variant (less
+        // arithmetic per entry, so occupancy matters more): lane l takes the
+        // W consecutive entries W l .. W l + W - 1 with one vector load
+        // (rows are 16-byte aligned, M a multiple of 4) and no prefetch.
+        constexpr bool STRIDED = COUL;
+        auto entry_index = [&](int k0, int q) {
+            return STRIDED ? k0 + lane + q * TPA : k0 + lane * W + q;
+        };
+        const int mlast = S.max_nbr - 1;
+        auto load_entries = [&](int k0, uint32_t(&ev)[W]) {
+            if constexpr (STRIDED) {
+#pragma unroll
+                for (int q = 0; q < W; ++q) {
+                    const int k = k0 + lane + q * TPA;
+                    ev[q] = (uint32_t)row[k < mlast ? k : mlast];
+                }
+            } else if constexpr (W == 4) {
+                const int4 jj = *reinterpret_cast<const int4*>(row + k0 + lane * W);
+                ev[0] = (uint32_t)jj.x;
+                ev[1] = (uint32_t)jj.y;
+                ev[2] = (uint32_t)jj.z;
+                ev[3] = (uint32_t)jj.w;
+            } else {
+                const int2 jj = *reinterpret_cast<const int2*>(row + k0 + lane * W);
+                ev[0] = (uint32_t)jj.x;
+                ev[1] = (uint32_t)jj.y;
+            }
+        };
+        // blocked lanes start at lane W, so their loop runs while that is
+        // below cnt (their vector load stays inside the row)
+        const int kbase = STRIDED ? 0 : lane * W;
         uint32_t ev[W];
-        if constexpr (W == 4) {
-            const int4 jj = *reinterpret_cast<const int4*>(row + k0);
-            ev[0] = (uint32_t)jj.x;
-            ev[1] = (uint32_t)jj.y;
-            ev[2] = (uint32_t)jj.z;
-            ev[3] = (uint32_t)jj.w;
-        } else {
-            const int2 jj = *reinterpret_cast<const int2*>(row + k0);
-            ev[0] = (uint32_t)jj.x;
-            ev[1] = (uint32_t)jj.y;
-        }
-        float4 pj[W];
-        int tj[W];
+        if constexpr (STRIDED) load_entries(0, ev);
+        for (int kp = kbase; kp < cnt; kp += STRIDE) {
+            const int k0 = kp - kbase;
+            if constexpr (!STRIDED) load_entries(k0, ev);
+            float4 pj[W];
+            int tj[W];
 #pragma unroll
-        for (int q = 0; q < W; ++q) {
-            const int64_t j = (k0 + q < cnt) ? a0 + (int64_t)(ev[q] & NBR_SLOT_MASK) : t;
-            pj[q] = lpos[j];
-            tj[q] = typed ? typ_s[j] : 0;
-        }
-        // the W entries of a pass are summed in FP32, then added in FP64
-        float px = 0.0f, py = 0.0f, pz = 0.0f, pel = 0.0f, pec = 0.0f;
+            for (int q = 0; q < W; ++q) {
+                const uint32_t j =
+                    (entry_index(k0, q) < cnt) ? (ev[q] & NBR_SLOT_MASK) : (uint32_t)lt;
+                pj[q] = lps[j];
+                tj[q] = TYPED ? tys[j] : 0;
+            }
+            uint32_t en[W];
+            if constexpr (STRIDED) load_entries(k0 + STRIDE, en);
+            // the W entries of a pass are summed in FP32, then added in FP64
+            float px = 0.0f, py = 0.0f, pz = 0.0f, pel = 0.0f, pec = 0.0f;
 #pragma unroll
-        for (int q = 0; q < W; ++q) {
-            if (k0 + q >= cnt) continue;
-            const uint32_t e = ev[q];
-            const float ox = (float)((int)((e >> NBR_SLOT_BITS) & 7u) - 2);
-            const float oy = (float)((int)((e >> (NBR_SLOT_BITS + 3)) & 7u) - 2);
-            const float oz = (float)((int)((e >> (NBR_SLOT_BITS + 6)) & 7u) - 2);
-            float dx = (pj[q].x - pi.x) + ox * wx;
-            float dy = (pj[q].y - pi.y) + oy * wy;
-            float dz = (pj[q].z - pi.z) + oz * wz;
-            if (wrap) {
-                dx -= rintf(dx * ix) * Lx;
-                dy -= rintf(dy * iy) * Ly;
-                dz -= rintf(dz * iz) * Lz;
-            }
-            const float r2 = dx * dx + dy * dy + dz * dz;
-            const float rinv2 = __builtin_amdgcn_rcpf(r2);  // 1 ulp hardware reciprocal
-            float fs = 0.0f, el = 0.0f, ecl = 0.0f;
-            if (lj && r2 < cut2) {
-                const float c12 = typed ? c12t[ti * nt + tj[q]] : c12_0;
-                const float c6 = typed ? c6t[ti * nt + tj[q]] : c6_0;
-                const float r6i = rinv2 * rinv2 * rinv2;
-                const float a12 = c12 * r6i * r6i, a6 = c6 * r6i;
-                fs += (12.0f * a12 - 6.0f * a6) * rinv2;
-                el = a12 - a6;
-            }
-            if (COUL && cp && r2 < ccut2) {
-                const float rinv = rsqrtf(r2);
-                const float r = r2 * rinv;
-                const float qq = qi * pj[q].w;
-                const float ex = __expf(-a2 * r2);
-                // erfc(a r) by Abramowitz & Stegun 7.1.26 (|error| <= 1.5e-7),
-                // sharing exp(-(a r)^2) with the force term
-                const float tt = __builtin_amdgcn_rcpf(1.0f + 0.3275911f * (alpha * r));
-                const float erfc_ar =
-                    tt * (0.254829592f + tt * (-0.284496736f + tt * (1.421413741f +
-                          tt * (-1.453152027f + tt * 1.061405429f)))) * ex;
-                fs += qq * (erfc_ar * rinv + two_a_rpi * ex) * rinv2;
-                ecl = qq * erfc_ar * rinv;
-            }
-            px += fs * dx;
-            py += fs * dy;
-            pz += fs * dz;
-            if (a0 + (int64_t)(e & NBR_SLOT_MASK) > t) {  // each unordered pair once
-                pel += el;
+            for (int q = 0; q < W; ++q) {
+                if (entry_index(k0, q) >= cnt) continue;
+                const uint32_t e = ev[q];
+                const float ox = (float)((int)(e << (32 - NBR_SLOT_BITS - 3)) >> 29);
+                const float oy = (float)((int)(e << (32 - NBR_SLOT_BITS - 6)) >> 29);
+                const float oz = (float)((int)e >> 29);
+                float dx = (pj[q].x - pi.x) + ox * wx;
+                float dy = (pj[q].y - pi.y) + oy * wy;
+                float dz = (pj[q].z - pi.z) + oz * wz;
+                if constexpr (WRAP) {
+                    dx -= rintf(dx * ix) * Lx;
+                    dy -= rintf(dy * iy) * Ly;
+                    dz -= rintf(dz * iz) * Lz;
+                }
+                const float r2 = dx * dx + dy * dy + dz * dz;
+                const float rinv2 = __builtin_amdgcn_rcpf(r2);  // 1 ulp hardware reciprocal
+                float fs = 0.0f, el = 0.0f, ecl = 0.0f;
+                if (lj && r2 < cut2) {
+                    const float c12 = TYPED ? c12t[ti * nt + tj[q]] : c12_0;
+                    const float c6 = TYPED ? c6t[ti * nt + tj[q]] : c6_0;
+                    const float r6i = rinv2 * rinv2 * rinv2;
+                    const float a12 = c12 * r6i * r6i, a6 = c6 * r6i;
+                    fs = (12.0f * a12 - 6.0f * a6) * rinv2;
+                    el = a12 - a6;
+                }
+                if (COUL && cp && r2 < ccut2) {
+                    const float rinv = rsqrtf(r2);
+                    const float r = r2 * rinv;
+                    const float qq = qi * pj[q].w;
+                    const float ex = __expf(-a2 * r2);
+                    // erfc(a r) by Abramowitz & Stegun 7.1.26 (|error| <= 1.5e-7),
+                    // sharing exp(-(a r)^2) with the force term
+                    const float tt = __builtin_amdgcn_rcpf(1.0f + 0.3275911f * (alpha * r));
+                    const float erfc_ar =
+                        tt * (0.254829592f + tt * (-0.284496736f + tt * (1.421413741f +
+                              tt * (-1.453152027f + tt * 1.061405429f)))) * ex;
+                    fs += qq * (erfc_ar * rinv + two_a_rpi * ex) * rinv2;
+                    ecl = qq * erfc_ar * rinv;
+                }
+                px += fs * dx;
+                py += fs * dy;
+                pz += fs * dz;
+                pel += el;  // rows hold both orders of a pair: halved below
                 pec += ecl;
             }
+            fxi -= (double)px;
+            fyi -= (double)py;
+            fzi -= (double)pz;
+            ej += (double)pel;
+            if (COUL) ec += (double)pec;
+            if constexpr (STRIDED) {
+#pragma unroll
+                for (int q = 0; q < W; ++q) ev[q] = en[q];
+            }
         }
-        fxi -= (double)px;
-        fyi -= (double)py;
-        fzi -= (double)pz;
-        ej += (double)pel;
-        ec += (double)pec;
+    };
+    using T_ = std::true_type;
+    using F_ = std::false_type;
+    if (wrap) {
+        if (typed) passes(T_{}, T_{});
+        else passes(T_{}, F_{});
+    } else {
+        if (typed) passes(F_{}, T_{});
+        else passes(F_{}, F_{});
     }
     fxi = group_sum<TPA>(fxi);
     fyi = group_sum<TPA>(fyi);
@@ -282,8 +343,8 @@ __device__ __forceinline__ void k_pair_mixed_unit(int64_t unit_,
         fx[i] = fxi;
         fy[i] = fyi;
         fz[i] = fzi;
-        e_lj_part[t] = ej;
-        e_c_part[t] = ec;
+        e_lj_part[t] = 0.5 * ej;
+        e_c_part[t] = 0.5 * ec;
     }
 }
 
@@ -334,10 +395,15 @@ hipError_t launch_pair(const Dev& d, hipStream_t s, int accumulate)
         k_pair_fp64<<<grid_for(blocks_for(d.nslots, BS)), BS, 0, s>>>(
             d.sys, d.st, d.slot_sys, d.nslots, d.order, d.nn, d.nbr, d.x, d.y, d.z, d.q, d.type,
             d.lj_sig, d.lj_eps, d.fx, d.fy, d.fz, d.p_lj, d.p_coul, accumulate, (int64_t)(blocks_for(d.nslots, BS)));
-    else if (d.any_coul_pairs)
-        return launch_pair_mixed<true>(d, s, accumulate);
-    else
-        return launch_pair_mixed<false>(d, s, accumulate);
+    else {
+        // one launch per variant present in the batch; each system is taken
+        // by exactly one of them
+        if (d.any_coul_pairs) {
+            const hipError_t e = launch_pair_mixed<true>(d, s, accumulate);
+            if (e != hipSuccess) return e;
+        }
+        if (d.any_plain_pairs) return launch_pair_mixed<false>(d, s, accumulate);
+    }
     return hipGetLastError();
 }
 

@@ -4,6 +4,10 @@
         kernel table (calls, total, average duration) from a
         `rocprofv3 --kernel-trace --stats` run, as markdown
 
+    python -m dmd_amd.profiling counters DB [--kernel NAME ...]
+        mean value per dispatch of every counter of a `rocprofv3 --pmc` run,
+        per kernel, as markdown
+
     python -m dmd_amd.profiling traffic FETCH_DB WRITE_DB --copy-bytes B
         HBM bytes per dispatch per kernel from two `rocprofv3 --pmc` runs
         (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch), each scaled by what
@@ -43,6 +47,17 @@ def counter_means(db: str, counter: str):
     return {short(r[0]): (float(r[1]), int(r[2])) for r in rows}
 
 
+def all_counter_means(db: str):
+    con = sqlite3.connect(db)
+    rows = con.execute(
+        "select kernel_name, counter_name, avg(value) from counters_collection "
+        "group by kernel_name, counter_name").fetchall()
+    out = {}
+    for k, c, v in rows:
+        out.setdefault(short(k), {})[c] = float(v)
+    return out
+
+
 def traffic(fetch_db: str, write_db: str, copy_bytes: int):
     f = counter_means(fetch_db, "FETCH_SIZE")
     w = counter_means(write_db, "WRITE_SIZE")
@@ -68,6 +83,9 @@ def main(argv=None):
     a = sub.add_parser("stats")
     a.add_argument("db")
     a.add_argument("--top", type=int, default=25)
+    c = sub.add_parser("counters")
+    c.add_argument("db")
+    c.add_argument("--kernel", action="append", default=None)
     b = sub.add_parser("traffic")
     b.add_argument("fetch_db")
     b.add_argument("write_db")
@@ -80,6 +98,15 @@ def main(argv=None):
         print("|---|---:|---:|---:|---:|")
         for name, calls, tot, avg, pct in rows[: args.top]:
             print(f"| {name} | {calls} | {tot / 1e3:.3f} | {avg:.1f} | {pct:.2f} |")
+    elif args.cmd == "counters":
+        m = all_counter_means(args.db)
+        names = sorted({c for v in m.values() for c in v})
+        print("| kernel | " + " | ".join(names) + " |")
+        print("|---|" + "---:|" * len(names))
+        for k in sorted(m):
+            if args.kernel and k not in args.kernel:
+                continue
+            print(f"| {k} | " + " | ".join(f"{m[k].get(c, float('nan')):.4g}" for c in names) + " |")
     else:
         print(json.dumps(traffic(args.fetch_db, args.write_db, args.copy_bytes), indent=1))
 
