@@ -80,6 +80,7 @@ struct SysDev {
     int32_t pme;
     int32_t ginf_static;  // mixed path: influence function tabulated (no barostat)
     int32_t cdiv;         // cells per list radius along an edge (FP64 1; mixed 1 or 2)
+    int32_t bin_wg;       // binned by one workgroup (k_bin_system)
     int32_t K[3];
     int64_t grid_off;    // complex elements
     int64_t tw_off[3];   // twiddle table offsets (complex elements)
@@ -143,6 +144,11 @@ enum { FLAG_NBR_OVERFLOW = 1, FLAG_NONFINITE = 2 };
 // +-cdiv cells).  With cell-local coordinates l
 // and cell width w the displacement is d = l_j - l_i + offset * w (the
 // minimum image when the system has >= 2 cdiv + 1 cells per edge).
+// k_bin_system: systems of at most BIN_WG_ATOMS atoms whose allocated cells
+// fit in LDS are binned by one workgroup each
+constexpr int BIN_WG_ATOMS = 8192;
+constexpr int BIN_WG_CELLS = 4096;
+
 constexpr int NBR_SLOT_BITS = 23;
 constexpr uint32_t NBR_SLOT_MASK = (1u << NBR_SLOT_BITS) - 1u;
 
@@ -249,11 +255,20 @@ struct Dev {
     int *k_i = nullptr, *k_j = nullptr;  // global slots
     double* k_d = nullptr;     // target distances
     bool any_andersen = false;
-    // rebuild worklist (k_rb_list): systems rebuilding their lists this step
+    // rebuild worklists (k_rb_list): systems rebuilding their lists this
+    // step -- all of them (rb_*), those binned by the multi-kernel path
+    // (rbm_*) and those binned one workgroup per system (rbw_*)
     int* rb_sys = nullptr;       // nsys
     int64_t* rb_pre = nullptr;   // nsys + 1: running slot totals
-    int64_t* rb_cpre = nullptr;  // nsys + 1: running cell totals
     int* rb_n = nullptr;         // 1
+    int* rbm_sys = nullptr;
+    int64_t* rbm_pre = nullptr;
+    int64_t* rbm_cpre = nullptr;  // running cell totals
+    int* rbm_n = nullptr;
+    int* rbw_sys = nullptr;
+    int* rbw_n = nullptr;
+    int n_bin_wg = 0;            // systems binned by k_bin_system
+    bool any_bin_multi = false;  // systems binned by the multi-kernel path
     SpreadTile* stiles = nullptr;  // charge-spreading tiles (atomic route)
     int nstiles = 0;
     GridTile* gtiles = nullptr;    // owner-computes spreading blocks

@@ -215,6 +215,7 @@ int Batch::create(const dmdx_system_desc* descs, int nsys, const BatchOptions& o
             return fail(DMDX_E_PARM, "a mixed-precision system holds at most 2^23 atoms");
         S.cell_off = ncells;
         int nc3 = S.nc[0] * S.nc[1] * S.nc[2];
+        S.bin_wg = (S.has_list && n <= BIN_WG_ATOMS && nc3 <= BIN_WG_CELLS) ? 1 : 0;
         ncells += nc3;
         for (int c = 0; c < nc3; ++c) cell_sys.push_back(s);
 
@@ -664,6 +665,10 @@ int Batch::create(const dmdx_system_desc* descs, int nsys, const BatchOptions& o
         d.any_pme |= S.pme != 0;
         d.any_coul_pairs |= S.coul_pairs != 0;
         d.any_plain_pairs |= S.coul_pairs == 0;
+        if (S.has_list) {
+            if (S.bin_wg) ++d.n_bin_wg;
+            else d.any_bin_multi = true;
+        }
         d.any_excl |= S.n_ex > 0;
         d.any_baro |= S.barostat != DMDX_BARO_NONE;
         d.any_andersen |= S.thermostat == DMDX_THERMO_ANDERSEN;
@@ -699,8 +704,13 @@ int Batch::create(const dmdx_system_desc* descs, int nsys, const BatchOptions& o
     TRY(dev_upload(&d.cell_sys, cell_sys, owned));
     TRY(dev_alloc_zero(&d.rb_sys, (size_t)nsys, owned));
     TRY(dev_alloc_zero(&d.rb_pre, (size_t)nsys + 1, owned));
-    TRY(dev_alloc_zero(&d.rb_cpre, (size_t)nsys + 1, owned));
     TRY(dev_alloc_zero(&d.rb_n, 1, owned));
+    TRY(dev_alloc_zero(&d.rbm_sys, (size_t)nsys, owned));
+    TRY(dev_alloc_zero(&d.rbm_pre, (size_t)nsys + 1, owned));
+    TRY(dev_alloc_zero(&d.rbm_cpre, (size_t)nsys + 1, owned));
+    TRY(dev_alloc_zero(&d.rbm_n, 1, owned));
+    TRY(dev_alloc_zero(&d.rbw_sys, (size_t)nsys, owned));
+    TRY(dev_alloc_zero(&d.rbw_n, 1, owned));
     {
         std::vector<int> ident(slot);
         for (int64_t g = 0; g < slot; ++g) ident[g] = (int)g;

@@ -67,65 +67,87 @@ __device__ __forceinline__ int rb_find(const int64_t* __restrict__ pre, int n, i
     return lo;
 }
 
-// one 1024-thread workgroup: block-wide scans over the systems
+// one 1024-thread workgroup: block-wide scans over the systems.  Three
+// lists: every rebuilding system (slots, for the list build), those binned by
+// the multi-kernel path (slots and cells), and those binned by k_bin_system
+// (one workgroup each).
+struct RbOut {
+    int* sys;
+    int64_t* pre;
+    int64_t* cpre;  // may be null
+    int* n;
+};
+
 __global__ void __launch_bounds__(1024) k_rb_list(const SysDev* __restrict__ sys,
                                                   const SysState* __restrict__ st, int nsys,
-                                                  int* __restrict__ rb_sys,
-                                                  int64_t* __restrict__ rb_pre,
-                                                  int64_t* __restrict__ rb_cpre,
-                                                  int* __restrict__ rb_n)
+                                                  RbOut all, RbOut multi, RbOut wg)
 {
-    __shared__ int sf[1024];
-    __shared__ int64_t sp[1024], sc[1024];
-    __shared__ int base_n;
-    __shared__ int64_t base_p, base_c;
-    if (threadIdx.x == 0) {
-        base_n = 0;
-        base_p = 0;
-        base_c = 0;
-    }
+    __shared__ int sf[3][1024];
+    __shared__ int64_t sp[2][1024], sc[1024];
+    __shared__ int base_n[3];
+    __shared__ int64_t base_p[2], base_c;
+    if (threadIdx.x < 3) base_n[threadIdx.x] = 0;
+    if (threadIdx.x < 2) base_p[threadIdx.x] = 0;
+    if (threadIdx.x == 0) base_c = 0;
     __syncthreads();
+    const int t = threadIdx.x;
     for (int s0 = 0; s0 < nsys; s0 += 1024) {
-        const int s = s0 + threadIdx.x;
-        const int f = (s < nsys && st[s].rebuild) ? 1 : 0;
-        const int64_t np = f ? sys[s].n_pad : 0;
-        const int64_t nc = f ? (int64_t)sys[s].nc[0] * sys[s].nc[1] * sys[s].nc[2] : 0;
-        sf[threadIdx.x] = f;
-        sp[threadIdx.x] = np;
-        sc[threadIdx.x] = nc;
+        const int s = s0 + t;
+        const bool rb = s < nsys && st[s].rebuild;
+        const bool w = rb && sys[s].bin_wg;
+        const int f[3] = {rb ? 1 : 0, (rb && !w) ? 1 : 0, w ? 1 : 0};
+        const int64_t np = rb ? sys[s].n_pad : 0;
+        const int64_t nc = (rb && !w) ? (int64_t)sys[s].nc[0] * sys[s].nc[1] * sys[s].nc[2] : 0;
+        const int64_t npm = f[1] ? np : 0;
+        for (int k = 0; k < 3; ++k) sf[k][t] = f[k];
+        sp[0][t] = np;
+        sp[1][t] = npm;
+        sc[t] = nc;
         __syncthreads();
         for (int o = 1; o < 1024; o <<= 1) {  // inclusive scans
-            int a = 0;
-            int64_t b = 0, c = 0;
-            if ((int)threadIdx.x >= o) {
-                a = sf[threadIdx.x - o];
-                b = sp[threadIdx.x - o];
-                c = sc[threadIdx.x - o];
+            int a[3] = {0, 0, 0};
+            int64_t b0 = 0, b1 = 0, c = 0;
+            if (t >= o) {
+                for (int k = 0; k < 3; ++k) a[k] = sf[k][t - o];
+                b0 = sp[0][t - o];
+                b1 = sp[1][t - o];
+                c = sc[t - o];
             }
             __syncthreads();
-            sf[threadIdx.x] += a;
-            sp[threadIdx.x] += b;
-            sc[threadIdx.x] += c;
+            for (int k = 0; k < 3; ++k) sf[k][t] += a[k];
+            sp[0][t] += b0;
+            sp[1][t] += b1;
+            sc[t] += c;
             __syncthreads();
         }
-        if (f) {
-            const int k = base_n + sf[threadIdx.x] - 1;
-            rb_sys[k] = s;
-            rb_pre[k] = base_p + sp[threadIdx.x] - np;
-            rb_cpre[k] = base_c + sc[threadIdx.x] - nc;
+        if (f[0]) {
+            const int k = base_n[0] + sf[0][t] - 1;
+            all.sys[k] = s;
+            all.pre[k] = base_p[0] + sp[0][t] - np;
         }
+        if (f[1]) {
+            const int k = base_n[1] + sf[1][t] - 1;
+            multi.sys[k] = s;
+            multi.pre[k] = base_p[1] + sp[1][t] - npm;
+            multi.cpre[k] = base_c + sc[t] - nc;
+        }
+        if (f[2]) wg.sys[base_n[2] + sf[2][t] - 1] = s;
         __syncthreads();
-        if (threadIdx.x == 1023) {
-            base_n += sf[1023];
-            base_p += sp[1023];
+        if (t == 1023) {
+            for (int k = 0; k < 3; ++k) base_n[k] += sf[k][1023];
+            base_p[0] += sp[0][1023];
+            base_p[1] += sp[1][1023];
             base_c += sc[1023];
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        rb_n[0] = base_n;
-        rb_pre[base_n] = base_p;
-        rb_cpre[base_n] = base_c;
+    if (t == 0) {
+        all.n[0] = base_n[0];
+        all.pre[base_n[0]] = base_p[0];
+        multi.n[0] = base_n[1];
+        multi.pre[base_n[1]] = base_p[1];
+        multi.cpre[base_n[1]] = base_c;
+        wg.n[0] = base_n[2];
     }
 }
 
@@ -208,6 +230,7 @@ __global__ void k_cell_scan(const SysDev* __restrict__ sys, SysState* __restrict
     int s = blockIdx.x;
     if (!st[s].rebuild) return;
     const SysDev& S = sys[s];
+    if (S.bin_wg) return;  // binned by k_bin_system
     int ncell = cells_in_use(S, 0) * cells_in_use(S, 1) * cells_in_use(S, 2);
     if (threadIdx.x < 3) st[s].nub[threadIdx.x] = cells_in_use(S, threadIdx.x);
     __shared__ int part[256];
@@ -383,6 +406,144 @@ __global__ void k_pack_build(const SysDev* __restrict__ sys, const SysState* __r
     DMDX_FOR_RB_UNITS(u_, (R.pre[*R.n] + 255) / 256)
         k_pack_build_unit(rb_slot(R, sys, u_ * 256 + threadIdx.x), sys, st, slot_sys, nslots,
                           cell_atoms, acell, x, y, z, q, pos4s, xref, pos4f, org);
+}
+
+// ---- one workgroup bins one system (systems of at most BIN_WG_ATOMS atoms
+// whose allocated cells fit in LDS): counts, exclusive scan, fill and the
+// deterministic order inside each cell all happen in LDS; global memory sees
+// one coalesced read of the positions and the cell arrays written.  Same
+// cells and the same cell order as the multi-kernel path above, in one
+// launch for all rebuilding systems of that kind.
+constexpr int BIN_THREADS = 1024;
+constexpr int BIN_PER_THREAD = BIN_WG_ATOMS / BIN_THREADS;
+constexpr int BIN_RANK_MAX = 512;  // cells ranked by one wave (8 elements per lane)
+
+__global__ void __launch_bounds__(BIN_THREADS) k_bin_system(
+    const SysDev* __restrict__ sys, SysState* __restrict__ st, const int* __restrict__ wg_sys,
+    const int* __restrict__ wg_n, const double* __restrict__ x, const double* __restrict__ y,
+    const double* __restrict__ z, int* __restrict__ acell, int* __restrict__ ccount,
+    int* __restrict__ cstart, int* __restrict__ cell_atoms)
+{
+    __shared__ int fill[BIN_WG_CELLS];   // counts, then fill pointers
+    __shared__ int start[BIN_WG_CELLS];
+    __shared__ int slots[BIN_WG_ATOMS];  // the system's atoms in cell order
+    __shared__ int wsum[BIN_THREADS / 64];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nw = *wg_n;
+    for (int k = blockIdx.x; k < nw; k += gridDim.x) {
+        const int s = wg_sys[k];
+        const SysDev& S = sys[s];
+        const int nx = cells_in_use(S, 0), ny = cells_in_use(S, 1), nz = cells_in_use(S, 2);
+        const int ncell = nx * ny * nz;
+        const int nc3 = S.nc[0] * S.nc[1] * S.nc[2];
+        const int64_t a0 = S.atom_off;
+        const int n = S.n;
+        if (tid < 3) st[s].nub[tid] = tid == 0 ? nx : (tid == 1 ? ny : nz);
+        for (int c = tid; c < ncell; c += BIN_THREADS) fill[c] = 0;
+        __syncthreads();
+        // counts; each thread keeps its atoms' cells in registers
+        int cl[BIN_PER_THREAD];
+#pragma unroll
+        for (int r = 0; r < BIN_PER_THREAD; ++r) {
+            const int g = tid + r * BIN_THREADS;
+            cl[r] = 0;
+            if (g < n) {
+                const int64_t a = a0 + g;
+                const int c = (cell_coord(z[a], S.L[2], nz) * ny + cell_coord(y[a], S.L[1], ny)) *
+                                  nx + cell_coord(x[a], S.L[0], nx);
+                cl[r] = c;
+                acell[a] = c;
+                atomicAdd(&fill[c], 1);
+            }
+        }
+        __syncthreads();
+        // exclusive scan over the cells: each thread a contiguous chunk, then
+        // a scan of the chunk sums over the workgroup
+        const int chunk = (ncell + BIN_THREADS - 1) / BIN_THREADS;
+        const int cb = tid * chunk, ce = min(ncell, cb + chunk);
+        int sum = 0;
+        for (int c = cb; c < ce; ++c) sum += fill[c];
+        int incl = sum;  // inclusive scan within the wave
+        for (int o = 1; o < 64; o <<= 1) {
+            const int v = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += v;
+        }
+        if (lane == 63) wsum[wave] = incl;
+        __syncthreads();
+        int wbase = 0;
+        for (int w = 0; w < wave; ++w) wbase += wsum[w];
+        int run = wbase + incl - sum;
+        for (int c = cb; c < ce; ++c) {
+            const int cnt = fill[c];
+            start[c] = run;
+            ccount[S.cell_off + c] = cnt;
+            cstart[S.cell_off + c] = run;
+            fill[c] = run;
+            run += cnt;
+        }
+        for (int c = ncell + tid; c < nc3; c += BIN_THREADS) {  // allocated, not in use
+            ccount[S.cell_off + c] = 0;
+            cstart[S.cell_off + c] = n;
+        }
+        __syncthreads();
+        // fill (arbitrary order inside a cell)
+#pragma unroll
+        for (int r = 0; r < BIN_PER_THREAD; ++r) {
+            const int g = tid + r * BIN_THREADS;
+            if (g < n) slots[atomicAdd(&fill[cl[r]], 1)] = (int)(a0 + g);
+        }
+        __syncthreads();
+        // order inside each cell by slot: one wave per cell, each element
+        // ranked against the others (distinct slots, so the ranks are a
+        // permutation); cells above BIN_RANK_MAX atoms by one lane's
+        // insertion sort
+        constexpr int RPL = BIN_RANK_MAX / 64;  // elements per lane
+        for (int c = wave; c < ncell; c += BIN_THREADS / 64) {
+            const int b = start[c], m = fill[c] - b;
+            if (m <= 1) continue;
+            int* a = slots + b;
+            if (m <= 64) {
+                const int v = lane < m ? a[lane] : 0x7fffffff;
+                int rank = 0;
+                // j is wave-uniform: a lane read into a scalar register
+                for (int j = 0; j < m; ++j) rank += __builtin_amdgcn_readlane(v, j) < v ? 1 : 0;
+                if (lane < m) a[rank] = v;
+            } else if (m <= BIN_RANK_MAX) {
+                int v[RPL], rank[RPL];
+#pragma unroll
+                for (int r = 0; r < RPL; ++r) {
+                    const int e = lane + 64 * r;
+                    v[r] = e < m ? a[e] : 0x7fffffff;
+                    rank[r] = 0;
+                }
+                for (int j = 0; j < m; ++j) {
+                    const int aj = a[j];  // same address in every lane: a broadcast
+#pragma unroll
+                    for (int r = 0; r < RPL; ++r) rank[r] += aj < v[r] ? 1 : 0;
+                }
+                // every lane of the wave has read all m values before any
+                // writes (one wave, lockstep)
+#pragma unroll
+                for (int r = 0; r < RPL; ++r)
+                    if (lane + 64 * r < m) a[rank[r]] = v[r];
+            } else if (lane == 0) {
+                for (int i = 1; i < m; ++i) {
+                    const int v = a[i];
+                    int j = i - 1;
+                    while (j >= 0 && a[j] > v) {
+                        a[j + 1] = a[j];
+                        --j;
+                    }
+                    a[j + 1] = v;
+                }
+            }
+        }
+        __syncthreads();
+        // cell order (the packing follows in k_pack_build, spread over the
+        // whole device)
+        for (int lt = tid; lt < n; lt += BIN_THREADS) cell_atoms[a0 + lt] = slots[lt];
+        __syncthreads();  // LDS is reused by the next system
+    }
 }
 
 // per atom: squared displacement since the last build, per-wave maximum
@@ -864,23 +1025,33 @@ hipError_t launch_remap_rows(hipStream_t s, int* nbr, int64_t count, int* order,
 hipError_t launch_build_lists(const Dev& d, hipStream_t s)
 {
     const int BS = 256;
-    // Worklist of rebuilding systems first; the cell and list kernels then
-    // visit only their cells and slots.  Grids are sized for a full rebuild
-    // and stride over the compact ranges, whose lengths they read on the
-    // device.
-    k_rb_list<<<1, 1024, 0, s>>>(d.sys, d.st, d.nsys, d.rb_sys, d.rb_pre, d.rb_cpre, d.rb_n);
-    const RbList R{d.rb_sys, d.rb_pre, d.rb_cpre, d.rb_n};
-    if (d.ncells > 0)
-        k_cells_reset<<<grid_for(blocks_for(d.ncells, BS)), BS, 0, s>>>(d.sys, d.st, d.cell_sys,
-                                                                        d.ncells, d.ccount, d.cfill, R);
-    k_cell_count<<<grid_for(blocks_for(d.nslots, BS)), BS, 0, s>>>(
-        d.sys, d.st, d.slot_sys, d.nslots, d.x, d.y, d.z, d.acell, d.ccount, R);
-    k_cell_scan<<<d.nsys, 256, 0, s>>>(d.sys, d.st, d.ccount, d.cstart);
-    k_cell_fill<<<grid_for(blocks_for(d.nslots, BS)), BS, 0, s>>>(
-        d.sys, d.st, d.slot_sys, d.nslots, d.acell, d.cstart, d.cfill, d.cell_atoms, R);
-    if (d.ncells > 0)
-        k_cell_sort<<<grid_for((unsigned)d.ncells), 64, 0, s>>>(d.sys, d.st, d.cell_sys, d.ncells,
-                                                               d.ccount, d.cstart, d.cell_atoms, R);
+    // Worklists of rebuilding systems first; the binning and list kernels
+    // then visit only their cells and slots.  Grids are sized for a full
+    // rebuild and stride over the compact ranges, whose lengths they read on
+    // the device.
+    const RbOut all{d.rb_sys, d.rb_pre, nullptr, d.rb_n};
+    const RbOut multi{d.rbm_sys, d.rbm_pre, d.rbm_cpre, d.rbm_n};
+    const RbOut wg{d.rbw_sys, nullptr, nullptr, d.rbw_n};
+    k_rb_list<<<1, 1024, 0, s>>>(d.sys, d.st, d.nsys, all, multi, wg);
+    if (d.n_bin_wg > 0)
+        k_bin_system<<<std::min(d.n_bin_wg, 1024), BIN_THREADS, 0, s>>>(
+            d.sys, d.st, d.rbw_sys, d.rbw_n, d.x, d.y, d.z, d.acell, d.ccount, d.cstart,
+            d.cell_atoms);
+    if (d.any_bin_multi) {
+        const RbList R{d.rbm_sys, d.rbm_pre, d.rbm_cpre, d.rbm_n};
+        if (d.ncells > 0)
+            k_cells_reset<<<grid_for(blocks_for(d.ncells, BS)), BS, 0, s>>>(
+                d.sys, d.st, d.cell_sys, d.ncells, d.ccount, d.cfill, R);
+        k_cell_count<<<grid_for(blocks_for(d.nslots, BS)), BS, 0, s>>>(
+            d.sys, d.st, d.slot_sys, d.nslots, d.x, d.y, d.z, d.acell, d.ccount, R);
+        k_cell_scan<<<d.nsys, 256, 0, s>>>(d.sys, d.st, d.ccount, d.cstart);
+        k_cell_fill<<<grid_for(blocks_for(d.nslots, BS)), BS, 0, s>>>(
+            d.sys, d.st, d.slot_sys, d.nslots, d.acell, d.cstart, d.cfill, d.cell_atoms, R);
+        if (d.ncells > 0)
+            k_cell_sort<<<grid_for((unsigned)d.ncells), 64, 0, s>>>(
+                d.sys, d.st, d.cell_sys, d.ncells, d.ccount, d.cstart, d.cell_atoms, R);
+    }
+    const RbList R{d.rb_sys, d.rb_pre, nullptr, d.rb_n};
     k_pack_build<<<grid_for(blocks_for(d.nslots, BS)), BS, 0, s>>>(
         d.sys, d.st, d.slot_sys, d.nslots, d.cell_atoms, d.acell, d.x, d.y, d.z, d.q, d.pos4s,
         d.xref, d.pos4f, d.org, R);

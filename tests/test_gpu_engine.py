@@ -212,6 +212,25 @@ def test_batch_alone_equals_in_mix():
         assert alone.scalars()["pe"] == mix.scalars(k)["pe"]
 
 
+@pytest.mark.parametrize("precision", [_abi.FP64, _abi.MIXED])
+def test_both_binning_paths_in_one_batch(precision):
+    """A small system (binned by one workgroup) and one above the workgroup
+    binning limit (the multi-kernel path) in one batch: both pair sets equal
+    the oracle's, and each system's forces equal its forces alone."""
+    small = jittered_argon(n_cells=5, seed=12, skin=0.1).with_(precision=precision)
+    large = jittered_argon(n_cells=17, seed=13, skin=0.1).with_(precision=precision)
+    say(f"case: argon {small.n} + {large.n} atoms, precision {precision}")
+    e = engine([small, large])
+    e.compute()
+    for k, s in enumerate((small, large)):
+        gi, gj = e.pairs(k)
+        oi, oj = O.pairs(s.pos, s.box, s.lj_cutoff + s.list_skin, "cells")
+        assert np.array_equal(gi, oi) and np.array_equal(gj, oj), k
+        alone = engine(s)
+        alone.compute()
+        assert np.array_equal(alone.forces(), e.forces(k)), k
+
+
 def test_mixed_batch_alone_equals_in_mix():
     """Mixed precision: an LJ system batched with a water system (real-space
     Coulomb pairs) and a chain melt gives the bits it gives alone -- each
