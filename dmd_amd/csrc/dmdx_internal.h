@@ -215,6 +215,7 @@ struct Dev {
     float4* pos4f = nullptr;   // mixed list build: wrapped FP32 positions in cell order
     double4* org = nullptr;    // mixed path: origin of each slot's cell-local frame (FP64)
     float4* lpos = nullptr;    // mixed pair kernel: cell-local FP32 positions and charge
+    int* slot_of = nullptr;    // mixed path: cell-order slot of each atom slot, relative to its system
     double* p_disp = nullptr;  // per-wave max squared displacement
     bool any_auto = false;     // some system uses the displacement criterion
     int* typ_s = nullptr;      // mixed path: LJ type in cell order
@@ -305,6 +306,7 @@ hipError_t launch_kick_ke(const Dev& d, hipStream_t s);
 hipError_t launch_finalize(const Dev& d, hipStream_t s, int with_kinetic);
 hipError_t launch_barostat(const Dev& d, hipStream_t s);
 hipError_t launch_materialize(const Dev& d, hipStream_t s);
+hipError_t launch_pack_phase(const Dev& d, hipStream_t s, int after_drift);
 hipError_t launch_shake(const Dev& d, hipStream_t s);
 hipError_t launch_rattle(const Dev& d, hipStream_t s);
 hipError_t launch_andersen(const Dev& d, hipStream_t s);

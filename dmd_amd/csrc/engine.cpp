@@ -830,6 +830,7 @@ int Batch::create(const dmdx_system_desc* descs, int nsys, const BatchOptions& o
         TRY(dev_alloc_zero(&d.pos4f, slot, owned));
         TRY(dev_alloc_zero(&d.org, slot, owned));
         TRY(dev_alloc_zero(&d.lpos, slot, owned));
+        TRY(dev_alloc_zero(&d.slot_of, slot, owned));
     }
     TRY(dev_alloc_zero(&d.p_ke, nw, owned));
     TRY(dev_alloc_zero(&d.p_w, nw, owned));
@@ -942,12 +943,15 @@ int Batch::enqueue_compute(int phase, int accumulate, int force_rebuild, bool un
         bool dummy;
         mirror_compute(phase, force_rebuild, &dummy);
     }
-    if (d.any_auto && any_rebuild) LAUNCH(launch_displacement(d, stream));
+    // after the drift, the mixed path has its Verlet check and frames from
+    // k_kick_drift already, unless SHAKE moved atoms since
+    const bool after_drift = phase == 2 && d.precision != DMDX_FP64 && d.ncomp == 0;
+    if (d.any_auto && any_rebuild && !after_drift) LAUNCH(launch_displacement(d, stream));
     LAUNCH(launch_pre_compute(d, stream, phase, force_rebuild, options.reuse_forces));
     if (!any_force) return DMDX_OK;
     if (d.any_list && any_rebuild) LAUNCH(timed(KG_LIST, launch_build_lists));
     g_acc = accumulate;
-    LAUNCH(launch_pack(d, stream));
+    LAUNCH(launch_pack_phase(d, stream, phase == 2 ? 1 : 0));
     LAUNCH(timed(KG_PAIR, launch_pair_acc));
     if (d.nb || d.na || d.nd || d.ni || d.ne) LAUNCH(timed(KG_BONDED, launch_bonded));
     if (d.any_pme) LAUNCH(timed(KG_PME, launch_pme));
@@ -1205,6 +1209,10 @@ int Batch::adopt(Batch& src, int ss, int ds)
         const int64_t cnt = (int64_t)A.max_nbr * np;
         TRY(d2d(d.nbr + B.nbr_off, src.d.nbr + A.nbr_off, sizeof(int) * cnt));
         if (d.org) TRY(d2d(d.org + gb, src.d.org + ga, sizeof(double4) * np));
+        // the drift writes the frames through the cell-order map (relative
+        // to the system); the types in cell order change only at a build
+        if (d.slot_of) TRY(d2d(d.slot_of + gb, src.d.slot_of + ga, sizeof(int) * np));
+        if (d.typ_s) TRY(d2d(d.typ_s + gb, src.d.typ_s + ga, sizeof(int) * np));
         // cells of the last build (read between builds by the owner-computes
         // charge spreading); starts are relative to the system's first slot
         const int64_t nc3 = (int64_t)A.nc[0] * A.nc[1] * A.nc[2];

@@ -221,34 +221,66 @@ __global__ void k_gather(const SysDev* __restrict__ sys, const SysState* __restr
 }
 
 
+// Mixed path, besides the kick and drift: the cell-local FP32 frame of each
+// moved atom (lpos[t] = x - org[t], t its cell-order slot), so the compute
+// that follows needs no separate packing pass, and the squared displacement
+// since the last list build (per-wave maximum, the Verlet criterion) -- both
+// only where SHAKE does not move the atoms again before that compute.
+struct DriftOut {
+    const double4* org;
+    float4* lpos;
+    const int* slot_of;
+    const double4* xref;
+    double* p_disp;
+};
+
 __device__ __forceinline__ void k_kick_drift_unit(int64_t unit_, const SysDev* __restrict__ sys, const SysState* __restrict__ st,
                              const int* __restrict__ slot_sys, int64_t nslots,
                              const double* __restrict__ invm, const double* __restrict__ fx,
                              const double* __restrict__ fy, const double* __restrict__ fz,
                              double* __restrict__ vx, double* __restrict__ vy,
                              double* __restrict__ vz, double* __restrict__ x,
-                             double* __restrict__ y, double* __restrict__ z)
+                             double* __restrict__ y, double* __restrict__ z, DriftOut o)
 {
-    int64_t g = unit_ * blockDim.x + threadIdx.x;
-    if (g >= nslots) return;
-    int s = slot_sys[g];
+    const int64_t g = unit_ * blockDim.x + threadIdx.x;
+    if (g >= nslots) return;  // whole waves: nslots is a multiple of 64
+    const int s = __builtin_amdgcn_readfirstlane(slot_sys[g]);  // one system per wave
     if (s < 0 || !st[s].active) return;
     const SysDev& S = sys[s];
-    if (g - S.atom_off >= S.n) return;
-    const double lam = st[s].lambda;
-    const double dt = S.dt;
-    const double half_dt = 0.5 * dt;
-    const double im = invm[g];
-    double a = lam * vx[g], b = lam * vy[g], c = lam * vz[g];
-    a += fx[g] * im * half_dt;
-    b += fy[g] * im * half_dt;
-    c += fz[g] * im * half_dt;
-    vx[g] = a;
-    vy[g] = b;
-    vz[g] = c;
-    x[g] += a * dt;
-    y[g] += b * dt;
-    z[g] += c * dt;
+    const bool real = g - S.atom_off < S.n;
+    double d2 = 0.0;
+    if (real) {
+        const double lam = st[s].lambda;
+        const double dt = S.dt;
+        const double half_dt = 0.5 * dt;
+        const double im = invm[g];
+        double a = lam * vx[g], b = lam * vy[g], c = lam * vz[g];
+        a += fx[g] * im * half_dt;
+        b += fy[g] * im * half_dt;
+        c += fz[g] * im * half_dt;
+        vx[g] = a;
+        vy[g] = b;
+        vz[g] = c;
+        const double nx = x[g] + a * dt, ny = y[g] + b * dt, nz = z[g] + c * dt;
+        x[g] = nx;
+        y[g] = ny;
+        z[g] = nz;
+        if (o.lpos && S.n_con == 0 && st[s].built) {
+            const int64_t t = S.atom_off + o.slot_of[g];
+            const double4 f = o.org[t];
+            o.lpos[t] = make_float4((float)(nx - f.x), (float)(ny - f.y), (float)(nz - f.z),
+                                    (float)f.w);
+            if (S.interval == 0) {
+                const double4 r = o.xref[g];
+                const double ex = nx - r.x, ey = ny - r.y, ez = nz - r.z;
+                d2 = ex * ex + ey * ey + ez * ez;
+            }
+        }
+    }
+    if (o.lpos && S.n_con == 0) {
+        for (int k = 32; k > 0; k >>= 1) d2 = fmax(d2, __shfl_xor(d2, k, 64));
+        if ((threadIdx.x & 63) == 0) o.p_disp[g >> 6] = d2;
+    }
 }
 
 __global__ void k_kick_drift(const SysDev* __restrict__ sys, const SysState* __restrict__ st,
@@ -257,9 +289,10 @@ __global__ void k_kick_drift(const SysDev* __restrict__ sys, const SysState* __r
                              const double* __restrict__ fy, const double* __restrict__ fz,
                              double* __restrict__ vx, double* __restrict__ vy,
                              double* __restrict__ vz, double* __restrict__ x,
-                             double* __restrict__ y, double* __restrict__ z, int64_t nunits_)
+                             double* __restrict__ y, double* __restrict__ z, DriftOut o,
+                             int64_t nunits_)
 {
-    DMDX_FOR_UNITS(u_, nunits_) k_kick_drift_unit(u_, sys, st, slot_sys, nslots, invm, fx, fy, fz, vx, vy, vz, x, y, z);
+    DMDX_FOR_UNITS(u_, nunits_) k_kick_drift_unit(u_, sys, st, slot_sys, nslots, invm, fx, fy, fz, vx, vy, vz, x, y, z, o);
 }
 
 __device__ __forceinline__ void k_kick_ke_unit(int64_t unit_, const SysDev* __restrict__ sys, const SysState* __restrict__ st,
@@ -617,9 +650,12 @@ hipError_t launch_gather(const Dev& d, hipStream_t s)
 hipError_t launch_kick_drift(const Dev& d, hipStream_t s)
 {
     const int BS = 256;
-    k_kick_drift<<<grid_for(blocks_for(d.nslots, BS)), BS, 0, s>>>(d.sys, d.st, d.slot_sys, d.nslots,
-                                                         d.invm, d.fx, d.fy, d.fz, d.vx, d.vy,
-                                                         d.vz, d.x, d.y, d.z, (int64_t)(blocks_for(d.nslots, BS)));
+    // the mixed path fuses its frames and the Verlet check into the drift
+    const DriftOut o{d.org, d.precision == DMDX_FP64 ? nullptr : d.lpos, d.slot_of, d.xref,
+                     d.p_disp};
+    k_kick_drift<<<grid_for(blocks_for(d.nslots, BS)), BS, 0, s>>>(
+        d.sys, d.st, d.slot_sys, d.nslots, d.invm, d.fx, d.fy, d.fz, d.vx, d.vy, d.vz, d.x, d.y,
+        d.z, o, (int64_t)(blocks_for(d.nslots, BS)));
     return hipGetLastError();
 }
 

@@ -738,8 +738,8 @@ __device__ __forceinline__ void k_nbr_build_mixed_unit(int64_t t,
     int64_t nslots, const int* __restrict__ acell, const int* __restrict__ ccount,
     const int* __restrict__ cstart, const int* __restrict__ cell_atoms,
     const int* __restrict__ ex_start, const int* __restrict__ ex_list, int* __restrict__ order,
-    int* __restrict__ nn, int* __restrict__ nbr, const double4* __restrict__ pos4s,
-    const float4* __restrict__ pos4f)
+    int* __restrict__ slot_of, int* __restrict__ nn, int* __restrict__ nbr,
+    const double4* __restrict__ pos4s, const float4* __restrict__ pos4f)
 {
     const int lane = threadIdx.x % TPA;
     if (t < 0 || t >= nslots) return;  // whole waves (compact ranges are multiples of 64)
@@ -757,7 +757,10 @@ __device__ __forceinline__ void k_nbr_build_mixed_unit(int64_t t,
         return;
     }
     const int i = cell_atoms[t];
-    if (lane == 0) order[t] = i;
+    if (lane == 0) {
+        order[t] = i;
+        slot_of[i] = lt;
+    }
     const int nx = cells_in_use(S, 0), ny = cells_in_use(S, 1), nz = cells_in_use(S, 2);
     const int ci = acell[i];
     const int cx = ci % nx, cy = (ci / nx) % ny, cz = ci / (nx * ny);
@@ -894,14 +897,14 @@ __global__ void __launch_bounds__(256) k_nbr_build_mixed(
     int64_t nslots, const int* __restrict__ acell, const int* __restrict__ ccount,
     const int* __restrict__ cstart, const int* __restrict__ cell_atoms,
     const int* __restrict__ ex_start, const int* __restrict__ ex_list, int* __restrict__ order,
-    int* __restrict__ nn, int* __restrict__ nbr, const double4* __restrict__ pos4s,
-    const float4* __restrict__ pos4f, RbList R)
+    int* __restrict__ slot_of, int* __restrict__ nn, int* __restrict__ nbr,
+    const double4* __restrict__ pos4s, const float4* __restrict__ pos4f, RbList R)
 {
     constexpr int GPB = 256 / TPA;  // atoms per workgroup pass
     DMDX_FOR_RB_UNITS(u_, (R.pre[*R.n] + GPB - 1) / GPB)
         k_nbr_build_mixed_unit<TPA>(rb_slot(R, sys, u_ * GPB + threadIdx.x / TPA), sys, st,
                                     slot_sys, nslots, acell, ccount, cstart, cell_atoms,
-                                    ex_start, ex_list, order, nn, nbr, pos4s, pos4f);
+                                    ex_start, ex_list, order, slot_of, nn, nbr, pos4s, pos4f);
 }
 
 // one 64-thread workgroup per atom slot: bitonic sort of its row in LDS
@@ -968,7 +971,7 @@ __device__ __forceinline__ void k_pack_unit(int64_t unit_, const SysDev* __restr
                        const int* __restrict__ order, const double* __restrict__ x,
                        const double* __restrict__ y, const double* __restrict__ z,
                        const int* __restrict__ type, const double4* __restrict__ org,
-                       float4* __restrict__ lpos, int* __restrict__ typ_s)
+                       float4* __restrict__ lpos, int* __restrict__ typ_s, int after_drift)
 {
     int64_t t = unit_ * blockDim.x + threadIdx.x;
     if (t >= nslots) return;
@@ -976,6 +979,9 @@ __device__ __forceinline__ void k_pack_unit(int64_t unit_, const SysDev* __restr
     if (s < 0 || st[s].reuse || !st[s].active) return;
     const SysDev& S = sys[s];
     if (t - S.atom_off >= S.n) return;
+    // after the drift, k_kick_drift has written the frames already, except
+    // where the cell order just changed or SHAKE moved atoms since
+    if (after_drift && !st[s].rebuild && S.n_con == 0) return;
     int i = order[t];
     const double4 o = org[t];
     lpos[t] = make_float4((float)(x[i] - o.x), (float)(y[i] - o.y), (float)(z[i] - o.z),
@@ -988,9 +994,10 @@ __global__ void k_pack(const SysDev* __restrict__ sys, const SysState* __restric
                        const int* __restrict__ order, const double* __restrict__ x,
                        const double* __restrict__ y, const double* __restrict__ z,
                        const int* __restrict__ type, const double4* __restrict__ org,
-                       float4* __restrict__ lpos, int* __restrict__ typ_s, int64_t nunits_)
+                       float4* __restrict__ lpos, int* __restrict__ typ_s, int after_drift,
+                       int64_t nunits_)
 {
-    DMDX_FOR_UNITS(u_, nunits_) k_pack_unit(u_, sys, st, slot_sys, nslots, order, x, y, z, type, org, lpos, typ_s);
+    DMDX_FOR_UNITS(u_, nunits_) k_pack_unit(u_, sys, st, slot_sys, nslots, order, x, y, z, type, org, lpos, typ_s, after_drift);
 }
 
 }  // namespace dmdx
@@ -1064,7 +1071,7 @@ hipError_t launch_build_lists(const Dev& d, hipStream_t s)
         const int64_t units = blocks_for(d.nslots * tpa, BS);
 #define DMDX_BUILD_ARGS                                                                       \
     d.sys, d.st, d.slot_sys, d.nslots, d.acell, d.ccount, d.cstart, d.cell_atoms, d.ex_start, \
-        d.ex_list, d.order, d.nn, d.nbr, d.pos4s, d.pos4f, R
+        d.ex_list, d.order, d.slot_of, d.nn, d.nbr, d.pos4s, d.pos4f, R
         if (tpa == 8) k_nbr_build_mixed<8><<<grid_for(units), BS, 0, s>>>(DMDX_BUILD_ARGS);
         else if (tpa == 32) k_nbr_build_mixed<32><<<grid_for(units), BS, 0, s>>>(DMDX_BUILD_ARGS);
         else k_nbr_build_mixed<16><<<grid_for(units), BS, 0, s>>>(DMDX_BUILD_ARGS);
@@ -1083,13 +1090,15 @@ hipError_t launch_displacement(const Dev& d, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_pack(const Dev& d, hipStream_t s)
+hipError_t launch_pack_phase(const Dev& d, hipStream_t s, int after_drift)
 {
     if (d.precision == DMDX_FP64) return hipSuccess;
-    k_pack<<<grid_for(blocks_for(d.nslots, 256)), 256, 0, s>>>(d.sys, d.st, d.slot_sys, d.nslots, d.order,
-                                                     d.x, d.y, d.z, d.type, d.org, d.lpos,
-                                                     d.typ_s, (int64_t)(blocks_for(d.nslots, 256)));
+    k_pack<<<grid_for(blocks_for(d.nslots, 256)), 256, 0, s>>>(
+        d.sys, d.st, d.slot_sys, d.nslots, d.order, d.x, d.y, d.z, d.type, d.org, d.lpos, d.typ_s,
+        after_drift, (int64_t)(blocks_for(d.nslots, 256)));
     return hipGetLastError();
 }
+
+hipError_t launch_pack(const Dev& d, hipStream_t s) { return launch_pack_phase(d, s, 0); }
 
 }  // namespace dmdx
