@@ -732,6 +732,10 @@ __global__ void __launch_bounds__(256) k_nbr_build(
 // radius are decided by the reference's own FP64 expression on pos4s, so
 // the pair set is exactly the FP64 one.  Entries carry the cell offset
 // (NBR_SLOT_BITS) for the pair kernel.
+// neighbour cells of one atom on the mixed path: 5^3 with half-width cells,
+// 3^3 otherwise, at most every cell of a small system (< 5 cells per edge)
+constexpr int NB_CELLS = 125;
+
 template <int TPA>
 __device__ __forceinline__ void k_nbr_build_mixed_unit(int64_t t,
     const SysDev* __restrict__ sys, SysState* __restrict__ st, const int* __restrict__ slot_sys,
@@ -739,7 +743,8 @@ __device__ __forceinline__ void k_nbr_build_mixed_unit(int64_t t,
     const int* __restrict__ cstart, const int* __restrict__ cell_atoms,
     const int* __restrict__ ex_start, const int* __restrict__ ex_list, int* __restrict__ order,
     int* __restrict__ slot_of, int* __restrict__ nn, int* __restrict__ nbr,
-    const double4* __restrict__ pos4s, const float4* __restrict__ pos4f)
+    const double4* __restrict__ pos4s, const float4* __restrict__ pos4f,
+    int (*tab_base)[NB_CELLS], int (*tab_pre)[NB_CELLS + 1], int (*tab_off)[NB_CELLS])
 {
     const int lane = threadIdx.x % TPA;
     if (t < 0 || t >= nslots) return;  // whole waves (compact ranges are multiples of 64)
@@ -790,98 +795,129 @@ __device__ __forceinline__ void k_nbr_build_mixed_unit(int64_t t,
     const int lox = lo_of(nx), hix = hi_of(nx);
     const int loy = lo_of(ny), hiy = hi_of(ny);
     const int loz = lo_of(nz), hiz = hi_of(nz);
+    // The atom's candidates, as one stream: the neighbour cells in scan order
+    // (z, then y, then x offset, each ascending), each cell's atoms in cell
+    // order -- the order of the candidate numbers k written below.  The
+    // group's TPA lanes first tabulate the cells (cell-order start, running
+    // count, offsets) in LDS, then take candidates k, k + TPA, ...: every
+    // pass but the last is full, however the atoms split over cells.
+    const int ex_n = hix - lox + 1, ey_n = hiy - loy + 1;
+    const int ncb = ex_n * ey_n * (hiz - loz + 1);  // <= NB_CELLS
+    const int grp = (threadIdx.x / TPA);
+    int* tb = tab_base[grp];
+    int* tp = tab_pre[grp];
+    int* to = tab_off[grp];
+    {
+        // lane l takes cells l, l + TPA, ...; all their loads are issued
+        // before any count is used
+        constexpr int Q = (NB_CELLS + TPA - 1) / TPA;
+        int cnt_q[Q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const int jc = q * TPA + lane;
+            cnt_q[q] = 0;
+            if (jc < ncb) {
+                const int ox = lox + jc % ex_n, oy = loy + (jc / ex_n) % ey_n,
+                          oz = loz + jc / (ex_n * ey_n);
+                int xx = cx + ox, yy = cy + oy, zz = cz + oz;
+                xx = xx < 0 ? xx + nx : (xx >= nx ? xx - nx : xx);
+                yy = yy < 0 ? yy + ny : (yy >= ny ? yy - ny : yy);
+                zz = zz < 0 ? zz + nz : (zz >= nz ? zz - nz : zz);
+                const int c = S.cell_off + (zz * ny + yy) * nx + xx;
+                tb[jc] = cstart[c];
+                cnt_q[q] = ccount[c];
+                to[jc] = (int)((((uint32_t)ox & 7u) << NBR_SLOT_BITS) |
+                               (((uint32_t)oy & 7u) << (NBR_SLOT_BITS + 3)) |
+                               (((uint32_t)oz & 7u) << (NBR_SLOT_BITS + 6)));
+            }
+        }
+        // running counts in scan order: a group scan per round of TPA cells
+        int run = 0;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            if (q * TPA >= ncb) break;
+            int incl = cnt_q[q];
+            for (int o = 1; o < TPA; o <<= 1) {
+                const int v = __shfl_up(incl, o, TPA);
+                if (lane >= o) incl += v;
+            }
+            const int jc = q * TPA + lane;
+            if (jc < ncb) tp[jc] = run + incl - cnt_q[q];
+            run += __shfl(incl, TPA - 1, TPA);
+        }
+        if (lane == 0) tp[ncb] = run;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const int ncand = tp[ncb];
+    const float nwx = -li.x, nwy = -li.y, nwz = -li.z;
     // the wave-uniform minimum-image switch selects a loop variant outside
     // the candidate loop
     auto scan = [&](auto wrap_c) -> int {
-    constexpr bool WRAP = decltype(wrap_c)::value;
-    int k = 0;
-    for (int oz = loz; oz <= hiz; ++oz) {
-        int zz = cz + oz;
-        zz = zz < 0 ? zz + nz : (zz >= nz ? zz - nz : zz);
-        const float sz = oz * wz - li.z;
-        for (int oy = loy; oy <= hiy; ++oy) {
-            int yy = cy + oy;
-            yy = yy < 0 ? yy + ny : (yy >= ny ? yy - ny : yy);
-            const float sy = oy * wy - li.y;
-            const int rowc = S.cell_off + (zz * ny + yy) * nx;
-            // the x neighbours of this (y, z) row of cells, in increasing
-            // offset; runs of consecutive cell indices hold their atoms
-            // contiguously in cell order, so each run is scanned as one range
-            // (same candidates in the same order as cell by cell)
-            int ox = lox;
-            while (ox <= hix) {
-                int xx = cx + ox;
-                xx = xx < 0 ? xx + nx : (xx >= nx ? xx - nx : xx);
-                int len = 1;
-                while (ox + len <= hix && xx + len < nx) ++len;
-                const int c0 = rowc + xx;
-                const int64_t base = S.atom_off + cstart[c0];
-                // ends of the run's cells, relative to the run start
-                int e[5];
-                int acc = 0;
-#pragma unroll
-                for (int q = 0; q < 5; ++q) {
-                    acc += q < len ? ccount[c0 + (q < len ? q : 0)] : 0;
-                    e[q] = q < len ? acc : 0x7fffffff;
-                }
-                const int cnt = acc;
-                const int ox0 = ox;
-                const uint32_t syz = (((uint32_t)oy & 7u) << (NBR_SLOT_BITS + 3)) |
-                                     (((uint32_t)oz & 7u) << (NBR_SLOT_BITS + 6));
-                // the next pass's position is loaded before this pass is
-                // processed, so the gather latency overlaps the arithmetic
-                float4 nxt = lane < cnt ? pos4f[base + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
-                for (int m0 = 0; m0 < cnt; m0 += TPA) {
-                    const int m = m0 + lane;
-                    const float4 lj = nxt;
-                    if (m + TPA < cnt) nxt = pos4f[base + m + TPA];
-                    const int oxm = ox0 + (m >= e[0] ? 1 : 0) + (m >= e[1] ? 1 : 0) +
-                                    (m >= e[2] ? 1 : 0) + (m >= e[3] ? 1 : 0);
-                    bool hit = false;
-                    if (m < cnt && base + m != t) {
-                        float dx = lj.x + (oxm * wx - li.x), dy = lj.y + sy, dz = lj.z + sz;
-                        if constexpr (WRAP) {
-                            dx -= rintf(dx * (float)S.invL[0]) * (float)S.L[0];
-                            dy -= rintf(dy * (float)S.invL[1]) * (float)S.L[1];
-                            dz -= rintf(dz * (float)S.invL[2]) * (float)S.L[2];
-                        }
-                        const float r2f = dx * dx + dy * dy + dz * dz;
-                        hit = r2f < lo2;
-                        if (!hit && r2f < hi2) {
-                            // the reference's expression (src/core/cell.cpp:26-38,
-                            // src/force/lennard_jones.cpp:29-36)
-                            const double4 pi4 = pos4s[t];
-                            const double4 pj = pos4s[base + m];
-                            double ddx = min_image_dmd(pj.x - pi4.x, S.L[0], S.invL[0]);
-                            double ddy = min_image_dmd(pj.y - pi4.y, S.L[1], S.invL[1]);
-                            double ddz = min_image_dmd(pj.z - pi4.z, S.L[2], S.invL[2]);
-                            hit = ddx * ddx + ddy * ddy + ddz * ddz < S.rlist2;
-                        }
-                        if (hit && check_ex) {
-                            const int j = cell_atoms[base + m];
-                            if (exn <= EXR) {
-                                bool ex = false;
-#pragma unroll
-                                for (int q = 0; q < EXR; ++q) ex |= exr[q] == j;
-                                hit = !ex;
-                            } else {
-                                hit = !is_excluded(ex_start, ex_list, i, j);
-                            }
-                        }
-                    }
-                    const uint64_t bal = __ballot(hit);
-                    const uint64_t mine = (bal >> gshift) & (TPA == 64 ? ~0ull : ((1ull << TPA) - 1));
-                    const int pos = k + __popcll(mine & below);
-                    if (hit && pos < M)
-                        row[pos] = (int)((uint32_t)(base + m - S.atom_off) | syz |
-                                         (((uint32_t)oxm & 7u) << NBR_SLOT_BITS));
-                    k += __popcll(mine);
-                }
-                ox += len;
+        constexpr bool WRAP = decltype(wrap_c)::value;
+        int k = 0;
+        int jc = 0;  // the lane's current cell: tp[jc] <= candidate < tp[jc + 1]
+        auto locate = [&](int kc) {
+            while (jc + 1 < ncb && tp[jc + 1] <= kc) ++jc;
+        };
+        locate(lane);
+        int slot_n = lane < ncand ? tb[jc] + (lane - tp[jc]) : 0;
+        float4 nxt = lane < ncand ? pos4f[S.atom_off + slot_n] : make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int k0 = 0; k0 < ncand; k0 += TPA) {
+            const int kc = k0 + lane;
+            const float4 lj = nxt;
+            const int slot = slot_n;
+            const uint32_t offs = kc < ncand ? (uint32_t)to[jc] : 0u;
+            // the next pass's candidate is located and its position loaded
+            // before this one is tested
+            if (kc + TPA < ncand) {
+                locate(kc + TPA);
+                slot_n = tb[jc] + (kc + TPA - tp[jc]);
+                nxt = pos4f[S.atom_off + slot_n];
             }
+            bool hit = false;
+            if (kc < ncand && S.atom_off + slot != t) {
+                const float ox = (float)((int)(offs << (32 - NBR_SLOT_BITS - 3)) >> 29);
+                const float oy = (float)((int)(offs << (32 - NBR_SLOT_BITS - 6)) >> 29);
+                const float oz = (float)((int)offs >> 29);
+                float dx = lj.x + (ox * wx + nwx), dy = lj.y + (oy * wy + nwy),
+                      dz = lj.z + (oz * wz + nwz);
+                if constexpr (WRAP) {
+                    dx -= rintf(dx * (float)S.invL[0]) * (float)S.L[0];
+                    dy -= rintf(dy * (float)S.invL[1]) * (float)S.L[1];
+                    dz -= rintf(dz * (float)S.invL[2]) * (float)S.L[2];
+                }
+                const float r2f = dx * dx + dy * dy + dz * dz;
+                hit = r2f < lo2;
+                if (!hit && r2f < hi2) {
+                    // the reference's expression (src/core/cell.cpp:26-38,
+                    // src/force/lennard_jones.cpp:29-36)
+                    const double4 pi4 = pos4s[t];
+                    const double4 pj = pos4s[S.atom_off + slot];
+                    double ddx = min_image_dmd(pj.x - pi4.x, S.L[0], S.invL[0]);
+                    double ddy = min_image_dmd(pj.y - pi4.y, S.L[1], S.invL[1]);
+                    double ddz = min_image_dmd(pj.z - pi4.z, S.L[2], S.invL[2]);
+                    hit = ddx * ddx + ddy * ddy + ddz * ddz < S.rlist2;
+                }
+                if (hit && check_ex) {
+                    const int j = cell_atoms[S.atom_off + slot];
+                    if (exn <= EXR) {
+                        bool ex = false;
+#pragma unroll
+                        for (int q = 0; q < EXR; ++q) ex |= exr[q] == j;
+                        hit = !ex;
+                    } else {
+                        hit = !is_excluded(ex_start, ex_list, i, j);
+                    }
+                }
+            }
+            const uint64_t bal = __ballot(hit);
+            const uint64_t mine = (bal >> gshift) & (TPA == 64 ? ~0ull : ((1ull << TPA) - 1));
+            const int pos = k + __popcll(mine & below);
+            if (hit && pos < M) row[pos] = (int)((uint32_t)slot | offs);
+            k += __popcll(mine);
         }
-    }
-    return k;
+        return k;
     };
     int k = wrap ? scan(std::true_type{}) : scan(std::false_type{});
     if (k > M) {
@@ -901,10 +937,12 @@ __global__ void __launch_bounds__(256) k_nbr_build_mixed(
     const double4* __restrict__ pos4s, const float4* __restrict__ pos4f, RbList R)
 {
     constexpr int GPB = 256 / TPA;  // atoms per workgroup pass
+    __shared__ int tab_base[GPB][NB_CELLS], tab_pre[GPB][NB_CELLS + 1], tab_off[GPB][NB_CELLS];
     DMDX_FOR_RB_UNITS(u_, (R.pre[*R.n] + GPB - 1) / GPB)
         k_nbr_build_mixed_unit<TPA>(rb_slot(R, sys, u_ * GPB + threadIdx.x / TPA), sys, st,
                                     slot_sys, nslots, acell, ccount, cstart, cell_atoms,
-                                    ex_start, ex_list, order, slot_of, nn, nbr, pos4s, pos4f);
+                                    ex_start, ex_list, order, slot_of, nn, nbr, pos4s, pos4f,
+                                    tab_base, tab_pre, tab_off);
 }
 
 // one 64-thread workgroup per atom slot: bitonic sort of its row in LDS
