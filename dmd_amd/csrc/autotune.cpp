@@ -1,9 +1,10 @@
 // autotune.cpp -- launch-shape autotuner for the mixed-precision kernels.
 //
-// The pair kernel and the list build of the mixed path give each atom a
-// group of 8, 16 or 32 lanes, and the pair kernel takes 2 or 4 list entries
-// per lane per pass (4 hides more gather latency per wave, 2 needs fewer
-// registers so more waves fit).  Which is fastest depends on the systems (how
+// The pair kernel of the mixed path gives each atom a group of 2 to 32
+// lanes, each taking 2, 4 or 8 list entries per pass (more entries per lane
+// keep more gathers in flight per wave, fewer need fewer registers so more
+// waves fit; fewer lanes per atom waste fewer lanes on the ragged end of a
+// row), and the list build gives each atom 8, 16 or 32 lanes.  Which is fastest depends on the systems (how
 // many neighbours an atom has, how many atoms a batch holds, how many rebuild
 // at once), so the choice is measured: a scratch copy of the engine's batch
 // is stepped with each candidate and the times of the two kernel groups are
@@ -81,17 +82,40 @@ bool cache_lookup(const char* path, uint64_t key, int* pair, int* width, int* bu
     return false;
 }
 
+// pair kernel candidates: (lanes per atom, entries per lane per pass);
+// list build candidates: lanes per atom
+constexpr int NPAIR = 8, NBUILD = 3;
+constexpr int PAIR_LANES[NPAIR] = {2, 4, 8, 4, 8, 16, 8, 16};
+constexpr int PAIR_WIDTH[NPAIR] = {8, 8, 8, 4, 4, 4, 2, 2};
+constexpr int BUILD_LANES[NBUILD] = {8, 16, 32};
+static_assert(NPAIR + NBUILD == DMDX_TUNE_TIMES, "times_ms layout");
+
+bool pair_candidate(int p, int w)
+{
+    for (int c = 0; c < NPAIR; ++c)
+        if (PAIR_LANES[c] == p && PAIR_WIDTH[c] == w) return true;
+    return false;
+}
+
 void cache_store(const char* path, uint64_t key, int pair, int width, int build, const double* t)
 {
     if (!path || !*path) return;
+    std::ostringstream o;
+    char buf[48];
+    std::snprintf(buf, sizeof(buf), "%016llx %d %d %d  # pair ms", (unsigned long long)key, pair,
+                  width, build);
+    o << buf;
+    for (int c = 0; c < NPAIR; ++c) {
+        std::snprintf(buf, sizeof(buf), " %dx%d %.4f", PAIR_LANES[c], PAIR_WIDTH[c], t[c]);
+        o << buf;
+    }
+    o << " build ms";
+    for (int c = 0; c < NBUILD; ++c) {
+        std::snprintf(buf, sizeof(buf), " %d %.4f", BUILD_LANES[c], t[NPAIR + c]);
+        o << buf;
+    }
     std::ofstream f(path, std::ios::app);
-    char buf[320];
-    std::snprintf(buf, sizeof(buf),
-                  "%016llx %d %d %d  # pair ms 8x2 %.4f 8x4 %.4f 16x2 %.4f 16x4 %.4f 32x2 %.4f "
-                  "32x4 %.4f build ms 8 %.4f 16 %.4f 32 %.4f\n",
-                  (unsigned long long)key, pair, width, build, t[0], t[1], t[2], t[3], t[4], t[5],
-                  t[6], t[7], t[8]);
-    f << buf;
+    f << o.str() << "\n";
 }
 
 }  // namespace
@@ -101,19 +125,17 @@ extern "C" int dmdx_engine_autotune(dmdx_engine* e, int32_t steps, const char* c
                                     int32_t* build_lanes, double* times_ms)
 {
     if (!e) return dmdx::fail(DMDX_E_INVALID, "null engine");
-    // pair kernel: (lanes per atom, entries per lane per pass); list build:
-    // lanes per atom.  Pass c times pair shape c and build shape c % 3.
-    const int pl[6] = {8, 8, 16, 16, 32, 32};
-    const int pw[6] = {2, 4, 2, 4, 2, 4};
-    const int bl[3] = {8, 16, 32};
-    double t[9];
+    const int* pl = PAIR_LANES;
+    const int* pw = PAIR_WIDTH;
+    const int* bl = BUILD_LANES;
+    double t[DMDX_TUNE_TIMES];
     for (double& v : t) v = -1.0;
     auto report = [&]() {
         if (pair_lanes) *pair_lanes = e->b.d.pair_tpa;
         if (pair_width) *pair_width = e->b.d.pair_width;
         if (build_lanes) *build_lanes = e->b.d.build_tpa;
         if (times_ms)
-            for (int k = 0; k < 9; ++k) times_ms[k] = t[k];
+            for (int k = 0; k < DMDX_TUNE_TIMES; ++k) times_ms[k] = t[k];
     };
     if (e->b.d.precision == DMDX_FP64) {  // one lane per atom, nothing to tune
         report();
@@ -121,8 +143,8 @@ extern "C" int dmdx_engine_autotune(dmdx_engine* e, int32_t steps, const char* c
     }
     const uint64_t key = signature(e);
     int cp = 0, cw = 0, cb = 0;
-    if (cache_lookup(cache_path, key, &cp, &cw, &cb) && (cp == 8 || cp == 16 || cp == 32) &&
-        (cw == 2 || cw == 4) && (cb == 8 || cb == 16 || cb == 32)) {
+    if (cache_lookup(cache_path, key, &cp, &cw, &cb) && pair_candidate(cp, cw) &&
+        (cb == 8 || cb == 16 || cb == 32)) {
         e->b.d.pair_tpa = cp;
         e->b.d.pair_width = cw;
         e->b.d.build_tpa = cb;
@@ -146,7 +168,7 @@ extern "C" int dmdx_engine_autotune(dmdx_engine* e, int32_t steps, const char* c
     double ms0, ms1;
     int64_t n0, n1;
     // pair kernel: per-launch time of each shape over n steps
-    for (int c = 0; c < 6; ++c) {
+    for (int c = 0; c < NPAIR; ++c) {
         scratch.d.pair_tpa = pl[c];
         scratch.d.pair_width = pw[c];
         rc = scratch.step(1);  // first launches of this shape
@@ -158,23 +180,23 @@ extern "C" int dmdx_engine_autotune(dmdx_engine* e, int32_t steps, const char* c
     }
     // list build: the candidates take turns step by step, so each sees the
     // same mix of full and partial rebuilds; time per step of the list group
-    double acc[3] = {0.0, 0.0, 0.0};
+    double acc[NBUILD] = {};
     rc = scratch.kernel_time(dmdx::KG_LIST, &ms0, &n0);
     if (rc) return rc;
-    for (int k = 0; k < 3 * n; ++k) {
-        scratch.d.build_tpa = bl[k % 3];
+    for (int k = 0; k < NBUILD * n; ++k) {
+        scratch.d.build_tpa = bl[k % NBUILD];
         rc = scratch.step(1);
         if (!rc) rc = scratch.kernel_time(dmdx::KG_LIST, &ms1, &n1);
         if (rc) return rc;
-        acc[k % 3] += ms1 - ms0;
+        acc[k % NBUILD] += ms1 - ms0;
         ms0 = ms1;
     }
-    for (int c = 0; c < 3; ++c) t[6 + c] = acc[c] / n;
+    for (int c = 0; c < NBUILD; ++c) t[NPAIR + c] = acc[c] / n;
     int bp = 0, bb = 0;
-    for (int c = 1; c < 6; ++c)
+    for (int c = 1; c < NPAIR; ++c)
         if (t[c] < t[bp]) bp = c;
-    for (int c = 1; c < 3; ++c)
-        if (t[6 + c] < t[6 + bb]) bb = c;
+    for (int c = 1; c < NBUILD; ++c)
+        if (t[NPAIR + c] < t[NPAIR + bb]) bb = c;
     e->b.d.pair_tpa = pl[bp];
     e->b.d.pair_width = pw[bp];
     e->b.d.build_tpa = bl[bb];

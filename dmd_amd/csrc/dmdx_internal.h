@@ -219,9 +219,9 @@ struct Dev {
     double* p_disp = nullptr;  // per-wave max squared displacement
     bool any_auto = false;     // some system uses the displacement criterion
     int* typ_s = nullptr;      // mixed path: LJ type in cell order
-    int pair_tpa = 16;         // mixed pair kernel: lanes per atom (8, 16, 32)
+    int pair_tpa = 4;          // mixed pair kernel: lanes per atom (2, 4, 8, 16, 32)
     int build_tpa = 16;        // mixed list build: lanes per atom (8, 16, 32)
-    int pair_width = 4;        // mixed pair kernel: entries per lane per pass (2, 4)
+    int pair_width = 8;        // mixed pair kernel: entries per lane per pass (2, 4, 8)
     // LJ tables
     double *lj_sig = nullptr, *lj_eps = nullptr;
     float *lj_c12 = nullptr, *lj_c6 = nullptr;
@@ -293,6 +293,7 @@ struct Dev {
 // host launchers (one per kernel group); return hipGetLastError()
 hipError_t launch_build_lists(const Dev& d, hipStream_t s);
 hipError_t launch_pair(const Dev& d, hipStream_t s, int accumulate);
+void pair_shape(const Dev& d, int& tpa, int& w);  // effective mixed pair-kernel launch shape
 hipError_t launch_pack(const Dev& d, hipStream_t s);
 hipError_t launch_bonded(const Dev& d, hipStream_t s);
 hipError_t launch_pme(const Dev& d, hipStream_t s);

@@ -186,7 +186,7 @@ int Batch::create(const dmdx_system_desc* descs, int nsys, const BatchOptions& o
             M = std::min(M, n - 1);
             M = std::max(M, 1);
             if (opt.max_nbr > 0) M = std::min(std::max(opt.max_nbr, 1), std::max(n - 1, 1));
-            M = (M + 3) / 4 * 4;
+            M = (M + 7) / 8 * 8;  // rows hold whole 8-entry vector loads
             if (M > 2048)
                 return fail(DMDX_E_PARM, "neighbour list would exceed 2048 entries per atom; "
                                          "reduce the cutoff");
@@ -1422,15 +1422,19 @@ int dmdx_engine_set_tuning(dmdx_engine* e, int32_t key, int32_t value)
 {
     if (!e) return dmdx::fail(DMDX_E_INVALID, "null engine");
     if (key == 2) {
-        if (value != 2 && value != 4)
-            return dmdx::fail(DMDX_E_PARM, "entries per lane must be 2 or 4");
+        if (value != 2 && value != 4 && value != 8)
+            return dmdx::fail(DMDX_E_PARM, "entries per lane must be 2, 4 or 8");
         e->b.d.pair_width = value;
-    } else {
+    } else if (key == 0) {
+        if (value != 2 && value != 4 && value != 8 && value != 16 && value != 32)
+            return dmdx::fail(DMDX_E_PARM, "pair lanes per atom must be 2, 4, 8, 16 or 32");
+        e->b.d.pair_tpa = value;
+    } else if (key == 1) {
         if (value != 8 && value != 16 && value != 32)
-            return dmdx::fail(DMDX_E_PARM, "lanes per atom must be 8, 16 or 32");
-        if (key == 0) e->b.d.pair_tpa = value;
-        else if (key == 1) e->b.d.build_tpa = value;
-        else return dmdx::fail(DMDX_E_INVALID, "unknown tuning key");
+            return dmdx::fail(DMDX_E_PARM, "build lanes per atom must be 8, 16 or 32");
+        e->b.d.build_tpa = value;
+    } else {
+        return dmdx::fail(DMDX_E_INVALID, "unknown tuning key");
     }
     e->b.invalidate_graph();
     return DMDX_OK;

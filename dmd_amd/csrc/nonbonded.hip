@@ -198,7 +198,7 @@ __device__ __forceinline__ void k_pair_mixed_unit(int64_t unit_,
     const int* __restrict__ tys = typ_s + a0;
     const int cnt = nn[t];
     double fxi = 0.0, fyi = 0.0, fzi = 0.0, ej = 0.0, ec = 0.0;
-    // Each lane takes W (2 or 4) entries per pass and issues their W gathers
+    // Each lane takes W (2, 4 or 8) entries per pass and issues their W gathers
     // before any arithmetic (memory-level parallelism for the dependent
     // loads; W=2 needs fewer registers, so more waves fit).  The wave-uniform
     // switches (minimum-image wrap, typed LJ tables) select a loop variant
@@ -215,8 +215,8 @@ __device__ __forceinline__ void k_pair_mixed_unit(int64_t unit_,
         // computed; entry reads are clamped into the row (index < M) and
         // entries at or past cnt never address memory.  LJ variant (less
         // arithmetic per entry, so occupancy matters more): lane l takes the
-        // W consecutive entries W l .. W l + W - 1 with one vector load
-        // (rows are 16-byte aligned, M a multiple of 4) and no prefetch.
+        // W consecutive entries W l .. W l + W - 1 with one or two vector
+        // loads (rows are 16-byte aligned, M a multiple of 8) and no prefetch.
         constexpr bool STRIDED = COUL;
         auto entry_index = [&](int k0, int q) {
             return STRIDED ? k0 + lane + q * TPA : k0 + lane * W + q;
@@ -229,6 +229,17 @@ __device__ __forceinline__ void k_pair_mixed_unit(int64_t unit_,
                     const int k = k0 + lane + q * TPA;
                     ev[q] = (uint32_t)row[k < mlast ? k : mlast];
                 }
+            } else if constexpr (W == 8) {
+                const int4 ja = *reinterpret_cast<const int4*>(row + k0 + lane * W);
+                const int4 jb = *reinterpret_cast<const int4*>(row + k0 + lane * W + 4);
+                ev[0] = (uint32_t)ja.x;
+                ev[1] = (uint32_t)ja.y;
+                ev[2] = (uint32_t)ja.z;
+                ev[3] = (uint32_t)ja.w;
+                ev[4] = (uint32_t)jb.x;
+                ev[5] = (uint32_t)jb.y;
+                ev[6] = (uint32_t)jb.z;
+                ev[7] = (uint32_t)jb.w;
             } else if constexpr (W == 4) {
                 const int4 jj = *reinterpret_cast<const int4*>(row + k0 + lane * W);
                 ev[0] = (uint32_t)jj.x;
@@ -365,25 +376,49 @@ __global__ void __launch_bounds__(256) k_pair_mixed(
 
 namespace dmdx {
 
+// Launch shape of the mixed pair kernel from the tuning values: lanes per
+// atom x entries per lane pass.  8 entries per pass come with 2, 4 or 8
+// lanes; 2 or 4 entries with 4 (4 entries only), 8, 16 or 32 lanes.  Other
+// combinations take the nearest of these.
+void pair_shape(const Dev& d, int& tpa, int& w)
+{
+    w = d.pair_width;
+    tpa = d.pair_tpa;
+    if (w == 8) {
+        if (tpa > 8) tpa = 8;
+    } else {
+        if (w != 2) w = 4;
+        if (tpa < 8 && !(tpa == 4 && w == 4)) tpa = 8;
+    }
+}
+
 template <bool COUL>
 static hipError_t launch_pair_mixed(const Dev& d, hipStream_t s, int accumulate)
 {
     const int BS = 256;
-    const int tpa = d.pair_tpa;
+    int tpa, w;
+    pair_shape(d, tpa, w);
     const int64_t units = blocks_for(d.nslots * tpa, BS);
 #define DMDX_PAIR_ARGS                                                                       \
     d.sys, d.st, d.slot_sys, d.nslots, d.order, d.nn, d.nbr, d.lpos, d.typ_s, d.lj_c12, d.lj_c6, \
         d.fx, d.fy, d.fz, d.p_lj, d.p_coul, accumulate, units
     const unsigned g = grid_for(units);
-    if (d.pair_width == 2) {
-        if (tpa == 8) k_pair_mixed<8, 2, COUL><<<g, BS, 0, s>>>(DMDX_PAIR_ARGS);
-        else if (tpa == 32) k_pair_mixed<32, 2, COUL><<<g, BS, 0, s>>>(DMDX_PAIR_ARGS);
-        else k_pair_mixed<16, 2, COUL><<<g, BS, 0, s>>>(DMDX_PAIR_ARGS);
-    } else {
-        if (tpa == 8) k_pair_mixed<8, 4, COUL><<<g, BS, 0, s>>>(DMDX_PAIR_ARGS);
-        else if (tpa == 32) k_pair_mixed<32, 4, COUL><<<g, BS, 0, s>>>(DMDX_PAIR_ARGS);
-        else k_pair_mixed<16, 4, COUL><<<g, BS, 0, s>>>(DMDX_PAIR_ARGS);
+#define DMDX_PAIR_SHAPE(T_, W_) \
+    case T_ * 100 + W_: k_pair_mixed<T_, W_, COUL><<<g, BS, 0, s>>>(DMDX_PAIR_ARGS); break
+    switch (tpa * 100 + w) {
+        DMDX_PAIR_SHAPE(2, 8);
+        DMDX_PAIR_SHAPE(4, 8);
+        DMDX_PAIR_SHAPE(8, 8);
+        DMDX_PAIR_SHAPE(4, 4);
+        DMDX_PAIR_SHAPE(8, 2);
+        DMDX_PAIR_SHAPE(8, 4);
+        DMDX_PAIR_SHAPE(16, 2);
+        DMDX_PAIR_SHAPE(16, 4);
+        DMDX_PAIR_SHAPE(32, 2);
+        DMDX_PAIR_SHAPE(32, 4);
+    default: return hipErrorInvalidValue;
     }
+#undef DMDX_PAIR_SHAPE
 #undef DMDX_PAIR_ARGS
     return hipGetLastError();
 }

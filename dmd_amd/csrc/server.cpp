@@ -57,7 +57,7 @@ int64_t estimate_bytes(const dmdx_system_desc& D)
         const double expect = 4.0 / 3.0 * PI * rl * rl * rl * dens;
         int64_t M = (int64_t)std::ceil(1.6 * expect + 48.0);
         M = std::max<int64_t>(1, std::min<int64_t>(M, n - 1));
-        M = (M + 3) / 4 * 4;
+        M = (M + 7) / 8 * 8;
         b += M * n_pad * 4;
         int64_t cells = 1;
         for (int k = 0; k < 3; ++k)

@@ -58,6 +58,11 @@ def _setup(lib):
 
 
 KERNEL_GROUPS = ("pair", "list", "bonded", "pme", "gather", "integrate")
+# autotuner candidates in the order of dmdx_engine_autotune's times_ms:
+# mixed pair kernel (lanes per atom x list entries per lane pass), list build
+# (lanes per atom)
+PAIR_SHAPES = ("2x8", "4x8", "8x8", "4x4", "8x4", "16x4", "8x2", "16x2")
+BUILD_SHAPES = (8, 16, 32)
 
 
 def copy_bandwidth(nbytes: int = 1 << 28, repeats: int = 20) -> float:
@@ -167,19 +172,18 @@ class Engine:
             _abi.check(self.lib.dmdx_engine_set_tuning(self.h, 2, int(pair_width)))
 
     def autotune(self, steps: int = 5, cache: str | None = None) -> dict:
-        """Time the launch shapes of the mixed pair kernel (8/16/32 lanes x
-        2/4 entries per pass) and list build (8/16/32 lanes) on a scratch
+        """Time launch shapes of the mixed pair kernel (lanes per atom x
+        entries per lane pass) and list build (8/16/32 lanes) on a scratch
         copy of the systems and apply the fastest (dmdx_engine_autotune).
         `cache`: text file of earlier results."""
         p, w, b = C.c_int32(), C.c_int32(), C.c_int32()
-        t = (C.c_double * 9)()
+        t = (C.c_double * (len(PAIR_SHAPES) + len(BUILD_SHAPES)))()
         _abi.check(self.lib.dmdx_engine_autotune(
             self.h, int(steps), cache.encode() if cache else None, C.byref(p), C.byref(w),
             C.byref(b), t))
-        shapes = ("8x2", "8x4", "16x2", "16x4", "32x2", "32x4")
         return {"pair_lanes": p.value, "pair_width": w.value, "build_lanes": b.value,
-                "pair_ms": {k: t[i] for i, k in enumerate(shapes)},
-                "build_ms": {8: t[6], 16: t[7], 32: t[8]}}
+                "pair_ms": {k: t[i] for i, k in enumerate(PAIR_SHAPES)},
+                "build_ms": {b: t[len(PAIR_SHAPES) + i] for i, b in enumerate(BUILD_SHAPES)}}
 
     def set_profiling(self, on: bool):
         _abi.check(self.lib.dmdx_engine_set_profiling(self.h, int(on)))

@@ -311,25 +311,29 @@ int dmdx_engine_set_profiling(dmdx_engine *e, int32_t on);
 int dmdx_engine_kernel_time(dmdx_engine *e, int32_t group, double *total_ms, int64_t *launches);
 
 /* Launch shape of the mixed-precision kernels (no counterpart in the
- * reference): key 0 = lanes per atom of the pair kernel, key 1 = lanes per
- * atom of the list build (value 8, 16 or 32); key 2 = list entries per lane
- * per pass of the pair kernel (2 or 4).  Results do not depend on it beyond
- * FP32 summation order. */
+ * reference): key 0 = lanes per atom of the pair kernel (2, 4, 8, 16 or 32),
+ * key 1 = lanes per atom of the list build (8, 16 or 32), key 2 = list
+ * entries per lane per pass of the pair kernel (2, 4 or 8).  Pair shapes
+ * (lanes x entries): 2x8, 4x8, 8x8, 4x4, 8x4, 16x4, 32x4, 8x2, 16x2, 32x2;
+ * another combination runs as the nearest of these (8 lanes where fewer
+ * are not built for the width, at most 8 lanes with 8 entries).  Results do
+ * not depend on the shape beyond FP32 summation order. */
 int dmdx_engine_set_tuning(dmdx_engine *e, int32_t key, int32_t value);
 
-/* Autotuner (no counterpart in the reference): times the launch shapes of
- * the mixed pair kernel (8/16/32 lanes per atom x 2/4 list entries per lane
- * per pass) and of the list build (8/16/32 lanes per atom) on a scratch copy
- * of the engine's systems stepped from their initial state (the engine's
- * own state is untouched), `steps` steps per candidate, and applies the
- * fastest of each.  cache_path (may be NULL): text file of earlier results
- * keyed by a signature of the systems; a hit applies the cached choice
- * without timing.  times_ms (9 doubles, may be NULL): per-launch ms of the
- * pair kernel at 8x2, 8x4, 16x2, 16x4, 32x2, 32x4, then per-step ms of the
- * list build at 8, 16, 32 lanes (-1 when not timed).  FP64 engines have
+/* Autotuner (no counterpart in the reference): times launch shapes of the
+ * mixed pair kernel (lanes per atom x list entries per lane pass) and of the
+ * list build (8/16/32 lanes per atom) on a scratch copy of the engine's
+ * systems stepped from their initial state (the engine's own state is
+ * untouched), `steps` steps per candidate, and applies the fastest of each.
+ * cache_path (may be NULL): text file of earlier results keyed by a
+ * signature of the systems; a hit applies the cached choice without timing.
+ * times_ms (DMDX_TUNE_TIMES doubles, may be NULL): per-launch ms of the pair
+ * kernel at 2x8, 4x8, 8x8, 4x4, 8x4, 16x4, 8x2, 16x2, then per-step ms of
+ * the list build at 8, 16, 32 lanes (-1 when not timed).  FP64 engines have
  * nothing to tune.
  * Mixed-precision results depend on the shape only through FP32 summation
  * order (well inside the mixed-precision tolerance). */
+#define DMDX_TUNE_TIMES 11
 int dmdx_engine_autotune(dmdx_engine *e, int32_t steps, const char *cache_path,
                          int32_t *pair_lanes, int32_t *pair_width, int32_t *build_lanes,
                          double *times_ms);

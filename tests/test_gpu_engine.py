@@ -262,3 +262,31 @@ def test_graph_replay_matches_eager():
         a.sync()
         assert np.array_equal(a.positions(), b.positions())
         assert np.array_equal(a.velocities(), b.velocities())
+
+
+PAIR_SHAPES = [(2, 8), (4, 4), (4, 8), (8, 2), (8, 4), (8, 8), (16, 2), (16, 4), (32, 2), (32, 4)]
+
+
+@pytest.mark.parametrize("kind", ["argon", "water"])
+def test_mixed_every_pair_shape(kind):
+    """Every launch shape of the mixed pair kernel (lanes per atom x entries
+    per lane pass) meets the FP64 force bar, for the LJ-only and the
+    Coulomb variants; the shapes differ only in FP32 summation order."""
+    say(f"case: mixed pair kernel shapes, {kind}")
+    spec = (jittered_argon(skin=0.1).with_(physics=CO) if kind == "argon"
+            else W.water_box(n_side=10, seed=2))
+    spec = spec.with_(precision=_abi.MIXED)
+    F_ref, _ = O.Sim(spec.with_(precision=_abi.FP64)).forces_exact()
+    first = None
+    for lanes, width in PAIR_SHAPES:
+        e = engine(spec)
+        e.set_tuning(pair_lanes=lanes, pair_width=width)
+        e.compute()
+        F = e.forces()
+        err = rms_rel(F, F_ref)
+        say(f"  {lanes}x{width}: force rms rel err {err:.3e}")
+        assert err <= 1e-5, (lanes, width)
+        if first is None:
+            first = F
+        else:
+            assert rms_rel(F, first) <= 2e-6, (lanes, width)
