@@ -88,6 +88,10 @@ Batch::~Batch()
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (graph) (void)hipGraphDestroy(graph);
     for (void* p : owned.ptrs) (void)hipFree(p);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    if (ev_list) (void)hipEventDestroy(ev_list);
+    if (side) (void)hipStreamDestroy(side);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -96,6 +100,10 @@ int Batch::create(const dmdx_system_desc* descs, int nsys, const BatchOptions& o
     if (nsys <= 0) return fail(DMDX_E_INVALID, "no systems");
     options = opt;
     DMDX_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    DMDX_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+    DMDX_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+    DMDX_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+    DMDX_HIP(hipEventCreateWithFlags(&ev_list, hipEventDisableTiming));
     const int prec = descs[0].precision;
     for (int s = 0; s < nsys; ++s)
         if (descs[s].precision != prec)
@@ -893,11 +901,13 @@ bool Batch::mirror_compute(int phase, int force_rebuild, bool* any_force)
 static int g_acc = 0;
 static hipError_t launch_pair_acc(const Dev& d, hipStream_t s) { return launch_pair(d, s, g_acc); }
 
-// launch fn, bracketed by a pair of HIP events on the batch stream when
-// profiling is on (events are read back by kernel_time)
-hipError_t Batch::timed(int group, hipError_t (*fn)(const Dev&, hipStream_t))
+// launch fn on stream `on` (default: the batch stream), bracketed by a pair
+// of HIP events on that stream when profiling is on (events are read back by
+// kernel_time)
+hipError_t Batch::timed(int group, hipError_t (*fn)(const Dev&, hipStream_t), hipStream_t on)
 {
-    if (!options.profile) return fn(d, stream);
+    if (!on) on = stream;
+    if (!options.profile || !(options.profile_mask >> group & 1u)) return fn(d, on);
     hipEvent_t a, b;
     for (hipEvent_t* ev : {&a, &b}) {
         if (!ev_pool.empty()) {
@@ -908,11 +918,11 @@ hipError_t Batch::timed(int group, hipError_t (*fn)(const Dev&, hipStream_t))
             if (e != hipSuccess) return e;
         }
     }
-    hipError_t e = hipEventRecord(a, stream);
+    hipError_t e = hipEventRecord(a, on);
     if (e != hipSuccess) return e;
-    e = fn(d, stream);
+    e = fn(d, on);
     if (e != hipSuccess) return e;
-    e = hipEventRecord(b, stream);
+    e = hipEventRecord(b, on);
     ev_used.push_back({group, {a, b}});
     return e;
 }
@@ -949,12 +959,32 @@ int Batch::enqueue_compute(int phase, int accumulate, int force_rebuild, bool un
     if (d.any_auto && any_rebuild && !after_drift) LAUNCH(launch_displacement(d, stream));
     LAUNCH(launch_pre_compute(d, stream, phase, force_rebuild, options.reuse_forces));
     if (!any_force) return DMDX_OK;
+    // The bonded terms and PME read the positions and write their own term
+    // forces, grid and energy partials; the list build and the pair kernel
+    // write the lists, cells, frames and per-atom forces.  So the bonded
+    // terms run on the side stream while the list build and the pair kernel
+    // run here (a rebuild often leaves most of the device idle), PME follows
+    // them there once the list build has left the cell order its charge
+    // spreading reads, and the gather, which reads everything, waits for the
+    // side stream.
+    const bool bonded = d.nb || d.na || d.nd || d.ni || d.ne;
+    const bool fork = bonded || d.any_pme;
+    if (fork) {
+        DMDX_HIP(hipEventRecord(ev_fork, stream));
+        DMDX_HIP(hipStreamWaitEvent(side, ev_fork, 0));
+        if (bonded) LAUNCH(timed(KG_BONDED, launch_bonded, side));
+    }
     if (d.any_list && any_rebuild) LAUNCH(timed(KG_LIST, launch_build_lists));
+    if (d.any_pme) {
+        DMDX_HIP(hipEventRecord(ev_list, stream));
+        DMDX_HIP(hipStreamWaitEvent(side, ev_list, 0));
+        LAUNCH(timed(KG_PME, launch_pme, side));
+    }
+    if (fork) DMDX_HIP(hipEventRecord(ev_join, side));
     g_acc = accumulate;
     LAUNCH(launch_pack_phase(d, stream, phase == 2 ? 1 : 0));
     LAUNCH(timed(KG_PAIR, launch_pair_acc));
-    if (d.nb || d.na || d.nd || d.ni || d.ne) LAUNCH(timed(KG_BONDED, launch_bonded));
-    if (d.any_pme) LAUNCH(timed(KG_PME, launch_pme));
+    if (fork) DMDX_HIP(hipStreamWaitEvent(stream, ev_join, 0));
     if (d.any_gather) LAUNCH(timed(KG_GATHER, launch_gather));
     return DMDX_OK;
 }
@@ -1409,6 +1439,17 @@ int dmdx_engine_set_profiling(dmdx_engine* e, int32_t on)
     if (on && e->b.options.use_graph)
         return dmdx::fail(DMDX_E_INVALID, "kernel timing needs eager launches (use_graph = 0)");
     e->b.options.profile = on != 0;
+    e->b.options.profile_mask = ~0u;
+    return DMDX_OK;
+}
+
+int dmdx_engine_set_profiling_groups(dmdx_engine* e, uint32_t mask)
+{
+    if (!e) return dmdx::fail(DMDX_E_INVALID, "null engine");
+    if (mask && e->b.options.use_graph)
+        return dmdx::fail(DMDX_E_INVALID, "kernel timing needs eager launches (use_graph = 0)");
+    e->b.options.profile = mask != 0;
+    e->b.options.profile_mask = mask;
     return DMDX_OK;
 }
 

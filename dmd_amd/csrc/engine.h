@@ -30,7 +30,8 @@ struct BatchOptions {
     bool use_graph = false;    // replay captured blocks of steps
     int graph_steps = 10;
     int max_nbr = 0;           // 0: estimate from the density
-    bool profile = false;      // HIP events around each kernel group
+    bool profile = false;      // HIP events around kernel groups
+    uint32_t profile_mask = ~0u;  // which groups (bit = group) when profiling
 };
 
 // kernel groups timed when profiling is on
@@ -157,6 +158,10 @@ public:
     std::vector<SysState> hstate;
     std::vector<HostSys> host;
     hipStream_t stream = nullptr;
+    // second stream for the terms that do not read the pair list (bonded,
+    // PME): forked after the per-compute decisions, joined before the gather
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_list = nullptr, ev_join = nullptr;
     size_t device_bytes() const { return owned.bytes; }
 
 private:
@@ -165,7 +170,8 @@ private:
     int enqueue_compute(int phase, int accumulate, int force_rebuild, bool uniform);
     int enqueue_step(bool uniform);
 
-    hipError_t timed(int group, hipError_t (*fn)(const Dev&, hipStream_t));
+    hipError_t timed(int group, hipError_t (*fn)(const Dev&, hipStream_t),
+                     hipStream_t on = nullptr);
     DevMem owned;
     std::vector<hipEvent_t> ev_pool;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_used;

@@ -216,7 +216,7 @@ __device__ __forceinline__ void k_pair_mixed_unit(int64_t unit_,
         // entries at or past cnt never address memory.  LJ variant (less
         // arithmetic per entry, so occupancy matters more): lane l takes the
         // W consecutive entries W l .. W l + W - 1 with one or two vector
-        // loads (rows are 16-byte aligned, M a multiple of 8) and no prefetch.
+        // loads (rows are 16-byte aligned, M a multiple of 8).
         constexpr bool STRIDED = COUL;
         auto entry_index = [&](int k0, int q) {
             return STRIDED ? k0 + lane + q * TPA : k0 + lane * W + q;
@@ -253,13 +253,13 @@ __device__ __forceinline__ void k_pair_mixed_unit(int64_t unit_,
             }
         };
         // blocked lanes start at lane W, so their loop runs while that is
-        // below cnt (their vector load stays inside the row)
+        // below cnt, and they load a pass's entries only when it starts
+        // below cnt (their vector loads stay inside the row)
         const int kbase = STRIDED ? 0 : lane * W;
         uint32_t ev[W];
-        if constexpr (STRIDED) load_entries(0, ev);
+        if (STRIDED || kbase < cnt) load_entries(0, ev);
         for (int kp = kbase; kp < cnt; kp += STRIDE) {
             const int k0 = kp - kbase;
-            if constexpr (!STRIDED) load_entries(k0, ev);
             float4 pj[W];
             int tj[W];
 #pragma unroll
@@ -269,8 +269,9 @@ __device__ __forceinline__ void k_pair_mixed_unit(int64_t unit_,
                 pj[q] = lps[j];
                 tj[q] = TYPED ? tys[j] : 0;
             }
+            // the next pass's entries load while this pass gathers
             uint32_t en[W];
-            if constexpr (STRIDED) load_entries(k0 + STRIDE, en);
+            if (STRIDED || kp + STRIDE < cnt) load_entries(k0 + STRIDE, en);
             // the W entries of a pass are summed in FP32, then added in FP64
             float px = 0.0f, py = 0.0f, pz = 0.0f, pel = 0.0f, pec = 0.0f;
 #pragma unroll
@@ -324,10 +325,8 @@ __device__ __forceinline__ void k_pair_mixed_unit(int64_t unit_,
             fzi -= (double)pz;
             ej += (double)pel;
             if (COUL) ec += (double)pec;
-            if constexpr (STRIDED) {
 #pragma unroll
-                for (int q = 0; q < W; ++q) ev[q] = en[q];
-            }
+            for (int q = 0; q < W; ++q) ev[q] = en[q];
         }
     };
     using T_ = std::true_type;

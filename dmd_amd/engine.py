@@ -47,6 +47,7 @@ def _setup(lib):
                                           C.POINTER(C.c_int64)]
     lib.dmdx_engine_n_systems.argtypes = [C.c_void_p, C.POINTER(C.c_int32)]
     lib.dmdx_engine_set_profiling.argtypes = [C.c_void_p, C.c_int32]
+    lib.dmdx_engine_set_profiling_groups.argtypes = [C.c_void_p, C.c_uint32]
     lib.dmdx_engine_kernel_time.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_double),
                                             C.POINTER(C.c_int64)]
     lib.dmdx_copy_bandwidth.argtypes = [C.c_int64, C.c_int32, C.POINTER(C.c_double)]
@@ -185,8 +186,16 @@ class Engine:
                 "pair_ms": {k: t[i] for i, k in enumerate(PAIR_SHAPES)},
                 "build_ms": {b: t[len(PAIR_SHAPES) + i] for i, b in enumerate(BUILD_SHAPES)}}
 
-    def set_profiling(self, on: bool):
-        _abi.check(self.lib.dmdx_engine_set_profiling(self.h, int(on)))
+    def set_profiling(self, on: bool, groups=None):
+        """HIP events around every kernel group, or only around `groups`
+        (names from KERNEL_GROUPS)."""
+        if groups is None or not on:
+            _abi.check(self.lib.dmdx_engine_set_profiling(self.h, int(on)))
+        else:
+            mask = 0
+            for g in groups:
+                mask |= 1 << KERNEL_GROUPS.index(g)
+            _abi.check(self.lib.dmdx_engine_set_profiling_groups(self.h, mask))
 
     def kernel_times(self) -> dict:
         """{group: (total_ms, launches)} of the HIP-event-timed kernel groups."""

@@ -308,6 +308,9 @@ int dmdx_component_destroy(dmdx_component *c);
  * 3 PME, 4 force gather, 5 kick/drift.  dmdx_engine_kernel_time waits for
  * the stream and returns the accumulated time and launch count of a group. */
 int dmdx_engine_set_profiling(dmdx_engine *e, int32_t on);
+/* Bracket only the groups whose bit (1 << group) is set in mask (0: off);
+ * fewer events per step leave fewer gaps between the untimed kernels. */
+int dmdx_engine_set_profiling_groups(dmdx_engine *e, uint32_t mask);
 int dmdx_engine_kernel_time(dmdx_engine *e, int32_t group, double *total_ms, int64_t *launches);
 
 /* Launch shape of the mixed-precision kernels (no counterpart in the
