@@ -70,7 +70,8 @@ def make_workload(name, n_replicas, seed):
 
 def count_pairs(eng, specs):
     """In-cutoff unique pairs of every system (list pairs filtered by the
-    cutoffs at the current positions) -> algorithmic FLOP per pair launch;
+    cutoffs at the current positions) -> algorithmic FLOP of the pair
+    forces of one step;
     also the unique pairs of each system's list (within the list radius)."""
     n_lj = n_c = n_both = 0
     list_pairs = []
@@ -103,11 +104,12 @@ def percentile(v, p):
 def run_batched(specs, steps, warmup, latency_steps):
     """Timed region: `steps` steps of every replica enqueued back to back,
     bracketed by device synchronisations, with HIP events around the pair
-    kernel and the list build (the two roofline kernels) on the engine's
-    stream.  Then, outside the timed region: a pass of events around every
-    kernel group for the per-group breakdown (events between all groups add
-    gaps, so the timed region carries only the two), and `latency_steps`
-    single steps, each synchronised, for the per-step latency percentiles."""
+    kernel (the roofline kernel) on the engine's stream.  Then, outside the
+    timed region: a pass with events around every kernel group for the
+    per-group breakdown and the list-build roofline (events between all
+    groups add gaps, so the timed region carries only the pair kernel's),
+    and `latency_steps` single steps, each synchronised, for the per-step
+    latency percentiles."""
     from dmd_amd import _abi
     from dmd_amd.engine import Engine
     mixed = [s.with_(precision=_abi.MIXED) for s in specs]
@@ -116,24 +118,26 @@ def run_batched(specs, steps, warmup, latency_steps):
     log(f"[bench] autotune: {tune}")
     eng.step(warmup)
     eng.sync()
-    eng.set_profiling(True, groups=("pair", "list"))
-    builds0 = [eng.scalars(k)["list_builds"] for k in range(len(specs))]
+    eng.set_profiling(True, groups=("pair",))
     base = eng.kernel_times()
     t0 = time.perf_counter()
     eng.step(steps)
     eng.sync()
     wall = time.perf_counter() - t0
     kt = eng.kernel_times()
-    builds = [eng.scalars(k)["list_builds"] - builds0[k] for k in range(len(specs))]
     kt = {k: (kt[k][0] - base[k][0], kt[k][1] - base[k][1]) for k in kt}
-    # per-group breakdown, outside the timed region
+    # every group, outside the timed region: breakdown and list roofline
     nb = min(steps, 500)
     eng.set_profiling(True)
+    builds0 = [eng.scalars(k)["list_builds"] for k in range(len(specs))]
     base = eng.kernel_times()
     eng.step(nb)
     eng.sync()
     kb = eng.kernel_times()
-    breakdown = {k: round((kb[k][0] - base[k][0]) / nb, 4) for k in kb}
+    builds = [eng.scalars(k)["list_builds"] - builds0[k] for k in range(len(specs))]
+    kb = {k: (kb[k][0] - base[k][0], kb[k][1] - base[k][1]) for k in kb}
+    breakdown = {"steps": nb, "ms_per_step": {k: round(v[0] / nb, 4) for k, v in kb.items()},
+                 "list": kb["list"]}
     eng.set_profiling(False)
     lat = []
     for _ in range(latency_steps):
@@ -306,24 +310,30 @@ def main():
     log(f"[bench] batched mixed: {value:.4g} atom-steps/s, {ms_per_step:.3f} ms/step")
 
     # rooflines: the pair kernel (FP32 VALU) and the list build (HBM bytes
-    # of the lists it writes); "roofline" is the one whose group dominates
-    dom = "list" if kt["list"][0] > kt["pair"][0] else "pair"
-    flop_per_launch, n_pairs, list_pairs = count_pairs(eng, specs)
+    # of the lists it writes)
+    gms = breakdown["ms_per_step"]
+    dom = "list" if gms["list"] > gms["pair"] else "pair"
+    # achieved = FLOP of a step's pair forces / pair-kernel time per step
+    # (the durations of its launches in the step)
+    flop_per_step, n_pairs, list_pairs = count_pairs(eng, specs)
     pair_ms, pair_n = kt["pair"]
     pair_avg_s = pair_ms / max(pair_n, 1) * 1e-3
-    achieved = flop_per_launch / pair_avg_s / 1e12 if pair_n else 0.0
+    pair_step_s = pair_ms / args.steps * 1e-3
+    achieved = flop_per_step / pair_step_s / 1e12 if pair_n else 0.0
     roof_pair = {
         "kernel": "k_pair_mixed (LJ + real-space erfc pair forces)",
         "bound": "valu",
         "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
         "frac": round(achieved / PEAK_FP32_TFLOPS, 5),
         "traffic": load_traffic("k_pair_mixed", args.workload),
-        "algorithmic_flop_per_launch": flop_per_launch, "pairs_in_cutoff": n_pairs,
+        "algorithmic_flop_per_step": flop_per_step, "pairs_in_cutoff": n_pairs,
+        "ms_per_step": round(pair_step_s * 1e3, 4),
         "avg_launch_ms": round(pair_avg_s * 1e3, 4), "launches": pair_n,
     }
     # list build: per rebuilt system, 24 B read per atom and 4 B per row
-    # entry (2 per unique pair within the list radius), over the timed steps
-    list_ms, list_n = kt["list"]
+    # entry (2 per unique pair within the list radius), over the breakdown
+    # pass
+    list_ms, list_n = breakdown["list"]
     list_bytes = sum(b * (24 * s.n + 8 * p) for b, s, p in zip(builds, specs, list_pairs))
     list_avg_s = list_ms / max(list_n, 1) * 1e-3
     gbs = list_bytes / max(list_n, 1) / list_avg_s / 1e9 if list_n else 0.0
@@ -336,10 +346,13 @@ def main():
         "system_rebuilds": int(sum(builds)), "avg_launch_ms": round(list_avg_s * 1e3, 4),
         "launches": list_n,
     }
-    roofline = dict(roof_list if dom == "list" else roof_pair)
-    roofline["dominant_group"] = dom
-    roofline["group_ms_per_step"] = breakdown  # separate pass, events around every group
-    other = roof_pair if dom == "list" else roof_list
+    # the pair kernel is the kernel timed inside the timed region; the
+    # breakdown pass names the largest group beside it
+    roofline = dict(roof_pair)
+    roofline["largest_group"] = dom
+    # separate pass after the timed region, events around every group
+    roofline["group_ms_per_step"] = gms
+    roofline["group_pass_steps"] = breakdown["steps"]
     eng.close()
 
     out = {
@@ -353,7 +366,7 @@ def main():
         "latency_ms": {"p50": percentile(lat, 50) * 1e3 if lat else None,
                        "p99": percentile(lat, 99) * 1e3 if lat else None},
         "roofline": roofline,
-        ("roofline_pair" if dom == "list" else "roofline_list"): other,
+        "roofline_list": roof_list,
         "failed_systems": int(sum(1 for f in flags if f)),
         "autotune": tune,
     }

@@ -8,8 +8,8 @@
 // many neighbours an atom has, how many atoms a batch holds, how many rebuild
 // at once), so the choice is measured: a scratch copy of the engine's batch
 // is stepped with each candidate and the times of the two kernel groups are
-// read from HIP events -- per launch for the pair kernel, per step for the
-// list build, whose candidates take turns step by step so that each sees the
+// read from HIP events -- per step for the pair kernel and for the list
+// build, whose candidates take turns step by step so that each sees the
 // same mix of full and partial rebuilds.  The engine's own state is not
 // touched.  Results are cached in a text file keyed by a signature of the
 // systems, so a workload is timed once.  There is no counterpart in the
@@ -167,7 +167,7 @@ extern "C" int dmdx_engine_autotune(dmdx_engine* e, int32_t steps, const char* c
     const int n = steps > 0 ? steps : 5;
     double ms0, ms1;
     int64_t n0, n1;
-    // pair kernel: per-launch time of each shape over n steps
+    // pair kernel: per-step time of each shape over n steps
     for (int c = 0; c < NPAIR; ++c) {
         scratch.d.pair_tpa = pl[c];
         scratch.d.pair_width = pw[c];
@@ -176,7 +176,7 @@ extern "C" int dmdx_engine_autotune(dmdx_engine* e, int32_t steps, const char* c
         if (!rc) rc = scratch.step(n);
         if (!rc) rc = scratch.kernel_time(dmdx::KG_PAIR, &ms1, &n1);
         if (rc) return rc;
-        t[c] = (ms1 - ms0) / (double)std::max<int64_t>(1, n1 - n0);
+        t[c] = (ms1 - ms0) / (double)n;
     }
     // list build: the candidates take turns step by step, so each sees the
     // same mix of full and partial rebuilds; time per step of the list group

@@ -98,6 +98,31 @@ __global__ void k_pre_rebuild_only(const SysDev* __restrict__ sys, SysState* __r
     }
 }
 
+// F += tf[contrib[c]] for c = c0, c0 + 1, ..., c1 - 1, in that order: eight
+// indices, then their eight term forces, are loaded before any is added
+__device__ __forceinline__ void add_terms(const int* __restrict__ contrib,
+                                          const TermF* __restrict__ tf, int c0, int c1,
+                                          double& Fx, double& Fy, double& Fz)
+{
+    for (int c = c0; c < c1; c += 8) {
+        const int n = c1 - c;
+        int idx[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) idx[u] = u < n ? contrib[c + u] : 0;
+        TermF v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (u < n) v[u] = tf[idx[u]];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (u < n) {
+                Fx += v[u].x;
+                Fy += v[u].y;
+                Fz += v[u].z;
+            }
+    }
+}
+
 // Per-atom gather.  cstart[g] .. cmid[g]: bonded contributions (bonds,
 // angles, dihedrals, impropers in term order); cmid[g] .. cstart[g+1]:
 // exclusion contributions.  PME sits between them, as in build_force_engine
@@ -122,12 +147,7 @@ __device__ __forceinline__ void k_gather_unit(int64_t unit_, const SysDev* __res
     double qi = S.pme ? q[g] : 0.0;
     if (c0 == c2 && qi == 0.0) return;
     double Fx = fx[g], Fy = fy[g], Fz = fz[g];
-    for (int c = c0; c < c1; ++c) {
-        TermF v = tf[contrib[c]];
-        Fx += v.x;
-        Fy += v.y;
-        Fz += v.z;
-    }
+    add_terms(contrib, tf, c0, c1, Fx, Fy, Fz);
     if (qi != 0.0) {
         const int nx = S.K[0], ny = S.K[1], nz = S.K[2];
         const double Lx = S.L[0], Ly = S.L[1], Lz = S.L[2];
@@ -196,12 +216,7 @@ __device__ __forceinline__ void k_gather_unit(int64_t unit_, const SysDev* __res
         Fy -= qi * fyf * (double)sy;
         Fz -= qi * fzf * (double)sz;
     }
-    for (int c = c1; c < c2; ++c) {
-        TermF v = tf[contrib[c]];
-        Fx += v.x;
-        Fy += v.y;
-        Fz += v.z;
-    }
+    add_terms(contrib, tf, c1, c2, Fx, Fy, Fz);
     fx[g] = Fx;
     fy[g] = Fy;
     fz[g] = Fz;
@@ -388,19 +403,39 @@ __device__ __forceinline__ double temperature_of(double ekin, int n)
     return 2.0 * ekin / (dof * KB);
 }
 
-// one 256-thread workgroup per system, after the closing kick
-__global__ void __launch_bounds__(256) k_finalize(const SysDev* __restrict__ sys,
-                                                  SysState* __restrict__ st, Parts P,
-                                                  int with_kinetic)
+// a + part[off + k] + part[off + k + BS] + ... (elements below n), in that
+// order, loading eight at a time (guarded) before adding them
+template <int BS>
+__device__ __forceinline__ double strided_sum_batched(const double* __restrict__ part,
+                                                      int64_t off, int64_t n, double a, int64_t k)
 {
-    __shared__ double red[9][4];
+    for (; k < n; k += 8 * BS) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = k + u * BS < n ? part[off + k + u * BS] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (k + u * BS < n) a += v[u];
+    }
+    return a;
+}
+
+// one FIN_THREADS-thread workgroup per system, after the closing kick
+constexpr int FIN_THREADS = 512;
+constexpr int FIN_WAVES = FIN_THREADS / 64;
+
+__global__ void __launch_bounds__(FIN_THREADS) k_finalize(const SysDev* __restrict__ sys,
+                                                          SysState* __restrict__ st, Parts P,
+                                                          int with_kinetic)
+{
+    __shared__ double red[9][FIN_WAVES];
     int s = blockIdx.x;
     SysState& T = st[s];
     if (!T.active) return;
     const SysDev& S = sys[s];
     // the nine sums (8 energy terms and the kinetic energy) in one pass: each
-    // thread's strided sums, then a fixed tree over waves and over the 4
-    // waves -- deterministic, one barrier
+    // thread's strided sums, then a fixed tree over each wave and a fixed
+    // pairwise tree over the FIN_WAVES waves -- deterministic, one barrier
     // pair energies are per atom slot, the others per wave or workgroup
     const double* arr[9] = {P.lj, P.coul, P.bond, P.angle, P.dih, P.imp, P.pme, P.excl, P.ke};
     const int64_t off[9] = {S.atom_off, S.atom_off, S.bw_off, S.gw_off, S.dw_off,
@@ -414,18 +449,32 @@ __global__ void __launch_bounds__(256) k_finalize(const SysDev* __restrict__ sys
 #pragma unroll
     for (int k = 0; k < 9; ++k) a[k] = (t0 < cnt[k]) ? arr[k][off[k] + t0] : 0.0;
 #pragma unroll
-    for (int k = 0; k < 9; ++k) a[k] = strided_sum_from(arr[k], off[k], cnt[k], 0.0 + a[k], t0 + 256);
+    for (int k = 0; k < 9; ++k)
+        a[k] = strided_sum_batched<FIN_THREADS>(arr[k], off[k], cnt[k], 0.0 + a[k],
+                                                t0 + FIN_THREADS);
 #pragma unroll
     for (int k = 0; k < 9; ++k) a[k] = wave_sum(a[k]);
     if (lane_id() == 0)
 #pragma unroll
         for (int k = 0; k < 9; ++k) red[k][threadIdx.x >> 6] = a[k];
     __syncthreads();
+    if (threadIdx.x != 0) return;
+    double r[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        double w[FIN_WAVES];
+#pragma unroll
+        for (int i = 0; i < FIN_WAVES; ++i) w[i] = red[k][i];
+#pragma unroll
+        for (int h = FIN_WAVES / 2; h > 0; h /= 2)
+#pragma unroll
+            for (int i = 0; i < h; ++i) w[i] = w[2 * i] + w[2 * i + 1];
+        r[k] = w[0];
+    }
     double e[8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) e[k] = (red[k][0] + red[k][1]) + (red[k][2] + red[k][3]);
-    double ke = (red[8][0] + red[8][1]) + (red[8][2] + red[8][3]);
-    if (threadIdx.x != 0) return;
+    for (int k = 0; k < 8; ++k) e[k] = r[k];
+    double ke = r[8];
     if (S.pme) {
         e[6] += S.self_energy;
         if (S.physics != DMDX_FAITHFUL && S.qsum != 0.0) {
@@ -671,7 +720,7 @@ hipError_t launch_kick_ke(const Dev& d, hipStream_t s)
 hipError_t launch_finalize(const Dev& d, hipStream_t s, int with_kinetic)
 {
     Parts P{d.p_lj, d.p_coul, d.p_bond, d.p_angle, d.p_dih, d.p_imp, d.p_pme, d.p_excl, d.p_ke};
-    k_finalize<<<d.nsys, 256, 0, s>>>(d.sys, d.st, P, with_kinetic);
+    k_finalize<<<d.nsys, FIN_THREADS, 0, s>>>(d.sys, d.st, P, with_kinetic);
     return hipGetLastError();
 }
 

@@ -330,7 +330,7 @@ int dmdx_engine_set_tuning(dmdx_engine *e, int32_t key, int32_t value);
  * untouched), `steps` steps per candidate, and applies the fastest of each.
  * cache_path (may be NULL): text file of earlier results keyed by a
  * signature of the systems; a hit applies the cached choice without timing.
- * times_ms (DMDX_TUNE_TIMES doubles, may be NULL): per-launch ms of the pair
+ * times_ms (DMDX_TUNE_TIMES doubles, may be NULL): per-step ms of the pair
  * kernel at 2x8, 4x8, 8x8, 4x4, 8x4, 16x4, 8x2, 16x2, then per-step ms of
  * the list build at 8, 16, 32 lanes (-1 when not timed).  FP64 engines have
  * nothing to tune.
