@@ -203,6 +203,10 @@ struct Dev {
     double *x = nullptr, *y = nullptr, *z = nullptr;
     double *vx = nullptr, *vy = nullptr, *vz = nullptr;
     double *fx = nullptr, *fy = nullptr, *fz = nullptr;
+    // mixed path with gathered terms: per-slot sum of the bonded, PME and
+    // exclusion forces of a step's closing compute, gathered on the side
+    // stream and added to the pair forces by the closing kick
+    double *fbx = nullptr, *fby = nullptr, *fbz = nullptr;
     double *invm = nullptr, *m = nullptr, *q = nullptr;
     int* type = nullptr;
     // lists
@@ -302,6 +306,7 @@ hipError_t launch_pre_compute(const Dev& d, hipStream_t s, int phase, int force_
 hipError_t launch_displacement(const Dev& d, hipStream_t s);
 hipError_t launch_pre_rebuild_only(const Dev& d, hipStream_t s);
 hipError_t launch_gather(const Dev& d, hipStream_t s);
+hipError_t launch_gather_terms(const Dev& d, hipStream_t s);  // into fb (mixed)
 hipError_t launch_kick_drift(const Dev& d, hipStream_t s);
 hipError_t launch_kick_ke(const Dev& d, hipStream_t s);
 hipError_t launch_finalize(const Dev& d, hipStream_t s, int with_kinetic);

@@ -700,6 +700,11 @@ int Batch::create(const dmdx_system_desc* descs, int nsys, const BatchOptions& o
     TRY(dev_alloc_zero(&d.fx, slot, owned));
     TRY(dev_alloc_zero(&d.fy, slot, owned));
     TRY(dev_alloc_zero(&d.fz, slot, owned));
+    if (d.precision != DMDX_FP64 && d.any_gather) {
+        TRY(dev_alloc_zero(&d.fbx, slot, owned));
+        TRY(dev_alloc_zero(&d.fby, slot, owned));
+        TRY(dev_alloc_zero(&d.fbz, slot, owned));
+    }
     TRY(dev_upload(&d.invm, invm, owned));
     TRY(dev_upload(&d.m, m, owned));
     TRY(dev_upload(&d.q, q, owned));
@@ -966,9 +971,12 @@ int Batch::enqueue_compute(int phase, int accumulate, int force_rebuild, bool un
     // run here (a rebuild often leaves most of the device idle), PME follows
     // them there once the list build has left the cell order its charge
     // spreading reads, and the gather, which reads everything, waits for the
-    // side stream.
+    // side stream.  On the mixed path a step's closing compute gathers the
+    // term forces on the side stream too, into their own arrays, and the
+    // closing kick adds them to the pair forces.
     const bool bonded = d.nb || d.na || d.nd || d.ni || d.ne;
     const bool fork = bonded || d.any_pme;
+    const bool defer = phase == 2 && d.fbx != nullptr;
     if (fork) {
         DMDX_HIP(hipEventRecord(ev_fork, stream));
         DMDX_HIP(hipStreamWaitEvent(side, ev_fork, 0));
@@ -980,12 +988,13 @@ int Batch::enqueue_compute(int phase, int accumulate, int force_rebuild, bool un
         DMDX_HIP(hipStreamWaitEvent(side, ev_list, 0));
         LAUNCH(timed(KG_PME, launch_pme, side));
     }
+    if (defer) LAUNCH(timed(KG_GATHER, launch_gather_terms, side));
     if (fork) DMDX_HIP(hipEventRecord(ev_join, side));
     g_acc = accumulate;
     LAUNCH(launch_pack_phase(d, stream, phase == 2 ? 1 : 0));
     LAUNCH(timed(KG_PAIR, launch_pair_acc));
     if (fork) DMDX_HIP(hipStreamWaitEvent(stream, ev_join, 0));
-    if (d.any_gather) LAUNCH(timed(KG_GATHER, launch_gather));
+    if (d.any_gather && !defer) LAUNCH(timed(KG_GATHER, launch_gather));
     return DMDX_OK;
 }
 

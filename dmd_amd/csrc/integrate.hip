@@ -123,11 +123,21 @@ __device__ __forceinline__ void add_terms(const int* __restrict__ contrib,
     }
 }
 
+// the system has bonded, exclusion or PME forces to gather
+__device__ __forceinline__ bool system_gathers(const SysDev& S)
+{
+    return (S.bw_n | S.gw_n | S.dw_n | S.iw_n | S.xw_n | S.pme) != 0;
+}
+
 // Per-atom gather.  cstart[g] .. cmid[g]: bonded contributions (bonds,
 // angles, dihedrals, impropers in term order); cmid[g] .. cstart[g+1]:
 // exclusion contributions.  PME sits between them, as in build_force_engine
-// (src/sim/simulation_engine.cpp:147-203).
-template <class R>
+// (src/sim/simulation_engine.cpp:147-203).  TERMS = false: added to the
+// pair forces in fx, fy, fz.  TERMS = true (mixed path, on the side stream
+// while the pair kernel runs): their sum from zero is written to fx, fy, fz
+// (the term-force arrays) for every atom of a system that has such terms;
+// the closing kick adds it to the pair forces.
+template <class R, bool TERMS>
 __device__ __forceinline__ void k_gather_unit(int64_t unit_, const SysDev* __restrict__ sys, const SysState* __restrict__ st,
                          const int* __restrict__ slot_sys, int64_t nslots,
                          const int* __restrict__ cstart, const int* __restrict__ cmid,
@@ -143,10 +153,11 @@ __device__ __forceinline__ void k_gather_unit(int64_t unit_, const SysDev* __res
     if (s < 0 || st[s].reuse || !st[s].active) return;
     const SysDev& S = sys[s];
     if (g - S.atom_off >= S.n) return;
+    if (TERMS && !system_gathers(S)) return;
     int c0 = cstart[g], c1 = cmid[g], c2 = cstart[g + 1];
     double qi = S.pme ? q[g] : 0.0;
-    if (c0 == c2 && qi == 0.0) return;
-    double Fx = fx[g], Fy = fy[g], Fz = fz[g];
+    if (!TERMS && c0 == c2 && qi == 0.0) return;
+    double Fx = TERMS ? 0.0 : fx[g], Fy = TERMS ? 0.0 : fy[g], Fz = TERMS ? 0.0 : fz[g];
     add_terms(contrib, tf, c0, c1, Fx, Fy, Fz);
     if (qi != 0.0) {
         const int nx = S.K[0], ny = S.K[1], nz = S.K[2];
@@ -222,7 +233,7 @@ __device__ __forceinline__ void k_gather_unit(int64_t unit_, const SysDev* __res
     fz[g] = Fz;
 }
 
-template <class R>
+template <class R, bool TERMS>
 __global__ void k_gather(const SysDev* __restrict__ sys, const SysState* __restrict__ st,
                          const int* __restrict__ slot_sys, int64_t nslots,
                          const int* __restrict__ cstart, const int* __restrict__ cmid,
@@ -232,7 +243,7 @@ __global__ void k_gather(const SysDev* __restrict__ sys, const SysState* __restr
                          const Cx2<R>* __restrict__ grid, double* __restrict__ fx,
                          double* __restrict__ fy, double* __restrict__ fz, int64_t nunits_)
 {
-    DMDX_FOR_UNITS(u_, nunits_) k_gather_unit<R>(u_, sys, st, slot_sys, nslots, cstart, cmid, contrib, tf, x, y, z, q, grid, fx, fy, fz);
+    DMDX_FOR_UNITS(u_, nunits_) k_gather_unit<R, TERMS>(u_, sys, st, slot_sys, nslots, cstart, cmid, contrib, tf, x, y, z, q, grid, fx, fy, fz);
 }
 
 
@@ -310,13 +321,17 @@ __global__ void k_kick_drift(const SysDev* __restrict__ sys, const SysState* __r
     DMDX_FOR_UNITS(u_, nunits_) k_kick_drift_unit(u_, sys, st, slot_sys, nslots, invm, fx, fy, fz, vx, vy, vz, x, y, z, o);
 }
 
+// Closing kick.  Mixed path with term forces gathered beside the pair
+// kernel (fbx non-null): a system with such terms first completes its forces,
+// F = pair force + term forces, and stores them for the next step's kick.
 __device__ __forceinline__ void k_kick_ke_unit(int64_t unit_, const SysDev* __restrict__ sys, const SysState* __restrict__ st,
                           const int* __restrict__ slot_sys, int64_t nslots,
                           const double* __restrict__ invm, const double* __restrict__ m,
-                          const double* __restrict__ fx, const double* __restrict__ fy,
-                          const double* __restrict__ fz, double* __restrict__ vx,
-                          double* __restrict__ vy, double* __restrict__ vz,
-                          double* __restrict__ ke_part)
+                          double* __restrict__ fx, double* __restrict__ fy,
+                          double* __restrict__ fz, const double* __restrict__ fbx,
+                          const double* __restrict__ fby, const double* __restrict__ fbz,
+                          double* __restrict__ vx, double* __restrict__ vy,
+                          double* __restrict__ vz, double* __restrict__ ke_part)
 {
     int64_t g = unit_ * blockDim.x + threadIdx.x;
     if (g >= nslots) return;
@@ -327,9 +342,18 @@ __device__ __forceinline__ void k_kick_ke_unit(int64_t unit_, const SysDev* __re
     if (g - S.atom_off < S.n) {
         const double half_dt = 0.5 * S.dt;
         const double im = invm[g];
-        double a = vx[g] + fx[g] * im * half_dt;
-        double b = vy[g] + fy[g] * im * half_dt;
-        double c = vz[g] + fz[g] * im * half_dt;
+        double Fx = fx[g], Fy = fy[g], Fz = fz[g];
+        if (fbx && system_gathers(S)) {
+            Fx += fbx[g];
+            Fy += fby[g];
+            Fz += fbz[g];
+            fx[g] = Fx;
+            fy[g] = Fy;
+            fz[g] = Fz;
+        }
+        double a = vx[g] + Fx * im * half_dt;
+        double b = vy[g] + Fy * im * half_dt;
+        double c = vz[g] + Fz * im * half_dt;
         vx[g] = a;
         vy[g] = b;
         vz[g] = c;
@@ -342,12 +366,13 @@ __device__ __forceinline__ void k_kick_ke_unit(int64_t unit_, const SysDev* __re
 __global__ void k_kick_ke(const SysDev* __restrict__ sys, const SysState* __restrict__ st,
                           const int* __restrict__ slot_sys, int64_t nslots,
                           const double* __restrict__ invm, const double* __restrict__ m,
-                          const double* __restrict__ fx, const double* __restrict__ fy,
-                          const double* __restrict__ fz, double* __restrict__ vx,
-                          double* __restrict__ vy, double* __restrict__ vz,
-                          double* __restrict__ ke_part, int64_t nunits_)
+                          double* __restrict__ fx, double* __restrict__ fy,
+                          double* __restrict__ fz, const double* __restrict__ fbx,
+                          const double* __restrict__ fby, const double* __restrict__ fbz,
+                          double* __restrict__ vx, double* __restrict__ vy,
+                          double* __restrict__ vz, double* __restrict__ ke_part, int64_t nunits_)
 {
-    DMDX_FOR_UNITS(u_, nunits_) k_kick_ke_unit(u_, sys, st, slot_sys, nslots, invm, m, fx, fy, fz, vx, vy, vz, ke_part);
+    DMDX_FOR_UNITS(u_, nunits_) k_kick_ke_unit(u_, sys, st, slot_sys, nslots, invm, m, fx, fy, fz, fbx, fby, fbz, vx, vy, vz, ke_part);
 }
 
 __device__ __forceinline__ double strided_sum_from(const double* __restrict__ part, int64_t off,
@@ -686,13 +711,24 @@ hipError_t launch_gather(const Dev& d, hipStream_t s)
     if (!d.any_gather) return hipSuccess;
     const int BS = 256;
     if (d.precision == DMDX_FP64)
-        k_gather<double><<<grid_for(blocks_for(d.nslots, BS)), BS, 0, s>>>(
+        k_gather<double, false><<<grid_for(blocks_for(d.nslots, BS)), BS, 0, s>>>(
             d.sys, d.st, d.slot_sys, d.nslots, d.c_start, d.c_mid, d.contrib, d.tf, d.x, d.y, d.z,
             d.q, reinterpret_cast<const Cx2<double>*>(d.grid), d.fx, d.fy, d.fz, (int64_t)(blocks_for(d.nslots, BS)));
     else
-        k_gather<float><<<grid_for(blocks_for(d.nslots, BS)), BS, 0, s>>>(
+        k_gather<float, false><<<grid_for(blocks_for(d.nslots, BS)), BS, 0, s>>>(
             d.sys, d.st, d.slot_sys, d.nslots, d.c_start, d.c_mid, d.contrib, d.tf, d.x, d.y, d.z,
             d.q, reinterpret_cast<const Cx2<float>*>(d.grid), d.fx, d.fy, d.fz, (int64_t)(blocks_for(d.nslots, BS)));
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_terms(const Dev& d, hipStream_t s)
+{
+    if (!d.fbx) return hipSuccess;
+    const int BS = 256;
+    k_gather<float, true><<<grid_for(blocks_for(d.nslots, BS)), BS, 0, s>>>(
+        d.sys, d.st, d.slot_sys, d.nslots, d.c_start, d.c_mid, d.contrib, d.tf, d.x, d.y, d.z,
+        d.q, reinterpret_cast<const Cx2<float>*>(d.grid), d.fbx, d.fby, d.fbz,
+        (int64_t)(blocks_for(d.nslots, BS)));
     return hipGetLastError();
 }
 
@@ -712,8 +748,9 @@ hipError_t launch_kick_ke(const Dev& d, hipStream_t s)
 {
     const int BS = 256;
     k_kick_ke<<<grid_for(blocks_for(d.nslots, BS)), BS, 0, s>>>(d.sys, d.st, d.slot_sys, d.nslots, d.invm,
-                                                      d.m, d.fx, d.fy, d.fz, d.vx, d.vy, d.vz,
-                                                      d.p_ke, (int64_t)(blocks_for(d.nslots, BS)));
+                                                      d.m, d.fx, d.fy, d.fz, d.fbx, d.fby, d.fbz,
+                                                      d.vx, d.vy, d.vz, d.p_ke,
+                                                      (int64_t)(blocks_for(d.nslots, BS)));
     return hipGetLastError();
 }
 
