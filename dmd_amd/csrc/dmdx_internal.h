@@ -265,6 +265,7 @@ struct Dev {
     // (rbm_*) and those binned one workgroup per system (rbw_*)
     int* rb_sys = nullptr;       // nsys
     int64_t* rb_pre = nullptr;   // nsys + 1: running slot totals
+    int64_t* rb_cpre = nullptr;  // nsys + 1: running allocated-cell totals
     int* rb_n = nullptr;         // 1
     int* rbm_sys = nullptr;
     int64_t* rbm_pre = nullptr;

@@ -717,6 +717,7 @@ int Batch::create(const dmdx_system_desc* descs, int nsys, const BatchOptions& o
     TRY(dev_upload(&d.cell_sys, cell_sys, owned));
     TRY(dev_alloc_zero(&d.rb_sys, (size_t)nsys, owned));
     TRY(dev_alloc_zero(&d.rb_pre, (size_t)nsys + 1, owned));
+    TRY(dev_alloc_zero(&d.rb_cpre, (size_t)nsys + 1, owned));
     TRY(dev_alloc_zero(&d.rb_n, 1, owned));
     TRY(dev_alloc_zero(&d.rbm_sys, (size_t)nsys, owned));
     TRY(dev_alloc_zero(&d.rbm_pre, (size_t)nsys + 1, owned));

@@ -10,11 +10,13 @@
 //
 // Kernels (all systems of a batch in one launch; systems whose st.rebuild is
 // 0 return at once):
-//   k_cells_reset  zero per-cell counters
-//   k_cell_count   cell of each atom from its wrapped position, count
-//   k_cell_scan    per-system exclusive scan of the counts
-//   k_cell_fill    scatter atoms into cell order
-//   k_cell_sort    sort each cell by slot (deterministic order)
+//   k_rb_list      worklists of the rebuilding systems (slots, cells)
+//   k_bin_system   one workgroup bins one system (counts, scan, fill in LDS)
+//   k_cells_reset  the multi-kernel binning of larger systems: zero per-cell
+//   k_cell_count   counters; cell of each atom from its wrapped position,
+//   k_cell_scan    count; per-system exclusive scan of the counts; scatter
+//   k_cell_fill    atoms into cell order
+//   k_cell_sort    sort each cell by slot (deterministic order), both paths
 //   k_nbr_build    per atom (cell order): scan the neighbour cells
 //   k_nbr_sort     per atom: ascending neighbour order (FP64 path: makes the
 //                  per-atom force sum follow the reference's summation order)
@@ -67,14 +69,14 @@ __device__ __forceinline__ int rb_find(const int64_t* __restrict__ pre, int n, i
     return lo;
 }
 
-// one 1024-thread workgroup: block-wide scans over the systems.  Three
-// lists: every rebuilding system (slots, for the list build), those binned by
-// the multi-kernel path (slots and cells), and those binned by k_bin_system
-// (one workgroup each).
+// one 1024-thread workgroup: scans over the systems (per wave with lane
+// shuffles, then over the 16 wave totals).  Three lists: every rebuilding
+// system (slots and cells), those binned by the multi-kernel path (slots and
+// cells), and those binned by k_bin_system (one workgroup each).
 struct RbOut {
     int* sys;
     int64_t* pre;
-    int64_t* cpre;  // may be null
+    int64_t* cpre;
     int* n;
 };
 
@@ -82,72 +84,66 @@ __global__ void __launch_bounds__(1024) k_rb_list(const SysDev* __restrict__ sys
                                                   const SysState* __restrict__ st, int nsys,
                                                   RbOut all, RbOut multi, RbOut wg)
 {
-    __shared__ int sf[3][1024];
-    __shared__ int64_t sp[2][1024], sc[1024];
-    __shared__ int base_n[3];
-    __shared__ int64_t base_p[2], base_c;
-    if (threadIdx.x < 3) base_n[threadIdx.x] = 0;
-    if (threadIdx.x < 2) base_p[threadIdx.x] = 0;
-    if (threadIdx.x == 0) base_c = 0;
+    constexpr int NW = 1024 / 64, NQ = 7;
+    // quantities: 0-2 list lengths (all, multi, wg), 3 slots of all, 4 slots
+    // of multi, 5 cells of multi, 6 cells of all
+    __shared__ int64_t wtot[NQ][NW];
+    __shared__ int64_t base[NQ];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    if (t < NQ) base[t] = 0;
     __syncthreads();
-    const int t = threadIdx.x;
     for (int s0 = 0; s0 < nsys; s0 += 1024) {
         const int s = s0 + t;
         const bool rb = s < nsys && st[s].rebuild;
         const bool w = rb && sys[s].bin_wg;
-        const int f[3] = {rb ? 1 : 0, (rb && !w) ? 1 : 0, w ? 1 : 0};
         const int64_t np = rb ? sys[s].n_pad : 0;
-        const int64_t nc = (rb && !w) ? (int64_t)sys[s].nc[0] * sys[s].nc[1] * sys[s].nc[2] : 0;
-        const int64_t npm = f[1] ? np : 0;
-        for (int k = 0; k < 3; ++k) sf[k][t] = f[k];
-        sp[0][t] = np;
-        sp[1][t] = npm;
-        sc[t] = nc;
-        __syncthreads();
-        for (int o = 1; o < 1024; o <<= 1) {  // inclusive scans
-            int a[3] = {0, 0, 0};
-            int64_t b0 = 0, b1 = 0, c = 0;
-            if (t >= o) {
-                for (int k = 0; k < 3; ++k) a[k] = sf[k][t - o];
-                b0 = sp[0][t - o];
-                b1 = sp[1][t - o];
-                c = sc[t - o];
+        const int64_t nc = rb ? (int64_t)sys[s].nc[0] * sys[s].nc[1] * sys[s].nc[2] : 0;
+        const bool m = rb && !w;
+        const int64_t v[NQ] = {rb ? 1 : 0, m ? 1 : 0, w ? 1 : 0, np, m ? np : 0, m ? nc : 0, nc};
+        int64_t inc[NQ];
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) {
+            inc[k] = v[k];
+            for (int o = 1; o < 64; o <<= 1) {
+                const int64_t y = __shfl_up(inc[k], o, 64);
+                if (lane >= o) inc[k] += y;
             }
-            __syncthreads();
-            for (int k = 0; k < 3; ++k) sf[k][t] += a[k];
-            sp[0][t] += b0;
-            sp[1][t] += b1;
-            sc[t] += c;
-            __syncthreads();
         }
-        if (f[0]) {
-            const int k = base_n[0] + sf[0][t] - 1;
-            all.sys[k] = s;
-            all.pre[k] = base_p[0] + sp[0][t] - np;
-        }
-        if (f[1]) {
-            const int k = base_n[1] + sf[1][t] - 1;
-            multi.sys[k] = s;
-            multi.pre[k] = base_p[1] + sp[1][t] - npm;
-            multi.cpre[k] = base_c + sc[t] - nc;
-        }
-        if (f[2]) wg.sys[base_n[2] + sf[2][t] - 1] = s;
+        if (lane == 63)
+#pragma unroll
+            for (int k = 0; k < NQ; ++k) wtot[k][wave] = inc[k];
         __syncthreads();
-        if (t == 1023) {
-            for (int k = 0; k < 3; ++k) base_n[k] += sf[k][1023];
-            base_p[0] += sp[0][1023];
-            base_p[1] += sp[1][1023];
-            base_c += sc[1023];
+        int64_t ex[NQ];
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) {
+            int64_t b = base[k];
+            for (int q = 0; q < wave; ++q) b += wtot[k][q];
+            ex[k] = b + inc[k] - v[k];
         }
+        if (v[0]) {
+            all.sys[ex[0]] = s;
+            all.pre[ex[0]] = ex[3];
+            all.cpre[ex[0]] = ex[6];
+        }
+        if (v[1]) {
+            multi.sys[ex[1]] = s;
+            multi.pre[ex[1]] = ex[4];
+            multi.cpre[ex[1]] = ex[5];
+        }
+        if (v[2]) wg.sys[ex[2]] = s;
+        __syncthreads();
+        if (t < NQ)
+            for (int q = 0; q < NW; ++q) base[t] += wtot[t][q];
         __syncthreads();
     }
     if (t == 0) {
-        all.n[0] = base_n[0];
-        all.pre[base_n[0]] = base_p[0];
-        multi.n[0] = base_n[1];
-        multi.pre[base_n[1]] = base_p[1];
-        multi.cpre[base_n[1]] = base_c;
-        wg.n[0] = base_n[2];
+        all.n[0] = (int)base[0];
+        all.pre[base[0]] = base[3];
+        all.cpre[base[0]] = base[6];
+        multi.n[0] = (int)base[1];
+        multi.pre[base[1]] = base[4];
+        multi.cpre[base[1]] = base[5];
+        wg.n[0] = (int)base[2];
     }
 }
 
@@ -171,6 +167,32 @@ __device__ __forceinline__ int rb_cell(const RbList& R, const SysDev* __restrict
 
 // loop of a fixed grid over the compact units [0, U), U read on the device
 #define DMDX_FOR_RB_UNITS(u, U) for (int64_t u = blockIdx.x; u < (int64_t)(U); u += gridDim.x)
+
+// A worklist's length and running totals (slots, or cells when `cells`)
+// staged in LDS by one coalesced read per workgroup when the list is short,
+// so the binary search of every lookup reads LDS instead of a chain of
+// dependent global loads.  Every thread of the workgroup calls it, before
+// the unit loop (it synchronises the workgroup).
+constexpr int RB_LDS = 1024;
+struct RbStage {
+    int64_t tot[RB_LDS + 1];
+    int n;
+};
+
+__device__ __forceinline__ RbList rb_stage(RbList R, bool cells, RbStage& L)
+{
+    const int n = *R.n;
+    if (n <= RB_LDS) {  // uniform over the workgroup
+        const int64_t* src = cells ? R.cpre : R.pre;
+        for (int i = threadIdx.x; i <= n; i += blockDim.x) L.tot[i] = src[i];
+        if (threadIdx.x == 0) L.n = n;
+        __syncthreads();
+        if (cells) R.cpre = L.tot;
+        else R.pre = L.tot;
+        R.n = &L.n;
+    }
+    return R;
+}
 
 __device__ __forceinline__ void k_cells_reset_unit(int c, const SysDev* __restrict__ sys, const SysState* __restrict__ st,
                               const int* __restrict__ cell_sys, int ncells_total,
@@ -282,31 +304,64 @@ __global__ void k_cell_fill(const SysDev* __restrict__ sys, const SysState* __re
 // one 64-thread workgroup per cell: bitonic sort of the cell's slots in LDS
 // (deterministic order within a cell; cells above the LDS capacity fall back
 // to an insertion sort by one thread)
-constexpr int CELL_SORT_CAP = 1024;
+// Sort each cell of the rebuilding systems (both binning paths) by slot,
+// the deterministic order inside a cell: one wave per cell, four cells per
+// workgroup, spread over the whole device.  Up to 64 atoms: each lane holds
+// one, ranked against the others read back lane by lane (distinct slots, so
+// the ranks are a permutation); up to CELL_RANK_MAX: eight per lane; larger
+// cells: one lane's insertion sort.  Every lane has read all m values before
+// any writes (one wave, and the ranks need every value).
+constexpr int CELL_RANK_MAX = 512;
 
-__device__ __forceinline__ void k_cell_sort_unit(int cell, const SysDev* __restrict__ sys,
-                                                  const SysState* __restrict__ st,
-                                                  const int* __restrict__ cell_sys,
-                                                  int ncells_total,
-                                                  const int* __restrict__ ccount,
-                                                  const int* __restrict__ cstart,
-                                                  int* __restrict__ cell_atoms)
+__global__ void __launch_bounds__(256) k_cell_sort(const SysDev* __restrict__ sys,
+                                                   const int* __restrict__ ccount,
+                                                   const int* __restrict__ cstart,
+                                                   int* __restrict__ cell_atoms, RbList R)
 {
-    const int c = cell;
-    if (c < 0 || c >= ncells_total) return;
-    const int s = cell_sys[c];
-    if (!st[s].rebuild) return;
-    const SysDev& S = sys[s];
-    const int lc = c - S.cell_off;
-    const int ncell = cells_in_use(S, 0) * cells_in_use(S, 1) * cells_in_use(S, 2);
-    if (lc >= ncell) return;
-    int* a = cell_atoms + S.atom_off + cstart[c];
-    const int n = ccount[c];
-    if (n <= 1) return;
-    if (n > CELL_SORT_CAP) {
-        if (threadIdx.x == 0) {
-            for (int i = 1; i < n; ++i) {
-                int v = a[i];
+    constexpr int RPL = CELL_RANK_MAX / 64;
+    const int lane = threadIdx.x & 63;
+    __shared__ RbStage rs;
+    R = rb_stage(R, true, rs);
+    const int nrb = *R.n;
+    const int64_t U = R.cpre[nrb];
+    for (int64_t u = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); u < U;
+         u += (int64_t)gridDim.x * 4) {
+        const int k = rb_find(R.cpre, nrb, u);
+        const SysDev& S = sys[R.sys[k]];
+        const int lc = (int)(u - R.cpre[k]);
+        if (lc >= cells_in_use(S, 0) * cells_in_use(S, 1) * cells_in_use(S, 2)) continue;
+        const int c = S.cell_off + lc;
+        const int m = ccount[c];
+        if (m <= 1) continue;
+        int* a = cell_atoms + S.atom_off + cstart[c];
+        if (m <= 64) {
+            const int v = lane < m ? a[lane] : 0x7fffffff;
+            int rank = 0;
+            // j is wave-uniform: a lane read into a scalar register
+            for (int j = 0; j < m; ++j) rank += __builtin_amdgcn_readlane(v, j) < v ? 1 : 0;
+            if (lane < m) a[rank] = v;
+        } else if (m <= CELL_RANK_MAX) {
+            int v[RPL], rank[RPL];
+#pragma unroll
+            for (int r = 0; r < RPL; ++r) {
+                v[r] = lane + 64 * r < m ? a[lane + 64 * r] : 0x7fffffff;
+                rank[r] = 0;
+            }
+#pragma unroll
+            for (int b = 0; b < RPL; ++b) {
+                const int lim = min(64, m - 64 * b);
+                for (int j = 0; j < lim; ++j) {
+                    const int aj = __builtin_amdgcn_readlane(v[b], j);
+#pragma unroll
+                    for (int r = 0; r < RPL; ++r) rank[r] += aj < v[r] ? 1 : 0;
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < RPL; ++r)
+                if (lane + 64 * r < m) a[rank[r]] = v[r];
+        } else if (lane == 0) {
+            for (int i = 1; i < m; ++i) {
+                const int v = a[i];
                 int j = i - 1;
                 while (j >= 0 && a[j] > v) {
                     a[j + 1] = a[j];
@@ -315,45 +370,6 @@ __device__ __forceinline__ void k_cell_sort_unit(int cell, const SysDev* __restr
                 a[j + 1] = v;
             }
         }
-        return;
-    }
-    __shared__ int buf[CELL_SORT_CAP];
-    int p = 1;
-    while (p < n) p <<= 1;
-    for (int k = threadIdx.x; k < p; k += 64) buf[k] = (k < n) ? a[k] : 0x7fffffff;
-    __syncthreads();
-    for (int size = 2; size <= p; size <<= 1) {
-        for (int half = size >> 1; half > 0; half >>= 1) {
-            for (int k = threadIdx.x; k < p; k += 64) {
-                int l = k ^ half;
-                if (l > k) {
-                    bool up = (k & size) == 0;
-                    int u = buf[k], v = buf[l];
-                    if ((u > v) == up) {
-                        buf[k] = v;
-                        buf[l] = u;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-    }
-    for (int k = threadIdx.x; k < n; k += 64) a[k] = buf[k];
-}
-
-__global__ void __launch_bounds__(64) k_cell_sort(const SysDev* __restrict__ sys,
-                                                  const SysState* __restrict__ st,
-                                                  const int* __restrict__ cell_sys,
-                                                  int ncells_total,
-                                                  const int* __restrict__ ccount,
-                                                  const int* __restrict__ cstart,
-                                                  int* __restrict__ cell_atoms, RbList R)
-{
-    DMDX_FOR_RB_UNITS(u_, R.cpre[*R.n])
-    {
-        k_cell_sort_unit(rb_cell(R, sys, u_), sys, st, cell_sys, ncells_total, ccount, cstart,
-                         cell_atoms);
-        __syncthreads();  // LDS is reused by the next unit
     }
 }
 
@@ -403,6 +419,8 @@ __global__ void k_pack_build(const SysDev* __restrict__ sys, const SysState* __r
                              double4* __restrict__ xref, float4* __restrict__ pos4f,
                              double4* __restrict__ org, RbList R)
 {
+    __shared__ RbStage rs;
+    R = rb_stage(R, false, rs);
     DMDX_FOR_RB_UNITS(u_, (R.pre[*R.n] + 255) / 256)
         k_pack_build_unit(rb_slot(R, sys, u_ * 256 + threadIdx.x), sys, st, slot_sys, nslots,
                           cell_atoms, acell, x, y, z, q, pos4s, xref, pos4f, org);
@@ -416,7 +434,6 @@ __global__ void k_pack_build(const SysDev* __restrict__ sys, const SysState* __r
 // launch for all rebuilding systems of that kind.
 constexpr int BIN_THREADS = 1024;
 constexpr int BIN_PER_THREAD = BIN_WG_ATOMS / BIN_THREADS;
-constexpr int BIN_RANK_MAX = 512;  // cells ranked by one wave (8 elements per lane)
 
 __global__ void __launch_bounds__(BIN_THREADS) k_bin_system(
     const SysDev* __restrict__ sys, SysState* __restrict__ st, const int* __restrict__ wg_sys,
@@ -425,7 +442,6 @@ __global__ void __launch_bounds__(BIN_THREADS) k_bin_system(
     int* __restrict__ cstart, int* __restrict__ cell_atoms)
 {
     __shared__ int fill[BIN_WG_CELLS];   // counts, then fill pointers
-    __shared__ int start[BIN_WG_CELLS];
     __shared__ int slots[BIN_WG_ATOMS];  // the system's atoms in cell order
     __shared__ int wsum[BIN_THREADS / 64];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -475,7 +491,6 @@ __global__ void __launch_bounds__(BIN_THREADS) k_bin_system(
         int run = wbase + incl - sum;
         for (int c = cb; c < ce; ++c) {
             const int cnt = fill[c];
-            start[c] = run;
             ccount[S.cell_off + c] = cnt;
             cstart[S.cell_off + c] = run;
             fill[c] = run;
@@ -493,54 +508,9 @@ __global__ void __launch_bounds__(BIN_THREADS) k_bin_system(
             if (g < n) slots[atomicAdd(&fill[cl[r]], 1)] = (int)(a0 + g);
         }
         __syncthreads();
-        // order inside each cell by slot: one wave per cell, each element
-        // ranked against the others (distinct slots, so the ranks are a
-        // permutation); cells above BIN_RANK_MAX atoms by one lane's
-        // insertion sort
-        constexpr int RPL = BIN_RANK_MAX / 64;  // elements per lane
-        for (int c = wave; c < ncell; c += BIN_THREADS / 64) {
-            const int b = start[c], m = fill[c] - b;
-            if (m <= 1) continue;
-            int* a = slots + b;
-            if (m <= 64) {
-                const int v = lane < m ? a[lane] : 0x7fffffff;
-                int rank = 0;
-                // j is wave-uniform: a lane read into a scalar register
-                for (int j = 0; j < m; ++j) rank += __builtin_amdgcn_readlane(v, j) < v ? 1 : 0;
-                if (lane < m) a[rank] = v;
-            } else if (m <= BIN_RANK_MAX) {
-                int v[RPL], rank[RPL];
-#pragma unroll
-                for (int r = 0; r < RPL; ++r) {
-                    const int e = lane + 64 * r;
-                    v[r] = e < m ? a[e] : 0x7fffffff;
-                    rank[r] = 0;
-                }
-                for (int j = 0; j < m; ++j) {
-                    const int aj = a[j];  // same address in every lane: a broadcast
-#pragma unroll
-                    for (int r = 0; r < RPL; ++r) rank[r] += aj < v[r] ? 1 : 0;
-                }
-                // every lane of the wave has read all m values before any
-                // writes (one wave, lockstep)
-#pragma unroll
-                for (int r = 0; r < RPL; ++r)
-                    if (lane + 64 * r < m) a[rank[r]] = v[r];
-            } else if (lane == 0) {
-                for (int i = 1; i < m; ++i) {
-                    const int v = a[i];
-                    int j = i - 1;
-                    while (j >= 0 && a[j] > v) {
-                        a[j + 1] = a[j];
-                        --j;
-                    }
-                    a[j + 1] = v;
-                }
-            }
-        }
-        __syncthreads();
-        // cell order (the packing follows in k_pack_build, spread over the
-        // whole device)
+        // cells in order, atoms of a cell in arbitrary order: k_cell_sort
+        // orders each cell, spread over the whole device, and k_pack_build
+        // packs
         for (int lt = tid; lt < n; lt += BIN_THREADS) cell_atoms[a0 + lt] = slots[lt];
         __syncthreads();  // LDS is reused by the next system
     }
@@ -938,6 +908,8 @@ __global__ void __launch_bounds__(256) k_nbr_build_mixed(
 {
     constexpr int GPB = 256 / TPA;  // atoms per workgroup pass
     __shared__ int tab_base[GPB][NB_CELLS], tab_pre[GPB][NB_CELLS + 1], tab_off[GPB][NB_CELLS];
+    __shared__ RbStage rs;
+    R = rb_stage(R, false, rs);
     DMDX_FOR_RB_UNITS(u_, (R.pre[*R.n] + GPB - 1) / GPB)
         k_nbr_build_mixed_unit<TPA>(rb_slot(R, sys, u_ * GPB + threadIdx.x / TPA), sys, st,
                                     slot_sys, nslots, acell, ccount, cstart, cell_atoms,
@@ -1074,7 +1046,7 @@ hipError_t launch_build_lists(const Dev& d, hipStream_t s)
     // then visit only their cells and slots.  Grids are sized for a full
     // rebuild and stride over the compact ranges, whose lengths they read on
     // the device.
-    const RbOut all{d.rb_sys, d.rb_pre, nullptr, d.rb_n};
+    const RbOut all{d.rb_sys, d.rb_pre, d.rb_cpre, d.rb_n};
     const RbOut multi{d.rbm_sys, d.rbm_pre, d.rbm_cpre, d.rbm_n};
     const RbOut wg{d.rbw_sys, nullptr, nullptr, d.rbw_n};
     k_rb_list<<<1, 1024, 0, s>>>(d.sys, d.st, d.nsys, all, multi, wg);
@@ -1092,11 +1064,11 @@ hipError_t launch_build_lists(const Dev& d, hipStream_t s)
         k_cell_scan<<<d.nsys, 256, 0, s>>>(d.sys, d.st, d.ccount, d.cstart);
         k_cell_fill<<<grid_for(blocks_for(d.nslots, BS)), BS, 0, s>>>(
             d.sys, d.st, d.slot_sys, d.nslots, d.acell, d.cstart, d.cfill, d.cell_atoms, R);
-        if (d.ncells > 0)
-            k_cell_sort<<<grid_for((unsigned)d.ncells), 64, 0, s>>>(
-                d.sys, d.st, d.cell_sys, d.ncells, d.ccount, d.cstart, d.cell_atoms, R);
     }
-    const RbList R{d.rb_sys, d.rb_pre, nullptr, d.rb_n};
+    const RbList R{d.rb_sys, d.rb_pre, d.rb_cpre, d.rb_n};
+    if (d.ncells > 0)
+        k_cell_sort<<<grid_for(blocks_for(d.ncells, 4)), 256, 0, s>>>(d.sys, d.ccount, d.cstart,
+                                                                       d.cell_atoms, R);
     k_pack_build<<<grid_for(blocks_for(d.nslots, BS)), BS, 0, s>>>(
         d.sys, d.st, d.slot_sys, d.nslots, d.cell_atoms, d.acell, d.x, d.y, d.z, d.q, d.pos4s,
         d.xref, d.pos4f, d.org, R);
