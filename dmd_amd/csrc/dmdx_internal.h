@@ -224,7 +224,7 @@ struct Dev {
     bool any_auto = false;     // some system uses the displacement criterion
     int* typ_s = nullptr;      // mixed path: LJ type in cell order
     int pair_tpa = 4;          // mixed pair kernel: lanes per atom (2, 4, 8, 16, 32)
-    int build_tpa = 16;        // mixed list build: lanes per atom (8, 16, 32)
+    int build_tpa = 32;        // mixed list build: lanes per atom (8, 16, 32)
     int pair_width = 8;        // mixed pair kernel: entries per lane per pass (2, 4, 8)
     // LJ tables
     double *lj_sig = nullptr, *lj_eps = nullptr;

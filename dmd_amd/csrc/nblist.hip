@@ -20,7 +20,9 @@
 //   k_nbr_build    per atom (cell order): scan the neighbour cells
 //   k_nbr_sort     per atom: ascending neighbour order (FP64 path: makes the
 //                  per-atom force sum follow the reference's summation order)
-//   k_pack         mixed path, every compute: cell-local FP32 positions
+//   k_pack         mixed path: cell-local FP32 positions of a compute that
+//                  follows no drift, or SHAKE (after a drift, k_kick_drift
+//                  and k_pack_build have written them)
 #include "dmdx_internal.h"
 
 #include <type_traits>
@@ -382,7 +384,8 @@ __device__ __forceinline__ void k_pack_build_unit(int64_t t, const SysDev* __res
                              const double* __restrict__ y, const double* __restrict__ z,
                              const double* __restrict__ q, double4* __restrict__ pos4s,
                              double4* __restrict__ xref, float4* __restrict__ pos4f,
-                             double4* __restrict__ org)
+                             double4* __restrict__ org, const int* __restrict__ type,
+                             float4* __restrict__ lpos, int* __restrict__ typ_s)
 {
     if (t < 0 || t >= nslots) return;
     int s = slot_sys[t];
@@ -406,7 +409,12 @@ __device__ __forceinline__ void k_pack_build_unit(int64_t t, const SysDev* __res
         const double ly = wy - (cy + 0.5) * (S.L[1] / ny);
         const double lz = wz - (cz + 0.5) * (S.L[2] / nz);
         pos4f[t] = make_float4((float)lx, (float)ly, (float)lz, 0.0f);
-        org[t] = make_double4(x[i] - lx, y[i] - ly, z[i] - lz, q[i]);
+        const double4 o = make_double4(x[i] - lx, y[i] - ly, z[i] - lz, q[i]);
+        org[t] = o;
+        // the pair kernel's frame position and type, as k_pack writes them
+        lpos[t] = make_float4((float)(x[i] - o.x), (float)(y[i] - o.y), (float)(z[i] - o.z),
+                              (float)o.w);
+        typ_s[t] = type[i];
     }
 }
 
@@ -417,13 +425,15 @@ __global__ void k_pack_build(const SysDev* __restrict__ sys, const SysState* __r
                              const double* __restrict__ y, const double* __restrict__ z,
                              const double* __restrict__ q, double4* __restrict__ pos4s,
                              double4* __restrict__ xref, float4* __restrict__ pos4f,
-                             double4* __restrict__ org, RbList R)
+                             double4* __restrict__ org, const int* __restrict__ type,
+                             float4* __restrict__ lpos, int* __restrict__ typ_s, RbList R)
 {
     __shared__ RbStage rs;
     R = rb_stage(R, false, rs);
     DMDX_FOR_RB_UNITS(u_, (R.pre[*R.n] + 255) / 256)
         k_pack_build_unit(rb_slot(R, sys, u_ * 256 + threadIdx.x), sys, st, slot_sys, nslots,
-                          cell_atoms, acell, x, y, z, q, pos4s, xref, pos4f, org);
+                          cell_atoms, acell, x, y, z, q, pos4s, xref, pos4f, org, type, lpos,
+                          typ_s);
 }
 
 // ---- one workgroup bins one system (systems of at most BIN_WG_ATOMS atoms
@@ -1071,7 +1081,7 @@ hipError_t launch_build_lists(const Dev& d, hipStream_t s)
                                                                        d.cell_atoms, R);
     k_pack_build<<<grid_for(blocks_for(d.nslots, BS)), BS, 0, s>>>(
         d.sys, d.st, d.slot_sys, d.nslots, d.cell_atoms, d.acell, d.x, d.y, d.z, d.q, d.pos4s,
-        d.xref, d.pos4f, d.org, R);
+        d.xref, d.pos4f, d.org, d.type, d.lpos, d.typ_s, R);
     if (d.precision == DMDX_FP64)
         k_nbr_build<1><<<grid_for(blocks_for(d.nslots, BS)), BS, 0, s>>>(
             d.sys, d.st, d.slot_sys, d.nslots, d.x, d.y, d.z, d.acell, d.ccount, d.cstart,

@@ -163,14 +163,9 @@ __device__ __forceinline__ void k_pair_mixed_unit(int64_t unit_,
     // its forces are bit-identical alone and in any mix.
     if ((S.coul_pairs != 0) != COUL) return;
     const int lt = (int)(t - S.atom_off);
-    if (lt >= S.n) {
-        if (lane == 0) {
-            e_lj_part[t] = 0.0;
-            e_c_part[t] = 0.0;
-        }
-        return;
-    }
-    const float4 pi = lpos[t];
+    // padding slots stay in the wave (no entries) for the wave's energy sum
+    const bool real = lt < S.n;
+    const float4 pi = real ? lpos[t] : make_float4(0.f, 0.f, 0.f, 0.f);
     // cell widths of the frames (cells of the last build; a barostat scales
     // the box, the positions and the frame origins together)
     const float wx = (float)(S.L[0] / T.nub[0]);
@@ -186,7 +181,7 @@ __device__ __forceinline__ void k_pair_mixed_unit(int64_t unit_,
     const bool typed = lj && nt > 1;
     const float* c12t = lj_c12 + S.lj_off;
     const float* c6t = lj_c6 + S.lj_off;
-    const int ti = typed ? typ_s[t] : 0;
+    const int ti = typed && real ? typ_s[t] : 0;
     const float c12_0 = c12t[0], c6_0 = c6t[0];
     const float qi = cp ? (float)(KE * (double)pi.w) : 0.0f;
     const float alpha = (float)S.alpha;
@@ -196,7 +191,7 @@ __device__ __forceinline__ void k_pair_mixed_unit(int64_t unit_,
     const int* row = nbr + S.nbr_off + (int64_t)lt * S.max_nbr;
     const float4* __restrict__ lps = lpos + a0;  // this system's slots
     const int* __restrict__ tys = typ_s + a0;
-    const int cnt = nn[t];
+    const int cnt = real ? nn[t] : 0;
     double fxi = 0.0, fyi = 0.0, fzi = 0.0, ej = 0.0, ec = 0.0;
     // Each lane takes W (2, 4 or 8) entries per pass and issues their W gathers
     // before any arithmetic (memory-level parallelism for the dependent
@@ -343,7 +338,10 @@ __device__ __forceinline__ void k_pair_mixed_unit(int64_t unit_,
     fzi = group_sum<TPA>(fzi);
     ej = group_sum<TPA>(ej);
     ec = group_sum<TPA>(ec);
-    if (lane == 0) {
+    // one energy partial per wave (its 64 / TPA atom slots), a fixed tree
+    const double ejw = wave_sum(lane == 0 ? ej : 0.0);
+    const double ecw = COUL ? wave_sum(lane == 0 ? ec : 0.0) : 0.0;
+    if (lane == 0 && real) {
         const int i = order[t];
         if (accumulate) {
             fxi += fx[i];
@@ -353,8 +351,10 @@ __device__ __forceinline__ void k_pair_mixed_unit(int64_t unit_,
         fx[i] = fxi;
         fy[i] = fyi;
         fz[i] = fzi;
-        e_lj_part[t] = 0.5 * ej;
-        e_c_part[t] = 0.5 * ec;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        e_lj_part[t / (64 / TPA)] = 0.5 * ejw;
+        e_c_part[t / (64 / TPA)] = 0.5 * ecw;
     }
 }
 
