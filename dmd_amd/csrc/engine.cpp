@@ -992,7 +992,7 @@ int Batch::enqueue_compute(int phase, int accumulate, int force_rebuild, bool un
     if (defer) LAUNCH(timed(KG_GATHER, launch_gather_terms, side));
     if (fork) DMDX_HIP(hipEventRecord(ev_join, side));
     g_acc = accumulate;
-    // after the drift, k_kick_drift (kept lists) and k_pack_build (rebuilt
+    // after the drift, k_kick_drift (kept lists) and k_cell_sort (rebuilt
     // lists) have written every frame position already
     if (!after_drift) LAUNCH(launch_pack_phase(d, stream, phase == 2 ? 1 : 0));
     LAUNCH(timed(KG_PAIR, launch_pair_acc));

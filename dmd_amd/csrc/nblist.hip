@@ -16,13 +16,14 @@
 //   k_cell_count   counters; cell of each atom from its wrapped position,
 //   k_cell_scan    count; per-system exclusive scan of the counts; scatter
 //   k_cell_fill    atoms into cell order
-//   k_cell_sort    sort each cell by slot (deterministic order), both paths
+//   k_cell_sort    sort each cell by slot (deterministic order), both paths,
+//                  and pack its atoms in cell order
 //   k_nbr_build    per atom (cell order): scan the neighbour cells
 //   k_nbr_sort     per atom: ascending neighbour order (FP64 path: makes the
 //                  per-atom force sum follow the reference's summation order)
 //   k_pack         mixed path: cell-local FP32 positions of a compute that
 //                  follows no drift, or SHAKE (after a drift, k_kick_drift
-//                  and k_pack_build have written them)
+//                  and k_cell_sort have written them)
 #include "dmdx_internal.h"
 
 #include <type_traits>
@@ -306,19 +307,71 @@ __global__ void k_cell_fill(const SysDev* __restrict__ sys, const SysState* __re
 // one 64-thread workgroup per cell: bitonic sort of the cell's slots in LDS
 // (deterministic order within a cell; cells above the LDS capacity fall back
 // to an insertion sort by one thread)
+// After the cell sort, per atom (global slot i at cell-order slot t of cell
+// lc of system S): positions in cell order (contiguous per cell, read by the
+// list build) and the reference positions of the displacement criterion;
+// mixed path: the cell-local frame, its origin, and the pair kernel's frame
+// position and type.
+struct PackOut {
+    double4* pos4s;
+    double4* xref;
+    float4* pos4f;  // null: FP64 path
+    double4* org;
+    float4* lpos;
+    int* typ_s;
+};
+
+__device__ __forceinline__ void pack_atom(const SysDev& S, int64_t t, int i, int lc,
+                                          const double* __restrict__ x,
+                                          const double* __restrict__ y,
+                                          const double* __restrict__ z,
+                                          const double* __restrict__ q,
+                                          const int* __restrict__ type, const PackOut& o)
+{
+    o.pos4s[t] = make_double4(x[i], y[i], z[i], q[i]);
+    o.xref[i] = make_double4(x[i], y[i], z[i], 0.0);
+    if (o.pos4f) {
+        // mixed path: cell-local frame centred on the atom's cell,
+        // l = wrapped position - cell centre (|l| <= w/2 at the build, so FP32
+        // keeps ~3e-8 nm), origin = x - l; pos4f holds l for the list build
+        double wx = x[i] - floor(x[i] * S.invL[0]) * S.L[0];
+        double wy = y[i] - floor(y[i] * S.invL[1]) * S.L[1];
+        double wz = z[i] - floor(z[i] * S.invL[2]) * S.L[2];
+        const int nx = cells_in_use(S, 0), ny = cells_in_use(S, 1), nz = cells_in_use(S, 2);
+        const int cx = lc % nx, cy = (lc / nx) % ny, cz = lc / (nx * ny);
+        const double lx = wx - (cx + 0.5) * (S.L[0] / nx);
+        const double ly = wy - (cy + 0.5) * (S.L[1] / ny);
+        const double lz = wz - (cz + 0.5) * (S.L[2] / nz);
+        o.pos4f[t] = make_float4((float)lx, (float)ly, (float)lz, 0.0f);
+        const double4 g = make_double4(x[i] - lx, y[i] - ly, z[i] - lz, q[i]);
+        o.org[t] = g;
+        // the pair kernel's frame position and type, as k_pack writes them
+        o.lpos[t] = make_float4((float)(x[i] - g.x), (float)(y[i] - g.y), (float)(z[i] - g.z),
+                                (float)g.w);
+        o.typ_s[t] = type[i];
+    }
+}
+
 // Sort each cell of the rebuilding systems (both binning paths) by slot,
-// the deterministic order inside a cell: one wave per cell, four cells per
-// workgroup, spread over the whole device.  Up to 64 atoms: each lane holds
-// one, ranked against the others read back lane by lane (distinct slots, so
-// the ranks are a permutation); up to CELL_RANK_MAX: eight per lane; larger
-// cells: one lane's insertion sort.  Every lane has read all m values before
-// any writes (one wave, and the ranks need every value).
+// the deterministic order inside a cell, and pack its atoms (pack_atom): one
+// wave per cell, four cells per workgroup, spread over the whole device.  Up
+// to 64 atoms: each lane holds one, ranked against the others read back lane
+// by lane (distinct slots, so the ranks are a permutation); up to
+// CELL_RANK_MAX: eight per lane; larger cells: one lane's insertion sort.
+// Every lane has read all m values before any writes (one wave, and the
+// ranks need every value).
 constexpr int CELL_RANK_MAX = 512;
 
 __global__ void __launch_bounds__(256) k_cell_sort(const SysDev* __restrict__ sys,
                                                    const int* __restrict__ ccount,
                                                    const int* __restrict__ cstart,
-                                                   int* __restrict__ cell_atoms, RbList R)
+                                                   int* __restrict__ cell_atoms,
+                                                   const double* __restrict__ x,
+                                                   const double* __restrict__ y,
+                                                   const double* __restrict__ z,
+                                                   const double* __restrict__ q,
+                                                   const int* __restrict__ type, PackOut po,
+                                                   RbList R)
 {
     constexpr int RPL = CELL_RANK_MAX / 64;
     const int lane = threadIdx.x & 63;
@@ -334,14 +387,18 @@ __global__ void __launch_bounds__(256) k_cell_sort(const SysDev* __restrict__ sy
         if (lc >= cells_in_use(S, 0) * cells_in_use(S, 1) * cells_in_use(S, 2)) continue;
         const int c = S.cell_off + lc;
         const int m = ccount[c];
-        if (m <= 1) continue;
-        int* a = cell_atoms + S.atom_off + cstart[c];
+        if (m == 0) continue;
+        const int64_t t0 = S.atom_off + cstart[c];  // cell-order slot of the cell's first atom
+        int* a = cell_atoms + t0;
         if (m <= 64) {
             const int v = lane < m ? a[lane] : 0x7fffffff;
             int rank = 0;
             // j is wave-uniform: a lane read into a scalar register
             for (int j = 0; j < m; ++j) rank += __builtin_amdgcn_readlane(v, j) < v ? 1 : 0;
-            if (lane < m) a[rank] = v;
+            if (lane < m) {
+                a[rank] = v;
+                pack_atom(S, t0 + rank, v, lc, x, y, z, q, type, po);
+            }
         } else if (m <= CELL_RANK_MAX) {
             int v[RPL], rank[RPL];
 #pragma unroll
@@ -360,80 +417,27 @@ __global__ void __launch_bounds__(256) k_cell_sort(const SysDev* __restrict__ sy
             }
 #pragma unroll
             for (int r = 0; r < RPL; ++r)
-                if (lane + 64 * r < m) a[rank[r]] = v[r];
-        } else if (lane == 0) {
-            for (int i = 1; i < m; ++i) {
-                const int v = a[i];
-                int j = i - 1;
-                while (j >= 0 && a[j] > v) {
-                    a[j + 1] = a[j];
-                    --j;
+                if (lane + 64 * r < m) {
+                    a[rank[r]] = v[r];
+                    pack_atom(S, t0 + rank[r], v[r], lc, x, y, z, q, type, po);
                 }
-                a[j + 1] = v;
+        } else {
+            if (lane == 0) {
+                for (int i = 1; i < m; ++i) {
+                    const int v = a[i];
+                    int j = i - 1;
+                    while (j >= 0 && a[j] > v) {
+                        a[j + 1] = a[j];
+                        --j;
+                    }
+                    a[j + 1] = v;
+                }
             }
+            __threadfence_block();  // lane 0's stores, visible to the wave
+            __builtin_amdgcn_wave_barrier();
+            for (int k = lane; k < m; k += 64) pack_atom(S, t0 + k, a[k], lc, x, y, z, q, type, po);
         }
     }
-}
-
-// after the cell sort: positions in cell order (contiguous per cell, read by
-// the scan below) and the reference positions of the displacement criterion
-__device__ __forceinline__ void k_pack_build_unit(int64_t t, const SysDev* __restrict__ sys, const SysState* __restrict__ st,
-                             const int* __restrict__ slot_sys, int64_t nslots,
-                             const int* __restrict__ cell_atoms, const int* __restrict__ acell,
-                             const double* __restrict__ x,
-                             const double* __restrict__ y, const double* __restrict__ z,
-                             const double* __restrict__ q, double4* __restrict__ pos4s,
-                             double4* __restrict__ xref, float4* __restrict__ pos4f,
-                             double4* __restrict__ org, const int* __restrict__ type,
-                             float4* __restrict__ lpos, int* __restrict__ typ_s)
-{
-    if (t < 0 || t >= nslots) return;
-    int s = slot_sys[t];
-    if (s < 0 || !st[s].rebuild) return;
-    const SysDev& S = sys[s];
-    if (t - S.atom_off >= S.n) return;
-    int i = cell_atoms[t];
-    pos4s[t] = make_double4(x[i], y[i], z[i], q[i]);
-    xref[i] = make_double4(x[i], y[i], z[i], 0.0);
-    if (pos4f) {
-        // mixed path: cell-local frame centred on the atom's cell,
-        // l = wrapped position - cell centre (|l| <= w/2 at the build, so FP32
-        // keeps ~3e-8 nm), origin = x - l; pos4f holds l for the list build
-        double wx = x[i] - floor(x[i] * S.invL[0]) * S.L[0];
-        double wy = y[i] - floor(y[i] * S.invL[1]) * S.L[1];
-        double wz = z[i] - floor(z[i] * S.invL[2]) * S.L[2];
-        const int nx = cells_in_use(S, 0), ny = cells_in_use(S, 1), nz = cells_in_use(S, 2);
-        const int c = acell[i];
-        const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
-        const double lx = wx - (cx + 0.5) * (S.L[0] / nx);
-        const double ly = wy - (cy + 0.5) * (S.L[1] / ny);
-        const double lz = wz - (cz + 0.5) * (S.L[2] / nz);
-        pos4f[t] = make_float4((float)lx, (float)ly, (float)lz, 0.0f);
-        const double4 o = make_double4(x[i] - lx, y[i] - ly, z[i] - lz, q[i]);
-        org[t] = o;
-        // the pair kernel's frame position and type, as k_pack writes them
-        lpos[t] = make_float4((float)(x[i] - o.x), (float)(y[i] - o.y), (float)(z[i] - o.z),
-                              (float)o.w);
-        typ_s[t] = type[i];
-    }
-}
-
-__global__ void k_pack_build(const SysDev* __restrict__ sys, const SysState* __restrict__ st,
-                             const int* __restrict__ slot_sys, int64_t nslots,
-                             const int* __restrict__ cell_atoms, const int* __restrict__ acell,
-                             const double* __restrict__ x,
-                             const double* __restrict__ y, const double* __restrict__ z,
-                             const double* __restrict__ q, double4* __restrict__ pos4s,
-                             double4* __restrict__ xref, float4* __restrict__ pos4f,
-                             double4* __restrict__ org, const int* __restrict__ type,
-                             float4* __restrict__ lpos, int* __restrict__ typ_s, RbList R)
-{
-    __shared__ RbStage rs;
-    R = rb_stage(R, false, rs);
-    DMDX_FOR_RB_UNITS(u_, (R.pre[*R.n] + 255) / 256)
-        k_pack_build_unit(rb_slot(R, sys, u_ * 256 + threadIdx.x), sys, st, slot_sys, nslots,
-                          cell_atoms, acell, x, y, z, q, pos4s, xref, pos4f, org, type, lpos,
-                          typ_s);
 }
 
 // ---- one workgroup bins one system (systems of at most BIN_WG_ATOMS atoms
@@ -519,8 +523,7 @@ __global__ void __launch_bounds__(BIN_THREADS) k_bin_system(
         }
         __syncthreads();
         // cells in order, atoms of a cell in arbitrary order: k_cell_sort
-        // orders each cell, spread over the whole device, and k_pack_build
-        // packs
+        // orders and packs each cell, spread over the whole device
         for (int lt = tid; lt < n; lt += BIN_THREADS) cell_atoms[a0 + lt] = slots[lt];
         __syncthreads();  // LDS is reused by the next system
     }
@@ -1076,12 +1079,10 @@ hipError_t launch_build_lists(const Dev& d, hipStream_t s)
             d.sys, d.st, d.slot_sys, d.nslots, d.acell, d.cstart, d.cfill, d.cell_atoms, R);
     }
     const RbList R{d.rb_sys, d.rb_pre, d.rb_cpre, d.rb_n};
+    const PackOut po{d.pos4s, d.xref, d.pos4f, d.org, d.lpos, d.typ_s};
     if (d.ncells > 0)
-        k_cell_sort<<<grid_for(blocks_for(d.ncells, 4)), 256, 0, s>>>(d.sys, d.ccount, d.cstart,
-                                                                       d.cell_atoms, R);
-    k_pack_build<<<grid_for(blocks_for(d.nslots, BS)), BS, 0, s>>>(
-        d.sys, d.st, d.slot_sys, d.nslots, d.cell_atoms, d.acell, d.x, d.y, d.z, d.q, d.pos4s,
-        d.xref, d.pos4f, d.org, d.type, d.lpos, d.typ_s, R);
+        k_cell_sort<<<grid_for(blocks_for(d.ncells, 4)), 256, 0, s>>>(
+            d.sys, d.ccount, d.cstart, d.cell_atoms, d.x, d.y, d.z, d.q, d.type, po, R);
     if (d.precision == DMDX_FP64)
         k_nbr_build<1><<<grid_for(blocks_for(d.nslots, BS)), BS, 0, s>>>(
             d.sys, d.st, d.slot_sys, d.nslots, d.x, d.y, d.z, d.acell, d.ccount, d.cstart,
