@@ -140,7 +140,7 @@ __device__ __forceinline__ void k_pair_mixed_unit(int64_t unit_,
     const int* __restrict__ typ_s, const float* __restrict__ lj_c12,
     const float* __restrict__ lj_c6, double* __restrict__ fx, double* __restrict__ fy,
     double* __restrict__ fz, double* __restrict__ e_lj_part, double* __restrict__ e_c_part,
-    int accumulate)
+    int accumulate, float4* __restrict__ tab, int& tab_sys)
 {
     // FP32 arithmetic of this kernel may fuse into FMAs (the library is built
     // with contraction off for the FP64 parity kernels)
@@ -171,6 +171,20 @@ __device__ __forceinline__ void k_pair_mixed_unit(int64_t unit_,
     const float wx = (float)(S.L[0] / T.nub[0]);
     const float wy = (float)(S.L[1] / T.nub[1]);
     const float wz = (float)(S.L[2] / T.nub[2]);
+    // the wave's table of cell offsets times cell widths, indexed by the
+    // 9-bit offset field of an entry; rebuilt when the wave moves to another
+    // system (its units are mostly consecutive blocks of one system)
+    if (s != tab_sys) {
+        for (int c = threadIdx.x & 63; c < 125; c += 64) {
+            const int ox = c % 5 - 2, oy = (c / 5) % 5 - 2, oz = c / 25 - 2;
+            const uint32_t code = ((uint32_t)ox & 7u) | (((uint32_t)oy & 7u) << 3) |
+                                  (((uint32_t)oz & 7u) << 6);
+            tab[code] = make_float4((float)ox * wx, (float)oy * wy, (float)oz * wz, 0.0f);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        tab_sys = s;
+    }
     const int full = 2 * S.cdiv + 1;  // cells per edge for offsets to be minimum images
     const bool wrap = T.nub[0] < full || T.nub[1] < full || T.nub[2] < full;
     const int64_t a0 = S.atom_off;
@@ -269,45 +283,50 @@ __device__ __forceinline__ void k_pair_mixed_unit(int64_t unit_,
             if (STRIDED || kp + STRIDE < cnt) load_entries(k0 + STRIDE, en);
             // the W entries of a pass are summed in FP32, then added in FP64
             float px = 0.0f, py = 0.0f, pz = 0.0f, pel = 0.0f, pec = 0.0f;
+            // Branch-free entries: an entry past the row end (its gather read
+            // the atom itself) or outside a cutoff computes on r2 = 1 and
+            // contributes exactly zero through a select.
 #pragma unroll
             for (int q = 0; q < W; ++q) {
-                if (entry_index(k0, q) >= cnt) continue;
+                const bool valid = entry_index(k0, q) < cnt;
                 const uint32_t e = ev[q];
-                const float ox = (float)((int)(e << (32 - NBR_SLOT_BITS - 3)) >> 29);
-                const float oy = (float)((int)(e << (32 - NBR_SLOT_BITS - 6)) >> 29);
-                const float oz = (float)((int)e >> 29);
-                float dx = (pj[q].x - pi.x) + ox * wx;
-                float dy = (pj[q].y - pi.y) + oy * wy;
-                float dz = (pj[q].z - pi.z) + oz * wz;
+                // the neighbour cell's offset times the cell widths, from the
+                // wave's table (the entry's top nine bits index it)
+                const float4 sh = tab[valid ? e >> NBR_SLOT_BITS : 0u];
+                float dx = (pj[q].x - pi.x) + sh.x;
+                float dy = (pj[q].y - pi.y) + sh.y;
+                float dz = (pj[q].z - pi.z) + sh.z;
                 if constexpr (WRAP) {
                     dx -= rintf(dx * ix) * Lx;
                     dy -= rintf(dy * iy) * Ly;
                     dz -= rintf(dz * iz) * Lz;
                 }
                 const float r2 = dx * dx + dy * dy + dz * dz;
-                const float rinv2 = __builtin_amdgcn_rcpf(r2);  // 1 ulp hardware reciprocal
-                float fs = 0.0f, el = 0.0f, ecl = 0.0f;
-                if (lj && r2 < cut2) {
-                    const float c12 = TYPED ? c12t[ti * nt + tj[q]] : c12_0;
-                    const float c6 = TYPED ? c6t[ti * nt + tj[q]] : c6_0;
-                    const float r6i = rinv2 * rinv2 * rinv2;
-                    const float a12 = c12 * r6i * r6i, a6 = c6 * r6i;
-                    fs = (12.0f * a12 - 6.0f * a6) * rinv2;
-                    el = a12 - a6;
-                }
-                if (COUL && cp && r2 < ccut2) {
-                    const float rinv = rsqrtf(r2);
-                    const float r = r2 * rinv;
+                const bool in_lj = lj && valid && r2 < cut2;
+                const float r2l = in_lj ? r2 : 1.0f;
+                const float rinv2 = __builtin_amdgcn_rcpf(r2l);  // 1 ulp hardware reciprocal
+                const float c12 = TYPED ? c12t[ti * nt + tj[q]] : c12_0;
+                const float c6 = TYPED ? c6t[ti * nt + tj[q]] : c6_0;
+                const float r6i = rinv2 * rinv2 * rinv2;
+                const float a12 = c12 * r6i * r6i, a6 = c6 * r6i;
+                float fs = in_lj ? (12.0f * a12 - 6.0f * a6) * rinv2 : 0.0f;
+                const float el = in_lj ? a12 - a6 : 0.0f;
+                float ecl = 0.0f;
+                if constexpr (COUL) {
+                    const bool in_c = cp && valid && r2 < ccut2;
+                    const float r2c = in_c ? r2 : 1.0f;
+                    const float rinv = rsqrtf(r2c);
+                    const float r = r2c * rinv;
                     const float qq = qi * pj[q].w;
-                    const float ex = __expf(-a2 * r2);
+                    const float ex = __expf(-a2 * r2c);
                     // erfc(a r) by Abramowitz & Stegun 7.1.26 (|error| <= 1.5e-7),
                     // sharing exp(-(a r)^2) with the force term
                     const float tt = __builtin_amdgcn_rcpf(1.0f + 0.3275911f * (alpha * r));
                     const float erfc_ar =
                         tt * (0.254829592f + tt * (-0.284496736f + tt * (1.421413741f +
                               tt * (-1.453152027f + tt * 1.061405429f)))) * ex;
-                    fs += qq * (erfc_ar * rinv + two_a_rpi * ex) * rinv2;
-                    ecl = qq * erfc_ar * rinv;
+                    fs += in_c ? qq * (erfc_ar * rinv + two_a_rpi * ex) * (rinv * rinv) : 0.0f;
+                    ecl = in_c ? qq * erfc_ar * rinv : 0.0f;
                 }
                 px += fs * dx;
                 py += fs * dy;
@@ -359,7 +378,7 @@ __device__ __forceinline__ void k_pair_mixed_unit(int64_t unit_,
 }
 
 template <int TPA, int W, bool COUL>
-__global__ void __launch_bounds__(256) k_pair_mixed(
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(COUL ? 4 : 5))) k_pair_mixed(
     const SysDev* __restrict__ sys, const SysState* __restrict__ st,
     const int* __restrict__ slot_sys, int64_t nslots, const int* __restrict__ order,
     const int* __restrict__ nn, const int* __restrict__ nbr, const float4* __restrict__ lpos,
@@ -368,7 +387,9 @@ __global__ void __launch_bounds__(256) k_pair_mixed(
     double* __restrict__ fz, double* __restrict__ e_lj_part, double* __restrict__ e_c_part,
     int accumulate, int64_t nunits_)
 {
-    DMDX_FOR_UNITS(u_, nunits_) k_pair_mixed_unit<TPA, W, COUL>(u_, sys, st, slot_sys, nslots, order, nn, nbr, lpos, typ_s, lj_c12, lj_c6, fx, fy, fz, e_lj_part, e_c_part, accumulate);
+    __shared__ float4 shift_tab[4][512];  // per wave
+    int tab_sys = -1;
+    DMDX_FOR_UNITS(u_, nunits_) k_pair_mixed_unit<TPA, W, COUL>(u_, sys, st, slot_sys, nslots, order, nn, nbr, lpos, typ_s, lj_c12, lj_c6, fx, fy, fz, e_lj_part, e_c_part, accumulate, shift_tab[threadIdx.x >> 6], tab_sys);
 }
 
 }  // namespace dmdx
