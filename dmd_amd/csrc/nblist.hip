@@ -176,7 +176,7 @@ __device__ __forceinline__ int rb_cell(const RbList& R, const SysDev* __restrict
 // so the binary search of every lookup reads LDS instead of a chain of
 // dependent global loads.  Every thread of the workgroup calls it, before
 // the unit loop (it synchronises the workgroup).
-constexpr int RB_LDS = 1024;
+constexpr int RB_LDS = 512;
 struct RbStage {
     int64_t tot[RB_LDS + 1];
     int n;
@@ -727,7 +727,7 @@ __device__ __forceinline__ void k_nbr_build_mixed_unit(int64_t t,
     const int* __restrict__ ex_start, const int* __restrict__ ex_list, int* __restrict__ order,
     int* __restrict__ slot_of, int* __restrict__ nn, int* __restrict__ nbr,
     const double4* __restrict__ pos4s, const float4* __restrict__ pos4f,
-    int (*tab_base)[NB_CELLS], int (*tab_pre)[NB_CELLS + 1], int (*tab_off)[NB_CELLS])
+    int (*tab_base)[NB_CELLS], int (*tab_pre)[NB_CELLS + 1], float4 (*tab_sh)[NB_CELLS])
 {
     const int lane = threadIdx.x % TPA;
     if (t < 0 || t >= nslots) return;  // whole waves (compact ranges are multiples of 64)
@@ -789,7 +789,9 @@ __device__ __forceinline__ void k_nbr_build_mixed_unit(int64_t t,
     const int grp = (threadIdx.x / TPA);
     int* tb = tab_base[grp];
     int* tp = tab_pre[grp];
-    int* to = tab_off[grp];
+    // per neighbour cell: its offset times the cell widths, minus l_i (x, y,
+    // z), and the offset fields of its list entries (w, as int bits)
+    float4* tsh = tab_sh[grp];
     {
         // lane l takes cells l, l + TPA, ...; all their loads are issued
         // before any count is used
@@ -809,9 +811,11 @@ __device__ __forceinline__ void k_nbr_build_mixed_unit(int64_t t,
                 const int c = S.cell_off + (zz * ny + yy) * nx + xx;
                 tb[jc] = cstart[c];
                 cnt_q[q] = ccount[c];
-                to[jc] = (int)((((uint32_t)ox & 7u) << NBR_SLOT_BITS) |
-                               (((uint32_t)oy & 7u) << (NBR_SLOT_BITS + 3)) |
-                               (((uint32_t)oz & 7u) << (NBR_SLOT_BITS + 6)));
+                const uint32_t code = (((uint32_t)ox & 7u) << NBR_SLOT_BITS) |
+                                      (((uint32_t)oy & 7u) << (NBR_SLOT_BITS + 3)) |
+                                      (((uint32_t)oz & 7u) << (NBR_SLOT_BITS + 6));
+                tsh[jc] = make_float4((float)ox * wx - li.x, (float)oy * wy - li.y,
+                                      (float)oz * wz - li.z, __uint_as_float(code));
             }
         }
         // running counts in scan order: a group scan per round of TPA cells
@@ -833,7 +837,6 @@ __device__ __forceinline__ void k_nbr_build_mixed_unit(int64_t t,
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
     const int ncand = tp[ncb];
-    const float nwx = -li.x, nwy = -li.y, nwz = -li.z;
     // the wave-uniform minimum-image switch selects a loop variant outside
     // the candidate loop
     auto scan = [&](auto wrap_c) -> int {
@@ -850,7 +853,8 @@ __device__ __forceinline__ void k_nbr_build_mixed_unit(int64_t t,
             const int kc = k0 + lane;
             const float4 lj = nxt;
             const int slot = slot_n;
-            const uint32_t offs = kc < ncand ? (uint32_t)to[jc] : 0u;
+            const float4 sh = tsh[kc < ncand ? jc : 0];
+            const uint32_t offs = kc < ncand ? __float_as_uint(sh.w) : 0u;
             // the next pass's candidate is located and its position loaded
             // before this one is tested
             if (kc + TPA < ncand) {
@@ -860,11 +864,7 @@ __device__ __forceinline__ void k_nbr_build_mixed_unit(int64_t t,
             }
             bool hit = false;
             if (kc < ncand && S.atom_off + slot != t) {
-                const float ox = (float)((int)(offs << (32 - NBR_SLOT_BITS - 3)) >> 29);
-                const float oy = (float)((int)(offs << (32 - NBR_SLOT_BITS - 6)) >> 29);
-                const float oz = (float)((int)offs >> 29);
-                float dx = lj.x + (ox * wx + nwx), dy = lj.y + (oy * wy + nwy),
-                      dz = lj.z + (oz * wz + nwz);
+                float dx = lj.x + sh.x, dy = lj.y + sh.y, dz = lj.z + sh.z;
                 if constexpr (WRAP) {
                     dx -= rintf(dx * (float)S.invL[0]) * (float)S.L[0];
                     dy -= rintf(dy * (float)S.invL[1]) * (float)S.L[1];
@@ -920,14 +920,15 @@ __global__ void __launch_bounds__(256) k_nbr_build_mixed(
     const double4* __restrict__ pos4s, const float4* __restrict__ pos4f, RbList R)
 {
     constexpr int GPB = 256 / TPA;  // atoms per workgroup pass
-    __shared__ int tab_base[GPB][NB_CELLS], tab_pre[GPB][NB_CELLS + 1], tab_off[GPB][NB_CELLS];
+    __shared__ int tab_base[GPB][NB_CELLS], tab_pre[GPB][NB_CELLS + 1];
+    __shared__ float4 tab_sh[GPB][NB_CELLS];
     __shared__ RbStage rs;
     R = rb_stage(R, false, rs);
     DMDX_FOR_RB_UNITS(u_, (R.pre[*R.n] + GPB - 1) / GPB)
         k_nbr_build_mixed_unit<TPA>(rb_slot(R, sys, u_ * GPB + threadIdx.x / TPA), sys, st,
                                     slot_sys, nslots, acell, ccount, cstart, cell_atoms,
                                     ex_start, ex_list, order, slot_of, nn, nbr, pos4s, pos4f,
-                                    tab_base, tab_pre, tab_off);
+                                    tab_base, tab_pre, tab_sh);
 }
 
 // one 64-thread workgroup per atom slot: bitonic sort of its row in LDS
