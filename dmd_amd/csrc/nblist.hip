@@ -41,9 +41,12 @@ __device__ __forceinline__ int cells_in_use(const SysDev& S, int d)
     return nu;
 }
 
-__device__ __forceinline__ int cell_coord(double x, double L, int nu)
+// cell of a coordinate along one edge: the fractional coordinate x * (1/L)
+// (a multiplication: the cells only bound where the list build looks, and
+// the list decisions are exact distance tests, so no division is needed)
+__device__ __forceinline__ int cell_coord(double x, double invL, int nu)
 {
-    double f = x / L;
+    double f = x * invL;
     f -= floor(f);
     if (!(f >= 0.0 && f <= 1.0)) f = 0.0;  // non-finite coordinate (a failed system)
     int c = (int)(f * nu);
@@ -72,8 +75,8 @@ __device__ __forceinline__ int rb_find(const int64_t* __restrict__ pre, int n, i
     return lo;
 }
 
-// one 1024-thread workgroup: scans over the systems (per wave with lane
-// shuffles, then over the 16 wave totals).  Three lists: every rebuilding
+// one RB_THREADS-thread workgroup: scans over the systems, RB_THREADS at a
+// time (per wave with lane shuffles, then over the wave totals).  Three lists: every rebuilding
 // system (slots and cells), those binned by the multi-kernel path (slots and
 // cells), and those binned by k_bin_system (one workgroup each).
 struct RbOut {
@@ -83,11 +86,14 @@ struct RbOut {
     int* n;
 };
 
-__global__ void __launch_bounds__(1024) k_rb_list(const SysDev* __restrict__ sys,
-                                                  const SysState* __restrict__ st, int nsys,
-                                                  RbOut all, RbOut multi, RbOut wg)
+constexpr int RB_THREADS = 256;
+
+__global__ void __launch_bounds__(RB_THREADS) k_rb_list(const SysDev* __restrict__ sys,
+                                                        const SysState* __restrict__ st,
+                                                        int nsys, RbOut all, RbOut multi,
+                                                        RbOut wg)
 {
-    constexpr int NW = 1024 / 64, NQ = 7;
+    constexpr int NW = RB_THREADS / 64, NQ = 7;
     // quantities: 0-2 list lengths (all, multi, wg), 3 slots of all, 4 slots
     // of multi, 5 cells of multi, 6 cells of all
     __shared__ int64_t wtot[NQ][NW];
@@ -95,7 +101,7 @@ __global__ void __launch_bounds__(1024) k_rb_list(const SysDev* __restrict__ sys
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     if (t < NQ) base[t] = 0;
     __syncthreads();
-    for (int s0 = 0; s0 < nsys; s0 += 1024) {
+    for (int s0 = 0; s0 < nsys; s0 += RB_THREADS) {
         const int s = s0 + t;
         const bool rb = s < nsys && st[s].rebuild;
         const bool w = rb && sys[s].bin_wg;
@@ -229,9 +235,9 @@ __device__ __forceinline__ void k_cell_count_unit(int64_t g, const SysDev* __res
     const SysDev& S = sys[s];
     if (g - S.atom_off >= S.n) return;
     int nx = cells_in_use(S, 0), ny = cells_in_use(S, 1), nz = cells_in_use(S, 2);
-    int cx = cell_coord(x[g], S.L[0], nx);
-    int cy = cell_coord(y[g], S.L[1], ny);
-    int cz = cell_coord(z[g], S.L[2], nz);
+    int cx = cell_coord(x[g], S.invL[0], nx);
+    int cy = cell_coord(y[g], S.invL[1], ny);
+    int cz = cell_coord(z[g], S.invL[2], nz);
     int c = (cz * ny + cy) * nx + cx;
     acell[g] = c;
     atomicAdd(&ccount[S.cell_off + c], 1);
@@ -479,8 +485,9 @@ __global__ void __launch_bounds__(BIN_THREADS) k_bin_system(
             cl[r] = 0;
             if (g < n) {
                 const int64_t a = a0 + g;
-                const int c = (cell_coord(z[a], S.L[2], nz) * ny + cell_coord(y[a], S.L[1], ny)) *
-                                  nx + cell_coord(x[a], S.L[0], nx);
+                const int c = (cell_coord(z[a], S.invL[2], nz) * ny +
+                               cell_coord(y[a], S.invL[1], ny)) * nx +
+                              cell_coord(x[a], S.invL[0], nx);
                 cl[r] = c;
                 acell[a] = c;
                 atomicAdd(&fill[c], 1);
@@ -1063,7 +1070,7 @@ hipError_t launch_build_lists(const Dev& d, hipStream_t s)
     const RbOut all{d.rb_sys, d.rb_pre, d.rb_cpre, d.rb_n};
     const RbOut multi{d.rbm_sys, d.rbm_pre, d.rbm_cpre, d.rbm_n};
     const RbOut wg{d.rbw_sys, nullptr, nullptr, d.rbw_n};
-    k_rb_list<<<1, 1024, 0, s>>>(d.sys, d.st, d.nsys, all, multi, wg);
+    k_rb_list<<<1, RB_THREADS, 0, s>>>(d.sys, d.st, d.nsys, all, multi, wg);
     if (d.n_bin_wg > 0)
         k_bin_system<<<std::min(d.n_bin_wg, 1024), BIN_THREADS, 0, s>>>(
             d.sys, d.st, d.rbw_sys, d.rbw_n, d.x, d.y, d.z, d.acell, d.ccount, d.cstart,
