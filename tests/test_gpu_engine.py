@@ -311,3 +311,19 @@ def test_mixed_every_pair_shape(kind):
             first = F
         else:
             assert rms_rel(F, first) <= 2e-6, (lanes, width)
+
+
+def test_mixed_chain_forces_after_steps():
+    """After steps on the mixed path, a chain melt's forces are completed by
+    the closing kick (pair forces plus the bonded forces gathered beside the
+    pair kernel): they meet the FP64 bar at the reached positions."""
+    spec = W.chain_melt(2000, seed=4).with_(precision=_abi.MIXED)
+    say(f"case: mixed chain melt {spec.n} atoms, 30 steps; forces vs FP64 oracle")
+    e = engine(spec)
+    e.step(30)
+    ref = O.Sim(spec.with_(precision=_abi.FP64))
+    ref.set_positions(e.positions())
+    F_ref, _ = ref.forces_exact()
+    err = rms_rel(e.forces(), F_ref)
+    say(f"  force rms rel err {err:.3e}")
+    assert err <= 1e-5
