@@ -4,6 +4,12 @@
         kernel table (calls, total, average duration) from a
         `rocprofv3 --kernel-trace --stats` run, as markdown
 
+    python -m dmd_amd.profiling window DB --steps N [--anchor KERNEL]
+        per-kernel time per step over the last N steps of a kernel trace (the
+        span from the N-th last dispatch of the anchor kernel, one per step,
+        to the end of its last one), with each kernel's stream, grid,
+        workgroup size, VGPRs and LDS: the steady state after warmup
+
     python -m dmd_amd.profiling counters DB [--kernel NAME ...]
         mean value per dispatch of every counter of a `rocprofv3 --pmc` run,
         per kernel, as markdown
@@ -37,6 +43,29 @@ def kernel_stats(db: str):
         "select name, total_calls, total_duration, average, percentage from top_kernels "
         "order by total_duration desc").fetchall()
     return [(short(r[0]), int(r[1]), float(r[2]), float(r[3]), float(r[4])) for r in rows]
+
+
+def window_stats(db: str, steps: int, anchor: str = "k_pair_mixed"):
+    """Per-kernel totals over the last `steps` steps: the dispatches that
+    start at or after the `steps`-th last dispatch of `anchor` and end by the
+    end of its last one.  Returns (wall ns, rows), rows sorted by time."""
+    con = sqlite3.connect(db)
+    rows = con.execute(
+        "select name, start, end, stream_id, grid_x, workgroup_x, vgpr_count, lds_size "
+        "from kernels order by start").fetchall()
+    marks = [r for r in rows if short(r[0]) == anchor]
+    if len(marks) < steps:
+        raise SystemExit(f"only {len(marks)} dispatches of {anchor}")
+    t0, t1 = marks[-steps][1], marks[-1][2]
+    agg = {}
+    for r in rows:
+        if r[1] < t0 or r[2] > t1:
+            continue
+        a = agg.setdefault(short(r[0]), [0, 0, r[3], r[4], r[5], r[6], r[7]])
+        a[0] += 1
+        a[1] += r[2] - r[1]
+    out = sorted(((k, *v) for k, v in agg.items()), key=lambda x: -x[2])
+    return t1 - t0, out
 
 
 def counter_means(db: str, counter: str):
@@ -83,6 +112,10 @@ def main(argv=None):
     a = sub.add_parser("stats")
     a.add_argument("db")
     a.add_argument("--top", type=int, default=25)
+    w = sub.add_parser("window")
+    w.add_argument("db")
+    w.add_argument("--steps", type=int, required=True)
+    w.add_argument("--anchor", default="k_pair_mixed")
     c = sub.add_parser("counters")
     c.add_argument("db")
     c.add_argument("--kernel", action="append", default=None)
@@ -91,6 +124,16 @@ def main(argv=None):
     b.add_argument("write_db")
     b.add_argument("--copy-bytes", type=int, default=1 << 28)
     args = ap.parse_args(argv)
+    if args.cmd == "window":
+        wall, rows = window_stats(args.db, args.steps, args.anchor)
+        print(f"{args.steps} steps, {wall / 1e6:.2f} ms, {wall / 1e3 / args.steps:.1f} us per step")
+        print()
+        print("| kernel | calls | us per step | avg us | stream | grid | workgroup | VGPRs | LDS B |")
+        print("|---|---:|---:|---:|---:|---:|---:|---:|---:|")
+        for k, n, ns, stream, grid, wg, vgpr, lds in rows:
+            print(f"| {k} | {n} | {ns / 1e3 / args.steps:.1f} | {ns / 1e3 / n:.1f} | {stream} | "
+                  f"{grid} | {wg} | {vgpr} | {lds} |")
+        return
     if args.cmd == "stats":
         rows = kernel_stats(args.db)
         # the top_kernels view reports durations in microseconds

@@ -231,6 +231,41 @@ def test_both_binning_paths_in_one_batch(precision):
         assert np.array_equal(alone.forces(), e.forces(k)), k
 
 
+@pytest.mark.parametrize("kind", ["argon", "water", "chain", "small_box"])
+def test_mixed_list_build_every_lane_count(kind):
+    """The mixed list build at 8, 16 and 32 lanes per atom lists the
+    oracle's pair set, in rows of the same order: forces are bit-identical
+    across lane counts.  Water uses half-width cells searched +-2; the small
+    box (fewer than three cells per edge) takes a minimum image per
+    candidate."""
+    if kind == "argon":
+        spec = jittered_argon(skin=0.1).with_(physics=CO)
+    elif kind == "water":
+        spec = W.water_box(n_side=10, seed=2)
+    elif kind == "chain":
+        spec = W.chain_melt(2000, seed=9)
+    else:
+        spec = jittered_argon(n_cells=3, skin=0.1, seed=5).with_(physics=CO)
+    spec = spec.with_(precision=_abi.MIXED)
+    say(f"case: mixed list build lanes 8/16/32, {kind} ({spec.n} atoms)")
+    rl = max(spec.lj_cutoff, spec.coulomb_cutoff if spec.coulomb_type else 0.0) + spec.list_skin
+    oi, oj = O.pairs(spec.pos, spec.box, rl, "cells")
+    if spec.exclusions is not None:
+        oi, oj = O.filter_excl(oi, oj, *spec.exclusions)
+    first = None
+    for lanes in (8, 16, 32):
+        e = engine(spec)
+        e.set_tuning(build_lanes=lanes)
+        e.compute()
+        gi, gj = e.pairs()
+        assert np.array_equal(gi, oi) and np.array_equal(gj, oj), lanes
+        F = e.forces()
+        if first is None:
+            first = F
+        else:
+            assert np.array_equal(F, first), lanes
+
+
 def test_mixed_batch_alone_equals_in_mix():
     """Mixed precision: an LJ system batched with a water system (real-space
     Coulomb pairs) and a chain melt gives the bits it gives alone -- each
