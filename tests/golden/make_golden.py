@@ -1,0 +1,214 @@
+"""Write tests/golden/dmd_ref_vectors.npz: input/output vectors of DMD's own
+translation units, for tests/test_golden_ref.py.
+
+    python tests/golden/make_golden.py
+
+Needs oracle/_ref/libdmdref.so, which oracle/build_ref.sh compiles from the
+reference sources (src/force, src/integrate, src/thermostat, src/barostat,
+src/core) where they lie, with the standard library only.  The vectors are
+what those units return for seeded inputs; the inputs are stored with them,
+so the test needs neither the reference nor the generators that made them.
+
+Cases (reference files the units come from):
+  lj_*          LennardJones::compute over its rebuilt list
+                (src/force/lennard_jones.cpp:18-92)
+  coul_*, excl_* CoulombDirect with rebuild_pair_list, CoulombExclusion
+                (src/force/coulomb_direct.cpp:14-86, coulomb_exclusion.cpp:16-56)
+  bond_*, angle_*, dih_*, imp_*  the bonded components, FAITHFUL arithmetic
+                (src/force/harmonic_bond.cpp, harmonic_angle.cpp,
+                periodic_dihedral.cpp, harmonic_improper.cpp)
+  integ<k>_*    half kick, drift, Berendsen, Nose-Hoover and the barostats as
+                one operation each (src/integrate/integrator.cpp,
+                src/thermostat/*.cpp, src/barostat/*.cpp)
+  red_*         kinetic energy, temperature, virial, pressure
+  step<t><b>_*  SimulationEngine::step sequences (60 steps) of a small argon
+                system with thermostat t and barostat b
+  chain_*       a FAITHFUL bead-spring chain melt through 20 steps
+  shake_*, rattle_*  Constraints::apply_shake / apply_rattle
+                (src/integrate/constraints.cpp:9-91)
+  and_*         AndersenThermostat::apply (src/thermostat/andersen.cpp:14-32)
+  water_*       rigid water with SHAKE/RATTLE through 30 steps
+"""
+import ctypes as C
+import math
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+import oracle as O  # noqa: E402
+from dmd_amd import _abi  # noqa: E402
+from dmd_amd import workloads as W  # noqa: E402
+
+OUT = os.path.join(HERE, "dmd_ref_vectors.npz")
+
+
+def ref_call(fn, pos, box, *args):
+    x, y, z = O.xyz(pos)
+    n = len(x)
+    fx, fy, fz = O.zeros3(n)
+    e = np.zeros(1)
+    b = O.f64(box)
+    fn(C.c_int64(n), O.P(x), O.P(y), O.P(z), O.P(b), *args, O.P(fx), O.P(fy), O.P(fz), O.P(e))
+    return np.stack([fx, fy, fz], 1), e[0]
+
+
+def rand_sys(seed, n, L):
+    rng = np.random.default_rng(seed)
+    pos = rng.uniform(-0.5 * L, 1.5 * L, (n, 3))  # unwrapped
+    return rng, pos, np.array([L, L * 1.05, L * 0.97])
+
+
+def main():
+    if not O.ref_available():
+        raise SystemExit("oracle/_ref/libdmdref.so is not built (oracle/build_ref.sh)")
+    R = O.ref()
+    v = {}
+
+    # Lennard-Jones, two types
+    rng, pos, box = rand_sys(0, 300, 3.0)
+    nt = 2
+    sig = O.f64(rng.uniform(0.25, 0.35, (nt, nt))).ravel().copy()
+    eps = O.f64(rng.uniform(0.2, 1.0, (nt, nt))).ravel().copy()
+    types = O.i32(rng.integers(0, nt, len(pos)))
+    F, E = ref_call(R.ref_lj, pos, box, C.c_double(0.9), C.c_int32(nt), O.P(sig), O.P(eps),
+                    O.P(types))
+    v.update(lj_pos=pos, lj_box=box, lj_rc=0.9, lj_sig=sig.reshape(nt, nt),
+             lj_eps=eps.reshape(nt, nt), lj_types=types, lj_F=F, lj_E=E)
+
+    # real-space Coulomb (list rebuilt) and the exclusion correction
+    rng, pos, box = rand_sys(3, 200, 3.0)
+    q = O.f64(rng.uniform(-1, 1, len(pos)))
+    F, E = ref_call(R.ref_coulomb_direct, pos, box, C.c_double(1.0), C.c_double(3.1), O.P(q),
+                    C.c_int(1))
+    v.update(coul_pos=pos, coul_box=box, coul_q=q, coul_rc=1.0, coul_alpha=3.1, coul_F=F,
+             coul_E=E)
+    ei = O.i32(rng.integers(0, len(pos), 50))
+    ej = O.i32((ei + 1 + rng.integers(0, 5, 50)) % len(pos))
+    F, E = ref_call(R.ref_exclusion, pos, box, O.P(q), C.c_int64(len(ei)), O.P(ei), O.P(ej))
+    v.update(excl_i=ei, excl_j=ej, excl_F=F, excl_E=E)
+
+    # bonded terms
+    rng, pos, box = rand_sys(4, 40, 2.0)
+    n, nb = len(pos), 60
+    bi, bj = O.i32(rng.integers(0, n, nb)), O.i32(rng.integers(0, n, nb))
+    bj = O.i32(np.where(bi == bj, (bj + 1) % n, bj))
+    bk, br = O.f64(rng.uniform(100, 1000, nb)), O.f64(rng.uniform(0.1, 0.3, nb))
+    F, E = ref_call(R.ref_bonds, pos, box, C.c_int64(nb), O.P(bi), O.P(bj), O.P(bk), O.P(br))
+    v.update(bnd_pos=pos, bnd_box=box, bond_i=bi, bond_j=bj, bond_k=bk, bond_r0=br, bond_F=F,
+             bond_E=E)
+    trip = np.array([rng.permutation(n)[:3] for _ in range(nb)], np.int32)
+    ai, aj, ak = (O.i32(trip[:, c]) for c in range(3))
+    kt, t0 = O.f64(rng.uniform(50, 400, nb)), O.f64(rng.uniform(1.5, 2.5, nb))
+    F, E = ref_call(R.ref_angles, pos, box, C.c_int64(nb), O.P(ai), O.P(aj), O.P(ak), O.P(kt),
+                    O.P(t0))
+    v.update(angle_ijk=trip, angle_k=kt, angle_t0=t0, angle_F=F, angle_E=E)
+    quad = np.array([rng.permutation(n)[:4] for _ in range(nb)], np.int32)
+    di, dj, dk, dl = (O.i32(quad[:, c]) for c in range(4))
+    per = O.i32(rng.integers(1, 4, nb))
+    kp, p0 = O.f64(rng.uniform(1, 10, nb)), O.f64(rng.uniform(-math.pi, math.pi, nb))
+    F, E = ref_call(R.ref_dihedrals, pos, box, C.c_int64(nb), O.P(di), O.P(dj), O.P(dk),
+                    O.P(dl), O.P(per), O.P(kp), O.P(p0))
+    v.update(dih_ijkl=quad, dih_n=per, dih_k=kp, dih_p0=p0, dih_F=F, dih_E=E)
+    F, E = ref_call(R.ref_impropers, pos, box, C.c_int64(nb), O.P(di), O.P(dj), O.P(dk),
+                    O.P(dl), O.P(kp), O.P(p0))
+    v.update(imp_F=F, imp_E=E)
+
+    # integration, thermostat and barostat operations
+    for kind in range(6):
+        rng = np.random.default_rng(5 + kind)
+        n = 77
+        m = rng.uniform(1, 40, n)
+        pos = rng.uniform(0, 3, (n, 3))
+        vel = rng.standard_normal((n, 3))
+        f = rng.standard_normal((n, 3)) * 100
+        out = O.integrate_op(kind, m, pos, vel, f, (3.0, 3.1, 3.2), 0.002, 250.0, 0.1, 4.5e-5,
+                             0.05, use_ref=True)
+        v.update({f"integ{kind}_m": m, f"integ{kind}_pos": pos, f"integ{kind}_vel": vel,
+                  f"integ{kind}_f": f, f"integ{kind}_out0": out[0], f"integ{kind}_out1": out[1],
+                  f"integ{kind}_out2": np.asarray(out[2]), f"integ{kind}_out3": out[3]})
+
+    # scalar reductions
+    rng = np.random.default_rng(12)
+    n = 123
+    m = O.f64(rng.uniform(1, 40, n))
+    vel = rng.standard_normal((n, 3))
+    vx, vy, vz = O.xyz(vel)
+    ke = R.ref_kinetic_energy(C.c_int64(n), O.P(m), O.P(vx), O.P(vy), O.P(vz))
+    pos = rng.uniform(0, 3, (n, 3))
+    f = rng.standard_normal((n, 3))
+    x, y, z = O.xyz(pos)
+    fx, fy, fz = O.xyz(f)
+    v.update(red_m=m, red_vel=vel, red_ke=ke,
+             red_T=R.ref_temperature(C.c_double(ke), C.c_int64(n)), red_pos=pos, red_f=f,
+             red_virial=R.ref_virial(C.c_int64(n), O.P(x), O.P(y), O.P(z), O.P(fx), O.P(fy),
+                                     O.P(fz)),
+             red_pressure=R.ref_pressure(C.c_double(1.5), C.c_double(2.5), C.c_double(27.0)))
+
+    # step sequences of a small argon system
+    for th, ba in ((0, 0), (1, 0), (2, 0), (0, 1), (2, 2)):
+        spec = W.argon_fcc(n_cells=3, seed=7).with_(
+            thermostat=th, thermo_temperature=90.0, thermo_tau=0.1, barostat=ba,
+            baro_pressure=1.0, baro_tau=1.0, lj_cutoff=0.75)
+        b = O.RefSim(spec)
+        b.step(60)
+        st = b.state()
+        p = f"step{th}{ba}_"
+        v.update({p + "pos0": spec.pos, p + "vel0": spec.vel, p + "pos": st["pos"],
+                  p + "vel": st["vel"], p + "force": st["force"], p + "pe": st["pe"],
+                  p + "box_x": st["box_x"], p + "counter": st["counter"]})
+
+    # FAITHFUL chain melt through the step
+    spec = W.chain_melt(120, seed=3).with_(physics=_abi.FAITHFUL, thermostat=1, list_skin=0.0)
+    b = O.RefSim(spec)
+    b.step(20)
+    st = b.state()
+    v.update(chain_pos0=spec.pos, chain_vel0=spec.vel, chain_pos=st["pos"],
+             chain_vel=st["vel"], chain_force=st["force"])
+
+    # SHAKE / RATTLE on a 40-bead constrained chain
+    rng = np.random.default_rng(5)
+    n = 40
+    pos = np.cumsum(rng.normal(0, 0.1, (n, 3)), 0)
+    ci, cj = np.arange(n - 1), np.arange(1, n)
+    cd = np.full(n - 1, 0.15)
+    m = rng.uniform(1.0, 16.0, n)
+    vel = rng.normal(0, 0.5, (n, 3))
+    big = (1e3, 1e3, 1e3)
+    v.update(shake_pos=pos, shake_m=m, shake_ci=ci, shake_cj=cj, shake_cd=cd, shake_vel=vel)
+    for tol in (1e-4, 1e-14):
+        sp = O.shake(pos, big, m, ci, cj, cd, tol, use_ref=True)
+        v[f"shake_out_{tol:g}"] = sp
+        v[f"rattle_out_{tol:g}"] = O.rattle(sp, vel, big, m, ci, cj, tol, use_ref=True)
+
+    # Andersen collisions
+    for freq, calls in ((500.0, 0), (25.0, 3)):
+        rng = np.random.default_rng(int(freq) + calls)
+        n = 1000
+        m = rng.choice([1.008, 15.999, 39.948], n)
+        vel = rng.normal(0, 0.5, (n, 3))
+        p = f"and{int(freq)}_{calls}_"
+        v.update({p + "m": m, p + "vel": vel,
+                  p + "out": O.andersen(m, vel, 0.002, 300.0, freq, calls=calls, use_ref=True)})
+
+    # rigid water with SHAKE and RATTLE through the step
+    spec = W.water_box(n_mol=64, seed=11, rigid=True, dt=0.002).with_(
+        physics=_abi.FAITHFUL, coulomb_type=_abi.COULOMB_NONE, charges=None,
+        exclusions=None, list_skin=0.0, rebuild_interval=10, thermostat=_abi.THERMO_BERENDSEN)
+    b = O.RefSim(spec)
+    b.step(30)
+    st = b.state()
+    v.update(water_pos0=spec.pos, water_vel0=spec.vel, water_pos=st["pos"],
+             water_vel=st["vel"], water_force=st["force"])
+
+    arrays = {k: np.asarray(val) for k, val in v.items()}
+    np.savez_compressed(OUT, **arrays)
+    print(f"wrote {OUT}: {len(arrays)} arrays, {os.path.getsize(OUT)} bytes")
+
+
+if __name__ == "__main__":
+    main()
