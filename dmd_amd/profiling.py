@@ -10,11 +10,11 @@
         to the end of its last one), with each kernel's stream, grid,
         workgroup size, VGPRs and LDS: the steady state after warmup
 
-    python -m dmd_amd.profiling counters DB [--kernel NAME ...]
+    python -m dmd_amd.profiling counters DB [--kernel NAME ...] [--last N]
         mean value per dispatch of every counter of a `rocprofv3 --pmc` run,
         per kernel, as markdown
 
-    python -m dmd_amd.profiling traffic FETCH_DB WRITE_DB --copy-bytes B
+    python -m dmd_amd.profiling traffic FETCH_DB WRITE_DB --copy-bytes B [--last N]
         HBM bytes per dispatch per kernel from two `rocprofv3 --pmc` runs
         (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch), each scaled by what
         the same counter reads for k_copy_calibration, a plain copy kernel
@@ -68,28 +68,40 @@ def window_stats(db: str, steps: int, anchor: str = "k_pair_mixed"):
     return t1 - t0, out
 
 
-def counter_means(db: str, counter: str):
+def _counters(last):
+    """The counters view, or only each kernel's `last` dispatches (the
+    steady state after a warmup) when `last` is set."""
+    if not last:
+        return "counters_collection", ()
+    return ("(select * from (select *, row_number() over (partition by kernel_name, "
+            "counter_name order by dispatch_id desc) as rn_ from counters_collection) "
+            "where rn_ <= ?)", (int(last),))
+
+
+def counter_means(db: str, counter: str, last=None):
     con = sqlite3.connect(db)
+    src, a = _counters(last)
     rows = con.execute(
-        "select kernel_name, avg(value), count(*) from counters_collection "
-        "where counter_name = ? group by kernel_name", (counter,)).fetchall()
+        f"select kernel_name, avg(value), count(*) from {src} "
+        "where counter_name = ? group by kernel_name", (*a, counter)).fetchall()
     return {short(r[0]): (float(r[1]), int(r[2])) for r in rows}
 
 
-def all_counter_means(db: str):
+def all_counter_means(db: str, last=None):
     con = sqlite3.connect(db)
+    src, a = _counters(last)
     rows = con.execute(
-        "select kernel_name, counter_name, avg(value) from counters_collection "
-        "group by kernel_name, counter_name").fetchall()
+        f"select kernel_name, counter_name, avg(value) from {src} "
+        "group by kernel_name, counter_name", a).fetchall()
     out = {}
     for k, c, v in rows:
         out.setdefault(short(k), {})[c] = float(v)
     return out
 
 
-def traffic(fetch_db: str, write_db: str, copy_bytes: int):
-    f = counter_means(fetch_db, "FETCH_SIZE")
-    w = counter_means(write_db, "WRITE_SIZE")
+def traffic(fetch_db: str, write_db: str, copy_bytes: int, last=None):
+    f = counter_means(fetch_db, "FETCH_SIZE", last)
+    w = counter_means(write_db, "WRITE_SIZE", last)
     cf = f["k_copy_calibration"][0]
     cw = w["k_copy_calibration"][0]
     sf = copy_bytes / 1024.0 / cf
@@ -119,10 +131,12 @@ def main(argv=None):
     c = sub.add_parser("counters")
     c.add_argument("db")
     c.add_argument("--kernel", action="append", default=None)
+    c.add_argument("--last", type=int, default=None, help="each kernel's last N dispatches")
     b = sub.add_parser("traffic")
     b.add_argument("fetch_db")
     b.add_argument("write_db")
     b.add_argument("--copy-bytes", type=int, default=1 << 28)
+    b.add_argument("--last", type=int, default=None, help="each kernel's last N dispatches")
     args = ap.parse_args(argv)
     if args.cmd == "window":
         wall, rows = window_stats(args.db, args.steps, args.anchor)
@@ -142,7 +156,7 @@ def main(argv=None):
         for name, calls, tot, avg, pct in rows[: args.top]:
             print(f"| {name} | {calls} | {tot / 1e3:.3f} | {avg:.1f} | {pct:.2f} |")
     elif args.cmd == "counters":
-        m = all_counter_means(args.db)
+        m = all_counter_means(args.db, args.last)
         names = sorted({c for v in m.values() for c in v})
         print("| kernel | " + " | ".join(names) + " |")
         print("|---|" + "---:|" * len(names))
@@ -151,7 +165,8 @@ def main(argv=None):
                 continue
             print(f"| {k} | " + " | ".join(f"{m[k].get(c, float('nan')):.4g}" for c in names) + " |")
     else:
-        print(json.dumps(traffic(args.fetch_db, args.write_db, args.copy_bytes), indent=1))
+        print(json.dumps(traffic(args.fetch_db, args.write_db, args.copy_bytes, args.last),
+                         indent=1))
 
 
 if __name__ == "__main__":
