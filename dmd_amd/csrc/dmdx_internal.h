@@ -151,6 +151,11 @@ constexpr int BIN_WG_CELLS = 4096;
 
 constexpr int NBR_SLOT_BITS = 23;
 constexpr uint32_t NBR_SLOT_MASK = (1u << NBR_SLOT_BITS) - 1u;
+// Mixed path: rows are padded to a multiple of 8 entries with this offset
+// code (an x offset of 3 cells, which no neighbour cell has) over the atom's
+// own slot; the pair kernel's shift table maps it to a shift far beyond any
+// cutoff, so a padding entry adds exactly zero.
+constexpr uint32_t NBR_PAD_CODE = 3u;
 
 // Charges are spread onto the PME grid as 64-bit fixed-point integers with
 // 2^44 units per e: integer atomic adds are associative, so the grid (and

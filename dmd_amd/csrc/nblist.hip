@@ -915,6 +915,10 @@ __device__ __forceinline__ void k_nbr_build_mixed_unit(int64_t t,
         k = M;
     }
     if (lane == 0) nn[t] = k;
+    // pad the row to a multiple of 8 entries (M is one), so the pair kernel's
+    // 8-entry vector loads past the row end read entries that add zero
+    const uint32_t pad = (uint32_t)lt | (NBR_PAD_CODE << NBR_SLOT_BITS);
+    for (int p = k + lane; p < ((k + 7) & ~7); p += TPA) row[p] = (int)pad;
 }
 
 template <int TPA>

@@ -181,6 +181,9 @@ __device__ __forceinline__ void k_pair_mixed_unit(int64_t unit_,
                                   (((uint32_t)oz & 7u) << 6);
             tab[code] = make_float4((float)ox * wx, (float)oy * wy, (float)oz * wz, 0.0f);
         }
+        // row padding (the atom itself, 1e15 nm away): r2 = 1e30, outside
+        // every cutoff, and fs * dx = 0 exactly
+        if ((threadIdx.x & 63) == 0) tab[NBR_PAD_CODE] = make_float4(1e15f, 0.0f, 0.0f, 0.0f);
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
         tab_sys = s;
@@ -227,6 +230,13 @@ __device__ __forceinline__ void k_pair_mixed_unit(int64_t unit_,
         // W consecutive entries W l .. W l + W - 1 with one or two vector
         // loads (rows are 16-byte aligned, M a multiple of 8).
         constexpr bool STRIDED = COUL;
+        // Blocked lanes without the minimum image need no per-entry test of
+        // the row end: a lane's W-entry pass (W = 2, 4, 8) starts at a
+        // multiple of W below cnt, and the list build pads rows to a
+        // multiple of 8 with entries that add zero.  (Strided lanes reach
+        // past that padding; the minimum image would fold the padding's
+        // shift back into the box.)
+        constexpr bool CHECK_END = STRIDED || WRAP;
         auto entry_index = [&](int k0, int q) {
             return STRIDED ? k0 + lane + q * TPA : k0 + lane * W + q;
         };
@@ -273,8 +283,8 @@ __device__ __forceinline__ void k_pair_mixed_unit(int64_t unit_,
             int tj[W];
 #pragma unroll
             for (int q = 0; q < W; ++q) {
-                const uint32_t j =
-                    (entry_index(k0, q) < cnt) ? (ev[q] & NBR_SLOT_MASK) : (uint32_t)lt;
+                const uint32_t j = (!CHECK_END || entry_index(k0, q) < cnt)
+                                       ? (ev[q] & NBR_SLOT_MASK) : (uint32_t)lt;
                 pj[q] = lps[j];
                 tj[q] = TYPED ? tys[j] : 0;
             }
@@ -288,7 +298,7 @@ __device__ __forceinline__ void k_pair_mixed_unit(int64_t unit_,
             // contributes exactly zero through a select.
 #pragma unroll
             for (int q = 0; q < W; ++q) {
-                const bool valid = entry_index(k0, q) < cnt;
+                const bool valid = !CHECK_END || entry_index(k0, q) < cnt;
                 const uint32_t e = ev[q];
                 // the neighbour cell's offset times the cell widths, from the
                 // wave's table (the entry's top nine bits index it)
