@@ -252,6 +252,7 @@ def test_mixed_list_build_every_lane_count(kind):
     oi, oj = O.pairs(spec.pos, spec.box, rl, "cells")
     if spec.exclusions is not None:
         oi, oj = O.filter_excl(oi, oj, *spec.exclusions)
+    F_ref, _ = O.Sim(spec.with_(precision=_abi.FP64)).forces_exact()
     first = None
     for lanes in (8, 16, 32):
         e = engine(spec)
@@ -262,6 +263,9 @@ def test_mixed_list_build_every_lane_count(kind):
         F = e.forces()
         if first is None:
             first = F
+            err = rms_rel(F, F_ref)
+            say(f"  force rms rel err vs FP64 {err:.3e}")
+            assert err <= 1e-5
         else:
             assert np.array_equal(F, first), lanes
 
