@@ -1,6 +1,6 @@
 """Accuracy of the mixed-precision path against the FP64 path.
 
-    python -m dmd_amd.accuracy [--which c0,c1,c2] [--steps N] [--out FILE]
+    python -m dmd_amd.accuracy [--which c0,c1,c2] [--steps N] [--ensemble R] [--out FILE]
 
 For each workload, from a configuration relaxed by 200 FP64 steps: force
 RMS relative error and potential-energy error of the mixed path against
@@ -8,7 +8,11 @@ the FP64 path on identical positions, and the NVE energy drift (slope of a
 least-squares line through E_pot + E_kin, in kJ/mol/ns/atom) of both paths
 over the same run from the same start, plus the drift of a second FP64 run
 from positions moved by 1e-9 nm (the statistical noise floor of comparing
-drifts of decorrelating trajectories).  The
+drifts of decorrelating trajectories).  With --ensemble R, both precisions
+also run from R starts moved by independent 1e-9 nm perturbations: the mean
+and standard deviation of each precision's drift, and the difference of the
+means with its standard error, separate a change of precision from the
+spread of chaotic trajectories.  The
 FP64 device path is the reference here because it reproduces the FP64 CPU
 oracle (DMD's own arithmetic) to 1e-10 in forces; running the CPU oracle for
 10^4 steps of 10^5 atoms is not practical.
@@ -101,6 +105,8 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=0,
                     help="NVE steps (0: the workload's own run length)")
     ap.add_argument("--every", type=int, default=20)
+    ap.add_argument("--ensemble", type=int, default=0,
+                    help="perturbed starts per precision for drift statistics")
     ap.add_argument("--out", default="")
     args = ap.parse_args(argv)
     own = {"c0": 1000, "c1": 2000, "c2": 10000}
@@ -123,6 +129,21 @@ def main(argv=None):
         d0 = r["nve_fp64"]["drift_kj_mol_ns_atom"]
         r["drift_difference"] = r["nve_mixed"]["drift_kj_mol_ns_atom"] - d0
         r["drift_noise_floor"] = r["nve_fp64_perturbed"]["drift_kj_mol_ns_atom"] - d0
+        if args.ensemble > 1:
+            ens = {}
+            for prec, label in ((_abi.FP64, "fp64"), (_abi.MIXED, "mixed")):
+                dr = []
+                for k in range(args.ensemble):
+                    g = np.random.default_rng(100 + k)
+                    sk = spec.with_(pos=spec.pos + g.uniform(-1e-9, 1e-9, spec.pos.shape))
+                    dr.append(nve_drift(sk, steps, args.every, prec)["drift_kj_mol_ns_atom"])
+                ens[label] = {"drifts": dr, "mean": float(np.mean(dr)),
+                              "std": float(np.std(dr, ddof=1))}
+            n = args.ensemble
+            se = float(np.sqrt(ens["fp64"]["std"] ** 2 / n + ens["mixed"]["std"] ** 2 / n))
+            ens["mean_difference"] = ens["mixed"]["mean"] - ens["fp64"]["mean"]
+            ens["difference_stderr"] = se
+            r["drift_ensemble"] = ens
         rows[name] = r
         print(json.dumps({name: r}), file=sys.stderr, flush=True)
     out = json.dumps(rows, indent=1)
