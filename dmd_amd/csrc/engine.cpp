@@ -1220,7 +1220,15 @@ int Batch::adopt(Batch& src, int ss, int ds)
     SysDev A, B;
     TRY(src.copy(&A, src.d.sys + ss, sizeof(SysDev), hipMemcpyDeviceToHost));
     TRY(copy(&B, d.sys + ds, sizeof(SysDev), hipMemcpyDeviceToHost));
-    if (A.n != B.n || A.n_pad != B.n_pad || A.max_nbr != B.max_nbr || A.has_list != B.has_list)
+    // the copies below move whole per-system regions (list rows, cells,
+    // frames): both systems must have the same layout of every one of them
+    bool same = A.n == B.n && A.n_pad == B.n_pad && A.max_nbr == B.max_nbr &&
+                A.has_list == B.has_list && A.cdiv == B.cdiv && A.bin_wg == B.bin_wg &&
+                A.pme == B.pme && A.precision == B.precision && A.n_bond == B.n_bond &&
+                A.n_angle == B.n_angle && A.n_dih == B.n_dih && A.n_imp == B.n_imp &&
+                A.n_ex == B.n_ex && A.n_con == B.n_con;
+    for (int k = 0; k < 3; ++k) same = same && A.nc[k] == B.nc[k] && A.K[k] == B.K[k];
+    if (!same)
         return fail(DMDX_E_MISMATCH, "systems were not created from the same description");
     for (int k = 0; k < 3; ++k) {  // a barostat may have changed the box
         B.L[k] = A.L[k];
@@ -1436,6 +1444,13 @@ int dmdx_engine_set_active(dmdx_engine* e, int32_t sys, int32_t active)
 {
     if (!e) return dmdx::fail(DMDX_E_INVALID, "null engine");
     return e->b.set_active(sys, active);
+}
+
+int dmdx_engine_adopt(dmdx_engine* dst, int32_t dst_sys, dmdx_engine* src, int32_t src_sys)
+{
+    if (!dst || !src) return dmdx::fail(DMDX_E_INVALID, "null engine");
+    if (dst == src) return dmdx::fail(DMDX_E_INVALID, "source and destination are one engine");
+    return dst->b.adopt(src->b, src_sys, dst_sys);
 }
 
 int dmdx_engine_get_pairs(dmdx_engine* e, int32_t sys, int32_t* pi, int32_t* pj, int64_t cap,

@@ -111,7 +111,7 @@ struct dmdx_server {
     int64_t budget = 0, in_use = 0;
     double est_scale = 1.25;
     Clock::time_point t0;
-    int64_t rounds = 0, retired = 0, failed = 0, cohorts_made = 0, merges = 0;
+    int64_t rounds = 0, retired = 0, failed = 0, cohorts_made = 0, merges = 0, cancelled = 0;
     int64_t atom_steps = 0;  // atoms x steps advanced over all requests
     std::vector<double> lat_ms;
     std::vector<int64_t> lat_w;
@@ -121,8 +121,9 @@ struct dmdx_server {
 
 namespace {
 
-int retire(dmdx_server* srv, Cohort& c, Request& r, bool failed)
+int retire(dmdx_server* srv, Cohort& c, Request& r, int32_t state)
 {
+    const bool failed = state == DMDX_REQ_FAILED;
     Batch& b = *c.b;
     const int s = r.sys;
     const int64_t n = r.desc->d.n_atoms;
@@ -148,7 +149,8 @@ int retire(dmdx_server* srv, Cohort& c, Request& r, bool failed)
     r.sc.flags = T.flags;
     r.sc.steps_done = T.steps_done;
     r.sc.list_builds = T.rebuilds;
-    r.state = failed ? DMDX_REQ_FAILED : DMDX_REQ_DONE;
+    r.state = state;
+    r.steps_done = T.steps_done;
     if (failed)
         r.error = (T.flags & dmdx::FLAG_NBR_OVERFLOW) ? "neighbour list overflow"
                                                       : "non-finite energy";
@@ -156,9 +158,25 @@ int retire(dmdx_server* srv, Cohort& c, Request& r, bool failed)
     r.desc.reset();
     r.cohort = -1;
     c.live -= 1;
-    srv->retired += 1;
-    if (failed) srv->failed += 1;
+    if (state == DMDX_REQ_CANCELLED) {
+        srv->cancelled += 1;
+    } else {
+        srv->retired += 1;
+        if (failed) srv->failed += 1;
+    }
     return DMDX_OK;
+}
+
+// Device memory of a cohort's systems that no longer run, by estimate.
+int64_t dead_bytes(const dmdx_server* srv, const Cohort& c)
+{
+    int64_t dead = 0;
+    for (int64_t id : c.ids) {
+        auto it = srv->reqs.find(id);
+        if (it == srv->reqs.end() || it->second.state != DMDX_REQ_RUNNING)
+            dead += it == srv->reqs.end() ? 0 : it->second.est_bytes;
+    }
+    return dead;
 }
 
 constexpr int COHORT_OVER_BUDGET = 1;
@@ -230,7 +248,9 @@ int make_cohort(dmdx_server* srv, const std::vector<int64_t>& ids, std::vector<i
 
 // Merge every cohort of each precision into one batch: a new batch is built
 // from the live requests' descriptions and each system continues there with
-// its whole state (Batch::adopt).  Skipped when the new batch would not fit
+// its whole state (Batch::adopt).  A lone cohort is rebuilt the same way
+// (compacted) once systems that no longer run -- retired or cancelled --
+// hold a quarter of its memory.  Skipped when the new batch would not fit
 // next to the old ones.
 int merge(dmdx_server* srv)
 {
@@ -238,7 +258,16 @@ int merge(dmdx_server* srv)
         std::vector<int> cs;
         for (size_t ci = 0; ci < srv->cohorts.size(); ++ci)
             if (srv->cohorts[ci]->b->d.precision == prec) cs.push_back((int)ci);
-        if (cs.size() < 2) continue;
+        if (cs.empty()) continue;
+        if (cs.size() == 1) {
+            const Cohort& c = *srv->cohorts[cs[0]];
+            int64_t est = 0;
+            for (int64_t id : c.ids) {
+                auto it = srv->reqs.find(id);
+                if (it != srv->reqs.end()) est += it->second.est_bytes;
+            }
+            if (c.live == 0 || 4 * dead_bytes(srv, c) < est) continue;
+        }
         std::vector<int64_t> ids;
         int64_t need = 0;
         for (int ci : cs)
@@ -470,8 +499,8 @@ int dmdx_server_pump(dmdx_server* srv, int32_t* n_retired)
             }
             r.steps_done = T.steps_done;
             r.flags = T.flags;
-            if (T.flags != 0) rc = retire(srv, c, r, true);
-            else if (r.steps_done >= r.n_steps) rc = retire(srv, c, r, false);
+            if (T.flags != 0) rc = retire(srv, c, r, DMDX_REQ_FAILED);
+            else if (r.steps_done >= r.n_steps) rc = retire(srv, c, r, DMDX_REQ_DONE);
             if (rc) return rc;
         }
     }
@@ -493,6 +522,33 @@ int dmdx_server_pump(dmdx_server* srv, int32_t* n_retired)
     }
     if (n_retired) *n_retired = (int32_t)(srv->retired - before);
     return DMDX_OK;
+}
+
+int dmdx_server_cancel(dmdx_server* srv, int64_t id)
+{
+    if (!srv) return dmdx::fail(DMDX_E_INVALID, "null server");
+    auto it = srv->reqs.find(id);
+    if (it == srv->reqs.end()) return dmdx::fail(DMDX_E_INVALID, "unknown request");
+    Request& r = it->second;
+    if (r.state == DMDX_REQ_QUEUED) {
+        auto p = std::find(srv->pending.begin(), srv->pending.end(), id);
+        if (p != srv->pending.end()) srv->pending.erase(p);
+        r.state = DMDX_REQ_CANCELLED;
+        r.t_done = srv->now();
+        r.desc.reset();
+        srv->cancelled += 1;
+        return DMDX_OK;
+    }
+    if (r.state != DMDX_REQ_RUNNING) return dmdx::fail(DMDX_E_INVALID, "request has finished");
+    // the batch is idle between pumps: switch the system off (it takes no
+    // further step), keep its state as the result; the next merge or
+    // compaction leaves it out, which returns its device memory
+    Cohort& c = *srv->cohorts[r.cohort];
+    int rc = c.b->set_active(r.sys, 0);
+    if (rc) return rc;
+    rc = c.b->fetch_state();
+    if (rc) return rc;
+    return retire(srv, c, r, DMDX_REQ_CANCELLED);
 }
 
 int dmdx_server_request_info(dmdx_server* srv, int64_t id, dmdx_request_info* out)
@@ -521,7 +577,7 @@ int dmdx_server_result(dmdx_server* srv, int64_t id, int32_t what, double* x, do
     auto it = srv->reqs.find(id);
     if (it == srv->reqs.end()) return dmdx::fail(DMDX_E_INVALID, "unknown request");
     const Request& r = it->second;
-    if (r.state != DMDX_REQ_DONE && r.state != DMDX_REQ_FAILED)
+    if (r.state == DMDX_REQ_QUEUED || r.state == DMDX_REQ_RUNNING)
         return dmdx::fail(DMDX_E_BUSY, "request has not finished");
     if (what < 0 || what > 2) return dmdx::fail(DMDX_E_INVALID, "what must be 0, 1 or 2");
     if (r.res[0].empty()) return dmdx::fail(DMDX_E_INVALID, "request produced no state");
@@ -547,7 +603,7 @@ int dmdx_server_release(dmdx_server* srv, int64_t id)
     if (!srv) return dmdx::fail(DMDX_E_INVALID, "null server");
     auto it = srv->reqs.find(id);
     if (it == srv->reqs.end()) return dmdx::fail(DMDX_E_INVALID, "unknown request");
-    if (it->second.state != DMDX_REQ_DONE && it->second.state != DMDX_REQ_FAILED)
+    if (it->second.state == DMDX_REQ_QUEUED || it->second.state == DMDX_REQ_RUNNING)
         return dmdx::fail(DMDX_E_BUSY, "request has not finished");
     srv->reqs.erase(it);
     return DMDX_OK;
@@ -563,6 +619,7 @@ int dmdx_server_get_stats(dmdx_server* srv, dmdx_server_stats* out)
     out->running = running;
     out->retired = srv->retired;
     out->failed = srv->failed;
+    out->cancelled = srv->cancelled;
     out->cohorts = (int64_t)srv->cohorts.size();
     out->cohorts_made = srv->cohorts_made;
     out->merges = srv->merges;

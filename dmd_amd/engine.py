@@ -42,6 +42,7 @@ def _setup(lib):
     lib.dmdx_engine_get_scalars.argtypes = [C.c_void_p, C.c_int32, C.POINTER(Scalars)]
     lib.dmdx_engine_set_step_since_rebuild.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
     lib.dmdx_engine_set_active.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
+    lib.dmdx_engine_adopt.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32]
     lib.dmdx_engine_get_pairs.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_int32),
                                           C.POINTER(C.c_int32), C.c_int64,
                                           C.POINTER(C.c_int64)]
@@ -162,6 +163,11 @@ class Engine:
 
     def set_active(self, sys: int, active: bool):
         _abi.check(self.lib.dmdx_engine_set_active(self.h, sys, int(active)))
+
+    def adopt(self, sys: int, src: "Engine", src_sys: int = 0):
+        """Move system src_sys of `src`, with its whole state, into slot `sys`
+        (both created from the same description; dmdx_engine_adopt)."""
+        _abi.check(self.lib.dmdx_engine_adopt(self.h, int(sys), src.h, int(src_sys)))
 
     def set_tuning(self, pair_lanes: int | None = None, build_lanes: int | None = None,
                    pair_width: int | None = None):

@@ -20,8 +20,9 @@ from . import _abi
 from .engine import Scalars, _dp
 from .system import SystemSpec
 
-QUEUED, RUNNING, DONE, FAILED = 0, 1, 2, 3
-STATE_NAMES = {QUEUED: "queued", RUNNING: "running", DONE: "done", FAILED: "failed"}
+QUEUED, RUNNING, DONE, FAILED, CANCELLED = 0, 1, 2, 3, 4
+STATE_NAMES = {QUEUED: "queued", RUNNING: "running", DONE: "done", FAILED: "failed",
+               CANCELLED: "cancelled"}
 
 
 class ServerOptions(C.Structure):
@@ -49,6 +50,7 @@ class ServerStats(C.Structure):
         ("rounds", C.c_int64), ("merges", C.c_int64), ("bytes_in_use", C.c_int64),
         ("bytes_budget", C.c_int64),
         ("estimate_scale", C.c_double), ("clock_s", C.c_double), ("atom_steps", C.c_int64),
+        ("cancelled", C.c_int64),
     ]
 
 
@@ -64,6 +66,7 @@ def _setup(lib):
     lib.dmdx_server_result.argtypes = [vp, C.c_int64, C.c_int32, _dp, _dp, _dp]
     lib.dmdx_server_result_scalars.argtypes = [vp, C.c_int64, C.POINTER(Scalars)]
     lib.dmdx_server_release.argtypes = [vp, C.c_int64]
+    lib.dmdx_server_cancel.argtypes = [vp, C.c_int64]
     lib.dmdx_server_get_stats.argtypes = [vp, C.POINTER(ServerStats)]
     lib.dmdx_server_step_latency.argtypes = [vp, _dp, C.POINTER(C.c_int64), C.c_int64,
                                              C.POINTER(C.c_int64)]
@@ -134,6 +137,11 @@ class Server:
         out["steps_done"] = s.steps_done
         out["box"] = tuple(s.box)
         return out
+
+    def cancel(self, rid: int):
+        """Stop a queued or running request; a running one keeps the state
+        it reached as its result."""
+        _abi.check(self.lib.dmdx_server_cancel(self.h, int(rid)))
 
     def release(self, rid: int):
         _abi.check(self.lib.dmdx_server_release(self.h, int(rid)))

@@ -223,6 +223,16 @@ int dmdx_engine_set_step_since_rebuild(dmdx_engine *e, int32_t sys, int32_t coun
 /* Exclude (0) or include (1) a system in subsequent steps. */
 int dmdx_engine_set_active(dmdx_engine *e, int32_t sys, int32_t active);
 
+/* Move system src_sys of src, with its whole state (positions, velocities,
+ * forces, list rows, cells, frames, counters, thermostat and barostat
+ * variables, step count), into slot dst_sys of dst, replacing what was there.
+ * Both slots must have been created from the same description (same atoms,
+ * terms, cell grid, list rows and PME grid; DMDX_E_MISMATCH otherwise) in
+ * engines of one precision.  The moved system's later trajectory is the one
+ * it would have had in src.  No counterpart in the reference (one
+ * SimulationEngine owns one system); the server merges cohorts with it. */
+int dmdx_engine_adopt(dmdx_engine *dst, int32_t dst_sys, dmdx_engine *src, int32_t src_sys);
+
 /* Current pair list of a system as the sorted set {(i, j) : i < j} (local
  * atom indices), i.e. the pairs_i_/pairs_j_ of LennardJones
  * (src/force/lennard_jones.h:43).  *count receives the number of pairs; at
@@ -377,7 +387,13 @@ typedef struct dmdx_server_options {
                                       unchanged); 1: never merge             */
 } dmdx_server_options;
 
-enum { DMDX_REQ_QUEUED = 0, DMDX_REQ_RUNNING = 1, DMDX_REQ_DONE = 2, DMDX_REQ_FAILED = 3 };
+enum {
+    DMDX_REQ_QUEUED = 0,
+    DMDX_REQ_RUNNING = 1,
+    DMDX_REQ_DONE = 2,
+    DMDX_REQ_FAILED = 3,
+    DMDX_REQ_CANCELLED = 4
+};
 
 typedef struct dmdx_request_info {
     int32_t state;                 /* DMDX_REQ_*                              */
@@ -396,6 +412,7 @@ typedef struct dmdx_server_stats {
     double estimate_scale;         /* learned actual/estimate memory ratio    */
     double clock_s;                /* server clock now                        */
     int64_t atom_steps;            /* atoms x steps advanced so far           */
+    int64_t cancelled;             /* requests cancelled (not in retired)     */
 } dmdx_server_stats;
 
 int dmdx_server_create(const dmdx_server_options *opt, dmdx_server **out);
@@ -405,6 +422,14 @@ int dmdx_server_submit(dmdx_server *srv, const dmdx_system_desc *desc, int64_t n
                        int64_t *id);
 /* one round: admit, advance every cohort by up to chunk_steps, wait, retire */
 int dmdx_server_pump(dmdx_server *srv, int32_t *n_retired);
+/* Stop a request.  A queued request leaves the queue.  A running request
+ * takes no further step: its state at the end of the last pump (positions,
+ * velocities, forces, scalars, steps_done) is kept as its result, as for a
+ * finished request, and the next pump's merge or compaction of its batch
+ * returns its device memory; the other systems of the batch are untouched.
+ * DMDX_E_INVALID for a request that has already finished.  The reference
+ * stops a run only by ending its loop (src/sim/simulation_engine.cpp:89-106). */
+int dmdx_server_cancel(dmdx_server *srv, int64_t id);
 int dmdx_server_request_info(dmdx_server *srv, int64_t id, dmdx_request_info *out);
 /* final state of a finished request: what 0 positions, 1 velocities, 2 forces */
 int dmdx_server_result(dmdx_server *srv, int64_t id, int32_t what, double *x, double *y,

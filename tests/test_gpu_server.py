@@ -165,3 +165,70 @@ def test_server_merges_keep_results(precision):
     for rid, r in zip(ids, ref):
         assert srv.info(rid)["state"] == "done"
         same(srv.result(rid), r)
+
+
+def test_server_cancel_frees_memory_and_keeps_others():
+    """A queued and a running request are cancelled: the running one keeps
+    the state it reached, its device memory is returned at the next pump,
+    and the other requests' results equal the systems run alone."""
+    from dmd_amd.server import Server
+    reqs = [(W.lj_fluid(1500, 0.80, 1.0, 41, precision=M), 30),
+            (W.lj_fluid(1600, 0.85, 0.9, 42, precision=M), 200),
+            (W.chain_melt(900, 43, precision=M), 25),
+            (W.lj_fluid(900, 0.80, 1.0, 44, precision=M), 20)]
+    say("case: server cancel (one running, one queued)")
+    ref = {k: alone(*reqs[k]) for k in (0, 2)}
+    srv = Server(chunk_steps=5, max_cohorts=1)
+    ids = [srv.submit(s, n) for s, n in reqs[:3]]
+    srv.pump()  # one cohort of three, 5 steps
+    late = srv.submit(*reqs[3])
+    assert srv.info(late)["state"] == "queued"
+    srv.cancel(late)
+    assert srv.info(late)["state"] == "cancelled"
+    before = srv.stats()["bytes_in_use"]
+    srv.cancel(ids[1])
+    info = srv.info(ids[1])
+    assert info["state"] == "cancelled" and info["steps_done"] == 5, info
+    res = srv.result(ids[1])
+    assert np.isfinite(res["pos"]).all() and res["steps_done"] == 5
+    with pytest.raises(Exception):
+        srv.cancel(ids[1])  # finished requests cannot be cancelled
+    srv.pump()  # the cohort is compacted without the cancelled system
+    after = srv.stats()["bytes_in_use"]
+    say(f"  bytes in use {before} -> {after}")
+    assert after < before
+    drain(srv, 2 - srv.stats()["retired"])
+    st = srv.stats()
+    assert st["retired"] == 2 and st["cancelled"] == 2 and st["failed"] == 0
+    for k in (0, 2):
+        same(srv.result(ids[k]), ref[k])
+    assert srv.info(ids[1])["steps_done"] == 5  # it never stepped again
+    srv.close()
+
+
+def test_engine_adopt_moves_state_and_refuses_other_layouts():
+    """dmdx_engine_adopt: a system moved into another engine continues the
+    same trajectory; a slot made from another description is refused."""
+    from dmd_amd.engine import Engine
+    say("case: engine adopt (same description moves, another layout refused)")
+    a_spec = W.lj_fluid(1200, 0.80, 1.0, 51, precision=M)
+    b_spec = W.chain_melt(800, 52, precision=M)
+    src = Engine([a_spec, b_spec])
+    src.step(12)
+    dst = Engine([b_spec, a_spec])
+    dst.adopt(1, src, 0)
+    dst.adopt(0, src, 1)
+    ref = Engine([a_spec, b_spec])
+    ref.step(30)
+    src.step(18)  # the sources continue too: the moved copies must follow them
+    dst.step(18)
+    for k_src, k_dst in ((0, 1), (1, 0)):
+        assert np.array_equal(dst.positions(k_dst), ref.positions(k_src))
+        assert np.array_equal(dst.positions(k_dst), src.positions(k_src))
+    # same atom count, other box: other cells, refused
+    other = a_spec.with_(box=tuple(1.3 * np.asarray(a_spec.box)))
+    bad = Engine([other])
+    with pytest.raises(Exception):
+        bad.adopt(0, src, 0)
+    with pytest.raises(Exception):
+        dst.adopt(0, src, 0)  # different atoms
