@@ -46,6 +46,8 @@ def _setup(lib):
     lib.dmdx_component_set_charges.argtypes = [_cp, _dp, C.c_int64]
     lib.dmdx_component_rebuild_pair_list.argtypes = [_cp, _dp, _dp, _dp, C.c_int64, _dp]
     lib.dmdx_component_compute.argtypes = [_cp, _dp, _dp, _dp, C.c_int64, _dp, _dp, _dp, _dp, _dp]
+    lib.dmdx_component_compute_noalloc.argtypes = [_cp, _cp, _cp, _cp, C.c_int64, _dp, _cp, _cp,
+                                                   _cp, _cp]
     lib.dmdx_component_name.argtypes = [_cp, C.c_char_p, C.c_int64]
     lib.dmdx_component_set_mode.argtypes = [_cp, C.c_int32, C.c_int32]
     lib.dmdx_component_destroy.argtypes = [_cp]
@@ -167,6 +169,15 @@ class ForceComponent:
             self._h, _p(x), _p(y), _p(z), n, _p(_f64(cell.matrix)), _p(forces_x), _p(forces_y),
             _p(forces_z), C.byref(e)))
         return e.value
+
+    def compute_noalloc(self, n: int, x_ptr: int, y_ptr: int, z_ptr: int, cell: Cell,
+                        fx_ptr: int, fy_ptr: int, fz_ptr: int, energy_ptr: int):
+        """dmdx_component_compute_noalloc: the same compute on DEVICE arrays
+        (integer device addresses of n doubles each, and of one double for
+        the energy), with no device allocation by the library."""
+        _abi.check(_lib().dmdx_component_compute_noalloc(
+            self._h, x_ptr, y_ptr, z_ptr, int(n), _p(_f64(cell.matrix)), fx_ptr, fy_ptr, fz_ptr,
+            energy_ptr))
 
     def _rebuild(self, pos_x, pos_y, pos_z, cell):
         x, y, z = _f64(pos_x), _f64(pos_y), _f64(pos_z)

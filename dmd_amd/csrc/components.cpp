@@ -57,16 +57,16 @@ int check_cell(const double cell[9])
     return DMDX_OK;
 }
 
-// (Re)create the one-system batch for n atoms at the given positions/box.
-int ensure_batch(dmdx_component* c, int64_t n, const double* x, const double* y, const double* z,
-                 const double cell[9])
+// One-system batch of the component for n atoms at the given (host)
+// positions and box, with the launch options `opt` (component fields set
+// here).
+int make_batch(dmdx_component* c, int64_t n, const double* x, const double* y, const double* z,
+               const double cell[9], BatchOptions opt, Batch& b)
 {
-    if (c->b && c->n == n) return DMDX_OK;
     if (c->type == dmdx::COMP_LJ && !c->types.empty() && (int64_t)c->types.size() != n)
         return dmdx::fail(DMDX_E_MISMATCH, "atom types were set for a different atom count");
     if (!c->charges.empty() && (int64_t)c->charges.size() != n)
         return dmdx::fail(DMDX_E_MISMATCH, "charges were set for a different atom count");
-    if (c->b) c->counter = c->b->host[0].counter;
     std::vector<double> masses(n, 1.0);
     std::vector<int32_t> types = c->types;
     if (types.empty()) types.assign(n, 0);
@@ -92,7 +92,6 @@ int ensure_batch(dmdx_component* c, int64_t n, const double* x, const double* y,
     D.lj_cutoff = c->cutoff;
     D.coulomb_cutoff = c->cutoff;
     D.ewald_coeff = c->alpha;
-    BatchOptions opt;
     opt.component = c->type;
     opt.reuse_forces = false;
     switch (c->type) {
@@ -154,12 +153,23 @@ int ensure_batch(dmdx_component* c, int64_t n, const double* x, const double* y,
     default:
         return dmdx::fail(DMDX_E_INVALID, "unknown component");
     }
+    return b.create(&D, 1, opt);
+}
+
+// (Re)create the component's own batch for n atoms at the given positions/box.
+int ensure_batch(dmdx_component* c, int64_t n, const double* x, const double* y, const double* z,
+                 const double cell[9])
+{
+    if (c->b && c->n == n) return DMDX_OK;
+    if (c->b) c->counter = c->b->host[0].counter;
     auto b = std::make_unique<Batch>();
-    int rc = b->create(&D, 1, opt);
+    int rc = make_batch(c, n, x, y, z, cell, BatchOptions(), *b);
     if (rc != DMDX_OK) return rc;
     c->b = std::move(b);
     c->n = n;
-    for (int k = 0; k < 3; ++k) c->box[k] = D.box[k];
+    c->box[0] = cell[0];
+    c->box[1] = cell[4];
+    c->box[2] = cell[8];
     return DMDX_OK;
 }
 
@@ -458,6 +468,53 @@ int dmdx_component_compute(dmdx_component* c, const double* x, const double* y, 
         return dmdx::fail(DMDX_E_MISMATCH, "neighbour list overflow");
     *energy_out += T.pe;
     if (!std::isfinite(T.pe)) return dmdx::fail(DMDX_E_NAN, c->name + ": non-finite energy");
+    return DMDX_OK;
+}
+
+// dmdx_component_compute on device arrays with no device allocation: the
+// component's one-system batch is laid out in the device's scratch array
+// for the length of the call (same kernels, same order, so the same bits as
+// dmdx_component_compute).  Components whose compute depends on a pair list
+// kept between calls (LennardJones, CoulombDirect) are refused.
+int dmdx_component_compute_noalloc(dmdx_component* c, const double* x, const double* y,
+                                   const double* z, int64_t n, const double cell[9], double* fx,
+                                   double* fy, double* fz, double* energy_out)
+{
+    if (!c) return dmdx::fail(DMDX_E_INVALID, "null component");
+    if (!x || !y || !z || !fx || !fy || !fz || !energy_out || n <= 0)
+        return dmdx::fail(DMDX_E_INVALID, "null array or empty system");
+    if (c->type == dmdx::COMP_LJ || c->type == dmdx::COMP_COULOMB_DIRECT)
+        return dmdx::fail(DMDX_E_PARM, c->name + " keeps its pair list between calls: it has no "
+                                                 "allocation-free form");
+    int rc = check_cell(cell);
+    if (rc) return rc;
+    dmdx::NoallocDevice* nd = nullptr;
+    rc = dmdx::noalloc_device(&nd);
+    if (rc) return rc;
+    BatchOptions opt;
+    opt.arena = nd->scratch;
+    opt.arena_bytes = nd->bytes;
+    opt.arena_stream = nd->stream;
+    opt.arena_side = nd->side;
+    for (int k = 0; k < 3; ++k) opt.arena_events[k] = nd->events[k];
+    // the batch is filled from host arrays; positions arrive below, on the device
+    std::vector<double> zero((size_t)n, 0.0);
+    Batch b;
+    rc = make_batch(c, n, zero.data(), zero.data(), zero.data(), cell, opt, b);
+    if (rc == DMDX_OK) rc = b.set_arrays(0, 0, x, y, z, true);
+    if (rc == DMDX_OK) rc = b.set_arrays(0, 2, fx, fy, fz, true);
+    if (rc == DMDX_OK) rc = b.compute(1);
+    if (rc == DMDX_OK) rc = b.get_arrays(0, 2, fx, fy, fz, true);
+    if (rc == DMDX_OK) rc = b.fetch_state();
+    if (rc != DMDX_OK) {
+        (void)hipStreamSynchronize(nd->stream);
+        (void)hipStreamSynchronize(nd->side);
+        return rc;
+    }
+    const dmdx::SysState& T = b.hstate[0];
+    if (!std::isfinite(T.pe)) return dmdx::fail(DMDX_E_NAN, c->name + ": non-finite energy");
+    DMDX_HIP(dmdx::launch_add_energy(energy_out, b.d.st, nd->stream));
+    DMDX_HIP(hipStreamSynchronize(nd->stream));
     return DMDX_OK;
 }
 

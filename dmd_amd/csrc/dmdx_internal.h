@@ -326,6 +326,23 @@ hipError_t launch_ke_refresh(const Dev& d, hipStream_t s);
 hipError_t launch_remap_rows(hipStream_t s, int* nbr, int64_t count, int* order, int64_t n_order,
                              int64_t delta);
 
+// allocation-free compute (noalloc.hip): zero `bytes` (a multiple of 4) at p
+// with a fill kernel; add the potential energy of system 0 to *energy
+hipError_t launch_zero_bytes(void* p, size_t bytes, hipStream_t s);
+hipError_t launch_add_energy(double* energy, const SysState* st, hipStream_t s);
+
+// The device-resident scratch array of the allocation-free entry point on
+// the current device, with the streams and events its batches borrow.  Set
+// up once per device on first use (the array's address is looked up with
+// hipGetSymbolAddress while that device is current).
+struct NoallocDevice {
+    void* scratch = nullptr;  // device address of the array on this device
+    size_t bytes = 0;
+    hipStream_t stream = nullptr, side = nullptr;
+    hipEvent_t events[3] = {nullptr, nullptr, nullptr};
+};
+int noalloc_device(NoallocDevice** out);
+
 inline unsigned blocks_for(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 // Fixed-size grids: a launch covers `units` work units with at most

@@ -35,25 +35,41 @@ const char* last_error() { return g_last_error.c_str(); }
 
 static int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 
+// Region of n elements: a new allocation, or in arena mode the next region of
+// the arena, cleared by a fill kernel (arena contents are left over from the
+// previous user).
+template <class T>
+static int dev_region(T** dst, size_t n, DevMem& owned, bool zero)
+{
+    const size_t bytes = (n * sizeof(T) + 3) / 4 * 4;
+    if (owned.arena) {
+        void* p = owned.take(bytes);
+        if (!p)
+            return fail(DMDX_E_OOM, "the system needs more device scratch than the " +
+                                        std::to_string(owned.cap) + "-byte array holds");
+        *dst = (T*)p;
+        if (zero) DMDX_HIP(launch_zero_bytes(p, bytes, owned.zero_stream));
+        return DMDX_OK;
+    }
+    DMDX_HIP(hipMalloc((void**)dst, bytes));
+    owned.add(*dst, bytes);
+    if (zero) DMDX_HIP(hipMemset(*dst, 0, bytes));
+    return DMDX_OK;
+}
+
 template <class T>
 static int dev_upload(T** dst, const std::vector<T>& src, DevMem& owned)
 {
-    size_t n = std::max<size_t>(src.size(), 1);
-    DMDX_HIP(hipMalloc((void**)dst, n * sizeof(T)));
-    owned.add(*dst, n * sizeof(T));
+    const int rc = dev_region(dst, std::max<size_t>(src.size(), 1), owned, src.empty());
+    if (rc != DMDX_OK) return rc;
     if (!src.empty()) DMDX_HIP(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
-    else DMDX_HIP(hipMemset(*dst, 0, sizeof(T)));
     return DMDX_OK;
 }
 
 template <class T>
 static int dev_alloc_zero(T** dst, size_t n, DevMem& owned)
 {
-    n = std::max<size_t>(n, 1);
-    DMDX_HIP(hipMalloc((void**)dst, n * sizeof(T)));
-    owned.add(*dst, n * sizeof(T));
-    DMDX_HIP(hipMemset(*dst, 0, n * sizeof(T)));
-    return DMDX_OK;
+    return dev_region(dst, std::max<size_t>(n, 1), owned, true);
 }
 
 #define TRY(expr)                 \
@@ -88,6 +104,7 @@ Batch::~Batch()
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (graph) (void)hipGraphDestroy(graph);
     for (void* p : owned.ptrs) (void)hipFree(p);
+    if (borrowed) return;  // arena mode: the streams and events are the caller's
     if (ev_fork) (void)hipEventDestroy(ev_fork);
     if (ev_join) (void)hipEventDestroy(ev_join);
     if (ev_list) (void)hipEventDestroy(ev_list);
@@ -99,11 +116,26 @@ int Batch::create(const dmdx_system_desc* descs, int nsys, const BatchOptions& o
 {
     if (nsys <= 0) return fail(DMDX_E_INVALID, "no systems");
     options = opt;
-    DMDX_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-    DMDX_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
-    DMDX_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
-    DMDX_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
-    DMDX_HIP(hipEventCreateWithFlags(&ev_list, hipEventDisableTiming));
+    if (opt.arena) {
+        if (opt.use_graph || opt.profile || !opt.arena_stream || !opt.arena_side ||
+            !opt.arena_events[0] || !opt.arena_events[1] || !opt.arena_events[2])
+            return fail(DMDX_E_INVALID, "an arena batch needs streams and events and runs eagerly");
+        borrowed = true;
+        stream = opt.arena_stream;
+        side = opt.arena_side;
+        ev_fork = opt.arena_events[0];
+        ev_join = opt.arena_events[1];
+        ev_list = opt.arena_events[2];
+        owned.arena = (char*)opt.arena;
+        owned.cap = opt.arena_bytes;
+        owned.zero_stream = stream;
+    } else {
+        DMDX_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        DMDX_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+        DMDX_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+        DMDX_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+        DMDX_HIP(hipEventCreateWithFlags(&ev_list, hipEventDisableTiming));
+    }
     const int prec = descs[0].precision;
     for (int s = 0; s < nsys; ++s)
         if (descs[s].precision != prec)
@@ -1116,7 +1148,7 @@ int Batch::check_sys(int s) const
     return DMDX_OK;
 }
 
-int Batch::get_arrays(int s, int what, double* a, double* b, double* c)
+int Batch::get_arrays(int s, int what, double* a, double* b, double* c, bool on_device)
 {
     TRY(check_sys(s));
     if (what == 1) LAUNCH(launch_materialize(d, stream));
@@ -1129,11 +1161,14 @@ int Batch::get_arrays(int s, int what, double* a, double* b, double* c)
     else return fail(DMDX_E_INVALID, "what must be 0 (positions), 1 (velocities) or 2 (forces)");
     double* dst[3] = {a, b, c};
     for (int k = 0; k < 3; ++k)
-        if (dst[k]) TRY(copy(dst[k], src[k] + S.atom_off, sizeof(double) * S.n, hipMemcpyDeviceToHost));
+        if (dst[k])
+            TRY(copy(dst[k], src[k] + S.atom_off, sizeof(double) * S.n,
+                     on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost));
     return DMDX_OK;
 }
 
-int Batch::set_arrays(int s, int what, const double* a, const double* b, const double* c)
+int Batch::set_arrays(int s, int what, const double* a, const double* b, const double* c,
+                      bool on_device)
 {
     TRY(check_sys(s));
     TRY(sync());
@@ -1148,7 +1183,9 @@ int Batch::set_arrays(int s, int what, const double* a, const double* b, const d
     else return fail(DMDX_E_INVALID, "what must be 0, 1 or 2");
     const double* src[3] = {a, b, c};
     for (int k = 0; k < 3; ++k)
-        if (src[k]) TRY(copy(dst[k] + S.atom_off, src[k], sizeof(double) * S.n, hipMemcpyHostToDevice));
+        if (src[k])
+            TRY(copy(dst[k] + S.atom_off, src[k], sizeof(double) * S.n,
+                     on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
     if (what == 0) {
         // positions changed: forces of the last compute no longer apply
         TRY(fetch_state());

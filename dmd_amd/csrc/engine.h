@@ -32,6 +32,13 @@ struct BatchOptions {
     int max_nbr = 0;           // 0: estimate from the density
     bool profile = false;      // HIP events around kernel groups
     uint32_t profile_mask = ~0u;  // which groups (bit = group) when profiling
+    // allocation-free mode: device arrays are carved from `arena` (capacity
+    // arena_bytes) and the streams and events are the caller's, so creating
+    // and destroying the batch allocates and frees no device memory
+    void* arena = nullptr;
+    size_t arena_bytes = 0;
+    hipStream_t arena_stream = nullptr, arena_side = nullptr;
+    hipEvent_t arena_events[3] = {nullptr, nullptr, nullptr};  // fork, join, list
 };
 
 // kernel groups timed when profiling is on
@@ -112,14 +119,27 @@ struct DescCopy {
     DescCopy& operator=(const DescCopy&) = delete;
 };
 
-// device allocations of a batch, with their total size
+// device allocations of a batch, with their total size.  In arena mode every
+// array is a 256-byte-aligned region carved from [arena, arena + cap) in
+// creation order; nothing is allocated and nothing is freed.
 struct DevMem {
     std::vector<void*> ptrs;
     size_t bytes = 0;
+    char* arena = nullptr;
+    size_t cap = 0;
+    hipStream_t zero_stream = nullptr;  // where arena regions are cleared
     void add(void* p, size_t b)
     {
         ptrs.push_back(p);
         bytes += b;
+    }
+    // next region of `b` bytes, or nullptr when the arena is full
+    void* take(size_t b)
+    {
+        const size_t at = (bytes + 255) / 256 * 256;
+        if (at > cap || b > cap - at) return nullptr;
+        bytes = at + b;
+        return arena + at;
     }
 };
 
@@ -134,8 +154,10 @@ public:
     int copy(void* dst, const void* src, size_t bytes, hipMemcpyKind kind);
     int fetch_state();
     int check_sys(int s) const;
-    int get_arrays(int s, int what, double* a, double* b, double* c);
-    int set_arrays(int s, int what, const double* a, const double* b, const double* c);
+    // host arrays, or device arrays when on_device is set
+    int get_arrays(int s, int what, double* a, double* b, double* c, bool on_device = false);
+    int set_arrays(int s, int what, const double* a, const double* b, const double* c,
+                   bool on_device = false);
     int set_box(int s, const double L[3]);
     int set_counter(int s, int counter);
     int set_active(int s, int active);
@@ -173,6 +195,7 @@ private:
     hipError_t timed(int group, hipError_t (*fn)(const Dev&, hipStream_t),
                      hipStream_t on = nullptr);
     DevMem owned;
+    bool borrowed = false;  // streams and events belong to the caller (arena mode)
     std::vector<hipEvent_t> ev_pool;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_used;
 

@@ -304,6 +304,24 @@ int dmdx_component_compute(dmdx_component *c, const double *x, const double *y, 
                            int64_t n, const double cell[9], double *fx, double *fy, double *fz,
                            double *energy_out);
 
+/* Allocation-free ForceComponent::compute.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  x, y, z,
+ * fx, fy, fz and *energy_out are DEVICE arrays (energy_out: one double);
+ * forces and energy accumulate as in dmdx_component_compute, with the same
+ * result bit for bit.  Everything else the kernels need lives in one
+ * file-scope device array of DMDX_NOALLOC_SCRATCH_BYTES per GPU, held until
+ * the process ends; no hipMalloc, hipFree or other device allocation is
+ * made.  A system that does not fit returns DMDX_E_OOM.  Calls on one GPU
+ * share the array: calls that overlap on one GPU (several host threads)
+ * race.  Stateless components only (bond, angle, dihedral, improper,
+ * exclusion, PME); LennardJones and CoulombDirect keep a pair list between
+ * calls and return DMDX_E_PARM.  Returns after the device work is done. */
+#define DMDX_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int dmdx_component_compute_noalloc(dmdx_component *c, const double *x, const double *y,
+                                   const double *z, int64_t n, const double cell[9], double *fx,
+                                   double *fy, double *fz, double *energy_out);
+
 /* ForceComponent::name(): "LennardJones", "CoulombDirect", ... */
 int dmdx_component_name(dmdx_component *c, char *buf, int64_t cap);
 
