@@ -16,9 +16,17 @@ replica; ``value`` = sum(atoms) * K / wall seconds of the K timed steps
 melts relax from a hot initial packing during it).  Per-step latency
 percentiles come from separately synchronised steps after the timed region.
 
+The default run also reports, beside ``value``: the plain path (each
+replica alone, FP64, best of eager and hipGraph launches), the CPU oracle
+on one pinned core and on all host cores (one replica per worker), and a
+served leg -- a bounded Poisson stream of LJ / SPME-water requests on
+dmdx_server with its p50/p99 step and completion latencies and its speedup
+over the plain path on the same requests (``serving``; --no-serving skips
+it).
+
     python bench.py --workload serving [--requests N --rate R ...]
 
-Serving: Poisson arrivals of independent LJ / water requests on
+Serving alone: Poisson arrivals of independent LJ / water requests on
 dmdx_server (see run_serving).
 
 Prints ONE JSON line.
@@ -60,6 +68,11 @@ def make_workload(name, n_replicas, seed):
     elif name == "water":
         specs = [W.water_box(n_side=16, grid=48, seed=seed)]
         desc = "water_box: 4096 flexible 3-site waters (12288 atoms), SPME 48^3, NH 300 K"
+    elif name == "water_large":
+        specs = [W.water_box(n_side=32, cutoff=1.0, grid=100, seed=seed,
+                             thermostat=0)]
+        desc = ("water_box: 32768 flexible 3-site waters (98304 atoms), SPME 100^3, rc 1.0 nm, "
+                "NVE")
     elif name == "argon":
         specs = [W.argon_fcc(skin=0.1, seed=seed).with_(physics=1)]
         desc = "argon_fcc: 4000 LJ argon atoms, rho* 0.8442, NVE, dt 1 fs"
@@ -149,43 +162,97 @@ def run_batched(specs, steps, warmup, latency_steps):
     return eng, wall, lat, kt, breakdown, flags, tune, builds
 
 
+PLAIN_GRAPH_STEPS = 25
+
+
 def run_plain(specs, steps, budget_s):
-    """One system at a time, FP64, eager launches: each replica of the sample
-    stepped alone."""
+    """One system at a time, FP64: each system of the sample stepped alone,
+    timed both with eager launches and with hipGraph-captured blocks of
+    PLAIN_GRAPH_STEPS steps (the same trajectory); the faster of the two
+    counts for that system (the plain path at its best launch mode).
+    Returns (best atom-steps/s, systems used, {eager, graph} atom-steps/s)."""
     from dmd_amd.engine import Engine
-    atoms_steps, t = 0, 0.0
+    atom_steps, t_best, t_mode = 0, 0.0, {"eager": 0.0, "graph": 0.0}
     used = 0
     for s in specs:
-        eng = Engine(s, reuse_forces=True)
-        eng.step(5)
-        eng.sync()
-        a = time.perf_counter()
-        eng.step(steps)
-        eng.sync()
-        t += time.perf_counter() - a
-        atoms_steps += s.n * steps
+        times = {}
+        for mode in ("eager", "graph"):
+            eng = Engine(s, reuse_forces=True, use_graph=(mode == "graph"),
+                         graph_steps=PLAIN_GRAPH_STEPS)
+            eng.step(PLAIN_GRAPH_STEPS)  # warm-up (captures the graph)
+            eng.sync()
+            a = time.perf_counter()
+            eng.step(steps)
+            eng.sync()
+            times[mode] = time.perf_counter() - a
+            eng.close()
+        atom_steps += s.n * steps
+        t_best += min(times.values())
+        for mode in times:
+            t_mode[mode] += times[mode]
         used += 1
-        eng.close()
-        if t > budget_s:
+        if t_best > budget_s:
             break
-    return atoms_steps / t, used
+    return atom_steps / t_best, used, {m: atom_steps / t for m, t in t_mode.items()}
+
+
+CPU_STEPS = 10
 
 
 def run_cpu(specs, budget_s):
     """The FP64 oracle (single-threaded C restatement of DMD's loops, O(N^2)
-    list rebuild every 10 computes) on a bounded sample of the replicas."""
+    list rebuild every 10 computes) on a bounded sample of the replicas,
+    pinned to one host core."""
     import oracle
-    atoms_steps, t, used, nsteps = 0, 0.0, 0, 10
-    for s in specs:
-        sim = oracle.Sim(s)
+    atoms_steps, t, used = 0, 0.0, 0
+    prev = os.sched_getaffinity(0)
+    core = min(prev)
+    os.sched_setaffinity(0, {core})
+    try:
+        for s in specs:
+            sim = oracle.Sim(s)
+            a = time.perf_counter()
+            sim.step(CPU_STEPS)
+            t += time.perf_counter() - a
+            atoms_steps += s.n * CPU_STEPS
+            used += 1
+            if t > budget_s:
+                break
+    finally:
+        os.sched_setaffinity(0, prev)
+    return atoms_steps / t, used, core
+
+
+def _cpu_worker(spec):
+    import oracle
+    sim = oracle.Sim(spec)
+    a = time.perf_counter()
+    sim.step(CPU_STEPS)
+    return spec.n * CPU_STEPS, time.perf_counter() - a
+
+
+def run_cpu_all_cores(specs, n_workers):
+    """The same oracle with independent replicas spread one per worker
+    process over `n_workers` host cores: aggregate atom-steps/s over the
+    wall time of the sample (the replicas `specs`)."""
+    import multiprocessing as mp
+    ctx = mp.get_context("spawn")
+    with ctx.Pool(n_workers) as pool:
+        pool.map(_cpu_worker, specs[:n_workers])  # warm the workers (imports, library load)
         a = time.perf_counter()
-        sim.step(nsteps)
-        t += time.perf_counter() - a
-        atoms_steps += s.n * nsteps
-        used += 1
-        if t > budget_s:
-            break
-    return atoms_steps / t, used, nsteps
+        res = pool.map(_cpu_worker, specs, chunksize=1)
+        wall = time.perf_counter() - a
+    return sum(r[0] for r in res) / wall
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
 
 
 def accuracy(specs, n_check=4):
@@ -223,6 +290,33 @@ def run_serving(args):
     216-8192 waters with SPME; 1000-20000 steps log-uniform) served by
     dmdx_server: admission by device memory, batched cohorts, retirement at
     each request's step target."""
+    rep, plain = serve(args)
+    out = {
+        "metric": "serving: aggregate atom-steps/s under Poisson arrivals; p50/p99 per-step "
+                  "and completion latency",
+        "value": rep["atom_steps_per_s"], "unit": "atom-steps/s", "n_gpus": 1,
+        "steps": None, "warmup": 0, "ms_per_step": rep["step_latency_ms"]["p50"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32 pair/PME arithmetic, fp64 accumulation and integration",
+        "data": "synthetic (seeded generators; no datasets)",
+        "config": {"workload": f"serving: {rep['requests']} requests, 50% LJ argon N~U[1k,32k], "
+                               "50% water U[216,8192] molecules (SPME), steps log-uniform "
+                               f"[1000,20000] x {args.steps_scale}, Poisson {args.rate}/s",
+                   "chunk_steps": args.chunk, "max_cohorts": args.cohorts,
+                   "graph_steps": args.graph_steps},
+        "serving": rep,
+    }
+    if plain:
+        out["plain_gpu"] = plain
+        out["speedup_vs_plain"] = rep["atom_steps_per_s"] / plain["value"]
+        if rep.get("loaded_atom_steps_per_s"):
+            out["loaded_speedup_vs_plain"] = rep["loaded_atom_steps_per_s"] / plain["value"]
+    print(json.dumps(out))
+
+
+def serve(args):
+    """The served request stream and, unless --no-plain, the plain path on
+    the same requests.  Returns (run_poisson report, plain record or None)."""
     from dmd_amd import _abi
     from dmd_amd import workloads as W
     from dmd_amd.server import Server, run_poisson
@@ -232,37 +326,19 @@ def run_serving(args):
     srv = Server(chunk_steps=args.chunk, max_cohorts=args.cohorts, graph_steps=args.graph_steps)
     rep = run_poisson(srv, reqs, rate_per_s=args.rate, seed=args.seed, log=log)
     srv.close()
-    # the plain path on the same request mix: each request alone, FP64,
-    # eager, a bounded number of steps (throughput per request is flat in
-    # the step count)
-    plain_value, used = None, 0
-    if not args.no_plain:
-        log("[bench] serving: plain one-system-at-a-time FP64 GPU path on the same requests")
-        plain_value, used = run_plain([s.with_(precision=_abi.FP64) for s, _ in reqs],
-                                      args.plain_steps, args.plain_budget)
-    out = {
-        "metric": "serving: aggregate atom-steps/s under Poisson arrivals; p50/p99 per-step "
-                  "and completion latency",
-        "value": rep["atom_steps_per_s"], "unit": "atom-steps/s", "n_gpus": 1,
-        "steps": None, "warmup": 0, "ms_per_step": rep["step_latency_ms"]["p50"],
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp32 pair/PME arithmetic, fp64 accumulation and integration",
-        "data": "synthetic (seeded generators; no datasets)",
-        "config": {"workload": f"serving: {len(reqs)} requests, 50% LJ argon N~U[1k,32k], "
-                               "50% water U[216,8192] molecules (SPME), steps log-uniform "
-                               f"[1000,20000] x {args.steps_scale}, Poisson {args.rate}/s",
-                   "chunk_steps": args.chunk, "max_cohorts": args.cohorts,
-                   "graph_steps": args.graph_steps},
-        "serving": rep,
-    }
-    if plain_value:
-        out["plain_gpu"] = {"value": plain_value, "unit": "atom-steps/s",
-                            "sample": f"first {used} requests, each alone, {args.plain_steps} "
-                                      "steps, FP64, eager"}
-        out["speedup_vs_plain"] = rep["atom_steps_per_s"] / plain_value
-        if rep.get("loaded_atom_steps_per_s"):
-            out["loaded_speedup_vs_plain"] = rep["loaded_atom_steps_per_s"] / plain_value
-    print(json.dumps(out))
+    log(f"[bench] serving: {rep['atom_steps_per_s']:.4g} atom-steps/s over {rep['wall_s']:.1f} s")
+    if args.no_plain:
+        return rep, None
+    # the plain path on the same request mix: each request alone, FP64, at
+    # its best launch mode, a bounded number of steps (throughput per
+    # request is flat in the step count)
+    log("[bench] serving: plain one-system-at-a-time FP64 GPU path on the same requests")
+    value, used, modes = run_plain([s.with_(precision=_abi.FP64) for s, _ in reqs],
+                                   args.plain_steps, args.plain_budget)
+    return rep, {"value": value, "unit": "atom-steps/s", "by_launch_mode": modes,
+                 "sample": f"first {used} requests, each alone, {args.plain_steps} steps, "
+                           f"FP64, best of eager and {PLAIN_GRAPH_STEPS}-step hipGraph blocks "
+                           "per request"}
 
 
 def main():
@@ -279,6 +355,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serving", action="store_true", help="skip the served-stream leg")
     ap.add_argument("--quick", action="store_true", help="batched leg only (profiling runs)")
     ap.add_argument("--requests", type=int, default=120, help="serving: number of requests")
     ap.add_argument("--rate", type=float, default=8.0, help="serving: arrivals per second")
@@ -301,6 +378,25 @@ def main():
     log(f"[bench] building workload {args.workload}")
     specs, desc = make_workload(args.workload, args.replicas, args.seed)
     n_atoms = sum(s.n for s in specs)
+    cpu_rec = None
+    if not args.quick and not args.no_cpu:
+        # before anything touches the GPU: the worker processes of the
+        # all-cores leg start from a process without device state
+        log("[bench] CPU baseline (oracle): one pinned core, then one replica per core")
+        cpu, used, core = run_cpu(specs, args.cpu_budget)
+        n_workers = max(1, min(16, len(os.sched_getaffinity(0))))
+        cpu_all = run_cpu_all_cores(specs, n_workers)
+        cpu_rec = {
+            "value": cpu, "unit": "atom-steps/s", "cores": 1, "kind": "port",
+            "sample": f"first {used} replicas x {CPU_STEPS} steps of the same workload, FP64 "
+                      "oracle (DMD's loops: O(N^2) list rebuild every 10 computes, 2 "
+                      f"computes per step), pinned to host core {core}",
+            "host": cpu_model(), "nproc": os.cpu_count(),
+            "all_cores": {"value": cpu_all, "unit": "atom-steps/s", "cores": n_workers,
+                          "sample": f"all {len(specs)} replicas x {CPU_STEPS} steps, one "
+                                    f"replica at a time per worker process, {n_workers} "
+                                    "workers"}}
+        log(f"[bench] CPU: {cpu:.4g} atom-steps/s on 1 core, {cpu_all:.4g} on {n_workers}")
     log(f"[bench] {len(specs)} systems, {n_atoms} atoms; batched mixed: warmup {args.warmup}, "
         f"timed {args.steps}")
     eng, wall, lat, kt, breakdown, flags, tune, builds = run_batched(
@@ -374,22 +470,32 @@ def main():
         out["force_rms_rel_err_mixed_vs_fp64"] = accuracy(specs)
         if not args.no_plain:
             log("[bench] plain one-system-at-a-time FP64 GPU path")
-            plain, used = run_plain(specs, args.plain_steps, args.plain_budget)
-            out["plain_gpu"] = {"value": plain, "unit": "atom-steps/s",
+            plain, used, modes = run_plain(specs, args.plain_steps, args.plain_budget)
+            out["plain_gpu"] = {"value": plain, "unit": "atom-steps/s", "by_launch_mode": modes,
                                 "sample": f"first {used} replicas, each alone, "
-                                          f"{args.plain_steps} steps, FP64, eager"}
+                                          f"{args.plain_steps} steps, FP64, best of eager and "
+                                          f"{PLAIN_GRAPH_STEPS}-step hipGraph blocks per replica"}
             out["speedup_vs_plain"] = value / plain
-        if not args.no_cpu:
-            log("[bench] CPU baseline (oracle, 1 core)")
-            cpu, used, nst = run_cpu(specs, args.cpu_budget)
-            out["cpu_baseline"] = {
-                "value": cpu, "unit": "atom-steps/s", "cores": 1, "kind": "port",
-                "sample": f"first {used} replicas x {nst} steps of the same workload, FP64 "
-                          "oracle (DMD's loops: O(N^2) list rebuild every 10 computes, 2 "
-                          "computes per step)",
-                "host": platform.processor() or platform.machine(),
-                "nproc": os.cpu_count()}
-            out["speedup_vs_cpu"] = value / cpu
+        if cpu_rec:
+            out["cpu_baseline"] = cpu_rec
+            out["speedup_vs_cpu"] = value / cpu_rec["value"]
+            out["speedup_vs_cpu_all_cores"] = value / cpu_rec["all_cores"]["value"]
+    if not args.quick and not args.no_serving:
+        # the served form of the north-star target: a bounded Poisson stream
+        # of LJ / SPME-water requests, and the plain path on the same requests
+        rep, plain = serve(args)
+        sv = {"workload": f"{rep['requests']} requests, 50% LJ argon N~U[1k,32k], 50% water "
+                          "U[216,8192] molecules (SPME), steps log-uniform [1000,20000] x "
+                          f"{args.steps_scale}, Poisson {args.rate}/s",
+              "value": rep["atom_steps_per_s"], "unit": "atom-steps/s",
+              "loaded_value": rep.get("loaded_atom_steps_per_s"),
+              "wall_s": rep["wall_s"], "atom_steps": rep["atom_steps"],
+              "step_latency_ms": rep["step_latency_ms"], "completion_s": rep["completion_s"],
+              "queue_wait_s": rep.get("queue_wait_s"), "failed": rep.get("failed")}
+        if plain:
+            sv["plain_gpu"] = plain
+            sv["speedup_vs_plain"] = rep["atom_steps_per_s"] / plain["value"]
+        out["serving"] = sv
     # plain copy kernel of known size (256 MiB read + 256 MiB written per
     # launch): calibrates FETCH_SIZE / WRITE_SIZE counters in profiled runs
     out["copy_bandwidth_GBs"] = E.copy_bandwidth(1 << 28, 10)

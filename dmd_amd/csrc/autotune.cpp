@@ -4,7 +4,8 @@
 // lanes, each taking 2, 4 or 8 list entries per pass (more entries per lane
 // keep more gathers in flight per wave, fewer need fewer registers so more
 // waves fit; fewer lanes per atom waste fewer lanes on the ragged end of a
-// row), and the list build gives each atom 8, 16 or 32 lanes.  Which is fastest depends on the systems (how
+// row), and the list build gives each atom 8, 16 or 32 lanes or shares each
+// cell's candidates among 2, 4 or 8 waves.  Which is fastest depends on the systems (how
 // many neighbours an atom has, how many atoms a batch holds, how many rebuild
 // at once), so the choice is measured: a scratch copy of the engine's batch
 // is stepped with each candidate and the times of the two kernel groups are
@@ -62,7 +63,11 @@ uint64_t signature(const dmdx_engine* e)
     return h;
 }
 
-bool cache_lookup(const char* path, uint64_t key, int* pair, int* width, int* build)
+// A cached line: key, pair lanes, pair width, per-atom build lanes, then
+// optionally the per-cell build switch and its waves per cell (a line
+// without them stands for the per-atom build).
+bool cache_lookup(const char* path, uint64_t key, int* pair, int* width, int* build, int* cells,
+                  int* tasks)
 {
     if (!path || !*path) return false;
     std::ifstream f(path);
@@ -76,6 +81,10 @@ bool cache_lookup(const char* path, uint64_t key, int* pair, int* width, int* bu
             *pair = p;
             *width = w;
             *build = b;
+            int c = 0, t = 4;
+            if (!(ss >> c >> t) || (c != 0 && c != 1)) c = 0, t = 4;
+            *cells = c;
+            *tasks = t;
             return true;
         }
     }
@@ -83,11 +92,13 @@ bool cache_lookup(const char* path, uint64_t key, int* pair, int* width, int* bu
 }
 
 // pair kernel candidates: (lanes per atom, entries per lane per pass);
-// list build candidates: lanes per atom
-constexpr int NPAIR = 8, NBUILD = 3;
+// list build candidates: per atom with 8, 16 or 32 lanes, then per cell
+// with 2, 4 or 8 waves per cell
+constexpr int NPAIR = 8, NBUILD = 6, NBUILD_ATOM = 3;
 constexpr int PAIR_LANES[NPAIR] = {2, 4, 8, 4, 8, 16, 8, 16};
 constexpr int PAIR_WIDTH[NPAIR] = {8, 8, 8, 4, 4, 4, 2, 2};
-constexpr int BUILD_LANES[NBUILD] = {8, 16, 32};
+constexpr int BUILD_LANES[NBUILD] = {8, 16, 32, 0, 0, 0};
+constexpr int BUILD_TASKS[NBUILD] = {0, 0, 0, 2, 4, 8};
 static_assert(NPAIR + NBUILD == DMDX_TUNE_TIMES, "times_ms layout");
 
 bool pair_candidate(int p, int w)
@@ -97,13 +108,14 @@ bool pair_candidate(int p, int w)
     return false;
 }
 
-void cache_store(const char* path, uint64_t key, int pair, int width, int build, const double* t)
+void cache_store(const char* path, uint64_t key, int pair, int width, int build, int cells,
+                 int tasks, const double* t)
 {
     if (!path || !*path) return;
     std::ostringstream o;
-    char buf[48];
-    std::snprintf(buf, sizeof(buf), "%016llx %d %d %d  # pair ms", (unsigned long long)key, pair,
-                  width, build);
+    char buf[64];
+    std::snprintf(buf, sizeof(buf), "%016llx %d %d %d %d %d  # pair ms", (unsigned long long)key,
+                  pair, width, build, cells, tasks);
     o << buf;
     for (int c = 0; c < NPAIR; ++c) {
         std::snprintf(buf, sizeof(buf), " %dx%d %.4f", PAIR_LANES[c], PAIR_WIDTH[c], t[c]);
@@ -111,7 +123,10 @@ void cache_store(const char* path, uint64_t key, int pair, int width, int build,
     }
     o << " build ms";
     for (int c = 0; c < NBUILD; ++c) {
-        std::snprintf(buf, sizeof(buf), " %d %.4f", BUILD_LANES[c], t[NPAIR + c]);
+        if (BUILD_LANES[c])
+            std::snprintf(buf, sizeof(buf), " atom%d %.4f", BUILD_LANES[c], t[NPAIR + c]);
+        else
+            std::snprintf(buf, sizeof(buf), " cell%d %.4f", BUILD_TASKS[c], t[NPAIR + c]);
         o << buf;
     }
     std::ofstream f(path, std::ios::app);
@@ -127,7 +142,6 @@ extern "C" int dmdx_engine_autotune(dmdx_engine* e, int32_t steps, const char* c
     if (!e) return dmdx::fail(DMDX_E_INVALID, "null engine");
     const int* pl = PAIR_LANES;
     const int* pw = PAIR_WIDTH;
-    const int* bl = BUILD_LANES;
     double t[DMDX_TUNE_TIMES];
     for (double& v : t) v = -1.0;
     auto report = [&]() {
@@ -142,12 +156,14 @@ extern "C" int dmdx_engine_autotune(dmdx_engine* e, int32_t steps, const char* c
         return DMDX_OK;
     }
     const uint64_t key = signature(e);
-    int cp = 0, cw = 0, cb = 0;
-    if (cache_lookup(cache_path, key, &cp, &cw, &cb) && pair_candidate(cp, cw) &&
-        (cb == 8 || cb == 16 || cb == 32)) {
+    int cp = 0, cw = 0, cb = 0, cc = 0, ct = 4;
+    if (cache_lookup(cache_path, key, &cp, &cw, &cb, &cc, &ct) && pair_candidate(cp, cw) &&
+        (cb == 8 || cb == 16 || cb == 32) && (ct == 1 || ct == 2 || ct == 4 || ct == 8)) {
         e->b.d.pair_tpa = cp;
         e->b.d.pair_width = cw;
         e->b.d.build_tpa = cb;
+        e->b.d.build_cells = cc != 0;
+        e->b.d.build_tasks = ct;
         e->b.invalidate_graph();
         report();
         return DMDX_OK;
@@ -184,7 +200,10 @@ extern "C" int dmdx_engine_autotune(dmdx_engine* e, int32_t steps, const char* c
     rc = scratch.kernel_time(dmdx::KG_LIST, &ms0, &n0);
     if (rc) return rc;
     for (int k = 0; k < NBUILD * n; ++k) {
-        scratch.d.build_tpa = bl[k % NBUILD];
+        const int c = k % NBUILD;
+        scratch.d.build_cells = BUILD_LANES[c] == 0;
+        if (BUILD_LANES[c]) scratch.d.build_tpa = BUILD_LANES[c];
+        else scratch.d.build_tasks = BUILD_TASKS[c];
         rc = scratch.step(1);
         if (!rc) rc = scratch.kernel_time(dmdx::KG_LIST, &ms1, &n1);
         if (rc) return rc;
@@ -192,16 +211,24 @@ extern "C" int dmdx_engine_autotune(dmdx_engine* e, int32_t steps, const char* c
         ms0 = ms1;
     }
     for (int c = 0; c < NBUILD; ++c) t[NPAIR + c] = acc[c] / n;
-    int bp = 0, bb = 0;
+    // the fastest pair shape; the fastest list build overall, and the
+    // fastest per-atom lane count (the per-atom build serves the systems
+    // that need a minimum image even when the per-cell build is chosen)
+    int bp = 0, bb = 0, ba = 0;
     for (int c = 1; c < NPAIR; ++c)
         if (t[c] < t[bp]) bp = c;
     for (int c = 1; c < NBUILD; ++c)
         if (t[NPAIR + c] < t[NPAIR + bb]) bb = c;
+    for (int c = 1; c < NBUILD_ATOM; ++c)
+        if (t[NPAIR + c] < t[NPAIR + ba]) ba = c;
     e->b.d.pair_tpa = pl[bp];
     e->b.d.pair_width = pw[bp];
-    e->b.d.build_tpa = bl[bb];
+    e->b.d.build_tpa = BUILD_LANES[ba];
+    e->b.d.build_cells = BUILD_LANES[bb] == 0;
+    if (e->b.d.build_cells) e->b.d.build_tasks = BUILD_TASKS[bb];
     e->b.invalidate_graph();
-    cache_store(cache_path, key, pl[bp], pw[bp], bl[bb], t);
+    cache_store(cache_path, key, pl[bp], pw[bp], BUILD_LANES[ba], e->b.d.build_cells ? 1 : 0,
+                e->b.d.build_tasks, t);
     report();
     return DMDX_OK;
 }

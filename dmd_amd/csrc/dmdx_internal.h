@@ -230,6 +230,8 @@ struct Dev {
     int* typ_s = nullptr;      // mixed path: LJ type in cell order
     int pair_tpa = 4;          // mixed pair kernel: lanes per atom (2, 4, 8, 16, 32)
     int build_tpa = 32;        // mixed list build: lanes per atom (8, 16, 32)
+    bool build_cells = true;   // mixed list build: per cell where no minimum image is needed
+    int build_tasks = 4;       // mixed per-cell build: waves per cell (1, 2, 4, 8)
     int pair_width = 8;        // mixed pair kernel: entries per lane per pass (2, 4, 8)
     // LJ tables
     double *lj_sig = nullptr, *lj_eps = nullptr;
@@ -280,6 +282,11 @@ struct Dev {
     int* rbw_n = nullptr;
     int n_bin_wg = 0;            // systems binned by k_bin_system
     bool any_bin_multi = false;  // systems binned by the multi-kernel path
+    // mixed path: some system has (or, with a barostat, may come to have)
+    // fewer than 2 cdiv + 1 cells along an edge; its list is built per atom
+    // with an FP32 minimum image (k_nbr_build_mixed), every other one per
+    // cell (k_nbr_build_cells)
+    bool any_wrap_mixed = false;
     SpreadTile* stiles = nullptr;  // charge-spreading tiles (atomic route)
     int nstiles = 0;
     GridTile* gtiles = nullptr;    // owner-computes spreading blocks

@@ -78,6 +78,21 @@ static int dev_alloc_zero(T** dst, size_t n, DevMem& owned)
         if (rc_ != DMDX_OK) return rc_; \
     } while (0)
 
+// A mixed-path list that has (or, with a barostat, may come to have) fewer
+// than 2 cdiv + 1 cells along an edge, as cells_in_use counts them on the
+// device: such a system's list is built per atom with a minimum image.
+static bool mixed_list_wraps(const SysDev& S)
+{
+    if (!S.has_list || S.precision == DMDX_FP64) return false;
+    bool wraps = S.barostat != DMDX_BARO_NONE || S.cdiv > 2;
+    for (int k = 0; k < 3; ++k) {
+        int nu = (int)std::floor(S.L[k] * S.cdiv / (S.rlist * (1.0 + 1e-9)));
+        nu = std::max(1, std::min(nu, S.nc[k]));
+        wraps = wraps || nu < 2 * S.cdiv + 1;
+    }
+    return wraps;
+}
+
 // radices for a Stockham plan: 4s first, then 2, 3, 5, 7, then other primes
 static bool plan_radices(int K, FftPlan& P)
 {
@@ -708,6 +723,7 @@ int Batch::create(const dmdx_system_desc* descs, int nsys, const BatchOptions& o
         if (S.has_list) {
             if (S.bin_wg) ++d.n_bin_wg;
             else d.any_bin_multi = true;
+            d.any_wrap_mixed = d.any_wrap_mixed || mixed_list_wraps(S);
         }
         d.any_excl |= S.n_ex > 0;
         d.any_baro |= S.barostat != DMDX_BARO_NONE;
@@ -1210,6 +1226,10 @@ int Batch::set_box(int s, const double L[3])
         S.L[k] = L[k];
         S.invL[k] = 1.0 / L[k];
     }
+    if (!d.any_wrap_mixed && mixed_list_wraps(S)) {
+        d.any_wrap_mixed = true;  // the per-atom build joins the launches
+        invalidate_graph();
+    }
     TRY(copy(d.sys + s, &S, sizeof(SysDev), hipMemcpyHostToDevice));
     host[s].valid = false;
     TRY(fetch_state());
@@ -1273,6 +1293,10 @@ int Batch::adopt(Batch& src, int ss, int ds)
     }
     hsys[ds].L[0] = B.L[0], hsys[ds].L[1] = B.L[1], hsys[ds].L[2] = B.L[2];
     hsys[ds].invL[0] = B.invL[0], hsys[ds].invL[1] = B.invL[1], hsys[ds].invL[2] = B.invL[2];
+    if (!d.any_wrap_mixed && mixed_list_wraps(B)) {
+        d.any_wrap_mixed = true;  // the per-atom build joins the launches
+        invalidate_graph();
+    }
     TRY(copy(d.sys + ds, &B, sizeof(SysDev), hipMemcpyHostToDevice));
     hstate[ds] = src.hstate[ss];
     TRY(copy(d.st + ds, &hstate[ds], sizeof(SysState), hipMemcpyHostToDevice));
@@ -1538,10 +1562,32 @@ int dmdx_engine_set_tuning(dmdx_engine* e, int32_t key, int32_t value)
         if (value != 8 && value != 16 && value != 32)
             return dmdx::fail(DMDX_E_PARM, "build lanes per atom must be 8, 16 or 32");
         e->b.d.build_tpa = value;
+    } else if (key == 3) {
+        if (value != 0 && value != 1) return dmdx::fail(DMDX_E_PARM, "per-cell build is 0 or 1");
+        e->b.d.build_cells = value != 0;
+    } else if (key == 4) {
+        if (value != 1 && value != 2 && value != 4 && value != 8)
+            return dmdx::fail(DMDX_E_PARM, "waves per cell must be 1, 2, 4 or 8");
+        e->b.d.build_tasks = value;
     } else {
         return dmdx::fail(DMDX_E_INVALID, "unknown tuning key");
     }
     e->b.invalidate_graph();
+    return DMDX_OK;
+}
+
+int dmdx_engine_get_tuning(dmdx_engine* e, int32_t key, int32_t* value)
+{
+    if (!e || !value) return dmdx::fail(DMDX_E_INVALID, "null argument");
+    const dmdx::Dev& d = e->b.d;
+    switch (key) {
+    case 0: *value = d.pair_tpa; break;
+    case 1: *value = d.build_tpa; break;
+    case 2: *value = d.pair_width; break;
+    case 3: *value = d.build_cells ? 1 : 0; break;
+    case 4: *value = d.build_tasks; break;
+    default: return dmdx::fail(DMDX_E_INVALID, "unknown tuning key");
+    }
     return DMDX_OK;
 }
 

@@ -726,6 +726,15 @@ __global__ void __launch_bounds__(256) k_nbr_build(
 // 3^3 otherwise, at most every cell of a small system (< 5 cells per edge)
 constexpr int NB_CELLS = 125;
 
+// A mixed-path system with at least 2 cdiv + 1 cells along every edge needs
+// no minimum image in its list build: its list is built per cell.
+__device__ __forceinline__ bool mixed_builds_per_cell(const SysDev& S)
+{
+    const int full = 2 * S.cdiv + 1;
+    return full * full * full <= NB_CELLS && cells_in_use(S, 0) >= full &&
+           cells_in_use(S, 1) >= full && cells_in_use(S, 2) >= full;
+}
+
 template <int TPA>
 __device__ __forceinline__ void k_nbr_build_mixed_unit(int64_t t,
     const SysDev* __restrict__ sys, SysState* __restrict__ st, const int* __restrict__ slot_sys,
@@ -734,7 +743,8 @@ __device__ __forceinline__ void k_nbr_build_mixed_unit(int64_t t,
     const int* __restrict__ ex_start, const int* __restrict__ ex_list, int* __restrict__ order,
     int* __restrict__ slot_of, int* __restrict__ nn, int* __restrict__ nbr,
     const double4* __restrict__ pos4s, const float4* __restrict__ pos4f,
-    int (*tab_base)[NB_CELLS], int (*tab_pre)[NB_CELLS + 1], float4 (*tab_sh)[NB_CELLS])
+    int (*tab_base)[NB_CELLS], int (*tab_pre)[NB_CELLS + 1], float4 (*tab_sh)[NB_CELLS],
+    bool cells_on)
 {
     const int lane = threadIdx.x % TPA;
     if (t < 0 || t >= nslots) return;  // whole waves (compact ranges are multiples of 64)
@@ -743,6 +753,7 @@ __device__ __forceinline__ void k_nbr_build_mixed_unit(int64_t t,
     const int s = __builtin_amdgcn_readfirstlane(slot_sys[t]);
     if (s < 0 || !st[s].rebuild) return;
     const SysDev& S = sys[s];
+    if (cells_on && mixed_builds_per_cell(S)) return;  // k_nbr_build_cells owns this system
     const int lt = (int)(t - S.atom_off);
     if (lt >= S.n) {
         if (lane == 0) {
@@ -928,7 +939,8 @@ __global__ void __launch_bounds__(256) k_nbr_build_mixed(
     const int* __restrict__ cstart, const int* __restrict__ cell_atoms,
     const int* __restrict__ ex_start, const int* __restrict__ ex_list, int* __restrict__ order,
     int* __restrict__ slot_of, int* __restrict__ nn, int* __restrict__ nbr,
-    const double4* __restrict__ pos4s, const float4* __restrict__ pos4f, RbList R)
+    const double4* __restrict__ pos4s, const float4* __restrict__ pos4f, RbList R,
+    bool cells_on)
 {
     constexpr int GPB = 256 / TPA;  // atoms per workgroup pass
     __shared__ int tab_base[GPB][NB_CELLS], tab_pre[GPB][NB_CELLS + 1];
@@ -939,7 +951,225 @@ __global__ void __launch_bounds__(256) k_nbr_build_mixed(
         k_nbr_build_mixed_unit<TPA>(rb_slot(R, sys, u_ * GPB + threadIdx.x / TPA), sys, st,
                                     slot_sys, nslots, acell, ccount, cstart, cell_atoms,
                                     ex_start, ex_list, order, slot_of, nn, nbr, pos4s, pos4f,
-                                    tab_base, tab_pre, tab_sh);
+                                    tab_base, tab_pre, tab_sh, cells_on);
+}
+
+// Mixed path, systems built per cell (mixed_builds_per_cell): `tasks`
+// waves per cell of a rebuilding system, task t taking the cell's atoms t,
+// t + tasks, ...  Every atom of a cell has the same candidate stream -- the
+// neighbour cells in scan order (z, then y, then x offset, each ascending),
+// each cell's atoms in cell order, as in k_nbr_build_mixed -- so a wave
+// loads each candidate once for all its atoms: 64 at a time, CB_CHUNKS
+// chunks per batch held in registers, shifted into the frame of the wave's
+// cell (l_j + offset * w).  Each of the wave's atoms (broadcast from the
+// lane that preloaded it) is then tested against the batch and its hits are
+// appended to its row in stream order with a ballot.  Several waves per
+// cell keep each wave's serial chain short and the device full.  The decisions are
+// those of k_nbr_build_mixed (FP32 test outside a 1e-5 nm band around the
+// list radius, the reference's FP64 expression inside it), so the rows are
+// the same, entry for entry.
+constexpr int CB_WAVES = 4;   // cells per workgroup pass
+constexpr int CB_CHUNKS = 4;  // 64-candidate chunks per register batch
+constexpr int CB_EXR = 6;     // excluded partners held in registers
+
+__global__ void __launch_bounds__(64 * CB_WAVES) k_nbr_build_cells(
+    const SysDev* __restrict__ sys, SysState* __restrict__ st, const int* __restrict__ ccount,
+    const int* __restrict__ cstart, const int* __restrict__ cell_atoms,
+    const int* __restrict__ ex_start, const int* __restrict__ ex_list, int* __restrict__ order,
+    int* __restrict__ slot_of, int* __restrict__ nn, int* __restrict__ nbr,
+    const double4* __restrict__ pos4s, const float4* __restrict__ pos4f, RbList R, int tasks)
+{
+    __shared__ int tb_all[CB_WAVES][NB_CELLS];        // cell-order start of each neighbour cell
+    __shared__ int tp_all[CB_WAVES][NB_CELLS + 1];    // candidates before it in the stream
+    __shared__ float4 tsh_all[CB_WAVES][NB_CELLS];    // offset * w, and the entry's offset bits
+    __shared__ RbStage rs;
+    R = rb_stage(R, true, rs);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int* tb = tb_all[wv];
+    int* tp = tp_all[wv];
+    float4* tsh = tsh_all[wv];
+    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const int nrb = *R.n;
+    const int64_t U = R.cpre[nrb];
+    for (int64_t w = (int64_t)blockIdx.x * CB_WAVES + wv; w < U * tasks;
+         w += (int64_t)gridDim.x * CB_WAVES) {
+        const int64_t u = w / tasks;
+        const int task = (int)(w - u * tasks);
+        const int kq = rb_find(R.cpre, nrb, u);
+        const int s = R.sys[kq];
+        const SysDev& S = sys[s];
+        if (S.precision == DMDX_FP64 || !mixed_builds_per_cell(S)) continue;
+        const int lc = (int)(u - R.cpre[kq]);
+        if (lc == 0 && task == 0)  // the system's padding slots hold no atom
+            for (int p = S.n + lane; p < S.n_pad; p += 64) {
+                nn[S.atom_off + p] = 0;
+                order[S.atom_off + p] = -1;
+            }
+        const int nx = cells_in_use(S, 0), ny = cells_in_use(S, 1), nz = cells_in_use(S, 2);
+        if (lc >= nx * ny * nz) continue;
+        const int c = S.cell_off + lc;
+        const int m = ccount[c];
+        const int mt = (m - task + tasks - 1) / tasks;  // this task's atoms
+        if (mt <= 0) continue;
+        const int cx = lc % nx, cy = (lc / nx) % ny, cz = lc / (nx * ny);
+        const float wx = (float)(S.L[0] / nx), wy = (float)(S.L[1] / ny),
+                    wz = (float)(S.L[2] / nz);
+        const int reach = S.cdiv, full = 2 * reach + 1;
+        const int ncb = full * full * full;  // 27 or 125
+        // neighbour-cell table, two rounds of 64 cells, then the exclusive
+        // running count of candidates in scan order
+        int cnt[2];
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int jc = q * 64 + lane;
+            cnt[q] = 0;
+            if (jc < ncb) {
+                const int ox = jc % full - reach, oy = (jc / full) % full - reach,
+                          oz = jc / (full * full) - reach;
+                int xx = cx + ox, yy = cy + oy, zz = cz + oz;
+                xx = xx < 0 ? xx + nx : (xx >= nx ? xx - nx : xx);
+                yy = yy < 0 ? yy + ny : (yy >= ny ? yy - ny : yy);
+                zz = zz < 0 ? zz + nz : (zz >= nz ? zz - nz : zz);
+                const int cc = S.cell_off + (zz * ny + yy) * nx + xx;
+                tb[jc] = cstart[cc];
+                cnt[q] = ccount[cc];
+                const uint32_t code = (((uint32_t)ox & 7u) << NBR_SLOT_BITS) |
+                                      (((uint32_t)oy & 7u) << (NBR_SLOT_BITS + 3)) |
+                                      (((uint32_t)oz & 7u) << (NBR_SLOT_BITS + 6));
+                tsh[jc] = make_float4((float)ox * wx, (float)oy * wy, (float)oz * wz,
+                                      __uint_as_float(code));
+            }
+        }
+        int run = 0;
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            int incl = cnt[q];
+            for (int o = 1; o < 64; o <<= 1) {
+                const int v = __shfl_up(incl, o, 64);
+                if (lane >= o) incl += v;
+            }
+            if (q * 64 + lane < ncb) tp[q * 64 + lane] = run + incl - cnt[q];
+            run += __shfl(incl, 63, 64);
+        }
+        if (lane == 0) tp[ncb] = run;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        const int ncand = run;
+        const int M = S.max_nbr;
+        const float lo2 = (float)((S.rlist - 1e-5) * (S.rlist - 1e-5));
+        const float hi2 = (float)((S.rlist + 1e-5) * (S.rlist + 1e-5));
+        const int64_t t0 = S.atom_off + cstart[c];  // the cell's first cell-order slot
+        for (int ig = 0; ig < mt; ig += 64) {
+            const int gm = min(64, mt - ig);
+            // the group's atoms (the task's atoms ig, ig + 1, ...: cell atoms
+            // task + tasks * (ig + lane)), one per lane: frame position, atom
+            // slot, excluded partners; broadcast to the wave atom by atom below
+            const bool mine = lane < gm;
+            const int64_t tl = t0 + task + (int64_t)tasks * (ig + (mine ? lane : 0));
+            const float4 lmy = pos4f[tl];
+            const int imy = cell_atoms[tl];
+            const int exb = ex_start[imy];
+            const int exn = mine ? ex_start[imy + 1] - exb : 0;
+            int exr[CB_EXR];
+#pragma unroll
+            for (int q = 0; q < CB_EXR; ++q) exr[q] = q < exn ? ex_list[exb + q] : -1;
+            int kreg = 0;  // row length of the lane's atom so far
+            int jc = 0;    // the lane's current neighbour cell (candidates only increase)
+            for (int b0 = 0; b0 < ncand; b0 += 64 * CB_CHUNKS) {
+                float px[CB_CHUNKS], py[CB_CHUNKS], pz[CB_CHUNKS];
+                uint32_t ent[CB_CHUNKS];
+                int jid[CB_CHUNKS];  // the candidate's atom slot
+                const int nch = min(CB_CHUNKS, (ncand - b0 + 63) / 64);  // wave-uniform
+#pragma unroll
+                for (int r = 0; r < CB_CHUNKS; ++r) {
+                    const int kc = b0 + r * 64 + lane;
+                    // past the end of the stream: a candidate outside every cutoff
+                    px[r] = py[r] = pz[r] = 1e18f;
+                    ent[r] = 0u;
+                    jid[r] = -1;
+                    if (kc < ncand) {
+                        while (jc + 1 < ncb && tp[jc + 1] <= kc) ++jc;
+                        const int slot = tb[jc] + (kc - tp[jc]);
+                        const float4 p = pos4f[S.atom_off + slot];
+                        const float4 sh = tsh[jc];
+                        px[r] = p.x + sh.x;
+                        py[r] = p.y + sh.y;
+                        pz[r] = p.z + sh.z;
+                        ent[r] = (uint32_t)slot | __float_as_uint(sh.w);
+                        jid[r] = cell_atoms[S.atom_off + slot];
+                    }
+                }
+                for (int ii = 0; ii < gm; ++ii) {
+                    const float lx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(lmy.x), ii));
+                    const float ly = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(lmy.y), ii));
+                    const float lz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(lmy.z), ii));
+                    const int i = __builtin_amdgcn_readlane(imy, ii);
+                    const int nex = __builtin_amdgcn_readlane(exn, ii);
+                    int ex[CB_EXR];
+#pragma unroll
+                    for (int q = 0; q < CB_EXR; ++q) ex[q] = __builtin_amdgcn_readlane(exr[q], ii);
+                    const int lti = (int)(t0 + task + (int64_t)tasks * (ig + ii) - S.atom_off);
+                    int* row = nbr + S.nbr_off + (int64_t)lti * M;
+                    int k = __builtin_amdgcn_readlane(kreg, ii);
+#pragma unroll
+                    for (int r = 0; r < CB_CHUNKS; ++r) {
+                        if (r >= nch) break;
+                        const uint32_t e = ent[r];
+                        const int slot = (int)(e & NBR_SLOT_MASK);
+                        const float dx = px[r] - lx, dy = py[r] - ly, dz = pz[r] - lz;
+                        const float r2f = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                        const bool other = slot != lti;
+                        bool hit = other && r2f < lo2;
+                        const bool band = other && r2f >= lo2 && r2f < hi2;
+                        if (__ballot(band) != 0ull && band) {
+                            // within 1e-5 nm of the list radius: the reference's
+                            // expression decides (src/core/cell.cpp:26-38,
+                            // src/force/lennard_jones.cpp:29-36)
+                            const double4 pi4 = pos4s[S.atom_off + lti];
+                            const double4 pj = pos4s[S.atom_off + slot];
+                            const double ddx = min_image_dmd(pj.x - pi4.x, S.L[0], S.invL[0]);
+                            const double ddy = min_image_dmd(pj.y - pi4.y, S.L[1], S.invL[1]);
+                            const double ddz = min_image_dmd(pj.z - pi4.z, S.L[2], S.invL[2]);
+                            hit = ddx * ddx + ddy * ddy + ddz * ddz < S.rlist2;
+                        }
+                        uint64_t bal = __ballot(hit);
+                        if (nex > 0 && bal != 0ull) {  // wave-uniform
+                            bool x = false;
+                            if (nex <= CB_EXR) {
+                                const int j = jid[r];
+#pragma unroll
+                                for (int q = 0; q < CB_EXR; ++q) x |= ex[q] == j;
+                            } else if (hit) {
+                                x = is_excluded(ex_start, ex_list, i, jid[r]);
+                            }
+                            hit = hit && !x;
+                            bal = __ballot(hit);
+                        }
+                        const int pos = k + __popcll(bal & below);
+                        if (hit && pos < M) row[pos] = (int)e;
+                        k += __popcll(bal);
+                    }
+                    kreg = lane == ii ? k : kreg;
+                }
+            }
+            if (mine) {
+                const int64_t ti = tl;
+                const int lti = (int)(ti - S.atom_off);
+                order[ti] = imy;
+                slot_of[imy] = lti;
+                int kk = kreg;
+                if (kk > M) {
+                    atomicOr(&st[s].flags, FLAG_NBR_OVERFLOW);
+                    kk = M;
+                }
+                nn[ti] = kk;
+                // pad the row to a multiple of 8 entries (as k_nbr_build_mixed)
+                int* row = nbr + S.nbr_off + (int64_t)lti * M;
+                const uint32_t pad = (uint32_t)lti | (NBR_PAD_CODE << NBR_SLOT_BITS);
+                for (int p = kk; p < ((kk + 7) & ~7); ++p) row[p] = (int)pad;
+            }
+        }
+    }
 }
 
 // one 64-thread workgroup per atom slot: bitonic sort of its row in LDS
@@ -1100,15 +1330,22 @@ hipError_t launch_build_lists(const Dev& d, hipStream_t s)
             d.sys, d.st, d.slot_sys, d.nslots, d.x, d.y, d.z, d.acell, d.ccount, d.cstart,
             d.cell_atoms, d.ex_start, d.ex_list, d.order, d.nn, d.nbr, d.pos4s, nullptr, (int64_t)(blocks_for(d.nslots, BS)));
     else {
-        const int tpa = d.build_tpa;
-        const int64_t units = blocks_for(d.nslots * tpa, BS);
+        if (d.build_cells && d.ncells > 0)
+            k_nbr_build_cells<<<grid_for(blocks_for((int64_t)d.ncells * d.build_tasks, CB_WAVES)),
+                                64 * CB_WAVES, 0, s>>>(
+                d.sys, d.st, d.ccount, d.cstart, d.cell_atoms, d.ex_start, d.ex_list, d.order,
+                d.slot_of, d.nn, d.nbr, d.pos4s, d.pos4f, R, d.build_tasks);
+        if (d.any_wrap_mixed || !d.build_cells) {
+            const int tpa = d.build_tpa;
+            const int64_t units = blocks_for(d.nslots * tpa, BS);
 #define DMDX_BUILD_ARGS                                                                       \
     d.sys, d.st, d.slot_sys, d.nslots, d.acell, d.ccount, d.cstart, d.cell_atoms, d.ex_start, \
-        d.ex_list, d.order, d.slot_of, d.nn, d.nbr, d.pos4s, d.pos4f, R
-        if (tpa == 8) k_nbr_build_mixed<8><<<grid_for(units), BS, 0, s>>>(DMDX_BUILD_ARGS);
-        else if (tpa == 32) k_nbr_build_mixed<32><<<grid_for(units), BS, 0, s>>>(DMDX_BUILD_ARGS);
-        else k_nbr_build_mixed<16><<<grid_for(units), BS, 0, s>>>(DMDX_BUILD_ARGS);
+        d.ex_list, d.order, d.slot_of, d.nn, d.nbr, d.pos4s, d.pos4f, R, d.build_cells
+            if (tpa == 8) k_nbr_build_mixed<8><<<grid_for(units), BS, 0, s>>>(DMDX_BUILD_ARGS);
+            else if (tpa == 32) k_nbr_build_mixed<32><<<grid_for(units), BS, 0, s>>>(DMDX_BUILD_ARGS);
+            else k_nbr_build_mixed<16><<<grid_for(units), BS, 0, s>>>(DMDX_BUILD_ARGS);
 #undef DMDX_BUILD_ARGS
+        }
     }
     if (d.precision == DMDX_FP64)
         k_nbr_sort<<<grid_for((unsigned)d.nslots), 64, 0, s>>>(d.sys, d.st, d.slot_sys, d.nn, d.nbr, (int64_t)((unsigned)d.nslots));

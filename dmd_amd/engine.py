@@ -53,6 +53,7 @@ def _setup(lib):
                                             C.POINTER(C.c_int64)]
     lib.dmdx_copy_bandwidth.argtypes = [C.c_int64, C.c_int32, C.POINTER(C.c_double)]
     lib.dmdx_engine_set_tuning.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
+    lib.dmdx_engine_get_tuning.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_int32)]
     lib.dmdx_engine_autotune.argtypes = [C.c_void_p, C.c_int32, C.c_char_p,
                                          C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                          C.POINTER(C.c_int32), C.POINTER(C.c_double)]
@@ -62,9 +63,9 @@ def _setup(lib):
 KERNEL_GROUPS = ("pair", "list", "bonded", "pme", "gather", "integrate")
 # autotuner candidates in the order of dmdx_engine_autotune's times_ms:
 # mixed pair kernel (lanes per atom x list entries per lane pass), list build
-# (lanes per atom)
+# (per atom with 8/16/32 lanes, per cell with 2/4/8 waves per cell)
 PAIR_SHAPES = ("2x8", "4x8", "8x8", "4x4", "8x4", "16x4", "8x2", "16x2")
-BUILD_SHAPES = (8, 16, 32)
+BUILD_SHAPES = ("atom8", "atom16", "atom32", "cell2", "cell4", "cell8")
 
 
 def copy_bandwidth(nbytes: int = 1 << 28, repeats: int = 20) -> float:
@@ -170,7 +171,15 @@ class Engine:
         _abi.check(self.lib.dmdx_engine_adopt(self.h, int(sys), src.h, int(src_sys)))
 
     def set_tuning(self, pair_lanes: int | None = None, build_lanes: int | None = None,
-                   pair_width: int | None = None):
+                   pair_width: int | None = None, build_cells: bool | None = None,
+                   build_tasks: int | None = None):
+        """Launch shapes of the mixed kernels (dmdx_engine_set_tuning);
+        build_cells: list built per cell (default) or per atom; build_tasks:
+        waves per cell of the per-cell build (1, 2, 4, 8)."""
+        if build_cells is not None:
+            _abi.check(self.lib.dmdx_engine_set_tuning(self.h, 3, int(bool(build_cells))))
+        if build_tasks is not None:
+            _abi.check(self.lib.dmdx_engine_set_tuning(self.h, 4, int(build_tasks)))
         if pair_lanes is not None:
             _abi.check(self.lib.dmdx_engine_set_tuning(self.h, 0, int(pair_lanes)))
         if build_lanes is not None:
@@ -180,17 +189,27 @@ class Engine:
 
     def autotune(self, steps: int = 5, cache: str | None = None) -> dict:
         """Time launch shapes of the mixed pair kernel (lanes per atom x
-        entries per lane pass) and list build (8/16/32 lanes) on a scratch
-        copy of the systems and apply the fastest (dmdx_engine_autotune).
-        `cache`: text file of earlier results."""
+        entries per lane pass) and list build (8/16/32 lanes per atom, or per
+        cell with 2/4/8 waves per cell) on a scratch copy of the systems and
+        apply the fastest (dmdx_engine_autotune).  `cache`: text file of
+        earlier results.  "build" names the chosen list build."""
         p, w, b = C.c_int32(), C.c_int32(), C.c_int32()
         t = (C.c_double * (len(PAIR_SHAPES) + len(BUILD_SHAPES)))()
         _abi.check(self.lib.dmdx_engine_autotune(
             self.h, int(steps), cache.encode() if cache else None, C.byref(p), C.byref(w),
             C.byref(b), t))
+        cells, tasks = self.get_tuning(3), self.get_tuning(4)
         return {"pair_lanes": p.value, "pair_width": w.value, "build_lanes": b.value,
+                "build_cells": bool(cells), "build_tasks": tasks,
+                "build": f"cell{tasks}" if cells else f"atom{b.value}",
                 "pair_ms": {k: t[i] for i, k in enumerate(PAIR_SHAPES)},
-                "build_ms": {b: t[len(PAIR_SHAPES) + i] for i, b in enumerate(BUILD_SHAPES)}}
+                "build_ms": {k: t[len(PAIR_SHAPES) + i] for i, k in enumerate(BUILD_SHAPES)}}
+
+    def get_tuning(self, key: int) -> int:
+        """Current value of a tuning key (dmdx_engine_get_tuning)."""
+        v = C.c_int32()
+        _abi.check(self.lib.dmdx_engine_get_tuning(self.h, int(key), C.byref(v)))
+        return v.value
 
     def set_profiling(self, on: bool, groups=None):
         """HIP events around every kernel group, or only around `groups`

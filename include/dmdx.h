@@ -344,27 +344,36 @@ int dmdx_engine_kernel_time(dmdx_engine *e, int32_t group, double *total_ms, int
 /* Launch shape of the mixed-precision kernels (no counterpart in the
  * reference): key 0 = lanes per atom of the pair kernel (2, 4, 8, 16 or 32),
  * key 1 = lanes per atom of the list build (8, 16 or 32), key 2 = list
- * entries per lane per pass of the pair kernel (2, 4 or 8).  Pair shapes
+ * entries per lane per pass of the pair kernel (2, 4 or 8), key 3 = list
+ * build per cell (1, default: the atoms of a cell share their candidates;
+ * systems that need a minimum image are still built per atom) or per atom
+ * (0), key 4 = waves per cell of the per-cell build (1, 2, 4 or 8).  The
+ * list rows do not depend on keys 1, 3 or 4.  Pair shapes
  * (lanes x entries): 2x8, 4x8, 8x8, 4x4, 8x4, 16x4, 32x4, 8x2, 16x2, 32x2;
  * another combination runs as the nearest of these (8 lanes where fewer
  * are not built for the width, at most 8 lanes with 8 entries).  Results do
  * not depend on the shape beyond FP32 summation order. */
 int dmdx_engine_set_tuning(dmdx_engine *e, int32_t key, int32_t value);
+/* The current value of a tuning key (as dmdx_engine_set_tuning numbers them). */
+int dmdx_engine_get_tuning(dmdx_engine *e, int32_t key, int32_t *value);
 
 /* Autotuner (no counterpart in the reference): times launch shapes of the
  * mixed pair kernel (lanes per atom x list entries per lane pass) and of the
- * list build (8/16/32 lanes per atom) on a scratch copy of the engine's
+ * list build (8/16/32 lanes per atom, or per cell with 2/4/8 waves per
+ * cell) on a scratch copy of the engine's
  * systems stepped from their initial state (the engine's own state is
  * untouched), `steps` steps per candidate, and applies the fastest of each.
  * cache_path (may be NULL): text file of earlier results keyed by a
  * signature of the systems; a hit applies the cached choice without timing.
  * times_ms (DMDX_TUNE_TIMES doubles, may be NULL): per-step ms of the pair
  * kernel at 2x8, 4x8, 8x8, 4x4, 8x4, 16x4, 8x2, 16x2, then per-step ms of
- * the list build at 8, 16, 32 lanes (-1 when not timed).  FP64 engines have
+ * the list build at 8, 16, 32 lanes per atom and per cell at 2, 4, 8 waves
+ * per cell (-1 when not timed); dmdx_engine_get_tuning reads the chosen
+ * build (keys 1, 3, 4).  FP64 engines have
  * nothing to tune.
  * Mixed-precision results depend on the shape only through FP32 summation
  * order (well inside the mixed-precision tolerance). */
-#define DMDX_TUNE_TIMES 11
+#define DMDX_TUNE_TIMES 14
 int dmdx_engine_autotune(dmdx_engine *e, int32_t steps, const char *cache_path,
                          int32_t *pair_lanes, int32_t *pair_width, int32_t *build_lanes,
                          double *times_ms);

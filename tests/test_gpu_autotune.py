@@ -21,19 +21,21 @@ def test_autotune_choice_cache_and_state(tmp_path):
     print(r, flush=True)
     assert f"{r['pair_lanes']}x{r['pair_width']}" in PAIR_SHAPES
     assert r["build_lanes"] in (8, 16, 32)
+    assert r["build"] in ("atom8", "atom16", "atom32", "cell2", "cell4", "cell8")
     pm = r["pair_ms"]
     assert all(v > 0 for v in pm.values())
     assert pm[f"{r['pair_lanes']}x{r['pair_width']}"] == min(pm.values())
     bm = r["build_ms"]
-    assert bm[r["build_lanes"]] == min(bm.values())
+    assert bm[r["build"]] == min(bm.values())
+    assert bm[f"atom{r['build_lanes']}"] == min(bm[k] for k in bm if k.startswith("atom"))
     # the engine's own systems did not move
     for k in range(len(specs)):
         assert np.array_equal(e.positions(k), before[k])
     # a second engine on the same systems hits the cache
     e2 = Engine(specs)
     r2 = e2.autotune(steps=3, cache=cache)
-    assert (r2["pair_lanes"], r2["pair_width"], r2["build_lanes"]) == \
-        (r["pair_lanes"], r["pair_width"], r["build_lanes"])
+    assert (r2["pair_lanes"], r2["pair_width"], r2["build_lanes"], r2["build"]) == \
+        (r["pair_lanes"], r["pair_width"], r["build_lanes"], r["build"])
     assert all(v == -1.0 for v in r2["pair_ms"].values())
     # forces of the tuned engine against the FP64 path
     e.compute()

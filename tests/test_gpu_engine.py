@@ -366,3 +366,40 @@ def test_mixed_chain_forces_after_steps():
     err = rms_rel(e.forces(), F_ref)
     say(f"  force rms rel err {err:.3e}")
     assert err <= 1e-5
+
+
+@pytest.mark.parametrize("kind", ["replica_mix", "water"])
+def test_list_build_per_cell_equals_per_atom(kind):
+    """The per-cell list build (one wave per cell, candidates shared by the
+    cell's atoms) writes the same rows, entry for entry, as the per-atom
+    build: pair sets equal and trajectories identical bit for bit over
+    steps with rebuilds."""
+    if kind == "water":
+        specs = [W.water_box(n_mol=500, seed=8, precision=_abi.MIXED)]
+    else:
+        specs = W.replica_mix(n_replicas=12, n_min=500, n_max=4000, seed=11,
+                              precision=_abi.MIXED)
+    say(f"case: list build per cell vs per atom, {kind} ({sum(s.n for s in specs)} atoms)")
+    b = engine(specs)
+    b.set_tuning(build_cells=False)
+    b.compute()
+    b_pairs = [b.pairs(k) for k in range(len(specs))]
+    b_forces = [b.forces(k) for k in range(len(specs))]
+    b.step(30)
+    for tasks in (4, 1, 8):  # waves per cell of the per-cell build
+        a = engine(specs)
+        a.set_tuning(build_tasks=tasks)
+        a.compute()
+        for k in range(len(specs)):
+            pa = a.pairs(k)
+            assert np.array_equal(pa[0], b_pairs[k][0]), (tasks, k)
+            assert np.array_equal(pa[1], b_pairs[k][1]), (tasks, k)
+            assert np.array_equal(a.forces(k), b_forces[k]), (tasks, k)
+        a.step(30)
+        for k in range(len(specs)):
+            assert np.array_equal(a.positions(k), b.positions(k)), (tasks, k)
+            assert np.array_equal(a.forces(k), b.forces(k)), (tasks, k)
+        builds = [a.scalars(k)["list_builds"] for k in range(len(specs))]
+        say(f"  waves per cell {tasks}: list builds per system {builds}")
+        assert max(builds) > 1
+        a.close()
