@@ -591,11 +591,14 @@ int Batch::create(const dmdx_system_desc* descs, int nsys, const BatchOptions& o
             int K = S.K[k];
             int nlines = (k == 0) ? S.K[1] * S.K[2] : (k == 1 ? S.K[0] * S.K[2] : S.K[0] * S.K[1]);
             // lines per workgroup: enough workgroups to spread a small grid
-            // over the chip (about nlines/256, a power of two in 4..32),
-            // within 48 KiB of LDS for the two ping-pong buffers
+            // over the chip (about nlines/256, a power of two in 4..16: 16
+            // lines are whole 128-byte runs of the strided y and z passes in
+            // FP32 and keep several workgroups per CU on large grids), within
+            // 48 KiB of LDS for the two ping-pong buffers and the twiddles
             int B = 4;
-            while (B < 32 && 2 * B * 256 <= nlines) B *= 2;
-            B = (int)std::max<size_t>(1, std::min<size_t>((size_t)B, (48 * 1024) / (2 * K * cx)));
+            while (B < 16 && 2 * B * 256 <= nlines) B *= 2;
+            B = (int)std::max<size_t>(
+                1, std::min<size_t>((size_t)B, (48 * 1024 - (size_t)K * cx) / (2 * K * cx)));
             int local = 0;
             for (int l0 = 0; l0 < nlines; l0 += B) {
                 FftTile t;
@@ -604,7 +607,7 @@ int Batch::create(const dmdx_system_desc* descs, int nsys, const BatchOptions& o
                 t.nlines = std::min(B, nlines - l0);
                 t.tile_local = local++;
                 tiles[k].push_back(t);
-                d.lds_bytes[k] = std::max(d.lds_bytes[k], (size_t)(2 * t.nlines * K * cx));
+                d.lds_bytes[k] = std::max(d.lds_bytes[k], (size_t)((2 * t.nlines + 1) * K * cx));
             }
             if (k == 2) {
                 S.pw_n = local;

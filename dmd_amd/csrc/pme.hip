@@ -609,8 +609,13 @@ __device__ __forceinline__ void k_fft_pass_unit(int64_t unit_, const FftTile* __
     const int Kx = S.K[0], Ky = S.K[1];
     Cx<R>* bufA = reinterpret_cast<Cx<R>*>(smem_raw);
     Cx<R>* bufB = bufA + nl * K;
+    // the line's twiddle table (K entries) beside the two buffers: the
+    // stages read their twiddles from LDS instead of global memory
+    Cx<R>* twl = bufB + nl * K;
     Cx<R>* G = grid + S.grid_off;
-    const Cx<R>* tw = twiddles + S.tw_off[dim];
+    const Cx<R>* twg = twiddles + S.tw_off[dim];
+    for (int e = threadIdx.x; e < K; e += blockDim.x) twl[e] = twg[e];
+    const Cx<R>* tw = twl;
     // element (line b, index k) -> grid offset
     auto addr = [&](int b, int k) -> int64_t {
         int l = T.line0 + b;
