@@ -10,6 +10,7 @@ component's energy.  Non-zero C-ABI statuses raise ``ForceException``
 from __future__ import annotations
 
 import ctypes as C
+import enum
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -34,6 +35,7 @@ def _setup(lib):
     lib.dmdx_coulomb_direct_create.argtypes = [C.c_double, C.c_double, P]
     lib.dmdx_coulomb_exclusion_create.argtypes = [P]
     lib.dmdx_coulomb_exclusion_set_exclusions.argtypes = [_cp, _ip, _ip, C.c_int64]
+    lib.dmdx_coulomb_exclusion_set_ewald_coefficient.argtypes = [_cp, C.c_double]
     lib.dmdx_pme_create.argtypes = [C.c_double, C.c_double, C.c_int32, C.c_int32, C.c_int32,
                                     C.c_int32, P]
     lib.dmdx_pme_theta4.argtypes = [C.c_double, _dp]
@@ -46,6 +48,10 @@ def _setup(lib):
     lib.dmdx_component_set_charges.argtypes = [_cp, _dp, C.c_int64]
     lib.dmdx_component_rebuild_pair_list.argtypes = [_cp, _dp, _dp, _dp, C.c_int64, _dp]
     lib.dmdx_component_compute.argtypes = [_cp, _dp, _dp, _dp, C.c_int64, _dp, _dp, _dp, _dp, _dp]
+    lib.dmdx_component_rebuild_pair_list_cell.argtypes = [_cp, _dp, _dp, _dp, C.c_int64, _dp,
+                                                          C.c_int32]
+    lib.dmdx_component_compute_cell.argtypes = [_cp, _dp, _dp, _dp, C.c_int64, _dp, C.c_int32,
+                                                _dp, _dp, _dp, _dp]
     lib.dmdx_component_compute_noalloc.argtypes = [_cp, _cp, _cp, _cp, C.c_int64, _dp, _cp, _cp,
                                                    _cp, _cp]
     lib.dmdx_component_name.argtypes = [_cp, C.c_char_p, C.c_int64]
@@ -74,12 +80,31 @@ def _p(a):
 
 # ------------------------------------------------------------------ core
 
+class CellType(enum.IntEnum):
+    """src/core/cell.h:3-8 (values of DMDX_CELL_*)."""
+    orthorhombic = 0
+    triclinic = 1
+    hexagonal = 2
+    rhombic_dodecahedron = 3
+
+
 class Cell:
-    """src/core/cell.h:10-23 (orthorhombic)."""
+    """src/core/cell.h:10-23: Cell(a, b, c) is orthorhombic; Cell.from_matrix
+    (the Cell(m[9], type) constructor) takes any row-major matrix and type.
+    The minimum image is the identity unless orthorhombic
+    (src/core/cell.cpp:26-38); the volume is the determinant."""
 
     def __init__(self, a: float = 1.0, b: float = 1.0, c: float = 1.0):
         self.matrix = np.zeros(9)
         self.matrix[0], self.matrix[4], self.matrix[8] = a, b, c
+        self.type = CellType.orthorhombic
+
+    @classmethod
+    def from_matrix(cls, m, t=CellType.orthorhombic):
+        cell = cls()
+        cell.matrix = np.array(m, dtype=np.float64).reshape(9)
+        cell.type = CellType(t)
+        return cell
 
     def volume(self) -> float:
         m = self.matrix
@@ -165,24 +190,27 @@ class ForceComponent:
         if not (len(y) == len(z) == len(forces_x) == len(forces_y) == len(forces_z) == n):
             raise ForceException(-3, "span sizes differ")
         e = C.c_double(energy_out)
-        _abi.check(_lib().dmdx_component_compute(
-            self._h, _p(x), _p(y), _p(z), n, _p(_f64(cell.matrix)), _p(forces_x), _p(forces_y),
-            _p(forces_z), C.byref(e)))
+        _abi.check(_lib().dmdx_component_compute_cell(
+            self._h, _p(x), _p(y), _p(z), n, _p(_f64(cell.matrix)), int(cell.type), _p(forces_x),
+            _p(forces_y), _p(forces_z), C.byref(e)))
         return e.value
 
     def compute_noalloc(self, n: int, x_ptr: int, y_ptr: int, z_ptr: int, cell: Cell,
                         fx_ptr: int, fy_ptr: int, fz_ptr: int, energy_ptr: int):
         """dmdx_component_compute_noalloc: the same compute on DEVICE arrays
         (integer device addresses of n doubles each, and of one double for
-        the energy), with no device allocation by the library."""
+        the energy), with no device allocation by the library.  Orthorhombic
+        cells only, as the C entry point."""
+        if cell.type != CellType.orthorhombic:
+            raise ForceException(-4, "compute_noalloc takes orthorhombic cells only")
         _abi.check(_lib().dmdx_component_compute_noalloc(
             self._h, x_ptr, y_ptr, z_ptr, int(n), _p(_f64(cell.matrix)), fx_ptr, fy_ptr, fz_ptr,
             energy_ptr))
 
     def _rebuild(self, pos_x, pos_y, pos_z, cell):
         x, y, z = _f64(pos_x), _f64(pos_y), _f64(pos_z)
-        _abi.check(_lib().dmdx_component_rebuild_pair_list(self._h, _p(x), _p(y), _p(z), len(x),
-                                                           _p(_f64(cell.matrix))))
+        _abi.check(_lib().dmdx_component_rebuild_pair_list_cell(
+            self._h, _p(x), _p(y), _p(z), len(x), _p(_f64(cell.matrix)), int(cell.type)))
 
 
 @dataclass
@@ -261,6 +289,11 @@ class CoulombExclusion(_Charged):
         if len(a) != len(b):
             raise ForceException(-3, "excl_i and excl_j differ in length")
         _abi.check(_lib().dmdx_coulomb_exclusion_set_exclusions(self._h, _p(a), _p(b), len(a)))
+
+    def set_ewald_coefficient(self, alpha: float):
+        """Ewald coefficient of the CORRECTED (erf) exclusion term; no
+        counterpart in the reference, whose term is the bare 1/r."""
+        _abi.check(_lib().dmdx_coulomb_exclusion_set_ewald_coefficient(self._h, float(alpha)))
 
 
 @dataclass

@@ -30,9 +30,9 @@ struct BondedArgs {
 
 __device__ __forceinline__ void mi(const SysDev& S, double& dx, double& dy, double& dz)
 {
-    dx = min_image_dmd(dx, S.L[0], S.invL[0]);
-    dy = min_image_dmd(dy, S.L[1], S.invL[1]);
-    dz = min_image_dmd(dz, S.L[2], S.invL[2]);
+    dx = min_image_dmd(dx, S.L[0], S.iminv[0]);
+    dy = min_image_dmd(dy, S.L[1], S.iminv[1]);
+    dz = min_image_dmd(dz, S.L[2], S.iminv[2]);
 }
 
 __device__ __forceinline__ void put(TermF* tf, int64_t k, double a, double b, double c)

@@ -9,6 +9,7 @@
 // tests/unit/force/test_force_engine.cpp:46-57).  Positions and forces are
 // host arrays, like the std::span arguments of the reference, so every call
 // copies them to and from HBM; the device-resident path is dmdx_engine_*.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -37,6 +38,7 @@ struct dmdx_component {
     std::vector<double> k1, k2;
     int64_t n_terms = 0;
     int32_t counter = 10;  // LennardJones::step_since_rebuild_ (lennard_jones.cpp:9-10)
+    int32_t max_nbr = 0;   // list row width; 0 = estimated from the density
     // device side
     std::unique_ptr<Batch> b;
     int64_t n = -1;
@@ -45,15 +47,16 @@ struct dmdx_component {
 
 namespace {
 
-int check_cell(const double cell[9])
+// Cell (src/core/cell.h:10-23): any row-major matrix with a positive
+// diagonal; its type selects the minimum image (identity unless
+// orthorhombic, src/core/cell.cpp:26-38) and its determinant is the volume.
+int check_cell(const double cell[9], int type)
 {
     if (!cell) return dmdx::fail(DMDX_E_INVALID, "null cell");
-    if (cell[1] != 0.0 || cell[2] != 0.0 || cell[3] != 0.0 || cell[5] != 0.0 || cell[6] != 0.0 ||
-        cell[7] != 0.0)
-        return dmdx::fail(DMDX_E_PARM, "only orthorhombic cells are supported (the reference's "
-                                       "minimum image is an identity stub for the other types)");
+    if (type < DMDX_CELL_ORTHORHOMBIC || type > DMDX_CELL_RHOMBIC_DODECAHEDRON)
+        return dmdx::fail(DMDX_E_PARM, "unknown cell type");
     if (!(cell[0] > 0.0 && cell[4] > 0.0 && cell[8] > 0.0))
-        return dmdx::fail(DMDX_E_PARM, "cell edges must be positive");
+        return dmdx::fail(DMDX_E_PARM, "cell edges (the matrix diagonal) must be positive");
     return DMDX_OK;
 }
 
@@ -94,6 +97,7 @@ int make_batch(dmdx_component* c, int64_t n, const double* x, const double* y, c
     D.ewald_coeff = c->alpha;
     opt.component = c->type;
     opt.reuse_forces = false;
+    opt.max_nbr = c->max_nbr;
     switch (c->type) {
     case dmdx::COMP_LJ:
         D.use_lj = 1;
@@ -174,12 +178,35 @@ int ensure_batch(dmdx_component* c, int64_t n, const double* x, const double* y,
 }
 
 int upload_state(dmdx_component* c, const double* x, const double* y, const double* z,
-                 const double cell[9])
+                 const double cell[9], int type)
 {
-    double L[3] = {cell[0], cell[4], cell[8]};
-    int rc = c->b->set_box(0, L);
+    int rc = c->b->set_cell(0, cell, type);
     if (rc) return rc;
     return c->b->set_arrays(0, 0, x, y, z);
+}
+
+// DMD's pair lists are vectors that grow with the pairs found
+// (lennard_jones.cpp:18-40); the device rows have a fixed width, estimated
+// from the mean density.  After a build that overflowed (atoms far denser
+// than the box average), the batch is made again with rows twice as wide and
+// the caller repeats the call; `counter` is the list counter to restore.
+int widen_list(dmdx_component* c, int64_t n, int32_t counter)
+{
+    const int have = c->b ? c->b->hsys[0].max_nbr : 0;
+    if (have >= 2048 || have >= n - 1)
+        return dmdx::fail(DMDX_E_MISMATCH, "neighbour list overflow (more than 2048 neighbours "
+                                           "of one atom)");
+    c->max_nbr = std::min(std::max(2 * have, 64), 2048);
+    c->b.reset();
+    c->n = -1;
+    c->counter = counter;
+    return DMDX_OK;
+}
+
+bool overflowed(dmdx_component* c, int* rc)
+{
+    *rc = c->b->fetch_state();
+    return *rc == DMDX_OK && (c->b->hstate[0].flags & dmdx::FLAG_NBR_OVERFLOW);
 }
 
 int out(dmdx_component* c, dmdx_component** dst)
@@ -409,21 +436,43 @@ int dmdx_coulomb_exclusion_set_exclusions(dmdx_component* c, const int32_t* ei, 
     return DMDX_OK;
 }
 
+int dmdx_coulomb_exclusion_set_ewald_coefficient(dmdx_component* c, double alpha)
+{
+    if (!c || c->type != dmdx::COMP_EXCLUSION)
+        return dmdx::fail(DMDX_E_INVALID, "not a CoulombExclusion");
+    if (!(alpha > 0.0)) return dmdx::fail(DMDX_E_PARM, "the Ewald coefficient must be positive");
+    c->alpha = alpha;
+    c->b.reset();  // alpha is part of the system description
+    c->n = -1;
+    return DMDX_OK;
+}
+
 int dmdx_component_rebuild_pair_list(dmdx_component* c, const double* x, const double* y,
                                      const double* z, int64_t n, const double cell[9])
+{
+    return dmdx_component_rebuild_pair_list_cell(c, x, y, z, n, cell, DMDX_CELL_ORTHORHOMBIC);
+}
+
+int dmdx_component_rebuild_pair_list_cell(dmdx_component* c, const double* x, const double* y,
+                                          const double* z, int64_t n, const double cell[9],
+                                          int32_t cell_type)
 {
     if (!c || (c->type != dmdx::COMP_LJ && c->type != dmdx::COMP_COULOMB_DIRECT))
         return dmdx::fail(DMDX_E_INVALID, "component has no pair list");
     if (!x || !y || !z || n <= 0) return dmdx::fail(DMDX_E_INVALID, "bad positions");
-    int rc = check_cell(cell);
+    int rc = check_cell(cell, cell_type);
     if (rc) return rc;
-    rc = ensure_batch(c, n, x, y, z, cell);
-    if (rc) return rc;
-    rc = upload_state(c, x, y, z, cell);
-    if (rc) return rc;
-    rc = c->b->rebuild_only();
-    if (rc) return rc;
-    return c->b->sync();
+    for (;;) {
+        rc = ensure_batch(c, n, x, y, z, cell);
+        if (rc) return rc;
+        rc = upload_state(c, x, y, z, cell, cell_type);
+        if (rc) return rc;
+        rc = c->b->rebuild_only();
+        if (rc) return rc;
+        if (!overflowed(c, &rc)) return rc;
+        rc = widen_list(c, n, 0);
+        if (rc) return rc;
+    }
 }
 
 int dmdx_lj_get_step_since_rebuild(dmdx_component* c, int32_t* outv)
@@ -446,26 +495,40 @@ int dmdx_component_compute(dmdx_component* c, const double* x, const double* y, 
                            int64_t n, const double cell[9], double* fx, double* fy, double* fz,
                            double* energy_out)
 {
+    return dmdx_component_compute_cell(c, x, y, z, n, cell, DMDX_CELL_ORTHORHOMBIC, fx, fy, fz,
+                                       energy_out);
+}
+
+int dmdx_component_compute_cell(dmdx_component* c, const double* x, const double* y,
+                                const double* z, int64_t n, const double cell[9],
+                                int32_t cell_type, double* fx, double* fy, double* fz,
+                                double* energy_out)
+{
     if (!c) return dmdx::fail(DMDX_E_INVALID, "null component");
     if (!x || !y || !z || !fx || !fy || !fz || !energy_out || n <= 0)
         return dmdx::fail(DMDX_E_INVALID, "null array or empty system");
-    int rc = check_cell(cell);
+    int rc = check_cell(cell, cell_type);
     if (rc) return rc;
-    rc = ensure_batch(c, n, x, y, z, cell);
-    if (rc) return rc;
-    rc = upload_state(c, x, y, z, cell);
-    if (rc) return rc;
-    rc = c->b->set_arrays(0, 2, fx, fy, fz);
-    if (rc) return rc;
-    rc = c->b->compute(1);
+    for (;;) {
+        rc = ensure_batch(c, n, x, y, z, cell);
+        if (rc) return rc;
+        const int32_t counter = c->b->host[0].counter;
+        rc = upload_state(c, x, y, z, cell, cell_type);
+        if (rc) return rc;
+        rc = c->b->set_arrays(0, 2, fx, fy, fz);
+        if (rc) return rc;
+        rc = c->b->compute(1);
+        if (rc) return rc;
+        if (!overflowed(c, &rc)) break;
+        // the list was built in this call (a kept list never overflows):
+        // repeat it with wider rows, the caller's forces untouched
+        rc = widen_list(c, n, counter);
+        if (rc) return rc;
+    }
     if (rc) return rc;
     rc = c->b->get_arrays(0, 2, fx, fy, fz);
     if (rc) return rc;
-    rc = c->b->fetch_state();
-    if (rc) return rc;
     const dmdx::SysState& T = c->b->hstate[0];
-    if (T.flags & dmdx::FLAG_NBR_OVERFLOW)
-        return dmdx::fail(DMDX_E_MISMATCH, "neighbour list overflow");
     *energy_out += T.pe;
     if (!std::isfinite(T.pe)) return dmdx::fail(DMDX_E_NAN, c->name + ": non-finite energy");
     return DMDX_OK;
@@ -486,7 +549,7 @@ int dmdx_component_compute_noalloc(dmdx_component* c, const double* x, const dou
     if (c->type == dmdx::COMP_LJ || c->type == dmdx::COMP_COULOMB_DIRECT)
         return dmdx::fail(DMDX_E_PARM, c->name + " keeps its pair list between calls: it has no "
                                                  "allocation-free form");
-    int rc = check_cell(cell);
+    int rc = check_cell(cell, DMDX_CELL_ORTHORHOMBIC);
     if (rc) return rc;
     dmdx::NoallocDevice* nd = nullptr;
     rc = dmdx::noalloc_device(&nd);
@@ -501,6 +564,7 @@ int dmdx_component_compute_noalloc(dmdx_component* c, const double* x, const dou
     std::vector<double> zero((size_t)n, 0.0);
     Batch b;
     rc = make_batch(c, n, zero.data(), zero.data(), zero.data(), cell, opt, b);
+    if (rc == DMDX_OK) rc = b.set_cell(0, cell, DMDX_CELL_ORTHORHOMBIC);
     if (rc == DMDX_OK) rc = b.set_arrays(0, 0, x, y, z, true);
     if (rc == DMDX_OK) rc = b.set_arrays(0, 2, fx, fy, fz, true);
     if (rc == DMDX_OK) rc = b.compute(1);

@@ -36,9 +36,9 @@ __device__ __forceinline__ void con_vector(const SysDev& S, const double* __rest
     dx = x[aj] - x[ai];
     dy = y[aj] - y[ai];
     dz = z[aj] - z[ai];
-    dx = min_image_dmd(dx, S.L[0], S.invL[0]);
-    dy = min_image_dmd(dy, S.L[1], S.invL[1]);
-    dz = min_image_dmd(dz, S.L[2], S.invL[2]);
+    dx = min_image_dmd(dx, S.L[0], S.iminv[0]);
+    dy = min_image_dmd(dy, S.L[1], S.iminv[1]);
+    dz = min_image_dmd(dz, S.L[2], S.iminv[2]);
 }
 
 }  // namespace

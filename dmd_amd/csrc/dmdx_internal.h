@@ -54,6 +54,12 @@ struct SysDev {
     int32_t precision;  // DMDX_FP64 / DMDX_MIXED
     double L[3];        // box
     double invL[3];     // 1.0 / L (FP64 division, as the reference)
+    // factor of the minimum image d - round(d * iminv) * L: 1 / L for an
+    // orthorhombic cell, 0 for DMD's other cell types, whose image is the
+    // identity (src/core/cell.cpp:26-38)
+    double iminv[3];
+    double vdet;        // > 0: volume of the cell matrix (Cell::volume, the
+                        // determinant), used by PME; 0: L[0] * L[1] * L[2]
     // pair terms
     int32_t use_lj;
     int32_t coul_pairs;  // real-space erfc pairs computed from the list
@@ -374,6 +380,15 @@ inline unsigned grid_for(int64_t units)
 __device__ __forceinline__ double min_image_dmd(double d, double L, double inv)
 {
     return d - round(d * inv) * L;
+}
+
+// Volume of a system's cell, Cell::volume (src/core/cell.cpp:19-24): the
+// determinant of its matrix where one was given, else the determinant of the
+// diagonal matrix of the edges, which that expansion evaluates as
+// L0 * (L1 * L2) exactly (every other term adds a signed zero).
+__host__ __device__ __forceinline__ double sys_volume(const SysDev& S)
+{
+    return S.vdet > 0.0 ? S.vdet : S.L[0] * (S.L[1] * S.L[2]);
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs (workgroup p runs on

@@ -196,7 +196,9 @@ int Batch::create(const dmdx_system_desc* descs, int nsys, const BatchOptions& o
         for (int k = 0; k < 3; ++k) {
             S.L[k] = D.box[k];
             S.invL[k] = 1.0 / D.box[k];
+            S.iminv[k] = S.invL[k];
         }
+        S.vdet = 0.0;
         const bool faithful = D.physics == DMDX_FAITHFUL;
         // pair terms: LJ always from the list; real-space erfc only in
         // CORRECTED (build_simulation never fills CoulombDirect's list)
@@ -627,7 +629,7 @@ int Batch::create(const dmdx_system_desc* descs, int nsys, const BatchOptions& o
             S.ginf_static = 1;
             const int Kx = S.K[0], Ky = S.K[1], Kz = S.K[2];
             const double Lx = S.L[0], Ly = S.L[1], Lz = S.L[2];
-            const double V = Lx * Ly * Lz;
+            const double V = sys_volume(S);
             const double* bmx = bmod.data() + S.bm_off[0];
             const double* bmy = bmod.data() + S.bm_off[1];
             const double* bmz = bmod.data() + S.bm_off[2];
@@ -1215,20 +1217,40 @@ int Batch::set_arrays(int s, int what, const double* a, const double* b, const d
     return DMDX_OK;
 }
 
-int Batch::set_box(int s, const double L[3])
+int Batch::set_cell(int s, const double m[9], int type)
 {
     TRY(check_sys(s));
+    if (type < DMDX_CELL_ORTHORHOMBIC || type > DMDX_CELL_RHOMBIC_DODECAHEDRON)
+        return fail(DMDX_E_PARM, "unknown cell type");
+    const double L[3] = {m[0], m[4], m[8]};
+    for (int k = 0; k < 3; ++k)
+        if (!(L[k] > 0.0)) return fail(DMDX_E_PARM, "cell edges (the matrix diagonal) must be positive");
+    // Cell::volume (src/core/cell.cpp:19-24), in its operation order
+    const double vdet = m[0] * (m[4] * m[8] - m[5] * m[7]) - m[1] * (m[3] * m[8] - m[5] * m[6]) +
+                        m[2] * (m[3] * m[7] - m[4] * m[6]);
+    if (!(vdet > 0.0)) return fail(DMDX_E_PARM, "the cell matrix must have a positive determinant");
+    const bool open = type != DMDX_CELL_ORTHORHOMBIC;
     SysDev& S = hsys[s];
-    bool same = S.L[0] == L[0] && S.L[1] == L[1] && S.L[2] == L[2];
-    if (same) return DMDX_OK;
+    if (open && S.precision != DMDX_FP64)
+        return fail(DMDX_E_PARM, "non-orthorhombic cells (identity minimum image) run on the FP64 "
+                                 "path only: the mixed path's cell-local frames are periodic");
+    const bool same_box = S.L[0] == L[0] && S.L[1] == L[1] && S.L[2] == L[2];
+    if (same_box && S.vdet == vdet && (S.iminv[0] == 0.0) == open) return DMDX_OK;
     TRY(sync());
     // the device copy may have been changed by a barostat: read it back first
     TRY(copy(&S, d.sys + s, sizeof(SysDev), hipMemcpyDeviceToHost));
+    const bool box_changed = !(S.L[0] == L[0] && S.L[1] == L[1] && S.L[2] == L[2]);
+    const double v_before = sys_volume(S);
     for (int k = 0; k < 3; ++k) {
-        if (!(L[k] > 0.0)) return fail(DMDX_E_PARM, "box edges must be positive");
         S.L[k] = L[k];
         S.invL[k] = 1.0 / L[k];
+        // the reference's image is the identity for non-orthorhombic types
+        S.iminv[k] = open ? 0.0 : S.invL[k];
     }
+    S.vdet = vdet;
+    // the mixed path's influence function was tabulated for the box and
+    // volume the system was created with: another cell computes it per call
+    if (box_changed || vdet != v_before) S.ginf_static = 0;
     if (!d.any_wrap_mixed && mixed_list_wraps(S)) {
         d.any_wrap_mixed = true;  // the per-atom build joins the launches
         invalidate_graph();
@@ -1293,7 +1315,9 @@ int Batch::adopt(Batch& src, int ss, int ds)
     for (int k = 0; k < 3; ++k) {  // a barostat may have changed the box
         B.L[k] = A.L[k];
         B.invL[k] = A.invL[k];
+        B.iminv[k] = A.iminv[k];
     }
+    B.vdet = A.vdet;
     hsys[ds].L[0] = B.L[0], hsys[ds].L[1] = B.L[1], hsys[ds].L[2] = B.L[2];
     hsys[ds].invL[0] = B.invL[0], hsys[ds].invL[1] = B.invL[1], hsys[ds].invL[2] = B.invL[2];
     if (!d.any_wrap_mixed && mixed_list_wraps(B)) {

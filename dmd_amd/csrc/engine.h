@@ -158,7 +158,10 @@ public:
     int get_arrays(int s, int what, double* a, double* b, double* c, bool on_device = false);
     int set_arrays(int s, int what, const double* a, const double* b, const double* c,
                    bool on_device = false);
-    int set_box(int s, const double L[3]);
+    // DMD's Cell: box edges from the diagonal of m, the minimum image of
+    // `type` (DMDX_CELL_*: identity unless orthorhombic), PME volume from the
+    // determinant
+    int set_cell(int s, const double m[9], int type);
     int set_counter(int s, int counter);
     int set_active(int s, int active);
     // step targets: system s leaves the batch (becomes inactive) at the start

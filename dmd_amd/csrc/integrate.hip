@@ -506,7 +506,7 @@ __global__ void __launch_bounds__(FIN_THREADS) k_finalize(const SysDev* __restri
     if (S.pme) {
         e[6] += S.self_energy;
         if (S.physics != DMDX_FAITHFUL && S.qsum != 0.0) {
-            double V = S.L[0] * S.L[1] * S.L[2];
+            double V = sys_volume(S);
             e[6] += -PI * KE * S.qsum * S.qsum / (2.0 * V * S.alpha * S.alpha);
         }
     }
@@ -601,7 +601,7 @@ __global__ void __launch_bounds__(256) k_baro_finalize(SysDev* __restrict__ sys,
     if (threadIdx.x != 0) return;
     T.lambda = 1.0;
     T.ke_out = ke;
-    double volume = S.L[0] * S.L[1] * S.L[2];
+    double volume = sys_volume(S);
     double scale = 1.0;
     T.xscale = 1.0;
     if (volume <= 0.0) return;
@@ -626,7 +626,9 @@ __global__ void __launch_bounds__(256) k_baro_finalize(SysDev* __restrict__ sys,
     for (int d = 0; d < 3; ++d) {
         S.L[d] *= scale;
         S.invL[d] = 1.0 / S.L[d];
+        S.iminv[d] = S.invL[d];  // engine systems have orthorhombic cells
     }
+    S.vdet = 0.0;  // the volume follows the scaled edges
     T.valid = 0;
 }
 

@@ -634,7 +634,7 @@ __device__ __forceinline__ void k_nbr_build_unit(int64_t unit_,
     const double4 pi4 = pos4s[t];
     const double xi = pi4.x, yi = pi4.y, zi = pi4.z;
     const double Lx = S.L[0], Ly = S.L[1], Lz = S.L[2];
-    const double ix = S.invL[0], iy = S.invL[1], iz = S.invL[2];
+    const double ix = S.iminv[0], iy = S.iminv[1], iz = S.iminv[2];
     const double rl2 = S.rlist2;
     const bool check_ex = ex_start[i + 1] > ex_start[i];
     const bool sorted_ids = S.precision != DMDX_FP64;
@@ -884,9 +884,9 @@ __device__ __forceinline__ void k_nbr_build_mixed_unit(int64_t t,
             if (kc < ncand && S.atom_off + slot != t) {
                 float dx = lj.x + sh.x, dy = lj.y + sh.y, dz = lj.z + sh.z;
                 if constexpr (WRAP) {
-                    dx -= rintf(dx * (float)S.invL[0]) * (float)S.L[0];
-                    dy -= rintf(dy * (float)S.invL[1]) * (float)S.L[1];
-                    dz -= rintf(dz * (float)S.invL[2]) * (float)S.L[2];
+                    dx -= rintf(dx * (float)S.iminv[0]) * (float)S.L[0];
+                    dy -= rintf(dy * (float)S.iminv[1]) * (float)S.L[1];
+                    dz -= rintf(dz * (float)S.iminv[2]) * (float)S.L[2];
                 }
                 const float r2f = dx * dx + dy * dy + dz * dz;
                 hit = r2f < lo2;
@@ -895,9 +895,9 @@ __device__ __forceinline__ void k_nbr_build_mixed_unit(int64_t t,
                     // src/force/lennard_jones.cpp:29-36)
                     const double4 pi4 = pos4s[t];
                     const double4 pj = pos4s[S.atom_off + slot];
-                    double ddx = min_image_dmd(pj.x - pi4.x, S.L[0], S.invL[0]);
-                    double ddy = min_image_dmd(pj.y - pi4.y, S.L[1], S.invL[1]);
-                    double ddz = min_image_dmd(pj.z - pi4.z, S.L[2], S.invL[2]);
+                    double ddx = min_image_dmd(pj.x - pi4.x, S.L[0], S.iminv[0]);
+                    double ddy = min_image_dmd(pj.y - pi4.y, S.L[1], S.iminv[1]);
+                    double ddz = min_image_dmd(pj.z - pi4.z, S.L[2], S.iminv[2]);
                     hit = ddx * ddx + ddy * ddy + ddz * ddz < S.rlist2;
                 }
                 if (hit && check_ex) {
@@ -1127,9 +1127,9 @@ __global__ void __launch_bounds__(64 * CB_WAVES) k_nbr_build_cells(
                             // src/force/lennard_jones.cpp:29-36)
                             const double4 pi4 = pos4s[S.atom_off + lti];
                             const double4 pj = pos4s[S.atom_off + slot];
-                            const double ddx = min_image_dmd(pj.x - pi4.x, S.L[0], S.invL[0]);
-                            const double ddy = min_image_dmd(pj.y - pi4.y, S.L[1], S.invL[1]);
-                            const double ddz = min_image_dmd(pj.z - pi4.z, S.L[2], S.invL[2]);
+                            const double ddx = min_image_dmd(pj.x - pi4.x, S.L[0], S.iminv[0]);
+                            const double ddy = min_image_dmd(pj.y - pi4.y, S.L[1], S.iminv[1]);
+                            const double ddz = min_image_dmd(pj.z - pi4.z, S.L[2], S.iminv[2]);
                             hit = ddx * ddx + ddy * ddy + ddz * ddz < S.rlist2;
                         }
                         uint64_t bal = __ballot(hit);

@@ -56,7 +56,7 @@ __device__ __forceinline__ void k_pair_fp64_unit(int64_t unit_, const SysDev* __
             fzi = fz[i];
         }
         const double Lx = S.L[0], Ly = S.L[1], Lz = S.L[2];
-        const double ix = S.invL[0], iy = S.invL[1], iz = S.invL[2];
+        const double ix = S.iminv[0], iy = S.iminv[1], iz = S.iminv[2];
         const int nt = S.ntypes;
         const double* sg = lj_sig + S.lj_off;
         const double* ep = lj_eps + S.lj_off;
@@ -192,7 +192,7 @@ __device__ __forceinline__ void k_pair_mixed_unit(int64_t unit_,
     const bool wrap = T.nub[0] < full || T.nub[1] < full || T.nub[2] < full;
     const int64_t a0 = S.atom_off;
     const float Lx = (float)S.L[0], Ly = (float)S.L[1], Lz = (float)S.L[2];
-    const float ix = (float)S.invL[0], iy = (float)S.invL[1], iz = (float)S.invL[2];
+    const float ix = (float)S.iminv[0], iy = (float)S.iminv[1], iz = (float)S.iminv[2];
     const int nt = S.ntypes;
     const bool lj = S.use_lj, cp = COUL;
     const bool typed = lj && nt > 1;

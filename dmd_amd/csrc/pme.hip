@@ -644,7 +644,7 @@ __device__ __forceinline__ void k_fft_pass_unit(int64_t unit_, const FftTile* __
         // expressions of src/force/coulomb_pme.cpp:165-196.
         __shared__ double red[256];
         const double Lx = S.L[0], Ly = S.L[1], Lz = S.L[2];
-        const double V = Lx * Ly * Lz;
+        const double V = sys_volume(S);
         const double alpha = S.alpha;
         const double* bmx = bmod + S.bm_off[0];
         const double* bmy = bmod + S.bm_off[1];

@@ -56,6 +56,17 @@ enum { DMDX_FAITHFUL = 0, DMDX_CORRECTED = 1 };
  *        integration, thermostat and energy/virial reductions in FP64. */
 enum { DMDX_FP64 = 0, DMDX_MIXED = 1 };
 
+/* ---- cell type (CellType, src/core/cell.h:3-8).  The minimum image wraps
+ *      by the matrix diagonal for ORTHORHOMBIC and is the identity for every
+ *      other type (src/core/cell.cpp:26-38); the volume is the determinant
+ *      of the matrix for all types (:19-24). */
+enum {
+    DMDX_CELL_ORTHORHOMBIC = 0,
+    DMDX_CELL_TRICLINIC = 1,
+    DMDX_CELL_HEXAGONAL = 2,
+    DMDX_CELL_RHOMBIC_DODECAHEDRON = 3
+};
+
 /* ---- electrostatics (SimulationConfig::coulomb_type,
  *      src/sim/simulation_config.h:52-56) */
 enum { DMDX_COULOMB_NONE = 0, DMDX_COULOMB_CUTOFF = 1, DMDX_COULOMB_PME = 2 };
@@ -266,6 +277,11 @@ int dmdx_coulomb_exclusion_create(dmdx_component **out);
 /* CoulombExclusion::set_exclusions (coulomb_exclusion.h:21) */
 int dmdx_coulomb_exclusion_set_exclusions(dmdx_component *c, const int32_t *excl_i,
                                           const int32_t *excl_j, int64_t n);
+/* No counterpart in the reference (its exclusion term is the bare 1/r of
+ * FAITHFUL, coulomb_exclusion.cpp:29-54): the Ewald coefficient of the erf
+ * form that DMDX_CORRECTED subtracts, which must be the PME's.  Default 3.5,
+ * PMEParams' default (coulomb_pme.h:7-12); unused under DMDX_FAITHFUL. */
+int dmdx_coulomb_exclusion_set_ewald_coefficient(dmdx_component *c, double alpha);
 
 /* CoulombPME(PMEParams{cutoff, ewald_coefficient, nx, ny, nz, spline_order})
  * (src/force/coulomb_pme.h:7-16); spline_order 4 only, as the reference. */
@@ -295,7 +311,8 @@ int dmdx_harmonic_improper_create(int64_t n, const int32_t *i, const int32_t *j,
 int dmdx_component_set_charges(dmdx_component *c, const double *charges, int64_t n);
 
 /* rebuild_pair_list of LennardJones / CoulombDirect (lennard_jones.h:29-34,
- * coulomb_direct.h:27-32); cell is Cell::matrix (row-major 3x3, orthorhombic). */
+ * coulomb_direct.h:27-32); cell is Cell::matrix (row-major 3x3) of an
+ * orthorhombic Cell. */
 int dmdx_component_rebuild_pair_list(dmdx_component *c, const double *x, const double *y,
                                      const double *z, int64_t n, const double cell[9]);
 
@@ -303,6 +320,21 @@ int dmdx_component_rebuild_pair_list(dmdx_component *c, const double *x, const d
 int dmdx_component_compute(dmdx_component *c, const double *x, const double *y, const double *z,
                            int64_t n, const double cell[9], double *fx, double *fy, double *fz,
                            double *energy_out);
+
+/* The same two calls for a Cell of any CellType (cell_type = DMDX_CELL_*):
+ * the Cell(matrix, type) constructor (src/core/cell.h:12) and the
+ * std::span/Cell signatures of rebuild_pair_list / compute
+ * (force_component.h:15-16).  The matrix diagonal must be positive.  Cells
+ * other than orthorhombic run on the FP64 path only (DMDX_MIXED keeps
+ * cell-local frames that assume a wrapped box) and return DMDX_E_PARM under
+ * DMDX_MIXED. */
+int dmdx_component_rebuild_pair_list_cell(dmdx_component *c, const double *x, const double *y,
+                                          const double *z, int64_t n, const double cell[9],
+                                          int32_t cell_type);
+int dmdx_component_compute_cell(dmdx_component *c, const double *x, const double *y,
+                                const double *z, int64_t n, const double cell[9],
+                                int32_t cell_type, double *fx, double *fy, double *fz,
+                                double *energy_out);
 
 /* Allocation-free ForceComponent::compute.  No counterpart in the reference:
  * for callers that cannot allocate device memory (code under HIP graph

@@ -57,15 +57,13 @@ struct ComponentHandle {
     ~ComponentHandle() { dmdx_component_destroy(h); }
 };
 
-// Cell::minimum_image is the identity for the non-orthorhombic types
-// (src/core/cell.cpp:35-37): no periodic images at all.  The device path
-// implements the orthorhombic image only and refuses the others.
-inline void require_orthorhombic(const Cell& cell, const std::string& name)
-{
-    if (cell.type != CellType::orthorhombic)
-        throw ForceException(ForceError::ParmError, name.c_str(),
-                             "only orthorhombic cells are supported on the device");
-}
+// CellType (src/core/cell.h:3-8) has the values of DMDX_CELL_*: the device
+// path takes the type with the matrix (identity minimum image unless
+// orthorhombic, src/core/cell.cpp:26-38; volume from the determinant).
+inline int32_t cell_type_of(const Cell& cell) { return (int32_t)cell.type; }
+static_assert((int)CellType::triclinic == DMDX_CELL_TRICLINIC &&
+                  (int)CellType::rhombic_dodecahedron == DMDX_CELL_RHOMBIC_DODECAHEDRON,
+              "CellType and DMDX_CELL_* differ");
 
 inline void compute_on(dmdx_component* h, const std::string& name, std::span<const double> x,
                        std::span<const double> y, std::span<const double> z, const Cell& cell,
@@ -75,9 +73,9 @@ inline void compute_on(dmdx_component* h, const std::string& name, std::span<con
     if (y.size() != x.size() || z.size() != x.size() || fx.size() != x.size() ||
         fy.size() != x.size() || fz.size() != x.size())
         throw ForceException(ForceError::Mismatch, name.c_str(), "span sizes differ");
-    require_orthorhombic(cell, name);
-    check(dmdx_component_compute(h, x.data(), y.data(), z.data(), (int64_t)x.size(), cell.matrix,
-                                 fx.data(), fy.data(), fz.data(), &energy_out),
+    check(dmdx_component_compute_cell(h, x.data(), y.data(), z.data(), (int64_t)x.size(),
+                                      cell.matrix, cell_type_of(cell), fx.data(), fy.data(),
+                                      fz.data(), &energy_out),
           name);
 }
 
@@ -118,9 +116,9 @@ public:
     void rebuild_pair_list_gpu(std::span<const double> x, std::span<const double> y,
                                std::span<const double> z, const Cell& cell)
     {
-        require_orthorhombic(cell, "LennardJones");
-        check(dmdx_component_rebuild_pair_list(h_.h, x.data(), y.data(), z.data(),
-                                               (int64_t)x.size(), cell.matrix),
+        check(dmdx_component_rebuild_pair_list_cell(h_.h, x.data(), y.data(), z.data(),
+                                                    (int64_t)x.size(), cell.matrix,
+                                                    cell_type_of(cell)),
               "LennardJones");
         LennardJones::set_step_since_rebuild(0);
     }
@@ -158,20 +156,23 @@ public:
     void rebuild_pair_list(std::span<const double> x, std::span<const double> y,
                            std::span<const double> z, const Cell& cell)
     {
-        require_orthorhombic(cell, name_);
-        check(dmdx_component_rebuild_pair_list(h_.h, x.data(), y.data(), z.data(),
-                                               (int64_t)x.size(), cell.matrix),
+        check(dmdx_component_rebuild_pair_list_cell(h_.h, x.data(), y.data(), z.data(),
+                                                    (int64_t)x.size(), cell.matrix,
+                                                    cell_type_of(cell)),
               name_);
     }
 };
 
 class GpuCoulombExclusion : public GpuComponent {
 public:
+    // alpha: Ewald coefficient of the DMDX_CORRECTED erf form (unused by
+    // DMD's own bare 1/r term)
     GpuCoulombExclusion(std::span<const double> charges, std::span<const int> ei,
-                        std::span<const int> ej)
+                        std::span<const int> ej, double alpha = 3.5)
         : GpuComponent("CoulombExclusion")
     {
         check(dmdx_coulomb_exclusion_create(&h_.h), name_);
+        check(dmdx_coulomb_exclusion_set_ewald_coefficient(h_.h, alpha), name_);
         check(dmdx_component_set_charges(h_.h, charges.data(), (int64_t)charges.size()), name_);
         std::vector<int32_t> a(ei.begin(), ei.end()), b(ej.begin(), ej.end());
         check(dmdx_coulomb_exclusion_set_exclusions(h_.h, a.data(), b.data(), (int64_t)a.size()),
@@ -260,10 +261,12 @@ inline std::unique_ptr<ForceEngine> build_force_engine_gpu(const SimulationConfi
         int g = std::max(8, static_cast<int>(cfg.box_size / cfg.pme_grid_spacing + 0.5));
         fe->add_component(std::make_unique<GpuCoulombPME>(cfg.coulomb_cutoff, cfg.ewald_coeff, g, g,
                                                           g, cfg.pme_order, q));
-        fe->add_component(std::make_unique<GpuCoulombExclusion>(q, cfg.excl_i, cfg.excl_j));
+        fe->add_component(
+            std::make_unique<GpuCoulombExclusion>(q, cfg.excl_i, cfg.excl_j, cfg.ewald_coeff));
     } else if ((cfg.coulomb_type == "direct" || cfg.coulomb_type == "cutoff") &&
                !cfg.excl_i.empty()) {
-        fe->add_component(std::make_unique<GpuCoulombExclusion>(q, cfg.excl_i, cfg.excl_j));
+        fe->add_component(
+            std::make_unique<GpuCoulombExclusion>(q, cfg.excl_i, cfg.excl_j, cfg.ewald_coeff));
     }
     return fe;
 }

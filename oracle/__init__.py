@@ -92,8 +92,13 @@ def P(a):
 
 
 def cell9(box):
+    """Row-major Cell::matrix from three edges (orthorhombic) or a full
+    3x3 / 9-element matrix (any cell type)."""
+    b = np.asarray(box, dtype=float).ravel()
+    if b.size == 9:
+        return b.copy()
     c = np.zeros(9)
-    c[0], c[4], c[8] = box
+    c[0], c[4], c[8] = b
     return c
 
 
@@ -107,6 +112,42 @@ def zeros3(n):
 
 
 # ------------------------------------------------------------- core
+
+CELL_TYPES = ("orthorhombic", "triclinic", "hexagonal", "rhombic_dodecahedron")
+
+
+class cell_type:
+    """Context manager: every cell the oracle (and, with ref=True, the
+    reference build) sees inside the block has this CellType
+    (src/core/cell.h:3-8); orthorhombic again on exit."""
+
+    def __init__(self, t, ref=False):
+        self.t = CELL_TYPES.index(t) if isinstance(t, str) else int(t)
+        self.ref = ref
+
+    def __enter__(self):
+        lib().orc_set_cell_type(C.c_int(self.t))
+        if self.ref:
+            ref().ref_set_cell_type(C.c_int(self.t))
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_cell_type(C.c_int(0))
+        if self.ref:
+            ref().ref_set_cell_type(C.c_int(0))
+        return False
+
+
+def cell_volume(m):
+    """Cell::volume of a 3x3 row-major matrix (src/core/cell.cpp:19-24)."""
+    return lib().orc_cell_volume(P(f64(np.asarray(m, float).ravel())))
+
+
+def ref_cell_volume(m, t):
+    r = ref()
+    r.ref_cell_volume.restype = C.c_double
+    return r.ref_cell_volume(P(f64(np.asarray(m, float).ravel())), C.c_int(int(t)))
+
 
 def min_image(box, d):
     c = cell9(box)

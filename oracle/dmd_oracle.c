@@ -22,10 +22,18 @@
 
 /* ------------------------------------------------------------------ core */
 
-/* src/core/cell.cpp:26-38 (orthorhombic branch; the triclinic branch of the
- * reference is an identity stub and no configuration uses it). */
+/* Cell type of every cell passed to the functions below (CellType,
+ * src/core/cell.h:3-8: 0 orthorhombic, 1 triclinic, 2 hexagonal,
+ * 3 rhombic dodecahedron).  Test-side switch, orthorhombic by default. */
+static int g_cell_type = 0;
+
+void orc_set_cell_type(int type) { g_cell_type = type; }
+
+/* src/core/cell.cpp:26-38: the orthorhombic image; every other cell type
+ * leaves the displacement unchanged (the reference's stub). */
 void orc_min_image(const double cell[9], double *dx, double *dy, double *dz)
 {
+    if (g_cell_type != 0) return;
     double inv_a = 1.0 / cell[0];
     double inv_b = 1.0 / cell[4];
     double inv_c = 1.0 / cell[8];

@@ -62,6 +62,7 @@ extern "C" {
 #define ORC_PI 3.14159265358979323846
 
 /* ---- core */
+void orc_set_cell_type(int type);
 void orc_min_image(const double cell[9], double *dx, double *dy, double *dz);
 double orc_cell_volume(const double cell[9]);
 

@@ -63,14 +63,36 @@ std::vector<T> vec(const T* p, int64_t n)
     return std::vector<T>(p, p + n);
 }
 
+// Cell type of the cells built below (set by ref_set_cell_type): DMD's own
+// Cell(matrix, type) with the box on the diagonal.
+CellType g_cell_type = CellType::orthorhombic;
+
+Cell make_cell(const double box[3])
+{
+    if (g_cell_type == CellType::orthorhombic) return Cell(box[0], box[1], box[2]);
+    const double m[9] = {box[0], 0.0, 0.0, 0.0, box[1], 0.0, 0.0, 0.0, box[2]};
+    return Cell(m, g_cell_type);
+}
+
 }  // namespace
 
 extern "C" {
 
 void ref_min_image(const double cell[9], double* dx, double* dy, double* dz)
 {
-    Cell c(cell[0], cell[4], cell[8]);
+    Cell c = g_cell_type == CellType::orthorhombic ? Cell(cell[0], cell[4], cell[8])
+                                                    : Cell(cell, g_cell_type);
     c.minimum_image(*dx, *dy, *dz, *dx, *dy, *dz);
+}
+
+// CellType of the cells the functions of this file build (0 orthorhombic,
+// 1 triclinic, 2 hexagonal, 3 rhombic dodecahedron; src/core/cell.h:3-8)
+void ref_set_cell_type(int type) { g_cell_type = static_cast<CellType>(type); }
+
+// Cell::volume of a full 3x3 matrix (src/core/cell.cpp:19-24)
+double ref_cell_volume(const double m[9], int type)
+{
+    return Cell(m, static_cast<CellType>(type)).volume();
 }
 
 // LennardJones: rebuild (as the first compute does) then compute `ncompute`
@@ -88,7 +110,7 @@ void ref_lj(int64_t n, const double* x, const double* y, const double* z, const 
     if (types)
         for (int64_t i = 0; i < n; ++i) s.atom_types[i] = types[i];
     lj.set_atom_types(s.atom_types);
-    Cell cell(box[0], box[1], box[2]);
+    Cell cell = make_cell(box);
     lj.compute(s.x(), s.y(), s.z(), cell, s.fx(), s.fy(), s.fz(), *energy);
     add_forces(s, fx, fy, fz);
 }
@@ -103,7 +125,7 @@ void ref_coulomb_direct(int64_t n, const double* x, const double* y, const doubl
     CoulombDirect cd(p);
     SystemData s = make_sys(n, x, y, z);
     cd.set_charges(std::span<const double>(q, size_t(n)));
-    Cell cell(box[0], box[1], box[2]);
+    Cell cell = make_cell(box);
     if (rebuild) cd.rebuild_pair_list(s.x(), s.y(), s.z(), cell);
     cd.compute(s.x(), s.y(), s.z(), cell, s.fx(), s.fy(), s.fz(), *energy);
     add_forces(s, fx, fy, fz);
@@ -118,7 +140,7 @@ void ref_exclusion(int64_t n, const double* x, const double* y, const double* z,
     ce.set_charges(std::span<const double>(q, size_t(n)));
     std::vector<int> vi(ei, ei + ne), vj(ej, ej + ne);
     ce.set_exclusions(vi, vj);
-    Cell cell(box[0], box[1], box[2]);
+    Cell cell = make_cell(box);
     ce.compute(s.x(), s.y(), s.z(), cell, s.fx(), s.fy(), s.fz(), *energy);
     add_forces(s, fx, fy, fz);
 }
@@ -135,7 +157,7 @@ void ref_bonds(int64_t n, const double* x, const double* y, const double* z, con
     p.r0 = vec(r0, nb);
     HarmonicBond b(std::move(p));
     SystemData s = make_sys(n, x, y, z);
-    Cell cell(box[0], box[1], box[2]);
+    Cell cell = make_cell(box);
     b.compute(s.x(), s.y(), s.z(), cell, s.fx(), s.fy(), s.fz(), *energy);
     add_forces(s, fx, fy, fz);
 }
@@ -154,7 +176,7 @@ void ref_angles(int64_t n, const double* x, const double* y, const double* z, co
     p.theta0 = vec(t0, na);
     HarmonicAngle a(std::move(p));
     SystemData s = make_sys(n, x, y, z);
-    Cell cell(box[0], box[1], box[2]);
+    Cell cell = make_cell(box);
     a.compute(s.x(), s.y(), s.z(), cell, s.fx(), s.fy(), s.fz(), *energy);
     add_forces(s, fx, fy, fz);
 }
@@ -175,7 +197,7 @@ void ref_dihedrals(int64_t n, const double* x, const double* y, const double* z,
     p.phi0 = vec(p0, nd);
     PeriodicDihedral d(std::move(p));
     SystemData s = make_sys(n, x, y, z);
-    Cell cell(box[0], box[1], box[2]);
+    Cell cell = make_cell(box);
     d.compute(s.x(), s.y(), s.z(), cell, s.fx(), s.fy(), s.fz(), *energy);
     add_forces(s, fx, fy, fz);
 }
@@ -195,7 +217,7 @@ void ref_impropers(int64_t n, const double* x, const double* y, const double* z,
     p.phi0 = vec(p0, nd);
     HarmonicImproper d(std::move(p));
     SystemData s = make_sys(n, x, y, z);
-    Cell cell(box[0], box[1], box[2]);
+    Cell cell = make_cell(box);
     d.compute(s.x(), s.y(), s.z(), cell, s.fx(), s.fy(), s.fz(), *energy);
     add_forces(s, fx, fy, fz);
 }
@@ -270,7 +292,7 @@ void ref_apply(int kind, int64_t n, const double* m, double* x, double* y, doubl
                double box[3], double dt, double target, double tau, double beta, double* state)
 {
     SystemData s = fill_state(n, m, x, y, z, vx, vy, vz, fx, fy, fz);
-    Cell cell(box[0], box[1], box[2]);
+    Cell cell = make_cell(box);
     Integrator integ;
     switch (kind) {
     case 0: integ.half_kick(s, dt); break;
@@ -329,7 +351,7 @@ void ref_constraints(int kind, int64_t nc, const int32_t* ci, const int32_t* cj,
 {
     std::vector<double> zero(size_t(n), 0.0);
     SystemData s = fill_state(n, m, x, y, z, vx, vy, vz, zero.data(), zero.data(), zero.data());
-    Cell cell(box[0], box[1], box[2]);
+    Cell cell = make_cell(box);
     ConstraintsParams p;
     p.i.assign(ci, ci + nc);
     p.j.assign(cj, cj + nc);

@@ -14,6 +14,8 @@
 //                                 SimulationEngine::step
 //                                 (src/sim/simulation_engine.cpp:54-86);
 //                              B: the same loop with build_force_engine_gpu;
+//                              B': A and B again under a triclinic Cell
+//                                  (identity minimum image);
 //                              C: DeviceSimulationEngine (FP64, faithful),
 //                                 plus a checkpoint save/load round trip in
 //                                 DMD's checkpoint format.
@@ -234,13 +236,13 @@ void interface_checks(bool have_gpu)
     expect(mismatch, "span size mismatch raises ForceException(Mismatch)");
     bool refused = false;
     try {
-        const double tri[9] = {3.0, 0.0, 0.0, 0.5, 3.0, 0.0, 0.0, 0.0, 3.0};
+        const double flat[9] = {3.0, 0.0, 0.0, 0.5, 0.0, 0.0, 0.0, 0.0, 3.0};
         std::vector<double> fx(4, 0.0), fy(4, 0.0), fz(4, 0.0);
-        c->compute(x, y, z, Cell(tri, CellType::triclinic), fx, fy, fz, e);
+        c->compute(x, y, z, Cell(flat, CellType::triclinic), fx, fy, fz, e);
     } catch (const ForceException& ex) {
         refused = ex.err == ForceError::ParmError;
     }
-    expect(refused, "a triclinic Cell raises ForceException(ParmError)");
+    expect(refused, "a Cell with a zero diagonal entry raises ForceException(ParmError)");
     if (!have_gpu) {
         bool gpu_err = false;
         std::vector<double> fx(4, 0.0), fy(4, 0.0), fz(4, 0.0);
@@ -292,6 +294,25 @@ int main(int argc, char** argv)
     expect(lj_a && lj_b && lj_a->step_since_rebuild() == lj_b->step_since_rebuild(),
            "LJ rebuild counters agree");
 
+    // the same loop under a triclinic Cell: DMD's minimum image is then the
+    // identity (src/core/cell.cpp:26-38), on DMD's components and on the GPU
+    const double tri_m[9] = {3.0, 0.0, 0.0, 0.4, 3.0, 0.0, 0.3, 0.2, 3.0};
+    const Cell tri(tri_m, CellType::triclinic);
+    auto fe_cpu_t = cpu_force_engine(cfg);
+    auto fe_gpu_t = dmdx_dmd::build_force_engine_gpu(cfg);
+    SystemData at = initial_state(cfg), bt = initial_state(cfg);
+    double worst_t = 0.0;
+    for (int k = 0; k < n_steps; ++k) {
+        dmd_step(*fe_cpu_t, in, at, tri, cfg.dt);
+        dmd_step(*fe_gpu_t, in, bt, tri, cfg.dt);
+        worst_t = std::fmax(worst_t, rel3(at, bt, 2));
+    }
+    const double pos_t = rel3(at, bt, 0);
+    const double pe_t = std::fabs(at.potential_energy - bt.potential_energy) /
+                        std::fmax(1e-300, std::fabs(at.potential_energy));
+    // the identity image changes the physics: the periodic run differs
+    expect(rel3(a, at, 2) > 1e-6, "the triclinic run differs from the periodic one");
+
     dmdx_dmd::DeviceSimulationEngine dev(cfg, initial_state(cfg), DMDX_FP64, DMDX_FAITHFUL);
     dev.steps(n_steps);
     SystemData c = dev.system();
@@ -320,6 +341,8 @@ int main(int argc, char** argv)
     const double tol = 1e-10;
     expect(worst_b <= tol && pos_b <= tol && vel_b <= tol && pe_b <= tol,
            "GPU components inside DMD's loop match DMD's components");
+    expect(worst_t <= tol && pos_t <= tol && pe_t <= tol,
+           "GPU components under a triclinic Cell match DMD's components");
     expect(pos_c <= tol && vel_c <= tol && frc_c <= tol,
            "DeviceSimulationEngine matches DMD's loop");
     expect(restart == 0.0, "restart from the checkpoint is bit-exact");
@@ -328,8 +351,10 @@ int main(int argc, char** argv)
         "\"components_force_rel\": %.3e, \"components_pos_rel\": %.3e, "
         "\"components_vel_rel\": %.3e, \"components_pe_rel\": %.3e, "
         "\"device_pos_rel\": %.3e, \"device_vel_rel\": %.3e, \"device_force_rel\": %.3e, "
+        "\"triclinic_force_rel\": %.3e, \"triclinic_pos_rel\": %.3e, "
+        "\"triclinic_pe_rel\": %.3e, "
         "\"restart_rel\": %.3e, \"lj_counter\": %d, \"ok\": %s}\n",
-        a.n_atoms, n_steps, worst_b, pos_b, vel_b, pe_b, pos_c, vel_c, frc_c, restart,
-        lj_counter, g_fail ? "false" : "true");
+        a.n_atoms, n_steps, worst_b, pos_b, vel_b, pe_b, pos_c, vel_c, frc_c, worst_t, pos_t,
+        pe_t, restart, lj_counter, g_fail ? "false" : "true");
     return g_fail;
 }

@@ -205,6 +205,58 @@ def main():
     v.update(water_pos0=spec.pos, water_vel0=spec.vel, water_pos=st["pos"],
              water_vel=st["vel"], water_force=st["force"])
 
+    # non-orthorhombic cells: DMD's minimum image is the identity for every
+    # cell type but orthorhombic (src/core/cell.cpp:26-38) and its volume is
+    # the determinant (:19-24).  LJ (list rebuilt), real-space Coulomb,
+    # exclusions and the four bonded terms under a triclinic cell whose
+    # diagonal is the box, on an unwrapped jittered lattice that reaches past
+    # the box; the volume of full matrices of every type.
+    R.ref_set_cell_type(C.c_int(1))
+    try:
+        rng = np.random.default_rng(31)
+        g = np.stack(np.meshgrid(*[np.arange(-1, 8)] * 3, indexing="ij"), -1).reshape(-1, 3)
+        pos = g * 0.31 + rng.uniform(-0.04, 0.04, g.shape)
+        box = np.array([2.2, 2.4, 2.3])
+        sig1, eps1 = O.f64([0.3]), O.f64([0.5])
+        F, E = ref_call(R.ref_lj, pos, box, C.c_double(0.9), C.c_int32(1), O.P(sig1),
+                        O.P(eps1), None)
+        q = O.f64(rng.uniform(-1, 1, len(pos)))
+        Fc, Ec = ref_call(R.ref_coulomb_direct, pos, box, C.c_double(0.8), C.c_double(3.3),
+                          O.P(q), C.c_int(1))
+        ei = O.i32(rng.integers(0, len(pos), 60))
+        ej = O.i32((ei + 1 + rng.integers(0, 30, 60)) % len(pos))
+        Fx, Ex = ref_call(R.ref_exclusion, pos, box, O.P(q), C.c_int64(len(ei)), O.P(ei),
+                          O.P(ej))
+        v.update(tric_pos=pos, tric_box=box, tric_lj_F=F, tric_lj_E=E, tric_q=q,
+                 tric_coul_F=Fc, tric_coul_E=Ec, tric_excl_i=ei, tric_excl_j=ej,
+                 tric_excl_F=Fx, tric_excl_E=Ex)
+        n, nb = len(pos), 80
+        quad = np.array([rng.permutation(n)[:4] for _ in range(nb)], np.int32)
+        qi, qj, qk, ql = (O.i32(quad[:, c]) for c in range(4))
+        bk, br = O.f64(rng.uniform(100, 1000, nb)), O.f64(rng.uniform(0.1, 0.3, nb))
+        kt, t0 = O.f64(rng.uniform(50, 400, nb)), O.f64(rng.uniform(1.5, 2.5, nb))
+        per = O.i32(rng.integers(1, 4, nb))
+        kp, p0 = O.f64(rng.uniform(1, 10, nb)), O.f64(rng.uniform(-math.pi, math.pi, nb))
+        Fb, Eb = ref_call(R.ref_bonds, pos, box, C.c_int64(nb), O.P(qi), O.P(qj), O.P(bk),
+                          O.P(br))
+        Fa, Ea = ref_call(R.ref_angles, pos, box, C.c_int64(nb), O.P(qi), O.P(qj), O.P(qk),
+                          O.P(kt), O.P(t0))
+        Fd, Ed = ref_call(R.ref_dihedrals, pos, box, C.c_int64(nb), O.P(qi), O.P(qj), O.P(qk),
+                          O.P(ql), O.P(per), O.P(kp), O.P(p0))
+        Fi, Ei = ref_call(R.ref_impropers, pos, box, C.c_int64(nb), O.P(qi), O.P(qj),
+                          O.P(qk), O.P(ql), O.P(kp), O.P(p0))
+        v.update(tric_quad=quad, tric_bk=bk, tric_br=br, tric_kt=kt, tric_t0=t0, tric_per=per,
+                 tric_kp=kp, tric_p0=p0, tric_bond_F=Fb, tric_bond_E=Eb, tric_angle_F=Fa,
+                 tric_angle_E=Ea, tric_dih_F=Fd, tric_dih_E=Ed, tric_imp_F=Fi, tric_imp_E=Ei)
+    finally:
+        R.ref_set_cell_type(C.c_int(0))
+    mats = np.array([[2.0, 0.0, 0.0, 0.0, 2.5, 0.0, 0.0, 0.0, 3.0],
+                     [2.0, 0.3, 0.1, 0.2, 2.5, 0.4, 0.1, 0.05, 3.0],
+                     [3.1, 0.0, 0.0, -1.55, 2.684679, 0.0, 0.0, 0.0, 4.2],
+                     [2.0, 2.0, 0.0, 2.0, 0.0, 2.0, 0.0, 2.0, 2.0]])
+    v.update(cellvol_m=mats, cellvol_V=np.array(
+        [[O.ref_cell_volume(m, t) for t in range(4)] for m in mats]))
+
     arrays = {k: np.asarray(val) for k, val in v.items()}
     np.savez_compressed(OUT, **arrays)
     print(f"wrote {OUT}: {len(arrays)} arrays, {os.path.getsize(OUT)} bytes")
