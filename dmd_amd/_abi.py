@@ -16,7 +16,7 @@ DMDX_OK = 0
 STATUS_NAMES = {
     0: "DMDX_OK", -1: "DMDX_E_NAN", -2: "DMDX_E_BOND_EXPLOSION", -3: "DMDX_E_MISMATCH",
     -4: "DMDX_E_PARM", -5: "DMDX_E_GPU", -6: "DMDX_E_OOM", -7: "DMDX_E_INVALID",
-    -8: "DMDX_E_BUSY",
+    -8: "DMDX_E_BUSY", -9: "DMDX_E_IO",
 }
 
 FAITHFUL, CORRECTED = 0, 1

@@ -111,6 +111,7 @@ static bool plan_radices(int K, FftPlan& P)
 
 Batch::~Batch()
 {
+    snap.reset();  // waits for frames in flight, which read the state arrays
     for (auto& u : ev_used) {
         (void)hipEventDestroy(u.second.first);
         (void)hipEventDestroy(u.second.second);

@@ -44,6 +44,12 @@ struct BatchOptions {
 // kernel groups timed when profiling is on
 enum KernelGroup { KG_PAIR = 0, KG_LIST, KG_BONDED, KG_PME, KG_GATHER, KG_INTEGRATE, KG_COUNT };
 
+// staging of asynchronous frames (snapshot.hip)
+struct SnapshotRing;
+struct SnapshotRingDeleter {
+    void operator()(SnapshotRing* r) const;
+};
+
 // host mirror of what the host needs to decide launches
 struct HostSys {
     int n = 0;
@@ -174,6 +180,11 @@ public:
     // trajectory is the same as if it had stayed in `src`
     int adopt(Batch& src, int src_sys, int dst);
     int get_pairs(int s, int32_t* pi, int32_t* pj, int64_t cap, int64_t* count);
+    // asynchronous frame of system s at the current end of the stream
+    // (dmdx_engine_snapshot_begin / _read)
+    int snapshot_begin(int s, int64_t* ticket);
+    int snapshot_read(int64_t ticket, double* pos, double* vel, double box[3],
+                      int64_t* steps_done);
     int upload_sys(int s);
     void invalidate_graph();
 
@@ -198,6 +209,7 @@ private:
     hipError_t timed(int group, hipError_t (*fn)(const Dev&, hipStream_t),
                      hipStream_t on = nullptr);
     DevMem owned;
+    std::unique_ptr<SnapshotRing, SnapshotRingDeleter> snap;
     bool borrowed = false;  // streams and events belong to the caller (arena mode)
     std::vector<hipEvent_t> ev_pool;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_used;

@@ -15,8 +15,14 @@ instead.  Differences, all explicit:
 * ``constraint_type = "shake"`` (SHAKE after the drift, RATTLE after the
   closing kick) and the Andersen thermostat (DMD's own mt19937_64 stream)
   run on the device like the other couplings.
-* trajectory and energy files (``nstxout``, H5MD) are not written; the
-  state is read through the properties.
+* trajectories: ``run_n`` writes an H5MD frame (DMD's layout,
+  dmd_amd.trajectory) every ``nstxout`` steps, as EngineWrapper::run_n
+  (core.cpp:29-44), and ``run`` also writes DMD's binary checkpoint every
+  ``checkpoint_interval`` steps, as SimulationEngine::run
+  (simulation_engine.cpp:88-106).  Frames come from asynchronous snapshots
+  written while the next steps run; ``nstvout`` and the energy file are not
+  written, as in DMD's engine.  ``run`` and ``run_n`` share one writer
+  (DMD opens a second one on the same path, which HDF5 refuses).
 * ``precision`` (``"fp64"`` or ``"mixed"``) and ``physics`` (``"faithful"``
   reproduces DMD's arithmetic including its quirks, ``"corrected"`` the
   textbook forms) are extra attributes; the defaults reproduce DMD.
@@ -187,25 +193,89 @@ class Engine:
 
     def __init__(self, cfg: SimulationConfig):
         from .engine import Engine as _Engine
+        from .trajectory import Clock
         self._spec = spec_from_config(cfg)
         self._e = _Engine(self._spec)
         self._cfg = cfg
-        self._step = 0
+        self._clock = Clock.start(float(cfg.dt))
+        # one writer for run and run_n: DMD holds one in EngineWrapper and
+        # one in SimulationEngine, and the second to open the same path
+        # fails (HDF5 will not truncate an open file)
+        self._writer = None
+
+    def _steps(self, steps: int, every: int):
+        from . import trajectory as T
+        if every > 0:
+            if self._writer is None:
+                self._writer = T.H5MDWriter(self._cfg.trajectory_path, self._spec.n, True)
+            T.run(self._e, steps, every, self._clock, self._writer)
+            return
+        self._e.step(steps)
+        c = self._clock
+        for _ in range(steps):  # SimulationEngine::step's clock (simulation_engine.cpp:54, 82-85)
+            c.sys_time = c.time
+            c.step += 1
+            c.time += c.dt
 
     def run(self):
-        self.run_n(int(self._cfg.n_steps))
+        """SimulationEngine::run: n_steps steps, a frame every nstxout steps,
+        DMD's binary checkpoint every checkpoint_interval steps."""
+        left = int(self._cfg.n_steps)
+        every = int(self._cfg.checkpoint_interval)
+        while left > 0:
+            k = left if every <= 0 else min(left, every - self._clock.step % every)
+            self._steps(k, int(self._cfg.nstxout))
+            left -= k
+            if every > 0 and self._clock.step % every == 0:
+                self.save_checkpoint(self._cfg.checkpoint_path)
 
     def run_n(self, steps: int):
-        self._e.step(int(steps))
-        self._step += int(steps)
+        """EngineWrapper::run_n: steps, with a frame every nstxout steps."""
+        self._steps(int(steps), int(self._cfg.nstxout))
+
+    def save_checkpoint(self, path: str):
+        """SimulationEngine::save_checkpoint (simulation_engine.cpp:117-125):
+        sys_ and the LJ counter in DMD's binary format."""
+        from . import checkpoint as CK
+        pos, vel, frc = self._e.positions(), self._e.velocities(), self._e.forces()
+        sc = self._e.scalars()
+        arrays = {"pos_x": pos[:, 0], "pos_y": pos[:, 1], "pos_z": pos[:, 2],
+                  "vel_x": vel[:, 0], "vel_y": vel[:, 1], "vel_z": vel[:, 2],
+                  "forces_x": frc[:, 0], "forces_y": frc[:, 1], "forces_z": frc[:, 2],
+                  "masses": np.asarray(self._spec.masses, float),
+                  "charges": (np.zeros(self._spec.n) if self._spec.charges is None
+                              else np.asarray(self._spec.charges, float)),
+                  "atom_types": (np.zeros(self._spec.n, np.int32)
+                                 if self._spec.atom_types is None
+                                 else np.asarray(self._spec.atom_types, np.int32))}
+        counter = sc["counter"] if self._spec.use_lj else 0
+        CK.save(path, CK.Checkpoint(int(self._clock.step), float(self._clock.sys_time),
+                                    float(sc["pe"]), arrays, int(counter)))
+
+    def load_checkpoint(self, path: str):
+        """SimulationEngine::load_checkpoint (simulation_engine.cpp:127-137):
+        positions, velocities, step, time and the LJ counter."""
+        from . import checkpoint as CK
+        ck = CK.load(path)
+        if ck.n != self._spec.n:
+            raise ValueError(f"{path} holds {ck.n} atoms, the engine {self._spec.n}")
+        a = ck.arrays
+        self._e.set_positions(np.stack([a["pos_x"], a["pos_y"], a["pos_z"]], 1))
+        self._e.set_velocities(np.stack([a["vel_x"], a["vel_y"], a["vel_z"]], 1))
+        if self._spec.use_lj:
+            self._e.set_step_since_rebuild(ck.lj_step_since_rebuild)
+        self._clock.step = ck.step
+        self._clock.time = ck.time
+        self._clock.sys_time = ck.time
 
     @property
     def current_step(self) -> int:
-        return self._step
+        return int(self._clock.step)
 
     @property
     def current_time(self) -> float:
-        return self._step * float(self._cfg.dt)
+        """SimulationEngine::time_: dt added once per step."""
+        return float(self._clock.time)
 
     @property
     def positions(self):

@@ -57,6 +57,9 @@ def _setup(lib):
     lib.dmdx_engine_autotune.argtypes = [C.c_void_p, C.c_int32, C.c_char_p,
                                          C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                          C.POINTER(C.c_int32), C.POINTER(C.c_double)]
+    lib.dmdx_engine_snapshot_begin.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_int64)]
+    lib.dmdx_engine_snapshot_read.argtypes = [C.c_void_p, C.c_int64, _dp, _dp, _dp,
+                                              C.POINTER(C.c_int64)]
     lib._engine_ready = True
 
 
@@ -230,6 +233,27 @@ class Engine:
             _abi.check(self.lib.dmdx_engine_kernel_time(self.h, k, C.byref(ms), C.byref(n)))
             out[name] = (ms.value, n.value)
         return out
+
+    def snapshot_begin(self, sys: int = 0) -> int:
+        """Enqueue a frame of system `sys` after the work enqueued so far and
+        return its ticket (dmdx_engine_snapshot_begin); later steps overlap
+        its copy to host memory."""
+        t = C.c_int64()
+        _abi.check(self.lib.dmdx_engine_snapshot_begin(self.h, int(sys), C.byref(t)))
+        return t.value
+
+    def snapshot_read(self, ticket: int, sys: int = 0) -> dict:
+        """Wait for a frame (dmdx_engine_snapshot_read): positions and
+        velocities as (n, 3) arrays, box edges and steps_done."""
+        n = self.specs[sys].n
+        pos = np.empty((n, 3))
+        vel = np.empty((n, 3))
+        box = np.empty(3)
+        done = C.c_int64()
+        _abi.check(self.lib.dmdx_engine_snapshot_read(
+            self.h, int(ticket), pos.ctypes.data_as(_dp), vel.ctypes.data_as(_dp),
+            box.ctypes.data_as(_dp), C.byref(done)))
+        return {"pos": pos, "vel": vel, "box": box, "steps_done": done.value}
 
     def pairs(self, sys: int = 0):
         cnt = C.c_int64(0)

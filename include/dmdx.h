@@ -39,7 +39,9 @@ enum {
     DMDX_E_GPU = -5,            /* ForceError::GPUError: HIP runtime failure  */
     DMDX_E_OOM = -6,            /* device allocation failed / over budget     */
     DMDX_E_INVALID = -7,        /* invalid handle or argument                 */
-    DMDX_E_BUSY = -8            /* server: request cannot be admitted now     */
+    DMDX_E_BUSY = -8,           /* server: request cannot be admitted now;
+                                   snapshots: every staging slot is in use    */
+    DMDX_E_IO = -9              /* file I/O (H5MD trajectory) failed          */
 };
 
 /* ---- physics mode.
@@ -250,6 +252,27 @@ int dmdx_engine_adopt(dmdx_engine *dst, int32_t dst_sys, dmdx_engine *src, int32
  * most cap are written. */
 int dmdx_engine_get_pairs(dmdx_engine *e, int32_t sys, int32_t *pi, int32_t *pj, int64_t cap,
                           int64_t *count);
+
+/* Frames of a running engine without stopping it: the state behind
+ * SimulationEngine::save_frame / save_checkpoint (simulation_engine.cpp:
+ * 88-106, :117-143), taken where the host reads sys_ after a step.
+ * _begin enqueues, after the work enqueued so far, a copy of system sys's
+ * positions and velocities (velocities with any pending thermostat scale
+ * applied; the engine's own state is not changed), its box edges and its
+ * steps_done into a device staging slot, then an asynchronous copy of the
+ * slot to pinned host memory on a copy stream of its own; it returns at
+ * once with a ticket, and later steps overlap the copy.  _read waits for
+ * that ticket's copy and writes positions and velocities as interleaved
+ * (n, 3) rows (the layout of the H5MD position/velocity datasets,
+ * h5md_writer.cpp:200-225; either may be NULL), the box edges and
+ * steps_done.  At most DMDX_SNAPSHOT_SLOTS frames are in flight: a further
+ * _begin returns DMDX_E_BUSY until one is read.  The staging (device and
+ * pinned host, DMDX_SNAPSHOT_SLOTS x (64 + 48 n_max) bytes) is allocated on
+ * the first snapshot and freed with the engine. */
+#define DMDX_SNAPSHOT_SLOTS 4
+int dmdx_engine_snapshot_begin(dmdx_engine *e, int32_t sys, int64_t *ticket);
+int dmdx_engine_snapshot_read(dmdx_engine *e, int64_t ticket, double *pos_xyz, double *vel_xyz,
+                              double box[3], int64_t *steps_done);
 
 /* ---- components: ForceComponent (src/force/force_component.h:8-22) ----- */
 

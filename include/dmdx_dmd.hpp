@@ -1,6 +1,7 @@
 // dmdx_dmd.hpp -- C++ adapters that put the MI355X step engine behind DMD's
 // own interfaces.  Header-only; compile inside DMD's tree (its src/ on the
-// include path) and link libdmdx.so.
+// include path) and link libdmdx.so and libdmdx_h5md.so (trajectory frames
+// of DeviceSimulationEngine::run; HDF5, which DMD already links).
 //
 //   * dmdx_dmd::GpuLennardJones        : LennardJones (src/force/lennard_jones.h)
 //   * dmdx_dmd::GpuCoulombDirect, GpuCoulombExclusion, GpuCoulombPME,
@@ -32,6 +33,7 @@
 #include "sim/checkpoint.h"
 #include "sim/simulation_config.h"
 #include "dmdx.h"
+#include "dmdx_h5md.h"
 
 namespace dmdx_dmd {
 
@@ -283,8 +285,11 @@ public:
     // DMDX_FP64 or DMDX_MIXED, physics DMDX_FAITHFUL or DMDX_CORRECTED.
     DeviceSimulationEngine(const SimulationConfig& cfg, const SystemData& sys, int precision,
                            int physics)
-        : n_(sys.n_atoms), dt_(cfg.dt), n_steps_(cfg.n_steps)
+        : n_(sys.n_atoms), n_steps_(cfg.n_steps), traj_interval_(cfg.nstxout),
+          ckpt_interval_(cfg.checkpoint_interval), traj_path_(cfg.trajectory_path),
+          ckpt_path_(cfg.checkpoint_path), statics_(sys)
     {
+        clock_.dt = cfg.dt;
         dmdx_system_desc d;
         std::memset(&d, 0, sizeof(d));
         d.n_atoms = (int64_t)sys.n_atoms;
@@ -400,7 +405,11 @@ public:
         d.precision = precision;
         check(dmdx_engine_create(&d, 1, &e_), "SimulationEngine");
     }
-    ~DeviceSimulationEngine() { dmdx_engine_destroy(e_); }
+    ~DeviceSimulationEngine()
+    {
+        if (traj_) dmdx_h5md_close(traj_);
+        dmdx_engine_destroy(e_);
+    }
     DeviceSimulationEngine(const DeviceSimulationEngine&) = delete;
     DeviceSimulationEngine& operator=(const DeviceSimulationEngine&) = delete;
 
@@ -408,12 +417,39 @@ public:
     void steps(int64_t k)
     {
         check(dmdx_engine_step(e_, k), "SimulationEngine");
-        step_ += (int)k;
-        time_ += dt_ * (double)k;
+        for (int64_t i = 0; i < k; ++i) {  // SimulationEngine::step's clock (:54, :82-85)
+            clock_.sys_time = clock_.time;
+            clock_.step += 1;
+            clock_.time += clock_.dt;
+        }
     }
-    void run() { steps(n_steps_); }
-    int current_step() const { return step_; }
-    double current_time() const { return time_; }
+    // SimulationEngine::run (simulation_engine.cpp:88-106): n_steps steps, an
+    // H5MD frame every nstxout steps (taken asynchronously and written while
+    // the device runs on), a checkpoint every checkpoint_interval steps
+    void run()
+    {
+        if (traj_interval_ > 0 && !traj_)
+            h5(dmdx_h5md_open(traj_path_.c_str(), (int64_t)n_, 1, &traj_));
+        int64_t left = n_steps_;
+        while (left > 0) {
+            int64_t k = left;
+            if (ckpt_interval_ > 0)
+                k = std::min<int64_t>(k, ckpt_interval_ - clock_.step % ckpt_interval_);
+            h5(dmdx_h5md_run(e_, 0, k, traj_interval_ > 0 ? traj_interval_ : 0, &clock_, traj_));
+            left -= k;
+            if (ckpt_interval_ > 0 && clock_.step % ckpt_interval_ == 0)
+                save_checkpoint(ckpt_path_, statics_);
+        }
+    }
+    int current_step() const { return (int)clock_.step; }
+    double current_time() const { return clock_.time; }
+    // frames written so far by run()
+    int64_t trajectory_frames() const
+    {
+        int64_t f = 0;
+        if (traj_) dmdx_h5md_frame_count(traj_, &f);
+        return f;
+    }
 
     SystemData system() const
     {
@@ -428,8 +464,8 @@ public:
         dmdx_scalars sc;
         check(dmdx_engine_get_scalars(e_, 0, &sc), "SimulationEngine");
         s.potential_energy = sc.potential_energy;
-        s.step = (size_t)step_;
-        s.time = time_;
+        s.step = (size_t)clock_.step;
+        s.time = clock_.sys_time;  // set at the start of the last step (:55)
         return s;
     }
 
@@ -455,8 +491,9 @@ public:
         check(dmdx_engine_set_arrays(e_, 0, 1, s.vel_x.data(), s.vel_y.data(), s.vel_z.data()),
               "SimulationEngine");
         check(dmdx_engine_set_step_since_rebuild(e_, 0, lj_counter), "SimulationEngine");
-        step_ = (int)s.step;
-        time_ = s.time;
+        clock_.step = (int64_t)s.step;
+        clock_.time = s.time;
+        clock_.sys_time = s.time;
     }
 
 private:
@@ -475,12 +512,20 @@ private:
         a.insert(a.end(), x.begin(), x.end());
     }
 
+    static void h5(int rc)
+    {
+        if (rc != DMDX_OK) throw std::runtime_error(std::string("trajectory: ") +
+                                                    dmdx_h5md_last_error());
+    }
+
     dmdx_engine* e_ = nullptr;
     size_t n_;
-    double dt_;
     int n_steps_;
-    int step_ = 0;
-    double time_ = 0.0;
+    int traj_interval_, ckpt_interval_;
+    std::string traj_path_, ckpt_path_;
+    SystemData statics_;  // masses, charges and types for checkpoints
+    dmdx_clock clock_{0, 0.0, 0.0, 0.0};
+    dmdx_h5md* traj_ = nullptr;
     std::vector<int32_t> types_, bi_, bj_, ai_, aj_, ak_, di_, dj_, dk_, dl_, dp_, ii_, ij_, ik_,
         il_, ei_, ej_, ci_, cj_;
     std::vector<double> bk_, br_, akt_, at0_, dkp_, dp0_, ikp_, ip0_;

@@ -22,6 +22,7 @@ SRCS=(
   thermostat/thermostat.cpp thermostat/berendsen.cpp thermostat/nose_hoover.cpp
   thermostat/andersen.cpp
   barostat/barostat.cpp barostat/berendsen_barostat.cpp barostat/andersen_barostat.cpp
+  sim/checkpoint.cpp
 )
 ARGS=()
 for s in "${SRCS[@]}"; do ARGS+=("$REF/src/$s"); done
@@ -33,7 +34,9 @@ echo "build_ref: $OUT/libdmdref.so"
 # tests/adapter_check.cpp: include/dmdx_dmd.hpp inside DMD's own classes,
 # linked against DMD's translation units and the HIP library
 LIB="$HERE/../dmd_amd/libdmdx.so"
-if [ -f "$LIB" ]; then
+H5LIB="$HERE/../dmd_amd/libdmdx_h5md.so"
+HDF5_ROOT=${DMDX_HDF5_ROOT:-/opt/conda}
+if [ -f "$LIB" ] && [ -f "$H5LIB" ]; then
   ADP=(
     core/cell.cpp core/system_data.cpp core/error.cpp
     force/force_component.cpp force/force_engine.cpp force/lennard_jones.cpp
@@ -44,8 +47,12 @@ if [ -f "$LIB" ]; then
   AARGS=()
   for s in "${ADP[@]}"; do AARGS+=("$REF/src/$s"); done
   g++ -std=c++20 -O2 -ffp-contract=off -fno-fast-math -I"$REF/src" -I"$HERE/../include" \
+      -I"$HDF5_ROOT/include" \
       -o "$OUT/adapter_check.tmp" "$HERE/../tests/adapter_check.cpp" "${AARGS[@]}" \
-      -L"$HERE/../dmd_amd" -ldmdx -Wl,-rpath,'$ORIGIN/../../dmd_amd'
+      -L"$HERE/../dmd_amd" -ldmdx -ldmdx_h5md "$HDF5_ROOT/lib/libhdf5.so" \
+      -Wl,-rpath,"\$ORIGIN/../../dmd_amd:/usr/lib/x86_64-linux-gnu:$HDF5_ROOT/lib"
   mv "$OUT/adapter_check.tmp" "$OUT/adapter_check"
   echo "build_ref: $OUT/adapter_check"
+else
+  echo "build_ref: adapter_check skipped (libdmdx.so or libdmdx_h5md.so not built)" >&2
 fi

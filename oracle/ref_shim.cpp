@@ -29,6 +29,7 @@
 #include "barostat/barostat.h"
 #include "barostat/berendsen_barostat.h"
 #include "barostat/andersen_barostat.h"
+#include "sim/checkpoint.h"
 
 #include <cstdint>
 #include <memory>
@@ -581,5 +582,23 @@ void ref_sim_get(const ref_sim* r, double* x, double* y, double* z, double* vx, 
 }
 
 void ref_sim_destroy(ref_sim* r) { delete r; }
+
+// checkpoint::save of a SystemData filled from the arrays: v holds the 11
+// FP64 vectors in SystemData order (pos xyz, vel xyz, forces xyz, masses,
+// charges), n each
+void ref_checkpoint_save(const char* path, int64_t n, int64_t step, double time, double pe,
+                         const double* v, const int32_t* types, int32_t lj_counter)
+{
+    SystemData s(static_cast<size_t>(n));
+    std::vector<double>* dst[11] = {&s.pos_x,    &s.pos_y,    &s.pos_z,    &s.vel_x,
+                                    &s.vel_y,    &s.vel_z,    &s.forces_x, &s.forces_y,
+                                    &s.forces_z, &s.masses,   &s.charges};
+    for (int k = 0; k < 11; ++k) dst[k]->assign(v + k * n, v + (k + 1) * n);
+    s.atom_types.assign(types, types + n);
+    s.step = static_cast<size_t>(step);
+    s.time = time;
+    s.potential_energy = pe;
+    checkpoint::save(path, s, lj_counter);
+}
 
 }  // extern "C"

@@ -18,7 +18,8 @@
 //                                  (identity minimum image);
 //                              C: DeviceSimulationEngine (FP64, faithful),
 //                                 plus a checkpoint save/load round trip in
-//                                 DMD's checkpoint format.
+//                                 DMD's checkpoint format, and run() with
+//                                 H5MD frames and checkpoints on cadence.
 //                            Prints one JSON line; exit status 0 when B and C
 //                            match A to 1e-10 relative and the restart is
 //                            bit-exact.
@@ -44,6 +45,8 @@
 #include "sim/simulation_config.h"
 
 #include "dmdx_dmd.hpp"
+
+#include <hdf5.h>
 
 namespace {
 
@@ -198,6 +201,26 @@ double rel_diff(const std::vector<double>& a, const std::vector<double>& b)
     return den > 0.0 ? num / den : num;
 }
 
+// a whole dataset of a file written by the device engine's run()
+template <class T>
+std::vector<T> read_h5(const std::string& path, const char* name, hid_t type)
+{
+    std::vector<T> out;
+    hid_t f = H5Fopen(path.c_str(), H5F_ACC_RDONLY, H5P_DEFAULT);
+    if (f < 0) return out;
+    hid_t ds = H5Dopen2(f, name, H5P_DEFAULT);
+    if (ds >= 0) {
+        hid_t sp = H5Dget_space(ds);
+        out.resize((size_t)H5Sget_simple_extent_npoints(sp));
+        if (!out.empty() && H5Dread(ds, type, H5S_ALL, H5S_ALL, H5P_DEFAULT, out.data()) < 0)
+            out.clear();
+        H5Sclose(sp);
+        H5Dclose(ds);
+    }
+    H5Fclose(f);
+    return out;
+}
+
 double rel3(const SystemData& a, const SystemData& b, int what)
 {
     if (what == 0)
@@ -281,10 +304,12 @@ int main(int argc, char** argv)
     auto fe_gpu = dmdx_dmd::build_force_engine_gpu(cfg);
     SystemData b = initial_state(cfg);
     double worst_b = 0.0;
+    std::vector<SystemData> a_at;  // DMD's state after steps 10, 20, 30
     for (int k = 0; k < n_steps; ++k) {
         dmd_step(*fe_cpu, in, a, cell, cfg.dt);
         dmd_step(*fe_gpu, in, b, cell, cfg.dt);
         worst_b = std::fmax(worst_b, rel3(a, b, 2));
+        if ((k + 1) % 10 == 0) a_at.push_back(a);
     }
     double pos_b = rel3(a, b, 0), vel_b = rel3(a, b, 1);
     double pe_b = std::fabs(a.potential_energy - b.potential_energy) /
@@ -338,6 +363,66 @@ int main(int argc, char** argv)
     std::remove(path.c_str());
     expect(chk.step == (size_t)n_steps, "checkpoint::load reads the device engine's file");
 
+    // SimulationEngine::run on the device: an H5MD frame every 10 steps, a
+    // checkpoint every 15 (the file keeps the last, at step 30)
+    double traj_rel = 1.0;
+    {
+        SimulationConfig rc = cfg;
+        rc.n_steps = n_steps;
+        rc.nstxout = 10;
+        rc.checkpoint_interval = 15;
+        rc.trajectory_path = std::string(tmp ? tmp : "/tmp") + "/dmdx_adapter_check.h5";
+        rc.checkpoint_path = std::string(tmp ? tmp : "/tmp") + "/dmdx_adapter_check_run.ckpt";
+        SystemData last(statics.n_atoms);
+        {
+            dmdx_dmd::DeviceSimulationEngine dr(rc, initial_state(cfg), DMDX_FP64,
+                                                DMDX_FAITHFUL);
+            dr.run();
+            expect(dr.current_step() == n_steps && dr.trajectory_frames() == 3,
+                   "run() takes n_steps steps and writes a frame every nstxout");
+            last = dr.system();
+        }  // closes the trajectory
+        const size_t n = statics.n_atoms;
+        auto pos = read_h5<double>(rc.trajectory_path, "/particles/atoms/position/value",
+                                   H5T_NATIVE_DOUBLE);
+        auto steps = read_h5<int>(rc.trajectory_path, "/particles/atoms/position/step",
+                                  H5T_NATIVE_INT);
+        auto times = read_h5<double>(rc.trajectory_path, "/particles/atoms/position/time",
+                                     H5T_NATIVE_DOUBLE);
+        std::remove(rc.trajectory_path.c_str());
+        expect(pos.size() == 3 * n * 3 && steps == std::vector<int>({10, 20, 30}),
+               "the trajectory holds frames 10, 20, 30");
+        // SystemData::time of a frame: the engine's running time at the start
+        // of the frame's last step (simulation_engine.cpp:55, :85)
+        double t = 0.0, t_start = 0.0;
+        std::vector<double> want_t;
+        for (int k = 1; k <= n_steps; ++k) {
+            t_start = t;
+            t += cfg.dt;
+            if (k % 10 == 0) want_t.push_back(t_start);
+        }
+        expect(times == want_t, "frame times are SystemData::time");
+        if (pos.size() == 3 * n * 3) {
+            traj_rel = 0.0;
+            for (int f = 0; f < 3; ++f) {
+                SystemData fr(n);
+                for (size_t i = 0; i < n; ++i) {
+                    fr.pos_x[i] = pos[(f * n + i) * 3 + 0];
+                    fr.pos_y[i] = pos[(f * n + i) * 3 + 1];
+                    fr.pos_z[i] = pos[(f * n + i) * 3 + 2];
+                }
+                traj_rel = std::fmax(traj_rel, rel3(a_at[f], fr, 0));
+            }
+        }
+        SystemData ck(n);
+        int ck_counter = -1;
+        checkpoint::load(rc.checkpoint_path, ck, ck_counter);
+        std::remove(rc.checkpoint_path.c_str());
+        expect(ck.step == (size_t)n_steps && ck.time == want_t.back() &&
+                   ck.pos_x == last.pos_x && ck.vel_z == last.vel_z,
+               "run() checkpoints the state at checkpoint_interval");
+    }
+
     const double tol = 1e-10;
     expect(worst_b <= tol && pos_b <= tol && vel_b <= tol && pe_b <= tol,
            "GPU components inside DMD's loop match DMD's components");
@@ -346,6 +431,7 @@ int main(int argc, char** argv)
     expect(pos_c <= tol && vel_c <= tol && frc_c <= tol,
            "DeviceSimulationEngine matches DMD's loop");
     expect(restart == 0.0, "restart from the checkpoint is bit-exact");
+    expect(traj_rel <= tol, "run()'s frames match DMD's loop at the frame steps");
     std::printf(
         "{\"adapter_check\": \"full\", \"n_atoms\": %zu, \"steps\": %d, "
         "\"components_force_rel\": %.3e, \"components_pos_rel\": %.3e, "
@@ -353,8 +439,9 @@ int main(int argc, char** argv)
         "\"device_pos_rel\": %.3e, \"device_vel_rel\": %.3e, \"device_force_rel\": %.3e, "
         "\"triclinic_force_rel\": %.3e, \"triclinic_pos_rel\": %.3e, "
         "\"triclinic_pe_rel\": %.3e, "
-        "\"restart_rel\": %.3e, \"lj_counter\": %d, \"ok\": %s}\n",
+        "\"restart_rel\": %.3e, \"trajectory_pos_rel\": %.3e, \"lj_counter\": %d, "
+        "\"ok\": %s}\n",
         a.n_atoms, n_steps, worst_b, pos_b, vel_b, pe_b, pos_c, vel_c, frc_c, worst_t, pos_t,
-        pe_t, restart, lj_counter, g_fail ? "false" : "true");
+        pe_t, restart, traj_rel, lj_counter, g_fail ? "false" : "true");
     return g_fail;
 }

@@ -9,6 +9,9 @@ src/core) where they lie, with the standard library only.  The vectors are
 what those units return for seeded inputs; the inputs are stored with them,
 so the test needs neither the reference nor the generators that made them.
 
+Also writes tests/golden/dmd_checkpoint_n5.bin with checkpoint::save
+(src/sim/checkpoint.cpp:9-38); its inputs are the ckpt_* arrays.
+
 Cases (reference files the units come from):
   lj_*          LennardJones::compute over its rebuilt list
                 (src/force/lennard_jones.cpp:18-92)
@@ -45,6 +48,7 @@ from dmd_amd import _abi  # noqa: E402
 from dmd_amd import workloads as W  # noqa: E402
 
 OUT = os.path.join(HERE, "dmd_ref_vectors.npz")
+CKPT = os.path.join(HERE, "dmd_checkpoint_n5.bin")
 
 
 def ref_call(fn, pos, box, *args):
@@ -256,6 +260,20 @@ def main():
                      [2.0, 2.0, 0.0, 2.0, 0.0, 2.0, 0.0, 2.0, 2.0]])
     v.update(cellvol_m=mats, cellvol_V=np.array(
         [[O.ref_cell_volume(m, t) for t in range(4)] for m in mats]))
+
+    # DMD's binary checkpoint (src/sim/checkpoint.cpp:9-38), written by
+    # checkpoint::save itself: the file next to the vectors
+    rng = np.random.default_rng(77)
+    n_ck = 5
+    ck_v = rng.standard_normal(11 * n_ck)
+    ck_v[3] = -0.0
+    ck_types = np.array([0, 1, 2, 1, 0], np.int32)
+    ck_scal = np.array([1234.0, 2.466, -12.5, 7.0])  # step, time, PE, LJ counter
+    R.ref_checkpoint_save(os.fsencode(CKPT), C.c_int64(n_ck), C.c_int64(int(ck_scal[0])),
+                          C.c_double(ck_scal[1]), C.c_double(ck_scal[2]), O.P(ck_v),
+                          O.P(ck_types), C.c_int32(int(ck_scal[3])))
+    v.update(ckpt_v=ck_v, ckpt_types=ck_types, ckpt_scalars=ck_scal)
+    print(f"wrote {CKPT}: {os.path.getsize(CKPT)} bytes")
 
     arrays = {k: np.asarray(val) for k, val in v.items()}
     np.savez_compressed(OUT, **arrays)

@@ -9,11 +9,16 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB = os.path.join(ROOT, "dmd_amd", "libdmdx.so")
+H5MD_LIB = os.path.join(ROOT, "dmd_amd", "libdmdx_h5md.so")
+H5MD_HEADER = os.path.join(ROOT, "include", "dmdx_h5md.h")
 
 
-def declared_functions():
+def declared_functions(headers=None):
+    """dmdx_* functions of include/*.h; dmdx_h5md.h belongs to libdmdx_h5md.so."""
+    if headers is None:
+        headers = [h for h in glob.glob(os.path.join(ROOT, "include", "*.h")) if h != H5MD_HEADER]
     names = set()
-    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+    for h in headers:
         text = open(h).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         for m in re.finditer(r"^[A-Za-z_][\w\s\*]*?\b(dmdx_\w+)\s*\(", text, flags=re.M):
@@ -31,6 +36,16 @@ def test_header_declares_entry_points():
 def test_library_exports_every_declared_symbol():
     lib = C.CDLL(LIB)
     missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+@pytest.mark.skipif(not os.path.exists(H5MD_LIB), reason="libdmdx_h5md.so not built (no HDF5)")
+def test_h5md_library_exports_every_declared_symbol():
+    C.CDLL(LIB)
+    lib = C.CDLL(H5MD_LIB)
+    names = declared_functions([H5MD_HEADER])
+    assert "dmdx_h5md_run" in names
+    missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
 
 

@@ -39,8 +39,10 @@ def test_adapter_interface_cpu():
 @pytest.mark.gpu
 def test_adapter_in_dmd_loop_gpu():
     """30 velocity-Verlet steps of a 200-atom chain system: GPU components in
-    DMD's loop and DeviceSimulationEngine both match DMD's CPU components to
-    1e-10 relative; a restart from DMD's checkpoint format is bit-exact."""
+    DMD's loop (periodic and under a triclinic Cell) and DeviceSimulationEngine
+    both match DMD's CPU components to 1e-10 relative; a restart from DMD's
+    checkpoint format is bit-exact; run() writes H5MD frames that match DMD's
+    loop at the frame steps and checkpoints on cadence."""
     out = _run(timeout=110)
     print(out)
     assert out["ok"]
