@@ -130,3 +130,54 @@ def run(engine, nsteps: int, interval: int, clock: Clock, writer: H5MDWriter | N
     _check(load().dmdx_h5md_run(engine.h, int(sys), int(nsteps), int(interval), C.byref(clock),
                                 writer.h if writer is not None else None))
     return clock
+
+
+def _bench(argv=None):
+    """python -m dmd_amd.trajectory [--steps N] [--side S]: ms per step of a
+    MIXED water box (3 S^3 atoms) without frames, with asynchronous frames
+    every 100, 10 and 1 steps, and with frames read by get_arrays and written
+    synchronously every 10 steps."""
+    import argparse
+    import tempfile
+    import time
+
+    from . import workloads as W
+    from .engine import Engine
+
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--side", type=int, default=16)
+    a = ap.parse_args(argv)
+    spec = W.water_box(n_side=a.side, precision=_abi.MIXED)
+    n_steps = a.steps
+    tmp = tempfile.mkdtemp()
+    e = Engine(spec)
+    e.step(50)
+    e.sync()
+
+    def timed(fn):
+        t0 = time.perf_counter()
+        fn()
+        return (time.perf_counter() - t0) * 1e3 / n_steps
+
+    base = timed(lambda: (e.step(n_steps), e.sync()))
+    print(f"water {spec.n} atoms, MIXED: no frames {base:.4f} ms/step", flush=True)
+    for every in (100, 10, 1):
+        clock = Clock.start(spec.dt)
+        with H5MDWriter(os.path.join(tmp, f"a{every}.h5"), spec.n, True) as w:
+            ms = timed(lambda: run(e, n_steps, every, clock, w))
+            frames = w.frame_count()
+        print(f"asynchronous frames every {every}: {ms:.4f} ms/step, {frames} frames, "
+              f"+{(ms - base) * n_steps / max(frames, 1):.3f} ms/frame", flush=True)
+    with H5MDWriter(os.path.join(tmp, "sync.h5"), spec.n, True) as w:
+        def loop():
+            for k in range(n_steps // 10):
+                e.step(10)
+                w.write_rows(e.positions(), e.velocities(), np.diag(e.scalars()["box"]),
+                             10 * k, 0.0)
+        ms = timed(loop)
+    print(f"get_arrays + synchronous frames every 10: {ms:.4f} ms/step", flush=True)
+
+
+if __name__ == "__main__":
+    _bench()
