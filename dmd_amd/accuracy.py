@@ -1,6 +1,7 @@
 """Accuracy of the mixed-precision path against the FP64 path.
 
-    python -m dmd_amd.accuracy [--which c0,c1,c2] [--steps N] [--ensemble R] [--out FILE]
+    python -m dmd_amd.accuracy [--which c0,c1,c2] [--steps N] [--ensemble R]
+                               [--power S] [--variant V] [--out FILE]
 
 For each workload, from a configuration relaxed by 200 FP64 steps: force
 RMS relative error and potential-energy error of the mixed path against
@@ -16,6 +17,20 @@ spread of chaotic trajectories.  The
 FP64 device path is the reference here because it reproduces the FP64 CPU
 oracle (DMD's own arithmetic) to 1e-10 in forces; running the CPU oracle for
 10^4 steps of 10^5 atoms is not practical.
+
+--power S measures the rate at which the mixed path's force error does
+work: P = sum_i (F_mixed(x) - F_fp64(x)) . v_i per atom, at S states spaced
+--every steps apart along one FP64 NVE trajectory, each mixed force from a
+fresh engine on the state (lists and cell frames built there); with
+--along-mixed, along the mixed trajectory with the forces its own steps
+computed (lists and cell-local frames as old as its last rebuild), each FP64
+force from a fresh engine.  Its mean is the first-order energy injection of
+the force error, with a standard error far below that of drift ensembles,
+because both forces see the same positions and velocities.  --variant changes the electrostatics of every
+workload first, to localize an effect: "no-recip" keeps the real-space erfc
+term and drops PME (COULOMB_CUTOFF), "no-elec" drops all electrostatics,
+"rebuild-every-step" rebuilds lists (and the mixed path's cell-local frames)
+every step.
 """
 from __future__ import annotations
 
@@ -87,6 +102,48 @@ def nve_drift(spec, steps: int, every: int = 10, precision=_abi.MIXED):
             "flags": flags, "wall_s": wall}
 
 
+def force_error_power(spec, samples: int, every: int, along_mixed: bool = False):
+    """Mean and standard error of P = sum (F_mixed - F_fp64) . v / n, in
+    kJ/mol/ns per atom (the unit of the drifts), at `samples` states `every`
+    steps apart of one NVE trajectory.
+    along_mixed=False: an FP64 trajectory; each mixed force from a fresh
+    mixed engine on the state (list and cell-local frames built there).
+    along_mixed=True: the mixed trajectory itself, with the forces its own
+    step computed (lists and frames as old as its last rebuild); each FP64
+    force from a fresh FP64 engine on that state."""
+    from .engine import Engine
+    s = spec.with_(precision=_abi.FP64, thermostat=_abi.THERMO_NONE, barostat=_abi.BARO_NONE)
+    lead = _abi.MIXED if along_mixed else _abi.FP64
+    other = _abi.FP64 if along_mixed else _abi.MIXED
+    ref = Engine(s.with_(precision=lead))
+    p = []
+    for _ in range(samples):
+        ref.step(every)
+        x, v, f_lead = ref.positions(), ref.velocities(), ref.forces()
+        m = Engine(s.with_(pos=x, vel=v, precision=other))
+        m.compute()
+        f_other = m.forces()
+        m.close()
+        df = (f_lead - f_other) if along_mixed else (f_other - f_lead)
+        p.append(float((df * v).sum()) / s.n * 1000.0)  # kJ/mol/ps -> /ns
+    ref.close()
+    p = np.array(p)
+    return {"samples": samples, "every": every, "mean": float(p.mean()),
+            "stderr": float(p.std(ddof=1) / np.sqrt(len(p))), "rms": float(np.sqrt((p * p).mean()))}
+
+
+def variant(spec, name: str):
+    if name in ("", "as-is"):
+        return spec
+    if name == "no-recip":
+        return spec.with_(coulomb_type=_abi.COULOMB_CUTOFF)
+    if name == "no-elec":
+        return spec.with_(coulomb_type=_abi.COULOMB_NONE)
+    if name == "rebuild-every-step":
+        return spec.with_(rebuild_interval=1)
+    raise ValueError(name)
+
+
 def workload(name: str, seed: int = 42):
     if name == "c0":
         return W.argon_fcc(skin=0.1, seed=seed).with_(physics=_abi.CORRECTED,
@@ -107,17 +164,32 @@ def main(argv=None):
     ap.add_argument("--every", type=int, default=20)
     ap.add_argument("--ensemble", type=int, default=0,
                     help="perturbed starts per precision for drift statistics")
+    ap.add_argument("--power", type=int, default=0,
+                    help="states for the force-error power along an FP64 trajectory")
+    ap.add_argument("--variant", default="as-is",
+                    choices=["as-is", "no-recip", "no-elec", "rebuild-every-step"])
+    ap.add_argument("--along-mixed", action="store_true",
+                    help="--power along the mixed trajectory (its own forces)")
+    ap.add_argument("--no-drift", action="store_true", help="skip the single NVE runs")
     ap.add_argument("--out", default="")
     args = ap.parse_args(argv)
     own = {"c0": 1000, "c1": 2000, "c2": 10000}
     rows = {}
     for name in args.which.split(","):
-        spec = workload(name)
+        spec = variant(workload(name), args.variant)
         steps = args.steps or own[name]
-        print(f"[accuracy] {name}: {spec.n} atoms, {steps} NVE steps", file=sys.stderr,
-              flush=True)
+        print(f"[accuracy] {name} ({args.variant}): {spec.n} atoms, {steps} NVE steps",
+              file=sys.stderr, flush=True)
         spec = relaxed(spec)
-        r = {"atoms": spec.n, **force_error(spec)}
+        r = {"atoms": spec.n, "variant": args.variant, **force_error(spec)}
+        if args.power:
+            r["force_error_power"] = force_error_power(spec, args.power, args.every,
+                                                       args.along_mixed)
+            r["force_error_power"]["along"] = "mixed" if args.along_mixed else "fp64"
+            print(json.dumps({name: r["force_error_power"]}), file=sys.stderr, flush=True)
+        if args.no_drift:
+            rows[name] = r
+            continue
         r["nve_fp64"] = nve_drift(spec, steps, args.every, _abi.FP64)
         r["nve_mixed"] = nve_drift(spec, steps, args.every, _abi.MIXED)
         # noise floor: the FP64 run again from positions moved by 1e-9 nm

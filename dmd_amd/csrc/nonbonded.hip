@@ -15,10 +15,17 @@
 //   reference's lexicographic pair loop reaches atom i, so an LJ-only system
 //   gets bit-identical forces.
 //
-// k_pair_mixed: the mixed path.  The minimum-image separation is formed in
-//   FP64 (FP32 positions of a 5-10 nm box would carry ~5e-7 nm of error into
-//   every separation), then everything else -- 1/r^2, the LJ powers, erfc,
-//   exp -- runs in FP32 and each pair force is added to an FP64 accumulator.
+// k_pair_mixed: the mixed path.  Positions are FP32 offsets from the origin
+//   of the atom's list-build cell (lpos = x - org, formed in FP64 and
+//   rounded once), so they stay below a cell width and carry ~1e-8 nm of
+//   error instead of the ~5e-7 nm of absolute FP32 coordinates in a 5-10 nm
+//   box.  A list entry holds the neighbour's slot and the offset of its cell
+//   from the atom's cell; the separation is l_j - l_i + offset * width, with
+//   offset * width read from a per-wave table of the 125 offsets (the minimum
+//   image when the system has >= 2 cdiv + 1 cells per edge; otherwise the
+//   separation is wrapped by the box in FP32).  Everything else -- 1/r^2, the
+//   LJ powers, erfc, exp -- runs in FP32 and each pair force is added to an
+//   FP64 accumulator.
 #include "dmdx_internal.h"
 
 #include <type_traits>

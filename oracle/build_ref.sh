@@ -53,6 +53,17 @@ if [ -f "$LIB" ] && [ -f "$H5LIB" ]; then
       -Wl,-rpath,"\$ORIGIN/../../dmd_amd:/usr/lib/x86_64-linux-gnu:$HDF5_ROOT/lib"
   mv "$OUT/adapter_check.tmp" "$OUT/adapter_check"
   echo "build_ref: $OUT/adapter_check"
+  # the same program with its host code (DMD's units, the adapters) under
+  # AddressSanitizer + UndefinedBehaviorSanitizer, for `--no-gpu`; g++ only,
+  # no device code is built here
+  g++ -std=c++20 -O1 -g -fno-omit-frame-pointer -ffp-contract=off -fno-fast-math \
+      -fsanitize=address,undefined -fno-sanitize-recover=all \
+      -I"$REF/src" -I"$HERE/../include" -I"$HDF5_ROOT/include" \
+      -o "$OUT/adapter_check_san.tmp" "$HERE/../tests/adapter_check.cpp" "${AARGS[@]}" \
+      -L"$HERE/../dmd_amd" -ldmdx -ldmdx_h5md "$HDF5_ROOT/lib/libhdf5.so" \
+      -Wl,-rpath,"\$ORIGIN/../../dmd_amd:/usr/lib/x86_64-linux-gnu:$HDF5_ROOT/lib"
+  mv "$OUT/adapter_check_san.tmp" "$OUT/adapter_check_san"
+  echo "build_ref: $OUT/adapter_check_san"
 else
   echo "build_ref: adapter_check skipped (libdmdx.so or libdmdx_h5md.so not built)" >&2
 fi
