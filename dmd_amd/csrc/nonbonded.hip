@@ -26,6 +26,17 @@
 //   separation is wrapped by the box in FP32).  Everything else -- 1/r^2, the
 //   LJ powers, erfc, exp -- runs in FP32 and each pair force is added to an
 //   FP64 accumulator.
+//
+//   Charged systems also add the part of offset * width that FP32 drops:
+//   the table holds offset * fl32(w) exactly (|offset| <= 2), which misses
+//   offset * (w - fl32(w)), ~3e-8 nm per cell crossed.  That error is the
+//   same for every pair of a cell offset and stays fixed between list
+//   builds, so it is not noise: measured as the work the force error does
+//   along a mixed NVE run of 12,288-atom water, it injected 1e-3
+//   kJ/mol/ns/atom, and 3e-4 with the correction (fma(shift, rho, d),
+//   rho = (w - fl32(w)) / fl32(w)); the wrap by the box gets the same
+//   treatment.  LJ-only systems, where the same term was 1e-4, skip the
+//   three FMAs.
 #include "dmdx_internal.h"
 
 #include <type_traits>
@@ -200,6 +211,12 @@ __device__ __forceinline__ void k_pair_mixed_unit(int64_t unit_,
     const int64_t a0 = S.atom_off;
     const float Lx = (float)S.L[0], Ly = (float)S.L[1], Lz = (float)S.L[2];
     const float ix = (float)S.iminv[0], iy = (float)S.iminv[1], iz = (float)S.iminv[2];
+    // the FP32 shift table's and box's missing low parts (charged systems)
+    const float rhox = (float)((S.L[0] / T.nub[0] - (double)wx) / (double)wx);
+    const float rhoy = (float)((S.L[1] / T.nub[1] - (double)wy) / (double)wy);
+    const float rhoz = (float)((S.L[2] / T.nub[2] - (double)wz) / (double)wz);
+    const float Lxlo = (float)(S.L[0] - (double)Lx), Lylo = (float)(S.L[1] - (double)Ly),
+                Lzlo = (float)(S.L[2] - (double)Lz);
     const int nt = S.ntypes;
     const bool lj = S.use_lj, cp = COUL;
     const bool typed = lj && nt > 1;
@@ -313,10 +330,21 @@ __device__ __forceinline__ void k_pair_mixed_unit(int64_t unit_,
                 float dx = (pj[q].x - pi.x) + sh.x;
                 float dy = (pj[q].y - pi.y) + sh.y;
                 float dz = (pj[q].z - pi.z) + sh.z;
+                if constexpr (COUL) {
+                    dx = fmaf(sh.x, rhox, dx);
+                    dy = fmaf(sh.y, rhoy, dy);
+                    dz = fmaf(sh.z, rhoz, dz);
+                }
                 if constexpr (WRAP) {
-                    dx -= rintf(dx * ix) * Lx;
-                    dy -= rintf(dy * iy) * Ly;
-                    dz -= rintf(dz * iz) * Lz;
+                    const float kx = rintf(dx * ix), ky = rintf(dy * iy), kz = rintf(dz * iz);
+                    dx -= kx * Lx;
+                    dy -= ky * Ly;
+                    dz -= kz * Lz;
+                    if constexpr (COUL) {
+                        dx = fmaf(-kx, Lxlo, dx);
+                        dy = fmaf(-ky, Lylo, dy);
+                        dz = fmaf(-kz, Lzlo, dz);
+                    }
                 }
                 const float r2 = dx * dx + dy * dy + dz * dz;
                 const bool in_lj = lj && valid && r2 < cut2;
