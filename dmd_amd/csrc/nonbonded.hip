@@ -224,6 +224,7 @@ __device__ __forceinline__ void k_pair_mixed_unit(int64_t unit_,
     const float* c6t = lj_c6 + S.lj_off;
     const int ti = typed && real ? typ_s[t] : 0;
     const float c12_0 = c12t[0], c6_0 = c6t[0];
+    const float c12x12 = 12.0f * c12_0, c6x6 = 6.0f * c6_0;
     const float qi = cp ? (float)(KE * (double)pi.w) : 0.0f;
     const float alpha = (float)S.alpha;
     const float two_a_rpi = (float)(2.0 * S.alpha / sqrt(PI));
@@ -315,11 +316,13 @@ __device__ __forceinline__ void k_pair_mixed_unit(int64_t unit_,
             // the next pass's entries load while this pass gathers
             uint32_t en[W];
             if (STRIDED || kp + STRIDE < cnt) load_entries(k0 + STRIDE, en);
-            // the W entries of a pass are summed in FP32, then added in FP64
+            // the W entries of a pass are summed in FP32 (starting from the
+            // first entry's terms), then added in FP64
             float px = 0.0f, py = 0.0f, pz = 0.0f, pel = 0.0f, pec = 0.0f;
             // Branch-free entries: an entry past the row end (its gather read
-            // the atom itself) or outside a cutoff computes on r2 = 1 and
-            // contributes exactly zero through a select.
+            // the atom itself, r2 = 0) or outside a cutoff computes on its own
+            // r2 -- infinities and NaNs included -- and contributes exactly
+            // zero through the selects on its force and energy terms.
 #pragma unroll
             for (int q = 0; q < W; ++q) {
                 const bool valid = !CHECK_END || entry_index(k0, q) < cnt;
@@ -348,22 +351,26 @@ __device__ __forceinline__ void k_pair_mixed_unit(int64_t unit_,
                 }
                 const float r2 = dx * dx + dy * dy + dz * dz;
                 const bool in_lj = lj && valid && r2 < cut2;
-                const float r2l = in_lj ? r2 : 1.0f;
-                const float rinv2 = __builtin_amdgcn_rcpf(r2l);  // 1 ulp hardware reciprocal
-                const float c12 = TYPED ? c12t[ti * nt + tj[q]] : c12_0;
-                const float c6 = TYPED ? c6t[ti * nt + tj[q]] : c6_0;
+                const float rinv2 = __builtin_amdgcn_rcpf(r2);  // 1 ulp hardware reciprocal
                 const float r6i = rinv2 * rinv2 * rinv2;
-                const float a12 = c12 * r6i * r6i, a6 = c6 * r6i;
-                float fs = in_lj ? (12.0f * a12 - 6.0f * a6) * rinv2 : 0.0f;
-                const float el = in_lj ? a12 - a6 : 0.0f;
+                // f/r = r^-8 (12 c12 r^-6 - 6 c6), e = r^-6 (c12 r^-6 - c6),
+                // with c12 = 4 eps sig^12, c6 = 4 eps sig^6
+                float fs, el;
+                if constexpr (TYPED) {
+                    const float c12 = c12t[ti * nt + tj[q]], c6 = c6t[ti * nt + tj[q]];
+                    fs = in_lj ? fmaf(12.0f * c12, r6i, -6.0f * c6) * r6i * rinv2 : 0.0f;
+                    el = in_lj ? fmaf(c12, r6i, -c6) * r6i : 0.0f;
+                } else {
+                    fs = in_lj ? fmaf(c12x12, r6i, -c6x6) * r6i * rinv2 : 0.0f;
+                    el = in_lj ? fmaf(c12_0, r6i, -c6_0) * r6i : 0.0f;
+                }
                 float ecl = 0.0f;
                 if constexpr (COUL) {
                     const bool in_c = cp && valid && r2 < ccut2;
-                    const float r2c = in_c ? r2 : 1.0f;
-                    const float rinv = rsqrtf(r2c);
-                    const float r = r2c * rinv;
+                    const float rinv = rsqrtf(r2);
+                    const float r = r2 * rinv;
                     const float qq = qi * pj[q].w;
-                    const float ex = __expf(-a2 * r2c);
+                    const float ex = __expf(-a2 * r2);
                     // erfc(a r) by Abramowitz & Stegun 7.1.26 (|error| <= 1.5e-7),
                     // sharing exp(-(a r)^2) with the force term
                     const float tt = __builtin_amdgcn_rcpf(1.0f + 0.3275911f * (alpha * r));
@@ -373,11 +380,19 @@ __device__ __forceinline__ void k_pair_mixed_unit(int64_t unit_,
                     fs += in_c ? qq * (erfc_ar * rinv + two_a_rpi * ex) * (rinv * rinv) : 0.0f;
                     ecl = in_c ? qq * erfc_ar * rinv : 0.0f;
                 }
-                px += fs * dx;
-                py += fs * dy;
-                pz += fs * dz;
-                pel += el;  // rows hold both orders of a pair: halved below
-                pec += ecl;
+                if (q == 0) {
+                    px = fs * dx;
+                    py = fs * dy;
+                    pz = fs * dz;
+                    pel = el;  // rows hold both orders of a pair: halved below
+                    pec = ecl;
+                } else {
+                    px += fs * dx;
+                    py += fs * dy;
+                    pz += fs * dz;
+                    pel += el;
+                    pec += ecl;
+                }
             }
             fxi -= (double)px;
             fyi -= (double)py;
