@@ -15,12 +15,31 @@
 // touched.  Results are cached in a text file keyed by a signature of the
 // systems, so a workload is timed once.  There is no counterpart in the
 // reference (its loops are serial).
+//
+// Before the shapes, on an engine that has not stepped yet: the list skin of
+// systems that rebuild by the displacement criterion (rebuild_interval -1).
+// A wider skin means longer rows (the pair kernel's work grows as
+// (rc + skin)^3) and fewer rebuilds (an atom may move skin / 2 between
+// them); hot, dense systems such as the melts of a replica mix rebuild every
+// few steps at their default skin.  Systems of the same cutoffs and skin form
+// a group; each group's skin is scaled by 1, 1.3, 1.6 or 2, one group at a
+// time (largest first), timing a scratch batch over 20 steps from the start
+// and 20 steps after 60 more; a change must win by 2%.  The chosen skins
+// replace the engine's (the batch is made again from its descriptions,
+// which it has not left).  The forces do not depend on the skin: only the
+// FP32 summation order of the mixed path changes.
+#include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <map>
+#include <new>
 #include <sstream>
 #include <string>
+#include <tuple>
 
 #include "engine.h"
 
@@ -133,28 +152,208 @@ void cache_store(const char* path, uint64_t key, int pair, int width, int build,
     f << o.str() << "\n";
 }
 
-}  // namespace
+constexpr int NSKIN = 4;
+constexpr double SKIN_MULT[NSKIN] = {1.0, 1.3, 1.6, 2.0};
+constexpr int SKIN_WINDOW = 20;  // steps per timed window
+constexpr int SKIN_GAP = 60;     // steps between the two windows
 
-extern "C" int dmdx_engine_autotune(dmdx_engine* e, int32_t steps, const char* cache_path,
-                                    int32_t* pair_lanes, int32_t* pair_width,
-                                    int32_t* build_lanes, double* times_ms)
+struct SkinGroup {
+    std::vector<int> sys;
+    int64_t atoms = 0;
+};
+
+// Verlet-criterion systems with a skin and a pair list, grouped by cutoffs
+// and skin, largest group first
+std::vector<SkinGroup> skin_groups(const dmdx_engine* e)
 {
-    if (!e) return dmdx::fail(DMDX_E_INVALID, "null engine");
+    std::map<std::tuple<int64_t, int64_t, int64_t>, int> index;
+    std::vector<SkinGroup> g;
+    for (size_t s = 0; s < e->descs.size(); ++s) {
+        const dmdx_system_desc& D = e->descs[s]->d;
+        const bool pairs = D.use_lj || (D.coulomb_type != DMDX_COULOMB_NONE &&
+                                        D.physics == DMDX_CORRECTED);
+        if (D.rebuild_interval != -1 || !(D.list_skin > 0.0) || !pairs) continue;
+        const auto key = std::make_tuple((int64_t)std::llround(D.lj_cutoff * 1e6),
+                                         (int64_t)std::llround(D.coulomb_cutoff * 1e6),
+                                         (int64_t)std::llround(D.list_skin * 1e6));
+        auto it = index.find(key);
+        if (it == index.end()) {
+            it = index.emplace(key, (int)g.size()).first;
+            g.emplace_back();
+        }
+        g[it->second].sys.push_back((int)s);
+        g[it->second].atoms += D.n_atoms;
+    }
+    std::stable_sort(g.begin(), g.end(),
+                     [](const SkinGroup& a, const SkinGroup& b) { return a.atoms > b.atoms; });
+    return g;
+}
+
+std::vector<dmdx_system_desc> descs_with(const dmdx_engine* e, const std::vector<SkinGroup>& g,
+                                         const std::vector<int>& choice)
+{
+    std::vector<dmdx_system_desc> ds;
+    for (const auto& dc : e->descs) ds.push_back(dc->d);
+    for (size_t k = 0; k < g.size(); ++k)
+        for (int s : g[k].sys) ds[s].list_skin *= SKIN_MULT[choice[k]];
+    return ds;
+}
+
+// ms per step of a scratch batch with these skins and the engine's launch
+// shapes, the list built per cell (cells) or per atom: a window from the
+// start and one later (the hot first steps rebuild most)
+int time_skins_build(dmdx_engine* e, const std::vector<dmdx_system_desc>& ds, bool cells,
+                     double* ms)
+{
+    dmdx::BatchOptions bo;
+    bo.reuse_forces = e->b.options.reuse_forces;
+    dmdx::Batch b;
+    int rc = b.create(ds.data(), (int)ds.size(), bo);
+    if (rc) return rc;
+    b.d.pair_tpa = e->b.d.pair_tpa;
+    b.d.pair_width = e->b.d.pair_width;
+    b.d.build_tpa = e->b.d.build_tpa;
+    b.d.build_cells = cells;
+    b.d.build_tasks = e->b.d.build_tasks;
+    using clk = std::chrono::steady_clock;
+    rc = b.step(1);
+    if (!rc) rc = b.sync();
+    const auto t0 = clk::now();
+    if (!rc) rc = b.step(SKIN_WINDOW);
+    if (!rc) rc = b.sync();
+    const auto t1 = clk::now();
+    if (!rc) rc = b.step(SKIN_GAP);
+    if (!rc) rc = b.sync();
+    const auto t2 = clk::now();
+    if (!rc) rc = b.step(SKIN_WINDOW);
+    if (!rc) rc = b.sync();
+    const auto t3 = clk::now();
+    if (rc) return rc;
+    const double s = std::chrono::duration<double>((t1 - t0) + (t3 - t2)).count();
+    *ms = s * 1e3 / (2 * SKIN_WINDOW);
+    return DMDX_OK;
+}
+
+// the faster of the two list builds with these skins (which build wins
+// moves with the row length); *cells receives it
+int time_skins(dmdx_engine* e, const std::vector<SkinGroup>& g, const std::vector<int>& choice,
+               double* ms, bool* cells)
+{
+    const std::vector<dmdx_system_desc> ds = descs_with(e, g, choice);
+    double tc = 0.0, ta = 0.0;
+    int rc = time_skins_build(e, ds, true, &tc);
+    if (!rc) rc = time_skins_build(e, ds, false, &ta);
+    if (rc) return rc;
+    *cells = tc <= ta;
+    *ms = std::min(tc, ta);
+    return DMDX_OK;
+}
+
+bool skin_cache_lookup(const char* path, uint64_t key, size_t ngroups, std::vector<int>* choice)
+{
+    if (!path || !*path) return false;
+    std::ifstream f(path);
+    std::string line;
+    while (std::getline(f, line)) {
+        std::istringstream ss(line);
+        std::string k, tag;
+        if (!(ss >> k >> tag) || tag != "skin") continue;
+        if (std::strtoull(k.c_str(), nullptr, 16) != key) continue;
+        std::vector<int> c;
+        int v;
+        while (ss >> v) {
+            if (v < 0 || v >= NSKIN) return false;
+            c.push_back(v);
+        }
+        if (c.size() != ngroups) return false;
+        *choice = c;
+        return true;
+    }
+    return false;
+}
+
+void skin_cache_store(const char* path, uint64_t key, const std::vector<int>& choice)
+{
+    if (!path || !*path) return;
+    std::ofstream f(path, std::ios::app);
+    char buf[32];
+    std::snprintf(buf, sizeof(buf), "%016llx skin", (unsigned long long)key);
+    f << buf;
+    for (int c : choice) f << " " << c;
+    f << "\n";
+}
+
+// Make the engine's batch again from its descriptions with the chosen skins
+// (only for a batch that has not left the state it was created with).
+int apply_skins(dmdx_engine* e, const std::vector<SkinGroup>& g, const std::vector<int>& choice)
+{
+    std::vector<dmdx_system_desc> ds = descs_with(e, g, choice);
+    const dmdx::BatchOptions opt = e->b.options;
+    const dmdx::Dev keep = e->b.d;  // launch shapes
+    e->b.~Batch();
+    new (&e->b) dmdx::Batch();
+    int rc = e->b.create(ds.data(), (int)ds.size(), opt);
+    if (rc) {
+        // back to the descriptions the engine was created with
+        std::vector<dmdx_system_desc> orig;
+        for (const auto& dc : e->descs) orig.push_back(dc->d);
+        e->b.~Batch();
+        new (&e->b) dmdx::Batch();
+        const int rc2 = e->b.create(orig.data(), (int)orig.size(), opt);
+        return rc2 ? rc2 : rc;
+    }
+    for (size_t s = 0; s < ds.size(); ++s) e->descs[s]->d.list_skin = ds[s].list_skin;
+    e->b.d.pair_tpa = keep.pair_tpa;
+    e->b.d.pair_width = keep.pair_width;
+    e->b.d.build_tpa = keep.build_tpa;
+    e->b.d.build_cells = keep.build_cells;
+    e->b.d.build_tasks = keep.build_tasks;
+    return DMDX_OK;
+}
+
+// *changed: some group's skin was widened (the batch was made again)
+int tune_skins(dmdx_engine* e, const char* cache_path, bool* changed)
+{
+    *changed = false;
+    const std::vector<SkinGroup> g = skin_groups(e);
+    if (g.empty()) return DMDX_OK;
+    const uint64_t key = signature(e);
+    std::vector<int> best(g.size(), 0);
+    if (!skin_cache_lookup(cache_path, key, g.size(), &best)) {
+        double tbest = 0.0;
+        bool cells = true;
+        int rc = time_skins(e, g, best, &tbest, &cells);
+        if (rc) return rc;
+        for (size_t k = 0; k < g.size(); ++k) {
+            for (int c = 1; c < NSKIN; ++c) {
+                std::vector<int> trial = best;
+                trial[k] = c;
+                double t = 0.0;
+                rc = time_skins(e, g, trial, &t, &cells);
+                if (rc == DMDX_E_PARM) break;  // rows beyond their maximum: no wider skin
+                if (rc) return rc;
+                if (t < 0.98 * tbest) {
+                    best = trial;
+                    tbest = t;
+                }
+            }
+        }
+        skin_cache_store(cache_path, key, best);
+    }
+    for (int c : best)
+        if (c != 0) {
+            *changed = true;
+            return apply_skins(e, g, best);
+        }
+    return DMDX_OK;
+}
+
+// Launch shapes (see the top of this file) for the engine's systems as they
+// are; t receives the per-step times of the candidates (-1: not timed).
+int tune_shapes(dmdx_engine* e, int32_t steps, const char* cache_path, double* t)
+{
     const int* pl = PAIR_LANES;
     const int* pw = PAIR_WIDTH;
-    double t[DMDX_TUNE_TIMES];
-    for (double& v : t) v = -1.0;
-    auto report = [&]() {
-        if (pair_lanes) *pair_lanes = e->b.d.pair_tpa;
-        if (pair_width) *pair_width = e->b.d.pair_width;
-        if (build_lanes) *build_lanes = e->b.d.build_tpa;
-        if (times_ms)
-            for (int k = 0; k < DMDX_TUNE_TIMES; ++k) times_ms[k] = t[k];
-    };
-    if (e->b.d.precision == DMDX_FP64) {  // one lane per atom, nothing to tune
-        report();
-        return DMDX_OK;
-    }
     const uint64_t key = signature(e);
     int cp = 0, cw = 0, cb = 0, cc = 0, ct = 4;
     if (cache_lookup(cache_path, key, &cp, &cw, &cb, &cc, &ct) && pair_candidate(cp, cw) &&
@@ -165,7 +364,6 @@ extern "C" int dmdx_engine_autotune(dmdx_engine* e, int32_t steps, const char* c
         e->b.d.build_cells = cc != 0;
         e->b.d.build_tasks = ct;
         e->b.invalidate_graph();
-        report();
         return DMDX_OK;
     }
     // scratch batch: the same systems from their initial state, stepped as
@@ -229,6 +427,44 @@ extern "C" int dmdx_engine_autotune(dmdx_engine* e, int32_t steps, const char* c
     e->b.invalidate_graph();
     cache_store(cache_path, key, pl[bp], pw[bp], BUILD_LANES[ba], e->b.d.build_cells ? 1 : 0,
                 e->b.d.build_tasks, t);
+    return DMDX_OK;
+}
+
+}  // namespace
+
+extern "C" int dmdx_engine_autotune(dmdx_engine* e, int32_t steps, const char* cache_path,
+                                    int32_t* pair_lanes, int32_t* pair_width,
+                                    int32_t* build_lanes, double* times_ms)
+{
+    if (!e) return dmdx::fail(DMDX_E_INVALID, "null engine");
+    double t[DMDX_TUNE_TIMES];
+    for (double& v : t) v = -1.0;
+    auto report = [&]() {
+        if (pair_lanes) *pair_lanes = e->b.d.pair_tpa;
+        if (pair_width) *pair_width = e->b.d.pair_width;
+        if (build_lanes) *build_lanes = e->b.d.build_tpa;
+        if (times_ms)
+            for (int k = 0; k < DMDX_TUNE_TIMES; ++k) times_ms[k] = t[k];
+    };
+    if (e->b.d.precision == DMDX_FP64) {  // one lane per atom, nothing to tune
+        report();
+        return DMDX_OK;
+    }
+    if (e->tune_skin && e->b.pristine && !skin_groups(e).empty()) {
+        // shapes for the skins as they are (the skin trials run with them),
+        // then the skins, then the shapes again if a skin changed
+        int rc = tune_shapes(e, steps, cache_path, t);
+        bool changed = false;
+        if (!rc) rc = tune_skins(e, cache_path, &changed);
+        if (rc) return rc;
+        if (!changed) {
+            report();
+            return DMDX_OK;
+        }
+        for (double& v : t) v = -1.0;
+    }
+    const int rc = tune_shapes(e, steps, cache_path, t);
+    if (rc) return rc;
     report();
     return DMDX_OK;
 }

@@ -1076,6 +1076,7 @@ int Batch::enqueue_step(bool uniform)
 
 int Batch::step(int64_t nsteps)
 {
+    pristine = false;
     if (nsteps < 0) return fail(DMDX_E_INVALID, "negative step count");
     if (!options.use_graph) {
         for (int64_t i = 0; i < nsteps; ++i) TRY(enqueue_step(false));
@@ -1127,6 +1128,7 @@ void Batch::invalidate_graph()
 
 int Batch::compute(int accumulate)
 {
+    pristine = false;
     TRY(enqueue_compute(0, accumulate, 0, false));
     LAUNCH(launch_finalize(d, stream, 0));
     return DMDX_OK;
@@ -1134,6 +1136,7 @@ int Batch::compute(int accumulate)
 
 int Batch::rebuild_only()
 {
+    pristine = false;
     LAUNCH(launch_pre_rebuild_only(d, stream));
     if (d.any_list) LAUNCH(launch_build_lists(d, stream));
     for (auto& H : host)
@@ -1193,6 +1196,7 @@ int Batch::set_arrays(int s, int what, const double* a, const double* b, const d
                       bool on_device)
 {
     TRY(check_sys(s));
+    pristine = false;
     TRY(sync());
     const SysDev& S = hsys[s];
     double* dst[3];
@@ -1256,6 +1260,7 @@ int Batch::set_cell(int s, const double m[9], int type)
         d.any_wrap_mixed = true;  // the per-atom build joins the launches
         invalidate_graph();
     }
+    pristine = false;
     TRY(copy(d.sys + s, &S, sizeof(SysDev), hipMemcpyHostToDevice));
     host[s].valid = false;
     TRY(fetch_state());
@@ -1272,6 +1277,7 @@ int Batch::upload_sys(int s)
 
 int Batch::set_counter(int s, int counter)
 {
+    pristine = false;
     TRY(check_sys(s));
     TRY(fetch_state());
     hstate[s].counter = counter;
@@ -1282,6 +1288,7 @@ int Batch::set_counter(int s, int counter)
 
 int Batch::set_active(int s, int active)
 {
+    pristine = false;
     TRY(check_sys(s));
     TRY(fetch_state());
     hstate[s].active = active ? 1 : 0;
@@ -1296,6 +1303,7 @@ int Batch::set_active(int s, int active)
 int Batch::adopt(Batch& src, int ss, int ds)
 {
     TRY(check_sys(ds));
+    pristine = false;
     TRY(src.check_sys(ss));
     if (src.d.precision != d.precision) return fail(DMDX_E_MISMATCH, "precisions differ");
     TRY(src.fetch_state());
@@ -1367,6 +1375,7 @@ int Batch::adopt(Batch& src, int ss, int ds)
 
 int Batch::set_stop_at(const std::vector<int64_t>& stop)
 {
+    pristine = false;
     if ((int)stop.size() != d.nsys) return fail(DMDX_E_MISMATCH, "one step target per system");
     TRY(fetch_state());
     for (int s = 0; s < d.nsys; ++s) hstate[s].stop_at = stop[s];
@@ -1597,6 +1606,9 @@ int dmdx_engine_set_tuning(dmdx_engine* e, int32_t key, int32_t value)
         if (value != 1 && value != 2 && value != 4 && value != 8)
             return dmdx::fail(DMDX_E_PARM, "waves per cell must be 1, 2, 4 or 8");
         e->b.d.build_tasks = value;
+    } else if (key == 5) {
+        if (value != 0 && value != 1) return dmdx::fail(DMDX_E_PARM, "skin tuning is 0 or 1");
+        e->tune_skin = value != 0;
     } else {
         return dmdx::fail(DMDX_E_INVALID, "unknown tuning key");
     }
@@ -1614,8 +1626,17 @@ int dmdx_engine_get_tuning(dmdx_engine* e, int32_t key, int32_t* value)
     case 2: *value = d.pair_width; break;
     case 3: *value = d.build_cells ? 1 : 0; break;
     case 4: *value = d.build_tasks; break;
+    case 5: *value = e->tune_skin ? 1 : 0; break;
     default: return dmdx::fail(DMDX_E_INVALID, "unknown tuning key");
     }
+    return DMDX_OK;
+}
+
+int dmdx_engine_get_list_skin(dmdx_engine* e, int32_t sys, double* skin)
+{
+    if (!e || !skin) return dmdx::fail(DMDX_E_INVALID, "null argument");
+    if (sys < 0 || sys >= (int32_t)e->descs.size()) return dmdx::fail(DMDX_E_INVALID, "bad system");
+    *skin = e->descs[sys]->d.list_skin;
     return DMDX_OK;
 }
 

@@ -190,6 +190,10 @@ public:
 
     Dev d;
     BatchOptions options;
+    // no call has changed the state the batch was created with (no step,
+    // compute, list build or set/adopt of any system): re-creating the batch
+    // from its descriptions gives the same batch
+    bool pristine = true;
     std::vector<SysDev> hsys;
     std::vector<SysState> hstate;
     std::vector<HostSys> host;
@@ -231,4 +235,5 @@ struct dmdx_engine {
     dmdx::Batch b;
     // host copies of the descriptions (the autotuner builds scratch batches)
     std::vector<std::unique_ptr<dmdx::DescCopy>> descs;
+    bool tune_skin = true;  // the autotuner may widen Verlet-criterion skins
 };

@@ -57,6 +57,7 @@ def _setup(lib):
     lib.dmdx_engine_autotune.argtypes = [C.c_void_p, C.c_int32, C.c_char_p,
                                          C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                          C.POINTER(C.c_int32), C.POINTER(C.c_double)]
+    lib.dmdx_engine_get_list_skin.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]
     lib.dmdx_engine_snapshot_begin.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_int64)]
     lib.dmdx_engine_snapshot_read.argtypes = [C.c_void_p, C.c_int64, _dp, _dp, _dp,
                                               C.POINTER(C.c_int64)]
@@ -175,7 +176,7 @@ class Engine:
 
     def set_tuning(self, pair_lanes: int | None = None, build_lanes: int | None = None,
                    pair_width: int | None = None, build_cells: bool | None = None,
-                   build_tasks: int | None = None):
+                   build_tasks: int | None = None, tune_skin: bool | None = None):
         """Launch shapes of the mixed kernels (dmdx_engine_set_tuning);
         build_cells: list built per cell (default) or per atom; build_tasks:
         waves per cell of the per-cell build (1, 2, 4, 8)."""
@@ -183,6 +184,8 @@ class Engine:
             _abi.check(self.lib.dmdx_engine_set_tuning(self.h, 3, int(bool(build_cells))))
         if build_tasks is not None:
             _abi.check(self.lib.dmdx_engine_set_tuning(self.h, 4, int(build_tasks)))
+        if tune_skin is not None:
+            _abi.check(self.lib.dmdx_engine_set_tuning(self.h, 5, int(bool(tune_skin))))
         if pair_lanes is not None:
             _abi.check(self.lib.dmdx_engine_set_tuning(self.h, 0, int(pair_lanes)))
         if build_lanes is not None:
@@ -194,19 +197,36 @@ class Engine:
         """Time launch shapes of the mixed pair kernel (lanes per atom x
         entries per lane pass) and list build (8/16/32 lanes per atom, or per
         cell with 2/4/8 waves per cell) on a scratch copy of the systems and
-        apply the fastest (dmdx_engine_autotune).  `cache`: text file of
-        earlier results.  "build" names the chosen list build."""
+        apply the fastest (dmdx_engine_autotune); on an engine that has not
+        stepped, first the list skins of displacement-criterion systems.
+        `cache`: text file of earlier results.  "build" names the chosen list
+        build, "skins" the widened skins."""
         p, w, b = C.c_int32(), C.c_int32(), C.c_int32()
         t = (C.c_double * (len(PAIR_SHAPES) + len(BUILD_SHAPES)))()
         _abi.check(self.lib.dmdx_engine_autotune(
             self.h, int(steps), cache.encode() if cache else None, C.byref(p), C.byref(w),
             C.byref(b), t))
         cells, tasks = self.get_tuning(3), self.get_tuning(4)
+        # skins the autotuner widened: (original skin, chosen skin) -> systems
+        widened = {}
+        for k, s in enumerate(self.specs):
+            now = self.list_skin(k)
+            if now != s.list_skin:
+                key = (round(s.list_skin, 6), round(now, 6))
+                widened[key] = widened.get(key, 0) + 1
         return {"pair_lanes": p.value, "pair_width": w.value, "build_lanes": b.value,
                 "build_cells": bool(cells), "build_tasks": tasks,
                 "build": f"cell{tasks}" if cells else f"atom{b.value}",
+                "skins": [{"skin": a, "tuned": t, "systems": n}
+                          for (a, t), n in sorted(widened.items())],
                 "pair_ms": {k: t[i] for i, k in enumerate(PAIR_SHAPES)},
                 "build_ms": {k: t[len(PAIR_SHAPES) + i] for i, k in enumerate(BUILD_SHAPES)}}
+
+    def list_skin(self, sys: int = 0) -> float:
+        """The list skin system `sys` is built with (dmdx_engine_get_list_skin)."""
+        v = C.c_double()
+        _abi.check(self.lib.dmdx_engine_get_list_skin(self.h, int(sys), C.byref(v)))
+        return v.value
 
     def get_tuning(self, key: int) -> int:
         """Current value of a tuning key (dmdx_engine_get_tuning)."""

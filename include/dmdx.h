@@ -402,8 +402,9 @@ int dmdx_engine_kernel_time(dmdx_engine *e, int32_t group, double *total_ms, int
  * entries per lane per pass of the pair kernel (2, 4 or 8), key 3 = list
  * build per cell (1, default: the atoms of a cell share their candidates;
  * systems that need a minimum image are still built per atom) or per atom
- * (0), key 4 = waves per cell of the per-cell build (1, 2, 4 or 8).  The
- * list rows do not depend on keys 1, 3 or 4.  Pair shapes
+ * (0), key 4 = waves per cell of the per-cell build (1, 2, 4 or 8), key 5
+ * = whether dmdx_engine_autotune may widen list skins (1, default, or 0).
+ * The list rows do not depend on keys 1, 3 or 4.  Pair shapes
  * (lanes x entries): 2x8, 4x8, 8x8, 4x4, 8x4, 16x4, 32x4, 8x2, 16x2, 32x2;
  * another combination runs as the nearest of these (8 lanes where fewer
  * are not built for the width, at most 8 lanes with 8 entries).  Results do
@@ -426,12 +427,22 @@ int dmdx_engine_get_tuning(dmdx_engine *e, int32_t key, int32_t *value);
  * per cell (-1 when not timed); dmdx_engine_get_tuning reads the chosen
  * build (keys 1, 3, 4).  FP64 engines have
  * nothing to tune.
- * Mixed-precision results depend on the shape only through FP32 summation
- * order (well inside the mixed-precision tolerance). */
+ * First, on an engine that has not stepped or been changed since creation
+ * (and unless tuning key 5 is 0): the list skin of systems that rebuild by
+ * the displacement criterion (rebuild_interval -1), scaled by 1, 1.3, 1.6 or
+ * 2 per group of systems with the same cutoffs and skin, timed on scratch
+ * batches (20 steps from the start and 20 after 60 more); a wider skin must
+ * win by 2%.  The engine's batch is then made again with the chosen skins
+ * (dmdx_engine_get_list_skin reads them).
+ * Mixed-precision results depend on the shape and the skin only through
+ * FP32 summation order (well inside the mixed-precision tolerance). */
 #define DMDX_TUNE_TIMES 14
 int dmdx_engine_autotune(dmdx_engine *e, int32_t steps, const char *cache_path,
                          int32_t *pair_lanes, int32_t *pair_width, int32_t *build_lanes,
                          double *times_ms);
+/* The list skin system sys is built with (its description's, or the
+ * autotuner's choice). */
+int dmdx_engine_get_list_skin(dmdx_engine *e, int32_t sys, double *skin);
 
 /* Device-to-device copy of nbytes, repeated; reports the achieved GB/s
  * (read + write).  Calibrates hardware byte counters. */
