@@ -1,10 +1,12 @@
 // autotune.cpp -- launch-shape autotuner for the mixed-precision kernels.
 //
-// The pair kernel of the mixed path gives each atom a group of 2 to 32
-// lanes, each taking 2, 4 or 8 list entries per pass (more entries per lane
-// keep more gathers in flight per wave, fewer need fewer registers so more
-// waves fit; fewer lanes per atom waste fewer lanes on the ragged end of a
-// row), and the list build gives each atom 8, 16 or 32 lanes or shares each
+// The per-atom pair kernel of the mixed path gives each atom a group of 2
+// to 32 lanes, each taking 2, 4 or 8 list entries per pass (more entries per
+// lane keep more gathers in flight per wave, fewer need fewer registers so
+// more waves fit; fewer lanes per atom waste fewer lanes on the ragged end of
+// a row); LJ-only systems can instead run on row tiles staged in LDS with 4
+// or 8 entries per lane block, timed after the best per-atom shape (which
+// charged systems keep); and the list build gives each atom 8, 16 or 32 lanes or shares each
 // cell's candidates among 2, 4 or 8 waves.  Which is fastest depends on the systems (how
 // many neighbours an atom has, how many atoms a batch holds, how many rebuild
 // at once), so the choice is measured: a scratch copy of the engine's batch
@@ -84,9 +86,10 @@ uint64_t signature(const dmdx_engine* e)
 
 // A cached line: key, pair lanes, pair width, per-atom build lanes, then
 // optionally the per-cell build switch and its waves per cell (a line
-// without them stands for the per-atom build).
+// without them stands for the per-atom build) and the row-tile setting (0
+// when absent).
 bool cache_lookup(const char* path, uint64_t key, int* pair, int* width, int* build, int* cells,
-                  int* tasks)
+                  int* tasks, int* rows)
 {
     if (!path || !*path) return false;
     std::ifstream f(path);
@@ -100,25 +103,30 @@ bool cache_lookup(const char* path, uint64_t key, int* pair, int* width, int* bu
             *pair = p;
             *width = w;
             *build = b;
-            int c = 0, t = 4;
+            int c = 0, t = 4, r = 0;
             if (!(ss >> c >> t) || (c != 0 && c != 1)) c = 0, t = 4;
+            else if (!(ss >> r) || (r != 0 && r != 4 && r != 8)) r = 0;
             *cells = c;
             *tasks = t;
+            *rows = r;
             return true;
         }
     }
     return false;
 }
 
-// pair kernel candidates: (lanes per atom, entries per lane per pass);
-// list build candidates: per atom with 8, 16 or 32 lanes, then per cell
-// with 2, 4 or 8 waves per cell
-constexpr int NPAIR = 8, NBUILD = 6, NBUILD_ATOM = 3;
+// pair kernel candidates: (lanes per atom, entries per lane per pass) of the
+// per-atom kernel, then row tiles with 4 or 8 entries per block; list build
+// candidates: per atom with 8, 16 or 32 lanes, then per cell with 2, 4 or 8
+// waves per cell
+constexpr int NPAIR = 8, NROWS = 2, NBUILD = 6, NBUILD_ATOM = 3;
 constexpr int PAIR_LANES[NPAIR] = {2, 4, 8, 4, 8, 16, 8, 16};
 constexpr int PAIR_WIDTH[NPAIR] = {8, 8, 8, 4, 4, 4, 2, 2};
+constexpr int ROWS_BLOCK[NROWS] = {4, 8};
 constexpr int BUILD_LANES[NBUILD] = {8, 16, 32, 0, 0, 0};
 constexpr int BUILD_TASKS[NBUILD] = {0, 0, 0, 2, 4, 8};
-static_assert(NPAIR + NBUILD == DMDX_TUNE_TIMES, "times_ms layout");
+constexpr int T_ROWS = NPAIR, T_BUILD = NPAIR + NROWS;  // times_ms layout
+static_assert(NPAIR + NROWS + NBUILD == DMDX_TUNE_TIMES, "times_ms layout");
 
 bool pair_candidate(int p, int w)
 {
@@ -128,24 +136,28 @@ bool pair_candidate(int p, int w)
 }
 
 void cache_store(const char* path, uint64_t key, int pair, int width, int build, int cells,
-                 int tasks, const double* t)
+                 int tasks, int rows, const double* t)
 {
     if (!path || !*path) return;
     std::ostringstream o;
     char buf[64];
-    std::snprintf(buf, sizeof(buf), "%016llx %d %d %d %d %d  # pair ms", (unsigned long long)key,
-                  pair, width, build, cells, tasks);
+    std::snprintf(buf, sizeof(buf), "%016llx %d %d %d %d %d %d  # pair ms",
+                  (unsigned long long)key, pair, width, build, cells, tasks, rows);
     o << buf;
     for (int c = 0; c < NPAIR; ++c) {
         std::snprintf(buf, sizeof(buf), " %dx%d %.4f", PAIR_LANES[c], PAIR_WIDTH[c], t[c]);
         o << buf;
     }
+    for (int c = 0; c < NROWS; ++c) {
+        std::snprintf(buf, sizeof(buf), " rows%d %.4f", ROWS_BLOCK[c], t[T_ROWS + c]);
+        o << buf;
+    }
     o << " build ms";
     for (int c = 0; c < NBUILD; ++c) {
         if (BUILD_LANES[c])
-            std::snprintf(buf, sizeof(buf), " atom%d %.4f", BUILD_LANES[c], t[NPAIR + c]);
+            std::snprintf(buf, sizeof(buf), " atom%d %.4f", BUILD_LANES[c], t[T_BUILD + c]);
         else
-            std::snprintf(buf, sizeof(buf), " cell%d %.4f", BUILD_TASKS[c], t[NPAIR + c]);
+            std::snprintf(buf, sizeof(buf), " cell%d %.4f", BUILD_TASKS[c], t[T_BUILD + c]);
         o << buf;
     }
     std::ofstream f(path, std::ios::app);
@@ -212,6 +224,7 @@ int time_skins_build(dmdx_engine* e, const std::vector<dmdx_system_desc>& ds, bo
     if (rc) return rc;
     b.d.pair_tpa = e->b.d.pair_tpa;
     b.d.pair_width = e->b.d.pair_width;
+    b.d.pair_rows = e->b.d.pair_rows;
     b.d.build_tpa = e->b.d.build_tpa;
     b.d.build_cells = cells;
     b.d.build_tasks = e->b.d.build_tasks;
@@ -305,6 +318,7 @@ int apply_skins(dmdx_engine* e, const std::vector<SkinGroup>& g, const std::vect
     for (size_t s = 0; s < ds.size(); ++s) e->descs[s]->d.list_skin = ds[s].list_skin;
     e->b.d.pair_tpa = keep.pair_tpa;
     e->b.d.pair_width = keep.pair_width;
+    e->b.d.pair_rows = keep.pair_rows;
     e->b.d.build_tpa = keep.build_tpa;
     e->b.d.build_cells = keep.build_cells;
     e->b.d.build_tasks = keep.build_tasks;
@@ -355,11 +369,12 @@ int tune_shapes(dmdx_engine* e, int32_t steps, const char* cache_path, double* t
     const int* pl = PAIR_LANES;
     const int* pw = PAIR_WIDTH;
     const uint64_t key = signature(e);
-    int cp = 0, cw = 0, cb = 0, cc = 0, ct = 4;
-    if (cache_lookup(cache_path, key, &cp, &cw, &cb, &cc, &ct) && pair_candidate(cp, cw) &&
+    int cp = 0, cw = 0, cb = 0, cc = 0, ct = 4, cr = 0;
+    if (cache_lookup(cache_path, key, &cp, &cw, &cb, &cc, &ct, &cr) && pair_candidate(cp, cw) &&
         (cb == 8 || cb == 16 || cb == 32) && (ct == 1 || ct == 2 || ct == 4 || ct == 8)) {
         e->b.d.pair_tpa = cp;
         e->b.d.pair_width = cw;
+        e->b.d.pair_rows = cr;
         e->b.d.build_tpa = cb;
         e->b.d.build_cells = cc != 0;
         e->b.d.build_tasks = ct;
@@ -381,17 +396,36 @@ int tune_shapes(dmdx_engine* e, int32_t steps, const char* cache_path, double* t
     const int n = steps > 0 ? steps : 5;
     double ms0, ms1;
     int64_t n0, n1;
-    // pair kernel: per-step time of each shape over n steps
+    auto time_pair = [&](double* out) -> int {
+        int r = scratch.step(1);  // first launches of this shape
+        if (!r) r = scratch.kernel_time(dmdx::KG_PAIR, &ms0, &n0);
+        if (!r) r = scratch.step(n);
+        if (!r) r = scratch.kernel_time(dmdx::KG_PAIR, &ms1, &n1);
+        if (!r) *out = (ms1 - ms0) / (double)n;
+        return r;
+    };
+    // pair kernel: per-step time of each per-atom shape over n steps, then
+    // of the row tiles beside the best per-atom shape
+    scratch.d.pair_rows = 0;
     for (int c = 0; c < NPAIR; ++c) {
         scratch.d.pair_tpa = pl[c];
         scratch.d.pair_width = pw[c];
-        rc = scratch.step(1);  // first launches of this shape
-        if (!rc) rc = scratch.kernel_time(dmdx::KG_PAIR, &ms0, &n0);
-        if (!rc) rc = scratch.step(n);
-        if (!rc) rc = scratch.kernel_time(dmdx::KG_PAIR, &ms1, &n1);
+        rc = time_pair(&t[c]);
         if (rc) return rc;
-        t[c] = (ms1 - ms0) / (double)n;
     }
+    int bp = 0;
+    for (int c = 1; c < NPAIR; ++c)
+        if (t[c] < t[bp]) bp = c;
+    scratch.d.pair_tpa = pl[bp];
+    scratch.d.pair_width = pw[bp];
+    int br = -1;  // per-atom kernel for every system
+    for (int c = 0; c < NROWS; ++c) {
+        scratch.d.pair_rows = ROWS_BLOCK[c];
+        rc = time_pair(&t[T_ROWS + c]);
+        if (rc) return rc;
+        if (t[T_ROWS + c] < (br < 0 ? t[bp] : t[T_ROWS + br])) br = c;
+    }
+    scratch.d.pair_rows = br < 0 ? 0 : ROWS_BLOCK[br];
     // list build: the candidates take turns step by step, so each sees the
     // same mix of full and partial rebuilds; time per step of the list group
     double acc[NBUILD] = {};
@@ -408,25 +442,24 @@ int tune_shapes(dmdx_engine* e, int32_t steps, const char* cache_path, double* t
         acc[k % NBUILD] += ms1 - ms0;
         ms0 = ms1;
     }
-    for (int c = 0; c < NBUILD; ++c) t[NPAIR + c] = acc[c] / n;
-    // the fastest pair shape; the fastest list build overall, and the
+    for (int c = 0; c < NBUILD; ++c) t[T_BUILD + c] = acc[c] / n;
+    // the fastest pair kernel; the fastest list build overall, and the
     // fastest per-atom lane count (the per-atom build serves the systems
     // that need a minimum image even when the per-cell build is chosen)
-    int bp = 0, bb = 0, ba = 0;
-    for (int c = 1; c < NPAIR; ++c)
-        if (t[c] < t[bp]) bp = c;
+    int bb = 0, ba = 0;
     for (int c = 1; c < NBUILD; ++c)
-        if (t[NPAIR + c] < t[NPAIR + bb]) bb = c;
+        if (t[T_BUILD + c] < t[T_BUILD + bb]) bb = c;
     for (int c = 1; c < NBUILD_ATOM; ++c)
-        if (t[NPAIR + c] < t[NPAIR + ba]) ba = c;
+        if (t[T_BUILD + c] < t[T_BUILD + ba]) ba = c;
     e->b.d.pair_tpa = pl[bp];
     e->b.d.pair_width = pw[bp];
+    e->b.d.pair_rows = scratch.d.pair_rows;
     e->b.d.build_tpa = BUILD_LANES[ba];
     e->b.d.build_cells = BUILD_LANES[bb] == 0;
     if (e->b.d.build_cells) e->b.d.build_tasks = BUILD_TASKS[bb];
     e->b.invalidate_graph();
     cache_store(cache_path, key, pl[bp], pw[bp], BUILD_LANES[ba], e->b.d.build_cells ? 1 : 0,
-                e->b.d.build_tasks, t);
+                e->b.d.build_tasks, e->b.d.pair_rows, t);
     return DMDX_OK;
 }
 

@@ -78,6 +78,9 @@ struct SysDev {
     int32_t cell_off;    // start of this system's cells
     int32_t nc[3];       // cells per edge at allocation (upper bound)
     int32_t excl_off;    // per-atom exclusion CSR base (slot-indexed)
+    int32_t row_off;     // first of this system's nc[1] * nc[2] x-rows of cells
+                         // (mixed row-tile pair kernel units)
+    int32_t pad_row_;
     // bonded terms: [off, off + n_pad) slots of the packed term arrays
     int64_t bond_off, angle_off, dih_off, imp_off, ex_off;
     int32_t n_bond, n_angle, n_dih, n_imp, n_ex;
@@ -142,26 +145,32 @@ struct SysState {
 
 enum { FLAG_NBR_OVERFLOW = 1, FLAG_NONFINITE = 2 };
 
-// Mixed-path list entries: bits 0-22 the neighbour's cell-order slot
-// relative to its system's first slot, bits 23-31 the neighbour cell's
-// offset from the atom's cell along x, y, z (3-bit two's complement fields,
-// -2..2, read back with one signed bit-field extract; the mixed path's cells
-// are at least rlist / cdiv wide with cdiv 1 or 2, so neighbours lie within
-// +-cdiv cells).  With cell-local coordinates l
-// and cell width w the displacement is d = l_j - l_i + offset * w (the
-// minimum image when the system has >= 2 cdiv + 1 cells per edge).
+// Mixed-path list entries: bits 0-24 the neighbour's cell-order slot
+// relative to its system's first slot, bits 25-31 the neighbour cell's
+// offset from the atom's cell as one dense code, (ox + 2) + 5 (oy + 2) +
+// 25 (oz + 2) for offsets -2..2 (the mixed path's cells are at least
+// rlist / cdiv wide with cdiv 1 or 2, so neighbours lie within +-cdiv
+// cells).  With cell-local coordinates l and cell width w the displacement
+// is d = l_j - l_i + offset * w (the minimum image when the system has
+// >= 2 cdiv + 1 cells per edge).  The pair kernels look the code up in
+// 128-entry LDS tables.
 // k_bin_system: systems of at most BIN_WG_ATOMS atoms whose allocated cells
 // fit in LDS are binned by one workgroup each
 constexpr int BIN_WG_ATOMS = 8192;
 constexpr int BIN_WG_CELLS = 4096;
 
-constexpr int NBR_SLOT_BITS = 23;
+constexpr int NBR_SLOT_BITS = 25;
 constexpr uint32_t NBR_SLOT_MASK = (1u << NBR_SLOT_BITS) - 1u;
-// Mixed path: rows are padded to a multiple of 8 entries with this offset
-// code (an x offset of 3 cells, which no neighbour cell has) over the atom's
-// own slot; the pair kernel's shift table maps it to a shift far beyond any
-// cutoff, so a padding entry adds exactly zero.
-constexpr uint32_t NBR_PAD_CODE = 3u;
+constexpr int NBR_CODES = 128;  // table size of the offset codes
+__host__ __device__ __forceinline__ uint32_t nbr_code(int ox, int oy, int oz)
+{
+    return (uint32_t)((ox + 2) + 5 * (oy + 2) + 25 * (oz + 2));
+}
+// Mixed path: rows are padded to a multiple of 8 entries with this code,
+// which no neighbour cell has, over slot 0; the pair kernels map it to a
+// position far beyond any cutoff, so a padding entry adds exactly zero.
+constexpr uint32_t NBR_PAD_CODE = 125u;
+constexpr uint32_t NBR_PAD_ENTRY = NBR_PAD_CODE << NBR_SLOT_BITS;
 
 // Charges are spread onto the PME grid as 64-bit fixed-point integers with
 // 2^44 units per e: integer atomic adds are associative, so the grid (and
@@ -224,6 +233,8 @@ struct Dev {
     int *acell = nullptr, *ccount = nullptr, *cstart = nullptr, *cfill = nullptr;
     int *cell_atoms = nullptr, *cell_sys = nullptr, *order = nullptr, *nn = nullptr;
     int* nbr = nullptr;  // row-major: row of atom slot t at nbr_off + (t - atom_off) * M
+    int nrows = 0;         // x-rows of cells over all systems (allocated)
+    int* row_sys = nullptr;  // system of each x-row
     int *ex_start = nullptr, *ex_list = nullptr;
     double4* pos4s = nullptr;  // (x, y, z, q) in cell order (list build; mixed pair kernel)
     double4* xref = nullptr;   // positions at the last list build (displacement criterion)
@@ -239,6 +250,9 @@ struct Dev {
     bool build_cells = true;   // mixed list build: per cell where no minimum image is needed
     int build_tasks = 4;       // mixed per-cell build: waves per cell (1, 2, 4, 8)
     int pair_width = 8;        // mixed pair kernel: entries per lane per pass (2, 4, 8)
+    // mixed LJ pair forces: 0 = per-atom kernel (k_pair_mixed), 4 or 8 =
+    // row-tile kernel (k_pair_rows) with that many list entries per block
+    int pair_rows = 0;
     // LJ tables
     double *lj_sig = nullptr, *lj_eps = nullptr;
     float *lj_c12 = nullptr, *lj_c6 = nullptr;
@@ -310,12 +324,14 @@ struct Dev {
     bool any_list = false, any_pair = false, any_pme = false, any_excl = false;
     bool any_coul_pairs = false;  // some system computes real-space erfc pairs
     bool any_plain_pairs = false;  // some system does not (LJ-only pair variant)
+    bool any_plain_nolist = false;  // some system has no pair terms at all
     bool any_gather = false, any_baro = false;
 };
 
 // host launchers (one per kernel group); return hipGetLastError()
 hipError_t launch_build_lists(const Dev& d, hipStream_t s);
 hipError_t launch_pair(const Dev& d, hipStream_t s, int accumulate);
+hipError_t launch_pair_rows(const Dev& d, hipStream_t s, int accumulate);  // LJ-only systems
 void pair_shape(const Dev& d, int& tpa, int& w);  // effective mixed pair-kernel launch shape
 hipError_t launch_pack(const Dev& d, hipStream_t s);
 hipError_t launch_bonded(const Dev& d, hipStream_t s);

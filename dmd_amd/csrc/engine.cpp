@@ -168,6 +168,7 @@ int Batch::create(const dmdx_system_desc* descs, int nsys, const BatchOptions& o
     std::vector<double> lj_sig, lj_eps;
     std::vector<float> c12, c6;
     std::vector<int> cell_sys;
+    std::vector<int> row_sys;
     std::vector<int> b_sys, bi, bj, a_sys, ai, aj, ak, d_sys, di, dj, dk, dl, dper, i_sys, ii, ij,
         ik, il, e_sys, ei, ej;
     std::vector<double> bk, br0, akt, at0, dkp, dp0, ikp, ip0;
@@ -276,6 +277,8 @@ int Batch::create(const dmdx_system_desc* descs, int nsys, const BatchOptions& o
         S.bin_wg = (S.has_list && n <= BIN_WG_ATOMS && nc3 <= BIN_WG_CELLS) ? 1 : 0;
         ncells += nc3;
         for (int c = 0; c < nc3; ++c) cell_sys.push_back(s);
+        S.row_off = (int32_t)row_sys.size();
+        for (int r = 0; r < S.nc[1] * S.nc[2]; ++r) row_sys.push_back(s);
 
         double qsum = 0.0, sum_q2 = 0.0;
         for (int i = 0; i < n_pad; ++i) {
@@ -726,6 +729,7 @@ int Batch::create(const dmdx_system_desc* descs, int nsys, const BatchOptions& o
         d.any_pme |= S.pme != 0;
         d.any_coul_pairs |= S.coul_pairs != 0;
         d.any_plain_pairs |= S.coul_pairs == 0;
+        d.any_plain_nolist |= S.coul_pairs == 0 && !S.has_list;
         if (S.has_list) {
             if (S.bin_wg) ++d.n_bin_wg;
             else d.any_bin_multi = true;
@@ -769,6 +773,8 @@ int Batch::create(const dmdx_system_desc* descs, int nsys, const BatchOptions& o
     TRY(dev_alloc_zero(&d.cfill, ncells, owned));
     TRY(dev_alloc_zero(&d.cell_atoms, slot, owned));
     TRY(dev_upload(&d.cell_sys, cell_sys, owned));
+    d.nrows = (int)row_sys.size();
+    TRY(dev_upload(&d.row_sys, row_sys, owned));
     TRY(dev_alloc_zero(&d.rb_sys, (size_t)nsys, owned));
     TRY(dev_alloc_zero(&d.rb_pre, (size_t)nsys + 1, owned));
     TRY(dev_alloc_zero(&d.rb_cpre, (size_t)nsys + 1, owned));
@@ -1609,6 +1615,10 @@ int dmdx_engine_set_tuning(dmdx_engine* e, int32_t key, int32_t value)
     } else if (key == 5) {
         if (value != 0 && value != 1) return dmdx::fail(DMDX_E_PARM, "skin tuning is 0 or 1");
         e->tune_skin = value != 0;
+    } else if (key == 6) {
+        if (value != 0 && value != 4 && value != 8)
+            return dmdx::fail(DMDX_E_PARM, "row-tile pair kernel is 0 (off), 4 or 8 entries per block");
+        e->b.d.pair_rows = value;
     } else {
         return dmdx::fail(DMDX_E_INVALID, "unknown tuning key");
     }
@@ -1627,6 +1637,7 @@ int dmdx_engine_get_tuning(dmdx_engine* e, int32_t key, int32_t* value)
     case 3: *value = d.build_cells ? 1 : 0; break;
     case 4: *value = d.build_tasks; break;
     case 5: *value = e->tune_skin ? 1 : 0; break;
+    case 6: *value = d.pair_rows; break;
     default: return dmdx::fail(DMDX_E_INVALID, "unknown tuning key");
     }
     return DMDX_OK;

@@ -419,7 +419,7 @@ __device__ double block_range_sum(const double* part, int64_t off, int64_t n, do
 
 struct Parts {
     const double *lj, *coul, *bond, *angle, *dih, *imp, *pme, *excl, *ke;
-    int pair_div;  // atom slots per pair-energy partial (FP64: 1; mixed: one per wave)
+    int pair_div;  // atom slots per pair-energy partial (1: one per slot)
 };
 
 __device__ __forceinline__ double temperature_of(double ekin, int n)
@@ -462,8 +462,7 @@ __global__ void __launch_bounds__(FIN_THREADS) k_finalize(const SysDev* __restri
     // the nine sums (8 energy terms and the kinetic energy) in one pass: each
     // thread's strided sums, then a fixed tree over each wave and a fixed
     // pairwise tree over the FIN_WAVES waves -- deterministic, one barrier
-    // pair energies are per atom slot (FP64) or per pair-kernel wave
-    // (mixed), the others per wave or workgroup
+    // pair energies are per atom slot, the others per wave or workgroup
     const double* arr[9] = {P.lj, P.coul, P.bond, P.angle, P.dih, P.imp, P.pme, P.excl, P.ke};
     const int64_t po = S.atom_off / P.pair_div, pn = S.n_pad / P.pair_div;
     const int64_t off[9] = {po, po, S.bw_off, S.gw_off, S.dw_off,
@@ -761,10 +760,8 @@ hipError_t launch_kick_ke(const Dev& d, hipStream_t s)
 
 hipError_t launch_finalize(const Dev& d, hipStream_t s, int with_kinetic)
 {
-    int tpa = 1, w = 0;
-    if (d.precision != DMDX_FP64) pair_shape(d, tpa, w);
-    Parts P{d.p_lj, d.p_coul, d.p_bond, d.p_angle, d.p_dih, d.p_imp, d.p_pme, d.p_excl, d.p_ke,
-            d.precision == DMDX_FP64 ? 1 : 64 / tpa};
+    // pair energies: one partial per atom slot on both paths
+    Parts P{d.p_lj, d.p_coul, d.p_bond, d.p_angle, d.p_dih, d.p_imp, d.p_pme, d.p_excl, d.p_ke, 1};
     k_finalize<<<d.nsys, FIN_THREADS, 0, s>>>(d.sys, d.st, P, with_kinetic);
     return hipGetLastError();
 }

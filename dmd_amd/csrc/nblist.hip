@@ -653,9 +653,7 @@ __device__ __forceinline__ void k_nbr_build_unit(int64_t unit_,
                 int xx = cx + ox[c];
                 xx = xx < 0 ? xx + nx : (xx >= nx ? xx - nx : xx);
                 const int cell = S.cell_off + (zz * ny + yy) * nx + xx;
-                const uint32_t shift = ((uint32_t)(ox[c] + 1) << NBR_SLOT_BITS) |
-                                       ((uint32_t)(oy[b] + 1) << (NBR_SLOT_BITS + 2)) |
-                                       ((uint32_t)(oz[a] + 1) << (NBR_SLOT_BITS + 4));
+                const uint32_t shift = nbr_code(ox[c], oy[b], oz[a]) << NBR_SLOT_BITS;
                 const int64_t base = S.atom_off + cstart[cell];
                 const int cnt = ccount[cell];
                 for (int m0 = 0; m0 < cnt; m0 += TPA) {
@@ -829,9 +827,7 @@ __device__ __forceinline__ void k_nbr_build_mixed_unit(int64_t t,
                 const int c = S.cell_off + (zz * ny + yy) * nx + xx;
                 tb[jc] = cstart[c];
                 cnt_q[q] = ccount[c];
-                const uint32_t code = (((uint32_t)ox & 7u) << NBR_SLOT_BITS) |
-                                      (((uint32_t)oy & 7u) << (NBR_SLOT_BITS + 3)) |
-                                      (((uint32_t)oz & 7u) << (NBR_SLOT_BITS + 6));
+                const uint32_t code = nbr_code(ox, oy, oz) << NBR_SLOT_BITS;
                 tsh[jc] = make_float4((float)ox * wx - li.x, (float)oy * wy - li.y,
                                       (float)oz * wz - li.z, __uint_as_float(code));
             }
@@ -928,8 +924,7 @@ __device__ __forceinline__ void k_nbr_build_mixed_unit(int64_t t,
     if (lane == 0) nn[t] = k;
     // pad the row to a multiple of 8 entries (M is one), so the pair kernel's
     // 8-entry vector loads past the row end read entries that add zero
-    const uint32_t pad = (uint32_t)lt | (NBR_PAD_CODE << NBR_SLOT_BITS);
-    for (int p = k + lane; p < ((k + 7) & ~7); p += TPA) row[p] = (int)pad;
+    for (int p = k + lane; p < ((k + 7) & ~7); p += TPA) row[p] = (int)NBR_PAD_ENTRY;
 }
 
 template <int TPA>
@@ -1033,9 +1028,7 @@ __global__ void __launch_bounds__(64 * CB_WAVES) k_nbr_build_cells(
                 const int cc = S.cell_off + (zz * ny + yy) * nx + xx;
                 tb[jc] = cstart[cc];
                 cnt[q] = ccount[cc];
-                const uint32_t code = (((uint32_t)ox & 7u) << NBR_SLOT_BITS) |
-                                      (((uint32_t)oy & 7u) << (NBR_SLOT_BITS + 3)) |
-                                      (((uint32_t)oz & 7u) << (NBR_SLOT_BITS + 6));
+                const uint32_t code = nbr_code(ox, oy, oz) << NBR_SLOT_BITS;
                 tsh[jc] = make_float4((float)ox * wx, (float)oy * wy, (float)oz * wz,
                                       __uint_as_float(code));
             }
@@ -1165,8 +1158,7 @@ __global__ void __launch_bounds__(64 * CB_WAVES) k_nbr_build_cells(
                 nn[ti] = kk;
                 // pad the row to a multiple of 8 entries (as k_nbr_build_mixed)
                 int* row = nbr + S.nbr_off + (int64_t)lti * M;
-                const uint32_t pad = (uint32_t)lti | (NBR_PAD_CODE << NBR_SLOT_BITS);
-                for (int p = kk; p < ((kk + 7) & ~7); ++p) row[p] = (int)pad;
+                for (int p = kk; p < ((kk + 7) & ~7); ++p) row[p] = (int)NBR_PAD_ENTRY;
             }
         }
     }

@@ -158,7 +158,7 @@ __device__ __forceinline__ void k_pair_mixed_unit(int64_t unit_,
     const int* __restrict__ typ_s, const float* __restrict__ lj_c12,
     const float* __restrict__ lj_c6, double* __restrict__ fx, double* __restrict__ fy,
     double* __restrict__ fz, double* __restrict__ e_lj_part, double* __restrict__ e_c_part,
-    int accumulate, float4* __restrict__ tab, int& tab_sys)
+    int accumulate, int rows_on, float4* __restrict__ tab, int& tab_sys)
 {
     // FP32 arithmetic of this kernel may fuse into FMAs (the library is built
     // with contraction off for the FP64 parity kernels)
@@ -178,8 +178,10 @@ __device__ __forceinline__ void k_pair_mixed_unit(int64_t unit_,
     const SysDev& S = sys[s];
     // Each system is computed by the one variant its own terms select
     // (COUL: real-space Coulomb pairs), whatever else shares the batch, so
-    // its forces are bit-identical alone and in any mix.
+    // its forces are bit-identical alone and in any mix; with row tiles on,
+    // k_pair_rows takes every LJ-only system that has a list.
     if ((S.coul_pairs != 0) != COUL) return;
+    if (!COUL && rows_on && S.has_list) return;
     const int lt = (int)(t - S.atom_off);
     // padding slots stay in the wave (no entries) for the wave's energy sum
     const bool real = lt < S.n;
@@ -195,12 +197,10 @@ __device__ __forceinline__ void k_pair_mixed_unit(int64_t unit_,
     if (s != tab_sys) {
         for (int c = threadIdx.x & 63; c < 125; c += 64) {
             const int ox = c % 5 - 2, oy = (c / 5) % 5 - 2, oz = c / 25 - 2;
-            const uint32_t code = ((uint32_t)ox & 7u) | (((uint32_t)oy & 7u) << 3) |
-                                  (((uint32_t)oz & 7u) << 6);
-            tab[code] = make_float4((float)ox * wx, (float)oy * wy, (float)oz * wz, 0.0f);
+            tab[c] = make_float4((float)ox * wx, (float)oy * wy, (float)oz * wz, 0.0f);
         }
-        // row padding (the atom itself, 1e15 nm away): r2 = 1e30, outside
-        // every cutoff, and fs * dx = 0 exactly
+        // row padding (slot 0, 1e15 nm away): r2 = 1e30, outside every
+        // cutoff, and fs * dx = 0 exactly
         if ((threadIdx.x & 63) == 0) tab[NBR_PAD_CODE] = make_float4(1e15f, 0.0f, 0.0f, 0.0f);
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -384,7 +384,7 @@ __device__ __forceinline__ void k_pair_mixed_unit(int64_t unit_,
                     px = fs * dx;
                     py = fs * dy;
                     pz = fs * dz;
-                    pel = el;  // rows hold both orders of a pair: halved below
+                    pel = el;
                     pec = ecl;
                 } else {
                     px += fs * dx;
@@ -417,9 +417,11 @@ __device__ __forceinline__ void k_pair_mixed_unit(int64_t unit_,
     fzi = group_sum<TPA>(fzi);
     ej = group_sum<TPA>(ej);
     ec = group_sum<TPA>(ec);
-    // one energy partial per wave (its 64 / TPA atom slots), a fixed tree
-    const double ejw = wave_sum(lane == 0 ? ej : 0.0);
-    const double ecw = COUL ? wave_sum(lane == 0 ? ec : 0.0) : 0.0;
+    // one energy partial per slot (half the row: rows hold both orders)
+    if (lane == 0) {
+        e_lj_part[t] = 0.5 * ej;
+        e_c_part[t] = 0.5 * ec;
+    }
     if (lane == 0 && real) {
         const int i = order[t];
         if (accumulate) {
@@ -431,10 +433,6 @@ __device__ __forceinline__ void k_pair_mixed_unit(int64_t unit_,
         fy[i] = fyi;
         fz[i] = fzi;
     }
-    if ((threadIdx.x & 63) == 0) {
-        e_lj_part[t / (64 / TPA)] = 0.5 * ejw;
-        e_c_part[t / (64 / TPA)] = 0.5 * ecw;
-    }
 }
 
 template <int TPA, int W, bool COUL>
@@ -445,11 +443,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(COUL ?
     const int* __restrict__ typ_s, const float* __restrict__ lj_c12,
     const float* __restrict__ lj_c6, double* __restrict__ fx, double* __restrict__ fy,
     double* __restrict__ fz, double* __restrict__ e_lj_part, double* __restrict__ e_c_part,
-    int accumulate, int64_t nunits_)
+    int accumulate, int rows_on, int64_t nunits_)
 {
-    __shared__ float4 shift_tab[4][512];  // per wave
+    __shared__ float4 shift_tab[4][NBR_CODES];  // per wave
     int tab_sys = -1;
-    DMDX_FOR_UNITS(u_, nunits_) k_pair_mixed_unit<TPA, W, COUL>(u_, sys, st, slot_sys, nslots, order, nn, nbr, lpos, typ_s, lj_c12, lj_c6, fx, fy, fz, e_lj_part, e_c_part, accumulate, shift_tab[threadIdx.x >> 6], tab_sys);
+    DMDX_FOR_UNITS(u_, nunits_) k_pair_mixed_unit<TPA, W, COUL>(u_, sys, st, slot_sys, nslots, order, nn, nbr, lpos, typ_s, lj_c12, lj_c6, fx, fy, fz, e_lj_part, e_c_part, accumulate, rows_on, shift_tab[threadIdx.x >> 6], tab_sys);
 }
 
 }  // namespace dmdx
@@ -481,7 +479,7 @@ static hipError_t launch_pair_mixed(const Dev& d, hipStream_t s, int accumulate)
     const int64_t units = blocks_for(d.nslots * tpa, BS);
 #define DMDX_PAIR_ARGS                                                                       \
     d.sys, d.st, d.slot_sys, d.nslots, d.order, d.nn, d.nbr, d.lpos, d.typ_s, d.lj_c12, d.lj_c6, \
-        d.fx, d.fy, d.fz, d.p_lj, d.p_coul, accumulate, units
+        d.fx, d.fy, d.fz, d.p_lj, d.p_coul, accumulate, d.pair_rows != 0 ? 1 : 0, units
     const unsigned g = grid_for(units);
 #define DMDX_PAIR_SHAPE(T_, W_) \
     case T_ * 100 + W_: k_pair_mixed<T_, W_, COUL><<<g, BS, 0, s>>>(DMDX_PAIR_ARGS); break
@@ -517,7 +515,15 @@ hipError_t launch_pair(const Dev& d, hipStream_t s, int accumulate)
             const hipError_t e = launch_pair_mixed<true>(d, s, accumulate);
             if (e != hipSuccess) return e;
         }
-        if (d.any_plain_pairs) return launch_pair_mixed<false>(d, s, accumulate);
+        if (d.any_plain_pairs) {
+            // LJ-only systems with a list on row tiles; those without a list
+            // (no pair terms: zero forces) on the per-atom kernel
+            if (d.pair_rows != 0) {
+                const hipError_t e = launch_pair_rows(d, s, accumulate);
+                if (e != hipSuccess || !d.any_plain_nolist) return e;
+            }
+            return launch_pair_mixed<false>(d, s, accumulate);
+        }
     }
     return hipGetLastError();
 }

@@ -66,9 +66,11 @@ def _setup(lib):
 
 KERNEL_GROUPS = ("pair", "list", "bonded", "pme", "gather", "integrate")
 # autotuner candidates in the order of dmdx_engine_autotune's times_ms:
-# mixed pair kernel (lanes per atom x list entries per lane pass), list build
-# (per atom with 8/16/32 lanes, per cell with 2/4/8 waves per cell)
+# per-atom mixed pair kernel (lanes per atom x list entries per lane pass),
+# row tiles for LJ-only systems (entries per block), list build (per atom
+# with 8/16/32 lanes, per cell with 2/4/8 waves per cell)
 PAIR_SHAPES = ("2x8", "4x8", "8x8", "4x4", "8x4", "16x4", "8x2", "16x2")
+ROW_SHAPES = ("rows4", "rows8")
 BUILD_SHAPES = ("atom8", "atom16", "atom32", "cell2", "cell4", "cell8")
 
 
@@ -176,10 +178,15 @@ class Engine:
 
     def set_tuning(self, pair_lanes: int | None = None, build_lanes: int | None = None,
                    pair_width: int | None = None, build_cells: bool | None = None,
-                   build_tasks: int | None = None, tune_skin: bool | None = None):
+                   build_tasks: int | None = None, tune_skin: bool | None = None,
+                   pair_rows: int | None = None):
         """Launch shapes of the mixed kernels (dmdx_engine_set_tuning);
         build_cells: list built per cell (default) or per atom; build_tasks:
-        waves per cell of the per-cell build (1, 2, 4, 8)."""
+        waves per cell of the per-cell build (1, 2, 4, 8); pair_rows: LJ-only
+        systems on row tiles staged in LDS with 4 or 8 list entries per lane
+        block, or 0 (default) for the per-atom kernel."""
+        if pair_rows is not None:
+            _abi.check(self.lib.dmdx_engine_set_tuning(self.h, 6, int(pair_rows)))
         if build_cells is not None:
             _abi.check(self.lib.dmdx_engine_set_tuning(self.h, 3, int(bool(build_cells))))
         if build_tasks is not None:
@@ -200,9 +207,11 @@ class Engine:
         apply the fastest (dmdx_engine_autotune); on an engine that has not
         stepped, first the list skins of displacement-criterion systems.
         `cache`: text file of earlier results.  "build" names the chosen list
-        build, "skins" the widened skins."""
+        build, "pair_rows" the row-tile setting of LJ-only systems (0: the
+        per-atom kernel), "skins" the widened skins."""
         p, w, b = C.c_int32(), C.c_int32(), C.c_int32()
-        t = (C.c_double * (len(PAIR_SHAPES) + len(BUILD_SHAPES)))()
+        npr = len(PAIR_SHAPES) + len(ROW_SHAPES)
+        t = (C.c_double * (npr + len(BUILD_SHAPES)))()
         _abi.check(self.lib.dmdx_engine_autotune(
             self.h, int(steps), cache.encode() if cache else None, C.byref(p), C.byref(w),
             C.byref(b), t))
@@ -219,8 +228,10 @@ class Engine:
                 "build": f"cell{tasks}" if cells else f"atom{b.value}",
                 "skins": [{"skin": a, "tuned": t, "systems": n}
                           for (a, t), n in sorted(widened.items())],
+                "pair_rows": self.get_tuning(6),
                 "pair_ms": {k: t[i] for i, k in enumerate(PAIR_SHAPES)},
-                "build_ms": {k: t[len(PAIR_SHAPES) + i] for i, k in enumerate(BUILD_SHAPES)}}
+                "rows_ms": {k: t[len(PAIR_SHAPES) + i] for i, k in enumerate(ROW_SHAPES)},
+                "build_ms": {k: t[npr + i] for i, k in enumerate(BUILD_SHAPES)}}
 
     def list_skin(self, sys: int = 0) -> float:
         """The list skin system `sys` is built with (dmdx_engine_get_list_skin)."""

@@ -25,6 +25,13 @@ def test_autotune_choice_cache_and_state(tmp_path):
     pm = r["pair_ms"]
     assert all(v > 0 for v in pm.values())
     assert pm[f"{r['pair_lanes']}x{r['pair_width']}"] == min(pm.values())
+    assert r["pair_rows"] in (0, 4, 8)
+    rm = r["rows_ms"]
+    assert all(v > 0 for v in rm.values())
+    if r["pair_rows"]:
+        assert rm[f"rows{r['pair_rows']}"] == min(min(rm.values()), min(pm.values()))
+    else:
+        assert min(pm.values()) <= min(rm.values())
     bm = r["build_ms"]
     assert bm[r["build"]] == min(bm.values())
     assert bm[f"atom{r['build_lanes']}"] == min(bm[k] for k in bm if k.startswith("atom"))
@@ -36,8 +43,9 @@ def test_autotune_choice_cache_and_state(tmp_path):
     e2 = Engine(specs)
     e2.set_tuning(tune_skin=False)
     r2 = e2.autotune(steps=3, cache=cache)
-    assert (r2["pair_lanes"], r2["pair_width"], r2["build_lanes"], r2["build"]) == \
-        (r["pair_lanes"], r["pair_width"], r["build_lanes"], r["build"])
+    assert (r2["pair_lanes"], r2["pair_width"], r2["build_lanes"], r2["build"],
+            r2["pair_rows"]) == \
+        (r["pair_lanes"], r["pair_width"], r["build_lanes"], r["build"], r["pair_rows"])
     assert all(v == -1.0 for v in r2["pair_ms"].values())
     # forces of the tuned engine against the FP64 path
     e.compute()
