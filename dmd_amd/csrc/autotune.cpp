@@ -7,7 +7,8 @@
 // a row); LJ-only systems can instead run on row tiles staged in LDS with 4
 // or 8 entries per lane block, timed after the best per-atom shape (which
 // charged systems keep); and the list build gives each atom 8, 16 or 32 lanes or shares each
-// cell's candidates among 2, 4 or 8 waves.  Which is fastest depends on the systems (how
+// cell's candidates among 2, 4 or 8 waves or stages them in LDS for groups
+// of 8 or 16 lanes per atom.  Which is fastest depends on the systems (how
 // many neighbours an atom has, how many atoms a batch holds, how many rebuild
 // at once), so the choice is measured: a scratch copy of the engine's batch
 // is stepped with each candidate and the times of the two kernel groups are
@@ -86,10 +87,10 @@ uint64_t signature(const dmdx_engine* e)
 
 // A cached line: key, pair lanes, pair width, per-atom build lanes, then
 // optionally the per-cell build switch and its waves per cell (a line
-// without them stands for the per-atom build) and the row-tile setting (0
-// when absent).
+// without them stands for the per-atom build), the row-tile setting and
+// the staged per-cell build's lanes per atom (0 when absent).
 bool cache_lookup(const char* path, uint64_t key, int* pair, int* width, int* build, int* cells,
-                  int* tasks, int* rows)
+                  int* tasks, int* rows, int* stage)
 {
     if (!path || !*path) return false;
     std::ifstream f(path);
@@ -103,12 +104,14 @@ bool cache_lookup(const char* path, uint64_t key, int* pair, int* width, int* bu
             *pair = p;
             *width = w;
             *build = b;
-            int c = 0, t = 4, r = 0;
+            int c = 0, t = 4, r = 0, g = 0;
             if (!(ss >> c >> t) || (c != 0 && c != 1)) c = 0, t = 4;
             else if (!(ss >> r) || (r != 0 && r != 4 && r != 8)) r = 0;
+            else if (!(ss >> g) || (g != 0 && g != 8 && g != 16)) g = 0;
             *cells = c;
             *tasks = t;
             *rows = r;
+            *stage = g;
             return true;
         }
     }
@@ -118,13 +121,14 @@ bool cache_lookup(const char* path, uint64_t key, int* pair, int* width, int* bu
 // pair kernel candidates: (lanes per atom, entries per lane per pass) of the
 // per-atom kernel, then row tiles with 4 or 8 entries per block; list build
 // candidates: per atom with 8, 16 or 32 lanes, then per cell with 2, 4 or 8
-// waves per cell
-constexpr int NPAIR = 8, NROWS = 2, NBUILD = 6, NBUILD_ATOM = 3;
+// waves per cell, then per cell staged in LDS with 8 or 16 lanes per atom
+constexpr int NPAIR = 8, NROWS = 2, NBUILD = 8, NBUILD_ATOM = 3;
 constexpr int PAIR_LANES[NPAIR] = {2, 4, 8, 4, 8, 16, 8, 16};
 constexpr int PAIR_WIDTH[NPAIR] = {8, 8, 8, 4, 4, 4, 2, 2};
 constexpr int ROWS_BLOCK[NROWS] = {4, 8};
-constexpr int BUILD_LANES[NBUILD] = {8, 16, 32, 0, 0, 0};
-constexpr int BUILD_TASKS[NBUILD] = {0, 0, 0, 2, 4, 8};
+constexpr int BUILD_LANES[NBUILD] = {8, 16, 32, 0, 0, 0, 0, 0};
+constexpr int BUILD_TASKS[NBUILD] = {0, 0, 0, 2, 4, 8, 4, 4};
+constexpr int BUILD_STAGE[NBUILD] = {0, 0, 0, 0, 0, 0, 8, 16};
 constexpr int T_ROWS = NPAIR, T_BUILD = NPAIR + NROWS;  // times_ms layout
 static_assert(NPAIR + NROWS + NBUILD == DMDX_TUNE_TIMES, "times_ms layout");
 
@@ -136,13 +140,13 @@ bool pair_candidate(int p, int w)
 }
 
 void cache_store(const char* path, uint64_t key, int pair, int width, int build, int cells,
-                 int tasks, int rows, const double* t)
+                 int tasks, int rows, int stage, const double* t)
 {
     if (!path || !*path) return;
     std::ostringstream o;
-    char buf[64];
-    std::snprintf(buf, sizeof(buf), "%016llx %d %d %d %d %d %d  # pair ms",
-                  (unsigned long long)key, pair, width, build, cells, tasks, rows);
+    char buf[80];
+    std::snprintf(buf, sizeof(buf), "%016llx %d %d %d %d %d %d %d  # pair ms",
+                  (unsigned long long)key, pair, width, build, cells, tasks, rows, stage);
     o << buf;
     for (int c = 0; c < NPAIR; ++c) {
         std::snprintf(buf, sizeof(buf), " %dx%d %.4f", PAIR_LANES[c], PAIR_WIDTH[c], t[c]);
@@ -156,6 +160,8 @@ void cache_store(const char* path, uint64_t key, int pair, int width, int build,
     for (int c = 0; c < NBUILD; ++c) {
         if (BUILD_LANES[c])
             std::snprintf(buf, sizeof(buf), " atom%d %.4f", BUILD_LANES[c], t[T_BUILD + c]);
+        else if (BUILD_STAGE[c])
+            std::snprintf(buf, sizeof(buf), " stage%d %.4f", BUILD_STAGE[c], t[T_BUILD + c]);
         else
             std::snprintf(buf, sizeof(buf), " cell%d %.4f", BUILD_TASKS[c], t[T_BUILD + c]);
         o << buf;
@@ -228,6 +234,7 @@ int time_skins_build(dmdx_engine* e, const std::vector<dmdx_system_desc>& ds, bo
     b.d.build_tpa = e->b.d.build_tpa;
     b.d.build_cells = cells;
     b.d.build_tasks = e->b.d.build_tasks;
+    b.d.build_stage = e->b.d.build_stage;
     using clk = std::chrono::steady_clock;
     rc = b.step(1);
     if (!rc) rc = b.sync();
@@ -322,6 +329,7 @@ int apply_skins(dmdx_engine* e, const std::vector<SkinGroup>& g, const std::vect
     e->b.d.build_tpa = keep.build_tpa;
     e->b.d.build_cells = keep.build_cells;
     e->b.d.build_tasks = keep.build_tasks;
+    e->b.d.build_stage = keep.build_stage;
     return DMDX_OK;
 }
 
@@ -369,8 +377,8 @@ int tune_shapes(dmdx_engine* e, int32_t steps, const char* cache_path, double* t
     const int* pl = PAIR_LANES;
     const int* pw = PAIR_WIDTH;
     const uint64_t key = signature(e);
-    int cp = 0, cw = 0, cb = 0, cc = 0, ct = 4, cr = 0;
-    if (cache_lookup(cache_path, key, &cp, &cw, &cb, &cc, &ct, &cr) && pair_candidate(cp, cw) &&
+    int cp = 0, cw = 0, cb = 0, cc = 0, ct = 4, cr = 0, cg = 0;
+    if (cache_lookup(cache_path, key, &cp, &cw, &cb, &cc, &ct, &cr, &cg) && pair_candidate(cp, cw) &&
         (cb == 8 || cb == 16 || cb == 32) && (ct == 1 || ct == 2 || ct == 4 || ct == 8)) {
         e->b.d.pair_tpa = cp;
         e->b.d.pair_width = cw;
@@ -378,6 +386,7 @@ int tune_shapes(dmdx_engine* e, int32_t steps, const char* cache_path, double* t
         e->b.d.build_tpa = cb;
         e->b.d.build_cells = cc != 0;
         e->b.d.build_tasks = ct;
+        e->b.d.build_stage = cg;
         e->b.invalidate_graph();
         return DMDX_OK;
     }
@@ -436,6 +445,7 @@ int tune_shapes(dmdx_engine* e, int32_t steps, const char* cache_path, double* t
         scratch.d.build_cells = BUILD_LANES[c] == 0;
         if (BUILD_LANES[c]) scratch.d.build_tpa = BUILD_LANES[c];
         else scratch.d.build_tasks = BUILD_TASKS[c];
+        scratch.d.build_stage = BUILD_STAGE[c];
         rc = scratch.step(1);
         if (!rc) rc = scratch.kernel_time(dmdx::KG_LIST, &ms1, &n1);
         if (rc) return rc;
@@ -457,9 +467,10 @@ int tune_shapes(dmdx_engine* e, int32_t steps, const char* cache_path, double* t
     e->b.d.build_tpa = BUILD_LANES[ba];
     e->b.d.build_cells = BUILD_LANES[bb] == 0;
     if (e->b.d.build_cells) e->b.d.build_tasks = BUILD_TASKS[bb];
+    e->b.d.build_stage = BUILD_STAGE[bb];
     e->b.invalidate_graph();
     cache_store(cache_path, key, pl[bp], pw[bp], BUILD_LANES[ba], e->b.d.build_cells ? 1 : 0,
-                e->b.d.build_tasks, e->b.d.pair_rows, t);
+                e->b.d.build_tasks, e->b.d.pair_rows, e->b.d.build_stage, t);
     return DMDX_OK;
 }
 

@@ -249,6 +249,10 @@ struct Dev {
     int build_tpa = 32;        // mixed list build: lanes per atom (8, 16, 32)
     bool build_cells = true;   // mixed list build: per cell where no minimum image is needed
     int build_tasks = 4;       // mixed per-cell build: waves per cell (1, 2, 4, 8)
+    // mixed per-cell build with each cell's candidates staged in LDS
+    // (k_nbr_build_staged) with 8 or 16 lanes per atom; 0 (default) =
+    // k_nbr_build_cells
+    int build_stage = 0;
     int pair_width = 8;        // mixed pair kernel: entries per lane per pass (2, 4, 8)
     // mixed LJ pair forces: 0 = per-atom kernel (k_pair_mixed), 4 or 8 =
     // row-tile kernel (k_pair_rows) with that many list entries per block

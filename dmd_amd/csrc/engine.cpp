@@ -1619,6 +1619,10 @@ int dmdx_engine_set_tuning(dmdx_engine* e, int32_t key, int32_t value)
         if (value != 0 && value != 4 && value != 8)
             return dmdx::fail(DMDX_E_PARM, "row-tile pair kernel is 0 (off), 4 or 8 entries per block");
         e->b.d.pair_rows = value;
+    } else if (key == 7) {
+        if (value != 0 && value != 8 && value != 16)
+            return dmdx::fail(DMDX_E_PARM, "staged per-cell build is 0 (off), 8 or 16 lanes per atom");
+        e->b.d.build_stage = value;
     } else {
         return dmdx::fail(DMDX_E_INVALID, "unknown tuning key");
     }
@@ -1638,6 +1642,7 @@ int dmdx_engine_get_tuning(dmdx_engine* e, int32_t key, int32_t* value)
     case 4: *value = d.build_tasks; break;
     case 5: *value = e->tune_skin ? 1 : 0; break;
     case 6: *value = d.pair_rows; break;
+    case 7: *value = d.build_stage; break;
     default: return dmdx::fail(DMDX_E_INVALID, "unknown tuning key");
     }
     return DMDX_OK;

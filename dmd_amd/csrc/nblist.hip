@@ -863,18 +863,22 @@ __device__ __forceinline__ void k_nbr_build_mixed_unit(int64_t t,
         locate(lane);
         int slot_n = lane < ncand ? tb[jc] + (lane - tp[jc]) : 0;
         float4 nxt = lane < ncand ? pos4f[S.atom_off + slot_n] : make_float4(0.f, 0.f, 0.f, 0.f);
+        // the candidate's atom, for the exclusion test
+        int jid_n = check_ex && lane < ncand ? cell_atoms[S.atom_off + slot_n] : -1;
         for (int k0 = 0; k0 < ncand; k0 += TPA) {
             const int kc = k0 + lane;
             const float4 lj = nxt;
             const int slot = slot_n;
+            const int jid = jid_n;
             const float4 sh = tsh[kc < ncand ? jc : 0];
             const uint32_t offs = kc < ncand ? __float_as_uint(sh.w) : 0u;
-            // the next pass's candidate is located and its position loaded
-            // before this one is tested
+            // the next pass's candidate is located and its position (and
+            // atom) loaded before this one is tested
             if (kc + TPA < ncand) {
                 locate(kc + TPA);
                 slot_n = tb[jc] + (kc + TPA - tp[jc]);
                 nxt = pos4f[S.atom_off + slot_n];
+                if (check_ex) jid_n = cell_atoms[S.atom_off + slot_n];
             }
             bool hit = false;
             if (kc < ncand && S.atom_off + slot != t) {
@@ -897,7 +901,7 @@ __device__ __forceinline__ void k_nbr_build_mixed_unit(int64_t t,
                     hit = ddx * ddx + ddy * ddy + ddz * ddz < S.rlist2;
                 }
                 if (hit && check_ex) {
-                    const int j = cell_atoms[S.atom_off + slot];
+                    const int j = jid;
                     if (exn <= EXR) {
                         bool ex = false;
 #pragma unroll
@@ -1164,6 +1168,226 @@ __global__ void __launch_bounds__(64 * CB_WAVES) k_nbr_build_cells(
     }
 }
 
+// Mixed path, systems built per cell (mixed_builds_per_cell), candidates
+// staged in LDS: one workgroup per cell of a rebuilding system.  The cell's
+// candidate stream -- the neighbour cells in scan order (z, then y, then x
+// offset, each ascending), each cell's atoms in cell order, as in
+// k_nbr_build_mixed -- is copied into LDS once, SB_CAP candidates at a time,
+// each shifted into the frame of the workgroup's cell (l_j + offset * w)
+// with its list entry (slot | offset code) and atom id beside it.  Groups of
+// TPA lanes then scan it for the cell's atoms straight from LDS: no table
+// walk and no global load per candidate.  Decisions and order are those of
+// k_nbr_build_mixed (FP32 test outside a 1e-5 nm band around the list
+// radius, the reference's FP64 expression inside it), so the rows are the
+// same, entry for entry.
+constexpr int SB_THREADS = 256;
+constexpr int SB_CAP = 1024;    // staged candidates per pass over the stream
+constexpr int SB_ATOMS = 256;   // row lengths kept in LDS (atoms of one cell)
+constexpr int SB_EXR = 6;       // excluded partners held in registers
+
+template <int TPA>
+__global__ void __launch_bounds__(SB_THREADS) k_nbr_build_staged(
+    const SysDev* __restrict__ sys, SysState* __restrict__ st, const int* __restrict__ ccount,
+    const int* __restrict__ cstart, const int* __restrict__ cell_atoms,
+    const int* __restrict__ ex_start, const int* __restrict__ ex_list, int* __restrict__ order,
+    int* __restrict__ slot_of, int* __restrict__ nn, int* __restrict__ nbr,
+    const double4* __restrict__ pos4s, const float4* __restrict__ pos4f, RbList R)
+{
+    __shared__ float4 stage[SB_CAP];   // shifted position, list entry (w bits)
+    __shared__ int sid[SB_CAP];        // atom id (exclusions)
+    __shared__ int tb[NB_CELLS];       // cell-order start of each neighbour cell
+    __shared__ int tp[NB_CELLS + 1];   // candidates before it in the stream
+    __shared__ float4 tsh[NB_CELLS];   // offset * w, and the entry's code bits
+    __shared__ int kk[SB_ATOMS];       // row length so far of each atom of the cell
+    __shared__ int wtot[2];
+    __shared__ RbStage rs;
+    R = rb_stage(R, true, rs);
+    constexpr int G = SB_THREADS / TPA;  // atoms per round
+    const int tid = threadIdx.x, lane = tid % TPA, grp = tid / TPA;
+    const int gshift = (tid & 63) / TPA * TPA;  // first lane of my group in the wave
+    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const int nrb = *R.n;
+    const int64_t U = R.cpre[nrb];
+    for (int64_t u = blockIdx.x; u < U; u += gridDim.x) {
+        const int kq = rb_find(R.cpre, nrb, u);  // uniform over the workgroup
+        const int s = R.sys[kq];
+        const SysDev& S = sys[s];
+        if (S.precision == DMDX_FP64 || !mixed_builds_per_cell(S)) continue;
+        const int lc = (int)(u - R.cpre[kq]);
+        if (lc == 0)  // the system's padding slots hold no atom
+            for (int p = S.n + tid; p < S.n_pad; p += SB_THREADS) {
+                nn[S.atom_off + p] = 0;
+                order[S.atom_off + p] = -1;
+            }
+        const int nx = cells_in_use(S, 0), ny = cells_in_use(S, 1), nz = cells_in_use(S, 2);
+        if (lc >= nx * ny * nz) continue;
+        const int c = S.cell_off + lc;
+        const int m = ccount[c];
+        if (m == 0) continue;
+        const int cx = lc % nx, cy = (lc / nx) % ny, cz = lc / (nx * ny);
+        const float wx = (float)(S.L[0] / nx), wy = (float)(S.L[1] / ny),
+                    wz = (float)(S.L[2] / nz);
+        const int reach = S.cdiv, full = 2 * reach + 1;
+        const int ncb = full * full * full;  // 27 or 125
+        // neighbour-cell table (threads 0-127), running candidate counts
+        // over the two waves
+        int cnt = 0;
+        if (tid < 128) {
+            if (tid < ncb) {
+                const int ox = tid % full - reach, oy = (tid / full) % full - reach,
+                          oz = tid / (full * full) - reach;
+                int xx = cx + ox, yy = cy + oy, zz = cz + oz;
+                xx = xx < 0 ? xx + nx : (xx >= nx ? xx - nx : xx);
+                yy = yy < 0 ? yy + ny : (yy >= ny ? yy - ny : yy);
+                zz = zz < 0 ? zz + nz : (zz >= nz ? zz - nz : zz);
+                const int cc = S.cell_off + (zz * ny + yy) * nx + xx;
+                tb[tid] = cstart[cc];
+                cnt = ccount[cc];
+                const uint32_t code = nbr_code(ox, oy, oz) << NBR_SLOT_BITS;
+                tsh[tid] = make_float4((float)ox * wx, (float)oy * wy, (float)oz * wz,
+                                       __uint_as_float(code));
+            }
+            int incl = cnt;
+            for (int o = 1; o < 64; o <<= 1) {
+                const int v = __shfl_up(incl, o, 64);
+                if ((tid & 63) >= o) incl += v;
+            }
+            if ((tid & 63) == 63) wtot[tid >> 6] = incl;
+            cnt = incl - cnt;  // exclusive, within the wave
+        }
+        __syncthreads();
+        if (tid < ncb) tp[tid] = cnt + (tid >= 64 ? wtot[0] : 0);
+        if (tid == 0) tp[ncb] = wtot[0] + (ncb > 64 ? wtot[1] : 0);
+        for (int a = tid; a < m; a += SB_THREADS)
+            if (a < SB_ATOMS) kk[a] = 0;
+        __syncthreads();
+        const int ncand = tp[ncb];
+        const int64_t a0 = S.atom_off;
+        const int64_t t0 = a0 + cstart[c];  // the cell's first cell-order slot
+        const int M = S.max_nbr;
+        const float lo2 = (float)((S.rlist - 1e-5) * (S.rlist - 1e-5));
+        const float hi2 = (float)((S.rlist + 1e-5) * (S.rlist + 1e-5));
+        // the system's atoms have list exclusions (CSR over atom slots)
+        const bool any_ex = ex_start[a0 + S.n_pad] != ex_start[a0];
+        for (int k0 = 0; k0 < ncand; k0 += SB_CAP) {
+            const int k1 = min(ncand, k0 + SB_CAP);
+            {
+                // stage candidates [k0, k1), four loads in flight per thread
+                constexpr int UNR = 4;
+                int jc = 0;  // neighbour cell of candidate k (k only grows)
+                for (int kb = k0 + tid; kb < k1; kb += UNR * SB_THREADS) {
+                    float4 p[UNR];
+                    int slot[UNR], id[UNR], cell[UNR];
+#pragma unroll
+                    for (int q = 0; q < UNR; ++q) {
+                        const int k = kb + q * SB_THREADS;
+                        if (k < k1) {
+                            while (jc + 1 < ncb && tp[jc + 1] <= k) ++jc;
+                            cell[q] = jc;
+                            slot[q] = tb[jc] + (k - tp[jc]);
+                            p[q] = pos4f[a0 + slot[q]];
+                            id[q] = any_ex ? cell_atoms[a0 + slot[q]] : -1;
+                        }
+                    }
+#pragma unroll
+                    for (int q = 0; q < UNR; ++q) {
+                        const int k = kb + q * SB_THREADS;
+                        if (k < k1) {
+                            const float4 sh = tsh[cell[q]];
+                            stage[k - k0] = make_float4(
+                                p[q].x + sh.x, p[q].y + sh.y, p[q].z + sh.z,
+                                __uint_as_float((uint32_t)slot[q] | __float_as_uint(sh.w)));
+                            sid[k - k0] = id[q];
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+            const int nk = k1 - k0;
+            for (int ai = grp; ai < m; ai += G) {
+                const int64_t t = t0 + ai;
+                const int lt = (int)(t - a0);
+                const float4 li = pos4f[t];
+                const int i = cell_atoms[t];
+                const int exb = ex_start[i], exn = ex_start[i + 1] - exb;
+                int exr[SB_EXR];
+#pragma unroll
+                for (int q = 0; q < SB_EXR; ++q) exr[q] = q < exn ? ex_list[exb + q] : -1;
+                int* row = nbr + S.nbr_off + (int64_t)lt * M;
+                int k = ai < SB_ATOMS ? kk[ai] : 0;
+                if (ai >= SB_ATOMS && k0 > 0) {
+                    // an atom past the LDS table (a cell of more than SB_ATOMS
+                    // atoms): its row so far is its own count of entries
+                    k = nn[t];
+                }
+                for (int kc0 = 0; kc0 < nk; kc0 += TPA) {
+                    const int kc = kc0 + lane;
+                    bool hit = false;
+                    uint32_t e = 0u;
+                    if (kc < nk) {
+                        const float4 pj = stage[kc];
+                        e = __float_as_uint(pj.w);
+                        const int slot = (int)(e & NBR_SLOT_MASK);
+                        const float dx = pj.x - li.x, dy = pj.y - li.y, dz = pj.z - li.z;
+                        const float r2f = dx * dx + dy * dy + dz * dz;
+                        if (slot != lt) {
+                            hit = r2f < lo2;
+                            if (!hit && r2f < hi2) {
+                                // the reference's expression (src/core/cell.cpp:26-38,
+                                // src/force/lennard_jones.cpp:29-36)
+                                const double4 pi4 = pos4s[t];
+                                const double4 pq = pos4s[a0 + slot];
+                                const double ddx = min_image_dmd(pq.x - pi4.x, S.L[0], S.iminv[0]);
+                                const double ddy = min_image_dmd(pq.y - pi4.y, S.L[1], S.iminv[1]);
+                                const double ddz = min_image_dmd(pq.z - pi4.z, S.L[2], S.iminv[2]);
+                                hit = ddx * ddx + ddy * ddy + ddz * ddz < S.rlist2;
+                            }
+                            if (hit && exn > 0) {
+                                const int j = sid[kc];
+                                if (exn <= SB_EXR) {
+                                    bool x = false;
+#pragma unroll
+                                    for (int q = 0; q < SB_EXR; ++q) x |= exr[q] == j;
+                                    hit = !x;
+                                } else {
+                                    hit = !is_excluded(ex_start, ex_list, i, j);
+                                }
+                            }
+                        }
+                    }
+                    const uint64_t bal = __ballot(hit);
+                    const uint64_t mine = (bal >> gshift) & (TPA == 64 ? ~0ull : ((1ull << TPA) - 1));
+                    const int pos = k + __popcll(mine & below);
+                    if (hit && pos < M) row[pos] = (int)e;
+                    k += __popcll(mine);
+                }
+                if (lane == 0) {
+                    if (ai < SB_ATOMS) kk[ai] = k;
+                    else nn[t] = k;  // carried to the next pass over the stream
+                }
+            }
+            __syncthreads();
+        }
+        // rows done: length, overflow flag, padding, atom maps
+        for (int ai = tid; ai < m; ai += SB_THREADS) {
+            const int64_t t = t0 + ai;
+            const int lt = (int)(t - a0);
+            const int i = cell_atoms[t];
+            order[t] = i;
+            slot_of[i] = lt;
+            int k = ai < SB_ATOMS ? kk[ai] : nn[t];
+            if (k > M) {
+                atomicOr(&st[s].flags, FLAG_NBR_OVERFLOW);
+                k = M;
+            }
+            nn[t] = k;
+            int* row = nbr + S.nbr_off + (int64_t)lt * M;
+            for (int p = k; p < ((k + 7) & ~7); ++p) row[p] = (int)NBR_PAD_ENTRY;
+        }
+        __syncthreads();  // the next cell rewrites the shared tables
+    }
+}
+
 // one 64-thread workgroup per atom slot: bitonic sort of its row in LDS
 constexpr int SORT_CAP = 2048;
 
@@ -1322,7 +1546,17 @@ hipError_t launch_build_lists(const Dev& d, hipStream_t s)
             d.sys, d.st, d.slot_sys, d.nslots, d.x, d.y, d.z, d.acell, d.ccount, d.cstart,
             d.cell_atoms, d.ex_start, d.ex_list, d.order, d.nn, d.nbr, d.pos4s, nullptr, (int64_t)(blocks_for(d.nslots, BS)));
     else {
-        if (d.build_cells && d.ncells > 0)
+        if (d.build_cells && d.ncells > 0 && d.build_stage != 0) {
+            const unsigned g = (unsigned)std::min<int64_t>(2048, std::max(1, d.ncells));
+#define DMDX_STAGED_ARGS                                                                  \
+    d.sys, d.st, d.ccount, d.cstart, d.cell_atoms, d.ex_start, d.ex_list, d.order, d.slot_of, \
+        d.nn, d.nbr, d.pos4s, d.pos4f, R
+            if (d.build_stage == 16)
+                k_nbr_build_staged<16><<<g, SB_THREADS, 0, s>>>(DMDX_STAGED_ARGS);
+            else
+                k_nbr_build_staged<8><<<g, SB_THREADS, 0, s>>>(DMDX_STAGED_ARGS);
+#undef DMDX_STAGED_ARGS
+        } else if (d.build_cells && d.ncells > 0)
             k_nbr_build_cells<<<grid_for(blocks_for((int64_t)d.ncells * d.build_tasks, CB_WAVES)),
                                 64 * CB_WAVES, 0, s>>>(
                 d.sys, d.st, d.ccount, d.cstart, d.cell_atoms, d.ex_start, d.ex_list, d.order,

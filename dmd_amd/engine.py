@@ -68,10 +68,11 @@ KERNEL_GROUPS = ("pair", "list", "bonded", "pme", "gather", "integrate")
 # autotuner candidates in the order of dmdx_engine_autotune's times_ms:
 # per-atom mixed pair kernel (lanes per atom x list entries per lane pass),
 # row tiles for LJ-only systems (entries per block), list build (per atom
-# with 8/16/32 lanes, per cell with 2/4/8 waves per cell)
+# with 8/16/32 lanes, per cell with 2/4/8 waves per cell, per cell staged in
+# LDS with 8/16 lanes per atom)
 PAIR_SHAPES = ("2x8", "4x8", "8x8", "4x4", "8x4", "16x4", "8x2", "16x2")
 ROW_SHAPES = ("rows4", "rows8")
-BUILD_SHAPES = ("atom8", "atom16", "atom32", "cell2", "cell4", "cell8")
+BUILD_SHAPES = ("atom8", "atom16", "atom32", "cell2", "cell4", "cell8", "stage8", "stage16")
 
 
 def copy_bandwidth(nbytes: int = 1 << 28, repeats: int = 20) -> float:
@@ -179,12 +180,17 @@ class Engine:
     def set_tuning(self, pair_lanes: int | None = None, build_lanes: int | None = None,
                    pair_width: int | None = None, build_cells: bool | None = None,
                    build_tasks: int | None = None, tune_skin: bool | None = None,
-                   pair_rows: int | None = None):
+                   pair_rows: int | None = None, build_stage: int | None = None):
         """Launch shapes of the mixed kernels (dmdx_engine_set_tuning);
         build_cells: list built per cell (default) or per atom; build_tasks:
         waves per cell of the per-cell build (1, 2, 4, 8); pair_rows: LJ-only
         systems on row tiles staged in LDS with 4 or 8 list entries per lane
-        block, or 0 (default) for the per-atom kernel."""
+        block, or 0 (default) for the per-atom kernel; build_stage: the
+        per-cell build with each cell's candidates staged in LDS for 8 or 16
+        lanes per atom, or 0 (default) for candidates shared in registers by
+        build_tasks waves."""
+        if build_stage is not None:
+            _abi.check(self.lib.dmdx_engine_set_tuning(self.h, 7, int(build_stage)))
         if pair_rows is not None:
             _abi.check(self.lib.dmdx_engine_set_tuning(self.h, 6, int(pair_rows)))
         if build_cells is not None:
@@ -215,7 +221,7 @@ class Engine:
         _abi.check(self.lib.dmdx_engine_autotune(
             self.h, int(steps), cache.encode() if cache else None, C.byref(p), C.byref(w),
             C.byref(b), t))
-        cells, tasks = self.get_tuning(3), self.get_tuning(4)
+        cells, tasks, stage = self.get_tuning(3), self.get_tuning(4), self.get_tuning(7)
         # skins the autotuner widened: (original skin, chosen skin) -> systems
         widened = {}
         for k, s in enumerate(self.specs):
@@ -225,7 +231,8 @@ class Engine:
                 widened[key] = widened.get(key, 0) + 1
         return {"pair_lanes": p.value, "pair_width": w.value, "build_lanes": b.value,
                 "build_cells": bool(cells), "build_tasks": tasks,
-                "build": f"cell{tasks}" if cells else f"atom{b.value}",
+                "build": (f"stage{stage}" if stage else f"cell{tasks}") if cells
+                else f"atom{b.value}",
                 "skins": [{"skin": a, "tuned": t, "systems": n}
                           for (a, t), n in sorted(widened.items())],
                 "pair_rows": self.get_tuning(6),

@@ -406,8 +406,11 @@ int dmdx_engine_kernel_time(dmdx_engine *e, int32_t group, double *total_ms, int
  * = whether dmdx_engine_autotune may widen list skins (1, default, or 0),
  * key 6 = LJ-only systems' pair kernel: 4 or 8 = row tiles staged in LDS
  * with that many list entries per lane block, 0 (default) = the per-atom
- * kernel of keys 0 and 2 (charged systems always use the per-atom kernel).
- * The list rows do not depend on keys 1, 3 or 4.  Pair shapes
+ * kernel of keys 0 and 2 (charged systems always use the per-atom kernel),
+ * key 7 = the per-cell build's kernel: 8 or 16 = each cell's candidates
+ * staged in LDS and scanned by that many lanes per atom, 0 (default) =
+ * candidates shared in registers by the waves of key 4.
+ * The list rows do not depend on keys 1, 3, 4 or 7.  Pair shapes
  * (lanes x entries): 2x8, 4x8, 8x8, 4x4, 8x4, 16x4, 32x4, 8x2, 16x2, 32x2;
  * another combination runs as the nearest of these (8 lanes where fewer
  * are not built for the width, at most 8 lanes with 8 entries).  Results do
@@ -429,9 +432,10 @@ int dmdx_engine_get_tuning(dmdx_engine *e, int32_t key, int32_t *value);
  * kernels at 2x8, 4x8, 8x8, 4x4, 8x4, 16x4, 8x2, 16x2 (per-atom kernel for
  * every system), then with LJ-only systems on row tiles of 4 and 8 entries
  * per block (tuning key 6) beside the fastest of those shapes, then per-step
- * ms of the list build at 8, 16, 32 lanes per atom and per cell at 2, 4, 8
- * waves per cell (-1 when not timed); dmdx_engine_get_tuning reads the
- * chosen row tiles and build (keys 6, 1, 3, 4).  FP64 engines have
+ * ms of the list build at 8, 16, 32 lanes per atom, per cell at 2, 4, 8
+ * waves per cell and per cell staged in LDS at 8, 16 lanes per atom (-1
+ * when not timed); dmdx_engine_get_tuning reads the chosen row tiles and
+ * build (keys 6, 1, 3, 4, 7).  FP64 engines have
  * nothing to tune.
  * First, on an engine that has not stepped or been changed since creation
  * (and unless tuning key 5 is 0): the list skin of systems that rebuild by
@@ -442,7 +446,7 @@ int dmdx_engine_get_tuning(dmdx_engine *e, int32_t key, int32_t *value);
  * (dmdx_engine_get_list_skin reads them).
  * Mixed-precision results depend on the shape and the skin only through
  * FP32 summation order (well inside the mixed-precision tolerance). */
-#define DMDX_TUNE_TIMES 16
+#define DMDX_TUNE_TIMES 18
 int dmdx_engine_autotune(dmdx_engine *e, int32_t steps, const char *cache_path,
                          int32_t *pair_lanes, int32_t *pair_width, int32_t *build_lanes,
                          double *times_ms);

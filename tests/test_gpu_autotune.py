@@ -21,7 +21,8 @@ def test_autotune_choice_cache_and_state(tmp_path):
     print(r, flush=True)
     assert f"{r['pair_lanes']}x{r['pair_width']}" in PAIR_SHAPES
     assert r["build_lanes"] in (8, 16, 32)
-    assert r["build"] in ("atom8", "atom16", "atom32", "cell2", "cell4", "cell8")
+    assert r["build"] in ("atom8", "atom16", "atom32", "cell2", "cell4", "cell8", "stage8",
+                          "stage16")
     pm = r["pair_ms"]
     assert all(v > 0 for v in pm.values())
     assert pm[f"{r['pair_lanes']}x{r['pair_width']}"] == min(pm.values())
