@@ -145,32 +145,40 @@ struct SysState {
 
 enum { FLAG_NBR_OVERFLOW = 1, FLAG_NONFINITE = 2 };
 
-// Mixed-path list entries: bits 0-24 the neighbour's cell-order slot
-// relative to its system's first slot, bits 25-31 the neighbour cell's
-// offset from the atom's cell as one dense code, (ox + 2) + 5 (oy + 2) +
-// 25 (oz + 2) for offsets -2..2 (the mixed path's cells are at least
-// rlist / cdiv wide with cdiv 1 or 2, so neighbours lie within +-cdiv
-// cells).  With cell-local coordinates l and cell width w the displacement
-// is d = l_j - l_i + offset * w (the minimum image when the system has
-// >= 2 cdiv + 1 cells per edge).  The pair kernels look the code up in
-// 128-entry LDS tables.
+// Mixed-path list entries: bits 0-22 the neighbour's cell-order slot
+// relative to its system's first slot, bits 23-31 the neighbour cell's
+// offset from the atom's cell along x, y, z (three 3-bit two's complement
+// fields, -2..2; the mixed path's cells are at least rlist / cdiv wide with
+// cdiv 1 or 2, so neighbours lie within +-cdiv cells).  With cell-local
+// coordinates l and cell width w the displacement is d = l_j - l_i + offset
+// * w (the minimum image when the system has >= 2 cdiv + 1 cells per edge).
+// The pair kernels look the 9-bit code up in 512-entry LDS tables.  (A
+// dense 7-bit code (ox + 2) + 5 (oy + 2) + 25 (oz + 2) in a 128-entry
+// table made the per-atom pair kernel 8% slower on the replica mix: its
+// table reads from LDS took longer.)
 // k_bin_system: systems of at most BIN_WG_ATOMS atoms whose allocated cells
 // fit in LDS are binned by one workgroup each
 constexpr int BIN_WG_ATOMS = 8192;
 constexpr int BIN_WG_CELLS = 4096;
 
-constexpr int NBR_SLOT_BITS = 25;
+constexpr int NBR_SLOT_BITS = 23;
 constexpr uint32_t NBR_SLOT_MASK = (1u << NBR_SLOT_BITS) - 1u;
-constexpr int NBR_CODES = 128;  // table size of the offset codes
+constexpr int NBR_CODES = 512;  // table size of the offset codes
 __host__ __device__ __forceinline__ uint32_t nbr_code(int ox, int oy, int oz)
 {
-    return (uint32_t)((ox + 2) + 5 * (oy + 2) + 25 * (oz + 2));
+    return ((uint32_t)ox & 7u) | (((uint32_t)oy & 7u) << 3) | (((uint32_t)oz & 7u) << 6);
 }
-// Mixed path: rows are padded to a multiple of 8 entries with this code,
-// which no neighbour cell has, over slot 0; the pair kernels map it to a
-// position far beyond any cutoff, so a padding entry adds exactly zero.
-constexpr uint32_t NBR_PAD_CODE = 125u;
-constexpr uint32_t NBR_PAD_ENTRY = NBR_PAD_CODE << NBR_SLOT_BITS;
+// Mixed path: rows are padded to a multiple of 8 entries with this code (an
+// x offset of 3 cells, which no neighbour cell has) over the atom's own
+// slot; the pair kernels map it to a position far beyond any cutoff, so a
+// padding entry adds exactly zero.  (Padding over slot 0 instead made every
+// padding gather of a system read one address, and the replica-mix step 2%
+// slower.)
+constexpr uint32_t NBR_PAD_CODE = 3u;
+__host__ __device__ __forceinline__ uint32_t nbr_pad_entry(int own_slot)
+{
+    return (NBR_PAD_CODE << NBR_SLOT_BITS) | (uint32_t)own_slot;
+}
 
 // Charges are spread onto the PME grid as 64-bit fixed-point integers with
 // 2^44 units per e: integer atomic adds are associative, so the grid (and
@@ -321,7 +329,10 @@ struct Dev {
     int ntiles[3] = {0, 0, 0};
     size_t lds_bytes[3] = {0, 0, 0};
     // partial sums (one per wave / PME tile)
+    // pair energies: one partial per pair-kernel wave (per-atom kernel) or
+    // per atom slot (row-tile kernel, p_rows)
     double *p_lj = nullptr, *p_coul = nullptr, *p_bond = nullptr, *p_angle = nullptr;
+    double* p_rows = nullptr;
     double *p_dih = nullptr, *p_imp = nullptr, *p_pme = nullptr, *p_excl = nullptr;
     double *p_ke = nullptr, *p_w = nullptr;
     // what exists in the batch

@@ -900,6 +900,7 @@ int Batch::create(const dmdx_system_desc* descs, int nsys, const BatchOptions& o
     TRY(dev_alloc_zero(&d.xref, slot, owned));
     TRY(dev_alloc_zero(&d.p_disp, nw, owned));
     if (prec != DMDX_FP64) {
+        TRY(dev_alloc_zero(&d.p_rows, slot, owned));
         TRY(dev_alloc_zero(&d.typ_s, slot, owned));
         TRY(dev_alloc_zero(&d.pos4f, slot, owned));
         TRY(dev_alloc_zero(&d.org, slot, owned));

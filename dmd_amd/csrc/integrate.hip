@@ -419,7 +419,11 @@ __device__ double block_range_sum(const double* part, int64_t off, int64_t n, do
 
 struct Parts {
     const double *lj, *coul, *bond, *angle, *dih, *imp, *pme, *excl, *ke;
-    int pair_div;  // atom slots per pair-energy partial (1: one per slot)
+    int pair_div;  // atom slots per pair-energy partial (FP64: 1; mixed: one per wave)
+    // row-tile pair kernel on: LJ-only systems with a list have their LJ
+    // energy as one partial per atom slot here (their Coulomb partials stay
+    // zero in coul)
+    const double* rows;
 };
 
 __device__ __forceinline__ double temperature_of(double ekin, int n)
@@ -463,11 +467,13 @@ __global__ void __launch_bounds__(FIN_THREADS) k_finalize(const SysDev* __restri
     // thread's strided sums, then a fixed tree over each wave and a fixed
     // pairwise tree over the FIN_WAVES waves -- deterministic, one barrier
     // pair energies are per atom slot, the others per wave or workgroup
-    const double* arr[9] = {P.lj, P.coul, P.bond, P.angle, P.dih, P.imp, P.pme, P.excl, P.ke};
+    const bool rows = P.rows && S.has_list && !S.coul_pairs;  // uniform per workgroup
+    const double* arr[9] = {rows ? P.rows : P.lj, P.coul, P.bond, P.angle, P.dih, P.imp,
+                            P.pme, P.excl, P.ke};
     const int64_t po = S.atom_off / P.pair_div, pn = S.n_pad / P.pair_div;
-    const int64_t off[9] = {po, po, S.bw_off, S.gw_off, S.dw_off,
+    const int64_t off[9] = {rows ? S.atom_off : po, po, S.bw_off, S.gw_off, S.dw_off,
                             S.iw_off, S.pw_off, S.xw_off, S.aw_off};
-    const int64_t cnt[9] = {pn, pn, S.bw_n, S.gw_n, S.dw_n,
+    const int64_t cnt[9] = {rows ? S.n_pad : pn, rows ? 0 : pn, S.bw_n, S.gw_n, S.dw_n,
                             S.iw_n, S.pw_n, S.xw_n, with_kinetic ? S.aw_n : 0};
     // the first element of each of the nine sums is loaded before any is
     // added (nine loads in flight), then the strided remainders
@@ -760,8 +766,11 @@ hipError_t launch_kick_ke(const Dev& d, hipStream_t s)
 
 hipError_t launch_finalize(const Dev& d, hipStream_t s, int with_kinetic)
 {
-    // pair energies: one partial per atom slot on both paths
-    Parts P{d.p_lj, d.p_coul, d.p_bond, d.p_angle, d.p_dih, d.p_imp, d.p_pme, d.p_excl, d.p_ke, 1};
+    int tpa = 1, w = 0;
+    if (d.precision != DMDX_FP64) pair_shape(d, tpa, w);
+    Parts P{d.p_lj, d.p_coul, d.p_bond, d.p_angle, d.p_dih, d.p_imp, d.p_pme, d.p_excl, d.p_ke,
+            d.precision == DMDX_FP64 ? 1 : 64 / tpa,
+            d.precision != DMDX_FP64 && d.pair_rows != 0 ? d.p_rows : nullptr};
     k_finalize<<<d.nsys, FIN_THREADS, 0, s>>>(d.sys, d.st, P, with_kinetic);
     return hipGetLastError();
 }

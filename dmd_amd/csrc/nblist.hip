@@ -863,22 +863,18 @@ __device__ __forceinline__ void k_nbr_build_mixed_unit(int64_t t,
         locate(lane);
         int slot_n = lane < ncand ? tb[jc] + (lane - tp[jc]) : 0;
         float4 nxt = lane < ncand ? pos4f[S.atom_off + slot_n] : make_float4(0.f, 0.f, 0.f, 0.f);
-        // the candidate's atom, for the exclusion test
-        int jid_n = check_ex && lane < ncand ? cell_atoms[S.atom_off + slot_n] : -1;
         for (int k0 = 0; k0 < ncand; k0 += TPA) {
             const int kc = k0 + lane;
             const float4 lj = nxt;
             const int slot = slot_n;
-            const int jid = jid_n;
             const float4 sh = tsh[kc < ncand ? jc : 0];
             const uint32_t offs = kc < ncand ? __float_as_uint(sh.w) : 0u;
-            // the next pass's candidate is located and its position (and
-            // atom) loaded before this one is tested
+            // the next pass's candidate is located and its position loaded
+            // before this one is tested
             if (kc + TPA < ncand) {
                 locate(kc + TPA);
                 slot_n = tb[jc] + (kc + TPA - tp[jc]);
                 nxt = pos4f[S.atom_off + slot_n];
-                if (check_ex) jid_n = cell_atoms[S.atom_off + slot_n];
             }
             bool hit = false;
             if (kc < ncand && S.atom_off + slot != t) {
@@ -901,7 +897,7 @@ __device__ __forceinline__ void k_nbr_build_mixed_unit(int64_t t,
                     hit = ddx * ddx + ddy * ddy + ddz * ddz < S.rlist2;
                 }
                 if (hit && check_ex) {
-                    const int j = jid;
+                    const int j = cell_atoms[S.atom_off + slot];
                     if (exn <= EXR) {
                         bool ex = false;
 #pragma unroll
@@ -928,7 +924,7 @@ __device__ __forceinline__ void k_nbr_build_mixed_unit(int64_t t,
     if (lane == 0) nn[t] = k;
     // pad the row to a multiple of 8 entries (M is one), so the pair kernel's
     // 8-entry vector loads past the row end read entries that add zero
-    for (int p = k + lane; p < ((k + 7) & ~7); p += TPA) row[p] = (int)NBR_PAD_ENTRY;
+    for (int p = k + lane; p < ((k + 7) & ~7); p += TPA) row[p] = (int)nbr_pad_entry(lt);
 }
 
 template <int TPA>
@@ -1162,7 +1158,7 @@ __global__ void __launch_bounds__(64 * CB_WAVES) k_nbr_build_cells(
                 nn[ti] = kk;
                 // pad the row to a multiple of 8 entries (as k_nbr_build_mixed)
                 int* row = nbr + S.nbr_off + (int64_t)lti * M;
-                for (int p = kk; p < ((kk + 7) & ~7); ++p) row[p] = (int)NBR_PAD_ENTRY;
+                for (int p = kk; p < ((kk + 7) & ~7); ++p) row[p] = (int)nbr_pad_entry(lti);
             }
         }
     }
@@ -1382,7 +1378,7 @@ __global__ void __launch_bounds__(SB_THREADS) k_nbr_build_staged(
             }
             nn[t] = k;
             int* row = nbr + S.nbr_off + (int64_t)lt * M;
-            for (int p = k; p < ((k + 7) & ~7); ++p) row[p] = (int)NBR_PAD_ENTRY;
+            for (int p = k; p < ((k + 7) & ~7); ++p) row[p] = (int)nbr_pad_entry(lt);
         }
         __syncthreads();  // the next cell rewrites the shared tables
     }

@@ -197,10 +197,11 @@ __device__ __forceinline__ void k_pair_mixed_unit(int64_t unit_,
     if (s != tab_sys) {
         for (int c = threadIdx.x & 63; c < 125; c += 64) {
             const int ox = c % 5 - 2, oy = (c / 5) % 5 - 2, oz = c / 25 - 2;
-            tab[c] = make_float4((float)ox * wx, (float)oy * wy, (float)oz * wz, 0.0f);
+            tab[nbr_code(ox, oy, oz)] =
+                make_float4((float)ox * wx, (float)oy * wy, (float)oz * wz, 0.0f);
         }
-        // row padding (slot 0, 1e15 nm away): r2 = 1e30, outside every
-        // cutoff, and fs * dx = 0 exactly
+        // row padding (the atom itself, 1e15 nm away): r2 = 1e30, outside
+        // every cutoff, and fs * dx = 0 exactly
         if ((threadIdx.x & 63) == 0) tab[NBR_PAD_CODE] = make_float4(1e15f, 0.0f, 0.0f, 0.0f);
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -417,11 +418,9 @@ __device__ __forceinline__ void k_pair_mixed_unit(int64_t unit_,
     fzi = group_sum<TPA>(fzi);
     ej = group_sum<TPA>(ej);
     ec = group_sum<TPA>(ec);
-    // one energy partial per slot (half the row: rows hold both orders)
-    if (lane == 0) {
-        e_lj_part[t] = 0.5 * ej;
-        e_c_part[t] = 0.5 * ec;
-    }
+    // one energy partial per wave (its 64 / TPA atom slots), a fixed tree
+    const double ejw = wave_sum(lane == 0 ? ej : 0.0);
+    const double ecw = COUL ? wave_sum(lane == 0 ? ec : 0.0) : 0.0;
     if (lane == 0 && real) {
         const int i = order[t];
         if (accumulate) {
@@ -432,6 +431,10 @@ __device__ __forceinline__ void k_pair_mixed_unit(int64_t unit_,
         fx[i] = fxi;
         fy[i] = fyi;
         fz[i] = fzi;
+    }
+    if ((threadIdx.x & 63) == 0) {  // rows hold both orders of a pair: halved
+        e_lj_part[t / (64 / TPA)] = 0.5 * ejw;
+        e_c_part[t / (64 / TPA)] = 0.5 * ecw;
     }
 }
 

@@ -12,9 +12,10 @@
 // global memory:
 //     staged index = slot + (row base in LDS - row's first slot),
 // where the difference depends only on the entry's offset code and sits in
-// the w field of the 128-entry code table beside the cell-offset shift.
-// The rows padding (code NBR_PAD_CODE over slot 0) maps to a sentinel LDS
-// entry 1e15 nm away, so a padding entry adds exactly zero.
+// the w field of the code table beside the cell-offset shift.
+// The rows' padding (code NBR_PAD_CODE over the atom's own slot) maps past
+// the staged rows and is clamped to a sentinel LDS entry 1e15 nm away, so a
+// padding entry adds exactly zero.
 //
 // Work split.  The tile's atoms are taken up to PR_ATOMS at a time; their
 // rows are cut into blocks of W entries (rows are padded to multiples of 8),
@@ -40,7 +41,8 @@ constexpr int PR_ROWS = 25;    // j-rows of a tile: 3 x 3, or 5 x 5 with half-wi
 
 struct PrShared {
     float4 stage[PR_CAP + 1];  // staged rows; [PR_CAP] is the far sentinel
-    float4 tab[NBR_CODES];     // per offset code: shift (x, y, z), index delta (w bits)
+    float4 tab[NBR_CODES];     // per offset code: shift (x, y, z), index delta (w bits);
+                               // only the codes of the tile's offsets are written
     float4 head[PR_THREADS];   // per lane: partial sum of its first atom (begun earlier)
     float4 tail[PR_THREADS];   // per lane: partial sum of its last atom (continued later)
     int bpre[PR_ATOMS + 1];    // running block counts of the chunk's atoms
@@ -165,7 +167,8 @@ __device__ __forceinline__ void pr_blocks(PrShared& sh, const SysDev& S, int R0,
             for (int q = 0; q < 4; ++q) {
                 const uint32_t e = ev[h + q];
                 shv[q] = sh.tab[e >> NBR_SLOT_BITS];
-                if constexpr (STAGED) pj[q] = sh.stage[e + (uint32_t)__float_as_int(shv[q].w)];
+                if constexpr (STAGED)
+                    pj[q] = sh.stage[min(e + (uint32_t)__float_as_int(shv[q].w), (uint32_t)PR_CAP)];
                 else pj[q] = lps[e & NBR_SLOT_MASK];
                 if constexpr (TYPED)
                     tj[q] = STAGED ? __float_as_int(pj[q].w) : tys[e & NBR_SLOT_MASK];
@@ -306,23 +309,23 @@ __global__ void __launch_bounds__(PR_THREADS) __attribute__((amdgpu_waves_per_eu
         // code table: shift of the neighbour cell's frame, and the staged
         // index delta of its row (entries hold the code in their top bits,
         // so the delta subtracts it)
-        if (tid < NBR_CODES) {
+        if (tid < 125) {  // the offsets -2..2 along each edge
             const int ox = tid % 5 - 2, oy = (tid / 5) % 5 - 2, oz = tid / 25 - 2;
-            float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            if (tid < 125 && ox >= lox && ox <= hix && oy >= loy && oy <= hiy && oz >= loz &&
-                oz <= hiz) {
+            if (ox >= lox && ox <= hix && oy >= loy && oy <= hiy && oz >= loz && oz <= hiz) {
                 const int r = (oy - loy) + eyn * (oz - loz);
+                const uint32_t code = nbr_code(ox, oy, oz);
                 const uint32_t delta = (uint32_t)(sh.rbase[r] - sh.rs0[r]) -
-                                       ((uint32_t)tid << NBR_SLOT_BITS);
-                v = make_float4((float)ox * wx, (float)oy * wy, (float)oz * wz,
-                                __int_as_float((int)delta));
-            } else if (tid == (int)NBR_PAD_CODE) {
-                // padding entries (slot 0): the sentinel when staged, else a
-                // shift beyond every cutoff
-                const uint32_t delta = (uint32_t)PR_CAP - NBR_PAD_ENTRY;
-                v = make_float4(staged ? 0.0f : 1e15f, 0.0f, 0.0f, __int_as_float((int)delta));
+                                       (code << NBR_SLOT_BITS);
+                sh.tab[code] = make_float4((float)ox * wx, (float)oy * wy, (float)oz * wz,
+                                           __int_as_float((int)delta));
             }
-            sh.tab[tid] = v;
+        } else if (tid == 125) {
+            // padding entries (the atom's own slot): past the staged rows,
+            // clamped to the sentinel, when staged; else a shift beyond every
+            // cutoff
+            const uint32_t delta = (uint32_t)PR_CAP - (NBR_PAD_CODE << NBR_SLOT_BITS);
+            sh.tab[NBR_PAD_CODE] =
+                make_float4(staged ? 0.0f : 1e15f, 0.0f, 0.0f, __int_as_float((int)delta));
         }
         const bool typed = S.use_lj && S.ntypes > 1;
         if (staged) {
@@ -448,7 +451,7 @@ hipError_t launch_pair_rows(const Dev& d, hipStream_t s, int accumulate)
     const unsigned g = grid_for(d.nrows);
 #define DMDX_ROWS_ARGS                                                                      \
     d.sys, d.st, d.row_sys, d.nrows, d.ccount, d.cstart, d.order, d.nn, d.nbr, d.lpos, d.typ_s, \
-        d.lj_c12, d.lj_c6, d.fx, d.fy, d.fz, d.p_lj, accumulate
+        d.lj_c12, d.lj_c6, d.fx, d.fy, d.fz, d.p_rows, accumulate
     if (d.pair_rows == 4) k_pair_rows<4><<<g, PR_THREADS, 0, s>>>(DMDX_ROWS_ARGS);
     else k_pair_rows<8><<<g, PR_THREADS, 0, s>>>(DMDX_ROWS_ARGS);
 #undef DMDX_ROWS_ARGS
