@@ -27,10 +27,18 @@
 // few steps at their default skin.  Systems of the same cutoffs and skin form
 // a group; each group's skin is scaled by 1, 1.3, 1.6 or 2, one group at a
 // time (largest first), timing a scratch batch over 20 steps from the start
-// and 20 steps after 60 more; a change must win by 2%.  The chosen skins
-// replace the engine's (the batch is made again from its descriptions,
-// which it has not left).  The forces do not depend on the skin: only the
-// FP32 summation order of the mixed path changes.
+// and 20 steps after 60 more; a change must win by 2%.  Then, the same way,
+// each group's cell grid: cells of the list radius or of half of it (the
+// density rule picks half-width cells from 40 atoms per list-radius cube;
+// finer cells cut the candidates of the list build and change the order in
+// which the pair kernel gathers, coarser ones are fewer to scan).  The
+// chosen skins and cells replace the engine's (the batch is made again from
+// its descriptions, which it has not left).  The forces do not depend on
+// either: only the FP32 summation order of the mixed path changes.
+//
+// Batches with PME then time the FFT's lines per workgroup tile (the
+// grid-size rule, or 4, 8, 16) on the PME group of a scratch batch, and the
+// engine's batch is made again with a tiling that wins by 2%.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -220,11 +228,13 @@ std::vector<dmdx_system_desc> descs_with(const dmdx_engine* e, const std::vector
 // ms per step of a scratch batch with these skins and the engine's launch
 // shapes, the list built per cell (cells) or per atom: a window from the
 // start and one later (the hot first steps rebuild most)
-int time_skins_build(dmdx_engine* e, const std::vector<dmdx_system_desc>& ds, bool cells,
-                     double* ms)
+int time_skins_build(dmdx_engine* e, const std::vector<dmdx_system_desc>& ds,
+                     const std::vector<int>& cdiv, bool cells, double* ms)
 {
     dmdx::BatchOptions bo;
     bo.reuse_forces = e->b.options.reuse_forces;
+    bo.cdiv = cdiv;
+    bo.fft_lines = e->b.options.fft_lines;
     dmdx::Batch b;
     int rc = b.create(ds.data(), (int)ds.size(), bo);
     if (rc) return rc;
@@ -254,22 +264,39 @@ int time_skins_build(dmdx_engine* e, const std::vector<dmdx_system_desc>& ds, bo
     return DMDX_OK;
 }
 
-// the faster of the two list builds with these skins (which build wins
-// moves with the row length); *cells receives it
+// per-system cells per list radius: each group's choice (0: the density
+// rule), the engine's own setting for systems outside the groups
+std::vector<int> cdiv_with(const dmdx_engine* e, const std::vector<SkinGroup>& g,
+                           const std::vector<int>& cd)
+{
+    std::vector<int> v(e->descs.size(), 0);
+    for (size_t s = 0; s < v.size() && s < e->b.options.cdiv.size(); ++s)
+        v[s] = e->b.options.cdiv[s];
+    for (size_t k = 0; k < g.size(); ++k)
+        for (int s : g[k].sys) v[s] = cd[k];
+    return v;
+}
+
+// the faster of the two list builds with these skins and cells (which build
+// wins moves with the row length); *cells receives it
 int time_skins(dmdx_engine* e, const std::vector<SkinGroup>& g, const std::vector<int>& choice,
-               double* ms, bool* cells)
+               const std::vector<int>& cd, double* ms, bool* cells)
 {
     const std::vector<dmdx_system_desc> ds = descs_with(e, g, choice);
+    const std::vector<int> cdv = cdiv_with(e, g, cd);
     double tc = 0.0, ta = 0.0;
-    int rc = time_skins_build(e, ds, true, &tc);
-    if (!rc) rc = time_skins_build(e, ds, false, &ta);
+    int rc = time_skins_build(e, ds, cdv, true, &tc);
+    if (!rc) rc = time_skins_build(e, ds, cdv, false, &ta);
     if (rc) return rc;
     *cells = tc <= ta;
     *ms = std::min(tc, ta);
     return DMDX_OK;
 }
 
-bool skin_cache_lookup(const char* path, uint64_t key, size_t ngroups, std::vector<int>* choice)
+// "KEY skin s0 s1 ... [cdiv c0 c1 ...]": per group, the skin multiplier's
+// index and the cells per list radius (0: the density rule; absent: 0)
+bool skin_cache_lookup(const char* path, uint64_t key, size_t ngroups, std::vector<int>* choice,
+                       std::vector<int>* cd)
 {
     if (!path || !*path) return false;
     std::ifstream f(path);
@@ -279,20 +306,34 @@ bool skin_cache_lookup(const char* path, uint64_t key, size_t ngroups, std::vect
         std::string k, tag;
         if (!(ss >> k >> tag) || tag != "skin") continue;
         if (std::strtoull(k.c_str(), nullptr, 16) != key) continue;
-        std::vector<int> c;
-        int v;
-        while (ss >> v) {
-            if (v < 0 || v >= NSKIN) return false;
-            c.push_back(v);
+        std::vector<int> c, d;
+        std::string tok;
+        bool in_cdiv = false;
+        while (ss >> tok) {
+            if (tok == "cdiv") {
+                in_cdiv = true;
+                continue;
+            }
+            const int v = std::atoi(tok.c_str());
+            if (in_cdiv) {
+                if (v < 0 || v > 2) return false;
+                d.push_back(v);
+            } else {
+                if (v < 0 || v >= NSKIN) return false;
+                c.push_back(v);
+            }
         }
-        if (c.size() != ngroups) return false;
+        if (c.size() != ngroups || (!d.empty() && d.size() != ngroups)) return false;
+        if (d.empty()) d.assign(ngroups, 0);
         *choice = c;
+        *cd = d;
         return true;
     }
     return false;
 }
 
-void skin_cache_store(const char* path, uint64_t key, const std::vector<int>& choice)
+void skin_cache_store(const char* path, uint64_t key, const std::vector<int>& choice,
+                      const std::vector<int>& cd)
 {
     if (!path || !*path) return;
     std::ofstream f(path, std::ios::app);
@@ -300,15 +341,17 @@ void skin_cache_store(const char* path, uint64_t key, const std::vector<int>& ch
     std::snprintf(buf, sizeof(buf), "%016llx skin", (unsigned long long)key);
     f << buf;
     for (int c : choice) f << " " << c;
+    f << " cdiv";
+    for (int c : cd) f << " " << c;
     f << "\n";
 }
 
-// Make the engine's batch again from its descriptions with the chosen skins
-// (only for a batch that has not left the state it was created with).
-int apply_skins(dmdx_engine* e, const std::vector<SkinGroup>& g, const std::vector<int>& choice)
+// Make the engine's batch again from these descriptions and options (only
+// for a batch that has not left the state it was created with); on failure
+// the engine keeps a batch of its own descriptions and options.
+int remake(dmdx_engine* e, const std::vector<dmdx_system_desc>& ds, const dmdx::BatchOptions& opt)
 {
-    std::vector<dmdx_system_desc> ds = descs_with(e, g, choice);
-    const dmdx::BatchOptions opt = e->b.options;
+    const dmdx::BatchOptions old_opt = e->b.options;
     const dmdx::Dev keep = e->b.d;  // launch shapes
     e->b.~Batch();
     new (&e->b) dmdx::Batch();
@@ -319,7 +362,7 @@ int apply_skins(dmdx_engine* e, const std::vector<SkinGroup>& g, const std::vect
         for (const auto& dc : e->descs) orig.push_back(dc->d);
         e->b.~Batch();
         new (&e->b) dmdx::Batch();
-        const int rc2 = e->b.create(orig.data(), (int)orig.size(), opt);
+        const int rc2 = e->b.create(orig.data(), (int)orig.size(), old_opt);
         return rc2 ? rc2 : rc;
     }
     for (size_t s = 0; s < ds.size(); ++s) e->descs[s]->d.list_skin = ds[s].list_skin;
@@ -333,25 +376,25 @@ int apply_skins(dmdx_engine* e, const std::vector<SkinGroup>& g, const std::vect
     return DMDX_OK;
 }
 
-// *changed: some group's skin was widened (the batch was made again)
+// *changed: some group's skin or cells changed (the batch was made again)
 int tune_skins(dmdx_engine* e, const char* cache_path, bool* changed)
 {
     *changed = false;
     const std::vector<SkinGroup> g = skin_groups(e);
     if (g.empty()) return DMDX_OK;
     const uint64_t key = signature(e);
-    std::vector<int> best(g.size(), 0);
-    if (!skin_cache_lookup(cache_path, key, g.size(), &best)) {
+    std::vector<int> best(g.size(), 0), cd(g.size(), 0);
+    if (!skin_cache_lookup(cache_path, key, g.size(), &best, &cd)) {
         double tbest = 0.0;
         bool cells = true;
-        int rc = time_skins(e, g, best, &tbest, &cells);
+        int rc = time_skins(e, g, best, cd, &tbest, &cells);
         if (rc) return rc;
         for (size_t k = 0; k < g.size(); ++k) {
             for (int c = 1; c < NSKIN; ++c) {
                 std::vector<int> trial = best;
                 trial[k] = c;
                 double t = 0.0;
-                rc = time_skins(e, g, trial, &t, &cells);
+                rc = time_skins(e, g, trial, cd, &t, &cells);
                 if (rc == DMDX_E_PARM) break;  // rows beyond their maximum: no wider skin
                 if (rc) return rc;
                 if (t < 0.98 * tbest) {
@@ -360,14 +403,101 @@ int tune_skins(dmdx_engine* e, const char* cache_path, bool* changed)
                 }
             }
         }
-        skin_cache_store(cache_path, key, best);
-    }
-    for (int c : best)
-        if (c != 0) {
-            *changed = true;
-            return apply_skins(e, g, best);
+        // cells of the list radius (1) or half of it (2), with the chosen skins
+        for (size_t k = 0; k < g.size(); ++k) {
+            for (int c = 1; c <= 2; ++c) {
+                std::vector<int> trial = cd;
+                trial[k] = c;
+                double t = 0.0;
+                rc = time_skins(e, g, best, trial, &t, &cells);
+                if (rc) return rc;
+                if (t < 0.98 * tbest) {
+                    cd = trial;
+                    tbest = t;
+                }
+            }
         }
-    return DMDX_OK;
+        skin_cache_store(cache_path, key, best, cd);
+    }
+    bool any = false;
+    for (size_t k = 0; k < g.size(); ++k) any = any || best[k] != 0 || cd[k] != 0;
+    if (!any) return DMDX_OK;
+    *changed = true;
+    dmdx::BatchOptions opt = e->b.options;
+    opt.cdiv = cdiv_with(e, g, cd);
+    return remake(e, descs_with(e, g, best), opt);
+}
+
+// FFT lines per workgroup tile for batches with PME (see the top of this
+// file); *changed: the batch was made again with another tiling
+constexpr int NFFT = 4;
+constexpr int FFT_LINES[NFFT] = {0, 4, 8, 16};  // 0: the grid-size rule
+
+int tune_fft(dmdx_engine* e, int32_t steps, const char* cache_path, bool* changed)
+{
+    *changed = false;
+    if (!e->b.d.any_pme) return DMDX_OK;
+    const uint64_t key = signature(e);
+    int best = -1;
+    if (cache_path && *cache_path) {
+        std::ifstream f(cache_path);
+        std::string line;
+        while (std::getline(f, line)) {
+            std::istringstream ss(line);
+            std::string k, tag;
+            int v = -1;
+            if (!(ss >> k >> tag >> v) || tag != "fft") continue;
+            if (std::strtoull(k.c_str(), nullptr, 16) == key &&
+                (v == 0 || v == 4 || v == 8 || v == 16))
+                best = v;
+        }
+    }
+    if (best < 0) {
+        std::vector<dmdx_system_desc> ds;
+        for (const auto& dc : e->descs) ds.push_back(dc->d);
+        const int n = steps > 0 ? steps : 5;
+        double t[NFFT];
+        for (int c = 0; c < NFFT; ++c) {
+            dmdx::BatchOptions bo;
+            bo.reuse_forces = e->b.options.reuse_forces;
+            bo.cdiv = e->b.options.cdiv;
+            bo.fft_lines = FFT_LINES[c];
+            bo.profile = true;
+            bo.profile_mask = 1u << dmdx::KG_PME;
+            dmdx::Batch b;
+            int rc = b.create(ds.data(), (int)ds.size(), bo);
+            if (rc) return rc;
+            b.d.pair_tpa = e->b.d.pair_tpa;
+            b.d.pair_width = e->b.d.pair_width;
+            double ms0, ms1;
+            int64_t n0, n1;
+            rc = b.step(1);
+            if (!rc) rc = b.kernel_time(dmdx::KG_PME, &ms0, &n0);
+            if (!rc) rc = b.step(n);
+            if (!rc) rc = b.kernel_time(dmdx::KG_PME, &ms1, &n1);
+            if (rc) return rc;
+            t[c] = (ms1 - ms0) / n;
+        }
+        int bc = 0;
+        for (int c = 1; c < NFFT; ++c)
+            if (t[c] < 0.98 * t[bc]) bc = c;
+        best = FFT_LINES[bc];
+        if (cache_path && *cache_path) {
+            std::ofstream f(cache_path, std::ios::app);
+            char buf[160];
+            std::snprintf(buf, sizeof(buf),
+                          "%016llx fft %d  # pme ms rule %.4f lines4 %.4f lines8 %.4f lines16 %.4f\n",
+                          (unsigned long long)key, best, t[0], t[1], t[2], t[3]);
+            f << buf;
+        }
+    }
+    if (best == e->b.options.fft_lines) return DMDX_OK;
+    *changed = true;
+    std::vector<dmdx_system_desc> ds;
+    for (const auto& dc : e->descs) ds.push_back(dc->d);
+    dmdx::BatchOptions opt = e->b.options;
+    opt.fft_lines = best;
+    return remake(e, ds, opt);
 }
 
 // Launch shapes (see the top of this file) for the engine's systems as they
@@ -476,6 +606,13 @@ int tune_shapes(dmdx_engine* e, int32_t steps, const char* cache_path, double* t
 
 }  // namespace
 
+int dmdx_engine_remake(dmdx_engine* e, const dmdx::BatchOptions& opt)
+{
+    std::vector<dmdx_system_desc> ds;
+    for (const auto& dc : e->descs) ds.push_back(dc->d);
+    return remake(e, ds, opt);
+}
+
 extern "C" int dmdx_engine_autotune(dmdx_engine* e, int32_t steps, const char* cache_path,
                                     int32_t* pair_lanes, int32_t* pair_width,
                                     int32_t* build_lanes, double* times_ms)
@@ -494,14 +631,16 @@ extern "C" int dmdx_engine_autotune(dmdx_engine* e, int32_t steps, const char* c
         report();
         return DMDX_OK;
     }
-    if (e->tune_skin && e->b.pristine && !skin_groups(e).empty()) {
-        // shapes for the skins as they are (the skin trials run with them),
-        // then the skins, then the shapes again if a skin changed
+    if (e->tune_skin && e->b.pristine && (!skin_groups(e).empty() || e->b.d.any_pme)) {
+        // shapes for the batch as it is (the skin, cell and FFT trials run
+        // with them), then the skins and cells, then the FFT tiling, then the
+        // shapes again if the batch was made again
         int rc = tune_shapes(e, steps, cache_path, t);
-        bool changed = false;
+        bool changed = false, fft_changed = false;
         if (!rc) rc = tune_skins(e, cache_path, &changed);
+        if (!rc) rc = tune_fft(e, steps, cache_path, &fft_changed);
         if (rc) return rc;
-        if (!changed) {
+        if (!changed && !fft_changed) {
             report();
             return DMDX_OK;
         }

@@ -256,7 +256,10 @@ int Batch::create(const dmdx_system_desc* descs, int nsys, const BatchOptions& o
             // water): cells of half the list radius, searched +-2, cut the
             // candidates by ~40%; sparse systems keep one cell per radius
             const double per_cell = n / (D.box[0] * D.box[1] * D.box[2]) * S.rlist * S.rlist * S.rlist;
-            S.cdiv = (D.precision != DMDX_FP64 && per_cell >= 40.0) ? 2 : 1;
+            const int forced = (size_t)s < opt.cdiv.size() ? opt.cdiv[s] : 0;
+            S.cdiv = D.precision == DMDX_FP64 ? 1
+                     : (forced == 1 || forced == 2) ? forced
+                     : (per_cell >= 40.0 ? 2 : 1);
             for (int k = 0; k < 3; ++k) {
                 int c = (int)std::floor(D.box[k] * S.cdiv / (S.rlist * (1.0 + 1e-9)));
                 // a barostat can grow the box: leave room for more cells
@@ -603,6 +606,7 @@ int Batch::create(const dmdx_system_desc* descs, int nsys, const BatchOptions& o
             // 48 KiB of LDS for the two ping-pong buffers and the twiddles
             int B = 4;
             while (B < 16 && 2 * B * 256 <= nlines) B *= 2;
+            if (opt.fft_lines > 0) B = opt.fft_lines;  // the autotuner's choice
             B = (int)std::max<size_t>(
                 1, std::min<size_t>((size_t)B, (48 * 1024 - (size_t)K * cx) / (2 * K * cx)));
             int local = 0;
@@ -1620,6 +1624,16 @@ int dmdx_engine_set_tuning(dmdx_engine* e, int32_t key, int32_t value)
         if (value != 0 && value != 4 && value != 8)
             return dmdx::fail(DMDX_E_PARM, "row-tile pair kernel is 0 (off), 4 or 8 entries per block");
         e->b.d.pair_rows = value;
+    } else if (key == 9) {
+        // cells per list radius of every list system, for a batch that has
+        // not left its initial state (the batch is made again)
+        if (value < 0 || value > 2)
+            return dmdx::fail(DMDX_E_PARM, "cells per list radius is 0 (density rule), 1 or 2");
+        if (!e->b.pristine)
+            return dmdx::fail(DMDX_E_INVALID, "the cell grid is set before the first step");
+        dmdx::BatchOptions opt = e->b.options;
+        opt.cdiv.assign(e->descs.size(), value);
+        return dmdx_engine_remake(e, opt);
     } else if (key == 7) {
         if (value != 0 && value != 8 && value != 16)
             return dmdx::fail(DMDX_E_PARM, "staged per-cell build is 0 (off), 8 or 16 lanes per atom");
@@ -1644,8 +1658,17 @@ int dmdx_engine_get_tuning(dmdx_engine* e, int32_t key, int32_t* value)
     case 5: *value = e->tune_skin ? 1 : 0; break;
     case 6: *value = d.pair_rows; break;
     case 7: *value = d.build_stage; break;
+    case 8: *value = e->b.options.fft_lines; break;
     default: return dmdx::fail(DMDX_E_INVALID, "unknown tuning key");
     }
+    return DMDX_OK;
+}
+
+int dmdx_engine_get_cell_div(dmdx_engine* e, int32_t sys, int32_t* cdiv)
+{
+    if (!e || !cdiv) return dmdx::fail(DMDX_E_INVALID, "null argument");
+    if (sys < 0 || sys >= (int32_t)e->descs.size()) return dmdx::fail(DMDX_E_INVALID, "bad system");
+    *cdiv = e->b.hsys[sys].cdiv;
     return DMDX_OK;
 }
 

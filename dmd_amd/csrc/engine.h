@@ -30,6 +30,12 @@ struct BatchOptions {
     bool use_graph = false;    // replay captured blocks of steps
     int graph_steps = 10;
     int max_nbr = 0;           // 0: estimate from the density
+    // mixed path, per system: cells per list radius along an edge (1 or 2;
+    // 0 or absent: the density rule, 2 from 40 atoms per list-radius cube)
+    std::vector<int> cdiv;
+    // PME: FFT lines per workgroup tile (4, 8 or 16; 0: the grid-size rule),
+    // within the LDS limit
+    int fft_lines = 0;
     bool profile = false;      // HIP events around kernel groups
     uint32_t profile_mask = ~0u;  // which groups (bit = group) when profiling
     // allocation-free mode: device arrays are carved from `arena` (capacity
@@ -235,5 +241,9 @@ struct dmdx_engine {
     dmdx::Batch b;
     // host copies of the descriptions (the autotuner builds scratch batches)
     std::vector<std::unique_ptr<dmdx::DescCopy>> descs;
-    bool tune_skin = true;  // the autotuner may widen Verlet-criterion skins
+    bool tune_skin = true;  // the autotuner may change skins, cells and FFT tiling
 };
+
+// Make an engine's batch again from its descriptions with these options
+// (autotune.cpp; for an engine that has not stepped since creation)
+int dmdx_engine_remake(dmdx_engine* e, const dmdx::BatchOptions& opt);

@@ -58,6 +58,7 @@ def _setup(lib):
                                          C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                          C.POINTER(C.c_int32), C.POINTER(C.c_double)]
     lib.dmdx_engine_get_list_skin.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]
+    lib.dmdx_engine_get_cell_div.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_int32)]
     lib.dmdx_engine_snapshot_begin.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_int64)]
     lib.dmdx_engine_snapshot_read.argtypes = [C.c_void_p, C.c_int64, _dp, _dp, _dp,
                                               C.POINTER(C.c_int64)]
@@ -180,7 +181,8 @@ class Engine:
     def set_tuning(self, pair_lanes: int | None = None, build_lanes: int | None = None,
                    pair_width: int | None = None, build_cells: bool | None = None,
                    build_tasks: int | None = None, tune_skin: bool | None = None,
-                   pair_rows: int | None = None, build_stage: int | None = None):
+                   pair_rows: int | None = None, build_stage: int | None = None,
+                   cell_div: int | None = None):
         """Launch shapes of the mixed kernels (dmdx_engine_set_tuning);
         build_cells: list built per cell (default) or per atom; build_tasks:
         waves per cell of the per-cell build (1, 2, 4, 8); pair_rows: LJ-only
@@ -188,7 +190,11 @@ class Engine:
         block, or 0 (default) for the per-atom kernel; build_stage: the
         per-cell build with each cell's candidates staged in LDS for 8 or 16
         lanes per atom, or 0 (default) for candidates shared in registers by
-        build_tasks waves."""
+        build_tasks waves; cell_div (before the first step): cells of the
+        list radius (1) or half of it (2) for every mixed system with a list,
+        0 for the density rule."""
+        if cell_div is not None:  # before the first step: the batch is made again
+            _abi.check(self.lib.dmdx_engine_set_tuning(self.h, 9, int(cell_div)))
         if build_stage is not None:
             _abi.check(self.lib.dmdx_engine_set_tuning(self.h, 7, int(build_stage)))
         if pair_rows is not None:
@@ -214,7 +220,9 @@ class Engine:
         stepped, first the list skins of displacement-criterion systems.
         `cache`: text file of earlier results.  "build" names the chosen list
         build, "pair_rows" the row-tile setting of LJ-only systems (0: the
-        per-atom kernel), "skins" the widened skins."""
+        per-atom kernel), "skins" the widened skins, "cell_div" how many
+        systems have cells of the list radius (1) or half of it (2),
+        "fft_lines" the FFT tiling (0: the grid-size rule)."""
         p, w, b = C.c_int32(), C.c_int32(), C.c_int32()
         npr = len(PAIR_SHAPES) + len(ROW_SHAPES)
         t = (C.c_double * (npr + len(BUILD_SHAPES)))()
@@ -236,9 +244,19 @@ class Engine:
                 "skins": [{"skin": a, "tuned": t, "systems": n}
                           for (a, t), n in sorted(widened.items())],
                 "pair_rows": self.get_tuning(6),
+                "fft_lines": self.get_tuning(8),
+                "cell_div": {c: sum(1 for k in range(len(self.specs)) if self.cell_div(k) == c)
+                             for c in (1, 2)},
                 "pair_ms": {k: t[i] for i, k in enumerate(PAIR_SHAPES)},
                 "rows_ms": {k: t[len(PAIR_SHAPES) + i] for i, k in enumerate(ROW_SHAPES)},
                 "build_ms": {k: t[npr + i] for i, k in enumerate(BUILD_SHAPES)}}
+
+    def cell_div(self, sys: int = 0) -> int:
+        """Cells per list radius of system `sys`'s list grid (1 or 2;
+        dmdx_engine_get_cell_div)."""
+        v = C.c_int32()
+        _abi.check(self.lib.dmdx_engine_get_cell_div(self.h, int(sys), C.byref(v)))
+        return v.value
 
     def list_skin(self, sys: int = 0) -> float:
         """The list skin system `sys` is built with (dmdx_engine_get_list_skin)."""

@@ -403,13 +403,19 @@ int dmdx_engine_kernel_time(dmdx_engine *e, int32_t group, double *total_ms, int
  * build per cell (1, default: the atoms of a cell share their candidates;
  * systems that need a minimum image are still built per atom) or per atom
  * (0), key 4 = waves per cell of the per-cell build (1, 2, 4 or 8), key 5
- * = whether dmdx_engine_autotune may widen list skins (1, default, or 0),
+ * = whether dmdx_engine_autotune may change list skins, cells and the FFT
+ * tiling (1, default, or 0),
  * key 6 = LJ-only systems' pair kernel: 4 or 8 = row tiles staged in LDS
  * with that many list entries per lane block, 0 (default) = the per-atom
  * kernel of keys 0 and 2 (charged systems always use the per-atom kernel),
  * key 7 = the per-cell build's kernel: 8 or 16 = each cell's candidates
  * staged in LDS and scanned by that many lanes per atom, 0 (default) =
  * candidates shared in registers by the waves of key 4.
+ * Key 8 (read only) = the FFT lines per workgroup tile the autotuner chose
+ * for PME (4, 8, 16; 0 = the grid-size rule).  Key 9 (write only, before
+ * the first step; the batch is made again) = cells per list radius of every
+ * mixed system with a list: 1, 2, or 0 for the density rule (read per system
+ * with dmdx_engine_get_cell_div).
  * The list rows do not depend on keys 1, 3, 4 or 7.  Pair shapes
  * (lanes x entries): 2x8, 4x8, 8x8, 4x4, 8x4, 16x4, 32x4, 8x2, 16x2, 32x2;
  * another combination runs as the nearest of these (8 lanes where fewer
@@ -440,12 +446,16 @@ int dmdx_engine_get_tuning(dmdx_engine *e, int32_t key, int32_t *value);
  * First, on an engine that has not stepped or been changed since creation
  * (and unless tuning key 5 is 0): the list skin of systems that rebuild by
  * the displacement criterion (rebuild_interval -1), scaled by 1, 1.3, 1.6 or
- * 2 per group of systems with the same cutoffs and skin, timed on scratch
- * batches (20 steps from the start and 20 after 60 more); a wider skin must
- * win by 2%.  The engine's batch is then made again with the chosen skins
- * (dmdx_engine_get_list_skin reads them).
- * Mixed-precision results depend on the shape and the skin only through
- * FP32 summation order (well inside the mixed-precision tolerance). */
+ * 2 per group of systems with the same cutoffs and skin, then each group's
+ * cells per list radius (1 or 2, against the density rule), timed on scratch
+ * batches (20 steps from the start and 20 after 60 more); a change must win
+ * by 2%.  Then, for a batch with PME, the FFT lines per workgroup tile (the
+ * grid-size rule, 4, 8 or 16) timed on the PME kernels.  The engine's batch
+ * is then made again with the choices (dmdx_engine_get_list_skin,
+ * dmdx_engine_get_cell_div and tuning key 8 read them).
+ * Mixed-precision results depend on the shape, the skin, the cells and the
+ * FFT tiling only through FP32 summation order (well inside the
+ * mixed-precision tolerance). */
 #define DMDX_TUNE_TIMES 18
 int dmdx_engine_autotune(dmdx_engine *e, int32_t steps, const char *cache_path,
                          int32_t *pair_lanes, int32_t *pair_width, int32_t *build_lanes,
@@ -453,6 +463,10 @@ int dmdx_engine_autotune(dmdx_engine *e, int32_t steps, const char *cache_path,
 /* The list skin system sys is built with (its description's, or the
  * autotuner's choice). */
 int dmdx_engine_get_list_skin(dmdx_engine *e, int32_t sys, double *skin);
+/* Cells per list radius along an edge of system `sys`'s list grid (1, or 2
+ * for cells of half the list radius searched +-2): the density rule's, or
+ * the autotuner's choice.  No counterpart in the reference. */
+int dmdx_engine_get_cell_div(dmdx_engine *e, int32_t sys, int32_t *cdiv);
 
 /* Device-to-device copy of nbytes, repeated; reports the achieved GB/s
  * (read + write).  Calibrates hardware byte counters. */

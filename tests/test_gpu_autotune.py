@@ -104,3 +104,65 @@ def test_autotune_skins(tmp_path):
         a, b = f64.forces(), e.forces(k)
         err = float(np.sqrt(((b - a) ** 2).sum() / (a ** 2).sum()))
         assert err <= 1e-5, (k, err)
+
+
+def test_cell_grid_forced():
+    """Tuning key 9 sets the cell grid before the first step: water (half-width
+    cells by the density rule) on cells of the list radius, an LJ fluid (cells
+    of the list radius) on half-width cells.  The pair sets stay the oracle's
+    and the forces meet the mixed-precision bar; after a step the key is
+    refused."""
+    import oracle as O
+    from dmd_amd.engine import Engine
+    cases = [(W.water_box(n_side=10, seed=2, precision=_abi.MIXED), 1),
+             (W.lj_fluid(4000, 0.8442, 0.72, seed=3, cutoff=0.85, skin=0.1,
+                         precision=_abi.MIXED), 2)]
+    for spec, cd in cases:
+        print(f"case: {spec.name} forced to {cd} cells per list radius", flush=True)
+        e = Engine(spec)
+        rule = e.cell_div()
+        e.set_tuning(cell_div=cd)
+        assert e.cell_div() == cd != rule
+        e.compute()
+        rl = max(spec.lj_cutoff, spec.coulomb_cutoff if spec.coulomb_type else 0.0) + spec.list_skin
+        oi, oj = O.pairs(spec.pos, spec.box, rl, "cells")
+        if spec.exclusions is not None:
+            oi, oj = O.filter_excl(oi, oj, *spec.exclusions)
+        gi, gj = e.pairs()
+        assert np.array_equal(gi, oi) and np.array_equal(gj, oj)
+        f64 = Engine(spec.with_(precision=_abi.FP64))
+        f64.compute()
+        a, b = f64.forces(), e.forces()
+        err = float(np.sqrt(((b - a) ** 2).sum() / (a ** 2).sum()))
+        assert err <= 1e-5, err
+        e.step(1)
+        with pytest.raises(RuntimeError):
+            e.set_tuning(cell_div=rule)
+
+
+def test_autotune_cells_and_fft(tmp_path):
+    """On a fresh batch with PME the autotuner also picks each group's cells
+    and the FFT tiling; a second engine replays both from the cache; the
+    tuned forces meet the mixed-precision bar against the FP64 path."""
+    from dmd_amd.engine import Engine
+    specs = [W.water_box(n_mol=512, seed=4, precision=_abi.MIXED),
+             W.lj_fluid(1500, 0.8, 1.0, seed=5, precision=_abi.MIXED),
+             W.chain_melt(1200, seed=6, precision=_abi.MIXED)]
+    cache = str(tmp_path / "tune.txt")
+    e = Engine(specs)
+    r = e.autotune(steps=3, cache=cache)
+    print({k: r[k] for k in ("fft_lines", "cell_div", "skins")}, flush=True)
+    assert r["fft_lines"] in (0, 4, 8, 16)
+    assert sum(r["cell_div"].values()) == len(specs)
+    e2 = Engine(specs)
+    r2 = e2.autotune(steps=3, cache=cache)
+    assert r2["fft_lines"] == r["fft_lines"]
+    assert [e2.cell_div(k) for k in range(3)] == [e.cell_div(k) for k in range(3)]
+    assert [e2.list_skin(k) for k in range(3)] == [e.list_skin(k) for k in range(3)]
+    e.compute()
+    for k, s in enumerate(specs):
+        f64 = Engine(s.with_(precision=_abi.FP64))
+        f64.compute()
+        a, b = f64.forces(), e.forces(k)
+        err = float(np.sqrt(((b - a) ** 2).sum() / (a ** 2).sum()))
+        assert err <= 1e-5, (k, err)
