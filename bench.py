@@ -1,6 +1,7 @@
 #!/usr/bin/env python3
 """Throughput of DMD's step path on one MI355X: batched mixed precision vs
-the plain one-system-at-a-time FP64 path vs the CPU (oracle, 1 core).
+the plain one-system-at-a-time FP64 path vs the CPU (DMD's own compiled
+components, 1 core).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 
@@ -17,8 +18,10 @@ melts relax from a hot initial packing during it).  Per-step latency
 percentiles come from separately synchronised steps after the timed region.
 
 The default run also reports, beside ``value``: the plain path (each
-replica alone, FP64, best of eager and hipGraph launches), the CPU oracle
-on one pinned core and on all host cores (one replica per worker), and a
+replica alone, FP64, best of eager and hipGraph launches), the CPU
+baseline -- DMD's own translation units built under oracle/_ref (the C
+restatement where they are absent or the workload needs PME) -- on one
+pinned core and on all host cores (one replica per worker), and a
 served leg -- a bounded Poisson stream of LJ / SPME-water requests on
 dmdx_server with its p50/p99 step and completion latencies and its speedup
 over the plain path on the same requests (``serving``; --no-serving skips
@@ -199,18 +202,34 @@ def run_plain(specs, steps, budget_s):
 CPU_STEPS = 10
 
 
-def run_cpu(specs, budget_s):
-    """The FP64 oracle (single-threaded C restatement of DMD's loops, O(N^2)
-    list rebuild every 10 computes) on a bounded sample of the replicas,
-    pinned to one host core."""
+def _cpu_sim(spec, kind):
+    """A CPU simulation of one replica: DMD's own translation units
+    (oracle/_ref, "reference") or the C restatement of DMD's loops
+    ("port"); both rebuild DMD's O(N^2) list every 10 computes and compute
+    twice per step."""
     import oracle
+    return oracle.RefSim(spec) if kind == "reference" else oracle.Sim(spec)
+
+
+def cpu_kind(specs):
+    """"reference" where DMD's translation units were compiled (oracle/_ref)
+    and the workload has no PME (DMD's PME unit needs FFTW, which is not
+    built), else "port"."""
+    import oracle
+    pme = any(s.coulomb_type == 2 for s in specs)
+    return "reference" if oracle.ref_available() and not pme else "port"
+
+
+def run_cpu(specs, budget_s, kind):
+    """One CPU simulation kind on a bounded sample of the replicas, pinned to
+    one host core."""
     atoms_steps, t, used = 0, 0.0, 0
     prev = os.sched_getaffinity(0)
     core = min(prev)
     os.sched_setaffinity(0, {core})
     try:
         for s in specs:
-            sim = oracle.Sim(s)
+            sim = _cpu_sim(s, kind)
             a = time.perf_counter()
             sim.step(CPU_STEPS)
             t += time.perf_counter() - a
@@ -223,24 +242,25 @@ def run_cpu(specs, budget_s):
     return atoms_steps / t, used, core
 
 
-def _cpu_worker(spec):
-    import oracle
-    sim = oracle.Sim(spec)
+def _cpu_worker(args):
+    spec, kind = args
+    sim = _cpu_sim(spec, kind)
     a = time.perf_counter()
     sim.step(CPU_STEPS)
     return spec.n * CPU_STEPS, time.perf_counter() - a
 
 
-def run_cpu_all_cores(specs, n_workers):
-    """The same oracle with independent replicas spread one per worker
-    process over `n_workers` host cores: aggregate atom-steps/s over the
-    wall time of the sample (the replicas `specs`)."""
+def run_cpu_all_cores(specs, n_workers, kind):
+    """The same CPU simulation with independent replicas spread one per
+    worker process over `n_workers` host cores: aggregate atom-steps/s over
+    the wall time of the sample (the replicas `specs`)."""
     import multiprocessing as mp
     ctx = mp.get_context("spawn")
     with ctx.Pool(n_workers) as pool:
-        pool.map(_cpu_worker, specs[:n_workers])  # warm the workers (imports, library load)
+        # warm the workers (imports, library load)
+        pool.map(_cpu_worker, [(s, kind) for s in specs[:n_workers]])
         a = time.perf_counter()
-        res = pool.map(_cpu_worker, specs, chunksize=1)
+        res = pool.map(_cpu_worker, [(s, kind) for s in specs], chunksize=1)
         wall = time.perf_counter() - a
     return sum(r[0] for r in res) / wall
 
@@ -382,21 +402,35 @@ def main():
     if not args.quick and not args.no_cpu:
         # before anything touches the GPU: the worker processes of the
         # all-cores leg start from a process without device state
-        log("[bench] CPU baseline (oracle): one pinned core, then one replica per core")
-        cpu, used, core = run_cpu(specs, args.cpu_budget)
+        kind = cpu_kind(specs)
+        what = {"reference": "DMD's own translation units (oracle/_ref: its ForceEngine, "
+                             "LennardJones, bonded components, Integrator and thermostat "
+                             "driven in SimulationEngine::step order)",
+                "port": "the FP64 oracle (C restatement of DMD's loops)"}
+        loops = "O(N^2) list rebuild every 10 computes, 2 computes per step"
+        log(f"[bench] CPU baseline ({kind}): one pinned core, then one replica per core")
+        cpu, used, core = run_cpu(specs, args.cpu_budget, kind)
         n_workers = max(1, min(16, len(os.sched_getaffinity(0))))
-        cpu_all = run_cpu_all_cores(specs, n_workers)
+        cpu_all = run_cpu_all_cores(specs, n_workers, kind)
         cpu_rec = {
-            "value": cpu, "unit": "atom-steps/s", "cores": 1, "kind": "port",
-            "sample": f"first {used} replicas x {CPU_STEPS} steps of the same workload, FP64 "
-                      "oracle (DMD's loops: O(N^2) list rebuild every 10 computes, 2 "
-                      f"computes per step), pinned to host core {core}",
+            "value": cpu, "unit": "atom-steps/s", "cores": 1, "kind": kind,
+            "sample": f"first {used} replicas x {CPU_STEPS} steps of the same workload, "
+                      f"{what[kind]}, FP64 ({loops}), pinned to host core {core}",
             "host": cpu_model(), "nproc": os.cpu_count(),
             "all_cores": {"value": cpu_all, "unit": "atom-steps/s", "cores": n_workers,
                           "sample": f"all {len(specs)} replicas x {CPU_STEPS} steps, one "
                                     f"replica at a time per worker process, {n_workers} "
                                     "workers"}}
-        log(f"[bench] CPU: {cpu:.4g} atom-steps/s on 1 core, {cpu_all:.4g} on {n_workers}")
+        log(f"[bench] CPU ({kind}): {cpu:.4g} atom-steps/s on 1 core, {cpu_all:.4g} on "
+            f"{n_workers}")
+        if kind == "reference":
+            # the C restatement beside it, on one core
+            port, used_p, _ = run_cpu(specs, args.cpu_budget / 2, "port")
+            cpu_rec["port"] = {
+                "value": port, "unit": "atom-steps/s", "cores": 1, "kind": "port",
+                "sample": f"first {used_p} replicas x {CPU_STEPS} steps, {what['port']}, "
+                          f"FP64 ({loops}), pinned to host core {core}"}
+            log(f"[bench] CPU (port): {port:.4g} atom-steps/s on 1 core")
     log(f"[bench] {len(specs)} systems, {n_atoms} atoms; batched mixed: warmup {args.warmup}, "
         f"timed {args.steps}")
     eng, wall, lat, kt, breakdown, flags, tune, builds = run_batched(
