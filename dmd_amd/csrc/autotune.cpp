@@ -72,7 +72,9 @@ uint64_t mix(uint64_t h, T v)
     return fnv(h, &v, sizeof(v));
 }
 
-// what the launch shapes depend on: sizes, densities, cutoffs and terms
+// what the launch shapes depend on: sizes, densities, cutoffs and terms,
+// and the cell grid and FFT tiling the batch was made with (so the shapes
+// are timed again, not read back, after the autotuner has changed those)
 uint64_t signature(const dmdx_engine* e)
 {
     uint64_t h = 1469598103934665603ull;
@@ -90,15 +92,20 @@ uint64_t signature(const dmdx_engine* e)
         h = mix(h, D.precision);
         h = mix(h, D.physics);
     }
+    for (const auto& S : e->b.hsys) h = mix(h, S.cdiv);
+    h = mix(h, e->b.options.fft_lines);
     return h;
 }
 
+bool pair_candidate(int p, int w);
+
 // A cached line: key, pair lanes, pair width, per-atom build lanes, then
 // optionally the per-cell build switch and its waves per cell (a line
-// without them stands for the per-atom build), the row-tile setting and
-// the staged per-cell build's lanes per atom (0 when absent).
+// without them stands for the per-atom build), the row-tile setting, the
+// staged per-cell build's lanes per atom (0 when absent) and the Coulomb
+// variant's pair lanes and width (8 x 4 when absent).
 bool cache_lookup(const char* path, uint64_t key, int* pair, int* width, int* build, int* cells,
-                  int* tasks, int* rows, int* stage)
+                  int* tasks, int* rows, int* stage, int* coul_pair, int* coul_width)
 {
     if (!path || !*path) return false;
     std::ifstream f(path);
@@ -112,14 +119,17 @@ bool cache_lookup(const char* path, uint64_t key, int* pair, int* width, int* bu
             *pair = p;
             *width = w;
             *build = b;
-            int c = 0, t = 4, r = 0, g = 0;
+            int c = 0, t = 4, r = 0, g = 0, cp = 8, cw = 4;
             if (!(ss >> c >> t) || (c != 0 && c != 1)) c = 0, t = 4;
             else if (!(ss >> r) || (r != 0 && r != 4 && r != 8)) r = 0;
             else if (!(ss >> g) || (g != 0 && g != 8 && g != 16)) g = 0;
+            else if (!(ss >> cp >> cw) || !pair_candidate(cp, cw)) cp = 8, cw = 4;
             *cells = c;
             *tasks = t;
             *rows = r;
             *stage = g;
+            *coul_pair = cp;
+            *coul_width = cw;
             return true;
         }
     }
@@ -127,9 +137,10 @@ bool cache_lookup(const char* path, uint64_t key, int* pair, int* width, int* bu
 }
 
 // pair kernel candidates: (lanes per atom, entries per lane per pass) of the
-// per-atom kernel, then row tiles with 4 or 8 entries per block; list build
-// candidates: per atom with 8, 16 or 32 lanes, then per cell with 2, 4 or 8
-// waves per cell, then per cell staged in LDS with 8 or 16 lanes per atom
+// per-atom kernel (for each of its two variants), then row tiles with 4 or
+// 8 entries per block; list build candidates: per atom with 8, 16 or 32
+// lanes, then per cell with 2, 4 or 8 waves per cell, then per cell staged
+// in LDS with 8 or 16 lanes per atom
 constexpr int NPAIR = 8, NROWS = 2, NBUILD = 8, NBUILD_ATOM = 3;
 constexpr int PAIR_LANES[NPAIR] = {2, 4, 8, 4, 8, 16, 8, 16};
 constexpr int PAIR_WIDTH[NPAIR] = {8, 8, 8, 4, 4, 4, 2, 2};
@@ -138,7 +149,22 @@ constexpr int BUILD_LANES[NBUILD] = {8, 16, 32, 0, 0, 0, 0, 0};
 constexpr int BUILD_TASKS[NBUILD] = {0, 0, 0, 2, 4, 8, 4, 4};
 constexpr int BUILD_STAGE[NBUILD] = {0, 0, 0, 0, 0, 0, 8, 16};
 constexpr int T_ROWS = NPAIR, T_BUILD = NPAIR + NROWS;  // times_ms layout
-static_assert(NPAIR + NROWS + NBUILD == DMDX_TUNE_TIMES, "times_ms layout");
+constexpr int T_COUL = NPAIR + NROWS + NBUILD;
+static_assert(T_COUL + NPAIR == DMDX_TUNE_TIMES, "times_ms layout");
+
+// the launch shapes of one batch given to another
+void copy_shapes(dmdx::Dev& to, const dmdx::Dev& from)
+{
+    to.pair_tpa = from.pair_tpa;
+    to.pair_width = from.pair_width;
+    to.coul_tpa = from.coul_tpa;
+    to.coul_width = from.coul_width;
+    to.pair_rows = from.pair_rows;
+    to.build_tpa = from.build_tpa;
+    to.build_cells = from.build_cells;
+    to.build_tasks = from.build_tasks;
+    to.build_stage = from.build_stage;
+}
 
 bool pair_candidate(int p, int w)
 {
@@ -148,13 +174,14 @@ bool pair_candidate(int p, int w)
 }
 
 void cache_store(const char* path, uint64_t key, int pair, int width, int build, int cells,
-                 int tasks, int rows, int stage, const double* t)
+                 int tasks, int rows, int stage, int coul_pair, int coul_width, const double* t)
 {
     if (!path || !*path) return;
     std::ostringstream o;
-    char buf[80];
-    std::snprintf(buf, sizeof(buf), "%016llx %d %d %d %d %d %d %d  # pair ms",
-                  (unsigned long long)key, pair, width, build, cells, tasks, rows, stage);
+    char buf[96];
+    std::snprintf(buf, sizeof(buf), "%016llx %d %d %d %d %d %d %d %d %d  # pair ms",
+                  (unsigned long long)key, pair, width, build, cells, tasks, rows, stage,
+                  coul_pair, coul_width);
     o << buf;
     for (int c = 0; c < NPAIR; ++c) {
         std::snprintf(buf, sizeof(buf), " %dx%d %.4f", PAIR_LANES[c], PAIR_WIDTH[c], t[c]);
@@ -162,6 +189,11 @@ void cache_store(const char* path, uint64_t key, int pair, int width, int build,
     }
     for (int c = 0; c < NROWS; ++c) {
         std::snprintf(buf, sizeof(buf), " rows%d %.4f", ROWS_BLOCK[c], t[T_ROWS + c]);
+        o << buf;
+    }
+    o << " coulomb";
+    for (int c = 0; c < NPAIR; ++c) {
+        std::snprintf(buf, sizeof(buf), " %dx%d %.4f", PAIR_LANES[c], PAIR_WIDTH[c], t[T_COUL + c]);
         o << buf;
     }
     o << " build ms";
@@ -238,13 +270,8 @@ int time_skins_build(dmdx_engine* e, const std::vector<dmdx_system_desc>& ds,
     dmdx::Batch b;
     int rc = b.create(ds.data(), (int)ds.size(), bo);
     if (rc) return rc;
-    b.d.pair_tpa = e->b.d.pair_tpa;
-    b.d.pair_width = e->b.d.pair_width;
-    b.d.pair_rows = e->b.d.pair_rows;
-    b.d.build_tpa = e->b.d.build_tpa;
+    copy_shapes(b.d, e->b.d);
     b.d.build_cells = cells;
-    b.d.build_tasks = e->b.d.build_tasks;
-    b.d.build_stage = e->b.d.build_stage;
     using clk = std::chrono::steady_clock;
     rc = b.step(1);
     if (!rc) rc = b.sync();
@@ -366,13 +393,7 @@ int remake(dmdx_engine* e, const std::vector<dmdx_system_desc>& ds, const dmdx::
         return rc2 ? rc2 : rc;
     }
     for (size_t s = 0; s < ds.size(); ++s) e->descs[s]->d.list_skin = ds[s].list_skin;
-    e->b.d.pair_tpa = keep.pair_tpa;
-    e->b.d.pair_width = keep.pair_width;
-    e->b.d.pair_rows = keep.pair_rows;
-    e->b.d.build_tpa = keep.build_tpa;
-    e->b.d.build_cells = keep.build_cells;
-    e->b.d.build_tasks = keep.build_tasks;
-    e->b.d.build_stage = keep.build_stage;
+    copy_shapes(e->b.d, keep);
     return DMDX_OK;
 }
 
@@ -484,8 +505,7 @@ int tune_fft(dmdx_engine* e, int32_t steps, const char* cache_path, bool* change
             dmdx::Batch b;
             int rc = b.create(ds.data(), (int)ds.size(), bo);
             if (rc) return rc;
-            b.d.pair_tpa = e->b.d.pair_tpa;
-            b.d.pair_width = e->b.d.pair_width;
+            copy_shapes(b.d, e->b.d);
             double ms0, ms1;
             int64_t n0, n1;
             rc = b.step(1);
@@ -524,11 +544,14 @@ int tune_shapes(dmdx_engine* e, int32_t steps, const char* cache_path, double* t
     const int* pl = PAIR_LANES;
     const int* pw = PAIR_WIDTH;
     const uint64_t key = signature(e);
-    int cp = 0, cw = 0, cb = 0, cc = 0, ct = 4, cr = 0, cg = 0;
-    if (cache_lookup(cache_path, key, &cp, &cw, &cb, &cc, &ct, &cr, &cg) && pair_candidate(cp, cw) &&
-        (cb == 8 || cb == 16 || cb == 32) && (ct == 1 || ct == 2 || ct == 4 || ct == 8)) {
+    int cp = 0, cw = 0, cb = 0, cc = 0, ct = 4, cr = 0, cg = 0, qp = 8, qw = 4;
+    if (cache_lookup(cache_path, key, &cp, &cw, &cb, &cc, &ct, &cr, &cg, &qp, &qw) &&
+        pair_candidate(cp, cw) && (cb == 8 || cb == 16 || cb == 32) &&
+        (ct == 1 || ct == 2 || ct == 4 || ct == 8)) {
         e->b.d.pair_tpa = cp;
         e->b.d.pair_width = cw;
+        e->b.d.coul_tpa = qp;
+        e->b.d.coul_width = qw;
         e->b.d.pair_rows = cr;
         e->b.d.build_tpa = cb;
         e->b.d.build_cells = cc != 0;
@@ -548,7 +571,7 @@ int tune_shapes(dmdx_engine* e, int32_t steps, const char* cache_path, double* t
     dmdx::Batch scratch;
     int rc = scratch.create(ds.data(), (int)ds.size(), bo);
     if (rc) return rc;
-    scratch.d.pair_width = e->b.d.pair_width;
+    copy_shapes(scratch.d, e->b.d);
     const int n = steps > 0 ? steps : 5;
     double ms0, ms1;
     int64_t n0, n1;
@@ -560,28 +583,48 @@ int tune_shapes(dmdx_engine* e, int32_t steps, const char* cache_path, double* t
         if (!r) *out = (ms1 - ms0) / (double)n;
         return r;
     };
-    // pair kernel: per-step time of each per-atom shape over n steps, then
-    // of the row tiles beside the best per-atom shape
+    // pair kernel: per-step time of each per-atom shape over n steps, first
+    // of the Coulomb variant (the LJ-only systems on the engine's shape),
+    // then of the LJ-only variant beside the fastest Coulomb shape, then of
+    // the row tiles beside the fastest LJ-only shape; a variant no system
+    // uses keeps the engine's shape
     scratch.d.pair_rows = 0;
-    for (int c = 0; c < NPAIR; ++c) {
-        scratch.d.pair_tpa = pl[c];
-        scratch.d.pair_width = pw[c];
-        rc = time_pair(&t[c]);
-        if (rc) return rc;
+    auto best_of = [&](const double* tt) {
+        int b = 0;
+        for (int c = 1; c < NPAIR; ++c)
+            if (tt[c] < tt[b]) b = c;
+        return b;
+    };
+    if (scratch.d.any_coul_pairs) {
+        for (int c = 0; c < NPAIR; ++c) {
+            scratch.d.coul_tpa = pl[c];
+            scratch.d.coul_width = pw[c];
+            rc = time_pair(&t[T_COUL + c]);
+            if (rc) return rc;
+        }
+        const int bq = best_of(t + T_COUL);
+        scratch.d.coul_tpa = pl[bq];
+        scratch.d.coul_width = pw[bq];
     }
-    int bp = 0;
-    for (int c = 1; c < NPAIR; ++c)
-        if (t[c] < t[bp]) bp = c;
-    scratch.d.pair_tpa = pl[bp];
-    scratch.d.pair_width = pw[bp];
-    int br = -1;  // per-atom kernel for every system
-    for (int c = 0; c < NROWS; ++c) {
-        scratch.d.pair_rows = ROWS_BLOCK[c];
-        rc = time_pair(&t[T_ROWS + c]);
-        if (rc) return rc;
-        if (t[T_ROWS + c] < (br < 0 ? t[bp] : t[T_ROWS + br])) br = c;
+    if (scratch.d.any_plain_pairs) {
+        for (int c = 0; c < NPAIR; ++c) {
+            scratch.d.pair_tpa = pl[c];
+            scratch.d.pair_width = pw[c];
+            rc = time_pair(&t[c]);
+            if (rc) return rc;
+        }
+        const int bp = best_of(t);
+        scratch.d.pair_tpa = pl[bp];
+        scratch.d.pair_width = pw[bp];
+        int br = -1;  // per-atom kernel for every system
+        for (int c = 0; c < NROWS; ++c) {
+            scratch.d.pair_rows = ROWS_BLOCK[c];
+            rc = time_pair(&t[T_ROWS + c]);
+            if (rc) return rc;
+            if (t[T_ROWS + c] < (br < 0 ? t[bp] : t[T_ROWS + br])) br = c;
+        }
+        scratch.d.pair_rows = br < 0 ? 0 : ROWS_BLOCK[br];
     }
-    scratch.d.pair_rows = br < 0 ? 0 : ROWS_BLOCK[br];
     // list build: the candidates take turns step by step, so each sees the
     // same mix of full and partial rebuilds; time per step of the list group
     double acc[NBUILD] = {};
@@ -608,16 +651,19 @@ int tune_shapes(dmdx_engine* e, int32_t steps, const char* cache_path, double* t
         if (t[T_BUILD + c] < t[T_BUILD + bb]) bb = c;
     for (int c = 1; c < NBUILD_ATOM; ++c)
         if (t[T_BUILD + c] < t[T_BUILD + ba]) ba = c;
-    e->b.d.pair_tpa = pl[bp];
-    e->b.d.pair_width = pw[bp];
+    e->b.d.pair_tpa = scratch.d.pair_tpa;
+    e->b.d.pair_width = scratch.d.pair_width;
+    e->b.d.coul_tpa = scratch.d.coul_tpa;
+    e->b.d.coul_width = scratch.d.coul_width;
     e->b.d.pair_rows = scratch.d.pair_rows;
     e->b.d.build_tpa = BUILD_LANES[ba];
     e->b.d.build_cells = BUILD_LANES[bb] == 0;
     if (e->b.d.build_cells) e->b.d.build_tasks = BUILD_TASKS[bb];
     e->b.d.build_stage = BUILD_STAGE[bb];
     e->b.invalidate_graph();
-    cache_store(cache_path, key, pl[bp], pw[bp], BUILD_LANES[ba], e->b.d.build_cells ? 1 : 0,
-                e->b.d.build_tasks, e->b.d.pair_rows, e->b.d.build_stage, t);
+    cache_store(cache_path, key, e->b.d.pair_tpa, e->b.d.pair_width, BUILD_LANES[ba],
+                e->b.d.build_cells ? 1 : 0, e->b.d.build_tasks, e->b.d.pair_rows,
+                e->b.d.build_stage, e->b.d.coul_tpa, e->b.d.coul_width, t);
     return DMDX_OK;
 }
 

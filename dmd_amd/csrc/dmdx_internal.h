@@ -262,6 +262,10 @@ struct Dev {
     // k_nbr_build_cells
     int build_stage = 0;
     int pair_width = 8;        // mixed pair kernel: entries per lane per pass (2, 4, 8)
+    // the same for the systems with real-space Coulomb pairs (their rows are
+    // several times longer and their entries cost several times more)
+    int coul_tpa = 8;
+    int coul_width = 4;
     // mixed LJ pair forces: 0 = per-atom kernel (k_pair_mixed), 4 or 8 =
     // row-tile kernel (k_pair_rows) with that many list entries per block
     int pair_rows = 0;
@@ -347,7 +351,8 @@ struct Dev {
 hipError_t launch_build_lists(const Dev& d, hipStream_t s);
 hipError_t launch_pair(const Dev& d, hipStream_t s, int accumulate);
 hipError_t launch_pair_rows(const Dev& d, hipStream_t s, int accumulate);  // LJ-only systems
-void pair_shape(const Dev& d, int& tpa, int& w);  // effective mixed pair-kernel launch shape
+// effective mixed pair-kernel launch shape of the LJ-only or the Coulomb variant
+void pair_shape(const Dev& d, int& tpa, int& w, bool coul);
 hipError_t launch_pack(const Dev& d, hipStream_t s);
 hipError_t launch_bonded(const Dev& d, hipStream_t s);
 hipError_t launch_pme(const Dev& d, hipStream_t s);

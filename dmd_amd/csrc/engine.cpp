@@ -1638,6 +1638,14 @@ int dmdx_engine_set_tuning(dmdx_engine* e, int32_t key, int32_t value)
         if (value != 0 && value != 8 && value != 16)
             return dmdx::fail(DMDX_E_PARM, "staged per-cell build is 0 (off), 8 or 16 lanes per atom");
         e->b.d.build_stage = value;
+    } else if (key == 10) {
+        if (value != 2 && value != 4 && value != 8 && value != 16 && value != 32)
+            return dmdx::fail(DMDX_E_PARM, "Coulomb pair lanes per atom must be 2, 4, 8, 16 or 32");
+        e->b.d.coul_tpa = value;
+    } else if (key == 11) {
+        if (value != 2 && value != 4 && value != 8)
+            return dmdx::fail(DMDX_E_PARM, "Coulomb entries per lane must be 2, 4 or 8");
+        e->b.d.coul_width = value;
     } else {
         return dmdx::fail(DMDX_E_INVALID, "unknown tuning key");
     }
@@ -1659,6 +1667,8 @@ int dmdx_engine_get_tuning(dmdx_engine* e, int32_t key, int32_t* value)
     case 6: *value = d.pair_rows; break;
     case 7: *value = d.build_stage; break;
     case 8: *value = e->b.options.fft_lines; break;
+    case 10: *value = d.coul_tpa; break;
+    case 11: *value = d.coul_width; break;
     default: return dmdx::fail(DMDX_E_INVALID, "unknown tuning key");
     }
     return DMDX_OK;

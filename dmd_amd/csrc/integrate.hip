@@ -419,7 +419,11 @@ __device__ double block_range_sum(const double* part, int64_t off, int64_t n, do
 
 struct Parts {
     const double *lj, *coul, *bond, *angle, *dih, *imp, *pme, *excl, *ke;
-    int pair_div;  // atom slots per pair-energy partial (FP64: 1; mixed: one per wave)
+    // atom slots per pair-energy partial (FP64: 1; mixed: one per wave) of
+    // the LJ-only systems and of the systems with Coulomb pairs (the two
+    // variants of the mixed pair kernel have their own launch shapes); a
+    // system's partials start at its first slot
+    int pair_div, coul_div;
     // row-tile pair kernel on: LJ-only systems with a list have their LJ
     // energy as one partial per atom slot here (their Coulomb partials stay
     // zero in coul)
@@ -470,7 +474,8 @@ __global__ void __launch_bounds__(FIN_THREADS) k_finalize(const SysDev* __restri
     const bool rows = P.rows && S.has_list && !S.coul_pairs;  // uniform per workgroup
     const double* arr[9] = {rows ? P.rows : P.lj, P.coul, P.bond, P.angle, P.dih, P.imp,
                             P.pme, P.excl, P.ke};
-    const int64_t po = S.atom_off / P.pair_div, pn = S.n_pad / P.pair_div;
+    const int div = S.coul_pairs ? P.coul_div : P.pair_div;
+    const int64_t po = S.atom_off, pn = S.n_pad / div;
     const int64_t off[9] = {rows ? S.atom_off : po, po, S.bw_off, S.gw_off, S.dw_off,
                             S.iw_off, S.pw_off, S.xw_off, S.aw_off};
     const int64_t cnt[9] = {rows ? S.n_pad : pn, rows ? 0 : pn, S.bw_n, S.gw_n, S.dw_n,
@@ -766,10 +771,13 @@ hipError_t launch_kick_ke(const Dev& d, hipStream_t s)
 
 hipError_t launch_finalize(const Dev& d, hipStream_t s, int with_kinetic)
 {
-    int tpa = 1, w = 0;
-    if (d.precision != DMDX_FP64) pair_shape(d, tpa, w);
+    int tpa = 64, w = 0, ctpa = 64;
+    if (d.precision != DMDX_FP64) {
+        pair_shape(d, tpa, w, false);
+        pair_shape(d, ctpa, w, true);
+    }
     Parts P{d.p_lj, d.p_coul, d.p_bond, d.p_angle, d.p_dih, d.p_imp, d.p_pme, d.p_excl, d.p_ke,
-            d.precision == DMDX_FP64 ? 1 : 64 / tpa,
+            64 / tpa, 64 / ctpa,
             d.precision != DMDX_FP64 && d.pair_rows != 0 ? d.p_rows : nullptr};
     k_finalize<<<d.nsys, FIN_THREADS, 0, s>>>(d.sys, d.st, P, with_kinetic);
     return hipGetLastError();

@@ -432,9 +432,12 @@ __device__ __forceinline__ void k_pair_mixed_unit(int64_t unit_,
         fy[i] = fyi;
         fz[i] = fzi;
     }
+    // the partial of the wave's atoms lt .. lt + 64 / TPA - 1, at
+    // atom_off + lt / (64 / TPA): inside the system's own slot range, whatever
+    // shape the other systems' variant has
     if ((threadIdx.x & 63) == 0) {  // rows hold both orders of a pair: halved
-        e_lj_part[t / (64 / TPA)] = 0.5 * ejw;
-        e_c_part[t / (64 / TPA)] = 0.5 * ecw;
+        e_lj_part[a0 + lt / (64 / TPA)] = 0.5 * ejw;
+        e_c_part[a0 + lt / (64 / TPA)] = 0.5 * ecw;
     }
 }
 
@@ -458,13 +461,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(COUL ?
 namespace dmdx {
 
 // Launch shape of the mixed pair kernel from the tuning values: lanes per
-// atom x entries per lane pass.  8 entries per pass come with 2, 4 or 8
-// lanes; 2 or 4 entries with 4 (4 entries only), 8, 16 or 32 lanes.  Other
-// combinations take the nearest of these.
-void pair_shape(const Dev& d, int& tpa, int& w)
+// atom x entries per lane pass, for the LJ-only variant or (coul) the
+// variant with real-space Coulomb pairs.  8 entries per pass come with 2, 4
+// or 8 lanes; 2 or 4 entries with 4 (4 entries only), 8, 16 or 32 lanes.
+// Other combinations take the nearest of these.
+void pair_shape(const Dev& d, int& tpa, int& w, bool coul)
 {
-    w = d.pair_width;
-    tpa = d.pair_tpa;
+    w = coul ? d.coul_width : d.pair_width;
+    tpa = coul ? d.coul_tpa : d.pair_tpa;
     if (w == 8) {
         if (tpa > 8) tpa = 8;
     } else {
@@ -478,7 +482,7 @@ static hipError_t launch_pair_mixed(const Dev& d, hipStream_t s, int accumulate)
 {
     const int BS = 256;
     int tpa, w;
-    pair_shape(d, tpa, w);
+    pair_shape(d, tpa, w, COUL);
     const int64_t units = blocks_for(d.nslots * tpa, BS);
 #define DMDX_PAIR_ARGS                                                                       \
     d.sys, d.st, d.slot_sys, d.nslots, d.order, d.nn, d.nbr, d.lpos, d.typ_s, d.lj_c12, d.lj_c6, \

@@ -293,6 +293,8 @@ int merge(dmdx_server* srv)
         // launch shapes of the first cohort (all cohorts use the defaults)
         b->d.pair_tpa = srv->cohorts[cs[0]]->b->d.pair_tpa;
         b->d.pair_width = srv->cohorts[cs[0]]->b->d.pair_width;
+        b->d.coul_tpa = srv->cohorts[cs[0]]->b->d.coul_tpa;
+        b->d.coul_width = srv->cohorts[cs[0]]->b->d.coul_width;
         b->d.pair_rows = srv->cohorts[cs[0]]->b->d.pair_rows;
         b->d.build_tpa = srv->cohorts[cs[0]]->b->d.build_tpa;
         for (size_t k = 0; k < ids.size(); ++k) {

@@ -182,7 +182,8 @@ class Engine:
                    pair_width: int | None = None, build_cells: bool | None = None,
                    build_tasks: int | None = None, tune_skin: bool | None = None,
                    pair_rows: int | None = None, build_stage: int | None = None,
-                   cell_div: int | None = None):
+                   cell_div: int | None = None, coul_lanes: int | None = None,
+                   coul_width: int | None = None):
         """Launch shapes of the mixed kernels (dmdx_engine_set_tuning);
         build_cells: list built per cell (default) or per atom; build_tasks:
         waves per cell of the per-cell build (1, 2, 4, 8); pair_rows: LJ-only
@@ -192,7 +193,9 @@ class Engine:
         lanes per atom, or 0 (default) for candidates shared in registers by
         build_tasks waves; cell_div (before the first step): cells of the
         list radius (1) or half of it (2) for every mixed system with a list,
-        0 for the density rule."""
+        0 for the density rule; coul_lanes, coul_width: the pair kernel's
+        shape for the systems with real-space Coulomb pairs (pair_lanes and
+        pair_width apply to the LJ-only systems)."""
         if cell_div is not None:  # before the first step: the batch is made again
             _abi.check(self.lib.dmdx_engine_set_tuning(self.h, 9, int(cell_div)))
         if build_stage is not None:
@@ -211,6 +214,10 @@ class Engine:
             _abi.check(self.lib.dmdx_engine_set_tuning(self.h, 1, int(build_lanes)))
         if pair_width is not None:
             _abi.check(self.lib.dmdx_engine_set_tuning(self.h, 2, int(pair_width)))
+        if coul_lanes is not None:
+            _abi.check(self.lib.dmdx_engine_set_tuning(self.h, 10, int(coul_lanes)))
+        if coul_width is not None:
+            _abi.check(self.lib.dmdx_engine_set_tuning(self.h, 11, int(coul_width)))
 
     def autotune(self, steps: int = 5, cache: str | None = None) -> dict:
         """Time launch shapes of the mixed pair kernel (lanes per atom x
@@ -222,10 +229,13 @@ class Engine:
         build, "pair_rows" the row-tile setting of LJ-only systems (0: the
         per-atom kernel), "skins" the widened skins, "cell_div" how many
         systems have cells of the list radius (1) or half of it (2),
-        "fft_lines" the FFT tiling (0: the grid-size rule)."""
+        "fft_lines" the FFT tiling (0: the grid-size rule), "coul_lanes" and
+        "coul_width" the shape of the systems with Coulomb pairs ("coul_ms":
+        their timings; -1 where no system has such pairs)."""
         p, w, b = C.c_int32(), C.c_int32(), C.c_int32()
         npr = len(PAIR_SHAPES) + len(ROW_SHAPES)
-        t = (C.c_double * (npr + len(BUILD_SHAPES)))()
+        nco = npr + len(BUILD_SHAPES)
+        t = (C.c_double * (nco + len(PAIR_SHAPES)))()
         _abi.check(self.lib.dmdx_engine_autotune(
             self.h, int(steps), cache.encode() if cache else None, C.byref(p), C.byref(w),
             C.byref(b), t))
@@ -249,7 +259,9 @@ class Engine:
                              for c in (1, 2)},
                 "pair_ms": {k: t[i] for i, k in enumerate(PAIR_SHAPES)},
                 "rows_ms": {k: t[len(PAIR_SHAPES) + i] for i, k in enumerate(ROW_SHAPES)},
-                "build_ms": {k: t[npr + i] for i, k in enumerate(BUILD_SHAPES)}}
+                "build_ms": {k: t[npr + i] for i, k in enumerate(BUILD_SHAPES)},
+                "coul_lanes": self.get_tuning(10), "coul_width": self.get_tuning(11),
+                "coul_ms": {k: t[nco + i] for i, k in enumerate(PAIR_SHAPES)}}
 
     def cell_div(self, sys: int = 0) -> int:
         """Cells per list radius of system `sys`'s list grid (1 or 2;

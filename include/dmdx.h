@@ -415,7 +415,10 @@ int dmdx_engine_kernel_time(dmdx_engine *e, int32_t group, double *total_ms, int
  * for PME (4, 8, 16; 0 = the grid-size rule).  Key 9 (write only, before
  * the first step; the batch is made again) = cells per list radius of every
  * mixed system with a list: 1, 2, or 0 for the density rule (read per system
- * with dmdx_engine_get_cell_div).
+ * with dmdx_engine_get_cell_div).  Keys 10 and 11 = lanes per atom and
+ * entries per lane pass of the pair kernel for the systems with real-space
+ * Coulomb pairs (default 8 x 4: their rows are several times longer than
+ * LJ rows); keys 0 and 2 then apply to the LJ-only systems.
  * The list rows do not depend on keys 1, 3, 4 or 7.  Pair shapes
  * (lanes x entries): 2x8, 4x8, 8x8, 4x4, 8x4, 16x4, 32x4, 8x2, 16x2, 32x2;
  * another combination runs as the nearest of these (8 lanes where fewer
@@ -435,14 +438,16 @@ int dmdx_engine_get_tuning(dmdx_engine *e, int32_t key, int32_t *value);
  * cache_path (may be NULL): text file of earlier results keyed by a
  * signature of the systems; a hit applies the cached choice without timing.
  * times_ms (DMDX_TUNE_TIMES doubles, may be NULL): per-step ms of the pair
- * kernels at 2x8, 4x8, 8x8, 4x4, 8x4, 16x4, 8x2, 16x2 (per-atom kernel for
- * every system), then with LJ-only systems on row tiles of 4 and 8 entries
- * per block (tuning key 6) beside the fastest of those shapes, then per-step
- * ms of the list build at 8, 16, 32 lanes per atom, per cell at 2, 4, 8
- * waves per cell and per cell staged in LDS at 8, 16 lanes per atom (-1
- * when not timed); dmdx_engine_get_tuning reads the chosen row tiles and
- * build (keys 6, 1, 3, 4, 7).  FP64 engines have
- * nothing to tune.
+ * kernels with the LJ-only systems' shape (keys 0, 2) at 2x8, 4x8, 8x8,
+ * 4x4, 8x4, 16x4, 8x2, 16x2, then with LJ-only systems on row tiles of 4
+ * and 8 entries per block (tuning key 6) beside the fastest of those
+ * shapes, then per-step ms of the list build at 8, 16, 32 lanes per atom,
+ * per cell at 2, 4, 8 waves per cell and per cell staged in LDS at 8, 16
+ * lanes per atom, then of the pair kernels with the Coulomb systems' shape
+ * (keys 10, 11) at the eight shapes, timed first, with the LJ-only shape
+ * unchanged (-1 when not timed, e.g. for a variant no system uses);
+ * dmdx_engine_get_tuning reads the chosen Coulomb shape, row tiles and
+ * build (keys 10, 11, 6, 1, 3, 4, 7).  FP64 engines have nothing to tune.
  * First, on an engine that has not stepped or been changed since creation
  * (and unless tuning key 5 is 0): the list skin of systems that rebuild by
  * the displacement criterion (rebuild_interval -1), scaled by 1, 1.3, 1.6 or
@@ -456,7 +461,7 @@ int dmdx_engine_get_tuning(dmdx_engine *e, int32_t key, int32_t *value);
  * Mixed-precision results depend on the shape, the skin, the cells and the
  * FFT tiling only through FP32 summation order (well inside the
  * mixed-precision tolerance). */
-#define DMDX_TUNE_TIMES 18
+#define DMDX_TUNE_TIMES 26
 int dmdx_engine_autotune(dmdx_engine *e, int32_t steps, const char *cache_path,
                          int32_t *pair_lanes, int32_t *pair_width, int32_t *build_lanes,
                          double *times_ms);

@@ -33,6 +33,9 @@ def test_autotune_choice_cache_and_state(tmp_path):
         assert rm[f"rows{r['pair_rows']}"] == min(min(rm.values()), min(pm.values()))
     else:
         assert min(pm.values()) <= min(rm.values())
+    # no system has Coulomb pairs: that variant is not timed and keeps 8x4
+    assert all(v == -1.0 for v in r["coul_ms"].values())
+    assert (r["coul_lanes"], r["coul_width"]) == (8, 4)
     bm = r["build_ms"]
     assert bm[r["build"]] == min(bm.values())
     assert bm[f"atom{r['build_lanes']}"] == min(bm[k] for k in bm if k.startswith("atom"))
@@ -154,9 +157,17 @@ def test_autotune_cells_and_fft(tmp_path):
     print({k: r[k] for k in ("fft_lines", "cell_div", "skins")}, flush=True)
     assert r["fft_lines"] in (0, 4, 8, 16)
     assert sum(r["cell_div"].values()) == len(specs)
+    # both pair-kernel variants timed, each with its own fastest shape
+    cm, pm = r["coul_ms"], r["pair_ms"]
+    print({"coul": cm, "pair": pm}, flush=True)
+    assert all(v > 0 for v in cm.values()) and all(v > 0 for v in pm.values())
+    assert cm[f"{r['coul_lanes']}x{r['coul_width']}"] == min(cm.values())
+    assert pm[f"{r['pair_lanes']}x{r['pair_width']}"] == min(pm.values())
     e2 = Engine(specs)
     r2 = e2.autotune(steps=3, cache=cache)
     assert r2["fft_lines"] == r["fft_lines"]
+    assert (r2["coul_lanes"], r2["coul_width"], r2["pair_lanes"], r2["pair_width"]) == \
+        (r["coul_lanes"], r["coul_width"], r["pair_lanes"], r["pair_width"])
     assert [e2.cell_div(k) for k in range(3)] == [e.cell_div(k) for k in range(3)]
     assert [e2.list_skin(k) for k in range(3)] == [e.list_skin(k) for k in range(3)]
     e.compute()

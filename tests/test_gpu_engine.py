@@ -340,7 +340,7 @@ def test_mixed_every_pair_shape(kind):
     first = None
     for lanes, width in PAIR_SHAPES:
         e = engine(spec)
-        e.set_tuning(pair_lanes=lanes, pair_width=width)
+        e.set_tuning(pair_lanes=lanes, pair_width=width, coul_lanes=lanes, coul_width=width)
         e.compute()
         F = e.forces()
         err = rms_rel(F, F_ref)
@@ -350,6 +350,45 @@ def test_mixed_every_pair_shape(kind):
             first = F
         else:
             assert rms_rel(F, first) <= 2e-6, (lanes, width)
+
+
+@pytest.mark.parametrize("lj_shape,coul_shape", [((4, 8), (8, 4)), ((2, 8), (32, 2)),
+                                                 ((16, 4), (4, 8))])
+def test_mixed_split_pair_shapes(lj_shape, coul_shape):
+    """LJ-only and Coulomb systems in one batch with different pair-kernel
+    shapes (tuning keys 0/2 and 10/11): each system's forces and potential
+    energy meet the FP64 bar -- the per-wave energy partials of the two
+    variants have different layouts, and each system's are summed with its
+    own -- and a Coulomb system's results do not depend on the LJ shape."""
+    specs = [W.water_box(n_side=8, seed=3), jittered_argon(skin=0.1).with_(physics=CO),
+             W.water_box(n_side=9, seed=4)]
+    specs = [s.with_(precision=_abi.MIXED) for s in specs]
+    say(f"case: split pair shapes LJ {lj_shape} Coulomb {coul_shape}, {len(specs)} systems")
+    e = engine(specs)
+    e.set_tuning(pair_lanes=lj_shape[0], pair_width=lj_shape[1], coul_lanes=coul_shape[0],
+                 coul_width=coul_shape[1])
+    e.step(5)
+    e.compute()
+    other = engine(specs)
+    other.set_tuning(pair_lanes=coul_shape[0], pair_width=coul_shape[1],
+                     coul_lanes=coul_shape[0], coul_width=coul_shape[1])
+    other.step(5)
+    other.compute()
+    for k, s in enumerate(specs):
+        f64 = engine(s.with_(precision=_abi.FP64, pos=e.positions(k), vel=e.velocities(k)))
+        f64.compute()
+        err = rms_rel(e.forces(k), f64.forces(0))
+        sc = e.scalars(k)
+        pe, pr = sc["pe"], f64.scalars(0)["pe"]
+        # FP32 error of a PE whose terms cancel scales with the terms
+        scale = max(abs(pr), sum(abs(v) for v in sc["terms"].values()))
+        say(f"  system {k} ({s.name}): force rms rel {err:.2e}, pe {pe:.9g} vs {pr:.9g} "
+            f"(terms {scale:.4g})")
+        assert err <= 1e-5, k
+        assert abs(pe - pr) <= 1e-5 * scale, (k, pe, pr, scale)
+        if s.coulomb_type != 0:
+            assert np.array_equal(e.positions(k), other.positions(k)), k
+            assert e.scalars(k)["pe"] == other.scalars(k)["pe"], k
 
 
 def test_mixed_chain_forces_after_steps():
