@@ -2,15 +2,18 @@
 //
 // Replaces CoulombPME::compute (src/force/coulomb_pme.cpp:287-318) and its
 // FFTW calls (:154, :198).  Per compute:
-//   k_pme_zero      clear the fixed-point charge grid
+//   k_pme_zero      clear the fixed-point charge grid (only when some system
+//                   spreads with atomics: owner-computes blocks store every
+//                   point of their grid)
 //   k_pme_spread_tiles / k_pme_spread_gather  order-4 B-spline charge
 //                   spreading (:79-136) in 64-bit fixed point: integer
 //                   atomics staged in LDS per tile of cell-ordered atoms, or
 //                   owner-computes grid blocks that gather the atoms of the
 //                   nearby list cells (no global atomics); both give the
 //                   same grid
-//   k_pme_convert   fixed point -> complex grid
-//   k_fft_pass x,y  forward 1D FFT passes along x then y
+//   k_fft_pass x,y  forward 1D FFT passes along x then y; the x pass reads
+//                   the fixed-point grid and converts it as it loads its lines
+//                   (k_pme_convert is the stand-alone conversion)
 //   k_fft_pass z+   forward z pass, influence function G(m) with the energy
 //                   0.5 G |Q(m)|^2 (:143-201), inverse z pass -- the whole
 //                   convolution of each z line happens in LDS
@@ -596,7 +599,8 @@ __device__ __forceinline__ void k_fft_pass_unit(int64_t unit_, const FftTile* __
                            const SysState* __restrict__ st, const FftPlan* __restrict__ plans,
                            Cx<R>* __restrict__ grid, const Cx<R>* __restrict__ twiddles,
                            const double* __restrict__ bmod, const float* __restrict__ ginf,
-                           int dim, int sign, int conv, double* __restrict__ epart)
+                           int dim, int sign, int conv, double* __restrict__ epart,
+                           const unsigned long long* __restrict__ qin)
 {
     extern __shared__ unsigned char smem_raw[];
     const FftTile T = tiles[unit_];
@@ -628,7 +632,13 @@ __device__ __forceinline__ void k_fft_pass_unit(int64_t unit_, const FftTile* __
         return ((int64_t)k * Ky + yy) * Kx + xx;
     };
     const int tot = nl * K;
-    if (dim == 0) {
+    if (dim == 0 && qin) {
+        // forward x pass: the fixed-point charge grid, converted as
+        // k_pme_convert converts it
+        const unsigned long long* Q = qin + S.grid_off + (int64_t)T.line0 * Kx;
+        for (int e = threadIdx.x; e < tot; e += blockDim.x)
+            bufA[e] = {R((double)(long long)Q[e] * (1.0 / QGRID_SCALE)), R(0)};
+    } else if (dim == 0) {
         for (int e = threadIdx.x; e < tot; e += blockDim.x) bufA[e] = G[(int64_t)T.line0 * Kx + e];
     } else {
         for (int e = threadIdx.x; e < tot; e += blockDim.x) {
@@ -704,12 +714,12 @@ __global__ void __launch_bounds__(256) k_fft_pass(const FftTile* __restrict__ ti
                            Cx<R>* __restrict__ grid, const Cx<R>* __restrict__ twiddles,
                            const double* __restrict__ bmod, const float* __restrict__ ginf,
                            int dim, int sign, int conv, double* __restrict__ epart,
-                           int64_t nunits_)
+                           const unsigned long long* __restrict__ qin, int64_t nunits_)
 {
     DMDX_FOR_UNITS(u_, nunits_)
     {
         k_fft_pass_unit<R>(u_, tiles, sys, st, plans, grid, twiddles, bmod, ginf, dim, sign, conv,
-                           epart);
+                           epart, qin);
         __syncthreads();  // LDS is reused by the next unit
     }
 }
@@ -726,7 +736,7 @@ static hipError_t launch_pme_t(const Dev& d, hipStream_t s)
     Cx<R>* grid = reinterpret_cast<Cx<R>*>(d.grid);
     const Cx<R>* tw = reinterpret_cast<const Cx<R>*>(d.tw);
     const int64_t gu = blocks_for(d.grid_total, BS);
-    k_pme_zero<<<grid_for(gu), BS, 0, s>>>(d.st, d.grid_sys, d.grid_total, d.qgrid, gu);
+    if (d.nstiles) k_pme_zero<<<grid_for(gu), BS, 0, s>>>(d.st, d.grid_sys, d.grid_total, d.qgrid, gu);
     if (d.nstiles)
         k_pme_spread_tiles<R><<<grid_for(d.nstiles), BS, 0, s>>>(d.stiles, d.sys, d.st, d.order,
                                                                    d.x, d.y, d.z, d.q, d.qgrid,
@@ -735,7 +745,9 @@ static hipError_t launch_pme_t(const Dev& d, hipStream_t s)
         k_pme_spread_gather<R><<<grid_for(d.ngtiles), BS, 0, s>>>(
             d.gtiles, d.sys, d.st, d.ccount, d.cstart, d.order, d.x, d.y, d.z, d.q, d.qgrid,
             (int64_t)d.ngtiles);
-    k_pme_convert<R><<<grid_for(gu), BS, 0, s>>>(d.st, d.grid_sys, d.grid_total, d.qgrid, grid, gu);
+    // the forward x pass converts the fixed-point grid as it loads it
+    if (!d.ntiles[0])
+        k_pme_convert<R><<<grid_for(gu), BS, 0, s>>>(d.st, d.grid_sys, d.grid_total, d.qgrid, grid, gu);
     // forward x, forward y, forward z + convolution + inverse z, inverse y, inverse x
     const int dims[5] = {0, 1, 2, 1, 0};
     const int signs[5] = {-1, -1, -1, +1, +1};
@@ -744,7 +756,7 @@ static hipError_t launch_pme_t(const Dev& d, hipStream_t s)
         if (!d.ntiles[dim]) continue;
         k_fft_pass<R><<<grid_for(d.ntiles[dim]), 256, d.lds_bytes[dim], s>>>(
             d.tiles[dim], d.sys, d.st, d.plans, grid, tw, d.bmod, d.ginf, dim, signs[p], p == 2 ? 1 : 0,
-            d.p_pme, (int64_t)(d.ntiles[dim]));
+            d.p_pme, p == 0 ? d.qgrid : nullptr, (int64_t)(d.ntiles[dim]));
     }
     return hipGetLastError();
 }
