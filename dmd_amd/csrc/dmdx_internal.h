@@ -213,6 +213,13 @@ struct GridTile {
     int32_t g0[3];
 };
 
+// one xy plane (fixed z) of a system's grid, transformed along x and y in
+// LDS by one workgroup (k_fft_plane)
+struct PlaneTile {
+    int32_t sys;
+    int32_t z;
+};
+
 struct FftPlan {  // per system and dimension
     int32_t K;
     int32_t nrad;
@@ -332,6 +339,11 @@ struct Dev {
     FftTile* tiles[3] = {nullptr, nullptr, nullptr};
     int ntiles[3] = {0, 0, 0};
     size_t lds_bytes[3] = {0, 0, 0};
+    // mixed path: systems whose xy planes fit in LDS run their x and y
+    // passes as one plane per workgroup (they have no x or y tiles)
+    PlaneTile* ptiles = nullptr;
+    int nptiles = 0;
+    size_t plane_lds = 0;
     // partial sums (one per wave / PME tile)
     // pair energies: one partial per pair-kernel wave (per-atom kernel) or
     // per atom slot (row-tile kernel, p_rows)

@@ -566,7 +566,24 @@ int Batch::create(const dmdx_system_desc* descs, int nsys, const BatchOptions& o
     std::vector<double> twd, bmod;
     std::vector<FftPlan> plans(3 * nsys);
     std::vector<FftTile> tiles[3];
+    std::vector<PlaneTile> ptiles;
     const size_t cx = prec == DMDX_FP64 ? 16 : 8;
+    // mixed path: an xy plane (two buffers and the x and y twiddles) within
+    // 64 KiB of LDS can be transformed along x and y by one workgroup.  That
+    // saves two launches and a grid round trip per direction, but gives a
+    // system only K_z workgroups: by default (opt.fft_planes 2) planes are
+    // used when the batch has at least one per CU (a single 48^3 system is
+    // faster on line tiles, a served cohort of several water boxes on
+    // planes).
+    auto plane_bytes = [&](const SysDev& S) {
+        return (2 * (size_t)S.K[0] * S.K[1] + S.K[0] + S.K[1]) * cx;
+    };
+    int64_t plane_count = 0;
+    for (int s = 0; s < nsys; ++s)
+        if (hsys[s].pme && prec != DMDX_FP64 && plane_bytes(hsys[s]) <= 64 * 1024)
+            plane_count += hsys[s].K[2];
+    const bool use_planes =
+        opt.fft_planes == 1 ? plane_count > 0 : (opt.fft_planes == 2 && plane_count >= 256);
     for (int s = 0; s < nsys; ++s) {
         SysDev& S = hsys[s];
         if (!S.pme) continue;
@@ -594,9 +611,16 @@ int Batch::create(const dmdx_system_desc* descs, int nsys, const BatchOptions& o
                 bmod.push_back(B);
             }
         }
+        const size_t plane = plane_bytes(S);
+        const bool planes = use_planes && prec != DMDX_FP64 && plane <= 64 * 1024;
+        if (planes) {
+            for (int z = 0; z < S.K[2]; ++z) ptiles.push_back(PlaneTile{s, z});
+            d.plane_lds = std::max(d.plane_lds, plane);
+        }
         // tiles: lines along dim k are indexed by the other two coordinates
         S.pw_off = pw_total;
         for (int k = 0; k < 3; ++k) {
+            if (planes && k < 2) continue;
             int K = S.K[k];
             int nlines = (k == 0) ? S.K[1] * S.K[2] : (k == 1 ? S.K[0] * S.K[2] : S.K[0] * S.K[1]);
             // lines per workgroup: enough workgroups to spread a small grid
@@ -626,6 +650,7 @@ int Batch::create(const dmdx_system_desc* descs, int nsys, const BatchOptions& o
         }
     }
     for (int k = 0; k < 3; ++k) d.ntiles[k] = (int)tiles[k].size();
+    d.nptiles = (int)ptiles.size();
     // mixed path: the influence function of a fixed box, tabulated once
     // (same expression as the on-the-fly FP64 form in k_fft_pass)
     std::vector<float> ginf;
@@ -897,6 +922,7 @@ int Batch::create(const dmdx_system_desc* descs, int nsys, const BatchOptions& o
     TRY(dev_upload(&d.bmod, bmod, owned));
     TRY(dev_upload(&d.plans, plans, owned));
     for (int k = 0; k < 3; ++k) TRY(dev_upload(&d.tiles[k], tiles[k], owned));
+    TRY(dev_upload(&d.ptiles, ptiles, owned));
     const int64_t nw = slot / WAVE;
     TRY(dev_alloc_zero(&d.p_lj, slot, owned));
     TRY(dev_alloc_zero(&d.p_coul, slot, owned));
@@ -1634,6 +1660,16 @@ int dmdx_engine_set_tuning(dmdx_engine* e, int32_t key, int32_t value)
         dmdx::BatchOptions opt = e->b.options;
         opt.cdiv.assign(e->descs.size(), value);
         return dmdx_engine_remake(e, opt);
+    } else if (key == 12) {
+        // xy planes of the mixed PME transforms, for a batch that has not
+        // left its initial state (the batch is made again)
+        if (value < 0 || value > 2)
+            return dmdx::fail(DMDX_E_PARM, "FFT planes are 0 (never), 1 (where they fit) or 2 (rule)");
+        if (!e->b.pristine)
+            return dmdx::fail(DMDX_E_INVALID, "the FFT planes are set before the first step");
+        dmdx::BatchOptions opt = e->b.options;
+        opt.fft_planes = value;
+        return dmdx_engine_remake(e, opt);
     } else if (key == 7) {
         if (value != 0 && value != 8 && value != 16)
             return dmdx::fail(DMDX_E_PARM, "staged per-cell build is 0 (off), 8 or 16 lanes per atom");
@@ -1669,6 +1705,8 @@ int dmdx_engine_get_tuning(dmdx_engine* e, int32_t key, int32_t* value)
     case 8: *value = e->b.options.fft_lines; break;
     case 10: *value = d.coul_tpa; break;
     case 11: *value = d.coul_width; break;
+    case 12: *value = e->b.options.fft_planes; break;
+    case 13: *value = d.nptiles > 0 ? 1 : 0; break;
     default: return dmdx::fail(DMDX_E_INVALID, "unknown tuning key");
     }
     return DMDX_OK;

@@ -36,6 +36,10 @@ struct BatchOptions {
     // PME: FFT lines per workgroup tile (4, 8 or 16; 0: the grid-size rule),
     // within the LDS limit
     int fft_lines = 0;
+    // mixed PME: x and y passes as one xy plane per workgroup (planes within
+    // 64 KiB of LDS): 0 never, 1 wherever a plane fits, 2 when the batch has
+    // at least 256 such planes (one per CU)
+    int fft_planes = 2;
     bool profile = false;      // HIP events around kernel groups
     uint32_t profile_mask = ~0u;  // which groups (bit = group) when profiling
     // allocation-free mode: device arrays are carved from `arena` (capacity

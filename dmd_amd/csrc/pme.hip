@@ -13,11 +13,13 @@
 //                   same grid
 //   k_fft_pass x,y  forward 1D FFT passes along x then y; the x pass reads
 //                   the fixed-point grid and converts it as it loads its lines
-//                   (k_pme_convert is the stand-alone conversion)
+//                   (k_pme_convert is the stand-alone conversion); on the
+//                   mixed path, systems whose xy plane fits in LDS run both
+//                   as k_fft_plane, one workgroup per plane
 //   k_fft_pass z+   forward z pass, influence function G(m) with the energy
 //                   0.5 G |Q(m)|^2 (:143-201), inverse z pass -- the whole
 //                   convolution of each z line happens in LDS
-//   k_fft_pass y,x  inverse passes
+//   k_fft_pass y,x  inverse passes (k_fft_plane for the plane systems)
 //   (gather)        in k_gather (integrate.hip), per atom
 // The grid is complex (real input, imaginary part 0) with x fastest.  The FFT
 // is a mixed-radix Stockham autosort transform (Govindaraju et al. 2008,
@@ -724,6 +726,92 @@ __global__ void __launch_bounds__(256) k_fft_pass(const FftTile* __restrict__ ti
     }
 }
 
+// One xy plane (fixed z) of a system's grid per workgroup, in LDS: the
+// forward x then y passes (sign -1; the fixed-point grid is converted as it
+// loads, as in the x pass of k_fft_pass) or the inverse y then x passes
+// (sign +1), with the plane transposed in LDS between the two.  Every line
+// goes through the same Stockham stages and twiddles as in k_fft_pass, so
+// the plane holds what the two separate passes would leave.
+template <class R>
+__device__ __forceinline__ void k_fft_plane_unit(int64_t unit_, const PlaneTile* __restrict__ tiles,
+                                                 const SysDev* __restrict__ sys,
+                                                 const SysState* __restrict__ st,
+                                                 const FftPlan* __restrict__ plans,
+                                                 Cx<R>* __restrict__ grid,
+                                                 const Cx<R>* __restrict__ twiddles, int sign,
+                                                 const unsigned long long* __restrict__ qin)
+{
+    extern __shared__ unsigned char smem_raw[];
+    const PlaneTile T = tiles[unit_];
+    const int s = T.sys;
+    if (st[s].reuse || !st[s].active) return;
+    const SysDev& S = sys[s];
+    const int Kx = S.K[0], Ky = S.K[1], n = Kx * Ky;
+    const FftPlan& PX = plans[3 * s];
+    const FftPlan& PY = plans[3 * s + 1];
+    Cx<R>* a = reinterpret_cast<Cx<R>*>(smem_raw);
+    Cx<R>* b = a + n;
+    Cx<R>* twx = b + n;
+    Cx<R>* twy = twx + Kx;
+    for (int e = threadIdx.x; e < Kx; e += blockDim.x) twx[e] = twiddles[S.tw_off[0] + e];
+    for (int e = threadIdx.x; e < Ky; e += blockDim.x) twy[e] = twiddles[S.tw_off[1] + e];
+    const int64_t off = S.grid_off + (int64_t)T.z * n;  // the plane: y rows of Kx
+    Cx<R>* G = grid + off;
+    if (qin) {
+        for (int e = threadIdx.x; e < n; e += blockDim.x)
+            a[e] = {R((double)(long long)qin[off + e] * (1.0 / QGRID_SCALE)), R(0)};
+    } else {
+        for (int e = threadIdx.x; e < n; e += blockDim.x) a[e] = G[e];
+    }
+    __syncthreads();
+    if (sign < 0) {
+        // x lines (row y at y Kx), then y lines (column x at x Ky)
+        Cx<R>* r = stockham_lds<R>(a, b, Ky, PX, twx, sign);
+        Cx<R>* o = (r == a) ? b : a;
+        for (int e = threadIdx.x; e < n; e += blockDim.x) {
+            const int y = e / Kx, x = e - y * Kx;
+            o[x * Ky + y] = r[e];
+        }
+        __syncthreads();
+        Cx<R>* r2 = stockham_lds<R>(o, r, Kx, PY, twy, sign);
+        for (int e = threadIdx.x; e < n; e += blockDim.x) {
+            const int y = e / Kx, x = e - y * Kx;
+            G[e] = r2[x * Ky + y];
+        }
+    } else {
+        for (int e = threadIdx.x; e < n; e += blockDim.x) {
+            const int y = e / Kx, x = e - y * Kx;
+            b[x * Ky + y] = a[e];
+        }
+        __syncthreads();
+        Cx<R>* r = stockham_lds<R>(b, a, Kx, PY, twy, sign);
+        Cx<R>* o = (r == a) ? b : a;
+        for (int e = threadIdx.x; e < n; e += blockDim.x) {
+            const int y = e / Kx, x = e - y * Kx;
+            o[e] = r[x * Ky + y];
+        }
+        __syncthreads();
+        Cx<R>* r2 = stockham_lds<R>(o, r, Ky, PX, twx, sign);
+        for (int e = threadIdx.x; e < n; e += blockDim.x) G[e] = r2[e];
+    }
+}
+
+template <class R>
+__global__ void __launch_bounds__(256) k_fft_plane(const PlaneTile* __restrict__ tiles,
+                                                   const SysDev* __restrict__ sys,
+                                                   const SysState* __restrict__ st,
+                                                   const FftPlan* __restrict__ plans,
+                                                   Cx<R>* __restrict__ grid,
+                                                   const Cx<R>* __restrict__ twiddles, int sign,
+                                                   const unsigned long long* __restrict__ qin,
+                                                   int64_t nunits_)
+{
+    DMDX_FOR_UNITS(u_, nunits_)
+    {
+        k_fft_plane_unit<R>(u_, tiles, sys, st, plans, grid, twiddles, sign, qin);
+        __syncthreads();  // LDS is reused by the next unit
+    }
+}
 
 }  // namespace dmdx
 
@@ -745,9 +833,13 @@ static hipError_t launch_pme_t(const Dev& d, hipStream_t s)
         k_pme_spread_gather<R><<<grid_for(d.ngtiles), BS, 0, s>>>(
             d.gtiles, d.sys, d.st, d.ccount, d.cstart, d.order, d.x, d.y, d.z, d.q, d.qgrid,
             (int64_t)d.ngtiles);
-    // the forward x pass converts the fixed-point grid as it loads it
-    if (!d.ntiles[0])
+    // the forward x pass and the forward planes convert the fixed-point grid
+    // as they load it
+    if (!d.ntiles[0] && !d.nptiles)
         k_pme_convert<R><<<grid_for(gu), BS, 0, s>>>(d.st, d.grid_sys, d.grid_total, d.qgrid, grid, gu);
+    if (d.nptiles)
+        k_fft_plane<R><<<grid_for(d.nptiles), 256, d.plane_lds, s>>>(
+            d.ptiles, d.sys, d.st, d.plans, grid, tw, -1, d.qgrid, (int64_t)d.nptiles);
     // forward x, forward y, forward z + convolution + inverse z, inverse y, inverse x
     const int dims[5] = {0, 1, 2, 1, 0};
     const int signs[5] = {-1, -1, -1, +1, +1};
@@ -758,6 +850,9 @@ static hipError_t launch_pme_t(const Dev& d, hipStream_t s)
             d.tiles[dim], d.sys, d.st, d.plans, grid, tw, d.bmod, d.ginf, dim, signs[p], p == 2 ? 1 : 0,
             d.p_pme, p == 0 ? d.qgrid : nullptr, (int64_t)(d.ntiles[dim]));
     }
+    if (d.nptiles)
+        k_fft_plane<R><<<grid_for(d.nptiles), 256, d.plane_lds, s>>>(
+            d.ptiles, d.sys, d.st, d.plans, grid, tw, +1, nullptr, (int64_t)d.nptiles);
     return hipGetLastError();
 }
 

@@ -183,7 +183,7 @@ class Engine:
                    build_tasks: int | None = None, tune_skin: bool | None = None,
                    pair_rows: int | None = None, build_stage: int | None = None,
                    cell_div: int | None = None, coul_lanes: int | None = None,
-                   coul_width: int | None = None):
+                   coul_width: int | None = None, fft_planes: int | None = None):
         """Launch shapes of the mixed kernels (dmdx_engine_set_tuning);
         build_cells: list built per cell (default) or per atom; build_tasks:
         waves per cell of the per-cell build (1, 2, 4, 8); pair_rows: LJ-only
@@ -195,9 +195,14 @@ class Engine:
         list radius (1) or half of it (2) for every mixed system with a list,
         0 for the density rule; coul_lanes, coul_width: the pair kernel's
         shape for the systems with real-space Coulomb pairs (pair_lanes and
-        pair_width apply to the LJ-only systems)."""
+        pair_width apply to the LJ-only systems); fft_planes (before the first
+        step): the mixed PME x and y passes as one xy plane per workgroup, 0
+        never, 1 where a plane fits in LDS, 2 (default) when the batch has at
+        least 256 planes."""
         if cell_div is not None:  # before the first step: the batch is made again
             _abi.check(self.lib.dmdx_engine_set_tuning(self.h, 9, int(cell_div)))
+        if fft_planes is not None:  # likewise
+            _abi.check(self.lib.dmdx_engine_set_tuning(self.h, 12, int(fft_planes)))
         if build_stage is not None:
             _abi.check(self.lib.dmdx_engine_set_tuning(self.h, 7, int(build_stage)))
         if pair_rows is not None:

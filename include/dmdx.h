@@ -418,7 +418,12 @@ int dmdx_engine_kernel_time(dmdx_engine *e, int32_t group, double *total_ms, int
  * with dmdx_engine_get_cell_div).  Keys 10 and 11 = lanes per atom and
  * entries per lane pass of the pair kernel for the systems with real-space
  * Coulomb pairs (default 8 x 4: their rows are several times longer than
- * LJ rows); keys 0 and 2 then apply to the LJ-only systems.
+ * LJ rows); keys 0 and 2 then apply to the LJ-only systems.  Key 12
+ * (before the first step; the batch is made again) = the mixed PME
+ * transforms' x and y passes as one xy plane per workgroup: 0 never, 1
+ * wherever the plane fits in 64 KiB of LDS, 2 (default) when the batch has
+ * at least 256 such planes; key 13 (read only) = 1 if some system's
+ * transforms run on planes.  Planes give the same grid bit for bit.
  * The list rows do not depend on keys 1, 3, 4 or 7.  Pair shapes
  * (lanes x entries): 2x8, 4x8, 8x8, 4x4, 8x4, 16x4, 32x4, 8x2, 16x2, 32x2;
  * another combination runs as the nearest of these (8 lanes where fewer
