@@ -103,7 +103,7 @@ bool pair_candidate(int p, int w);
 // optionally the per-cell build switch and its waves per cell (a line
 // without them stands for the per-atom build), the row-tile setting, the
 // staged per-cell build's lanes per atom (0 when absent) and the Coulomb
-// variant's pair lanes and width (8 x 4 when absent).
+// variant's pair lanes and width (8 x 2 when absent).
 bool cache_lookup(const char* path, uint64_t key, int* pair, int* width, int* build, int* cells,
                   int* tasks, int* rows, int* stage, int* coul_pair, int* coul_width)
 {
@@ -119,11 +119,11 @@ bool cache_lookup(const char* path, uint64_t key, int* pair, int* width, int* bu
             *pair = p;
             *width = w;
             *build = b;
-            int c = 0, t = 4, r = 0, g = 0, cp = 8, cw = 4;
+            int c = 0, t = 4, r = 0, g = 0, cp = 8, cw = 2;
             if (!(ss >> c >> t) || (c != 0 && c != 1)) c = 0, t = 4;
             else if (!(ss >> r) || (r != 0 && r != 4 && r != 8)) r = 0;
             else if (!(ss >> g) || (g != 0 && g != 8 && g != 16)) g = 0;
-            else if (!(ss >> cp >> cw) || !pair_candidate(cp, cw)) cp = 8, cw = 4;
+            else if (!(ss >> cp >> cw) || !pair_candidate(cp, cw)) cp = 8, cw = 2;
             *cells = c;
             *tasks = t;
             *rows = r;
@@ -544,7 +544,7 @@ int tune_shapes(dmdx_engine* e, int32_t steps, const char* cache_path, double* t
     const int* pl = PAIR_LANES;
     const int* pw = PAIR_WIDTH;
     const uint64_t key = signature(e);
-    int cp = 0, cw = 0, cb = 0, cc = 0, ct = 4, cr = 0, cg = 0, qp = 8, qw = 4;
+    int cp = 0, cw = 0, cb = 0, cc = 0, ct = 4, cr = 0, cg = 0, qp = 8, qw = 2;
     if (cache_lookup(cache_path, key, &cp, &cw, &cb, &cc, &ct, &cr, &cg, &qp, &qw) &&
         pair_candidate(cp, cw) && (cb == 8 || cb == 16 || cb == 32) &&
         (ct == 1 || ct == 2 || ct == 4 || ct == 8)) {

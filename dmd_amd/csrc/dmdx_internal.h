@@ -272,7 +272,7 @@ struct Dev {
     // the same for the systems with real-space Coulomb pairs (their rows are
     // several times longer and their entries cost several times more)
     int coul_tpa = 8;
-    int coul_width = 4;
+    int coul_width = 2;
     // mixed LJ pair forces: 0 = per-atom kernel (k_pair_mixed), 4 or 8 =
     // row-tile kernel (k_pair_rows) with that many list entries per block
     int pair_rows = 0;

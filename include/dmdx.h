@@ -417,7 +417,7 @@ int dmdx_engine_kernel_time(dmdx_engine *e, int32_t group, double *total_ms, int
  * mixed system with a list: 1, 2, or 0 for the density rule (read per system
  * with dmdx_engine_get_cell_div).  Keys 10 and 11 = lanes per atom and
  * entries per lane pass of the pair kernel for the systems with real-space
- * Coulomb pairs (default 8 x 4: their rows are several times longer than
+ * Coulomb pairs (default 8 x 2: their rows are several times longer than
  * LJ rows); keys 0 and 2 then apply to the LJ-only systems.  Key 12
  * (before the first step; the batch is made again) = the mixed PME
  * transforms' x and y passes as one xy plane per workgroup: 0 never, 1

@@ -33,9 +33,9 @@ def test_autotune_choice_cache_and_state(tmp_path):
         assert rm[f"rows{r['pair_rows']}"] == min(min(rm.values()), min(pm.values()))
     else:
         assert min(pm.values()) <= min(rm.values())
-    # no system has Coulomb pairs: that variant is not timed and keeps 8x4
+    # no system has Coulomb pairs: that variant is not timed and keeps 8x2
     assert all(v == -1.0 for v in r["coul_ms"].values())
-    assert (r["coul_lanes"], r["coul_width"]) == (8, 4)
+    assert (r["coul_lanes"], r["coul_width"]) == (8, 2)
     bm = r["build_ms"]
     assert bm[r["build"]] == min(bm.values())
     assert bm[f"atom{r['build_lanes']}"] == min(bm[k] for k in bm if k.startswith("atom"))
