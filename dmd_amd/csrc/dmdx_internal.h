@@ -374,7 +374,9 @@ hipError_t launch_displacement(const Dev& d, hipStream_t s);
 hipError_t launch_pre_rebuild_only(const Dev& d, hipStream_t s);
 hipError_t launch_gather(const Dev& d, hipStream_t s);
 hipError_t launch_gather_terms(const Dev& d, hipStream_t s);  // into fb (mixed)
-hipError_t launch_kick_drift(const Dev& d, hipStream_t s);
+// pre: 0, or the step's phase-1 decisions taken in the drift (1; 2 with force
+// reuse enabled) when that compute only reuses forces
+hipError_t launch_kick_drift(const Dev& d, hipStream_t s, int pre = 0);
 hipError_t launch_kick_ke(const Dev& d, hipStream_t s);
 hipError_t launch_finalize(const Dev& d, hipStream_t s, int with_kinetic);
 hipError_t launch_barostat(const Dev& d, hipStream_t s);

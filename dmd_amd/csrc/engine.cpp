@@ -1042,13 +1042,20 @@ int Batch::kernel_time(int group, double* total_ms, int64_t* launches)
     return DMDX_OK;
 }
 
-int Batch::enqueue_compute(int phase, int accumulate, int force_rebuild, bool uniform)
+int Batch::enqueue_compute(int phase, int accumulate, int force_rebuild, bool uniform,
+                           bool* defer_pre)
 {
     bool any_force = true, any_rebuild = true;
     if (!uniform) any_rebuild = mirror_compute(phase, force_rebuild, &any_force);
     else {
         bool dummy;
         mirror_compute(phase, force_rebuild, &dummy);
+    }
+    // a compute that only reuses forces (no system rebuilds or computes)
+    // leaves its per-system decisions to the caller's next kernel
+    if (defer_pre && !any_force) {
+        *defer_pre = true;
+        return DMDX_OK;
     }
     // after the drift, the mixed path has its Verlet check and frames from
     // k_kick_drift already, unless SHAKE moved atoms since
@@ -1092,10 +1099,28 @@ int Batch::enqueue_compute(int phase, int accumulate, int force_rebuild, bool un
     return DMDX_OK;
 }
 
+static hipError_t launch_kick_drift_plain(const Dev& d, hipStream_t s)
+{
+    return launch_kick_drift(d, s, 0);
+}
+static hipError_t launch_kick_drift_pre(const Dev& d, hipStream_t s)
+{
+    return launch_kick_drift(d, s, 1);
+}
+static hipError_t launch_kick_drift_pre_reuse(const Dev& d, hipStream_t s)
+{
+    return launch_kick_drift(d, s, 2);
+}
+
 int Batch::enqueue_step(bool uniform)
 {
-    TRY(enqueue_compute(1, 0, 0, uniform));
-    LAUNCH(timed(KG_INTEGRATE, launch_kick_drift));
+    // eager steps whose first compute only reuses forces take its decisions
+    // in the drift kernel (one launch fewer per step)
+    bool deferred = false;
+    TRY(enqueue_compute(1, 0, 0, uniform, uniform ? nullptr : &deferred));
+    if (!deferred) LAUNCH(timed(KG_INTEGRATE, launch_kick_drift_plain));
+    else if (options.reuse_forces) LAUNCH(timed(KG_INTEGRATE, launch_kick_drift_pre_reuse));
+    else LAUNCH(timed(KG_INTEGRATE, launch_kick_drift_pre));
     if (d.ncomp) LAUNCH(timed(KG_INTEGRATE, launch_shake));
     TRY(enqueue_compute(2, 0, 0, uniform));
     LAUNCH(timed(KG_INTEGRATE, launch_kick_ke));

@@ -217,7 +217,8 @@ public:
 private:
     static bool exclusion_term_active(const dmdx_system_desc& D, const BatchOptions& opt);
     bool mirror_compute(int phase, int force_rebuild, bool* any_force);
-    int enqueue_compute(int phase, int accumulate, int force_rebuild, bool uniform);
+    int enqueue_compute(int phase, int accumulate, int force_rebuild, bool uniform,
+                        bool* defer_pre = nullptr);
     int enqueue_step(bool uniform);
 
     hipError_t timed(int group, hipError_t (*fn)(const Dev&, hipStream_t),

@@ -25,6 +25,10 @@ struct Cx2 {
     R re, im;
 };
 
+__device__ __forceinline__ bool pre_retires(const SysState& T);
+__device__ void pre_decide(const SysDev& S, SysState& T, int phase, int force_rebuild,
+                           int reuse_enabled, double dmax);
+
 // phase: 0 standalone compute, 1 first compute of a step, 2 second compute;
 // force_rebuild: 1 = rebuild_pair_list requested by the host.
 // Rebuild rule (SysDev::interval): > 0 DMD's cadence (counter reaches the
@@ -47,18 +51,29 @@ __global__ void __launch_bounds__(64) k_pre_compute(const SysDev* __restrict__ s
         for (int w = lane_id(); w < S0.aw_n; w += 64) dmax = fmax(dmax, p_disp[S0.aw_off + w]);
     for (int o = 32; o > 0; o >>= 1) dmax = fmax(dmax, __shfl_xor(dmax, o, 64));
     if (lane_id() != 0) return;
+    pre_decide(sys[s], T, phase, force_rebuild, reuse_enabled, dmax);
+}
+
+__device__ __forceinline__ bool pre_retires(const SysState& T)
+{
+    return T.flags != 0 || (T.stop_at > 0 && T.steps_done >= T.stop_at);
+}
+
+// One system's decisions before a compute (k_pre_compute's lane 0, or the
+// system's first wave of k_kick_drift when that compute only reuses forces).
+__device__ void pre_decide(const SysDev& S, SysState& T, int phase, int force_rebuild,
+                           int reuse_enabled, double dmax)
+{
     // a system leaves the batch before the first compute of a step once it
     // has reached its step target or has failed (neighbour overflow,
     // non-finite energy): its state stays that of the end of the last step,
     // and the other systems of the batch are unaffected
-    if (phase == 1 && T.active && (T.flags != 0 || (T.stop_at > 0 && T.steps_done >= T.stop_at)))
-        T.active = 0;
+    if (phase == 1 && T.active && pre_retires(T)) T.active = 0;
     if (!T.active) {
         T.rebuild = 0;
         T.reuse = 1;
         return;
     }
-    const SysDev& S = sys[s];
     int rebuild = 0;
     if (S.has_list) {
         if (force_rebuild) {
@@ -260,19 +275,34 @@ struct DriftOut {
     double* p_disp;
 };
 
-__device__ __forceinline__ void k_kick_drift_unit(int64_t unit_, const SysDev* __restrict__ sys, const SysState* __restrict__ st,
+// pre (1, or 2 with force reuse enabled): the step's first compute reuses
+// every system's forces (the host knows it from its mirror of the
+// decisions), so k_pre_compute's phase-1 decisions are taken here instead of
+// in a launch of their own: by lane 0 of the system's first wave, while
+// every wave reads the retirement condition itself (a system that retires
+// does not move).  The other waves read only fields those decisions leave
+// alone or set to the value the wave derives itself (active).
+__device__ __forceinline__ void k_kick_drift_unit(int64_t unit_, const SysDev* __restrict__ sys, SysState* __restrict__ st,
                              const int* __restrict__ slot_sys, int64_t nslots,
                              const double* __restrict__ invm, const double* __restrict__ fx,
                              const double* __restrict__ fy, const double* __restrict__ fz,
                              double* __restrict__ vx, double* __restrict__ vy,
                              double* __restrict__ vz, double* __restrict__ x,
-                             double* __restrict__ y, double* __restrict__ z, DriftOut o)
+                             double* __restrict__ y, double* __restrict__ z, DriftOut o, int pre)
 {
     const int64_t g = unit_ * blockDim.x + threadIdx.x;
     if (g >= nslots) return;  // whole waves: nslots is a multiple of 64
     const int s = __builtin_amdgcn_readfirstlane(slot_sys[g]);  // one system per wave
-    if (s < 0 || !st[s].active) return;
+    if (s < 0) return;
     const SysDev& S = sys[s];
+    if (pre) {
+        SysState& T = st[s];
+        const bool moves = T.active && !pre_retires(T);
+        if (g == S.atom_off) pre_decide(S, T, 1, 0, pre == 2 ? 1 : 0, 0.0);
+        if (!moves) return;
+    } else if (!st[s].active) {
+        return;
+    }
     const bool real = g - S.atom_off < S.n;
     double d2 = 0.0;
     if (real) {
@@ -309,16 +339,16 @@ __device__ __forceinline__ void k_kick_drift_unit(int64_t unit_, const SysDev* _
     }
 }
 
-__global__ void k_kick_drift(const SysDev* __restrict__ sys, const SysState* __restrict__ st,
+__global__ void k_kick_drift(const SysDev* __restrict__ sys, SysState* __restrict__ st,
                              const int* __restrict__ slot_sys, int64_t nslots,
                              const double* __restrict__ invm, const double* __restrict__ fx,
                              const double* __restrict__ fy, const double* __restrict__ fz,
                              double* __restrict__ vx, double* __restrict__ vy,
                              double* __restrict__ vz, double* __restrict__ x,
-                             double* __restrict__ y, double* __restrict__ z, DriftOut o,
+                             double* __restrict__ y, double* __restrict__ z, DriftOut o, int pre,
                              int64_t nunits_)
 {
-    DMDX_FOR_UNITS(u_, nunits_) k_kick_drift_unit(u_, sys, st, slot_sys, nslots, invm, fx, fy, fz, vx, vy, vz, x, y, z, o);
+    DMDX_FOR_UNITS(u_, nunits_) k_kick_drift_unit(u_, sys, st, slot_sys, nslots, invm, fx, fy, fz, vx, vy, vz, x, y, z, o, pre);
 }
 
 // Closing kick.  Mixed path with term forces gathered beside the pair
@@ -747,7 +777,7 @@ hipError_t launch_gather_terms(const Dev& d, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_kick_drift(const Dev& d, hipStream_t s)
+hipError_t launch_kick_drift(const Dev& d, hipStream_t s, int pre)
 {
     const int BS = 256;
     // the mixed path fuses its frames and the Verlet check into the drift
@@ -755,7 +785,7 @@ hipError_t launch_kick_drift(const Dev& d, hipStream_t s)
                      d.p_disp};
     k_kick_drift<<<grid_for(blocks_for(d.nslots, BS)), BS, 0, s>>>(
         d.sys, d.st, d.slot_sys, d.nslots, d.invm, d.fx, d.fy, d.fz, d.vx, d.vy, d.vz, d.x, d.y,
-        d.z, o, (int64_t)(blocks_for(d.nslots, BS)));
+        d.z, o, pre, (int64_t)(blocks_for(d.nslots, BS)));
     return hipGetLastError();
 }
 
