@@ -589,12 +589,24 @@ int tune_shapes(dmdx_engine* e, int32_t steps, const char* cache_path, double* t
     // the row tiles beside the fastest LJ-only shape; a variant no system
     // uses keeps the engine's shape
     scratch.d.pair_rows = 0;
-    auto best_of = [&](const double* tt) {
+    // the fastest candidate, unless the engine's current shape (cur, -1 if
+    // not a candidate) is within 2% of it: a near tie in a few timed steps
+    // is noise (4 x 4 has beaten 4 x 8 by 1% in the trial and lost by 6%
+    // over the replica mix's run)
+    auto best_of = [&](const double* tt, int cur) {
         int b = 0;
         for (int c = 1; c < NPAIR; ++c)
             if (tt[c] < tt[b]) b = c;
+        if (cur >= 0 && tt[b] >= 0.98 * tt[cur]) b = cur;
         return b;
     };
+    auto index_of = [&](int lanes, int width) {
+        for (int c = 0; c < NPAIR; ++c)
+            if (pl[c] == lanes && pw[c] == width) return c;
+        return -1;
+    };
+    const int cur_lj = index_of(e->b.d.pair_tpa, e->b.d.pair_width);
+    const int cur_coul = index_of(e->b.d.coul_tpa, e->b.d.coul_width);
     if (scratch.d.any_coul_pairs) {
         for (int c = 0; c < NPAIR; ++c) {
             scratch.d.coul_tpa = pl[c];
@@ -602,7 +614,7 @@ int tune_shapes(dmdx_engine* e, int32_t steps, const char* cache_path, double* t
             rc = time_pair(&t[T_COUL + c]);
             if (rc) return rc;
         }
-        const int bq = best_of(t + T_COUL);
+        const int bq = best_of(t + T_COUL, cur_coul);
         scratch.d.coul_tpa = pl[bq];
         scratch.d.coul_width = pw[bq];
     }
@@ -613,7 +625,7 @@ int tune_shapes(dmdx_engine* e, int32_t steps, const char* cache_path, double* t
             rc = time_pair(&t[c]);
             if (rc) return rc;
         }
-        const int bp = best_of(t);
+        const int bp = best_of(t, cur_lj);
         scratch.d.pair_tpa = pl[bp];
         scratch.d.pair_width = pw[bp];
         int br = -1;  // per-atom kernel for every system

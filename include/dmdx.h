@@ -439,7 +439,9 @@ int dmdx_engine_get_tuning(dmdx_engine *e, int32_t key, int32_t *value);
  * list build (8/16/32 lanes per atom, or per cell with 2/4/8 waves per
  * cell) on a scratch copy of the engine's
  * systems stepped from their initial state (the engine's own state is
- * untouched), `steps` steps per candidate, and applies the fastest of each.
+ * untouched), `steps` steps per candidate, and applies the fastest of each
+ * (a pair shape replaces the engine's current one only if at least 2%
+ * faster: near ties in a few steps are noise).
  * cache_path (may be NULL): text file of earlier results keyed by a
  * signature of the systems; a hit applies the cached choice without timing.
  * times_ms (DMDX_TUNE_TIMES doubles, may be NULL): per-step ms of the pair

@@ -25,14 +25,18 @@ def test_autotune_choice_cache_and_state(tmp_path):
                           "stage16")
     pm = r["pair_ms"]
     assert all(v > 0 for v in pm.values())
-    assert pm[f"{r['pair_lanes']}x{r['pair_width']}"] == min(pm.values())
+    # the fastest, or the default 4x8 when no candidate beats it by 2%
+    best = min(pm.values())
+    chosen = pm[f"{r['pair_lanes']}x{r['pair_width']}"]
+    assert chosen == best or ((r["pair_lanes"], r["pair_width"]) == (4, 8) and
+                              best >= 0.98 * chosen)
     assert r["pair_rows"] in (0, 4, 8)
     rm = r["rows_ms"]
     assert all(v > 0 for v in rm.values())
     if r["pair_rows"]:
-        assert rm[f"rows{r['pair_rows']}"] == min(min(rm.values()), min(pm.values()))
+        assert rm[f"rows{r['pair_rows']}"] < chosen
     else:
-        assert min(pm.values()) <= min(rm.values())
+        assert chosen <= min(rm.values())
     # no system has Coulomb pairs: that variant is not timed and keeps 8x2
     assert all(v == -1.0 for v in r["coul_ms"].values())
     assert (r["coul_lanes"], r["coul_width"]) == (8, 2)
@@ -161,8 +165,10 @@ def test_autotune_cells_and_fft(tmp_path):
     cm, pm = r["coul_ms"], r["pair_ms"]
     print({"coul": cm, "pair": pm}, flush=True)
     assert all(v > 0 for v in cm.values()) and all(v > 0 for v in pm.values())
-    assert cm[f"{r['coul_lanes']}x{r['coul_width']}"] == min(cm.values())
-    assert pm[f"{r['pair_lanes']}x{r['pair_width']}"] == min(pm.values())
+    for ms, shape, default in ((cm, (r["coul_lanes"], r["coul_width"]), (8, 2)),
+                               (pm, (r["pair_lanes"], r["pair_width"]), (4, 8))):
+        chosen, best = ms[f"{shape[0]}x{shape[1]}"], min(ms.values())
+        assert chosen == best or (shape == default and best >= 0.98 * chosen)
     e2 = Engine(specs)
     r2 = e2.autotune(steps=3, cache=cache)
     assert r2["fft_lines"] == r["fft_lines"]
