@@ -380,7 +380,7 @@ def main():
     ap.add_argument("--requests", type=int, default=120, help="serving: number of requests")
     ap.add_argument("--rate", type=float, default=8.0, help="serving: arrivals per second")
     ap.add_argument("--steps-scale", type=float, default=1.0, help="serving: step-count factor")
-    ap.add_argument("--chunk", type=int, default=50, help="serving: steps per round")
+    ap.add_argument("--chunk", type=int, default=100, help="serving: steps per round")
     ap.add_argument("--cohorts", type=int, default=8, help="serving: concurrent cohorts")
     ap.add_argument("--graph-steps", type=int, default=0, help="serving: captured block size")
     args = ap.parse_args()

@@ -408,7 +408,7 @@ int dmdx_server_create(const dmdx_server_options* opt, dmdx_server** out)
     std::memset(&srv->opt, 0, sizeof(srv->opt));
     if (opt) srv->opt = *opt;
     if (srv->opt.max_cohorts <= 0) srv->opt.max_cohorts = 8;
-    if (srv->opt.chunk_steps <= 0) srv->opt.chunk_steps = 50;
+    if (srv->opt.chunk_steps <= 0) srv->opt.chunk_steps = 100;
     if (srv->opt.min_admit <= 0) srv->opt.min_admit = 1;
     if (srv->opt.max_wait_ms < 0.0) srv->opt.max_wait_ms = 0.0;
     srv->budget = srv->opt.hbm_budget_bytes;

@@ -76,7 +76,7 @@ def _setup(lib):
 class Server:
     """Device-side request server (include/dmdx.h, dmdx_server_*)."""
 
-    def __init__(self, hbm_budget_bytes: int = 0, max_cohorts: int = 8, chunk_steps: int = 50,
+    def __init__(self, hbm_budget_bytes: int = 0, max_cohorts: int = 8, chunk_steps: int = 100,
                  graph_steps: int = 0, min_admit: int = 1, max_wait_ms: float = 0.0,
                  max_cohort_systems: int = 0, merge: bool = True):
         self.lib = _abi.load()

@@ -504,7 +504,7 @@ typedef struct dmdx_server_options {
     int64_t hbm_budget_bytes;      /* device memory the server may hold;
                                       0 = 90% of the free memory at creation */
     int32_t max_cohorts;           /* cohorts stepped concurrently (0: 8)     */
-    int32_t chunk_steps;           /* steps per cohort per pump (0: 50)       */
+    int32_t chunk_steps;           /* steps per cohort per pump (0: 100)      */
     int32_t graph_steps;           /* > 0: replay captured blocks of steps    */
     int32_t min_admit;             /* form a cohort once this many requests
                                       wait (0: 1) ...                          */
